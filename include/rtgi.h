@@ -1,0 +1,235 @@
+/*
+ * rtgi.h -- C ABI of the MI355X (gfx950) per-frame GI update: clipmap voxel rebuild, radiance-cascade probe-ray DDA
+ * trace, cascade merge and L2 SH projection.
+ *
+ * The entry points replace the reference renderer's pass interface (TypeScript factories recording WGSL compute passes
+ * into a Dawn encoder). Each declaration cites the reference interface it replaces; paths are relative to the
+ * reference repository (owoDra/real-time-gi-lab).
+ *
+ * Conventions (replacing the reference's synchronous JS throws, src/core/voxel.ts:749, src/core/rc.ts:1022):
+ *   - every entry point returns an int status: RTGI_OK (0) or a negative RTGI_E_* code;
+ *   - rtgi_last_error() returns a thread-local message for the last failing call of the calling thread;
+ *   - GPU-side pool overflow is not an error: it is reported through rtgi_counters (src/core/voxel.ts:908-918);
+ *   - host arrays passed in are copied during the call; readback buffers are caller-allocated;
+ *   - a world handle is not re-entrant: calls on one handle must not overlap; different handles may be driven from
+ *     different threads.
+ * Only plain C types cross this boundary (no HIP or torch types): streams are passed as void* (a hipStream_t).
+ */
+#ifndef RTGI_H
+#define RTGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTGI_OK 0
+#define RTGI_E_INVALID (-1)  /* bad argument or configuration */
+#define RTGI_E_HIP (-2)      /* a HIP runtime call failed */
+#define RTGI_E_NOMEM (-3)    /* device or host allocation failed, or the memory budget refused it */
+#define RTGI_E_STATE (-4)    /* call out of order (e.g. enqueue before a scene was uploaded) */
+#define RTGI_E_NODEVICE (-5) /* no gfx950 device visible */
+
+#define RTGI_MAX_LEVELS 8
+#define RTGI_MAX_CASCADES 8
+#define RTGI_MAX_POINT_LIGHTS 8
+
+/* Pointer-table sentinels, identical to the reference's VOX_EMPTY / VOX_DROP / VOX_MARK (src/core/voxel.ts:64-66). */
+#define RTGI_PTR_EMPTY 0xffffffffu
+#define RTGI_PTR_DROP 0xfffffffeu
+#define RTGI_PTR_MARK 0xfffffffdu
+/* Miss value of a per-ray hit id. */
+#define RTGI_NO_HIT 0xffffffffu
+
+/* Voxel clipmap parameters; mirrors VoxParams (src/core/voxel.ts:31-38) without the dynamic-layer pool. */
+typedef struct rtgi_vox_params {
+  double v0;          /* finest voxel size (m); level L has voxels of v0 * 2^L */
+  int32_t levels;     /* 1..8 */
+  int32_t G;          /* bricks per axis per level (a brick is 8^3 voxels); G^3 must be a multiple of 64 */
+  int32_t max_bricks; /* brick pool, all levels */
+  int32_t max_voxels; /* voxel attribute pool, all levels */
+} rtgi_vox_params;
+
+/* Flags of rtgi_rc_params.flags. */
+#define RTGI_RC_RECORD_HITS 1u /* also write the first-hit voxel id of every ray (rtgi_world_read_cascade) */
+#define RTGI_RC_NO_MERGE 2u    /* stop after the trace (the cascade buffers hold the traced intervals) */
+
+/* Radiance-cascade parameters; the subset of RcParams (src/core/rc.ts:42-91) that the world-space voxel tracer uses,
+ * with probes on a dense camera-centred grid instead of screen-registered sparse probes (relocation 'none'). */
+typedef struct rtgi_rc_params {
+  int32_t cascades;   /* N, 1..8 */
+  int32_t probes0;    /* c0 probes per axis; cascade c has probes0 >> c per axis; must be a multiple of 2^(N-1) */
+  double s0;          /* c0 probe spacing (m); cascade c is spaced s0 * 2^c */
+  double k_interval;  /* kInterval: interval [b_c, b_{c+1}] * s0, b_k = k (4^k - 1) / 3 */
+  double level_map;   /* levelMap */
+  double vox_scale;   /* voxScale */
+  double vox_offset;  /* voxOffset (the probe normal is 0 on the dense grid, so the start offset vanishes) */
+  double vox_skip;    /* voxSkip */
+  int32_t res_cap;    /* resCap: res_c = 4 << min(c, resCap) */
+  uint32_t flags;     /* RTGI_RC_* */
+} rtgi_rc_params;
+
+/* Per-frame inputs; the part of the Frame uniform (src/core/frame.ts:4-23) the path reads, plus the eye that
+ * VoxWorld.update takes (src/core/voxel.ts:868-892). */
+typedef struct rtgi_frame {
+  double eye[3];      /* clipmap and probe-grid centre (world m) */
+  float sun_dir[3];   /* towards the sun (normalised by the caller, as packFrame does) */
+  float sun_e[3];     /* sun irradiance */
+  float sky[3];       /* sky radiance returned by rays that pass the last interval */
+  float sun_vis;      /* per-voxel sun visibility used at every hit (1: unshadowed; the shadow map is not in this path) */
+  int32_t n_point;    /* point lights (0..8); 0 keeps the reference's sun-only shading */
+  float point_pos[RTGI_MAX_POINT_LIGHTS][3];
+  float point_i[RTGI_MAX_POINT_LIGHTS][3]; /* intensity (W/sr-like units): irradiance = I * max(n.l, 0) / r^2 */
+} rtgi_frame;
+
+/* Counters; the same semantics as VoxWorld.readCounters (src/core/voxel.ts:908-918). big_overflow is always 0 here:
+ * the big-triangle lists hold every triangle. */
+typedef struct rtgi_counters {
+  uint32_t bricks;          /* allocated bricks (min(marked, max_bricks)) */
+  uint32_t marked;          /* marked bricks before allocation */
+  uint32_t voxels;          /* occupied voxels requested (may exceed max_voxels) */
+  uint32_t brick_overflow;  /* bricks dropped by a full pool */
+  uint32_t big_overflow;
+  uint32_t voxel_overflow;  /* bricks whose voxels found no attribute slots */
+  uint32_t bricks_per_level[RTGI_MAX_LEVELS];
+  uint32_t big_per_level[RTGI_MAX_LEVELS];  /* triangles handed to the big (one workgroup each) voxel list */
+} rtgi_counters;
+
+/* Sizes of a world's buffers, for readback allocation. */
+typedef struct rtgi_layout {
+  int32_t levels, G, E;                       /* E = 8 G voxels per axis per level */
+  int32_t window_origin[RTGI_MAX_LEVELS][3];  /* first brick of each level's window (world brick coordinates) */
+  int32_t cascades;
+  int32_t probes[RTGI_MAX_CASCADES];          /* probes per axis of each cascade */
+  int32_t res[RTGI_MAX_CASCADES];             /* octahedral directions per axis of each cascade */
+  int32_t vox_level[RTGI_MAX_CASCADES];       /* voxLevel(s0, c) (src/core/rc.ts:210-213) */
+  int32_t grid_origin[3];                     /* absolute c0 grid index of probe (0,0,0) */
+  uint64_t rays[RTGI_MAX_CASCADES];           /* probes^3 * res^2 */
+  uint64_t total_rays;
+  float b0[RTGI_MAX_CASCADES], b1[RTGI_MAX_CASCADES]; /* interval bounds in units of s0 */
+  float omega0[16];                           /* solid angle of each c0 texel (octSolidAngles(4), src/core/frame.ts:105-114) */
+  uint64_t device_bytes;                      /* device memory held by the world */
+} rtgi_layout;
+
+/* Stages for rtgi_world_enqueue_stages (bit mask) and rtgi_world_stage_ms. */
+#define RTGI_STAGE_VOX 1u    /* clear .. resolveAttr (src/core/voxel.ts:708, VOX_STAGES without 'sun') */
+#define RTGI_STAGE_TRACE 2u  /* giTrace (src/core/rc.ts:1201-1210) */
+#define RTGI_STAGE_MERGE 4u  /* giMerge: mergeWorld N-1..0 (src/core/rc.ts:1212-1214) */
+#define RTGI_STAGE_SH 8u     /* projectSH (src/core/rc.ts:1216) */
+#define RTGI_STAGE_ALL 15u
+
+typedef struct rtgi_world rtgi_world;
+
+/* Message of the calling thread's last failing call ("" if none). */
+const char* rtgi_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails for that reason). */
+int rtgi_device_count(int32_t* n);
+
+/* Library version string. */
+const char* rtgi_version(void);
+
+/* octSolidAngles(res, samples) (src/core/frame.ts:105-114): solid angle of each texel of a res x res octahedral map,
+ * from a Fibonacci-sphere count in double precision. out: res*res floats. */
+int rtgi_oct_solid_angles(int32_t res, int32_t samples, float* out);
+
+/* Creates a world on the current HIP device. Replaces both factories createVoxWorld (src/core/voxel.ts:746-747) and
+ * createRc (src/core/rc.ts:1019), called by createRunContext (src/core/rcrun.ts:62-63) and runConfig
+ * (src/core/rcrun.ts:269-271). Throws become RTGI_E_INVALID: levels > 8 (voxel.ts:749), cascades > 8 (rc.ts:1022). */
+int rtgi_world_create(const rtgi_vox_params* vox, const rtgi_rc_params* rc, rtgi_world** out);
+
+/* Scene upload: positions and normals (3 floats per vertex), the triangle table (4 u32 per triangle: absolute vertex
+ * indices and material, triangleTable src/core/voxel.ts:730-739) and materials (baseColorFactor rgba per material;
+ * untextured: the white layer 0 of src/core/scene_gpu.ts:64-68). Replaces uploadScene (src/core/scene_gpu.ts:56) and the
+ * tri-table upload (src/core/rcrun.ts:53-59). Every triangle is voxelised (firstDynamicTri = triangleCount). */
+int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const float* nrm, uint32_t nverts, const uint32_t* tri,
+                            uint32_t ntris, const float* materials, uint32_t nmat);
+
+/* Replaces the posed-vertex writeBuffer (src/core/rcrun.ts:213-214): vertices [first, first+count). */
+int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_t count, const float* pos, const float* nrm);
+
+/* Replaces VoxWorld.update(eye) (src/core/voxel.ts:896, writeUniform 868-892) and the Frame uniform fields the path reads
+ * (packFrame src/core/frame.ts:76-94: sunDir, sunE, sky). Also places the dense probe grid around the eye. */
+int rtgi_world_set_frame(rtgi_world* w, const rtgi_frame* f);
+
+/* Enqueues one GI update (voxel rebuild, trace, merge, SH) on `stream` (a hipStream_t; NULL: the world's own stream).
+ * Replaces VoxWorld.encode (src/core/voxel.ts:897-899) followed by Rc.encode's giTrace and giMerge passes
+ * (src/core/rc.ts:1169-1216) and the queue submit (src/core/rcrun.ts:333). Asynchronous. */
+int rtgi_world_enqueue(rtgi_world* w, void* stream);
+
+/* The same restricted to a mask of RTGI_STAGE_* (run in pipeline order). Replaces encodeStaged
+ * (src/core/voxel.ts:900-907). With timing on, per-stage HIP events are recorded around each stage. */
+int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* stream);
+
+/* Per-stage timing with HIP events (replaces timestampWrites, src/core/rcrun.ts:325-326). on: 0/1. */
+int rtgi_world_set_timing(rtgi_world* w, int32_t on);
+/* Milliseconds of each stage of the last enqueue (after rtgi_world_sync): ms[0] vox, [1] trace, [2] merge, [3] sh. */
+int rtgi_world_stage_ms(rtgi_world* w, float ms[4]);
+
+/* Waits for the world's work (replaces the mapAsync waits, src/core/rcrun.ts:349-358). */
+int rtgi_world_sync(rtgi_world* w);
+
+/* Replaces VoxWorld.readCounters (src/core/voxel.ts:908-918). Synchronises. */
+int rtgi_world_read_counters(rtgi_world* w, rtgi_counters* out);
+
+int rtgi_world_get_layout(const rtgi_world* w, rtgi_layout* out);
+
+/* Canonical (layout-independent) readback of clipmap level L; replaces reading the vox buffer (layout
+ * src/core/voxel.ts:11,52-53,753-760). Arrays are indexed by the window cell x + E (y + E z), E = 8 G; any may be NULL:
+ *   occ_bits     E^3/32 words: bit (i & 31) of word (i >> 5) set if cell i is occupied;
+ *   rep          E^3: representative triangle + 1 (0: empty), the rep rule of src/core/voxel.ts:489 and :571-574;
+ *   albedo       E^3: pack4x8unorm(albedo, toon) of occupied cells with an attribute slot, else 0;
+ *   normal       E^3: pack2x16snorm(octEnc(n)), else 0;
+ *   plane        E^3: offset of the representative's plane from the cell centre (src/core/voxel.ts:619), else 0;
+ *   brick_state  G^3: 0 empty, 1 allocated with attributes, 2 dropped by a full pool, 3 allocated without attribute
+ *                slots (voxel pool full). Synchronises. */
+int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* occ_bits, uint32_t* rep, uint32_t* albedo,
+                            uint32_t* normal, float* plane, uint8_t* brick_state);
+
+/* Cascade c after the last update: rgba (4 floats per ray: radiance, transmittance; after the merge a = 1), in ray order
+ * probe-major (probe x + n (y + n z)), direction di = u + res v within a probe; hits (if recorded): first-hit voxel id
+ * level << 29 | (x + E (y + E z)) of the window cell, RTGI_NO_HIT for a miss. Either pointer may be NULL. Replaces reading
+ * the rad buffer (src/core/rc.ts:1064). Synchronises. */
+int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hits);
+
+/* SH irradiance of the c0 probes: 27 floats per probe (9 coefficients x rgb, scaled by {pi, 2pi/3, pi/4}), replacing
+ * the sh buffer written by projectSH (src/core/rc.ts:752-774) before its f16 packing. Synchronises. */
+int rtgi_world_read_sh(rtgi_world* w, float* sh);
+
+/* Frees the world's device memory (replaces VoxWorld.destroy src/core/voxel.ts:923 and Rc.destroy rc.ts:1234). */
+int rtgi_world_destroy(rtgi_world* w);
+
+/* ---- Seeded procedural scenes (inputs of the benchmark configurations; no reference counterpart beyond the Builder
+ * idiom of src/core/scenes.ts:176-211 and the rigid posing of src/core/scenes.ts:80-102). ---- */
+#define RTGI_SCENE_ATRIUM 1 /* 20 x 20 x 10 m atrium: walls, 24 pillars, tessellated arches, 16 moving boxes */
+#define RTGI_SCENE_CITY 2   /* city block: seeded buildings that translate and rotate per frame, street furniture */
+
+typedef struct rtgi_scene_params {
+  int32_t kind;
+  uint64_t seed;
+  int32_t frame;       /* pose of the moving parts */
+  double extent;       /* city: side of the square block (m); atrium: ignored (0: default) */
+  int32_t buildings;   /* city: building count (0: default 64) */
+  double detail;       /* tessellation step scale (1: default triangle count) */
+} rtgi_scene_params;
+
+typedef struct rtgi_scene {
+  uint32_t nverts, ntris, nmat;
+  float* pos;        /* nverts * 3 */
+  float* nrm;        /* nverts * 3 */
+  uint32_t* tri;     /* ntris * 4 */
+  float* materials;  /* nmat * 4 */
+  double bounds_lo[3], bounds_hi[3];
+  double eye[3];     /* a default eye for the scene */
+  float sun_dir[3];
+} rtgi_scene;
+
+int rtgi_scene_generate(const rtgi_scene_params* p, rtgi_scene** out);
+int rtgi_scene_free(rtgi_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTGI_H */

@@ -1,0 +1,242 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (liboracle.so, rtgi_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, and only as the checker.
+
+What pins each oracle function (beyond being an operation-by-operation restatement of the reference WGSL, whose
+citations are in rtgi_oracle.c); the reference ships no golden vectors or tests for this path:
+  triBox (voxel.ts:231-248)          exact rational SAT over the 13 axes on random inputs (tests/test_oracle.py)
+  closestBary (voxel.ts:250-267)     brute-force minimisation of the distance over a barycentric grid
+  pack4x8unorm / pack2x16snorm       WGSL specification formulas and known values (tests/golden/pack_vectors.json)
+  octEnc / octDecode                 round trip; closed-form values on the axes and diagonals
+  build (voxel.ts:338-620)           brute force: occupancy = {cells whose box the triangle overlaps, by exact SAT}
+                                     at level 0, coarse level = OR of its 8 children where the finer window holds
+                                     it, rep = max overlapping triangle + 1; ordered overflow drops the finest bricks
+  traceVox (voxel.ts:121-202)        FP64 voxel walk (first hit equal away from ties); axis-aligned rays closed form;
+                                     empty world -> T = 1; single occupied voxel
+  mergeWorld (rc.ts:681-708)         equal-weight parent mean = trilinear interpolation on the dense grid
+  projectSH (rc.ts:745-774)          constant radiance -> only L0, = pi * 0.282095 * 4 pi * L; sum(omega) = 4 pi
+  octSolidAngles (frame.ts:105-114)  sum = 4 pi, 8-fold symmetry of the 4x4 texels, library value agrees
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcVox(C.Structure):
+    _fields_ = [("v0", C.c_float), ("levels", C.c_int32), ("G", C.c_int32), ("max_bricks", C.c_int32),
+                ("max_voxels", C.c_int32), ("origin", (C.c_int32 * 3) * 8)]
+
+
+class OrcRc(C.Structure):
+    _fields_ = [("cascades", C.c_int32), ("probes0", C.c_int32), ("res_cap", C.c_int32), ("s0", C.c_float),
+                ("vox_skip", C.c_float), ("vox_level", C.c_int32 * 8), ("b0", C.c_float * 8), ("b1", C.c_float * 8),
+                ("G0", C.c_int32 * 3), ("sun_dir", C.c_float * 3), ("sun_e", C.c_float * 3), ("sky", C.c_float * 3),
+                ("sun_vis", C.c_float), ("n_point", C.c_int32), ("point_pos", (C.c_float * 3) * 8),
+                ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16)]
+
+
+_P = C.c_void_p
+_SIG = {
+    "orc_vox_words": (C.c_size_t, [C.POINTER(OrcVox)]),
+    "orc_origins": (None, [C.c_double, C.c_int, C.c_int, _P, _P]),
+    "orc_cascade_setup": (None, [C.POINTER(OrcRc), C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, _P]),
+    "orc_oct_solid_angles": (None, [C.c_int, C.c_int, _P]),
+    "orc_build": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, C.c_uint32, _P, _P, _P]),
+    "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
+    "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
+    "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, C.c_int]),
+    "orc_merge_sh": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
+    "orc_trace_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
+                                C.c_double, C.c_int]),
+    "orc_merge_sh_f64": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
+    "orc_tri_box": (C.c_int, [_P, _P, _P, _P, _P]),
+    "orc_closest_bary": (None, [_P, _P, _P, _P, _P]),
+    "orc_pack4x8unorm": (C.c_uint32, [_P]),
+    "orc_pack2x16snorm": (C.c_uint32, [C.c_float, C.c_float]),
+    "orc_oct_enc": (None, [_P, _P]),
+}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        for n, (r, a) in _SIG.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def f32(x):
+    return np.ascontiguousarray(x, np.float32)
+
+
+def tri_box(c, h, a, b, cc):
+    return bool(lib().orc_tri_box(_p(f32(c)), _p(f32(h)), _p(f32(a)), _p(f32(b)), _p(f32(cc))))
+
+
+def closest_bary(p, a, b, c):
+    out = np.zeros(3, np.float32)
+    lib().orc_closest_bary(_p(f32(p)), _p(f32(a)), _p(f32(b)), _p(f32(c)), _p(out))
+    return out
+
+
+def pack4x8unorm(e):
+    return int(lib().orc_pack4x8unorm(_p(f32(e))))
+
+
+def pack2x16snorm(x, y):
+    return int(lib().orc_pack2x16snorm(float(np.float32(x)), float(np.float32(y))))
+
+
+def oct_enc(n):
+    out = np.zeros(2, np.float32)
+    lib().orc_oct_enc(_p(f32(n)), _p(out))
+    return out
+
+
+def oct_solid_angles(res, samples=1 << 20):
+    out = np.zeros(res * res, np.float32)
+    lib().orc_oct_solid_angles(res, samples, _p(out))
+    return out
+
+
+def origins(v0, levels, G, eye):
+    e = np.ascontiguousarray(eye, np.float64)
+    out = np.zeros((8, 3), np.int32)
+    lib().orc_origins(v0, levels, G, _p(e), _p(out))
+    return out[:levels]
+
+
+class Oracle:
+    """CPU restatement of one world: build -> canonical clipmap; trace -> merge -> SH (f32), and the FP64 tracer."""
+
+    def __init__(self, vox, rc, frame, omega0=None):
+        self.vp = OrcVox()
+        self.vp.v0 = vox.v0
+        self.vp.levels = vox.levels
+        self.vp.G = vox.G
+        self.vp.max_bricks = vox.maxBricks
+        self.vp.max_voxels = vox.maxVoxels
+        org = origins(vox.v0, vox.levels, vox.G, frame.eye)
+        for L in range(vox.levels):
+            for a in range(3):
+                self.vp.origin[L][a] = int(org[L][a])
+        r = OrcRc()
+        r.cascades = rc.cascades
+        r.probes0 = rc.probes0
+        r.res_cap = rc.resCap
+        r.vox_skip = rc.voxSkip
+        eye = np.ascontiguousarray(frame.eye, np.float64)
+        lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
+                                _p(eye))
+        sd = np.asarray(frame.sunDir, np.float64)
+        sd = sd / np.linalg.norm(sd)
+        for a in range(3):
+            r.sun_dir[a] = float(sd[a])
+            r.sun_e[a] = float(frame.sunE[a])
+            r.sky[a] = float(frame.sky[a])
+        r.sun_vis = frame.sunVis
+        r.n_point = len(frame.points)
+        for k, (p, i) in enumerate(frame.points):
+            for a in range(3):
+                r.point_pos[k][a] = float(p[a])
+                r.point_i[k][a] = float(i[a])
+        om = omega0 if omega0 is not None else oct_solid_angles(4)
+        for i in range(16):
+            r.omega0[i] = float(om[i])
+        self.rc = r
+        self.vox = vox
+        self.E = vox.G * 8
+        self.words = lib().orc_vox_words(C.byref(self.vp))
+        self.buf = None
+
+    def cascade_rays(self):
+        return [int(lib().orc_rays(C.byref(self.rc), c)) for c in range(self.rc.cascades)]
+
+    def build(self, scene):
+        self.buf = np.zeros(self.words, np.uint32)
+        self.rep = np.zeros(self.vox.maxBricks * 512, np.uint32)
+        self._scene = (f32(scene.pos), f32(scene.nrm), np.ascontiguousarray(scene.tri, np.uint32),
+                       f32(scene.materials))
+        pos, nrm, tri, mat = self._scene
+        rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), tri.shape[0], _p(mat), _p(self.buf),
+                             _p(self.rep))
+        assert rc == 0
+
+    def counters(self):
+        b = self.buf
+        NL = self.vox.levels
+        return dict(bricks=int(min(b[0], self.vox.maxBricks)), marked=int(b[0]), voxels=int(b[1]),
+                    brickOverflow=int(b[2]), bigOverflow=int(b[3]), voxelOverflow=int(b[4]),
+                    bricksPerLevel=[int(x) for x in b[8:8 + NL]], bigPerLevel=[int(x) for x in b[16:16 + NL]])
+
+    def clipmap(self, level):
+        n = self.E ** 3
+        occ = np.zeros(n // 32, np.uint32)
+        rp = np.zeros(n, np.uint32)
+        alb = np.zeros(n, np.uint32)
+        nrm = np.zeros(n, np.uint32)
+        pl = np.zeros(n, np.float32)
+        st = np.zeros(self.vox.G ** 3, np.uint8)
+        rc = lib().orc_canonical(C.byref(self.vp), _p(self.buf), _p(self.rep), level, _p(occ), _p(rp), _p(alb),
+                                 _p(nrm), _p(pl), _p(st))
+        assert rc == 0
+        return dict(occ=occ, rep=rp, albedo=alb, normal=nrm, plane=pl, brick_state=st)
+
+    def trace(self, r0=0, r1=None, threads=0):
+        total = sum(self.cascade_rays())
+        r1 = total if r1 is None else r1
+        rgba = np.zeros((r1 - r0, 4), np.float32)
+        hits = np.zeros(r1 - r0, np.uint32)
+        lib().orc_trace(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgba), _p(hits), threads)
+        return rgba, hits
+
+    def merge_sh(self, rgba):
+        rgba = np.ascontiguousarray(rgba, np.float32).copy()
+        n = self.rc.probes0 ** 3
+        sh = np.zeros((n, 27), np.float32)
+        lib().orc_merge_sh(C.byref(self.rc), _p(rgba), _p(sh))
+        return rgba, sh
+
+    def trace_f64(self, r0=0, r1=None, eps=1e-6, threads=0):
+        total = sum(self.cascade_rays())
+        r1 = total if r1 is None else r1
+        rgbt = np.zeros((r1 - r0, 4), np.float64)
+        hits = np.zeros(r1 - r0, np.uint32)
+        tie = np.zeros(r1 - r0, np.uint8)
+        lib().orc_trace_f64(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgbt), _p(hits), _p(tie),
+                            eps, threads)
+        return rgbt, hits, tie
+
+    def merge_sh_f64(self, rgbt):
+        rgbt = np.ascontiguousarray(rgbt, np.float64).copy()
+        n = self.rc.probes0 ** 3
+        sh = np.zeros((n, 27), np.float64)
+        lib().orc_merge_sh_f64(C.byref(self.rc), _p(rgbt), _p(sh))
+        return rgbt, sh
+
+    def split(self, arr):
+        """Splits a full ray array into per-cascade arrays."""
+        out, o = [], 0
+        for n in self.cascade_rays():
+            out.append(arr[o:o + n])
+            o += n
+        return out

@@ -1,0 +1,1144 @@
+/*
+ * rtgi_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker for librtgi; see rtgi_oracle.h).
+ *
+ * Plain C99, compiled with -O2 -ffp-contract=off on x86-64 (SSE: IEEE f32, correctly rounded / and sqrt).
+ * Each function cites the reference WGSL it restates (paths relative to the reference repository,
+ * owoDra/real-time-gi-lab). GPU atomics are taken in thread-index order; the brick allocation uses the reference's
+ * ordered allocation (the one it uses when the pool overflows, voxel.ts:405-470) in every case, which is one of the
+ * schedules its atomic counter allows when every brick fits.
+ *
+ * Departures, each documented where it occurs: no alpha test and no texture (untextured, opaque materials); no
+ * dynamic layer, bounce or voxSun (the sun visibility is a frame constant); dense probe grids with relocation
+ * 'none' instead of screen-registered probes; point lights are an addition (0 lights = the reference shading).
+ */
+#include "rtgi_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define HDR 64u
+#define BW 18u /* BRICK_WORDS, voxel.ts:53 */
+#define V_EMPTY 0xffffffffu
+#define V_DROP 0xfffffffeu
+#define V_MARK 0xfffffffdu
+#define BIG 8
+#define BIG_CAP (1u << 18)
+
+typedef struct { float x, y, z; } v3;
+typedef struct { int x, y, z; } iv3;
+
+static v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 vmul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static v3 vscale(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static v3 vdivs(v3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+static v3 vneg(v3 a) { return V3(-a.x, -a.y, -a.z); }
+static float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 vabs(v3 a) { return V3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+static float fmn(float a, float b) { return a < b ? a : b; }
+static float fmx(float a, float b) { return a > b ? a : b; }
+static v3 vmin(v3 a, v3 b) { return V3(fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)); }
+static v3 vmax(v3 a, v3 b) { return V3(fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)); }
+static v3 vcross(v3 a, v3 b) { return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static v3 vnormalize(v3 a) { return vdivs(a, sqrtf(vdot(a, a))); }
+static v3 vfloor(v3 a) { return V3(floorf(a.x), floorf(a.y), floorf(a.z)); }
+static v3 vf(iv3 a) { return V3((float)a.x, (float)a.y, (float)a.z); }
+static iv3 I3(int x, int y, int z) { iv3 r = {x, y, z}; return r; }
+static int f2i(float f) { /* WGSL f32 -> i32 saturates */
+  if (f <= -2147483648.0f) return (-2147483647 - 1);
+  if (f >= 2147483520.0f) return 2147483520;
+  return (int)f;
+}
+static iv3 vi(v3 a) { return I3(f2i(a.x), f2i(a.y), f2i(a.z)); }
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+static iv3 iclamp(iv3 a, iv3 lo, iv3 hi) {
+  return I3(imin(imax(a.x, lo.x), hi.x), imin(imax(a.y, lo.y), hi.y), imin(imax(a.z, lo.z), hi.z));
+}
+static iv3 iadd(iv3 a, iv3 b) { return I3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static iv3 isub(iv3 a, iv3 b) { return I3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static iv3 imuls(iv3 a, int s) { return I3(a.x * s, a.y * s, a.z * s); }
+static iv3 iadds(iv3 a, int s) { return I3(a.x + s, a.y + s, a.z + s); }
+static iv3 ishr(iv3 a, int s) { return I3(a.x >> s, a.y >> s, a.z >> s); }
+static int popc(uint32_t x) { return __builtin_popcount(x); }
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Buffer layout (createVoxWorld, voxel.ts:753-760) without the bounce words and the sun word.                   */
+
+typedef struct {
+  const orc_vox* p;
+  uint32_t* vox;
+  uint32_t* rep;
+  const float* pos;
+  const float* nrm;
+  const uint32_t* tri;
+  const float* mat;
+  uint32_t ntris;
+  uint32_t ptrOff, brickOff, attrOff, plOff;
+  uint32_t* big[16]; /* 0-7 voxBits per level, 8-15 markBricks per level */
+} Ctx;
+
+static void layout(const orc_vox* p, uint32_t* ptrOff, uint32_t* brickOff, uint32_t* attrOff, uint32_t* plOff) {
+  uint32_t G = (uint32_t)p->G;
+  *ptrOff = HDR;
+  *brickOff = *ptrOff + (uint32_t)p->levels * G * G * G;
+  *attrOff = *brickOff + (uint32_t)p->max_bricks * BW;
+  *plOff = *attrOff + (uint32_t)p->max_voxels * 2u;
+}
+
+size_t orc_vox_words(const orc_vox* p) {
+  uint32_t a, b, c, d;
+  layout(p, &a, &b, &c, &d);
+  return (size_t)d + (size_t)p->max_voxels;
+}
+
+void orc_origins(double v0, int levels, int G, const double eye[3], int32_t out[8][3]) {
+  for (int L = 0; L < levels; L++) {
+    double bs = v0 * pow(2.0, L) * 8;
+    for (int a = 0; a < 3; a++) out[L][a] = (int32_t)(floor(eye[a] / bs) - G / 2);
+  }
+}
+
+static float voxSize(const orc_vox* p, uint32_t L) { return p->v0 * (float)(1u << L); }
+static uint32_t voxPtrIndex(const Ctx* X, uint32_t L, iv3 b) {
+  uint32_t G = (uint32_t)X->p->G;
+  return X->ptrOff + L * G * G * G + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x;
+}
+static iv3 org(const orc_vox* p, uint32_t L) { return I3(p->origin[L][0], p->origin[L][1], p->origin[L][2]); }
+
+/* axisTest / triBox, voxel.ts:231-248 (Akenine-Moller). */
+static int axisTest(v3 a, v3 v0, v3 v1, v3 v2, v3 h) {
+  float p0 = vdot(v0, a), p1 = vdot(v1, a), p2 = vdot(v2, a);
+  float r = vdot(h, vabs(a));
+  return !(fmn(fmn(p0, p1), p2) > r || fmx(fmx(p0, p1), p2) < -r);
+}
+static int triBox(v3 c, v3 h, v3 a, v3 b, v3 cc) {
+  v3 v0 = vsub(a, c), v1 = vsub(b, c), v2 = vsub(cc, c);
+  v3 mn = vmin(vmin(v0, v1), v2), mx = vmax(vmax(v0, v1), v2);
+  if (mn.x > h.x || mn.y > h.y || mn.z > h.z || mx.x < -h.x || mx.y < -h.y || mx.z < -h.z) return 0;
+  v3 e0 = vsub(v1, v0), e1 = vsub(v2, v1), e2 = vsub(v0, v2);
+  v3 n = vcross(e0, e1);
+  if (fabsf(vdot(n, v0)) > vdot(h, vabs(n))) return 0;
+  return axisTest(V3(0.0f, -e0.z, e0.y), v0, v1, v2, h) && axisTest(V3(0.0f, -e1.z, e1.y), v0, v1, v2, h) &&
+         axisTest(V3(0.0f, -e2.z, e2.y), v0, v1, v2, h) && axisTest(V3(e0.z, 0.0f, -e0.x), v0, v1, v2, h) &&
+         axisTest(V3(e1.z, 0.0f, -e1.x), v0, v1, v2, h) && axisTest(V3(e2.z, 0.0f, -e2.x), v0, v1, v2, h) &&
+         axisTest(V3(-e0.y, e0.x, 0.0f), v0, v1, v2, h) && axisTest(V3(-e1.y, e1.x, 0.0f), v0, v1, v2, h) &&
+         axisTest(V3(-e2.y, e2.x, 0.0f), v0, v1, v2, h);
+}
+int orc_tri_box(const float c[3], const float h[3], const float a[3], const float b[3], const float cc[3]) {
+  return triBox(V3(c[0], c[1], c[2]), V3(h[0], h[1], h[2]), V3(a[0], a[1], a[2]), V3(b[0], b[1], b[2]),
+                V3(cc[0], cc[1], cc[2]));
+}
+
+/* closestBary, voxel.ts:250-267 (Ericson, Real-Time Collision Detection 5.1.5). */
+static v3 closestBary(v3 p, v3 a, v3 b, v3 c) {
+  v3 ab = vsub(b, a), ac = vsub(c, a), ap = vsub(p, a);
+  float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return V3(1.0f, 0.0f, 0.0f);
+  v3 bp = vsub(p, b);
+  float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return V3(0.0f, 1.0f, 0.0f);
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float w = d1 / (d1 - d3);
+    return V3(1.0f - w, w, 0.0f);
+  }
+  v3 cp = vsub(p, c);
+  float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return V3(0.0f, 0.0f, 1.0f);
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    return V3(1.0f - w, 0.0f, w);
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return V3(0.0f, 1.0f - w, w);
+  }
+  float dn = 1.0f / (va + vb + vc);
+  float v = vb * dn, w = vc * dn;
+  return V3(1.0f - v - w, v, w);
+}
+void orc_closest_bary(const float p[3], const float a[3], const float b[3], const float c[3], float out[3]) {
+  v3 r = closestBary(V3(p[0], p[1], p[2]), V3(a[0], a[1], a[2]), V3(b[0], b[1], b[2]), V3(c[0], c[1], c[2]));
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+}
+
+typedef struct { v3 a, b, c; uint32_t i0, i1, i2, m; } Tri;
+static v3 P(const Ctx* X, uint32_t i) { return V3(X->pos[i * 3], X->pos[i * 3 + 1], X->pos[i * 3 + 2]); }
+static v3 N(const Ctx* X, uint32_t i) { return V3(X->nrm[i * 3], X->nrm[i * 3 + 1], X->nrm[i * 3 + 2]); }
+/* loadTri, voxel.ts:271-277 (instance 0: offset 0). */
+static Tri loadTri(const Ctx* X, uint32_t key) {
+  Tri T;
+  T.i0 = X->tri[key * 4];
+  T.i1 = X->tri[key * 4 + 1];
+  T.i2 = X->tri[key * 4 + 2];
+  T.m = X->tri[key * 4 + 3];
+  T.a = P(X, T.i0);
+  T.b = P(X, T.i1);
+  T.c = P(X, T.i2);
+  return T;
+}
+static v3 regionLo(const orc_vox* p, uint32_t L) { return vscale(vscale(vf(org(p, L)), voxSize(p, L)), 8.0f); }
+
+typedef struct { iv3 lo, n; } Range;
+/* voxRange, voxel.ts:293-300. */
+static Range voxRange(const Ctx* X, Tri T, uint32_t L, float cell) {
+  const orc_vox* p = X->p;
+  v3 lo = regionLo(p, L);
+  v3 mn = vmin(vmin(T.a, T.b), T.c), mx = vmax(vmax(T.a, T.b), T.c);
+  int lim = f2i((float)p->G * voxSize(p, L) * 8.0f / cell);
+  Range r;
+  iv3 a = iclamp(vi(vfloor(vdivs(vsub(mn, lo), cell))), I3(0, 0, 0), I3(lim, lim, lim));
+  iv3 b = iclamp(iadds(vi(vfloor(vdivs(vsub(mx, lo), cell))), 1), I3(0, 0, 0), I3(lim, lim, lim));
+  r.lo = a;
+  r.n = I3(imax(b.x - a.x, 0), imax(b.y - a.y, 0), imax(b.z - a.z, 0));
+  return r;
+}
+/* finerHolds / finerHoldsRange / finerBuilt / finerBuiltRange, voxel.ts:303-320. */
+static int finerHolds(const orc_vox* p, uint32_t L, iv3 c) {
+  if (L == 0) return 0;
+  iv3 f = isub(imuls(iadd(imuls(org(p, L), 8), c), 2), imuls(org(p, L - 1), 8));
+  int E = p->G * 8;
+  return f.x >= 0 && f.y >= 0 && f.z >= 0 && f.x < E && f.y < E && f.z < E;
+}
+static int finerHoldsRange(const orc_vox* p, uint32_t L, Range r, int s) {
+  return finerHolds(p, L, imuls(r.lo, s)) && finerHolds(p, L, iadds(imuls(iadd(r.lo, r.n), s), -1));
+}
+static int finerBuilt(const Ctx* X, uint32_t L, iv3 c) {
+  if (!finerHolds(X->p, L, c)) return 0;
+  if (X->vox[2] == 0u) return 1;
+  iv3 f = isub(imuls(iadd(imuls(org(X->p, L), 8), c), 2), imuls(org(X->p, L - 1), 8));
+  return X->vox[voxPtrIndex(X, L - 1, ishr(f, 3))] != V_DROP;
+}
+static int finerBuiltRange(const Ctx* X, uint32_t L, Range r) { return X->vox[2] == 0u && finerHoldsRange(X->p, L, r, 1); }
+
+static iv3 rangeCell(Range r, int q) { return iadd(r.lo, I3(q % r.n.x, (q / r.n.x) % r.n.y, q / (r.n.x * r.n.y))); }
+
+/* markBrick / markRange, voxel.ts:347-357. */
+static void markBrick(Ctx* X, uint32_t i) {
+  if (X->vox[i] == V_EMPTY) X->vox[i] = V_MARK;
+}
+static void markRange(Ctx* X, Tri T, uint32_t L, Range r) {
+  float bs = voxSize(X->p, L) * 8.0f;
+  v3 lo = regionLo(X->p, L);
+  int n = r.n.x * r.n.y * r.n.z;
+  for (int j = 0; j < n; j++) {
+    iv3 b = rangeCell(r, j);
+    if (finerHolds(X->p, L, imuls(b, 8)) && finerHolds(X->p, L, iadds(imuls(b, 8), 7))) continue;
+    float hh = 0.5f * bs * 1.0001f;
+    if (n == 1 || triBox(vadd(lo, vscale(vadd(vf(b), V3(0.5f, 0.5f, 0.5f)), bs)), V3(hh, hh, hh), T.a, T.b, T.c))
+      markBrick(X, voxPtrIndex(X, L, b));
+  }
+}
+/* markBricks, voxel.ts:359-376 (one invocation per triangle). */
+static void markBricks(Ctx* X, uint32_t t) {
+  Tri T = loadTri(X, t);
+  for (uint32_t L = 0; L < (uint32_t)X->p->levels; L++) {
+    Range r = voxRange(X, T, L, voxSize(X->p, L) * 8.0f);
+    int n = r.n.x * r.n.y * r.n.z;
+    if (n == 0 || finerHoldsRange(X->p, L, r, 8)) continue;
+    if (n > 8) {
+      uint32_t k = X->vox[24 + L]++;
+      if (k < BIG_CAP) X->big[8 + L][k] = t; else X->vox[3]++;
+      continue;
+    }
+    markRange(X, T, L, r);
+  }
+}
+/* markBricksBig, voxel.ts:377-387. */
+static void markBricksBig(Ctx* X) {
+  for (uint32_t L = 0; L < (uint32_t)X->p->levels; L++) {
+    uint32_t n = X->vox[24 + L] < BIG_CAP ? X->vox[24 + L] : BIG_CAP;
+    for (uint32_t k = 0; k < n; k++) {
+      Tri T = loadTri(X, X->big[8 + L][k]);
+      markRange(X, T, L, voxRange(X, T, L, voxSize(X->p, L) * 8.0f));
+    }
+  }
+}
+/* markParents, voxel.ts:390-403. */
+static void markParents(Ctx* X) {
+  uint32_t G = (uint32_t)X->p->G, per = G * G * G, NL = (uint32_t)X->p->levels;
+  for (uint32_t i = 0; i < (NL - 1) * per; i++) {
+    if (X->vox[X->ptrOff + i] != V_MARK) continue;
+    uint32_t L = i / per, r = i % per;
+    iv3 w = iadd(org(X->p, L), I3((int)(r % G), (int)((r / G) % G), (int)(r / (G * G))));
+    for (L++; L < NL; L++) {
+      w = ishr(w, 1);
+      iv3 b = isub(w, org(X->p, L));
+      if (b.x < 0 || b.y < 0 || b.z < 0 || b.x >= (int)G || b.y >= (int)G || b.z >= (int)G) break;
+      markBrick(X, voxPtrIndex(X, L, b));
+    }
+  }
+}
+/* countMarks / scanMarks / allocBricks, voxel.ts:411-470, in the ordered form (coarse levels first, pointer index
+ * ascending within a level; ranks past maxBricks dropped). */
+static void allocBricks(Ctx* X) {
+  uint32_t G = (uint32_t)X->p->G, per = G * G * G, NL = (uint32_t)X->p->levels, mb = (uint32_t)X->p->max_bricks;
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < NL * per; i++) total += X->vox[X->ptrOff + i] == V_MARK;
+  X->vox[5] = total;
+  X->vox[0] = total;
+  uint32_t idx = 0;
+  for (int L = (int)NL - 1; L >= 0; L--)
+    for (uint32_t r = 0; r < per; r++) {
+      uint32_t i = (uint32_t)L * per + r;
+      if (X->vox[X->ptrOff + i] != V_MARK) continue;
+      uint32_t k = idx++;
+      if (k >= mb) {
+        X->vox[X->ptrOff + i] = V_DROP;
+        X->vox[2]++;
+        continue;
+      }
+      X->vox[8 + L]++;
+      X->vox[X->ptrOff + i] = k;
+      for (uint32_t w = 0; w < 17; w++) X->vox[X->brickOff + k * BW + w] = 0;
+      X->vox[X->brickOff + k * BW + 17] =
+          ((uint32_t)L << 27) | ((r % G) << 18) | (((r / G) % G) << 9) | (r / (G * G));
+    }
+}
+static uint32_t nBricks(const Ctx* X) { return X->vox[0] < (uint32_t)X->p->max_bricks ? X->vox[0] : (uint32_t)X->p->max_bricks; }
+
+/* voxelizeOne, voxel.ts:476-490 (no alpha test: opaque untextured materials). */
+static void voxelizeOne(Ctx* X, Tri T, uint32_t t, uint32_t L, iv3 c, int single) {
+  if (finerBuilt(X, L, c)) return;
+  float v = voxSize(X->p, L);
+  v3 p = vadd(regionLo(X->p, L), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+  float hh = 0.5f * v * 1.0001f;
+  if (!single && !triBox(p, V3(hh, hh, hh), T.a, T.b, T.c)) return;
+  uint32_t ptr = X->vox[voxPtrIndex(X, L, ishr(c, 3))];
+  if (ptr >= V_MARK) return;
+  uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+  X->vox[X->brickOff + ptr * BW + (bit >> 5)] |= 1u << (bit & 31u);
+  uint32_t* rp = &X->rep[ptr * 512u + bit];
+  if (*rp < t + 1u) *rp = t + 1u;
+}
+/* voxelizeSmall, voxel.ts:492-510. */
+static void voxelizeSmall(Ctx* X, uint32_t t) {
+  Tri T = loadTri(X, t);
+  for (uint32_t L = 0; L < (uint32_t)X->p->levels; L++) {
+    Range r = voxRange(X, T, L, voxSize(X->p, L));
+    int n = r.n.x * r.n.y * r.n.z;
+    if (n == 0 || finerBuiltRange(X, L, r)) continue;
+    if (n > BIG) {
+      uint32_t k = X->vox[16 + L]++;
+      if (k < BIG_CAP) X->big[L][k] = t; else X->vox[3]++;
+      continue;
+    }
+    for (int k = 0; k < n; k++) voxelizeOne(X, T, t, L, rangeCell(r, k), n == 1);
+  }
+}
+/* voxelizeBig, voxel.ts:515-531. */
+static void voxelizeBig(Ctx* X) {
+  for (uint32_t L = 0; L < (uint32_t)X->p->levels; L++) {
+    uint32_t n = X->vox[16 + L] < BIG_CAP ? X->vox[16 + L] : BIG_CAP;
+    for (uint32_t k = 0; k < n; k++) {
+      uint32_t t = X->big[L][k];
+      Tri T = loadTri(X, t);
+      Range r = voxRange(X, T, L, voxSize(X->p, L));
+      int m = r.n.x * r.n.y * r.n.z;
+      for (int j = 0; j < m; j++) voxelizeOne(X, T, t, L, rangeCell(r, j), 0);
+    }
+  }
+}
+/* reduce, voxel.ts:540-577: level L from L-1, per occupancy word of a finer brick. */
+static void reduceLevel(Ctx* X, uint32_t L) {
+  uint32_t nb = nBricks(X);
+  for (uint32_t i = 0; i < nb * 16u; i++) {
+    uint32_t ptr = i >> 4, w = i & 15u;
+    uint32_t base = X->brickOff + ptr * BW;
+    uint32_t code = X->vox[base + 17];
+    uint32_t Lc = code >> 27;
+    if (Lc + 1 != L) continue;
+    uint32_t word = X->vox[base + w];
+    if (word == 0) continue;
+    iv3 b = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+    iv3 c0 = iadd(imuls(b, 8), I3(0, (int)((w & 1u) * 4u), (int)(w >> 1)));
+    iv3 pc0 = isub(ishr(iadd(imuls(org(X->p, Lc), 8), c0), 1), imuls(org(X->p, L), 8));
+    uint32_t pptr = X->vox[voxPtrIndex(X, L, ishr(pc0, 3))];
+    if (pptr >= V_MARK) continue;
+    uint32_t pbit0 = (uint32_t)((pc0.x & 7) + 8 * (pc0.y & 7) + 64 * (pc0.z & 7));
+    uint32_t mask = 0, lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+    while (word) {
+      uint32_t k = (uint32_t)__builtin_ctz(word);
+      word &= word - 1u;
+      uint32_t px = (k & 7u) >> 1, py = k >> 4;
+      mask |= 1u << ((pbit0 + px + 8u * py) & 31u);
+      uint32_t rv = X->rep[ptr * 512u + w * 32u + k];
+      if (py == 0) { if (rv > lo[px]) lo[px] = rv; } else { if (rv > hi[px]) hi[px] = rv; }
+    }
+    X->vox[X->brickOff + pptr * BW + (pbit0 >> 5)] |= mask;
+    for (uint32_t px = 0; px < 4; px++)
+      for (uint32_t py = 0; py < 2; py++) {
+        uint32_t rv = py == 0 ? lo[px] : hi[px];
+        if (rv == 0) continue;
+        uint32_t* rp = &X->rep[pptr * 512u + pbit0 + px + 8u * py];
+        if (*rp < rv) *rp = rv;
+      }
+  }
+}
+/* allocVoxels, voxel.ts:579-588, bricks in slot order. */
+static void allocVoxels(Ctx* X) {
+  uint32_t nb = nBricks(X);
+  for (uint32_t b = 0; b < nb; b++) {
+    uint32_t base = X->brickOff + b * BW, n = 0;
+    for (uint32_t w = 0; w < 16; w++) n += (uint32_t)popc(X->vox[base + w]);
+    uint32_t first = X->vox[1];
+    X->vox[1] += n;
+    if (first + n > (uint32_t)X->p->max_voxels) {
+      X->vox[base + 16] = V_EMPTY;
+      X->vox[4]++;
+      continue;
+    }
+    X->vox[base + 16] = first;
+  }
+}
+/* WGSL pack4x8unorm / pack2x16snorm by the specification's formulas. */
+static uint32_t pack4x8unorm(const float e[4]) {
+  uint32_t r = 0;
+  for (int i = 0; i < 4; i++) r |= ((uint32_t)floorf(0.5f + 255.0f * fmn(1.0f, fmx(0.0f, e[i])))) << (8 * i);
+  return r;
+}
+static uint32_t pack2x16snorm(float x, float y) {
+  int a = (int)floorf(0.5f + 32767.0f * fmn(1.0f, fmx(-1.0f, x)));
+  int b = (int)floorf(0.5f + 32767.0f * fmn(1.0f, fmx(-1.0f, y)));
+  return ((uint32_t)a & 0xffffu) | (((uint32_t)b & 0xffffu) << 16);
+}
+uint32_t orc_pack4x8unorm(const float e[4]) { return pack4x8unorm(e); }
+uint32_t orc_pack2x16snorm(float x, float y) { return pack2x16snorm(x, y); }
+/* octEnc, voxel.ts:584-588. */
+static void octEnc(v3 n, float* px, float* py) {
+  float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+  float x = n.x / s, y = n.y / s;
+  if (n.z < 0.0f) {
+    float ox = x, oy = y;
+    x = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+    y = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+  }
+  *px = x;
+  *py = y;
+}
+void orc_oct_enc(const float n[3], float out[2]) { octEnc(V3(n[0], n[1], n[2]), &out[0], &out[1]); }
+/* resolveAttr, voxel.ts:590-620, with surfaceAt(wantAttr) (voxel.ts:325-336): white texture -> albedo =
+ * baseColorFactor; toon flag 0. */
+static void resolveAttr(Ctx* X) {
+  uint32_t nb = nBricks(X);
+  for (uint32_t i = 0; i < nb * 512u; i++) {
+    uint32_t ptr = i >> 9, bit = i & 511u;
+    uint32_t base = X->brickOff + ptr * BW, w = bit >> 5;
+    uint32_t word = X->vox[base + w];
+    if ((word & (1u << (bit & 31u))) == 0) continue;
+    uint32_t first = X->vox[base + 16];
+    if (first == V_EMPTY) continue;
+    uint32_t r = (uint32_t)popc(word & ((1u << (bit & 31u)) - 1u));
+    for (uint32_t j = 0; j < w; j++) r += (uint32_t)popc(X->vox[base + j]);
+    uint32_t ai = first + r;
+    uint32_t code = X->vox[base + 17];
+    uint32_t L = code >> 27;
+    iv3 b = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+    iv3 c = iadd(imuls(b, 8), I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+    v3 p = vadd(regionLo(X->p, L), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), voxSize(X->p, L)));
+    Tri T = loadTri(X, X->rep[ptr * 512u + bit] - 1u);
+    v3 wb = closestBary(p, T.a, T.b, T.c);
+    v3 n = vadd(vadd(vscale(N(X, T.i0), wb.x), vscale(N(X, T.i1), wb.y)), vscale(N(X, T.i2), wb.z));
+    if (!(vdot(n, n) > 1e-12f)) n = vcross(vsub(T.b, T.a), vsub(T.c, T.a));
+    n = vnormalize(n);
+    v3 q = vadd(vadd(vscale(T.a, wb.x), vscale(T.b, wb.y)), vscale(T.c, wb.z));
+    const float* m = X->mat + (size_t)T.m * 4;
+    float alb[4] = {1.0f * m[0], 1.0f * m[1], 1.0f * m[2], 0.0f};
+    float ox, oy;
+    octEnc(n, &ox, &oy);
+    X->vox[X->attrOff + ai * 2] = pack4x8unorm(alb);
+    X->vox[X->attrOff + ai * 2 + 1] = pack2x16snorm(ox, oy);
+    float pl = vdot(n, vsub(q, p));
+    memcpy(&X->vox[X->plOff + ai], &pl, 4);
+  }
+}
+
+int orc_build(const orc_vox* p, const float* pos, const float* nrm, const uint32_t* tri, uint32_t ntris,
+              const float* mat, uint32_t* vox, uint32_t* rep) {
+  if (p->levels < 1 || p->levels > 8) return -1;
+  Ctx X;
+  memset(&X, 0, sizeof(X));
+  X.p = p;
+  X.vox = vox;
+  X.rep = rep;
+  X.pos = pos;
+  X.nrm = nrm;
+  X.tri = tri;
+  X.mat = mat;
+  X.ntris = ntris;
+  layout(p, &X.ptrOff, &X.brickOff, &X.attrOff, &X.plOff);
+  uint32_t cap = ntris < BIG_CAP ? ntris : BIG_CAP;
+  for (int s = 0; s < 16; s++) {
+    X.big[s] = (uint32_t*)malloc(sizeof(uint32_t) * (cap ? cap : 1));
+    if (!X.big[s]) {
+      for (int q = 0; q < s; q++) free(X.big[q]);
+      return -2;
+    }
+  }
+  /* clear, voxel.ts:338-344 */
+  uint32_t G = (uint32_t)p->G, NLper = (uint32_t)p->levels * G * G * G;
+  memset(vox, 0, orc_vox_words(p) * 4);
+  for (uint32_t i = 0; i < NLper; i++) vox[X.ptrOff + i] = V_EMPTY;
+  for (uint32_t t = 0; t < ntris; t++) markBricks(&X, t);
+  markBricksBig(&X);
+  if (p->levels > 1) markParents(&X);
+  allocBricks(&X);
+  /* clearKeys, voxel.ts:471-474 */
+  memset(rep, 0, (size_t)nBricks(&X) * 512u * 4u);
+  for (uint32_t t = 0; t < ntris; t++) voxelizeSmall(&X, t);
+  voxelizeBig(&X);
+  for (uint32_t L = 1; L < (uint32_t)p->levels; L++) reduceLevel(&X, L);
+  allocVoxels(&X);
+  resolveAttr(&X);
+  for (int s = 0; s < 16; s++) free(X.big[s]);
+  return 0;
+}
+
+int orc_canonical(const orc_vox* p, const uint32_t* vox, const uint32_t* rep, int level, uint32_t* occ_bits,
+                  uint32_t* rep_out, uint32_t* albedo, uint32_t* normal, float* plane, uint8_t* brick_state) {
+  uint32_t ptrOff, brickOff, attrOff, plOff;
+  layout(p, &ptrOff, &brickOff, &attrOff, &plOff);
+  int G = p->G, E = G * 8;
+  uint32_t per = (uint32_t)(G * G * G);
+  uint32_t nb = vox[0] < (uint32_t)p->max_bricks ? vox[0] : (uint32_t)p->max_bricks;
+  size_t ncell = (size_t)E * E * E;
+  if (occ_bits) memset(occ_bits, 0, ncell / 8);
+  if (rep_out) memset(rep_out, 0, ncell * 4);
+  if (albedo) memset(albedo, 0, ncell * 4);
+  if (normal) memset(normal, 0, ncell * 4);
+  if (plane) memset(plane, 0, ncell * 4);
+  for (int bz = 0; bz < G; bz++)
+    for (int by = 0; by < G; by++)
+      for (int bx = 0; bx < G; bx++) {
+        uint32_t bi = ((uint32_t)bz * G + by) * G + bx;
+        uint32_t ptr = vox[ptrOff + (uint32_t)level * per + bi];
+        uint8_t st = ptr == V_DROP ? 2 : (ptr >= V_MARK ? 0 : 1);
+        if (st == 1 && ptr >= nb) return -1;
+        if (st == 1 && vox[brickOff + ptr * BW + 16] == V_EMPTY) st = 3;
+        if (brick_state) brick_state[bi] = st;
+        if (st != 1 && st != 3) continue;
+        uint32_t rank = 0;
+        for (int bit = 0; bit < 512; bit++) {
+          uint32_t word = vox[brickOff + ptr * BW + (bit >> 5)];
+          if (!(word & (1u << (bit & 31)))) continue;
+          int x = bx * 8 + (bit & 7), y = by * 8 + ((bit >> 3) & 7), z = bz * 8 + (bit >> 6);
+          size_t ci = (size_t)x + (size_t)E * ((size_t)y + (size_t)E * z);
+          if (occ_bits) occ_bits[ci >> 5] |= 1u << (ci & 31);
+          if (rep_out) rep_out[ci] = rep[(size_t)ptr * 512 + bit];
+          if (st == 1) {
+            uint32_t ai = vox[brickOff + ptr * BW + 16] + rank;
+            if (albedo) albedo[ci] = vox[attrOff + ai * 2];
+            if (normal) normal[ci] = vox[attrOff + ai * 2 + 1];
+            if (plane) memcpy(&plane[ci], &vox[plOff + ai], 4);
+          }
+          rank++;
+        }
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Cascades.                                                                                                     */
+
+/* octEncode (frame.ts:97-103) in double; octSolidAngles (frame.ts:105-114). */
+static void octEncodeD(double x, double y, double z, double* u, double* v) {
+  double s = fabs(x) + fabs(y) + fabs(z);
+  double px = x / s, py = y / s;
+  if (z < 0) {
+    double ox = px, oy = py;
+    px = (1 - fabs(oy)) * (ox >= 0 ? 1 : -1);
+    py = (1 - fabs(ox)) * (oy >= 0 ? 1 : -1);
+  }
+  *u = px * 0.5 + 0.5;
+  *v = py * 0.5 + 0.5;
+}
+void orc_oct_solid_angles(int res, int samples, float* out) {
+  double* counts = (double*)calloc((size_t)res * res, sizeof(double));
+  double golden = M_PI * (3 - sqrt(5.0));
+  for (int i = 0; i < samples; i++) {
+    double z = 1 - (2.0 * i + 1) / samples, r = sqrt(1 - z * z), phi = golden * i, u, v;
+    octEncodeD(r * cos(phi), r * sin(phi), z, &u, &v);
+    int iv = (int)floor(v * res), iu = (int)floor(u * res);
+    if (iv > res - 1) iv = res - 1;
+    if (iu > res - 1) iu = res - 1;
+    counts[iv * res + iu] += 1;
+  }
+  for (int i = 0; i < res * res; i++) out[i] = (float)(counts[i] / samples * 4 * M_PI);
+  free(counts);
+}
+
+static int resOf(const orc_rc* rc, int c) { return 4 << (c < rc->res_cap ? c : rc->res_cap); }
+static int probesOf(const orc_rc* rc, int c) { return rc->probes0 >> c; }
+uint64_t orc_rays(const orc_rc* rc, int c) {
+  uint64_t n = (uint64_t)probesOf(rc, c), r = (uint64_t)resOf(rc, c);
+  return n * n * n * r * r;
+}
+static uint64_t rayOff(const orc_rc* rc, int c) {
+  uint64_t o = 0;
+  for (int k = 0; k < c; k++) o += orc_rays(rc, k);
+  return o;
+}
+
+void orc_cascade_setup(orc_rc* rc, double v0, int levels, double s0, double k_interval, double level_map,
+                       double vox_scale, const double eye[3]) {
+  rc->s0 = (float)s0;
+  for (int c = 0; c < rc->cascades; c++) {
+    rc->b0[c] = (float)(k_interval * (pow(4.0, c) - 1) / 3);
+    rc->b1[c] = (float)(k_interval * (pow(4.0, c + 1) - 1) / 3);
+    /* voxLevel, rc.ts:210-213, in double; round half to even like WGSL round(). */
+    double v = vox_scale * s0 * pow(2.0, level_map * c);
+    double L = nearbyint(log2(v / v0));
+    if (L < 0) L = 0;
+    if (L > levels - 1) L = levels - 1;
+    rc->vox_level[c] = (int)L;
+  }
+  /* Dense grid snapped to the top cascade's spacing (every grid nests in the next). */
+  int top = rc->cascades - 1;
+  double st = s0 * pow(2.0, top);
+  int ntop = rc->probes0 >> top;
+  for (int a = 0; a < 3; a++) rc->G0[a] = (int32_t)(floor(eye[a] / st) - ntop / 2) * (1 << top);
+}
+
+/* dirOf / octDecode, rc.ts:291-298, 308. */
+static v3 dirOf(int di, int res) {
+  float u = ((float)(di % res) + 0.5f) / (float)res, w = ((float)(di / res) + 0.5f) / (float)res;
+  float fx = u * 2.0f - 1.0f, fy = w * 2.0f - 1.0f;
+  v3 n = V3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
+  float t = fmx(-n.z, 0.0f);
+  n.x += n.x >= 0.0f ? -t : t;
+  n.y += n.y >= 0.0f ? -t : t;
+  return vnormalize(n);
+}
+
+typedef struct { const orc_vox* p; const uint32_t* vox; uint32_t ptrOff, brickOff, attrOff, plOff; } View;
+
+static uint32_t vPtr(const View* W, uint32_t L, iv3 b) {
+  uint32_t G = (uint32_t)W->p->G;
+  return W->vox[W->ptrOff + L * G * G * G + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x];
+}
+/* voxRank, voxel.ts:81-87; a brick without attribute slots has none (returns V_EMPTY). */
+static uint32_t voxRank(const View* W, uint32_t brick, uint32_t bit) {
+  uint32_t base = W->brickOff + brick * BW, w = bit >> 5;
+  if (W->vox[base + 16] == V_EMPTY) return V_EMPTY;
+  uint32_t r = (uint32_t)popc(W->vox[base + w] & ((1u << (bit & 31u)) - 1u));
+  for (uint32_t i = 0; i < w; i++) r += (uint32_t)popc(W->vox[base + i]);
+  return W->vox[base + 16] + r;
+}
+/* voxAt, voxel.ts:90-107. Returns 1 with (brick, bit, level, cell) for an occupied voxel. */
+static int voxAt(const View* W, uint32_t L, iv3 c, uint32_t* brick, uint32_t* bit, uint32_t* lev, iv3* cell) {
+  uint32_t l = L;
+  iv3 cc = c;
+  int E = W->p->G * 8;
+  for (;;) {
+    uint32_t ptr = vPtr(W, l, ishr(cc, 3));
+    if (ptr != V_DROP) {
+      if (ptr >= V_MARK) return 0;
+      uint32_t b = (uint32_t)((cc.x & 7) + 8 * (cc.y & 7) + 64 * (cc.z & 7));
+      if ((W->vox[W->brickOff + ptr * BW + (b >> 5)] & (1u << (b & 31u))) == 0u) return 0;
+      *brick = ptr; *bit = b; *lev = l; *cell = cc;
+      return 1;
+    }
+    if (l + 1u >= (uint32_t)W->p->levels) return 0;
+    iv3 w = ishr(iadd(imuls(org(W->p, l), 8), cc), 1);
+    l++;
+    cc = isub(w, imuls(org(W->p, l), 8));
+    if (cc.x < 0 || cc.y < 0 || cc.z < 0 || cc.x >= E || cc.y >= E || cc.z >= E) return 0;
+  }
+}
+
+#define PI_F ((float)3.14159265)
+
+/* shadeHitV (rc.ts:108-116) at a voxel hit with the voxel's attributes (voxel.ts:150-156, 185-192), no bounce,
+ * the frame's sun visibility; plus the point lights of this path (none: the reference shading). */
+static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entry) {
+  uint32_t pa = 0, pn = 0;
+  if (ai != V_EMPTY) {
+    pa = W->vox[W->attrOff + ai * 2];
+    pn = W->vox[W->attrOff + ai * 2 + 1];
+  }
+  v3 alb = V3((float)(pa & 255u) / 255.0f, (float)((pa >> 8) & 255u) / 255.0f, (float)((pa >> 16) & 255u) / 255.0f);
+  float nx = fmx((float)(int16_t)(uint16_t)(pn & 0xffffu) / 32767.0f, -1.0f);
+  float ny = fmx((float)(int16_t)(uint16_t)(pn >> 16) / 32767.0f, -1.0f);
+  v3 n0 = V3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
+  float q = fmx(-n0.z, 0.0f);
+  n0.x += n0.x >= 0.0f ? -q : q;
+  n0.y += n0.y >= 0.0f ? -q : q;
+  v3 n = vdot(n0, n0) > 0.01f ? n0 : vneg(d);
+  if (vdot(n, d) > 0.0f) n = vneg(n);
+  v3 b = vscale(vdivs(alb, PI_F), 0.0f);
+  v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]);
+  float ndl = fmx(vdot(n, sd), 0.0f);
+  v3 out = b;
+  if (ndl > 0.0f)
+    out = vadd(b, vscale(vscale(vmul(vdivs(alb, PI_F), V3(rc->sun_e[0], rc->sun_e[1], rc->sun_e[2])), ndl), rc->sun_vis));
+  for (int k = 0; k < rc->n_point; k++) {
+    v3 l = vsub(V3(rc->point_pos[k][0], rc->point_pos[k][1], rc->point_pos[k][2]), entry);
+    float r2 = vdot(l, l);
+    float nl = fmx(vdot(n, l) / sqrtf(r2), 0.0f);
+    if (nl > 0.0f)
+      out = vadd(out, vscale(vmul(vdivs(alb, PI_F), V3(rc->point_i[k][0], rc->point_i[k][1], rc->point_i[k][2])), nl / r2));
+  }
+  return out;
+}
+
+static uint32_t hitId(const orc_vox* p, uint32_t L, iv3 c) {
+  int E = p->G * 8;
+  return (L << 29) | (uint32_t)(c.x + E * (c.y + E * c.z));
+}
+
+/* traceVox, voxel.ts:121-202. out: rgb, T. */
+static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, float t1, float tSkip, uint32_t L,
+                     float out[4], uint32_t* hid) {
+  const orc_vox* p = W->p;
+  *hid = 0xffffffffu;
+  out[0] = out[1] = out[2] = 0.0f;
+  out[3] = 1.0f;
+  float v = voxSize(p, L);
+  int G = p->G;
+  v3 lo = vscale(vscale(vf(org(p, L)), v), 8.0f);
+  float span = (float)G * v * 8.0f;
+  v3 hi = vadd(lo, V3(span, span, span));
+  v3 dd = V3(fabsf(d.x) < 1e-9f ? 1e-9f : d.x, fabsf(d.y) < 1e-9f ? 1e-9f : d.y, fabsf(d.z) < 1e-9f ? 1e-9f : d.z);
+  v3 inv = V3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  v3 ta = vmul(vsub(lo, o), inv), tb = vmul(vsub(hi, o), inv);
+  v3 tn = vmin(ta, tb), tf = vmax(ta, tb);
+  float t = fmx(fmx(fmx(tn.x, tn.y), tn.z), t0);
+  float tEnd = fmn(fmn(fmn(tf.x, tf.y), tf.z), t1);
+  if (t >= tEnd) return;
+  iv3 st = I3(dd.x > 0 ? 1 : -1, dd.y > 0 ? 1 : -1, dd.z > 0 ? 1 : -1);
+  v3 sp = V3(st.x > 0 ? 1.0f : 0.0f, st.y > 0 ? 1.0f : 0.0f, st.z > 0 ? 1.0f : 0.0f);
+  int nVox = G * 8;
+  iv3 c = iclamp(vi(vfloor(vdivs(vsub(vadd(o, vscale(d, t)), lo), v))), I3(0, 0, 0), I3(nVox - 1, nVox - 1, nVox - 1));
+  v3 tMax = vmul(vsub(vadd(vscale(vadd(vf(c), sp), v), lo), o), inv);
+  v3 tDelta = vabs(vscale(inv, v));
+  float tIn = t;
+  for (int it = 0; it < 1024; it++) {
+    iv3 b = ishr(c, 3);
+    uint32_t ptr = vPtr(W, L, b);
+    if (ptr == V_DROP) {
+      uint32_t br, bit, lev;
+      iv3 cell;
+      if (voxAt(W, L, c, &br, &bit, &lev, &cell) && tIn >= tSkip) {
+        *hid = hitId(p, lev, cell);
+        v3 r = shadeVoxel(W, rc, voxRank(W, br, bit), d, vadd(o, vscale(d, tIn)));
+        out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
+        return;
+      }
+      float tx = fmn(fmn(tMax.x, tMax.y), tMax.z);
+      if (tx >= tEnd) break;
+      tIn = tx;
+      if (tMax.x <= tMax.y && tMax.x <= tMax.z) { c.x += st.x; tMax.x += tDelta.x; }
+      else if (tMax.y <= tMax.z) { c.y += st.y; tMax.y += tDelta.y; }
+      else { c.z += st.z; tMax.z += tDelta.z; }
+      if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= nVox || c.y >= nVox || c.z >= nVox) break;
+      continue;
+    }
+    if (ptr >= V_MARK) {
+      v3 bt = vmul(vsub(vadd(vscale(vadd(vf(imuls(b, 8)), vscale(sp, 8.0f)), v), lo), o), inv);
+      float te = fmn(fmn(bt.x, bt.y), bt.z);
+      if (te >= tEnd) break;
+      iv3 nb = b;
+      if (bt.x <= bt.y && bt.x <= bt.z) nb.x += st.x; else if (bt.y <= bt.z) nb.y += st.y; else nb.z += st.z;
+      if (nb.x < 0 || nb.y < 0 || nb.z < 0 || nb.x >= G || nb.y >= G || nb.z >= G) break;
+      t = te;
+      tIn = te;
+      c = iclamp(vi(vfloor(vdivs(vsub(vadd(o, vscale(d, t)), lo), v))), imuls(nb, 8), iadds(imuls(nb, 8), 7));
+      tMax = vmul(vsub(vadd(vscale(vadd(vf(c), sp), v), lo), o), inv);
+      continue;
+    }
+    uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+    uint32_t occ = W->vox[W->brickOff + ptr * BW + (bit >> 5)];
+    float tExit = fmn(fmn(tMax.x, tMax.y), tMax.z);
+    if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
+      *hid = hitId(p, L, c);
+      v3 r = shadeVoxel(W, rc, voxRank(W, ptr, bit), d, vadd(o, vscale(d, tIn)));
+      out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
+      return;
+    }
+    if (tExit >= tEnd) break;
+    tIn = tExit;
+    if (tMax.x <= tMax.y && tMax.x <= tMax.z) { c.x += st.x; tMax.x += tDelta.x; }
+    else if (tMax.y <= tMax.z) { c.y += st.y; tMax.y += tDelta.y; }
+    else { c.z += st.z; tMax.z += tDelta.z; }
+    if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= nVox || c.y >= nVox || c.z >= nVox) break;
+  }
+}
+
+/* voxInside / voxLevelFor, voxel.ts:108-119. */
+static uint32_t voxLevelFor(const orc_vox* p, uint32_t L, v3 e) {
+  uint32_t l = L, n = (uint32_t)p->levels;
+  for (;;) {
+    if (!(l + 1u < n)) break;
+    float bs = voxSize(p, l) * 8.0f;
+    v3 lo = vscale(vf(org(p, l)), bs);
+    float span = (float)p->G * bs;
+    v3 hi = vadd(lo, V3(span, span, span));
+    int in = e.x > lo.x && e.y > lo.y && e.z > lo.z && e.x < hi.x && e.y < hi.y && e.z < hi.z;
+    if (in) break;
+    l++;
+  }
+  return l;
+}
+
+static int cascadeOf(const orc_rc* rc, uint64_t ray, uint64_t* local) {
+  uint64_t off = 0;
+  for (int c = 0; c < rc->cascades; c++) {
+    uint64_t n = orc_rays(rc, c);
+    if (ray < off + n) { *local = ray - off; return c; }
+    off += n;
+  }
+  return -1;
+}
+
+/* trace (rc.ts:600-622) -> traceWorld (rc.ts:216-245) with the probe at its grid point, normal 0. */
+static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid) {
+  uint64_t r = 0;
+  int c = cascadeOf(rc, ray, &r);
+  int res = resOf(rc, c), n = probesOf(rc, c);
+  uint64_t dirs = (uint64_t)res * res;
+  uint64_t pi = r / dirs;
+  int di = (int)(r % dirs);
+  iv3 g = I3((rc->G0[0] >> c) + (int)(pi % n), (rc->G0[1] >> c) + (int)((pi / n) % n), (rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n)));
+  float s = rc->s0 * (float)(1u << c);
+  v3 o = vscale(vf(g), s);
+  v3 d = dirOf(di, res);
+  float t0 = rc->b0[c] * rc->s0, t1 = rc->b1[c] * rc->s0;
+  uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
+  float v = voxSize(W->p, L);
+  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid);
+}
+
+static void makeView(View* W, const orc_vox* p, const uint32_t* vox) {
+  W->p = p;
+  W->vox = vox;
+  layout(p, &W->ptrOff, &W->brickOff, &W->attrOff, &W->plOff);
+}
+
+int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
+              uint32_t* hits, int threads) {
+  View W;
+  makeView(&W, p, vox);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads)
+#endif
+  for (int64_t q = (int64_t)r0; q < (int64_t)r1; q++) {
+    float out[4];
+    uint32_t h;
+    traceRay(&W, rc, (uint64_t)q, out, &h);
+    if (rgba) memcpy(rgba + (size_t)(q - (int64_t)r0) * 4, out, 16);
+    if (hits) hits[q - (int64_t)r0] = h;
+  }
+  (void)threads;
+  return 0;
+}
+
+/* upperAvg / mergeWorld (rc.ts:681-708) with the dense grid's parents, then projectSH (rc.ts:745-774). */
+int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
+  int Nc = rc->cascades;
+  for (int c = Nc - 1; c >= 0; c--) {
+    int res = resOf(rc, c), n = probesOf(rc, c);
+    uint64_t off = rayOff(rc, c), nr = orc_rays(rc, c), dirs = (uint64_t)res * res;
+    for (uint64_t t = 0; t < nr; t++) {
+      float* v = rgba + (off + t) * 4;
+      v3 up = V3(rc->sky[0], rc->sky[1], rc->sky[2]);
+      if (c + 1 < Nc && v[3] > 0.0f) {
+        int ur = resOf(rc, c + 1), un = probesOf(rc, c + 1);
+        uint64_t uoff = rayOff(rc, c + 1);
+        uint64_t pi = t / dirs;
+        int di = (int)(t % dirs);
+        int u0 = di % res, w0 = di / res;
+        iv3 g = I3((int)(pi % n), (int)((pi / n) % n), (int)(pi / ((uint64_t)n * n)));
+        v3 acc = V3(0, 0, 0);
+        float ws = 0.0f;
+        for (int k = 0; k < 8; k++) {
+          iv3 o = I3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+          /* linkParents (rc.ts:540-570): an odd axis may take the next upper point; absent outside the grid. */
+          if ((o.x > 0 && !(g.x & 1)) || (o.y > 0 && !(g.y & 1)) || (o.z > 0 && !(g.z & 1))) continue;
+          iv3 q = iadd(ishr(g, 1), o);
+          if (q.x >= un || q.y >= un || q.z >= un) continue;
+          uint64_t qi = (uint64_t)q.x + (uint64_t)un * ((uint64_t)q.y + (uint64_t)un * q.z);
+          const float* U = rgba + (uoff + qi * (uint64_t)ur * ur) * 4;
+          v3 a;
+          if (ur == res) {
+            const float* x = U + ((uint64_t)w0 * ur + u0) * 4;
+            a = V3(x[0], x[1], x[2]);
+          } else {
+            const float* b = U + ((uint64_t)w0 * 2 * ur + (uint64_t)u0 * 2) * 4;
+            const float* b1 = b + 4;
+            const float* b2 = b + (uint64_t)ur * 4;
+            const float* b3 = b2 + 4;
+            a = vscale(vadd(vadd(vadd(V3(b[0], b[1], b[2]), V3(b1[0], b1[1], b1[2])), V3(b2[0], b2[1], b2[2])), V3(b3[0], b3[1], b3[2])), 0.25f);
+          }
+          acc = vadd(acc, a);
+          ws += 1.0f;
+        }
+        up = ws > 0.0f ? vdivs(acc, fmx(ws, 1.0f)) : V3(0, 0, 0);
+      }
+      v[0] = v[0] + v[3] * up.x;
+      v[1] = v[1] + v[3] * up.y;
+      v[2] = v[2] + v[3] * up.z;
+      v[3] = 1.0f;
+    }
+  }
+  if (sh) {
+    int n = probesOf(rc, 0);
+    uint64_t np = (uint64_t)n * n * n;
+    const float a0 = (float)3.14159265, a1 = (float)(2.0 * 3.14159265 / 3.0), a2 = (float)(3.14159265 / 4.0);
+    for (uint64_t pi = 0; pi < np; pi++) {
+      v3 cf[9];
+      for (int k = 0; k < 9; k++) cf[k] = V3(0, 0, 0);
+      for (int d = 0; d < 16; d++) {
+        const float* x = rgba + (pi * 16 + d) * 4;
+        v3 r = vscale(V3(x[0], x[1], x[2]), rc->omega0[d]);
+        v3 nn = dirOf(d, 4);
+        float y[9] = {0.282095f, 0.488603f * nn.y, 0.488603f * nn.z, 0.488603f * nn.x, 1.092548f * nn.x * nn.y,
+                      1.092548f * nn.y * nn.z, 0.315392f * (3.0f * nn.z * nn.z - 1.0f), 1.092548f * nn.x * nn.z,
+                      0.546274f * (nn.x * nn.x - nn.y * nn.y)};
+        for (int k = 0; k < 9; k++) cf[k] = vadd(cf[k], vscale(r, y[k]));
+      }
+      for (int k = 0; k < 9; k++) {
+        float a = k == 0 ? a0 : (k < 4 ? a1 : a2);
+        sh[pi * 27 + k * 3] = cf[k].x * a;
+        sh[pi * 27 + k * 3 + 1] = cf[k].y * a;
+        sh[pi * 27 + k * 3 + 2] = cf[k].z * a;
+      }
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* FP64 oracle: the same algorithm in double precision (directions, ray-voxel traversal, shading, merge, SH), on   */
+/* the same occupancy and attributes.                                                                            */
+
+typedef struct { double x, y, z; } d3;
+static d3 D3(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+static d3 dirOfD(int di, int res) {
+  double u = ((di % res) + 0.5) / res, w = ((di / res) + 0.5) / res;
+  double fx = u * 2 - 1, fy = w * 2 - 1;
+  d3 n = D3(fx, fy, 1 - fabs(fx) - fabs(fy));
+  double t = n.z < 0 ? -n.z : 0;
+  n.x += n.x >= 0 ? -t : t;
+  n.y += n.y >= 0 ? -t : t;
+  double l = sqrt(ddot(n, n));
+  return D3(n.x / l, n.y / l, n.z / l);
+}
+
+static d3 shadeVoxelD(const View* W, const orc_rc* rc, uint32_t ai, d3 d, d3 entry) {
+  uint32_t pa = 0, pn = 0;
+  if (ai != V_EMPTY) {
+    pa = W->vox[W->attrOff + ai * 2];
+    pn = W->vox[W->attrOff + ai * 2 + 1];
+  }
+  d3 alb = D3((pa & 255u) / 255.0, ((pa >> 8) & 255u) / 255.0, ((pa >> 16) & 255u) / 255.0);
+  double nx = fmax((int16_t)(uint16_t)(pn & 0xffffu) / 32767.0, -1.0), ny = fmax((int16_t)(uint16_t)(pn >> 16) / 32767.0, -1.0);
+  d3 n = D3(nx, ny, 1 - fabs(nx) - fabs(ny));
+  double q = n.z < 0 ? -n.z : 0;
+  n.x += n.x >= 0 ? -q : q;
+  n.y += n.y >= 0 ? -q : q;
+  if (!(ddot(n, n) > 0.01)) n = D3(-d.x, -d.y, -d.z);
+  if (ddot(n, d) > 0) n = D3(-n.x, -n.y, -n.z);
+  double ndl = fmax(n.x * rc->sun_dir[0] + n.y * rc->sun_dir[1] + n.z * rc->sun_dir[2], 0.0);
+  d3 out = D3(0, 0, 0);
+  if (ndl > 0) {
+    double k = ndl * rc->sun_vis / M_PI;
+    out = D3(alb.x * rc->sun_e[0] * k, alb.y * rc->sun_e[1] * k, alb.z * rc->sun_e[2] * k);
+  }
+  for (int i = 0; i < rc->n_point; i++) {
+    d3 l = D3(rc->point_pos[i][0] - entry.x, rc->point_pos[i][1] - entry.y, rc->point_pos[i][2] - entry.z);
+    double r2 = ddot(l, l), nl = ddot(n, l) / sqrt(r2);
+    if (nl > 0) {
+      double k = nl / r2 / M_PI;
+      out.x += alb.x * rc->point_i[i][0] * k;
+      out.y += alb.y * rc->point_i[i][1] * k;
+      out.z += alb.z * rc->point_i[i][2] * k;
+    }
+  }
+  return out;
+}
+
+/* Voxel-by-voxel traversal of level L in double; the first occupied voxel entered at t >= tSkip. */
+static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, double t1, double tSkip, uint32_t L,
+                      double eps, double out[4], uint32_t* hid, uint8_t* tie) {
+  const orc_vox* p = W->p;
+  out[0] = out[1] = out[2] = 0;
+  out[3] = 1;
+  *hid = 0xffffffffu;
+  *tie = 0;
+  double v = p->v0 * (double)(1u << L);
+  int G = p->G, nVox = G * 8;
+  double lo[3] = {p->origin[L][0] * v * 8, p->origin[L][1] * v * 8, p->origin[L][2] * v * 8};
+  double dv[3] = {d.x, d.y, d.z}, ov[3] = {o.x, o.y, o.z};
+  double dd[3], inv[3], t = t0, tEnd = t1;
+  for (int a = 0; a < 3; a++) {
+    dd[a] = fabs(dv[a]) < 1e-9 ? 1e-9 : dv[a];
+    inv[a] = 1.0 / dd[a];
+    double ta = (lo[a] - ov[a]) * inv[a], tb = (lo[a] + nVox * v - ov[a]) * inv[a];
+    double tn = ta < tb ? ta : tb, tf = ta < tb ? tb : ta;
+    if (tn > t) t = tn;
+    if (tf < tEnd) tEnd = tf;
+  }
+  double scale = fabs(t1) + 1.0;
+  if (fabs(t - tEnd) < eps * scale) *tie = 1;
+  if (t >= tEnd) return;
+  int c[3], st[3];
+  double tMax[3], tDelta[3];
+  for (int a = 0; a < 3; a++) {
+    st[a] = dd[a] > 0 ? 1 : -1;
+    double x = (ov[a] + dv[a] * t - lo[a]) / v;
+    int ci = (int)floor(x);
+    if (fabs(x - floor(x + 0.5)) < eps * (fabs(x) + 1) && fabs(dv[a]) >= 1e-9) *tie = 1;
+    c[a] = ci < 0 ? 0 : (ci > nVox - 1 ? nVox - 1 : ci);
+    tMax[a] = ((c[a] + (st[a] > 0 ? 1 : 0)) * v + lo[a] - ov[a]) * inv[a];
+    tDelta[a] = fabs(v * inv[a]);
+  }
+  double tIn = t;
+  for (int it = 0; it < 4 * nVox; it++) {
+    uint32_t br, bit, lev;
+    iv3 cell;
+    if (voxAt(W, L, I3(c[0], c[1], c[2]), &br, &bit, &lev, &cell)) {
+      if (fabs(tIn - tSkip) < eps * scale) *tie = 1;
+      if (tIn >= tSkip) {
+        *hid = hitId(p, lev, cell);
+        d3 e = D3(o.x + d.x * tIn, o.y + d.y * tIn, o.z + d.z * tIn);
+        d3 r = shadeVoxelD(W, rc, voxRank(W, br, bit), d, e);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0;
+        return;
+      }
+    }
+    int a = tMax[0] <= tMax[1] && tMax[0] <= tMax[2] ? 0 : (tMax[1] <= tMax[2] ? 1 : 2);
+    double tx = tMax[a];
+    for (int b2 = 0; b2 < 3; b2++)
+      if (b2 != a && fabs(tMax[b2] - tx) < eps * scale) *tie = 1;
+    if (fabs(tx - tEnd) < eps * scale) *tie = 1;
+    if (tx >= tEnd) break;
+    tIn = tx;
+    c[a] += st[a];
+    tMax[a] += tDelta[a];
+    if (c[a] < 0 || c[a] >= nVox) break;
+  }
+}
+
+int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, double* rgbt,
+                  uint32_t* hits, uint8_t* tie, double eps, int threads) {
+  View W;
+  makeView(&W, p, vox);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads)
+#endif
+  for (int64_t q = (int64_t)r0; q < (int64_t)r1; q++) {
+    uint64_t r = 0;
+    int c = cascadeOf(rc, (uint64_t)q, &r);
+    int res = resOf(rc, c), n = probesOf(rc, c);
+    uint64_t dirs = (uint64_t)res * res, pi = r / dirs;
+    int di = (int)(r % dirs);
+    double s = (double)rc->s0 * (double)(1u << c);
+    d3 o = D3(((rc->G0[0] >> c) + (int)(pi % n)) * s, ((rc->G0[1] >> c) + (int)((pi / n) % n)) * s,
+              ((rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n))) * s);
+    d3 d = dirOfD(di, res);
+    double t0 = (double)rc->b0[c] * rc->s0, t1 = (double)rc->b1[c] * rc->s0;
+    /* Level: the smallest level >= voxLevel whose window strictly contains the interval's end. */
+    d3 e = D3(o.x + d.x * t1, o.y + d.y * t1, o.z + d.z * t1);
+    uint32_t L = (uint32_t)rc->vox_level[c];
+    uint8_t tl = 0;
+    while (L + 1 < (uint32_t)p->levels) {
+      double bs = p->v0 * (double)(1u << L) * 8;
+      double ev[3] = {e.x, e.y, e.z};
+      int in = 1;
+      for (int a = 0; a < 3; a++) {
+        double lo = p->origin[L][a] * bs, hi = lo + p->G * bs;
+        if (fabs(ev[a] - lo) < eps * (fabs(lo) + 1) || fabs(ev[a] - hi) < eps * (fabs(hi) + 1)) tl = 1;
+        if (!(ev[a] > lo && ev[a] < hi)) in = 0;
+      }
+      if (in) break;
+      L++;
+    }
+    double v = p->v0 * (double)(1u << L);
+    double out[4];
+    uint32_t h;
+    uint8_t tt;
+    traceVoxD(&W, rc, o, d, t0, t1, (double)rc->vox_skip * v, L, eps, out, &h, &tt);
+    size_t i = (size_t)(q - (int64_t)r0);
+    if (rgbt) memcpy(rgbt + i * 4, out, 32);
+    if (hits) hits[i] = h;
+    if (tie) tie[i] = (uint8_t)(tt | tl);
+  }
+  (void)threads;
+  return 0;
+}
+
+int orc_merge_sh_f64(const orc_rc* rc, double* rgbt, double* sh) {
+  int Nc = rc->cascades;
+  for (int c = Nc - 1; c >= 0; c--) {
+    int res = resOf(rc, c), n = probesOf(rc, c);
+    uint64_t off = rayOff(rc, c), nr = orc_rays(rc, c), dirs = (uint64_t)res * res;
+    for (uint64_t t = 0; t < nr; t++) {
+      double* v = rgbt + (off + t) * 4;
+      double up[3] = {rc->sky[0], rc->sky[1], rc->sky[2]};
+      if (c + 1 < Nc && v[3] > 0) {
+        int ur = resOf(rc, c + 1), un = probesOf(rc, c + 1);
+        uint64_t uoff = rayOff(rc, c + 1), pi = t / dirs;
+        int di = (int)(t % dirs), u0 = di % res, w0 = di / res;
+        int g[3] = {(int)(pi % n), (int)((pi / n) % n), (int)(pi / ((uint64_t)n * n))};
+        double acc[3] = {0, 0, 0}, ws = 0;
+        for (int k = 0; k < 8; k++) {
+          int o[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1}, q[3], skip = 0;
+          for (int a = 0; a < 3; a++) {
+            if (o[a] && !(g[a] & 1)) skip = 1;
+            q[a] = (g[a] >> 1) + o[a];
+            if (q[a] >= un) skip = 1;
+          }
+          if (skip) continue;
+          uint64_t qi = (uint64_t)q[0] + (uint64_t)un * ((uint64_t)q[1] + (uint64_t)un * q[2]);
+          const double* U = rgbt + (uoff + qi * (uint64_t)ur * ur) * 4;
+          for (int ch = 0; ch < 3; ch++) {
+            if (ur == res) acc[ch] += U[((uint64_t)w0 * ur + u0) * 4 + ch];
+            else {
+              const double* b = U + ((uint64_t)w0 * 2 * ur + (uint64_t)u0 * 2) * 4;
+              acc[ch] += 0.25 * (b[ch] + b[4 + ch] + b[(uint64_t)ur * 4 + ch] + b[(uint64_t)ur * 4 + 4 + ch]);
+            }
+          }
+          ws += 1;
+        }
+        for (int ch = 0; ch < 3; ch++) up[ch] = ws > 0 ? acc[ch] / ws : 0;
+      }
+      for (int ch = 0; ch < 3; ch++) v[ch] += v[3] * up[ch];
+      v[3] = 1;
+    }
+  }
+  if (sh) {
+    int n = probesOf(rc, 0);
+    uint64_t np = (uint64_t)n * n * n;
+    for (uint64_t pi = 0; pi < np; pi++) {
+      double cf[27] = {0};
+      for (int d = 0; d < 16; d++) {
+        const double* x = rgbt + (pi * 16 + d) * 4;
+        d3 nn = dirOfD(d, 4);
+        double y[9] = {0.282095, 0.488603 * nn.y, 0.488603 * nn.z, 0.488603 * nn.x, 1.092548 * nn.x * nn.y,
+                       1.092548 * nn.y * nn.z, 0.315392 * (3 * nn.z * nn.z - 1), 1.092548 * nn.x * nn.z,
+                       0.546274 * (nn.x * nn.x - nn.y * nn.y)};
+        for (int k = 0; k < 9; k++)
+          for (int ch = 0; ch < 3; ch++) cf[k * 3 + ch] += x[ch] * rc->omega0[d] * y[k];
+      }
+      for (int k = 0; k < 9; k++) {
+        double a = k == 0 ? M_PI : (k < 4 ? 2 * M_PI / 3 : M_PI / 4);
+        for (int ch = 0; ch < 3; ch++) sh[pi * 27 + k * 3 + ch] = cf[k * 3 + ch] * a;
+      }
+    }
+  }
+  return 0;
+}

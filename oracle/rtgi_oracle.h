@@ -1,0 +1,76 @@
+/*
+ * rtgi_oracle.h -- TEST INFRASTRUCTURE ONLY. CPU restatement of the reference's per-frame GI update path, used by
+ * the tests, the smoke check and the CPU-baseline timing as the checker; never linked into or called by the product
+ * library (real-time-gi-lab_amd/librtgi.so).
+ *
+ * Two parts:
+ *   1. an f32 restatement of the reference WGSL, operation by operation (IEEE f32, no FMA contraction, correctly
+ *      rounded division and square root), in the reference's own buffer layout (header, pointer table, 18-word bricks,
+ *      2-word attributes; src/core/voxel.ts:11,52-53,753-760), with the GPU's atomics taken in thread order;
+ *   2. an FP64 tracer over the same occupancy: exact voxel traversal, shading, merge and SH in double.
+ */
+#ifndef RTGI_ORACLE_H
+#define RTGI_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_vox {
+  float v0;
+  int32_t levels, G, max_bricks, max_voxels;
+  int32_t origin[8][3];
+} orc_vox;
+
+typedef struct orc_rc {
+  int32_t cascades, probes0, res_cap;
+  float s0, vox_skip;
+  int32_t vox_level[8];
+  float b0[8], b1[8];
+  int32_t G0[3];
+  float sun_dir[3], sun_e[3], sky[3], sun_vis;
+  int32_t n_point;
+  float point_pos[8][3], point_i[8][3];
+  float omega0[16];
+} orc_rc;
+
+/* Words of the reference-layout clipmap buffer for these parameters. */
+size_t orc_vox_words(const orc_vox* p);
+/* Window origins for an eye (writeUniform, src/core/voxel.ts:872-875). */
+void orc_origins(double v0, int levels, int G, const double eye[3], int32_t out[8][3]);
+/* Cascade layout (createRc, src/core/rc.ts:1024-1035) on the dense grid: fills rc's vox_level, b0, b1, G0. */
+void orc_cascade_setup(orc_rc* rc, double v0, int levels, double s0, double k_interval, double level_map,
+                       double vox_scale, const double eye[3]);
+/* octSolidAngles (src/core/frame.ts:105-114). */
+void orc_oct_solid_angles(int res, int samples, float* out);
+
+/* Full voxel rebuild (src/core/voxel.ts:338-620 minus voxSun). vox: orc_vox_words() words; rep: max_bricks*512. */
+int orc_build(const orc_vox* p, const float* pos, const float* nrm, const uint32_t* tri, uint32_t ntris,
+              const float* mat, uint32_t* vox, uint32_t* rep);
+/* Canonical content of level L (the layout of rtgi_world_read_clipmap). Any output may be NULL. */
+int orc_canonical(const orc_vox* p, const uint32_t* vox, const uint32_t* rep, int level, uint32_t* occ_bits,
+                  uint32_t* rep_out, uint32_t* albedo, uint32_t* normal, float* plane, uint8_t* brick_state);
+
+/* Number of rays of cascade c and the total. */
+uint64_t orc_rays(const orc_rc* rc, int c);
+/* Trace of rays [r0, r1) of the whole ray range (cascade-major order) -> rgba, hit ids (either may be NULL).
+ * threads: OpenMP threads (<= 0: all). */
+int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
+              uint32_t* hits, int threads);
+/* mergeWorld N-1..0 in place on the full rgba array, then projectSH -> sh (27 floats per c0 probe). */
+int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
+
+/* FP64 oracle: trace in double over the same occupancy and attributes; rays [r0, r1). rgb: 3 doubles per ray, hit:
+ * 1 for a hit, and tie: 1 when the ray passes within eps (relative) of a voxel edge or corner before its first hit
+ * (its first-hit voxel is then not fixed by exact arithmetic's neighbourhood). Any output may be NULL. */
+int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, double* rgbt,
+                  uint32_t* hits, uint8_t* tie, double eps, int threads);
+/* FP64 merge and SH on a full double rgbt array (4 doubles per ray). */
+int orc_merge_sh_f64(const orc_rc* rc, double* rgbt, double* sh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

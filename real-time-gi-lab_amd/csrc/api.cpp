@@ -1,0 +1,602 @@
+// Host side of librtgi: the C ABI of include/rtgi.h. A world owns its HIP allocations (clipmap, cascades, scene copy)
+// and a stream; each update is a fixed kernel sequence with no host round trip (launch_* in build.hip / trace.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "rtgi.h"
+#include "rtgi_internal.h"
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+int rtgi_fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return rtgi_fail(RTGI_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                           __FILE__, __LINE__);                                      \
+  } while (0)
+
+struct rtgi_world {
+  int device = 0;
+  rtgi_vox_params vp{};
+  rtgi_rc_params rp{};
+  int NL = 0, G = 0, E = 0, per = 0;
+  int origin[RTGI_MAX_LEVELS][3] = {};
+  int N = 0;
+  int n[RTGI_MAX_CASCADES] = {}, res[RTGI_MAX_CASCADES] = {}, vlev[RTGI_MAX_CASCADES] = {};
+  float b0[RTGI_MAX_CASCADES] = {}, b1[RTGI_MAX_CASCADES] = {};
+  uint64_t ray_off[RTGI_MAX_CASCADES + 1] = {};
+  int G0[3] = {};
+  float omega0[16] = {};
+  rtgi_frame frame{};
+  bool frame_set = false, scene_set = false;
+  uint32_t ntris = 0, nverts = 0, nmat = 0;
+  uint32_t cap_tris = 0, cap_verts = 0, cap_mat = 0;
+  // device buffers
+  uint32_t *hdr = nullptr, *ptr = nullptr, *occ = nullptr, *battr = nullptr, *bcode = nullptr, *rep = nullptr,
+           *attr = nullptr;
+  float* plane = nullptr;
+  uint32_t *big_vox = nullptr, *big_mark = nullptr;
+  float *pos = nullptr, *nrm = nullptr, *mat = nullptr;
+  uint32_t* tri = nullptr;
+  float* rad = nullptr;
+  uint32_t* hits = nullptr;
+  float* sh = nullptr;
+  uint64_t bytes = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  uint32_t last_stages = 0;
+};
+
+namespace {
+
+template <class T>
+int dalloc(rtgi_world* w, T** p, size_t count) {
+  size_t b = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, b);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return rtgi_fail(RTGI_E_NOMEM, "hipMalloc(%zu bytes): %s", b, hipGetErrorString(e));
+  }
+  w->bytes += b;
+  return RTGI_OK;
+}
+
+template <class T>
+void dfree(rtgi_world* w, T*& p, size_t count) {
+  if (p) {
+    (void)hipFree(p);
+    w->bytes -= std::max<size_t>(count, 1) * sizeof(T);
+  }
+  p = nullptr;
+}
+
+// octEncode (src/core/frame.ts:97-103) in double.
+void oct_encode_d(double x, double y, double z, double& u, double& v) {
+  double s = std::fabs(x) + std::fabs(y) + std::fabs(z);
+  double px = x / s, py = y / s;
+  if (z < 0) {
+    double ox = px, oy = py;
+    px = (1 - std::fabs(oy)) * (ox >= 0 ? 1 : -1);
+    py = (1 - std::fabs(ox)) * (oy >= 0 ? 1 : -1);
+  }
+  u = px * 0.5 + 0.5;
+  v = py * 0.5 + 0.5;
+}
+
+VoxK vox_args(const rtgi_world* w) {
+  VoxK k{};
+  k.v0 = (float)w->vp.v0;
+  k.NL = w->NL;
+  k.G = w->G;
+  k.E = w->E;
+  k.per = w->per;
+  k.max_bricks = w->vp.max_bricks;
+  k.max_voxels = w->vp.max_voxels;
+  std::memcpy(k.origin, w->origin, sizeof(k.origin));
+  k.hdr = w->hdr;
+  k.ptr = w->ptr;
+  k.occ = w->occ;
+  k.battr = w->battr;
+  k.bcode = w->bcode;
+  k.rep = w->rep;
+  k.attr = w->attr;
+  k.plane = w->plane;
+  k.big_vox = w->big_vox;
+  k.big_mark = w->big_mark;
+  k.pos = w->pos;
+  k.nrm = w->nrm;
+  k.tri = w->tri;
+  k.mat = w->mat;
+  k.ntris = w->ntris;
+  return k;
+}
+
+TraceK trace_args(const rtgi_world* w) {
+  TraceK t{};
+  t.vx = vox_args(w);
+  t.N = w->N;
+  for (int c = 0; c < w->N; c++) {
+    t.c[c].n = w->n[c];
+    t.c[c].res = w->res[c];
+    t.c[c].vox_level = w->vlev[c];
+    t.c[c].b0 = w->b0[c];
+    t.c[c].b1 = w->b1[c];
+    t.c[c].ray_off = w->ray_off[c];
+    t.c[c].rays = w->ray_off[c + 1] - w->ray_off[c];
+  }
+  for (int a = 0; a < 3; a++) t.G0[a] = w->G0[a];
+  t.s0 = (float)w->rp.s0;
+  t.vox_skip = (float)w->rp.vox_skip;
+  t.vox_offset = (float)w->rp.vox_offset;
+  for (int a = 0; a < 3; a++) {
+    t.sun_dir[a] = w->frame.sun_dir[a];
+    t.sun_e[a] = w->frame.sun_e[a];
+    t.sky[a] = w->frame.sky[a];
+  }
+  t.sun_vis = w->frame.sun_vis;
+  t.n_point = w->frame.n_point;
+  std::memcpy(t.point_pos, w->frame.point_pos, sizeof(t.point_pos));
+  std::memcpy(t.point_i, w->frame.point_i, sizeof(t.point_i));
+  std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
+  t.rad = w->rad;
+  t.hits = w->hits;
+  t.sh = w->sh;
+  t.total_rays = w->ray_off[w->N];
+  return t;
+}
+
+void free_all(rtgi_world* w) {
+  size_t NLper = (size_t)w->NL * w->per, mb = (size_t)w->vp.max_bricks, mv = (size_t)w->vp.max_voxels;
+  dfree(w, w->hdr, HDR_WORDS);
+  dfree(w, w->ptr, NLper);
+  dfree(w, w->occ, mb * 16);
+  dfree(w, w->battr, mb);
+  dfree(w, w->bcode, mb);
+  dfree(w, w->rep, mb * 512);
+  dfree(w, w->attr, mv * 2);
+  dfree(w, w->plane, mv);
+  dfree(w, w->big_vox, (size_t)w->NL * w->cap_tris);
+  dfree(w, w->big_mark, (size_t)w->NL * w->cap_tris);
+  dfree(w, w->tri, (size_t)w->cap_tris * 4);
+  dfree(w, w->pos, (size_t)w->cap_verts * 3);
+  dfree(w, w->nrm, (size_t)w->cap_verts * 3);
+  dfree(w, w->mat, (size_t)w->cap_mat * 4);
+  dfree(w, w->rad, (size_t)w->ray_off[w->N] * 4);
+  dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
+  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
+  dfree(w, w->sh, np0 * 27);
+}
+
+int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
+  int e = 0;
+  if (stage == RTGI_STAGE_VOX) {
+    e = launch_vox_build(vox_args(w), s);
+  } else if (stage == RTGI_STAGE_TRACE) {
+    e = launch_trace(trace_args(w), s);
+  } else if (stage == RTGI_STAGE_MERGE) {
+    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) {
+      TraceK t = trace_args(w);
+      for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_merge(t, c, s);
+    }
+  } else if (stage == RTGI_STAGE_SH) {
+    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) e = launch_sh(trace_args(w), s);
+  }
+  if (e != 0) return rtgi_fail(RTGI_E_HIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)e));
+  return RTGI_OK;
+}
+
+}  // namespace
+
+extern "C" const char* rtgi_last_error(void) { return g_err; }
+
+extern "C" const char* rtgi_version(void) { return "rtgi-mi355x 0.1 (gfx950)"; }
+
+extern "C" int rtgi_device_count(int32_t* n) {
+  if (!n) return rtgi_fail(RTGI_E_INVALID, "rtgi_device_count: null argument");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_oct_solid_angles(int32_t res, int32_t samples, float* out) {
+  if (res <= 0 || samples <= 0 || !out) return rtgi_fail(RTGI_E_INVALID, "rtgi_oct_solid_angles: bad argument");
+  std::vector<double> counts((size_t)res * res, 0.0);
+  const double golden = M_PI * (3 - std::sqrt(5.0));
+  for (int i = 0; i < samples; i++) {
+    double z = 1 - (2.0 * i + 1) / samples, r = std::sqrt(1 - z * z), phi = golden * i;
+    double u, v;
+    oct_encode_d(r * std::cos(phi), r * std::sin(phi), z, u, v);
+    int iu = std::min(res - 1, (int)std::floor(u * res)), iv = std::min(res - 1, (int)std::floor(v * res));
+    counts[(size_t)iv * res + iu] += 1;
+  }
+  for (size_t i = 0; i < counts.size(); i++) out[i] = (float)(counts[i] / samples * 4 * M_PI);
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params* rp, rtgi_world** out) {
+  if (!vp || !rp || !out) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_create: null argument");
+  *out = nullptr;
+  if (vp->levels < 1 || vp->levels > RTGI_MAX_LEVELS) return rtgi_fail(RTGI_E_INVALID, "at most 8 levels");
+  if (vp->G < 1 || vp->G > 64 || ((int64_t)vp->G * vp->G * vp->G) % 64 != 0)
+    return rtgi_fail(RTGI_E_INVALID, "G must be in 1..64 with G^3 a multiple of 64 (got %d)", vp->G);
+  if (!(vp->v0 > 0)) return rtgi_fail(RTGI_E_INVALID, "v0 must be positive");
+  if (vp->max_bricks < 1 || vp->max_voxels < 1) return rtgi_fail(RTGI_E_INVALID, "pool sizes must be positive");
+  if ((int64_t)vp->max_bricks * 512 > 0xffffffffLL || (int64_t)vp->max_voxels * 2 > 0xffffffffLL)
+    return rtgi_fail(RTGI_E_INVALID, "pool sizes overflow 32-bit indices");
+  if (rp->cascades < 1 || rp->cascades > RTGI_MAX_CASCADES) return rtgi_fail(RTGI_E_INVALID, "at most 8 cascades");
+  int Nc = rp->cascades;
+  if (rp->probes0 < 1 || rp->probes0 % (1 << (Nc - 1)) != 0)
+    return rtgi_fail(RTGI_E_INVALID, "probes0 (%d) must be a positive multiple of 2^(cascades-1)", rp->probes0);
+  if (!(rp->s0 > 0)) return rtgi_fail(RTGI_E_INVALID, "s0 must be positive");
+  if (rp->res_cap < 0) return rtgi_fail(RTGI_E_INVALID, "res_cap must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return rtgi_fail(RTGI_E_NODEVICE, "no HIP device visible");
+
+  rtgi_world* w = new (std::nothrow) rtgi_world();
+  if (!w) return rtgi_fail(RTGI_E_NOMEM, "out of host memory");
+  (void)hipGetDevice(&w->device);
+  w->vp = *vp;
+  w->rp = *rp;
+  w->NL = vp->levels;
+  w->G = vp->G;
+  w->E = vp->G * 8;
+  w->per = vp->G * vp->G * vp->G;
+  w->N = Nc;
+  uint64_t off = 0;
+  for (int c = 0; c < Nc; c++) {
+    w->n[c] = rp->probes0 >> c;
+    w->res[c] = 4 << std::min(c, rp->res_cap);
+    auto b = [&](int k) { return (float)(rp->k_interval * (std::pow(4.0, k) - 1) / 3); };
+    w->b0[c] = b(c);
+    w->b1[c] = b(c + 1);
+    // voxLevel (src/core/rc.ts:210-213), evaluated in double once per cascade; round() is half-to-even as in WGSL.
+    double v = rp->vox_scale * rp->s0 * std::exp2(rp->level_map * c);
+    double L = std::nearbyint(std::log2(v / vp->v0));
+    w->vlev[c] = (int)std::min(std::max(L, 0.0), (double)(vp->levels - 1));
+    w->ray_off[c] = off;
+    off += (uint64_t)w->n[c] * w->n[c] * w->n[c] * w->res[c] * w->res[c];
+  }
+  w->ray_off[Nc] = off;
+  rtgi_oct_solid_angles(4, 1 << 20, w->omega0);
+
+  int e = RTGI_OK;
+  size_t NLper = (size_t)w->NL * w->per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
+  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
+  if (!e) e = dalloc(w, &w->hdr, HDR_WORDS);
+  if (!e) e = dalloc(w, &w->ptr, NLper);
+  if (!e) e = dalloc(w, &w->occ, mb * 16);
+  if (!e) e = dalloc(w, &w->battr, mb);
+  if (!e) e = dalloc(w, &w->bcode, mb);
+  if (!e) e = dalloc(w, &w->rep, mb * 512);
+  if (!e) e = dalloc(w, &w->attr, mv * 2);
+  if (!e) e = dalloc(w, &w->plane, mv);
+  if (!e) e = dalloc(w, &w->rad, (size_t)off * 4);
+  if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
+  if (!e) e = dalloc(w, &w->sh, np0 * 27);
+  if (!e && hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "hipStreamCreate failed");
+  for (int i = 0; i < 5 && !e; i++)
+    if (hipEventCreate(&w->ev[i]) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipEventCreate failed");
+  if (!e && hipMemsetAsync(w->hdr, 0, HDR_WORDS * 4, w->stream) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "hipMemsetAsync failed");
+  if (!e && hipStreamSynchronize(w->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipStreamSynchronize failed");
+  if (e) {
+    rtgi_world_destroy(w);
+    return e;
+  }
+  *out = w;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const float* nrm, uint32_t nverts,
+                                       const uint32_t* tri, uint32_t ntris, const float* materials, uint32_t nmat) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if ((nverts && (!pos || !nrm)) || (ntris && !tri) || (nmat && !materials))
+    return rtgi_fail(RTGI_E_INVALID, "rtgi_world_upload_scene: null array");
+  for (uint32_t t = 0; t < ntris; t++) {
+    if (tri[4 * t] >= nverts || tri[4 * t + 1] >= nverts || tri[4 * t + 2] >= nverts)
+      return rtgi_fail(RTGI_E_INVALID, "triangle %u indexes a vertex out of range", t);
+    if (tri[4 * t + 3] >= nmat) return rtgi_fail(RTGI_E_INVALID, "triangle %u indexes a material out of range", t);
+  }
+  if ((uint64_t)ntris >= 0xffffffffull) return rtgi_fail(RTGI_E_INVALID, "too many triangles");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  int e = RTGI_OK;
+  if (ntris > w->cap_tris) {
+    dfree(w, w->tri, (size_t)w->cap_tris * 4);
+    dfree(w, w->big_vox, (size_t)w->NL * w->cap_tris);
+    dfree(w, w->big_mark, (size_t)w->NL * w->cap_tris);
+    w->cap_tris = 0;
+    if (!e) e = dalloc(w, &w->tri, (size_t)ntris * 4);
+    if (!e) e = dalloc(w, &w->big_vox, (size_t)w->NL * ntris);
+    if (!e) e = dalloc(w, &w->big_mark, (size_t)w->NL * ntris);
+    if (e) return e;
+    w->cap_tris = ntris;
+  }
+  if (nverts > w->cap_verts) {
+    dfree(w, w->pos, (size_t)w->cap_verts * 3);
+    dfree(w, w->nrm, (size_t)w->cap_verts * 3);
+    w->cap_verts = 0;
+    if (!e) e = dalloc(w, &w->pos, (size_t)nverts * 3);
+    if (!e) e = dalloc(w, &w->nrm, (size_t)nverts * 3);
+    if (e) return e;
+    w->cap_verts = nverts;
+  }
+  if (nmat > w->cap_mat) {
+    dfree(w, w->mat, (size_t)w->cap_mat * 4);
+    w->cap_mat = 0;
+    if ((e = dalloc(w, &w->mat, (size_t)nmat * 4))) return e;
+    w->cap_mat = nmat;
+  }
+  if (ntris) HIPCHK(hipMemcpy(w->tri, tri, (size_t)ntris * 16, hipMemcpyHostToDevice));
+  if (nverts) {
+    HIPCHK(hipMemcpy(w->pos, pos, (size_t)nverts * 12, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w->nrm, nrm, (size_t)nverts * 12, hipMemcpyHostToDevice));
+  }
+  if (nmat) HIPCHK(hipMemcpy(w->mat, materials, (size_t)nmat * 16, hipMemcpyHostToDevice));
+  w->ntris = ntris;
+  w->nverts = nverts;
+  w->nmat = nmat;
+  w->scene_set = true;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_t count, const float* pos,
+                                          const float* nrm) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (!w->scene_set) return rtgi_fail(RTGI_E_STATE, "no scene uploaded");
+  if ((uint64_t)first + count > w->nverts) return rtgi_fail(RTGI_E_INVALID, "vertex range out of bounds");
+  if (count && (!pos || !nrm)) return rtgi_fail(RTGI_E_INVALID, "null array");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  if (count) {
+    HIPCHK(hipMemcpy(w->pos + (size_t)first * 3, pos, (size_t)count * 12, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w->nrm + (size_t)first * 3, nrm, (size_t)count * 12, hipMemcpyHostToDevice));
+  }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_set_frame(rtgi_world* w, const rtgi_frame* f) {
+  if (!w || !f) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  if (f->n_point < 0 || f->n_point > RTGI_MAX_POINT_LIGHTS) return rtgi_fail(RTGI_E_INVALID, "0..8 point lights");
+  for (int a = 0; a < 3; a++)
+    if (!std::isfinite(f->eye[a])) return rtgi_fail(RTGI_E_INVALID, "eye must be finite");
+  w->frame = *f;
+  // Clipmap windows centred on the eye: floor(eye / bs) - G/2 (writeUniform, src/core/voxel.ts:872-875).
+  for (int L = 0; L < w->NL; L++) {
+    double bs = w->vp.v0 * std::exp2((double)L) * 8;
+    for (int a = 0; a < 3; a++) {
+      double o = std::floor(f->eye[a] / bs) - w->G / 2;
+      if (std::fabs(o) > 1e6) return rtgi_fail(RTGI_E_INVALID, "eye too far from the origin");
+      w->origin[L][a] = (int)o;
+    }
+  }
+  // Dense probe grid, snapped to the coarsest cascade's spacing so that every cascade's grid nests in the next.
+  int top = w->N - 1;
+  double st = w->rp.s0 * std::exp2((double)top);
+  int ntop = w->n[top];
+  for (int a = 0; a < 3; a++) {
+    double g = std::floor(f->eye[a] / st) - ntop / 2;
+    if (std::fabs(g) > 1e6) return rtgi_fail(RTGI_E_INVALID, "eye too far from the origin");
+    w->G0[a] = (int)g * (1 << top);
+  }
+  w->frame_set = true;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* stream) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (!w->scene_set) return rtgi_fail(RTGI_E_STATE, "no scene uploaded");
+  if (!w->frame_set) return rtgi_fail(RTGI_E_STATE, "no frame set");
+  if (stages & ~RTGI_STAGE_ALL) return rtgi_fail(RTGI_E_INVALID, "unknown stage bits");
+  HIPCHK(hipSetDevice(w->device));
+  hipStream_t s = stream ? (hipStream_t)stream : w->stream;
+  const uint32_t order[4] = {RTGI_STAGE_VOX, RTGI_STAGE_TRACE, RTGI_STAGE_MERGE, RTGI_STAGE_SH};
+  if (w->timing) HIPCHK(hipEventRecord(w->ev[0], s));
+  for (int i = 0; i < 4; i++) {
+    if (stages & order[i]) {
+      int e = stage_launch(w, order[i], s);
+      if (e) return e;
+    }
+    if (w->timing) HIPCHK(hipEventRecord(w->ev[i + 1], s));
+  }
+  w->last_stages = stages;
+  if (s != w->stream) {
+    // Later calls on the world's own stream (readback, sync) must see this work.
+    HIPCHK(hipEventRecord(w->ev[4], s));
+    HIPCHK(hipStreamWaitEvent(w->stream, w->ev[4], 0));
+  }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_enqueue(rtgi_world* w, void* stream) {
+  return rtgi_world_enqueue_stages(w, RTGI_STAGE_ALL, stream);
+}
+
+extern "C" int rtgi_world_set_timing(rtgi_world* w, int32_t on) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  w->timing = on != 0;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_stage_ms(rtgi_world* w, float ms[4]) {
+  if (!w || !ms) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  if (!w->timing) return rtgi_fail(RTGI_E_STATE, "timing is off");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipEventSynchronize(w->ev[4]));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventElapsedTime(&ms[i], w->ev[i], w->ev[i + 1]));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_sync(rtgi_world* w) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_counters(rtgi_world* w, rtgi_counters* out) {
+  if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(w->device));
+  uint32_t h[HDR_WORDS];
+  HIPCHK(hipMemcpyAsync(h, w->hdr, sizeof(h), hipMemcpyDeviceToHost, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->marked = h[H_MARKED];
+  out->bricks = std::min(h[H_MARKED], (uint32_t)w->vp.max_bricks);
+  out->voxels = h[H_VOXELS];
+  out->brick_overflow = h[H_BRICK_OVF];
+  out->big_overflow = h[H_BIG_OVF];
+  out->voxel_overflow = h[H_VOXEL_OVF];
+  for (int L = 0; L < w->NL; L++) {
+    out->bricks_per_level[L] = h[H_BRICKS_L + L];
+    out->big_per_level[L] = h[H_BIGVOX_L + L];
+  }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_get_layout(const rtgi_world* w, rtgi_layout* out) {
+  if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->levels = w->NL;
+  out->G = w->G;
+  out->E = w->E;
+  std::memcpy(out->window_origin, w->origin, sizeof(out->window_origin));
+  out->cascades = w->N;
+  for (int c = 0; c < w->N; c++) {
+    out->probes[c] = w->n[c];
+    out->res[c] = w->res[c];
+    out->vox_level[c] = w->vlev[c];
+    out->rays[c] = w->ray_off[c + 1] - w->ray_off[c];
+    out->b0[c] = w->b0[c];
+    out->b1[c] = w->b1[c];
+  }
+  out->total_rays = w->ray_off[w->N];
+  for (int a = 0; a < 3; a++) out->grid_origin[a] = w->G0[a];
+  std::memcpy(out->omega0, w->omega0, sizeof(out->omega0));
+  out->device_bytes = w->bytes;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* occ_bits, uint32_t* rep,
+                                       uint32_t* albedo, uint32_t* normal, float* plane, uint8_t* brick_state) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (level < 0 || level >= w->NL) return rtgi_fail(RTGI_E_INVALID, "level out of range");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  uint32_t h[HDR_WORDS];
+  HIPCHK(hipMemcpy(h, w->hdr, sizeof(h), hipMemcpyDeviceToHost));
+  uint32_t nb = std::min(h[H_MARKED], (uint32_t)w->vp.max_bricks);
+  uint32_t nvox = std::min(h[H_VOXELS], (uint32_t)w->vp.max_voxels);
+  const int G = w->G, E = w->E, per = w->per;
+  std::vector<uint32_t> ptr(per), occ((size_t)nb * 16), battr(nb), rp, at;
+  std::vector<float> pl;
+  HIPCHK(hipMemcpy(ptr.data(), w->ptr + (size_t)level * per, (size_t)per * 4, hipMemcpyDeviceToHost));
+  if (nb) {
+    HIPCHK(hipMemcpy(occ.data(), w->occ, (size_t)nb * 64, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(battr.data(), w->battr, (size_t)nb * 4, hipMemcpyDeviceToHost));
+  }
+  if (rep && nb) {
+    rp.resize((size_t)nb * 512);
+    HIPCHK(hipMemcpy(rp.data(), w->rep, (size_t)nb * 2048, hipMemcpyDeviceToHost));
+  }
+  if ((albedo || normal) && nvox) {
+    at.resize((size_t)nvox * 2);
+    HIPCHK(hipMemcpy(at.data(), w->attr, (size_t)nvox * 8, hipMemcpyDeviceToHost));
+  }
+  if (plane && nvox) {
+    pl.resize(nvox);
+    HIPCHK(hipMemcpy(pl.data(), w->plane, (size_t)nvox * 4, hipMemcpyDeviceToHost));
+  }
+  size_t ncell = (size_t)E * E * E;
+  if (occ_bits) std::memset(occ_bits, 0, ncell / 8);
+  if (rep) std::memset(rep, 0, ncell * 4);
+  if (albedo) std::memset(albedo, 0, ncell * 4);
+  if (normal) std::memset(normal, 0, ncell * 4);
+  if (plane) std::memset(plane, 0, ncell * 4);
+  for (int bz = 0; bz < G; bz++)
+    for (int by = 0; by < G; by++)
+      for (int bx = 0; bx < G; bx++) {
+        size_t bi = ((size_t)bz * G + by) * G + bx;
+        uint32_t p = ptr[bi];
+        uint8_t state = p == PTR_DROP ? 2 : (p >= PTR_MARK ? 0 : 1);
+        if (state == 1 && p >= nb) return rtgi_fail(RTGI_E_STATE, "corrupt pointer table (brick %u >= %u)", p, nb);
+        if (state == 1 && battr[p] == ATTR_NONE) state = 3;
+        if (brick_state) brick_state[bi] = state;
+        if (state != 1 && state != 3) continue;
+        uint32_t rank = 0;
+        for (int bit = 0; bit < 512; bit++) {
+          uint32_t word = occ[(size_t)p * 16 + (bit >> 5)];
+          if (!(word & (1u << (bit & 31)))) continue;
+          int x = bx * 8 + (bit & 7), y = by * 8 + ((bit >> 3) & 7), z = bz * 8 + (bit >> 6);
+          size_t ci = (size_t)x + (size_t)E * ((size_t)y + (size_t)E * z);
+          if (occ_bits) occ_bits[ci >> 5] |= 1u << (ci & 31);
+          if (rep) rep[ci] = rp[(size_t)p * 512 + bit];
+          if (state == 1) {
+            uint32_t ai = battr[p] + rank;
+            if (ai < nvox) {
+              if (albedo) albedo[ci] = at[(size_t)ai * 2];
+              if (normal) normal[ci] = at[(size_t)ai * 2 + 1];
+              if (plane) plane[ci] = pl[ai];
+            }
+          }
+          rank++;
+        }
+      }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hits) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (c < 0 || c >= w->N) return rtgi_fail(RTGI_E_INVALID, "cascade out of range");
+  if (hits && !w->hits) return rtgi_fail(RTGI_E_STATE, "hit ids were not recorded (RTGI_RC_RECORD_HITS)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  size_t n = w->ray_off[c + 1] - w->ray_off[c];
+  if (rgba) HIPCHK(hipMemcpy(rgba, w->rad + w->ray_off[c] * 4, n * 16, hipMemcpyDeviceToHost));
+  if (hits) HIPCHK(hipMemcpy(hits, w->hits + w->ray_off[c], n * 4, hipMemcpyDeviceToHost));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
+  if (!w || !sh) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  size_t np = (size_t)w->n[0] * w->n[0] * w->n[0];
+  std::vector<float> planes(np * 27);
+  HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
+  for (size_t p = 0; p < np; p++)
+    for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + p];
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_destroy(rtgi_world* w) {
+  if (!w) return RTGI_OK;
+  (void)hipSetDevice(w->device);
+  if (w->stream) (void)hipStreamSynchronize(w->stream);
+  free_all(w);
+  for (int i = 0; i < 5; i++)
+    if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  delete w;
+  return RTGI_OK;
+}

@@ -1,0 +1,538 @@
+// Clipmap voxel rebuild on gfx950: every triangle is conservatively voxelised into an L-level camera-centred clipmap
+// of 8^3-voxel bricks (the StrataBrick build of src/core/voxel.ts:205-620).
+//
+// Pipeline (one stream, no host round trip; device-side counts bound every grid-stride loop instead of indirect
+// dispatch):
+//   k_clear        header and pointer table                          (clear, voxel.ts:338-344)
+//   k_mark_small   per triangle and level: overlapped bricks;        (markBricks, voxel.ts:347-375)
+//                  ranges of more than 8 bricks are appended, one
+//                  wavefront-aggregated atomic per wave, to a list
+//   k_mark_big     one wavefront per listed (triangle, level)        (markBricksBig, voxel.ts:377-387)
+//   k_mark_parents ancestors of marked bricks                        (markParents, voxel.ts:390-403)
+//   k_alloc_bricks one workgroup: ordered allocation, coarse levels  (countMarks/scanMarks/allocBricks,
+//                  first and by pointer index, finest dropped first   voxel.ts:405-470)
+//   k_init_bricks  zero occupancy and representatives of the pool    (clearKeys, voxel.ts:471-474)
+//   k_vox_small    per triangle and level: SAT per voxel, atomicOr   (voxBits, voxel.ts:476-512)
+//                  of the bit and atomicMax of triangle + 1
+//   k_vox_big      one wavefront per listed (triangle, level)        (voxBitsBig, voxel.ts:514-534)
+//   k_reduce(L)    coarse level L from L-1, one thread per word      (reduce, voxel.ts:540-577)
+//   k_alloc_voxels one workgroup: attribute slots by prefix sum      (allocVoxels, voxel.ts:579-588)
+//   k_resolve      per (brick, voxel): albedo, oct normal, plane     (resolveAttr, voxel.ts:590-620)
+//
+// Allocation is always the ordered one: the reference uses it when the pool overflows and an atomic counter
+// otherwise (any order is valid then); a single order makes the raw layout deterministic as well as the canonical
+// content, and voxel-pool overflow then drops the same bricks every run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "rtgi_internal.h"
+#include "vmath.h"
+
+namespace {
+
+constexpr int BLK = 256;
+
+struct Tri {
+  f3 a, b, c;
+  uint32_t i0, i1, i2, m;
+};
+
+__device__ __forceinline__ f3 ldp(const float* p, uint32_t i) { return mk3(p[i * 3u], p[i * 3u + 1u], p[i * 3u + 2u]); }
+
+__device__ __forceinline__ Tri load_tri(const VoxK& k, uint32_t t) {
+  const uint4 q = reinterpret_cast<const uint4*>(k.tri)[t];
+  Tri T;
+  T.i0 = q.x;
+  T.i1 = q.y;
+  T.i2 = q.z;
+  T.m = q.w;
+  T.a = ldp(k.pos, q.x);
+  T.b = ldp(k.pos, q.y);
+  T.c = ldp(k.pos, q.z);
+  return T;
+}
+
+__device__ __forceinline__ float vox_size(const VoxK& k, int L) { return k.v0 * (float)(1u << L); }
+__device__ __forceinline__ i3 origin(const VoxK& k, int L) { return i3{k.origin[L][0], k.origin[L][1], k.origin[L][2]}; }
+__device__ __forceinline__ f3 region_lo(const VoxK& k, int L) { return tof3(origin(k, L)) * vox_size(k, L) * 8.0f; }
+__device__ __forceinline__ uint32_t ptr_index(const VoxK& k, int L, i3 b) {
+  return (uint32_t)L * (uint32_t)k.per + ((uint32_t)b.z * (uint32_t)k.G + (uint32_t)b.y) * (uint32_t)k.G + (uint32_t)b.x;
+}
+
+// Akenine-Moller triangle / box overlap (axisTest and triBox, src/core/voxel.ts:231-248).
+__device__ __forceinline__ bool axis_test(f3 a, f3 v0, f3 v1, f3 v2, f3 h) {
+  float p0 = dot3(v0, a), p1 = dot3(v1, a), p2 = dot3(v2, a);
+  float r = dot3(h, abs3(a));
+  return !(fminf(fminf(p0, p1), p2) > r || fmaxf(fmaxf(p0, p1), p2) < -r);
+}
+__device__ bool tri_box(f3 c, f3 h, f3 a, f3 b, f3 cc) {
+  f3 v0 = a - c, v1 = b - c, v2 = cc - c;
+  f3 mn = min3(min3(v0, v1), v2), mx = max3(max3(v0, v1), v2);
+  if (mn.x > h.x || mn.y > h.y || mn.z > h.z || mx.x < -h.x || mx.y < -h.y || mx.z < -h.z) return false;
+  f3 e0 = v1 - v0, e1 = v2 - v1, e2 = v0 - v2;
+  f3 n = cross3(e0, e1);
+  if (fabsf(dot3(n, v0)) > dot3(h, abs3(n))) return false;
+  return axis_test(mk3(0.0f, -e0.z, e0.y), v0, v1, v2, h) && axis_test(mk3(0.0f, -e1.z, e1.y), v0, v1, v2, h) &&
+         axis_test(mk3(0.0f, -e2.z, e2.y), v0, v1, v2, h) && axis_test(mk3(e0.z, 0.0f, -e0.x), v0, v1, v2, h) &&
+         axis_test(mk3(e1.z, 0.0f, -e1.x), v0, v1, v2, h) && axis_test(mk3(e2.z, 0.0f, -e2.x), v0, v1, v2, h) &&
+         axis_test(mk3(-e0.y, e0.x, 0.0f), v0, v1, v2, h) && axis_test(mk3(-e1.y, e1.x, 0.0f), v0, v1, v2, h) &&
+         axis_test(mk3(-e2.y, e2.x, 0.0f), v0, v1, v2, h);
+}
+
+// Cell range of a triangle at level L, clipped to the window (voxRange, src/core/voxel.ts:293-300).
+struct Range {
+  i3 lo, n;
+};
+__device__ __forceinline__ Range vox_range(const VoxK& k, const Tri& T, int L, float cell) {
+  f3 lo = region_lo(k, L);
+  f3 mn = min3(min3(T.a, T.b), T.c), mx = max3(max3(T.a, T.b), T.c);
+  int lim = f2i((float)k.G * vox_size(k, L) * 8.0f / cell);
+  i3 zero{0, 0, 0}, l3{lim, lim, lim};
+  i3 a = clampi3(f2i3(floor3((mn - lo) / cell)), zero, l3);
+  i3 b = clampi3(f2i3(floor3((mx - lo) / cell)) + 1, zero, l3);
+  return Range{a, maxi3(b - a, zero)};
+}
+
+// Cell c of level L >= 1 lies inside level L-1's window (finerHolds, src/core/voxel.ts:303-307).
+__device__ __forceinline__ bool finer_holds(const VoxK& k, int L, i3 c) {
+  if (L == 0) return false;
+  i3 f = (origin(k, L) * 8 + c) * 2 - origin(k, L - 1) * 8;
+  return inside3(f, k.E);
+}
+__device__ __forceinline__ bool finer_holds_range(const VoxK& k, int L, const Range& r, int s) {
+  return finer_holds(k, L, r.lo * s) && finer_holds(k, L, (r.lo + r.n) * s - 1);
+}
+// finerBuilt / finerBuiltRange (src/core/voxel.ts:311-320): held by the finer level unless its brick was dropped.
+__device__ __forceinline__ bool finer_built(const VoxK& k, int L, i3 c, uint32_t drops) {
+  if (!finer_holds(k, L, c)) return false;
+  if (drops == 0u) return true;
+  i3 f = (origin(k, L) * 8 + c) * 2 - origin(k, L - 1) * 8;
+  return k.ptr[ptr_index(k, L - 1, shr3(f, 3))] != PTR_DROP;
+}
+
+__device__ __forceinline__ i3 cell_of(const Range& r, int q) {
+  return i3{r.lo.x + q % r.n.x, r.lo.y + (q / r.n.x) % r.n.y, r.lo.z + q / (r.n.x * r.n.y)};
+}
+
+// Wavefront-aggregated append: lanes with `want` get consecutive slots of list[] past *counter; one atomic per wave.
+__device__ __forceinline__ void wave_append(bool want, uint32_t* counter, uint32_t* list, uint32_t value) {
+  uint64_t m = __ballot(want);
+  if (m == 0) return;
+  uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (want) list[base + below] = value;
+}
+
+__device__ __forceinline__ void mark_brick(const VoxK& k, uint32_t i) {
+  if (__hip_atomic_load(&k.ptr[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == PTR_EMPTY) atomicMin(&k.ptr[i], PTR_MARK);
+}
+
+// markRange (src/core/voxel.ts:348-357).
+__device__ void mark_range(const VoxK& k, const Tri& T, int L, const Range& r, int start, int stride) {
+  float bs = vox_size(k, L) * 8.0f;
+  f3 lo = region_lo(k, L);
+  int n = r.n.x * r.n.y * r.n.z;
+  f3 h = splat3(0.5f * bs * 1.0001f);
+  for (int j = start; j < n; j += stride) {
+    i3 b = cell_of(r, j);
+    if (finer_holds(k, L, b * 8) && finer_holds(k, L, b * 8 + 7)) continue;
+    if (n == 1 || tri_box(lo + (tof3(b) + splat3(0.5f)) * bs, h, T.a, T.b, T.c)) mark_brick(k, ptr_index(k, L, b));
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_clear(VoxK k) {
+  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+  if (i < HDR_WORDS) k.hdr[i] = 0u;
+  if (i < (uint32_t)(k.NL * k.per)) k.ptr[i] = PTR_EMPTY;
+}
+
+__global__ void __launch_bounds__(BLK) k_mark_small(VoxK k) {
+  uint32_t t = blockIdx.x * BLK + threadIdx.x;
+  bool valid = t < k.ntris;
+  Tri T{};
+  if (valid) T = load_tri(k, t);
+  for (int L = 0; L < k.NL; L++) {
+    bool work = false, big = false;
+    Range r{};
+    if (valid) {
+      r = vox_range(k, T, L, vox_size(k, L) * 8.0f);
+      int n = r.n.x * r.n.y * r.n.z;
+      if (n != 0 && !finer_holds_range(k, L, r, 8)) {
+        big = n > 8;
+        work = !big;
+      }
+    }
+    wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.ntris, t);
+    if (work) mark_range(k, T, L, r, 0, 1);
+  }
+}
+
+// One wavefront per listed (triangle, level); entries of all levels are taken in level order.
+__device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hdr_base, int& L, uint32_t& idx) {
+  for (L = 0; L < k.NL; L++) {
+    uint32_t n = k.hdr[hdr_base + L];
+    if (e < n) {
+      idx = e;
+      return true;
+    }
+    e -= n;
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) {
+  uint32_t total = 0;
+  for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGMARK_L + L];
+  uint32_t wave = (blockIdx.x * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
+  uint32_t waves = (gridDim.x * BLK) >> 6;
+  for (uint32_t e = wave; e < total; e += waves) {
+    int L;
+    uint32_t idx;
+    big_entry(k, e, H_BIGMARK_L, L, idx);
+    uint32_t t = k.big_mark[(size_t)L * k.ntris + idx];
+    Tri T = load_tri(k, t);
+    mark_range(k, T, L, vox_range(k, T, L, vox_size(k, L) * 8.0f), (int)lane, 64);
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) {
+  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+  uint32_t per = (uint32_t)k.per;
+  if (i >= (uint32_t)(k.NL - 1) * per) return;
+  if (__hip_atomic_load(&k.ptr[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != PTR_MARK) return;
+  int L = (int)(i / per);
+  uint32_t r = i % per, G = (uint32_t)k.G;
+  i3 w = origin(k, L) + i3{(int)(r % G), (int)((r / G) % G), (int)(r / (G * G))};
+  for (L++; L < k.NL; L++) {
+    w = shr3(w, 1);
+    i3 b = w - origin(k, L);
+    if (!inside3(b, k.G)) return;
+    mark_brick(k, ptr_index(k, L, b));
+  }
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads); returns the exclusive prefix, *total the sum.
+__device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t* sh, uint32_t* total) {
+  uint32_t tid = threadIdx.x;
+  uint32_t lane = tid & 63u, wv = tid >> 6;
+  // Inclusive scan within the wavefront.
+  uint32_t v = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(v, o);
+    if (lane >= (uint32_t)o) v += y;
+  }
+  if (lane == 63u) sh[wv] = v;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t a = 0;
+    for (uint32_t q = 0; q < blockDim.x / 64u; q++) {
+      uint32_t s = sh[q];
+      sh[q] = a;
+      a += s;
+    }
+    sh[16] = a;
+  }
+  __syncthreads();
+  uint32_t ex = sh[wv] + v - x;
+  *total = sh[16];
+  __syncthreads();
+  return ex;
+}
+
+// Ordered brick allocation: order o = 0.. walks level NL-1 first, pointer index ascending within a level
+// (scanMarks, src/core/voxel.ts:428-448); ranks past max_bricks are dropped (allocBricks, voxel.ts:450-470).
+__global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) {
+  __shared__ uint32_t sh[32];
+  __shared__ uint32_t lvl[RTGI_MAX_LEVELS];
+  uint32_t per = (uint32_t)k.per, NL = (uint32_t)k.NL, total = NL * per;
+  uint32_t chunk = (total + blockDim.x - 1) / blockDim.x;
+  uint32_t o0 = threadIdx.x * chunk, o1 = min(o0 + chunk, total);
+  if (threadIdx.x < RTGI_MAX_LEVELS) lvl[threadIdx.x] = 0u;
+  uint32_t cnt = 0;
+  for (uint32_t o = o0; o < o1; o++) {
+    uint32_t i = (NL - 1u - o / per) * per + o % per;
+    cnt += k.ptr[i] == PTR_MARK ? 1u : 0u;
+  }
+  uint32_t all;
+  uint32_t rank = block_scan(cnt, sh, &all);
+  uint32_t drops = 0;
+  uint32_t G = (uint32_t)k.G;
+  for (uint32_t o = o0; o < o1; o++) {
+    uint32_t L = NL - 1u - o / per, r = o % per, i = L * per + r;
+    if (k.ptr[i] != PTR_MARK) continue;
+    if (rank >= (uint32_t)k.max_bricks) {
+      k.ptr[i] = PTR_DROP;
+      drops++;
+    } else {
+      k.ptr[i] = rank;
+      k.bcode[rank] = (L << 27) | ((r % G) << 18) | (((r / G) % G) << 9) | (r / (G * G));
+      atomicAdd(&lvl[L], 1u);
+    }
+    rank++;
+  }
+  if (drops) atomicAdd(&k.hdr[H_BRICK_OVF], drops);
+  __syncthreads();
+  if (threadIdx.x == 0) k.hdr[H_MARKED] = all;
+  if (threadIdx.x < NL) k.hdr[H_BRICKS_L + threadIdx.x] = lvl[threadIdx.x];
+}
+
+__device__ __forceinline__ uint32_t n_bricks(const VoxK& k) { return min(k.hdr[H_MARKED], (uint32_t)k.max_bricks); }
+
+__global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) {
+  uint32_t nb = n_bricks(k);
+  uint32_t n = nb * 512u;
+  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    k.rep[i] = 0u;
+    if ((i & 511u) < 16u) k.occ[(i >> 9) * 16u + (i & 511u)] = 0u;
+  }
+}
+
+// voxelizeOne (src/core/voxel.ts:476-490), alpha test omitted: the materials of this path are untextured and opaque.
+__device__ __forceinline__ void voxelize_one(const VoxK& k, const Tri& T, uint32_t t, int L, i3 c, bool single,
+                                             uint32_t drops) {
+  if (finer_built(k, L, c, drops)) return;
+  float v = vox_size(k, L);
+  f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v;
+  if (!single && !tri_box(p, splat3(0.5f * v * 1.0001f), T.a, T.b, T.c)) return;
+  uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
+  if (ptr >= PTR_MARK) return;
+  uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+  uint32_t* wp = &k.occ[ptr * 16u + (bit >> 5)];
+  uint32_t m = 1u << (bit & 31u);
+  if ((__hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) atomicOr(wp, m);
+  uint32_t* rp = &k.rep[ptr * 512u + bit];
+  if (__hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t + 1u) atomicMax(rp, t + 1u);
+}
+
+__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
+  uint32_t t = blockIdx.x * BLK + threadIdx.x;
+  bool valid = t < k.ntris;
+  uint32_t drops = k.hdr[H_BRICK_OVF];
+  Tri T{};
+  if (valid) T = load_tri(k, t);
+  for (int L = 0; L < k.NL; L++) {
+    bool work = false, big = false;
+    Range r{};
+    int n = 0;
+    if (valid) {
+      r = vox_range(k, T, L, vox_size(k, L));
+      n = r.n.x * r.n.y * r.n.z;
+      if (n != 0 && !(drops == 0u && finer_holds_range(k, L, r, 1))) {
+        big = n > 8;
+        work = !big;
+      }
+    }
+    wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.ntris, t);
+    if (work)
+      for (int q = 0; q < n; q++) voxelize_one(k, T, t, L, cell_of(r, q), n == 1, drops);
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) {
+  uint32_t total = 0;
+  for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGVOX_L + L];
+  uint32_t drops = k.hdr[H_BRICK_OVF];
+  uint32_t wave = (blockIdx.x * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
+  uint32_t waves = (gridDim.x * BLK) >> 6;
+  for (uint32_t e = wave; e < total; e += waves) {
+    int L;
+    uint32_t idx;
+    big_entry(k, e, H_BIGVOX_L, L, idx);
+    uint32_t t = k.big_vox[(size_t)L * k.ntris + idx];
+    Tri T = load_tri(k, t);
+    Range r = vox_range(k, T, L, vox_size(k, L));
+    int n = r.n.x * r.n.y * r.n.z;
+    for (int q = (int)lane; q < n; q += 64) voxelize_one(k, T, t, L, cell_of(r, q), false, drops);
+  }
+}
+
+// Level L from level L-1, one thread per occupancy word of a finer brick (reduce, src/core/voxel.ts:540-577).
+__global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) {
+  uint32_t nb = n_bricks(k);
+  uint32_t n = nb * 16u;
+  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    uint32_t ptr = i >> 4, w = i & 15u;
+    uint32_t code = k.bcode[ptr];
+    int Lc = (int)(code >> 27);
+    if (Lc + 1 != L) continue;
+    uint32_t word = k.occ[ptr * 16u + w];
+    if (word == 0u) continue;
+    i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c0 = b * 8 + i3{0, (int)((w & 1u) * 4u), (int)(w >> 1)};
+    i3 pc0 = shr3(origin(k, Lc) * 8 + c0, 1) - origin(k, L) * 8;
+    uint32_t pptr = k.ptr[ptr_index(k, L, shr3(pc0, 3))];
+    if (pptr >= PTR_MARK) continue;
+    uint32_t pbit0 = (uint32_t)((pc0.x & 7) + 8 * (pc0.y & 7) + 64 * (pc0.z & 7));
+    uint32_t mask = 0;
+    uint32_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+    while (word != 0u) {
+      uint32_t kb = (uint32_t)__builtin_ctz(word);
+      word &= word - 1u;
+      uint32_t px = (kb & 7u) >> 1, py = kb >> 4;
+      mask |= 1u << ((pbit0 + px + 8u * py) & 31u);
+      uint32_t rv = k.rep[ptr * 512u + w * 32u + kb];
+      if (py == 0u)
+        lo[px] = max(lo[px], rv);
+      else
+        hi[px] = max(hi[px], rv);
+    }
+    uint32_t* wp = &k.occ[pptr * 16u + (pbit0 >> 5)];
+    if ((__hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != mask) atomicOr(wp, mask);
+    for (uint32_t px = 0; px < 4u; px++)
+      for (uint32_t py = 0; py < 2u; py++) {
+        uint32_t rv = py == 0u ? lo[px] : hi[px];
+        if (rv == 0u) continue;
+        uint32_t* rp = &k.rep[pptr * 512u + pbit0 + px + 8u * py];
+        if (__hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rv) atomicMax(rp, rv);
+      }
+  }
+}
+
+// Attribute slots by prefix sum over bricks in slot order (allocVoxels, src/core/voxel.ts:579-588, with the atomic
+// counter's order fixed to the slot order).
+__global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) {
+  __shared__ uint32_t sh[32];
+  uint32_t nb = n_bricks(k);
+  uint32_t chunk = (nb + blockDim.x - 1) / blockDim.x;
+  uint32_t b0 = threadIdx.x * chunk, b1 = min(b0 + chunk, nb);
+  uint32_t cnt = 0;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
+    for (int q = 0; q < 4; q++) {
+      uint4 x = o[q];
+      cnt += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    }
+  }
+  uint32_t all;
+  uint32_t first = block_scan(cnt, sh, &all);
+  uint32_t ovf = 0;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
+    uint32_t n = 0;
+    for (int q = 0; q < 4; q++) {
+      uint4 x = o[q];
+      n += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    }
+    if (first + n > (uint32_t)k.max_voxels) {
+      k.battr[b] = ATTR_NONE;
+      ovf++;
+    } else {
+      k.battr[b] = first;
+    }
+    first += n;
+  }
+  if (ovf) atomicAdd(&k.hdr[H_VOXEL_OVF], ovf);
+  if (threadIdx.x == 0) k.hdr[H_VOXELS] = all;
+}
+
+// Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
+__device__ f3 closest_bary(f3 p, f3 a, f3 b, f3 c) {
+  f3 ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return mk3(1.0f, 0.0f, 0.0f);
+  f3 bp = p - b;
+  float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return mk3(0.0f, 1.0f, 0.0f);
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float w = d1 / (d1 - d3);
+    return mk3(1.0f - w, w, 0.0f);
+  }
+  f3 cp = p - c;
+  float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return mk3(0.0f, 0.0f, 1.0f);
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    return mk3(1.0f - w, 0.0f, w);
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return mk3(0.0f, 1.0f - w, w);
+  }
+  float dn = 1.0f / (va + vb + vc);
+  float v = vb * dn, w = vc * dn;
+  return mk3(1.0f - v - w, v, w);
+}
+
+// WGSL pack4x8unorm / pack2x16snorm, by their specified formulas.
+__device__ __forceinline__ uint32_t unorm8(float e) {
+  return (uint32_t)floorf(0.5f + 255.0f * fminf(1.0f, fmaxf(0.0f, e)));
+}
+__device__ __forceinline__ uint32_t snorm16(float e) {
+  return (uint32_t)(int)floorf(0.5f + 32767.0f * fminf(1.0f, fmaxf(-1.0f, e))) & 0xffffu;
+}
+
+__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) {
+  uint32_t nb = n_bricks(k);
+  uint32_t n = nb * 512u;
+  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
+    const uint32_t* o = k.occ + (size_t)ptr * 16u;
+    uint32_t word = o[w];
+    if ((word & (1u << (bit & 31u))) == 0u) continue;
+    uint32_t first = k.battr[ptr];
+    if (first == ATTR_NONE) continue;
+    uint32_t r = __popc(word & ((1u << (bit & 31u)) - 1u));
+    for (uint32_t j = 0; j < w; j++) r += __popc(o[j]);
+    uint32_t ai = first + r;
+    uint32_t code = k.bcode[ptr];
+    int L = (int)(code >> 27);
+    i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c = b * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * vox_size(k, L);
+    uint32_t rp = k.rep[ptr * 512u + bit];
+    if (rp == 0u) continue;  // every occupied voxel has a representative; kept as a guard on the load below
+    Tri T = load_tri(k, rp - 1u);
+    // surfaceAt with wantAttr (src/core/voxel.ts:325-336): white texture, so the albedo is baseColorFactor.
+    f3 wb = closest_bary(p, T.a, T.b, T.c);
+    f3 nn = ldp(k.nrm, T.i0) * wb.x + ldp(k.nrm, T.i1) * wb.y + ldp(k.nrm, T.i2) * wb.z;
+    if (!(dot3(nn, nn) > 1e-12f)) nn = cross3(T.b - T.a, T.c - T.a);
+    nn = normalize3(nn);
+    f3 q = T.a * wb.x + T.b * wb.y + T.c * wb.z;
+    const float4 m = reinterpret_cast<const float4*>(k.mat)[T.m];
+    float alb[3] = {1.0f * m.x, 1.0f * m.y, 1.0f * m.z};
+    uint32_t pa = unorm8(alb[0]) | (unorm8(alb[1]) << 8) | (unorm8(alb[2]) << 16) | (unorm8(0.0f) << 24);
+    // octEnc (src/core/voxel.ts:584-588).
+    float s = fabsf(nn.x) + fabsf(nn.y) + fabsf(nn.z);
+    float px = nn.x / s, py = nn.y / s;
+    if (nn.z < 0.0f) {
+      float ox = px, oy = py;
+      px = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+      py = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+    }
+    uint32_t pn = snorm16(px) | (snorm16(py) << 16);
+    k.attr[ai * 2u] = pa;
+    k.attr[ai * 2u + 1u] = pn;
+    k.plane[ai] = dot3(nn, q - p);
+  }
+}
+
+int grid_for(uint64_t n) {
+  uint64_t g = (n + BLK - 1) / BLK;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+int launch_vox_build(const VoxK& k, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int persist = 2048;  // 256 CUs x 8 blocks of 256 threads for the grid-stride stages
+  k_clear<<<grid_for((uint64_t)std::max(k.NL * k.per, (int)HDR_WORDS)), BLK, 0, s>>>(k);
+  if (k.ntris) k_mark_small<<<grid_for(k.ntris), BLK, 0, s>>>(k);
+  k_mark_big<<<persist, BLK, 0, s>>>(k);
+  if (k.NL > 1) k_mark_parents<<<grid_for((uint64_t)(k.NL - 1) * k.per), BLK, 0, s>>>(k);
+  k_alloc_bricks<<<1, 1024, 0, s>>>(k);
+  k_init_bricks<<<persist, BLK, 0, s>>>(k);
+  if (k.ntris) k_vox_small<<<grid_for(k.ntris), BLK, 0, s>>>(k);
+  k_vox_big<<<persist, BLK, 0, s>>>(k);
+  for (int L = 1; L < k.NL; L++) k_reduce<<<persist, BLK, 0, s>>>(k, L);
+  k_alloc_voxels<<<1, 1024, 0, s>>>(k);
+  k_resolve<<<persist, BLK, 0, s>>>(k);
+  return (int)hipGetLastError();
+}

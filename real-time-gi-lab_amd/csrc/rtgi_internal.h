@@ -1,0 +1,91 @@
+// Internal declarations of librtgi: kernel argument blocks, launchers and error helpers.
+#pragma once
+
+#include <stdint.h>
+
+#include "rtgi.h"
+
+// Records msg as the calling thread's last error and returns code.
+int rtgi_fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Header (counter) words at the start of the clipmap: the reference's header slots (src/core/voxel.ts:11), with
+// word 0 = every marked brick (allocated = min(word 0, max_bricks)) and words 16-31 = big-list lengths.
+enum : uint32_t {
+  HDR_WORDS = 64,
+  H_MARKED = 0,
+  H_VOXELS = 1,
+  H_BRICK_OVF = 2,
+  H_BIG_OVF = 3,
+  H_VOXEL_OVF = 4,
+  H_BRICKS_L = 8,      // + level
+  H_BIGVOX_L = 16,     // + level
+  H_BIGMARK_L = 24,    // + level
+};
+
+constexpr uint32_t PTR_EMPTY = RTGI_PTR_EMPTY;
+constexpr uint32_t PTR_DROP = RTGI_PTR_DROP;
+constexpr uint32_t PTR_MARK = RTGI_PTR_MARK;
+constexpr uint32_t ATTR_NONE = 0xffffffffu;
+
+// Clipmap of one world in HBM. Structure of arrays, so that each access class is dense and aligned:
+//   ptr    [levels][G^3]       brick pointer per window brick (index z-major: (bz G + by) G + bx), or a sentinel
+//   occ    [max_bricks][16]    occupancy words of a brick (bit = x + 8 y + 64 z), 64 B per brick
+//   battr  [max_bricks]        first attribute slot of the brick's voxels (ATTR_NONE: no slots)
+//   bcode  [max_bricks]        level << 27 | bx << 18 | by << 9 | bz (the reference's brick code word)
+//   rep    [max_bricks][512]   representative triangle + 1 per voxel (build temporary, kept for readback)
+//   attr   [max_voxels][2]     pack4x8unorm albedo, pack2x16snorm oct normal
+//   plane  [max_voxels]        representative plane offset from the voxel centre
+struct VoxK {
+  float v0;
+  int32_t NL, G, E, per;  // per = G^3
+  int32_t max_bricks, max_voxels;
+  int32_t origin[RTGI_MAX_LEVELS][3];
+  uint32_t* hdr;
+  uint32_t* ptr;
+  uint32_t* occ;
+  uint32_t* battr;
+  uint32_t* bcode;
+  uint32_t* rep;
+  uint32_t* attr;
+  float* plane;
+  uint32_t* big_vox;   // [levels][ntris]
+  uint32_t* big_mark;  // [levels][ntris]
+  const float* pos;
+  const float* nrm;
+  const uint32_t* tri;  // 4 words per triangle
+  const float* mat;     // 4 floats per material
+  uint32_t ntris;
+};
+
+struct CascK {
+  int32_t n;        // probes per axis
+  int32_t res;      // directions per axis
+  int32_t vox_level;
+  float b0, b1;     // interval in units of s0 (f32, as the Layout uniform stores them)
+  uint64_t ray_off; // first ray of the cascade
+  uint64_t rays;
+};
+
+struct TraceK {
+  VoxK vx;
+  int32_t N;
+  CascK c[RTGI_MAX_CASCADES];
+  int32_t G0[3];  // absolute c0 grid index of probe (0,0,0); a multiple of 2^(N-1)
+  float s0;
+  float vox_skip, vox_offset;
+  float sun_dir[3], sun_e[3], sky[3], sun_vis;
+  int32_t n_point;
+  float point_pos[RTGI_MAX_POINT_LIGHTS][3];
+  float point_i[RTGI_MAX_POINT_LIGHTS][3];
+  float omega0[16];
+  float* rad;       // 4 floats per ray
+  uint32_t* hits;   // per ray, or null
+  float* sh;        // [27][probes0^3]
+  uint64_t total_rays;
+};
+
+// Launchers (each returns a hipError_t as int).
+int launch_vox_build(const VoxK& k, void* stream);
+int launch_trace(const TraceK& k, void* stream);
+int launch_merge(const TraceK& k, int c, void* stream);
+int launch_sh(const TraceK& k, void* stream);

@@ -1,0 +1,56 @@
+// Device-side f32 vector helpers with a fixed evaluation order. Every expression is written so that the rounding
+// sequence is the one of the WGSL it restates (left-to-right sums, dot(a, b) = (a.x b.x + a.y b.y) + a.z b.z, no
+// contraction: the sources are compiled with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct f3 {
+  float x, y, z;
+};
+struct i3 {
+  int x, y, z;
+};
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 splat3(float s) { return f3{s, s, s}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 abs3(f3 a) { return f3{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+__device__ __forceinline__ f3 min3(f3 a, f3 b) { return f3{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ f3 max3(f3 a, f3 b) { return f3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+__device__ __forceinline__ f3 floor3(f3 a) { return f3{floorf(a.x), floorf(a.y), floorf(a.z)}; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 normalize3(f3 a) {
+  float l = sqrtf(dot3(a, a));
+  return a / l;
+}
+__device__ __forceinline__ f3 tof3(i3 a) { return f3{(float)a.x, (float)a.y, (float)a.z}; }
+// f32 -> i32 conversion that saturates like WGSL's instead of being undefined out of range.
+__device__ __forceinline__ int f2i(float f) {
+  f = fminf(fmaxf(f, -2147483648.0f), 2147483520.0f);
+  return (int)f;
+}
+__device__ __forceinline__ i3 f2i3(f3 a) { return i3{f2i(a.x), f2i(a.y), f2i(a.z)}; }
+__device__ __forceinline__ i3 operator+(i3 a, i3 b) { return i3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ i3 operator-(i3 a, i3 b) { return i3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ i3 operator*(i3 a, int s) { return i3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ i3 operator+(i3 a, int s) { return i3{a.x + s, a.y + s, a.z + s}; }
+__device__ __forceinline__ i3 operator-(i3 a, int s) { return i3{a.x - s, a.y - s, a.z - s}; }
+__device__ __forceinline__ i3 shr3(i3 a, int s) { return i3{a.x >> s, a.y >> s, a.z >> s}; }
+__device__ __forceinline__ i3 clampi3(i3 a, i3 lo, i3 hi) {
+  return i3{min(max(a.x, lo.x), hi.x), min(max(a.y, lo.y), hi.y), min(max(a.z, lo.z), hi.z)};
+}
+__device__ __forceinline__ i3 maxi3(i3 a, i3 b) { return i3{max(a.x, b.x), max(a.y, b.y), max(a.z, b.z)}; }
+__device__ __forceinline__ bool inside3(i3 a, int n) {
+  return a.x >= 0 && a.y >= 0 && a.z >= 0 && a.x < n && a.y < n && a.z < n;
+}

@@ -1,0 +1,309 @@
+"""Host-side mirror of the reference renderer's pass interface for the per-frame GI update, over librtgi's C ABI.
+
+Reference interface (owoDra/real-time-gi-lab) and what stands in for it here:
+  VoxParams / DEFAULT_VOX / voxParams      src/core/voxel.ts:31-50     -> VoxParams, DEFAULT_VOX, vox_params
+  RcParams / DEFAULT_PARAMS                src/core/rc.ts:42-91        -> RcParams, DEFAULT_PARAMS (world-tracer subset)
+  createVoxWorld(...) -> VoxWorld          src/core/voxel.ts:746-925   -> World (voxel half: update, encode_staged,
+                                                                          read_counters, used_bytes)
+  createRc(...) -> Rc                      src/core/rc.ts:1019-1236    -> World (cascade half: read_cascade, read_sh)
+  SUN_E, SKY                               src/core/rcrun.ts:26-27     -> SUN_E, SKY
+  parseConfig / named configs              src/runner/configs.ts:47-73 -> CONFIGS, config()
+
+Errors follow the reference's: bad configuration raises synchronously (RtgiError, like its JS throws); GPU pool
+overflow is not an error and shows in read_counters().
+"""
+import ctypes as C
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _lib as _L
+from ._lib import RtgiError, check, lib, ptr
+
+__all__ = ["VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World",
+           "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY", "device_count", "oct_solid_angles",
+           "lib"]
+
+SUN_E = (9.0, 8.5, 7.8)
+SKY = (0.45, 0.6, 0.9)
+
+
+@dataclasses.dataclass
+class VoxParams:
+    """VoxParams (src/core/voxel.ts:31-38) without the dynamic-layer pool."""
+    v0: float = 0.2
+    levels: int = 5
+    G: int = 32
+    maxBricks: int = 4608
+    maxVoxels: int = 524288
+
+
+DEFAULT_VOX = VoxParams()
+
+
+def vox_params(**o):
+    """voxParams (src/core/voxel.ts:46-50): DEFAULT_VOX with overrides; unknown keys raise."""
+    for k in o:
+        if k not in VoxParams.__dataclass_fields__:
+            raise ValueError(f"unknown voxel parameter {k}")
+    return dataclasses.replace(DEFAULT_VOX, **o)
+
+
+@dataclasses.dataclass
+class RcParams:
+    """The RcParams fields (src/core/rc.ts:42-91) the world-space voxel tracer uses, plus the dense probe grid."""
+    cascades: int = 5
+    probes0: int = 64        # c0 probes per axis of the dense grid (replaces screen registration)
+    s0: float = 0.2          # c0 spacing (m); the reference's sMin raised to v0 (src/core/rcrun.ts:242)
+    kInterval: float = 1.0
+    levelMap: float = 1.0
+    voxScale: float = 0.25
+    voxOffset: float = 1.0
+    voxSkip: float = 0.5
+    resCap: int = 8
+    recordHits: bool = False
+    noMerge: bool = False
+
+
+DEFAULT_PARAMS = RcParams()
+
+
+@dataclasses.dataclass
+class Frame:
+    eye: tuple
+    sunDir: tuple = (-0.3, 1.0, 0.25)
+    sunE: tuple = SUN_E
+    sky: tuple = SKY
+    sunVis: float = 1.0
+    points: tuple = ()       # ((pos3, intensity3), ...)
+
+    def to_c(self):
+        f = _L.FrameC()
+        sd = np.asarray(self.sunDir, np.float64)
+        sd = sd / np.linalg.norm(sd)  # packFrame normalises sunDir (src/core/frame.ts:84)
+        for a in range(3):
+            f.eye[a] = float(self.eye[a])
+            f.sun_dir[a] = float(sd[a])
+            f.sun_e[a] = float(self.sunE[a])
+            f.sky[a] = float(self.sky[a])
+        f.sun_vis = float(self.sunVis)
+        if len(self.points) > _L.MAX_POINT:
+            raise ValueError("at most 8 point lights")
+        f.n_point = len(self.points)
+        for k, (p, i) in enumerate(self.points):
+            for a in range(3):
+                f.point_pos[k][a] = float(p[a])
+                f.point_i[k][a] = float(i[a])
+        return f
+
+
+@dataclasses.dataclass
+class Scene:
+    """SceneData subset: positions, normals, triangle table (v0, v1, v2, material) and materials (rgba)."""
+    pos: np.ndarray
+    nrm: np.ndarray
+    tri: np.ndarray
+    materials: np.ndarray
+    eye: tuple = (0.0, 0.0, 0.0)
+    sunDir: tuple = (-0.3, 1.0, 0.25)
+    bounds: tuple = None
+
+    @property
+    def ntris(self):
+        return int(self.tri.shape[0])
+
+
+def generate_scene(kind, seed=1, frame=0, extent=0.0, buildings=0, detail=1.0):
+    """Seeded procedural scene from the library ('atrium' or 'city')."""
+    kinds = {"atrium": _L.SCENE_ATRIUM, "city": _L.SCENE_CITY}
+    p = _L.SceneParamsC(kinds[kind], seed, frame, extent, buildings, detail)
+    out = C.POINTER(_L.SceneC)()
+    check(lib().rtgi_scene_generate(C.byref(p), C.byref(out)))
+    s = out.contents
+    try:
+        sc = Scene(pos=_L.as_np(s.pos, s.nverts * 3, np.float32).reshape(-1, 3),
+                   nrm=_L.as_np(s.nrm, s.nverts * 3, np.float32).reshape(-1, 3),
+                   tri=_L.as_np(s.tri, s.ntris * 4, np.uint32).reshape(-1, 4),
+                   materials=_L.as_np(s.materials, s.nmat * 4, np.float32).reshape(-1, 4),
+                   eye=tuple(s.eye), sunDir=tuple(s.sun_dir),
+                   bounds=(tuple(s.bounds_lo), tuple(s.bounds_hi)))
+    finally:
+        lib().rtgi_scene_free(out)
+    return sc
+
+
+def device_count():
+    n = C.c_int32()
+    check(lib().rtgi_device_count(C.byref(n)))
+    return n.value
+
+
+def oct_solid_angles(res, samples=1 << 20):
+    """octSolidAngles (src/core/frame.ts:105-114)."""
+    out = np.zeros(res * res, np.float32)
+    check(lib().rtgi_oct_solid_angles(res, samples, ptr(out)))
+    return out
+
+
+class World:
+    """One GI world on the current GPU: createVoxWorld + createRc of the reference, backed by an rtgi_world."""
+
+    def __init__(self, vox: VoxParams, rc: RcParams):
+        self.vox, self.rc = vox, rc
+        vp = _L.VoxParamsC(vox.v0, vox.levels, vox.G, vox.maxBricks, vox.maxVoxels)
+        flags = (_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
+        rp = _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
+                          rc.voxSkip, rc.resCap, flags)
+        h = C.c_void_p()
+        check(lib().rtgi_world_create(C.byref(vp), C.byref(rp), C.byref(h)))
+        self._h = h
+        self.scene = None
+
+    # -- lifetime
+    def destroy(self):
+        if self._h:
+            lib().rtgi_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+    # -- inputs
+    def upload_scene(self, scene: Scene):
+        pos = np.ascontiguousarray(scene.pos, np.float32)
+        nrm = np.ascontiguousarray(scene.nrm, np.float32)
+        tri = np.ascontiguousarray(scene.tri, np.uint32)
+        mat = np.ascontiguousarray(scene.materials, np.float32)
+        check(lib().rtgi_world_upload_scene(self._h, ptr(pos), ptr(nrm), pos.shape[0], ptr(tri), tri.shape[0],
+                                            ptr(mat), mat.shape[0]))
+        self.scene = scene
+
+    def update_vertices(self, first, pos, nrm):
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        check(lib().rtgi_world_update_vertices(self._h, first, pos.shape[0], ptr(pos), ptr(nrm)))
+
+    def set_frame(self, frame: Frame):
+        self.frame = frame
+        f = frame.to_c()
+        check(lib().rtgi_world_set_frame(self._h, C.byref(f)))
+
+    def update(self, eye):
+        """VoxWorld.update(eye) (src/core/voxel.ts:896): re-centres the clipmap (and the probe grid) on eye."""
+        f = dataclasses.replace(self.frame, eye=tuple(eye)) if getattr(self, "frame", None) else Frame(eye=tuple(eye))
+        self.set_frame(f)
+
+    # -- execution
+    def encode(self, stream=None):
+        check(lib().rtgi_world_enqueue(self._h, stream))
+
+    def encode_staged(self, stages=_L.STAGE_ALL, stream=None):
+        check(lib().rtgi_world_enqueue_stages(self._h, stages, stream))
+
+    def set_timing(self, on=True):
+        check(lib().rtgi_world_set_timing(self._h, 1 if on else 0))
+
+    def stage_ms(self):
+        ms = np.zeros(4, np.float32)
+        check(lib().rtgi_world_stage_ms(self._h, ptr(ms)))
+        return dict(zip(("vox", "trace", "merge", "sh"), ms.tolist()))
+
+    def sync(self):
+        check(lib().rtgi_world_sync(self._h))
+
+    # -- outputs
+    def layout(self):
+        lo = _L.LayoutC()
+        check(lib().rtgi_world_get_layout(self._h, C.byref(lo)))
+        N = lo.cascades
+        return dict(levels=lo.levels, G=lo.G, E=lo.E,
+                    window_origin=[list(lo.window_origin[L]) for L in range(lo.levels)],
+                    cascades=N, probes=list(lo.probes[:N]), res=list(lo.res[:N]), vox_level=list(lo.vox_level[:N]),
+                    grid_origin=list(lo.grid_origin), rays=list(lo.rays[:N]), total_rays=lo.total_rays,
+                    b0=list(lo.b0[:N]), b1=list(lo.b1[:N]), omega0=np.array(lo.omega0[:], np.float32),
+                    device_bytes=lo.device_bytes)
+
+    def read_counters(self):
+        """VoxWorld.readCounters (src/core/voxel.ts:908-918)."""
+        c = _L.CountersC()
+        check(lib().rtgi_world_read_counters(self._h, C.byref(c)))
+        NL = self.vox.levels
+        return dict(bricks=c.bricks, marked=c.marked, voxels=c.voxels, brickOverflow=c.brick_overflow,
+                    bigOverflow=c.big_overflow, voxelOverflow=c.voxel_overflow,
+                    bricksPerLevel=list(c.bricks_per_level[:NL]), bigPerLevel=list(c.big_per_level[:NL]))
+
+    def used_bytes(self, c=None):
+        """VoxWorld.usedBytes (src/core/voxel.ts:919-922) for this layout: persistent and build-temporary bytes."""
+        c = c or self.read_counters()
+        NL, G = self.vox.levels, self.vox.G
+        vox = min(c["voxels"], self.vox.maxVoxels)
+        return dict(persistent=(64 + NL * G ** 3) * 4 + c["bricks"] * (64 + 8) + vox * 12,
+                    buildTemp=c["bricks"] * 512 * 4)
+
+    def read_clipmap(self, level, rep=True, attrs=True):
+        E = self.vox.G * 8
+        n = E ** 3
+        occ = np.zeros(n // 32, np.uint32)
+        rp = np.zeros(n, np.uint32) if rep else None
+        alb = np.zeros(n, np.uint32) if attrs else None
+        nrm = np.zeros(n, np.uint32) if attrs else None
+        pl = np.zeros(n, np.float32) if attrs else None
+        st = np.zeros(self.vox.G ** 3, np.uint8)
+        check(lib().rtgi_world_read_clipmap(self._h, level, ptr(occ), ptr(rp), ptr(alb), ptr(nrm), ptr(pl), ptr(st)))
+        return dict(occ=occ, rep=rp, albedo=alb, normal=nrm, plane=pl, brick_state=st)
+
+    def read_cascade(self, c, hits=False):
+        lo = self.layout()
+        n = lo["rays"][c]
+        rgba = np.zeros((n, 4), np.float32)
+        h = np.zeros(n, np.uint32) if hits else None
+        check(lib().rtgi_world_read_cascade(self._h, c, ptr(rgba), ptr(h)))
+        return (rgba, h) if hits else rgba
+
+    def read_sh(self):
+        n = self.rc.probes0 ** 3
+        sh = np.zeros((n, 27), np.float32)
+        check(lib().rtgi_world_read_sh(self._h, ptr(sh)))
+        return sh
+
+
+def _atrium_points(seed=1):
+    rng = np.random.default_rng(seed)
+    pts = []
+    for _ in range(4):
+        p = (float(rng.uniform(-7, 7)), float(rng.uniform(1.5, 6.0)), float(rng.uniform(-7, 7)))
+        i = tuple(float(x) for x in rng.uniform(2.0, 6.0, 3))
+        pts.append((p, i))
+    return tuple(pts)
+
+
+# Named configurations (the role of src/runner/configs.ts). 'atrium' and 'city' are the plain FP32 forms of the
+# 50k-triangle atrium (4 x 64^3 clipmap at 0.125 m, 32^3 c0 probes, 4 cascades) and the 500k-triangle city block
+# (4 x 128^3 clipmap at 0.25 m, 64^3 c0 probes, 5 cascades).
+CONFIGS = {
+    "atrium": dict(scene=dict(kind="atrium", seed=1), vox=VoxParams(v0=0.125, levels=4, G=8, maxBricks=4 * 8 ** 3,
+                                                                    maxVoxels=1 << 20),
+                   rc=RcParams(cascades=4, probes0=32, s0=0.625), points=True),
+    "city": dict(scene=dict(kind="city", seed=1), vox=VoxParams(v0=0.25, levels=4, G=16, maxBricks=4 * 16 ** 3,
+                                                                maxVoxels=1 << 22),
+                 rc=RcParams(cascades=5, probes0=64, s0=1.0), points=False),
+}
+
+
+def config(name, **rc_over):
+    """(scene, VoxParams, RcParams, Frame) of a named configuration."""
+    cf = CONFIGS[name]
+    scene = generate_scene(**cf["scene"])
+    rc = dataclasses.replace(cf["rc"], **rc_over)
+    frame = Frame(eye=scene.eye, sunDir=scene.sunDir, points=_atrium_points() if cf["points"] else ())
+    return scene, cf["vox"], rc, frame

@@ -1,0 +1,74 @@
+"""CPU-side checks of the boundary: the library loads, exports exactly the declared C ABI, validates arguments and
+reports errors the reference's way (synchronous failure with a message), and the scene generator is deterministic."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import rtgi
+from rtgi import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "rtgi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rtgi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_matches_binding():
+    assert declared_functions() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r" T (rtgi_[a-z0-9_]+)$", out.stdout, flags=re.M))
+    missing = set(declared_functions()) - exported
+    assert not missing, missing
+    for name in declared_functions():
+        assert hasattr(rtgi.lib(), name)
+
+
+def test_version_and_errors_without_gpu():
+    assert b"gfx950" in rtgi.lib().rtgi_version()
+    n = C.c_int32(-1)
+    assert rtgi.lib().rtgi_device_count(C.byref(n)) == 0 and n.value >= 0
+    # Invalid configurations fail synchronously with a message (voxel.ts:749 'at most 8 levels', rc.ts:1022).
+    with pytest.raises(rtgi.RtgiError, match="at most 8 levels"):
+        rtgi.World(rtgi.VoxParams(v0=0.25, levels=9, G=4, maxBricks=10, maxVoxels=10), rtgi.RcParams())
+    with pytest.raises(rtgi.RtgiError, match="at most 8 cascades"):
+        rtgi.World(rtgi.VoxParams(v0=0.25, levels=4, G=4, maxBricks=10, maxVoxels=10), rtgi.RcParams(cascades=9))
+    with pytest.raises(rtgi.RtgiError, match="multiple of 64"):
+        rtgi.World(rtgi.VoxParams(v0=0.25, levels=4, G=5, maxBricks=10, maxVoxels=10), rtgi.RcParams())
+    with pytest.raises(rtgi.RtgiError, match="multiple of 2"):
+        rtgi.World(rtgi.VoxParams(v0=0.25, levels=4, G=4, maxBricks=10, maxVoxels=10),
+                   rtgi.RcParams(cascades=4, probes0=12))
+    with pytest.raises(rtgi.RtgiError, match="null"):
+        rtgi._lib.check(rtgi.lib().rtgi_world_sync(None))
+    assert rtgi.lib().rtgi_last_error() != b""
+
+
+def test_vox_params_rejects_unknown_keys():
+    assert rtgi.vox_params(G=16).G == 16
+    with pytest.raises(ValueError, match="unknown voxel parameter"):
+        rtgi.vox_params(bogus=1)
+
+
+@pytest.mark.parametrize("kind,lo,hi", [("atrium", 45_000, 65_000), ("city", 450_000, 560_000)])
+def test_scene_generator(kind, lo, hi):
+    a = rtgi.generate_scene(kind, seed=3, frame=2)
+    b = rtgi.generate_scene(kind, seed=3, frame=2)
+    assert lo <= a.ntris <= hi
+    for k in ("pos", "nrm", "tri", "materials"):
+        assert np.array_equal(getattr(a, k), getattr(b, k))
+    assert a.tri[:, :3].max() < a.pos.shape[0]
+    assert np.all(a.tri[:, 3] == np.arange(a.ntris))
+    alb = a.materials[:, :3]
+    assert alb.min() >= 0.1 and alb.max() <= 0.9
+    c = rtgi.generate_scene(kind, seed=3, frame=5)
+    assert not np.array_equal(a.pos, c.pos)  # moving parts are posed per frame
+    assert np.array_equal(a.tri, c.tri)
