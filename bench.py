@@ -49,7 +49,20 @@ def main():
     ap.add_argument("--config", default="city")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+
+        n = a.copy_calibration << 18  # float32 elements
+        src = torch.ones(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (use --gpus 1)", flush=True)
         sys.exit(2)

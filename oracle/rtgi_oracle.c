@@ -970,6 +970,10 @@ static d3 shadeVoxelD(const View* W, const orc_rc* rc, uint32_t ai, d3 d, d3 ent
   return out;
 }
 
+/* Two values closer than eps: their order may differ between f32 and f64 evaluation. Exact equality counts too:
+ * values that tie exactly in double (a ray through a voxel edge or corner) need not tie in f32. */
+static int near(double a, double b, double eps) { return fabs(a - b) < eps; }
+
 /* Voxel-by-voxel traversal of level L in double; the first occupied voxel entered at t >= tSkip. */
 static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, double t1, double tSkip, uint32_t L,
                       double eps, double out[4], uint32_t* hid, uint8_t* tie) {
@@ -992,7 +996,7 @@ static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, do
     if (tf < tEnd) tEnd = tf;
   }
   double scale = fabs(t1) + 1.0;
-  if (fabs(t - tEnd) < eps * scale) *tie = 1;
+  if (near(t, tEnd, eps * scale)) *tie = 1;
   if (t >= tEnd) return;
   int c[3], st[3];
   double tMax[3], tDelta[3];
@@ -1000,7 +1004,8 @@ static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, do
     st[a] = dd[a] > 0 ? 1 : -1;
     double x = (ov[a] + dv[a] * t - lo[a]) / v;
     int ci = (int)floor(x);
-    if (fabs(x - floor(x + 0.5)) < eps * (fabs(x) + 1) && fabs(dv[a]) >= 1e-9) *tie = 1;
+    /* A start on a voxel plane is exact in both precisions when the interval starts at the probe (t0 = 0). */
+    if (t != 0 && near(x, floor(x + 0.5), eps * (fabs(x) + 1)) && fabs(dv[a]) >= 1e-9) *tie = 1;
     c[a] = ci < 0 ? 0 : (ci > nVox - 1 ? nVox - 1 : ci);
     tMax[a] = ((c[a] + (st[a] > 0 ? 1 : 0)) * v + lo[a] - ov[a]) * inv[a];
     tDelta[a] = fabs(v * inv[a]);
@@ -1010,7 +1015,7 @@ static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, do
     uint32_t br, bit, lev;
     iv3 cell;
     if (voxAt(W, L, I3(c[0], c[1], c[2]), &br, &bit, &lev, &cell)) {
-      if (fabs(tIn - tSkip) < eps * scale) *tie = 1;
+      if (near(tIn, tSkip, eps * scale)) *tie = 1;
       if (tIn >= tSkip) {
         *hid = hitId(p, lev, cell);
         d3 e = D3(o.x + d.x * tIn, o.y + d.y * tIn, o.z + d.z * tIn);
@@ -1022,8 +1027,8 @@ static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, do
     int a = tMax[0] <= tMax[1] && tMax[0] <= tMax[2] ? 0 : (tMax[1] <= tMax[2] ? 1 : 2);
     double tx = tMax[a];
     for (int b2 = 0; b2 < 3; b2++)
-      if (b2 != a && fabs(tMax[b2] - tx) < eps * scale) *tie = 1;
-    if (fabs(tx - tEnd) < eps * scale) *tie = 1;
+      if (b2 != a && near(tMax[b2], tx, eps * scale)) *tie = 1;
+    if (near(tx, tEnd, eps * scale)) *tie = 1;
     if (tx >= tEnd) break;
     tIn = tx;
     c[a] += st[a];
@@ -1061,7 +1066,7 @@ int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint6
       int in = 1;
       for (int a = 0; a < 3; a++) {
         double lo = p->origin[L][a] * bs, hi = lo + p->G * bs;
-        if (fabs(ev[a] - lo) < eps * (fabs(lo) + 1) || fabs(ev[a] - hi) < eps * (fabs(hi) + 1)) tl = 1;
+        if (near(ev[a], lo, eps * (fabs(lo) + 1)) || near(ev[a], hi, eps * (fabs(hi) + 1))) tl = 1;
         if (!(ev[a] > lo && ev[a] < hi)) in = 0;
       }
       if (in) break;

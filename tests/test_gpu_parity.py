@@ -208,3 +208,29 @@ def test_moving_frames(require_gpu):
     check_clipmap(w, o, vox)
     check_cascades(w, o, rc)
     w.destroy()
+
+
+def test_golden_fixture(require_gpu):
+    """The GPU path reproduces the committed fixture (tests/golden/tiny_world.npz) without running the oracle."""
+    import json
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tiny_world.npz"))
+    scene = rtgi.Scene(pos=g["pos"], nrm=g["nrm"], tri=g["tri"], materials=g["mat"])
+    vox = rtgi.VoxParams(**json.loads(str(g["vox"])))
+    rc = rtgi.RcParams(**json.loads(str(g["rc"])))
+    frame = rtgi.Frame(**json.loads(str(g["frame"])))
+    say("case: golden fixture tiny_world.npz")
+    w = run_world(scene, vox, rc, frame)
+    for L in range(vox.levels):
+        cm = w.read_clipmap(L)
+        assert np.array_equal(cm["occ"], g[f"occ{L}"]) and np.array_equal(cm["rep"], g[f"rep{L}"])
+    lay = w.layout()
+    off = 0
+    for c in range(rc.cascades):
+        rgba, h = w.read_cascade(c, hits=True)
+        n = lay["rays"][c]
+        assert np.array_equal(h, g["hits"][off:off + n])
+        np.testing.assert_allclose(rgba, g["merged"][off:off + n], rtol=RTOL, atol=ATOL)
+        off += n
+    np.testing.assert_allclose(w.read_sh(), g["sh"], rtol=RTOL, atol=ATOL)
+    w.destroy()

@@ -1,0 +1,454 @@
+"""Pins the CPU oracle (oracle/rtgi_oracle.c) against facts that do not come from its own formulas.
+
+The reference ships no golden vectors or tests for this path, so each oracle function is checked against an
+independent statement of what it computes: LP feasibility for triangle/box overlap, brute-force minimisation for the
+closest point, the WGSL specification's packing values, a double-precision brute-force voxeliser, trilinear
+interpolation for the merge, scipy's real spherical harmonics for the SH basis, and the FP64 tracer for first hits.
+"""
+import dataclasses
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import linprog
+
+import rtgi
+from conftest import random_scene
+from oracle import oracle as orc
+from oracle.oracle import Oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# triBox (voxel.ts:231-248) against LP feasibility: max s such that a point of the triangle lies in the box shrunk by s.
+
+def overlap_margin(c, h, a, b, cc):
+    V = np.stack([a, b, cc]).astype(np.float64)  # 3 x 3
+    # variables: l0 l1 l2 s ; maximise s  <=>  minimise -s
+    A_ub, b_ub = [], []
+    for ax in range(3):
+        row = np.zeros(4)
+        row[:3] = V[:, ax]
+        row[3] = 1.0
+        A_ub.append(row.copy())  # p_ax + s <= c + h
+        b_ub.append(c[ax] + h[ax])
+        row[:3] = -V[:, ax]
+        A_ub.append(row)  # -p_ax + s <= -(c - h)
+        b_ub.append(-(c[ax] - h[ax]))
+    r = linprog([0, 0, 0, -1], A_ub=np.array(A_ub), b_ub=np.array(b_ub), A_eq=[[1, 1, 1, 0]], b_eq=[1],
+                bounds=[(0, None)] * 3 + [(None, None)], method="highs")
+    assert r.status == 0
+    return -r.fun
+
+
+def test_tri_box_against_lp():
+    rng = np.random.default_rng(5)
+    agree, checked = 0, 0
+    for _ in range(3000):
+        c = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.05, 0.5, 3)
+        a, b, cc = (c + rng.normal(0, 0.8, 3) for _ in range(3))
+        c, h, a, b, cc = (np.float32(x) for x in (c, h, a, b, cc))
+        m = overlap_margin(c, h, a, b, cc)
+        if abs(m) < 1e-4:
+            continue  # within rounding of touching
+        checked += 1
+        agree += orc.tri_box(c, h, a, b, cc) == (m > 0)
+    assert checked > 2500 and agree == checked
+
+
+def test_tri_box_exact_dyadic_cases():
+    # Dyadic coordinates make every f32 operation exact: touching faces count as overlap (closed boxes).
+    h = [0.5, 0.5, 0.5]
+    assert orc.tri_box([0, 0, 0], h, [0.5, 0, 0], [1, 1, 0], [1, 0, 1])
+    assert not orc.tri_box([0, 0, 0], h, [0.5 + 2 ** -10, 0, 0], [1, 1, 0], [1, 0, 1])
+    # The triangle plane passes the box corner exactly (plane test boundary).
+    assert orc.tri_box([0, 0, 0], h, [1.5, 0, 0], [0, 1.5, 0], [0, 0, 1.5])
+    assert not orc.tri_box([0, 0, 0], h, [1.5 + 2 ** -8, 0, 0], [0, 1.5 + 2 ** -8, 0], [0, 0, 1.5 + 2 ** -8])
+    # An edge axis separates: a sliver crossing near a box edge but outside it (only the 9 cross axes catch this).
+    assert not orc.tri_box([0, 0, 0], h, [1.0, 0.75, -2], [0.75, 1.0, -2], [0.875, 0.875, 2])
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# closestBary (voxel.ts:250-267) against brute-force minimisation of the distance.
+
+def test_closest_bary_brute_force():
+    rng = np.random.default_rng(7)
+    g = np.linspace(0, 1, 401)
+    L0, L1 = np.meshgrid(g, g)
+    keep = L0 + L1 <= 1
+    bary = np.stack([L0[keep], L1[keep], 1 - L0[keep] - L1[keep]], 1)
+    for _ in range(200):
+        a, b, c = (rng.normal(0, 1, 3).astype(np.float32) for _ in range(3))
+        p = rng.normal(0, 1.5, 3).astype(np.float32)
+        w = orc.closest_bary(p, a, b, c)
+        assert np.all(w >= -1e-5) and abs(w.sum() - 1) < 1e-5
+        q = w[0] * a.astype(np.float64) + w[1] * b + w[2] * c
+        d_orc = np.linalg.norm(q - p)
+        pts = bary @ np.stack([a, b, c]).astype(np.float64)
+        d_bf = np.min(np.linalg.norm(pts - p, axis=1))
+        assert d_orc <= d_bf + 1e-4
+        assert d_orc >= d_bf - 1e-2  # the grid is 1/400 fine
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# pack4x8unorm / pack2x16snorm (WGSL spec) and octEnc / octDecode.
+
+def test_pack_golden_vectors():
+    vec = json.load(open(os.path.join(GOLDEN, "pack_vectors.json")))
+    for e, expect in vec["pack4x8unorm"]:
+        assert orc.pack4x8unorm(e) == int(expect, 16), e
+    for (x, y), expect in vec["pack2x16snorm"]:
+        assert orc.pack2x16snorm(x, y) == int(expect, 16), (x, y)
+
+
+def oct_decode(u, v):
+    """octDecode (rc.ts:291-298) in double, for round trips."""
+    f = np.array([u * 2 - 1, v * 2 - 1])
+    n = np.array([f[0], f[1], 1 - abs(f[0]) - abs(f[1])])
+    t = max(-n[2], 0)
+    n[0] += -t if n[0] >= 0 else t
+    n[1] += -t if n[1] >= 0 else t
+    return n / np.linalg.norm(n)
+
+
+def test_oct_encode_axes_and_round_trip():
+    assert np.allclose(orc.oct_enc([0, 0, 1]), [0, 0])
+    assert np.allclose(orc.oct_enc([1, 0, 0]), [1, 0])
+    assert np.allclose(orc.oct_enc([0, -1, 0]), [0, -1])
+    assert np.allclose(orc.oct_enc([0, 0, -1]), [1, 1])
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        n = rng.normal(0, 1, 3)
+        n /= np.linalg.norm(n)
+        e = orc.oct_enc(n.astype(np.float32))
+        assert np.all(np.abs(e) <= 1 + 1e-6)
+        m = oct_decode(e[0] * 0.5 + 0.5, e[1] * 0.5 + 0.5)
+        assert np.allclose(m, n, atol=1e-5)
+
+
+def test_oct_solid_angles():
+    w = orc.oct_solid_angles(4)
+    assert abs(w.astype(np.float64).sum() - 4 * np.pi) < 1e-5
+    W = w.reshape(4, 4)
+    # Mirror symmetries of the octahedral map (u -> 1-u, v -> 1-v, transpose) hold up to the Fibonacci count noise.
+    for M in (W[:, ::-1], W[::-1, :], W.T):
+        assert np.allclose(M, W, rtol=2e-3)
+    assert np.array_equal(w, rtgi.oct_solid_angles(4))  # the library computes the same table
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Voxel build against a brute-force voxeliser in double precision.
+
+def sat_margin_np(P, h, a, b, c):
+    """Vectorised SAT in double for boxes (centres P: n x 3, half h) against one triangle: the minimum over the 13
+    axes of the overlap slack (> 0 overlap, < 0 separated), normalised by the axis length."""
+    v0, v1, v2 = a - P, b - P, c - P
+    e = [v1 - v0, v2 - v1, v0 - v2]
+    axes = [np.eye(3)[i] for i in range(3)]
+    n = np.cross(b - a, c - a)
+    axes.append(n)
+    for ei in (b - a, c - b, a - c):
+        for k in range(3):
+            axes.append(np.cross(np.eye(3)[k], ei))
+    slack = np.full(P.shape[0], np.inf)
+    for ax in axes:
+        ln = np.linalg.norm(ax)
+        if ln < 1e-12:
+            continue
+        p0, p1, p2 = v0 @ ax, v1 @ ax, v2 @ ax
+        r = np.abs(ax) @ h
+        s = np.minimum(r - np.minimum(np.minimum(p0, p1), p2), np.maximum(np.maximum(p0, p1), p2) + r) / ln
+        slack = np.minimum(slack, s)
+    del e
+    return slack
+
+
+def brute_force_clipmap(scene, vox, origins):
+    """Occupancy and representative per level by direct overlap tests (double), coarse cells inside the finer
+    window as the OR / max of their 8 children, clipped single-cell ranges occupied without a test. Returns
+    (occ, rep, ambiguous) per level as E^3 arrays."""
+    G, E, NL = vox.G, vox.G * 8, vox.levels
+    idx = np.arange(E)
+    Z, Y, X = np.meshgrid(idx, idx, idx, indexing="ij")
+    cells = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    out = []
+    for L in range(NL):
+        v = vox.v0 * 2 ** L
+        lo = np.array(origins[L], np.float64) * v * 8
+        rep = np.zeros(E ** 3, np.int64)
+        amb = np.zeros(E ** 3, bool)
+        if L > 0:
+            f = (np.array(origins[L]) * 8 + cells) * 2 - np.array(origins[L - 1]) * 8
+            held = np.all((f >= 0) & (f < E), 1)
+        else:
+            held = np.zeros(E ** 3, bool)
+        P = lo + (cells + 0.5) * v
+        h = np.full(3, 0.5 * v * 1.0001)
+        for t in range(scene.ntris):
+            i0, i1, i2, _ = scene.tri[t]
+            a, b, c = (scene.pos[i].astype(np.float64) for i in (i0, i1, i2))
+            mn, mx = np.minimum(np.minimum(a, b), c), np.maximum(np.maximum(a, b), c)
+            ra = np.clip(np.floor((mn - lo) / v), 0, E).astype(int)
+            rb = np.clip(np.floor((mx - lo) / v) + 1, 0, E).astype(int)
+            n = np.maximum(rb - ra, 0)
+            if n.prod() == 0:
+                continue
+            inr = np.all((cells >= ra) & (cells < rb), 1) & ~held
+            if n.prod() == 1:
+                hit = inr
+            else:
+                s = sat_margin_np(P[inr], h, a, b, c)
+                hit = np.zeros(E ** 3, bool)
+                hit[np.nonzero(inr)[0][s >= 0]] = True
+                amb[np.nonzero(inr)[0][np.abs(s) < 1e-5]] = True
+            rep[hit] = np.maximum(rep[hit], t + 1)
+        if L > 0:
+            prev_rep, prev_amb = out[L - 1][1], out[L - 1][2]
+            fine = np.stack(np.unravel_index(np.arange(E ** 3), (E, E, E))[::-1], 1)  # x, y, z of finer cells
+            parent = ((np.array(origins[L - 1]) * 8 + fine) >> 1) - np.array(origins[L]) * 8
+            pi = parent[:, 0] + E * (parent[:, 1] + E * parent[:, 2])
+            np.maximum.at(rep, pi, prev_rep)
+            np.logical_or.at(amb, pi, prev_amb)
+        out.append((rep > 0, rep, amb))
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_build_against_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    scene = random_scene(rng, 25, -3.0, 3.0, 0.5)
+    vox = rtgi.VoxParams(v0=0.25, levels=3, G=2, maxBricks=3 * 8, maxVoxels=1 << 14)
+    rc = rtgi.RcParams(cascades=2, probes0=4, s0=0.5)
+    frame = rtgi.Frame(eye=(0.1, 0.05, -0.2))
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    org = orc.origins(vox.v0, vox.levels, vox.G, frame.eye)
+    bf = brute_force_clipmap(scene, vox, org)
+    E = vox.G * 8
+    for L in range(vox.levels):
+        cm = o.clipmap(L)
+        occ = np.unpackbits(cm["occ"].view(np.uint8), bitorder="little").astype(bool)[:E ** 3]
+        occ_bf, rep_bf, amb = bf[L]
+        ok = ~amb
+        assert np.array_equal(occ[ok], occ_bf[ok]), f"level {L}"
+        assert np.array_equal(cm["rep"][ok & occ].astype(np.int64), rep_bf[ok & occ]), f"level {L}"
+        assert occ_bf.sum() > 0
+
+
+def test_build_invariants_atrium():
+    scene, vox, rc, frame = rtgi.config("atrium")
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    E = vox.G * 8
+    org = orc.origins(vox.v0, vox.levels, vox.G, frame.eye)
+    prev = None
+    for L in range(vox.levels):
+        cm = o.clipmap(L)
+        occ = np.unpackbits(cm["occ"].view(np.uint8), bitorder="little").astype(bool)[:E ** 3].reshape(E, E, E)
+        rep = cm["rep"].reshape(E, E, E)
+        assert np.array_equal(occ, rep > 0)
+        assert rep.max() <= scene.ntris
+        # Every occupied voxel carries attributes (no voxel overflow here) and a unit-length oct normal code.
+        assert np.all(cm["albedo"].reshape(E, E, E)[occ] != 0)
+        if prev is not None:
+            # coarse = OR of children wherever the finer window holds the cell (voxel.ts:16-19).
+            pocc, prep = prev
+            d = (np.array(org[L - 1]) * 8 - np.array(org[L]) * 16) // 2  # finer window origin in coarse cells
+            sub = occ[d[2]:d[2] + E // 2, d[1]:d[1] + E // 2, d[0]:d[0] + E // 2]
+            kids = pocc.reshape(E // 2, 2, E // 2, 2, E // 2, 2).any(axis=(1, 3, 5))
+            assert np.array_equal(sub, kids)
+            kr = prep.reshape(E // 2, 2, E // 2, 2, E // 2, 2).max(axis=(1, 3, 5))
+            assert np.array_equal(rep[d[2]:d[2] + E // 2, d[1]:d[1] + E // 2, d[0]:d[0] + E // 2], kr)
+        prev = (occ, rep)
+
+
+def test_pool_overflow_drops_finest_first():
+    rng = np.random.default_rng(9)
+    scene = random_scene(rng, 300, -4.0, 4.0, 0.6)
+    base = rtgi.VoxParams(v0=0.25, levels=3, G=4, maxBricks=3 * 64, maxVoxels=1 << 16)
+    rc = rtgi.RcParams(cascades=2, probes0=4, s0=0.5)
+    frame = rtgi.Frame(eye=(0.1, 0.2, 0.3))
+    full = Oracle(base, rc, frame)
+    full.build(scene)
+    nb = full.counters()["bricks"]
+    small = dataclasses.replace(base, maxBricks=nb - 5)
+    o = Oracle(small, rc, frame)
+    o.build(scene)
+    c = o.counters()
+    assert c["brickOverflow"] == 5 and c["bricks"] == nb - 5
+    # Coarse levels keep every brick; only level-0 bricks are dropped (src/core/voxel.ts:405-410).
+    assert c["bricksPerLevel"][1:] == full.counters()["bricksPerLevel"][1:]
+    st0 = o.clipmap(0)["brick_state"]
+    assert np.count_nonzero(st0 == 2) == 5
+    # Dropped level-0 cells are voxelised at level 1 from the triangles, so level 1 still covers every occupied
+    # level-0 cell of the full build.
+    E = base.G * 8
+    occ1 = np.unpackbits(o.clipmap(1)["occ"].view(np.uint8), bitorder="little")[:E ** 3].reshape(E, E, E)
+    occ0f = np.unpackbits(full.clipmap(0)["occ"].view(np.uint8), bitorder="little")[:E ** 3].reshape(E, E, E)
+    org = orc.origins(base.v0, base.levels, base.G, frame.eye)
+    d = (np.array(org[0]) * 8 - np.array(org[1]) * 16) // 2
+    kids = occ0f.reshape(E // 2, 2, E // 2, 2, E // 2, 2).any(axis=(1, 3, 5))
+    assert np.all(occ1[d[2]:d[2] + E // 2, d[1]:d[1] + E // 2, d[0]:d[0] + E // 2][kids])
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Trace against the FP64 oracle, special cases, merge and SH.
+
+@pytest.mark.parametrize("s0,min_clean,sh_median", [(0.625, 0.35, 0.05), (0.61, 0.8, 1e-3)])
+def test_trace_matches_fp64_away_from_ties(s0, min_clean, sh_median):
+    """First hits equal the FP64 voxel walk on every ray that passes no voxel edge, corner, interval end or skip
+    distance within 1e-5 (relative); with probes on voxel planes (s0 = 0.625 = 5 v0) many rays pass exact edges."""
+    scene, vox, rc, frame = rtgi.config("atrium", s0=s0)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    rgba, hits = o.trace()
+    rgbt, hits64, tie = o.trace_f64(eps=1e-5)
+    clean = tie == 0
+    assert clean.mean() > min_clean
+    assert np.array_equal(hits[clean], hits64[clean])
+    assert (hits != hits64).mean() < 0.02
+    hit = clean & (hits != 0xFFFFFFFF)
+    assert hit.sum() > 1000
+    err = np.abs(rgba[hit, :3] - rgbt[hit, :3]) / np.maximum(np.abs(rgbt[hit, :3]), 1e-3)
+    assert err.max() < 1e-4 and np.median(err) < 1e-6
+    # Merged SH against the FP64 pipeline: dominated by the rays whose first hit differs at exact edges (each probe
+    # averages many upper-cascade rays), so the bound is on the median and depends on how many rays pass edges.
+    m32, sh32 = o.merge_sh(rgba)
+    m64, sh64 = o.merge_sh_f64(rgbt)
+    rel = np.abs(sh32 - sh64).max(axis=1) / np.maximum(np.abs(sh64).max(axis=1), 1e-6)
+    assert np.median(rel) < sh_median
+
+
+def test_trace_empty_world_and_single_voxel():
+    vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
+    rc = rtgi.RcParams(cascades=2, probes0=2, s0=0.5)
+    frame = rtgi.Frame(eye=(0.0, 0.0, 0.0))
+    empty = rtgi.Scene(pos=np.zeros((3, 3), np.float32), nrm=np.zeros((3, 3), np.float32),
+                       tri=np.zeros((0, 4), np.uint32), materials=np.ones((1, 4), np.float32))
+    o = Oracle(vox, rc, frame)
+    o.build(empty)
+    rgba, hits = o.trace()
+    assert np.all(hits == 0xFFFFFFFF) and np.all(rgba[:, 3] == 1) and np.all(rgba[:, :3] == 0)
+    m, sh = o.merge_sh(rgba)
+    assert np.allclose(m[:, :3], np.float32(frame.sky))
+    # One tiny triangle inside the level-0 cell (x, y, z) = (9, 8, 8) (centre (0.375, 0.125, 0.125) m): every hit
+    # is that voxel, and the ray from the probe at the origin along +x (within its direction bin) hits it.
+    p = np.array([[0.30, 0.10, 0.10], [0.45, 0.10, 0.12], [0.35, 0.20, 0.15]], np.float32)
+    one = rtgi.Scene(pos=p, nrm=np.tile([[0, 0, 1]], (3, 1)).astype(np.float32),
+                     tri=np.array([[0, 1, 2, 0]], np.uint32), materials=np.array([[0.5, 0.5, 0.5, 1]], np.float32))
+    o.build(one)
+    rgba, hits = o.trace()
+    E = vox.G * 8
+    got = set(int(h) for h in hits if h != 0xFFFFFFFF)
+    org = orc.origins(vox.v0, vox.levels, vox.G, frame.eye)
+    cell = (np.array([0.375, 0.125, 0.125]) / 0.25 - np.array(org[0]) * 8).astype(int)
+    expect0 = int(cell[0] + E * (cell[1] + E * cell[2]))
+    assert expect0 in got
+    assert all((h >> 29) in (0, 1) for h in got)
+    assert all((h & ((1 << 29) - 1)) == expect0 for h in got if (h >> 29) == 0)
+
+
+def trilinear_merge_reference(rgba_per_c, rc, sky):
+    """Merge by trilinear interpolation of the upper cascade at the probe (dense grid: equal-weight parents = trilinear
+    weights, rc.ts:700-705), with the upper 2x2 direction average."""
+    from scipy.ndimage import map_coordinates
+    N = rc.cascades
+    out = [None] * N
+    for c in range(N - 1, -1, -1):
+        n, res = rc.probes0 >> c, 4 << min(c, rc.resCap)
+        v = rgba_per_c[c].astype(np.float64).reshape(n, n, n, res, res, 4)  # z y x | v u
+        if c == N - 1:
+            up = np.broadcast_to(np.asarray(sky, np.float64), v[..., :3].shape)
+        else:
+            un, ur = n // 2, 4 << min(c + 1, rc.resCap)
+            U = out[c + 1].reshape(un, un, un, ur, ur, 4)[..., :3]
+            if ur != res:
+                U = U.reshape(un, un, un, res, 2, res, 2, 3).mean(axis=(4, 6))
+            g = np.arange(n) / 2.0
+            Zc, Yc, Xc = np.meshgrid(g, g, g, indexing="ij")
+            coords = np.stack([Zc.ravel(), Yc.ravel(), Xc.ravel()])
+            up = np.empty(v[..., :3].shape)
+            for dv in range(res):
+                for du in range(res):
+                    for ch in range(3):
+                        up[:, :, :, dv, du, ch] = map_coordinates(U[:, :, :, dv, du, ch], coords, order=1,
+                                                                  mode="nearest").reshape(n, n, n)
+        m = v.copy()
+        m[..., :3] = v[..., :3] + v[..., 3:4] * up
+        m[..., 3] = 1
+        out[c] = m.reshape(-1, 4)
+    return out
+
+
+def test_merge_is_trilinear_on_the_dense_grid():
+    rng = np.random.default_rng(4)
+    vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
+    for rc in (rtgi.RcParams(cascades=3, probes0=8, s0=0.5), rtgi.RcParams(cascades=3, probes0=8, s0=0.5, resCap=1)):
+        frame = rtgi.Frame(eye=(0.0, 0.0, 0.0), sky=(0.3, 0.5, 0.7))
+        o = Oracle(vox, rc, frame)
+        total = sum(o.cascade_rays())
+        rgba = np.zeros((total, 4), np.float32)
+        rgba[:, :3] = rng.uniform(0, 2, (total, 3))
+        rgba[:, 3] = rng.integers(0, 2, total)
+        m, _ = o.merge_sh(rgba)
+        ref = trilinear_merge_reference(o.split(rgba), rc, frame.sky)
+        for c, (got, exp) in enumerate(zip(o.split(m), ref)):
+            np.testing.assert_allclose(got, exp, rtol=1e-5, atol=1e-5, err_msg=f"cascade {c}")
+
+
+def real_sh(n):
+    """Real SH of degree <= 2 in the basis order of rc.ts:745-749 (0; y, z, x; xy, yz, 3z^2-1, xz, x^2-y^2), from
+    scipy's complex harmonics (Condon-Shortley phase removed)."""
+    from scipy.special import sph_harm_y
+    x, y, z = n
+    theta = np.arccos(np.clip(z, -1, 1))
+    phi = np.arctan2(y, x)
+    Y = lambda l, m: sph_harm_y(l, abs(m), theta, phi)  # noqa: E731
+    def r(l, m):
+        if m == 0:
+            return Y(l, 0).real
+        s = np.sqrt(2) * (-1) ** m
+        return s * (Y(l, m).imag if m < 0 else Y(l, m).real)
+    return np.array([r(0, 0), r(1, -1), r(1, 0), r(1, 1), r(2, -2), r(2, -1), r(2, 0), r(2, 1), r(2, 2)])
+
+
+def test_sh_projection_basis_and_constant_field():
+    vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
+    rc = rtgi.RcParams(cascades=1, probes0=1, s0=0.5)
+    frame = rtgi.Frame(eye=(0.0, 0.0, 0.0), sky=(0.0, 0.0, 0.0))
+    o = Oracle(vox, rc, frame)
+    om = orc.oct_solid_angles(4)
+    a = np.array([np.pi] + [2 * np.pi / 3] * 3 + [np.pi / 4] * 5)
+    for d in range(16):
+        rgba = np.zeros((16, 4), np.float32)
+        rgba[d, :3] = (1.0, 2.0, 0.5)
+        _, sh = o.merge_sh(rgba)
+        n = oct_decode((d % 4 + 0.5) / 4, (d // 4 + 0.5) / 4)
+        exp = np.outer(om[d] * real_sh(n) * a, [1.0, 2.0, 0.5]).ravel()
+        np.testing.assert_allclose(sh[0], exp, rtol=2e-5, atol=1e-6)
+    rgba = np.zeros((16, 4), np.float32)
+    rgba[:, :3] = 1.5
+    _, sh = o.merge_sh(rgba)
+    assert abs(sh[0, 0] - np.pi * 0.282095 * 4 * np.pi * 1.5) < 1e-4
+
+
+def test_golden_world_regression():
+    """The oracle still reproduces the committed fixture (tests/golden/make_golden.py)."""
+    g = np.load(os.path.join(GOLDEN, "tiny_world.npz"))
+    scene = rtgi.Scene(pos=g["pos"], nrm=g["nrm"], tri=g["tri"], materials=g["mat"])
+    vox = rtgi.VoxParams(**json.loads(str(g["vox"])))
+    rc = rtgi.RcParams(**json.loads(str(g["rc"])))
+    frame = rtgi.Frame(**json.loads(str(g["frame"])))
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    for L in range(vox.levels):
+        cm = o.clipmap(L)
+        assert np.array_equal(cm["occ"], g[f"occ{L}"]) and np.array_equal(cm["rep"], g[f"rep{L}"])
+    rgba, hits = o.trace()
+    assert np.array_equal(hits, g["hits"])
+    m, sh = o.merge_sh(rgba)
+    np.testing.assert_array_equal(m, g["merged"])
+    np.testing.assert_array_equal(sh, g["sh"])
