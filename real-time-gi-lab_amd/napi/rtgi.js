@@ -1,0 +1,64 @@
+'use strict';
+// JS face of the Node-API addon: the reference host's pass interface over librtgi.
+//   createVoxWorld(scene, params)  <- createVoxWorld (src/core/voxel.ts:746): update(eye), encode(), readCounters(),
+//                                     usedBytes(c), destroy()
+//   createRc(voxWorld, params)     <- createRc (src/core/rc.ts:1019): encode(), readSh(), readCascade(c), destroy()
+// One library world backs both: the voxel world owns it, and createRc re-creates it with the cascade parameters
+// (its scene upload is repeated). encode() on either runs the whole update (rebuild, trace, merge, SH): the reference
+// records both passes into one command buffer per frame (src/core/rcrun.ts:283-298).
+const path = require('path');
+const addon = require(path.join(__dirname, 'rtgi_napi.node'));
+
+const DEFAULT_VOX = { v0: 0.2, levels: 5, G: 32, maxBricks: 4608, maxVoxels: 524288 };
+const DEFAULT_PARAMS = { cascades: 5, probes0: 64, s0: 0.2, kInterval: 1, levelMap: 1, voxScale: 0.25, voxOffset: 1,
+  voxSkip: 0.5, resCap: 8, recordHits: 0 };
+const SUN_E = [9, 8.5, 7.8];
+const SKY = [0.45, 0.6, 0.9];
+
+function checked(defaults, o, what) {
+  for (const k of Object.keys(o)) if (!(k in defaults)) throw new Error(`unknown ${what} parameter ${k}`);
+  return Object.assign({}, defaults, o);
+}
+function voxParams(o) { return checked(DEFAULT_VOX, o, 'voxel'); }
+function rcParams(o) { return checked(DEFAULT_PARAMS, o, 'cascade'); }
+
+function createVoxWorld(scene, params, frame) {
+  const vw = {
+    params: voxParams(params || {}),
+    rcParams: { cascades: 1, probes0: 1, s0: 1 },
+    frame: Object.assign({ sunE: SUN_E, sky: SKY, sunVis: 1 }, frame || {}, { eye: (frame && frame.eye) || [0, 0, 0] }),
+    handle: null,
+  };
+  vw.rebuild = () => {
+    if (vw.handle) addon.destroy(vw.handle);
+    vw.handle = addon.createWorld(vw.params, vw.rcParams);
+    addon.uploadScene(vw.handle, scene.pos, scene.nrm, scene.tri, scene.materials);
+    addon.setFrame(vw.handle, vw.frame);
+  };
+  vw.update = (eye) => { vw.frame = Object.assign({}, vw.frame, { eye }); addon.setFrame(vw.handle, vw.frame); };
+  vw.encode = () => addon.encode(vw.handle);
+  vw.readCounters = () => Promise.resolve().then(() => addon.readCounters(vw.handle));
+  vw.usedBytes = (c) => {
+    const { levels: NL, G, maxVoxels } = vw.params;
+    return { persistent: (64 + NL * G * G * G) * 4 + c.bricks * 72 + Math.min(c.voxels, maxVoxels) * 12,
+      buildTemp: c.bricks * 512 * 4 };
+  };
+  vw.destroy = () => { if (vw.handle) addon.destroy(vw.handle); vw.handle = null; };
+  vw.rebuild();
+  return vw;
+}
+
+function createRc(voxWorld, params) {
+  voxWorld.rcParams = rcParams(params || {});
+  voxWorld.rebuild();
+  return {
+    params: voxWorld.rcParams,
+    encode: () => addon.encode(voxWorld.handle),
+    readSh: () => Promise.resolve().then(() => addon.readSh(voxWorld.handle)),
+    readCascade: (c) => Promise.resolve().then(() => addon.readCascade(voxWorld.handle, c)),
+    destroy: () => {},
+  };
+}
+
+module.exports = { addon, DEFAULT_VOX, DEFAULT_PARAMS, SUN_E, SKY, voxParams, rcParams, createVoxWorld, createRc,
+  generateScene: addon.generateScene, deviceCount: addon.deviceCount, version: addon.version };

@@ -1,0 +1,67 @@
+"""The Node-API addon (the drop-in binding for the reference's TypeScript host) over the same C ABI."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import rtgi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "real-time-gi-lab_amd", "napi", "rtgi_napi.node")
+NODE = shutil.which("node")
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+def node(script, *args, timeout=120):
+    return subprocess.run([NODE, "-e", script, *args] if not script.endswith(".js") else [NODE, script, *args],
+                          capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+
+
+def test_addon_loads_and_throws_like_the_reference():
+    assert os.path.exists(ADDON), "build the addon (make -C real-time-gi-lab_amd/napi)"
+    r = node("const r=require('./real-time-gi-lab_amd/napi/rtgi.js');"
+             "console.log(JSON.stringify({v:r.version(),n:r.deviceCount()}));"
+             "try{r.voxParams({bogus:1})}catch(e){console.log(e.message)}"
+             "try{r.addon.createWorld({levels:9},{})}catch(e){console.log(e.message)}")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert "gfx950" in json.loads(lines[0])["v"]
+    assert lines[1] == "unknown voxel parameter bogus"
+    assert "at most 8 levels" in lines[2]
+
+
+def test_addon_scene_matches_library():
+    r = node("const r=require('./real-time-gi-lab_amd/napi/rtgi.js');const s=r.generateScene('atrium',3,2);"
+             "let t=0;for(const x of s.tri)t=(t*31+x)>>>0;console.log(s.tri.length/4, t)")
+    assert r.returncode == 0, r.stderr
+    n, h = map(int, r.stdout.split())
+    sc = rtgi.generate_scene("atrium", seed=3, frame=2)
+    t = 0
+    for x in sc.tri.ravel().tolist():
+        t = (t * 31 + x) & 0xFFFFFFFF
+    assert n == sc.ntris and h == t
+
+
+@pytest.mark.gpu
+def test_addon_update_matches_python_path(require_gpu):
+    print("case: node addon, atrium seed 1", flush=True)
+    r = node(os.path.join(ROOT, "tests", "napi_world.js"), "atrium", "1", "0.125", "4", "8", "4", "32", "0.625",
+             timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    scene = rtgi.generate_scene("atrium", seed=1)
+    vox = rtgi.VoxParams(v0=0.125, levels=4, G=8, maxBricks=4 * 8 ** 3, maxVoxels=1 << 20)
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625)
+    with rtgi.World(vox, rc) as w:
+        w.upload_scene(scene)
+        w.set_frame(rtgi.Frame(eye=scene.eye, sunDir=scene.sunDir))
+        w.encode()
+        c = w.read_counters()
+        sh = w.read_sh()
+    for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow", "bricksPerLevel"):
+        assert out["counters"][k] == c[k]
+    np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
+    assert abs(out["shSum"] - float(sh.astype(np.float64).sum())) <= 1e-6 * abs(float(sh.sum()))
