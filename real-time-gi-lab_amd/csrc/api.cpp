@@ -42,6 +42,8 @@ struct rtgi_world {
   int n[RTGI_MAX_CASCADES] = {}, res[RTGI_MAX_CASCADES] = {}, vlev[RTGI_MAX_CASCADES] = {};
   float b0[RTGI_MAX_CASCADES] = {}, b1[RTGI_MAX_CASCADES] = {};
   uint64_t ray_off[RTGI_MAX_CASCADES + 1] = {};
+  uint32_t dir_off[RTGI_MAX_CASCADES + 1] = {};
+  uint64_t avg_off[RTGI_MAX_CASCADES + 1] = {};  // [c]: cascade c's 2x2 average; [N]: total
   int G0[3] = {};
   float omega0[16] = {};
   rtgi_frame frame{};
@@ -58,11 +60,14 @@ struct rtgi_world {
   float* rad = nullptr;
   uint32_t* hits = nullptr;
   float* sh = nullptr;
+  DirEntry* dirs = nullptr;
+  float* avg = nullptr;
   uint64_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
   hipEvent_t ev[5] = {};
   uint32_t last_stages = 0;
+  uint32_t pending_stages = 0;  // stages of the enqueue being recorded
 };
 
 namespace {
@@ -101,9 +106,16 @@ void oct_encode_d(double x, double y, double z, double& u, double& v) {
   v = py * 0.5 + 0.5;
 }
 
+bool is_pow2(double v) {
+  int e;
+  return std::frexp(v, &e) == 0.5;
+}
+
 VoxK vox_args(const rtgi_world* w) {
   VoxK k{};
   k.v0 = (float)w->vp.v0;
+  k.pow2 = is_pow2(w->vp.v0) && (double)k.v0 == w->vp.v0;
+  k.inv_v0 = k.pow2 ? 1.0f / k.v0 : 0.0f;
   k.NL = w->NL;
   k.G = w->G;
   k.E = w->E;
@@ -126,6 +138,8 @@ VoxK vox_args(const rtgi_world* w) {
   k.tri = w->tri;
   k.mat = w->mat;
   k.ntris = w->ntris;
+  k.lds_marks = 0;
+  k.lds_marks = vox_mark_lds_bytes(k) > 0;
   return k;
 }
 
@@ -140,6 +154,15 @@ TraceK trace_args(const rtgi_world* w) {
     t.c[c].b0 = w->b0[c];
     t.c[c].b1 = w->b1[c];
     t.c[c].ray_off = w->ray_off[c];
+    t.c[c].np = (uint32_t)w->n[c] * w->n[c] * w->n[c];
+    t.c[c].dir_off = w->dir_off[c];
+    t.c[c].avg_off = w->avg_off[c];
+    int nl = 0;
+    while ((1 << nl) < w->n[c]) nl++;
+    t.c[c].n_log2 = (1 << nl) == w->n[c] ? nl : -1;
+    int rl = 0;
+    while ((1 << rl) < w->res[c]) rl++;
+    t.c[c].res_log2 = rl;
     t.c[c].rays = w->ray_off[c + 1] - w->ray_off[c];
   }
   for (int a = 0; a < 3; a++) t.G0[a] = w->G0[a];
@@ -156,7 +179,9 @@ TraceK trace_args(const rtgi_world* w) {
   std::memcpy(t.point_pos, w->frame.point_pos, sizeof(t.point_pos));
   std::memcpy(t.point_i, w->frame.point_i, sizeof(t.point_i));
   std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
+  t.dirs = w->dirs;
   t.rad = w->rad;
+  t.avg = w->avg;
   t.hits = w->hits;
   t.sh = w->sh;
   t.total_rays = w->ray_off[w->N];
@@ -183,6 +208,8 @@ void free_all(rtgi_world* w) {
   dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
   size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
   dfree(w, w->sh, np0 * 27);
+  dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
+  dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
 }
 
 int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
@@ -204,6 +231,25 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
 }
 
 }  // namespace
+
+// dirOf (src/core/rc.ts:291-298, 308) and the slab-safe reciprocal of traceVox (src/core/voxel.ts:128-130) in f32,
+// the same operation sequence as the device evaluates (this file builds with -ffp-contract=off; SSE division and
+// square root are IEEE-correct), so the table is bit-identical to per-ray evaluation.
+void dir_entry(int di, int res, DirEntry* out) {
+  float u = ((float)(di % res) + 0.5f) / (float)res, v = ((float)(di / res) + 0.5f) / (float)res;
+  float fx = u * 2.0f - 1.0f, fy = v * 2.0f - 1.0f;
+  float n[3] = {fx, fy, 1.0f - std::fabs(fx) - std::fabs(fy)};
+  float t = std::max(-n[2], 0.0f);
+  n[0] += n[0] >= 0.0f ? -t : t;
+  n[1] += n[1] >= 0.0f ? -t : t;
+  float l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int a = 0; a < 3; a++) {
+    out->d[a] = n[a] / l;
+    float dd = std::fabs(out->d[a]) < 1e-9f ? 1e-9f : out->d[a];
+    out->inv[a] = 1.0f / dd;
+  }
+  out->d[3] = out->inv[3] = 0.0f;
+}
 
 extern "C" const char* rtgi_last_error(void) { return g_err; }
 
@@ -274,6 +320,9 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
     w->vlev[c] = (int)std::min(std::max(L, 0.0), (double)(vp->levels - 1));
     w->ray_off[c] = off;
     off += (uint64_t)w->n[c] * w->n[c] * w->n[c] * w->res[c] * w->res[c];
+    w->dir_off[c + 1] = w->dir_off[c] + (uint32_t)(w->res[c] * w->res[c]);
+    uint64_t npc = (uint64_t)w->n[c] * w->n[c] * w->n[c];
+    w->avg_off[c + 1] = w->avg_off[c] + (c > 0 ? npc * w->res[c - 1] * w->res[c - 1] : 0);
   }
   w->ray_off[Nc] = off;
   rtgi_oct_solid_angles(4, 1 << 20, w->omega0);
@@ -292,6 +341,15 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->rad, (size_t)off * 4);
   if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
   if (!e) e = dalloc(w, &w->sh, np0 * 27);
+  if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
+  if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
+  if (!e) {
+    std::vector<DirEntry> tab(w->dir_off[Nc]);
+    for (int c = 0; c < Nc; c++)
+      for (int di = 0; di < w->res[c] * w->res[c]; di++) dir_entry(di, w->res[c], &tab[w->dir_off[c] + di]);
+    if (hipMemcpy(w->dirs, tab.data(), tab.size() * sizeof(DirEntry), hipMemcpyHostToDevice) != hipSuccess)
+      e = rtgi_fail(RTGI_E_HIP, "direction table upload failed");
+  }
   if (!e && hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess)
     e = rtgi_fail(RTGI_E_HIP, "hipStreamCreate failed");
   for (int i = 0; i < 5 && !e; i++)
@@ -411,6 +469,7 @@ extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* s
   HIPCHK(hipSetDevice(w->device));
   hipStream_t s = stream ? (hipStream_t)stream : w->stream;
   const uint32_t order[4] = {RTGI_STAGE_VOX, RTGI_STAGE_TRACE, RTGI_STAGE_MERGE, RTGI_STAGE_SH};
+  w->pending_stages = stages;
   if (w->timing) HIPCHK(hipEventRecord(w->ev[0], s));
   for (int i = 0; i < 4; i++) {
     if (stages & order[i]) {
@@ -572,8 +631,20 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(hipStreamSynchronize(w->stream));
   size_t n = w->ray_off[c + 1] - w->ray_off[c];
-  if (rgba) HIPCHK(hipMemcpy(rgba, w->rad + w->ray_off[c] * 4, n * 16, hipMemcpyDeviceToHost));
-  if (hits) HIPCHK(hipMemcpy(hits, w->hits + w->ray_off[c], n * 4, hipMemcpyDeviceToHost));
+  size_t np = (size_t)w->n[c] * w->n[c] * w->n[c], dirs = (size_t)w->res[c] * w->res[c];
+  // Device order is direction-major (ray = di * np + probe); the readback is probe-major.
+  if (rgba) {
+    std::vector<float> t(n * 4);
+    HIPCHK(hipMemcpy(t.data(), w->rad + w->ray_off[c] * 4, n * 16, hipMemcpyDeviceToHost));
+    for (size_t d = 0; d < dirs; d++)
+      for (size_t p = 0; p < np; p++) std::memcpy(rgba + (p * dirs + d) * 4, &t[(d * np + p) * 4], 16);
+  }
+  if (hits) {
+    std::vector<uint32_t> t(n);
+    HIPCHK(hipMemcpy(t.data(), w->hits + w->ray_off[c], n * 4, hipMemcpyDeviceToHost));
+    for (size_t d = 0; d < dirs; d++)
+      for (size_t p = 0; p < np; p++) hits[p * dirs + d] = t[d * np + p];
+  }
   return RTGI_OK;
 }
 

@@ -3,22 +3,28 @@
 //
 // Pipeline (one stream, no host round trip; device-side counts bound every grid-stride loop instead of indirect
 // dispatch):
-//   k_clear        header and pointer table                          (clear, voxel.ts:338-344)
-//   k_mark_small   per triangle and level: overlapped bricks;        (markBricks, voxel.ts:347-375)
-//                  ranges of more than 8 bricks are appended, one
-//                  wavefront-aggregated atomic per wave, to a list
-//   k_mark_big     one wavefront per listed (triangle, level)        (markBricksBig, voxel.ts:377-387)
-//   k_mark_parents ancestors of marked bricks                        (markParents, voxel.ts:390-403)
-//   k_alloc_bricks one workgroup: ordered allocation, coarse levels  (countMarks/scanMarks/allocBricks,
-//                  first and by pointer index, finest dropped first   voxel.ts:405-470)
-//   k_init_bricks  zero occupancy and representatives of the pool    (clearKeys, voxel.ts:471-474)
-//   k_vox_small    per triangle and level: SAT per voxel, atomicOr   (voxBits, voxel.ts:476-512)
-//                  of the bit and atomicMax of triangle + 1
-//   k_vox_big      one wavefront per listed (triangle, level)        (voxBitsBig, voxel.ts:514-534)
-//   k_reduce(L)    coarse level L from L-1, one thread per word      (reduce, voxel.ts:540-577)
-//   k_alloc_voxels one workgroup: attribute slots by prefix sum      (allocVoxels, voxel.ts:579-588)
-//   k_resolve      per (brick, voxel): albedo, oct normal, plane     (resolveAttr, voxel.ts:590-620)
+//   k_clear        header and pointer table                               (clear, voxel.ts:338-344)
+//   k_mark         per triangle and level: overlapped bricks, marked in   (markBricks, voxel.ts:347-375)
+//                  an LDS bitmap of all pointers and flushed with plain
+//                  stores (no atomics); ranges of more than 8 bricks go
+//                  to a list with one wavefront-aggregated atomic per wave
+//   k_mark_big     one wavefront per listed (triangle, level), LDS bitmap (markBricksBig, voxel.ts:377-387)
+//   k_mark_parents ancestors of marked bricks                             (markParents, voxel.ts:390-403)
+//   k_alloc_bricks one workgroup, one scan: ordered allocation,         (countMarks/scanMarks/allocBricks,
+//                  coarse levels first, finest dropped first               voxel.ts:405-470)
+//   k_init_bricks  zero occupancy and representatives of the pool         (clearKeys, voxel.ts:471-474)
+//   k_vox_small    per triangle and level: SAT per voxel; bit and         (voxBits, voxel.ts:476-512)
+//                  triangle + 1 merged per workgroup in an LDS hash table,
+//                  then one atomicOr / atomicMax per unique voxel
+//   k_vox_big      one wavefront per listed (triangle, level)             (voxBitsBig, voxel.ts:514-534)
+//   k_reduce(L)    coarse level L from L-1, gathered per parent voxel     (reduce, voxel.ts:540-577)
+//                  (single writer per word: no atomics)
+//   k_alloc_voxels one workgroup, scan rounds: attribute slots by         (allocVoxels, voxel.ts:579-588)
+//                  prefix sum in slot order
+//   k_resolve      per (brick, voxel): albedo, oct normal, plane          (resolveAttr, voxel.ts:590-620)
 //
+// Device-scope atomics bypass the XCD-local L2 on this part, so the build avoids them where the result does not
+// need them (marks, reduce); the voxel updates that remain return nothing, so no thread waits on them.
 // Allocation is always the ordered one: the reference uses it when the pool overflows and an atomic counter
 // otherwise (any order is valid then); a single order makes the raw layout deterministic as well as the canonical
 // content, and voxel-pool overflow then drops the same bricks every run.
@@ -59,6 +65,8 @@ __device__ __forceinline__ f3 region_lo(const VoxK& k, int L) { return tof3(orig
 __device__ __forceinline__ uint32_t ptr_index(const VoxK& k, int L, i3 b) {
   return (uint32_t)L * (uint32_t)k.per + ((uint32_t)b.z * (uint32_t)k.G + (uint32_t)b.y) * (uint32_t)k.G + (uint32_t)b.x;
 }
+// x / cell, as a multiplication when cell is a power of two (then the two are the same f32 value).
+__device__ __forceinline__ float div_cell(float x, float cell, float inv, bool pow2) { return pow2 ? x * inv : x / cell; }
 
 // Akenine-Moller triangle / box overlap (axisTest and triBox, src/core/voxel.ts:231-248).
 __device__ __forceinline__ bool axis_test(f3 a, f3 v0, f3 v1, f3 v2, f3 h) {
@@ -80,17 +88,24 @@ __device__ bool tri_box(f3 c, f3 h, f3 a, f3 b, f3 cc) {
          axis_test(mk3(-e2.y, e2.x, 0.0f), v0, v1, v2, h);
 }
 
-// Cell range of a triangle at level L, clipped to the window (voxRange, src/core/voxel.ts:293-300).
+// Cell range of a triangle at level L, clipped to the window (voxRange, src/core/voxel.ts:293-300). brick: cells of
+// 8 voxels.
 struct Range {
   i3 lo, n;
 };
-__device__ __forceinline__ Range vox_range(const VoxK& k, const Tri& T, int L, float cell) {
+__device__ __forceinline__ Range vox_range(const VoxK& k, const Tri& T, int L, bool brick) {
+  float cell = brick ? vox_size(k, L) * 8.0f : vox_size(k, L);
+  float inv = k.inv_v0 * (1.0f / (float)(1u << L)) * (brick ? 0.125f : 1.0f);
+  bool p2 = k.pow2 != 0;
   f3 lo = region_lo(k, L);
   f3 mn = min3(min3(T.a, T.b), T.c), mx = max3(max3(T.a, T.b), T.c);
-  int lim = f2i((float)k.G * vox_size(k, L) * 8.0f / cell);
+  int lim = f2i(div_cell((float)k.G * vox_size(k, L) * 8.0f, cell, inv, p2));
   i3 zero{0, 0, 0}, l3{lim, lim, lim};
-  i3 a = clampi3(f2i3(floor3((mn - lo) / cell)), zero, l3);
-  i3 b = clampi3(f2i3(floor3((mx - lo) / cell)) + 1, zero, l3);
+  f3 a0 = mn - lo, b0 = mx - lo;
+  f3 af = mk3(div_cell(a0.x, cell, inv, p2), div_cell(a0.y, cell, inv, p2), div_cell(a0.z, cell, inv, p2));
+  f3 bf = mk3(div_cell(b0.x, cell, inv, p2), div_cell(b0.y, cell, inv, p2), div_cell(b0.z, cell, inv, p2));
+  i3 a = clampi3(f2i3(floor3(af)), zero, l3);
+  i3 b = clampi3(f2i3(floor3(bf)) + 1, zero, l3);
   return Range{a, maxi3(b - a, zero)};
 }
 
@@ -115,24 +130,31 @@ __device__ __forceinline__ i3 cell_of(const Range& r, int q) {
   return i3{r.lo.x + q % r.n.x, r.lo.y + (q / r.n.x) % r.n.y, r.lo.z + q / (r.n.x * r.n.y)};
 }
 
+__device__ __forceinline__ uint32_t lane_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Wavefront-aggregated append: lanes with `want` get consecutive slots of list[] past *counter; one atomic per wave.
 __device__ __forceinline__ void wave_append(bool want, uint32_t* counter, uint32_t* list, uint32_t value) {
   uint64_t m = __ballot(want);
   if (m == 0) return;
-  uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
   int leader = __ffsll((unsigned long long)m) - 1;
   uint32_t base = 0;
   if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
   base = __shfl(base, leader);
-  if (want) list[base + below] = value;
+  if (want) list[base + lane_below(m)] = value;
 }
 
-__device__ __forceinline__ void mark_brick(const VoxK& k, uint32_t i) {
-  if (__hip_atomic_load(&k.ptr[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == PTR_EMPTY) atomicMin(&k.ptr[i], PTR_MARK);
+// Mark of brick pointer i: a bit of the workgroup's LDS bitmap, or (no bitmap) the pointer itself.
+__device__ __forceinline__ void mark_brick(const VoxK& k, uint32_t* bits, uint32_t i) {
+  if (bits)
+    atomicOr(&bits[i >> 5], 1u << (i & 31u));
+  else if (k.ptr[i] == PTR_EMPTY)
+    k.ptr[i] = PTR_MARK;  // every writer stores the same value, and nothing else writes the table in this phase
 }
 
 // markRange (src/core/voxel.ts:348-357).
-__device__ void mark_range(const VoxK& k, const Tri& T, int L, const Range& r, int start, int stride) {
+__device__ void mark_range(const VoxK& k, uint32_t* bits, const Tri& T, int L, const Range& r, int start, int stride) {
   float bs = vox_size(k, L) * 8.0f;
   f3 lo = region_lo(k, L);
   int n = r.n.x * r.n.y * r.n.z;
@@ -140,7 +162,28 @@ __device__ void mark_range(const VoxK& k, const Tri& T, int L, const Range& r, i
   for (int j = start; j < n; j += stride) {
     i3 b = cell_of(r, j);
     if (finer_holds(k, L, b * 8) && finer_holds(k, L, b * 8 + 7)) continue;
-    if (n == 1 || tri_box(lo + (tof3(b) + splat3(0.5f)) * bs, h, T.a, T.b, T.c)) mark_brick(k, ptr_index(k, L, b));
+    if (n == 1 || tri_box(lo + (tof3(b) + splat3(0.5f)) * bs, h, T.a, T.b, T.c)) mark_brick(k, bits, ptr_index(k, L, b));
+  }
+}
+
+extern __shared__ uint32_t lds_bits[];
+
+__device__ __forceinline__ void bits_clear(const VoxK& k) {
+  uint32_t words = ((uint32_t)(k.NL * k.per) + 31u) >> 5;
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) lds_bits[i] = 0u;
+  __syncthreads();
+}
+// Marked pointers of the bitmap become PTR_MARK with plain stores (idempotent across workgroups).
+__device__ __forceinline__ void bits_flush(const VoxK& k) {
+  __syncthreads();
+  uint32_t words = ((uint32_t)(k.NL * k.per) + 31u) >> 5;
+  for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) {
+    uint32_t w = lds_bits[i];
+    while (w) {
+      uint32_t b = (uint32_t)__builtin_ctz(w);
+      w &= w - 1u;
+      k.ptr[i * 32u + b] = PTR_MARK;
+    }
   }
 }
 
@@ -150,25 +193,32 @@ __global__ void __launch_bounds__(BLK) k_clear(VoxK k) {
   if (i < (uint32_t)(k.NL * k.per)) k.ptr[i] = PTR_EMPTY;
 }
 
-__global__ void __launch_bounds__(BLK) k_mark_small(VoxK k) {
-  uint32_t t = blockIdx.x * BLK + threadIdx.x;
-  bool valid = t < k.ntris;
-  Tri T{};
-  if (valid) T = load_tri(k, t);
-  for (int L = 0; L < k.NL; L++) {
-    bool work = false, big = false;
-    Range r{};
-    if (valid) {
-      r = vox_range(k, T, L, vox_size(k, L) * 8.0f);
-      int n = r.n.x * r.n.y * r.n.z;
-      if (n != 0 && !finer_holds_range(k, L, r, 8)) {
-        big = n > 8;
-        work = !big;
+// markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level
+// list append sees converged waves).
+__global__ void __launch_bounds__(BLK) k_mark(VoxK k) {
+  uint32_t* bits = k.lds_marks ? lds_bits : nullptr;
+  if (bits) bits_clear(k);
+  for (uint32_t base = blockIdx.x * BLK; base < k.ntris; base += gridDim.x * BLK) {
+    uint32_t t = base + threadIdx.x;
+    bool valid = t < k.ntris;
+    Tri T{};
+    if (valid) T = load_tri(k, t);
+    for (int L = 0; L < k.NL; L++) {
+      bool work = false, big = false;
+      Range r{};
+      if (valid) {
+        r = vox_range(k, T, L, true);
+        int n = r.n.x * r.n.y * r.n.z;
+        if (n != 0 && !finer_holds_range(k, L, r, 8)) {
+          big = n > 8;
+          work = !big;
+        }
       }
+      wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.ntris, t);
+      if (work) mark_range(k, bits, T, L, r, 0, 1);
     }
-    wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.ntris, t);
-    if (work) mark_range(k, T, L, r, 0, 1);
   }
+  if (bits) bits_flush(k);
 }
 
 // One wavefront per listed (triangle, level); entries of all levels are taken in level order.
@@ -187,6 +237,9 @@ __device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hd
 __global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGMARK_L + L];
+  if (total == 0) return;  // uniform across the grid
+  uint32_t* bits = k.lds_marks ? lds_bits : nullptr;
+  if (bits) bits_clear(k);
   uint32_t wave = (blockIdx.x * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
   uint32_t waves = (gridDim.x * BLK) >> 6;
   for (uint32_t e = wave; e < total; e += waves) {
@@ -195,15 +248,16 @@ __global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) {
     big_entry(k, e, H_BIGMARK_L, L, idx);
     uint32_t t = k.big_mark[(size_t)L * k.ntris + idx];
     Tri T = load_tri(k, t);
-    mark_range(k, T, L, vox_range(k, T, L, vox_size(k, L) * 8.0f), (int)lane, 64);
+    mark_range(k, bits, T, L, vox_range(k, T, L, true), (int)lane, 64);
   }
+  if (bits) bits_flush(k);
 }
 
 __global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) {
   uint32_t i = blockIdx.x * BLK + threadIdx.x;
   uint32_t per = (uint32_t)k.per;
   if (i >= (uint32_t)(k.NL - 1) * per) return;
-  if (__hip_atomic_load(&k.ptr[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != PTR_MARK) return;
+  if (k.ptr[i] != PTR_MARK) return;
   int L = (int)(i / per);
   uint32_t r = i % per, G = (uint32_t)k.G;
   i3 w = origin(k, L) + i3{(int)(r % G), (int)((r / G) % G), (int)(r / (G * G))};
@@ -211,72 +265,89 @@ __global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) {
     w = shr3(w, 1);
     i3 b = w - origin(k, L);
     if (!inside3(b, k.G)) return;
-    mark_brick(k, ptr_index(k, L, b));
+    uint32_t j = ptr_index(k, L, b);
+    if (k.ptr[j] == PTR_EMPTY) k.ptr[j] = PTR_MARK;
   }
 }
 
-// Block-wide exclusive scan of one value per thread (1024 threads); returns the exclusive prefix, *total the sum.
-__device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t* sh, uint32_t* total) {
-  uint32_t tid = threadIdx.x;
-  uint32_t lane = tid & 63u, wv = tid >> 6;
-  // Inclusive scan within the wavefront.
+// Block-wide exclusive sum of one value per thread (1024 threads), thread order; *total gets the sum. Every thread
+// must call it.
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t* wsum, uint32_t* total) {
+  uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   uint32_t v = x;
   for (int o = 1; o < 64; o <<= 1) {
     uint32_t y = __shfl_up(v, o);
     if (lane >= (uint32_t)o) v += y;
   }
-  if (lane == 63u) sh[wv] = v;
+  if (lane == 63u) wsum[wv] = v;
   __syncthreads();
-  if (tid == 0) {
-    uint32_t a = 0;
-    for (uint32_t q = 0; q < blockDim.x / 64u; q++) {
-      uint32_t s = sh[q];
-      sh[q] = a;
-      a += s;
+  if (threadIdx.x < 64) {
+    uint32_t nw = blockDim.x >> 6;
+    uint32_t s = threadIdx.x < nw ? wsum[threadIdx.x] : 0u, u = s;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(u, o);
+      if (threadIdx.x >= (uint32_t)o) u += y;
     }
-    sh[16] = a;
+    if (threadIdx.x < nw) wsum[threadIdx.x] = u - s;
+    if (threadIdx.x == nw - 1) wsum[16] = u;
   }
   __syncthreads();
-  uint32_t ex = sh[wv] + v - x;
-  *total = sh[16];
+  uint32_t r = wsum[wv] + v - x;
+  *total = wsum[16];
   __syncthreads();
-  return ex;
+  return r;
 }
 
 // Ordered brick allocation: order o = 0.. walks level NL-1 first, pointer index ascending within a level
 // (scanMarks, src/core/voxel.ts:428-448); ranks past max_bricks are dropped (allocBricks, voxel.ts:450-470).
+// Each thread owns 32 consecutive orders (one level, since G^3 is a multiple of 64), read with 8 vector loads in
+// flight; one block scan of the per-thread counts gives every rank. Chunks of 32768 orders.
 __global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) {
-  __shared__ uint32_t sh[32];
+  __shared__ uint32_t wsum[32];
   __shared__ uint32_t lvl[RTGI_MAX_LEVELS];
-  uint32_t per = (uint32_t)k.per, NL = (uint32_t)k.NL, total = NL * per;
-  uint32_t chunk = (total + blockDim.x - 1) / blockDim.x;
-  uint32_t o0 = threadIdx.x * chunk, o1 = min(o0 + chunk, total);
   if (threadIdx.x < RTGI_MAX_LEVELS) lvl[threadIdx.x] = 0u;
-  uint32_t cnt = 0;
-  for (uint32_t o = o0; o < o1; o++) {
-    uint32_t i = (NL - 1u - o / per) * per + o % per;
-    cnt += k.ptr[i] == PTR_MARK ? 1u : 0u;
-  }
-  uint32_t all;
-  uint32_t rank = block_scan(cnt, sh, &all);
-  uint32_t drops = 0;
-  uint32_t G = (uint32_t)k.G;
-  for (uint32_t o = o0; o < o1; o++) {
-    uint32_t L = NL - 1u - o / per, r = o % per, i = L * per + r;
-    if (k.ptr[i] != PTR_MARK) continue;
-    if (rank >= (uint32_t)k.max_bricks) {
-      k.ptr[i] = PTR_DROP;
-      drops++;
-    } else {
-      k.ptr[i] = rank;
-      k.bcode[rank] = (L << 27) | ((r % G) << 18) | (((r / G) % G) << 9) | (r / (G * G));
-      atomicAdd(&lvl[L], 1u);
+  __syncthreads();
+  uint32_t per = (uint32_t)k.per, NL = (uint32_t)k.NL, total = NL * per, G = (uint32_t)k.G;
+  uint32_t base = 0, drops = 0;
+  for (uint32_t c0 = 0; c0 < total; c0 += 32u * blockDim.x) {
+    uint32_t o0 = c0 + 32u * threadIdx.x;
+    uint32_t mask = 0, L = 0, r0 = 0;
+    if (o0 < total) {
+      L = NL - 1u - o0 / per;
+      r0 = o0 % per;
+      const uint4* src = reinterpret_cast<const uint4*>(k.ptr + (size_t)L * per + r0);
+      uint4 q[8];
+      for (int j = 0; j < 8; j++) q[j] = src[j];
+      for (int j = 0; j < 8; j++) {
+        mask |= (q[j].x == PTR_MARK ? 1u : 0u) << (4 * j);
+        mask |= (q[j].y == PTR_MARK ? 1u : 0u) << (4 * j + 1);
+        mask |= (q[j].z == PTR_MARK ? 1u : 0u) << (4 * j + 2);
+        mask |= (q[j].w == PTR_MARK ? 1u : 0u) << (4 * j + 3);
+      }
     }
-    rank++;
+    uint32_t cnt = (uint32_t)__popc(mask), round;
+    uint32_t rank = base + block_excl_sum(cnt, wsum, &round);
+    uint32_t alloc = 0;
+    while (mask) {
+      uint32_t j = (uint32_t)__builtin_ctz(mask);
+      mask &= mask - 1u;
+      uint32_t r = r0 + j, i = L * per + r;
+      if (rank >= (uint32_t)k.max_bricks) {
+        k.ptr[i] = PTR_DROP;
+        drops++;
+      } else {
+        k.ptr[i] = rank;
+        k.bcode[rank] = (L << 27) | ((r % G) << 18) | (((r / G) % G) << 9) | (r / (G * G));
+        alloc++;
+      }
+      rank++;
+    }
+    if (alloc) atomicAdd(&lvl[L], alloc);
+    base += round;
   }
   if (drops) atomicAdd(&k.hdr[H_BRICK_OVF], drops);
   __syncthreads();
-  if (threadIdx.x == 0) k.hdr[H_MARKED] = all;
+  if (threadIdx.x == 0) k.hdr[H_MARKED] = base;
   if (threadIdx.x < NL) k.hdr[H_BRICKS_L + threadIdx.x] = lvl[threadIdx.x];
 }
 
@@ -291,47 +362,101 @@ __global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) {
   }
 }
 
-// voxelizeOne (src/core/voxel.ts:476-490), alpha test omitted: the materials of this path are untextured and opaque.
-__device__ __forceinline__ void voxelize_one(const VoxK& k, const Tri& T, uint32_t t, int L, i3 c, bool single,
-                                             uint32_t drops) {
-  if (finer_built(k, L, c, drops)) return;
-  float v = vox_size(k, L);
-  f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v;
-  if (!single && !tri_box(p, splat3(0.5f * v * 1.0001f), T.a, T.b, T.c)) return;
-  uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
-  if (ptr >= PTR_MARK) return;
-  uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
-  uint32_t* wp = &k.occ[ptr * 16u + (bit >> 5)];
-  uint32_t m = 1u << (bit & 31u);
-  if ((__hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) atomicOr(wp, m);
-  uint32_t* rp = &k.rep[ptr * 512u + bit];
-  if (__hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t + 1u) atomicMax(rp, t + 1u);
+// Voxel updates of a workgroup are first merged in an LDS hash table keyed by (brick slot, bit): each unique voxel
+// then costs one global atomicMax of its representative and one atomicOr of its bit, however many of the
+// workgroup's triangles touch it (consecutive triangles are spatially coherent; on coarse levels dozens share a
+// voxel). A full table falls back to direct global atomics for the update that found no slot.
+constexpr uint32_t VH_SIZE = 2048;  // entries (16 KiB of LDS)
+constexpr int VH_PROBES = 16;
+
+__device__ __forceinline__ void voxel_update(const VoxK& k, uint32_t* hkey, uint32_t* hval, uint32_t ptr, uint32_t bit,
+                                             uint32_t t) {
+  uint32_t key = ptr * 512u + bit + 1u;  // 0 marks an empty slot
+  uint32_t h = (key * 2654435761u) >> (32 - 11);
+  for (int p = 0; p < VH_PROBES; p++) {
+    uint32_t cur = atomicCAS(&hkey[h], 0u, key);
+    if (cur == 0u || cur == key) {
+      atomicMax(&hval[h], t + 1u);
+      return;
+    }
+    h = (h + 1u) & (VH_SIZE - 1u);
+  }
+  atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
+  atomicMax(&k.rep[(size_t)ptr * 512u + bit], t + 1u);
 }
 
-__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
-  uint32_t t = blockIdx.x * BLK + threadIdx.x;
-  bool valid = t < k.ntris;
-  uint32_t drops = k.hdr[H_BRICK_OVF];
-  Tri T{};
-  if (valid) T = load_tri(k, t);
-  for (int L = 0; L < k.NL; L++) {
-    bool work = false, big = false;
-    Range r{};
-    int n = 0;
-    if (valid) {
-      r = vox_range(k, T, L, vox_size(k, L));
-      n = r.n.x * r.n.y * r.n.z;
-      if (n != 0 && !(drops == 0u && finer_holds_range(k, L, r, 1))) {
-        big = n > 8;
-        work = !big;
-      }
-    }
-    wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.ntris, t);
-    if (work)
-      for (int q = 0; q < n; q++) voxelize_one(k, T, t, L, cell_of(r, q), n == 1, drops);
+__device__ __forceinline__ void voxel_table_flush(const VoxK& k, uint32_t* hkey, uint32_t* hval) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < VH_SIZE; i += blockDim.x) {
+    uint32_t key = hkey[i];
+    if (key == 0u) continue;
+    uint32_t v = key - 1u, ptr = v >> 9, bit = v & 511u;
+    atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
+    atomicMax(&k.rep[(size_t)ptr * 512u + bit], hval[i]);
   }
 }
 
+// voxelizeOne (src/core/voxel.ts:476-490) over the up to 8 cells of one small (triangle, level) range; alpha test
+// omitted (the materials of this path are untextured and opaque). The brick pointer is read once per brick.
+__device__ void voxelize_range(const VoxK& k, uint32_t* hkey, uint32_t* hval, const Tri& T, uint32_t t, int L,
+                               const Range& r, int n, uint32_t drops) {
+  float v = vox_size(k, L);
+  f3 lo = region_lo(k, L);
+  f3 h = splat3(0.5f * v * 1.0001f);
+  const uint32_t* ptrL = k.ptr + (size_t)L * (uint32_t)k.per;
+  uint32_t lastb = 0xffffffffu, ptr = PTR_EMPTY;
+  for (int q = 0; q < n; q++) {
+    i3 c = cell_of(r, q);
+    if (finer_built(k, L, c, drops)) continue;
+    f3 p = lo + (tof3(c) + splat3(0.5f)) * v;
+    if (n != 1 && !tri_box(p, h, T.a, T.b, T.c)) continue;
+    i3 b = shr3(c, 3);
+    uint32_t bi = ((uint32_t)b.z * (uint32_t)k.G + (uint32_t)b.y) * (uint32_t)k.G + (uint32_t)b.x;
+    if (bi != lastb) {
+      ptr = ptrL[bi];
+      lastb = bi;
+    }
+    if (ptr >= PTR_MARK) continue;
+    voxel_update(k, hkey, hval, ptr, (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7)), t);
+  }
+}
+
+// voxBits (src/core/voxel.ts:511-513) over VOX_CHUNK consecutive triangles per workgroup.
+constexpr int VOX_CHUNK = 4;
+__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
+  __shared__ uint32_t hkey[VH_SIZE];
+  __shared__ uint32_t hval[VH_SIZE];
+  for (uint32_t i = threadIdx.x; i < VH_SIZE; i += BLK) {
+    hkey[i] = 0u;
+    hval[i] = 0u;
+  }
+  __syncthreads();
+  uint32_t drops = k.hdr[H_BRICK_OVF];
+  for (int sub = 0; sub < VOX_CHUNK; sub++) {
+    uint32_t t = (blockIdx.x * VOX_CHUNK + sub) * BLK + threadIdx.x;
+    bool valid = t < k.ntris;
+    Tri T{};
+    if (valid) T = load_tri(k, t);
+    for (int L = 0; L < k.NL; L++) {
+      bool work = false, big = false;
+      Range r{};
+      int n = 0;
+      if (valid) {
+        r = vox_range(k, T, L, false);
+        n = r.n.x * r.n.y * r.n.z;
+        if (n != 0 && !(drops == 0u && finer_holds_range(k, L, r, 1))) {
+          big = n > 8;
+          work = !big;
+        }
+      }
+      wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.ntris, t);
+      if (work) voxelize_range(k, hkey, hval, T, t, L, r, n, drops);
+    }
+  }
+  voxel_table_flush(k, hkey, hval);
+}
+
+// voxelizeBig (src/core/voxel.ts:515-531): one wavefront per listed (triangle, level), one cell per lane.
 __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGVOX_L + L];
@@ -344,89 +469,115 @@ __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) {
     big_entry(k, e, H_BIGVOX_L, L, idx);
     uint32_t t = k.big_vox[(size_t)L * k.ntris + idx];
     Tri T = load_tri(k, t);
-    Range r = vox_range(k, T, L, vox_size(k, L));
+    Range r = vox_range(k, T, L, false);
     int n = r.n.x * r.n.y * r.n.z;
-    for (int q = (int)lane; q < n; q += 64) voxelize_one(k, T, t, L, cell_of(r, q), false, drops);
+    float v = vox_size(k, L);
+    f3 lo = region_lo(k, L);
+    f3 h = splat3(0.5f * v * 1.0001f);
+    for (int q = (int)lane; q < n; q += 64) {
+      i3 c = cell_of(r, q);
+      if (finer_built(k, L, c, drops)) continue;
+      f3 p = lo + (tof3(c) + splat3(0.5f)) * v;
+      if (!tri_box(p, h, T.a, T.b, T.c)) continue;
+      uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
+      if (ptr >= PTR_MARK) continue;
+      uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+      atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
+      atomicMax(&k.rep[(size_t)ptr * 512u + bit], t + 1u);
+    }
   }
 }
 
-// Level L from level L-1, one thread per occupancy word of a finer brick (reduce, src/core/voxel.ts:540-577).
+// Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
+// children's bits are ORed and their representatives maxed. One wave owns two occupancy words (64 consecutive voxels)
+// and each representative has one writer, so there are no atomics. The result equals the reference's scatter form:
+// a child brick is allocated only if its parent is (markParents and the coarse-first order), and a dropped child
+// contributes nothing in either form.
 __global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) {
   uint32_t nb = n_bricks(k);
-  uint32_t n = nb * 16u;
-  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    uint32_t ptr = i >> 4, w = i & 15u;
-    uint32_t code = k.bcode[ptr];
-    int Lc = (int)(code >> 27);
-    if (Lc + 1 != L) continue;
-    uint32_t word = k.occ[ptr * 16u + w];
-    if (word == 0u) continue;
-    i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
-    i3 c0 = b * 8 + i3{0, (int)((w & 1u) * 4u), (int)(w >> 1)};
-    i3 pc0 = shr3(origin(k, Lc) * 8 + c0, 1) - origin(k, L) * 8;
-    uint32_t pptr = k.ptr[ptr_index(k, L, shr3(pc0, 3))];
-    if (pptr >= PTR_MARK) continue;
-    uint32_t pbit0 = (uint32_t)((pc0.x & 7) + 8 * (pc0.y & 7) + 64 * (pc0.z & 7));
-    uint32_t mask = 0;
-    uint32_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
-    while (word != 0u) {
-      uint32_t kb = (uint32_t)__builtin_ctz(word);
-      word &= word - 1u;
-      uint32_t px = (kb & 7u) >> 1, py = kb >> 4;
-      mask |= 1u << ((pbit0 + px + 8u * py) & 31u);
-      uint32_t rv = k.rep[ptr * 512u + w * 32u + kb];
-      if (py == 0u)
-        lo[px] = max(lo[px], rv);
-      else
-        hi[px] = max(hi[px], rv);
-    }
-    uint32_t* wp = &k.occ[pptr * 16u + (pbit0 >> 5)];
-    if ((__hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & mask) != mask) atomicOr(wp, mask);
-    for (uint32_t px = 0; px < 4u; px++)
-      for (uint32_t py = 0; py < 2u; py++) {
-        uint32_t rv = py == 0u ? lo[px] : hi[px];
-        if (rv == 0u) continue;
-        uint32_t* rp = &k.rep[pptr * 512u + pbit0 + px + 8u * py];
-        if (__hip_atomic_load(rp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < rv) atomicMax(rp, rv);
+  uint32_t n = nb * 512u;
+  const i3 oL = origin(k, L), oF = origin(k, L - 1);
+  for (uint32_t base = blockIdx.x * BLK; base < n; base += gridDim.x * BLK) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t ptr = i >> 9, bit = i & 511u;
+    bool any = false;
+    uint32_t rv = 0;
+    bool mine = false;  // this wave's brick is a level-L brick (uniform per wave: 64 voxels of one brick)
+    if (i < n) {
+      uint32_t code = k.bcode[ptr];
+      mine = (int)(code >> 27) == L;
+      if (mine) {
+        i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+        i3 pc = b * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+        i3 f0 = (oL * 8 + pc) * 2 - oF * 8;
+        if (inside3(f0, k.E)) {  // the window corners are even cells, so all 8 children are inside together
+          uint32_t cptr = k.ptr[ptr_index(k, L - 1, shr3(f0, 3))];  // the 8 children share one finer brick
+          if (cptr < PTR_MARK) {
+            const uint32_t* co = k.occ + (size_t)cptr * 16u;
+            const uint32_t* cr = k.rep + (size_t)cptr * 512u;
+            for (int q = 0; q < 8; q++) {
+              i3 f = f0 + i3{q & 1, (q >> 1) & 1, (q >> 2) & 1};
+              uint32_t cb = (uint32_t)((f.x & 7) + 8 * (f.y & 7) + 64 * (f.z & 7));
+              if (co[cb >> 5] & (1u << (cb & 31u))) {
+                any = true;
+                rv = max(rv, cr[cb]);
+              }
+            }
+          }
+        }
       }
+    }
+    uint64_t m = __ballot(any);
+    if (any) {
+      uint32_t* rp = &k.rep[(size_t)ptr * 512u + bit];
+      if (*rp < rv) *rp = rv;
+    }
+    uint32_t lane = threadIdx.x & 63u;
+    if (mine && (lane == 0u || lane == 32u)) {
+      uint32_t word = (uint32_t)(m >> lane);
+      if (word) k.occ[(size_t)ptr * 16u + (bit >> 5)] |= word;
+    }
   }
+}
+
+__device__ __forceinline__ uint32_t brick_pop(const VoxK& k, uint32_t b) {
+  const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
+  uint32_t n = 0;
+  for (int q = 0; q < 4; q++) {
+    uint4 x = o[q];
+    n += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+  }
+  return n;
 }
 
 // Attribute slots by prefix sum over bricks in slot order (allocVoxels, src/core/voxel.ts:579-588, with the atomic
-// counter's order fixed to the slot order).
+// counter's order fixed to the slot order): rounds of 4096 bricks, four consecutive bricks per thread whose 16
+// occupancy loads are in flight together.
 __global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) {
-  __shared__ uint32_t sh[32];
+  __shared__ uint32_t wsum[32];
   uint32_t nb = n_bricks(k);
-  uint32_t chunk = (nb + blockDim.x - 1) / blockDim.x;
-  uint32_t b0 = threadIdx.x * chunk, b1 = min(b0 + chunk, nb);
-  uint32_t cnt = 0;
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
-    for (int q = 0; q < 4; q++) {
-      uint4 x = o[q];
-      cnt += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+  uint32_t first = 0, ovf = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 4u * blockDim.x) {
+    uint32_t b = b0 + 4u * threadIdx.x;
+    uint32_t n[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; j++)
+      if (b + j < nb) n[j] = brick_pop(k, b + j);
+    uint32_t round;
+    uint32_t f = first + block_excl_sum(n[0] + n[1] + n[2] + n[3], wsum, &round);
+    for (int j = 0; j < 4; j++) {
+      if (b + j >= nb) break;
+      if (f + n[j] > (uint32_t)k.max_voxels) {
+        k.battr[b + j] = ATTR_NONE;
+        ovf++;
+      } else {
+        k.battr[b + j] = f;
+      }
+      f += n[j];
     }
-  }
-  uint32_t all;
-  uint32_t first = block_scan(cnt, sh, &all);
-  uint32_t ovf = 0;
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
-    uint32_t n = 0;
-    for (int q = 0; q < 4; q++) {
-      uint4 x = o[q];
-      n += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
-    }
-    if (first + n > (uint32_t)k.max_voxels) {
-      k.battr[b] = ATTR_NONE;
-      ovf++;
-    } else {
-      k.battr[b] = first;
-    }
-    first += n;
+    first += round;
   }
   if (ovf) atomicAdd(&k.hdr[H_VOXEL_OVF], ovf);
-  if (threadIdx.x == 0) k.hdr[H_VOXELS] = all;
+  if (threadIdx.x == 0) k.hdr[H_VOXELS] = first;
 }
 
 // Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
@@ -520,16 +671,24 @@ int grid_for(uint64_t n) {
 
 }  // namespace
 
+// LDS bytes of the mark bitmap (0: the table is too large and marks go to global memory).
+size_t vox_mark_lds_bytes(const VoxK& k) {
+  size_t bytes = (((size_t)k.NL * k.per + 31) / 32) * 4;
+  return bytes <= 64 * 1024 ? bytes : 0;
+}
+
 int launch_vox_build(const VoxK& k, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const int persist = 2048;  // 256 CUs x 8 blocks of 256 threads for the grid-stride stages
+  const int persist = 2048;  // 256 CUs x 8 workgroups of 256 threads for the grid-stride stages
+  size_t lds = k.lds_marks ? vox_mark_lds_bytes(k) : 0;
+  int mark_grid = std::min(1024, grid_for(k.ntris));
   k_clear<<<grid_for((uint64_t)std::max(k.NL * k.per, (int)HDR_WORDS)), BLK, 0, s>>>(k);
-  if (k.ntris) k_mark_small<<<grid_for(k.ntris), BLK, 0, s>>>(k);
-  k_mark_big<<<persist, BLK, 0, s>>>(k);
+  if (k.ntris) k_mark<<<mark_grid, BLK, lds, s>>>(k);
+  k_mark_big<<<std::min(persist, 512), BLK, lds, s>>>(k);
   if (k.NL > 1) k_mark_parents<<<grid_for((uint64_t)(k.NL - 1) * k.per), BLK, 0, s>>>(k);
   k_alloc_bricks<<<1, 1024, 0, s>>>(k);
   k_init_bricks<<<persist, BLK, 0, s>>>(k);
-  if (k.ntris) k_vox_small<<<grid_for(k.ntris), BLK, 0, s>>>(k);
+  if (k.ntris) k_vox_small<<<grid_for((k.ntris + VOX_CHUNK - 1) / VOX_CHUNK), BLK, 0, s>>>(k);
   k_vox_big<<<persist, BLK, 0, s>>>(k);
   for (int L = 1; L < k.NL; L++) k_reduce<<<persist, BLK, 0, s>>>(k, L);
   k_alloc_voxels<<<1, 1024, 0, s>>>(k);

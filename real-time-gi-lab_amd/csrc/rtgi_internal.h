@@ -37,6 +37,8 @@ constexpr uint32_t ATTR_NONE = 0xffffffffu;
 //   plane  [max_voxels]        representative plane offset from the voxel centre
 struct VoxK {
   float v0;
+  float inv_v0;   // 1 / v0, used only when pow2 (then x / v == x * (1/v) exactly)
+  int32_t pow2;   // v0 is a power of two
   int32_t NL, G, E, per;  // per = G^3
   int32_t max_bricks, max_voxels;
   int32_t origin[RTGI_MAX_LEVELS][3];
@@ -55,6 +57,7 @@ struct VoxK {
   const uint32_t* tri;  // 4 words per triangle
   const float* mat;     // 4 floats per material
   uint32_t ntris;
+  int32_t lds_marks;    // brick marks accumulate in an LDS bitmap of levels * G^3 bits (0: global fallback)
 };
 
 struct CascK {
@@ -62,8 +65,21 @@ struct CascK {
   int32_t res;      // directions per axis
   int32_t vox_level;
   float b0, b1;     // interval in units of s0 (f32, as the Layout uniform stores them)
+  uint32_t np;      // probes (n^3)
+  int32_t n_log2;   // log2(n) when n is a power of two, else -1 (index arithmetic by shifts)
+  int32_t res_log2; // log2(res) (res = 4 << min(c, resCap) is always a power of two)
+  uint32_t dir_off; // first entry of the cascade in the direction table
+  uint64_t avg_off; // cascade c >= 1: first entry of its 2x2-direction average (probes x res_{c-1}^2 float4)
   uint64_t ray_off; // first ray of the cascade
   uint64_t rays;
+};
+
+// Direction table entry of an octahedral texel: the normalised direction (dirOf, src/core/rc.ts:308) and the
+// reciprocal of its slab-safe components (traceVox, src/core/voxel.ts:128-130), computed once on the host with the
+// same f32 operations the reference evaluates per ray.
+struct DirEntry {
+  float d[4];
+  float inv[4];
 };
 
 struct TraceK {
@@ -78,14 +94,21 @@ struct TraceK {
   float point_pos[RTGI_MAX_POINT_LIGHTS][3];
   float point_i[RTGI_MAX_POINT_LIGHTS][3];
   float omega0[16];
-  float* rad;       // 4 floats per ray
-  uint32_t* hits;   // per ray, or null
+  const DirEntry* dirs;  // all cascades' direction tables
+  float* rad;       // 4 floats per ray; within a cascade direction-major: ray = di * np + probe
+  float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
+  uint32_t* hits;   // per ray (same order), or null
   float* sh;        // [27][probes0^3]
   uint64_t total_rays;
 };
 
 // Launchers (each returns a hipError_t as int).
+size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
 int launch_trace(const TraceK& k, void* stream);
 int launch_merge(const TraceK& k, int c, void* stream);
 int launch_sh(const TraceK& k, void* stream);
+
+
+// Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).
+void dir_entry(int di, int res, DirEntry* out);

@@ -1,15 +1,22 @@
 // Radiance-cascade probe-interval trace, cascade merge and SH projection on gfx950.
 //
-//   k_trace   one thread per (cascade, probe, direction) ray over all cascades in a single launch: octahedral texel
-//             direction, interval [b_c, b_{c+1}] s0, voxel level from the cone footprint, hierarchical DDA through
-//             the clipmap (traceVox, src/core/voxel.ts:121-202), Lambert hit shading (shadeHitV, src/core/rc.ts:108-116)
-//             -> rgba f32 (radiance, transmittance) and optionally the first-hit voxel id.
-//   k_merge   one launch per cascade, N-1 down to 0: rad_c += T_c * mean over the grid parents of the upper 2x2
-//             direction average (mergeWorld/upperAvg, src/core/rc.ts:681-708).
-//   k_sh      c0 probes: solid-angle-weighted L2 SH, cosine-convolved (projectSH, src/core/rc.ts:745-774).
+//   k_trace      one thread per (cascade, direction, probe) ray over all cascades in one launch. Rays of a cascade are
+//                ordered direction-major (ray = di * probes + probe), so a wavefront marches 64 neighbouring probes
+//                along one direction: their DDA steps stay in lockstep and their cell reads and rgba writes coalesce.
+//                Per ray: interval [b_c, b_{c+1}] s0, voxel level from the cone footprint, hierarchical DDA through the
+//                clipmap (traceVox, src/core/voxel.ts:121-202), Lambert hit shading (shadeHitV, src/core/rc.ts:108-116)
+//                -> rgba f32 (radiance, transmittance), optionally the first-hit voxel id.
+//   k_merge      one launch per cascade N-1 .. 0: rad_c += T_c * mean over the grid parents of the upper 2x2
+//                direction average (mergeWorld/upperAvg, src/core/rc.ts:681-708). Each thread of a cascade c >= 1
+//                merges one 2x2 block of directions of one probe and also writes the block's average (upperAvg of
+//                the merged values) once, so the next cascade's merge reads 8 parent values per ray, not 32.
+//   k_sh         c0 probes: solid-angle-weighted L2 SH, cosine-convolved (projectSH, src/core/rc.ts:745-774).
+// Shading is deferred out of the DDA loop: a lane that hits leaves the loop with its voxel, and the wavefront shades
+// all its hits once after the loop instead of once per loop iteration in which some lane hit.
 //
 // The DDA restates the reference's f32 operation sequence exactly (compiled with -ffp-contract=off; IEEE division
-// and square root), so first-hit voxels are bit-identical to a CPU restatement of the same WGSL.
+// and square root; x / v becomes x * (1/v) only when v is a power of two, where both are the same value), so
+// first-hit voxels are bit-identical to a CPU restatement of the same WGSL.
 #include <hip/hip_runtime.h>
 
 #include "rtgi_internal.h"
@@ -111,38 +118,58 @@ __device__ __forceinline__ uint32_t hit_id(const VoxK& k, int L, i3 c) {
   return ((uint32_t)L << 29) | (uint32_t)(c.x + k.E * (c.y + k.E * c.z));
 }
 
-// traceVox (src/core/voxel.ts:121-202): hierarchical DDA over level L; empty bricks are skipped whole.
-__device__ float4 trace_vox(const TraceK& a, f3 o, f3 d, float t0, float t1, float tSkip, int L, uint32_t& hid) {
+// floor((p - lo) / v) per component; a multiplication by 1/v when v is a power of two (the same value).
+__device__ __forceinline__ i3 cell_at(f3 q, float v, float iv, bool pow2) {
+  return pow2 ? f2i3(floor3(q * iv)) : f2i3(floor3(q / v));
+}
+
+struct Hit {
+  bool hit;
+  uint32_t brick, bit;  // occupied voxel (brick slot, bit)
+  float tIn;            // entry distance
+  uint32_t id;          // level << 29 | window cell
+};
+
+// traceVox (src/core/voxel.ts:121-202): hierarchical DDA over level L; empty bricks are skipped whole. d and inv come
+// from the direction table (dd = d with |d| < 1e-9 components set to 1e-9, inv = 1 / dd). Returns the first occupied
+// voxel entered at tIn >= tSkip (shading happens after the loop).
+__device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1, float tSkip, int L) {
   const VoxK& k = a.vx;
-  hid = RTGI_NO_HIT;
+  Hit res{false, 0u, 0u, 0.0f, RTGI_NO_HIT};
+  const bool p2 = k.pow2 != 0;
   float v = vox_size(k, L);
+  float iv = k.inv_v0 * (1.0f / (float)(1u << L));
   int G = k.G;
   f3 lo = tof3(origin(k, L)) * v * 8.0f;
   f3 hi = lo + splat3((float)G * v * 8.0f);
-  f3 dd = mk3(fabsf(d.x) < 1e-9f ? 1e-9f : d.x, fabsf(d.y) < 1e-9f ? 1e-9f : d.y, fabsf(d.z) < 1e-9f ? 1e-9f : d.z);
-  f3 inv = mk3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
   f3 ta = (lo - o) * inv, tb = (hi - o) * inv;
   f3 tn = min3(ta, tb), tf = max3(ta, tb);
   float t = fmaxf(fmaxf(fmaxf(tn.x, tn.y), tn.z), t0);
   float tEnd = fminf(fminf(fminf(tf.x, tf.y), tf.z), t1);
-  if (t >= tEnd) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-  i3 st{dd.x > 0.0f ? 1 : -1, dd.y > 0.0f ? 1 : -1, dd.z > 0.0f ? 1 : -1};
+  if (t >= tEnd) return res;
+  // sign(dd): inv has the sign of dd.
+  i3 st{inv.x > 0.0f ? 1 : -1, inv.y > 0.0f ? 1 : -1, inv.z > 0.0f ? 1 : -1};
   f3 sp = mk3(st.x > 0 ? 1.0f : 0.0f, st.y > 0 ? 1.0f : 0.0f, st.z > 0 ? 1.0f : 0.0f);
   int nVox = G * 8;
   i3 zero{0, 0, 0}, top{nVox - 1, nVox - 1, nVox - 1};
-  i3 c = clampi3(f2i3(floor3((o + d * t - lo) / v)), zero, top);
+  i3 c = clampi3(cell_at(o + d * t - lo, v, iv, p2), zero, top);
   f3 tMax = ((tof3(c) + sp) * v + lo - o) * inv;
   f3 tDelta = abs3(inv * v);
   float tIn = t;
+  const uint32_t* ptrL = k.ptr + (size_t)L * (uint32_t)k.per;
+  uint32_t lastb = 0xffffffffu, ptr = 0;
   for (int it = 0; it < 1024; it++) {
     i3 b = shr3(c, 3);
-    uint32_t ptr = k.ptr[ptr_index(k, L, b)];
+    uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
+    if (bi != lastb) {  // the pointer of the brick is read once per brick visited
+      ptr = ptrL[bi];
+      lastb = bi;
+    }
     if (ptr == PTR_DROP) {
       VoxHit h;
       if (vox_at(k, L, c, h) && tIn >= tSkip) {
-        hid = hit_id(k, h.level, h.cell);
-        f3 rgb = shade_voxel(a, vox_rank(k, h.brick, h.bit), d, o + d * tIn);
-        return make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
+        res = Hit{true, h.brick, h.bit, tIn, hit_id(k, h.level, h.cell)};
+        return res;
       }
       float tx = fminf(fminf(tMax.x, tMax.y), tMax.z);
       if (tx >= tEnd) break;
@@ -174,7 +201,7 @@ __device__ float4 trace_vox(const TraceK& a, f3 o, f3 d, float t0, float t1, flo
       if (!inside3(nb, G)) break;
       t = te;
       tIn = te;
-      c = clampi3(f2i3(floor3((o + d * t - lo) / v)), nb * 8, nb * 8 + 7);
+      c = clampi3(cell_at(o + d * t - lo, v, iv, p2), nb * 8, nb * 8 + 7);
       tMax = ((tof3(c) + sp) * v + lo - o) * inv;
       continue;
     }
@@ -182,9 +209,8 @@ __device__ float4 trace_vox(const TraceK& a, f3 o, f3 d, float t0, float t1, flo
     uint32_t occ = k.occ[ptr * 16u + (bit >> 5)];
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
-      hid = hit_id(k, L, c);
-      f3 rgb = shade_voxel(a, vox_rank(k, ptr, bit), d, o + d * tIn);
-      return make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
+      res = Hit{true, ptr, bit, tIn, hit_id(k, L, c)};
+      return res;
     }
     if (tExit >= tEnd) break;
     tIn = tExit;
@@ -200,18 +226,7 @@ __device__ float4 trace_vox(const TraceK& a, f3 o, f3 d, float t0, float t1, flo
     }
     if (!inside3(c, nVox)) break;
   }
-  return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-}
-
-// octDecode of a texel centre (dirOf, src/core/rc.ts:291-298, 308).
-__device__ __forceinline__ f3 dir_of(int di, int res) {
-  float u = ((float)(di % res) + 0.5f) / (float)res, w = ((float)(di / res) + 0.5f) / (float)res;
-  float fx = u * 2.0f - 1.0f, fy = w * 2.0f - 1.0f;
-  f3 n = mk3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
-  float t = fmaxf(-n.z, 0.0f);
-  n.x += n.x >= 0.0f ? -t : t;
-  n.y += n.y >= 0.0f ? -t : t;
-  return normalize3(n);
+  return res;
 }
 
 // voxInside / voxLevelFor (src/core/voxel.ts:108-119).
@@ -228,106 +243,179 @@ __device__ __forceinline__ int vox_level_for(const VoxK& k, int L, f3 e) {
   return l;
 }
 
+__device__ __forceinline__ int cascade_of(const TraceK& a, uint64_t ray) {
+  int c = 0;
+  while (c + 1 < a.N && ray >= a.c[c + 1].ray_off) c++;
+  return c;
+}
+
+// Probe grid coordinates of probe index pi (x fastest) of a cascade.
+__device__ __forceinline__ i3 probe_xyz(const CascK& C, uint32_t pi) {
+  if (C.n_log2 >= 0) {
+    uint32_t m = (uint32_t)C.n - 1u;
+    return i3{(int)(pi & m), (int)((pi >> C.n_log2) & m), (int)(pi >> (2 * C.n_log2))};
+  }
+  uint32_t n = (uint32_t)C.n;
+  return i3{(int)(pi % n), (int)((pi / n) % n), (int)(pi / (n * n))};
+}
+// (direction, probe) of ray r of a cascade (direction-major).
+__device__ __forceinline__ void split_ray(const CascK& C, uint32_t r, uint32_t& di, uint32_t& pi) {
+  if (C.n_log2 >= 0) {
+    di = r >> (3 * C.n_log2);
+    pi = r & (C.np - 1u);
+  } else {
+    di = r / C.np;
+    pi = r % C.np;
+  }
+}
+
 __global__ void __launch_bounds__(BLK) k_trace(TraceK a) {
   uint64_t ray = (uint64_t)blockIdx.x * BLK + threadIdx.x;
   if (ray >= a.total_rays) return;
-  int c = 0;
-  while (c + 1 < a.N && ray >= a.c[c + 1].ray_off) c++;
+  int c = cascade_of(a, ray);
   const CascK& C = a.c[c];
-  uint32_t r = (uint32_t)(ray - C.ray_off);
-  uint32_t dirs = (uint32_t)(C.res * C.res);
-  uint32_t pi = r / dirs, di = r % dirs;
-  uint32_t n = (uint32_t)C.n;
-  i3 g{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c};
-  g = g + i3{(int)(pi % n), (int)((pi / n) % n), (int)(pi / (n * n))};
+  uint32_t di, pi;
+  split_ray(C, (uint32_t)(ray - C.ray_off), di, pi);
+  i3 g = i3{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c} + probe_xyz(C, pi);
   float s = a.s0 * (float)(1u << c);
   f3 o = tof3(g) * s;
-  f3 d = dir_of((int)di, C.res);
+  const DirEntry& de = a.dirs[C.dir_off + di];
+  f3 d = mk3(de.d[0], de.d[1], de.d[2]);
+  f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
   float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
   int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
   float v = vox_size(a.vx, L);
-  uint32_t hid;
-  float4 out = trace_vox(a, o, d, t0, t1, a.vox_skip * v, L, hid);
-  reinterpret_cast<float4*>(a.rad)[ray] = out;
-  if (a.hits) a.hits[ray] = hid;
-}
-
-// upperAvg (src/core/rc.ts:681-686) over the f32 cascade.
-__device__ __forceinline__ f3 upper_avg(const float4* rad, const CascK& U, uint32_t q, uint32_t u, uint32_t w,
-                                        uint32_t lr) {
-  uint32_t ur = (uint32_t)U.res;
-  uint64_t base0 = U.ray_off + (uint64_t)q * ur * ur;
-  if (ur == lr) {
-    float4 x = rad[base0 + w * ur + u];
-    return mk3(x.x, x.y, x.z);
+  Hit h = trace_vox(a, o, d, inv, t0, t1, a.vox_skip * v, L);
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (h.hit) {
+    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
+    out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
-  uint64_t base = base0 + w * 2u * ur + u * 2u;
-  float4 p0 = rad[base], p1 = rad[base + 1], p2 = rad[base + ur], p3 = rad[base + ur + 1];
-  return (mk3(p0.x, p0.y, p0.z) + mk3(p1.x, p1.y, p1.z) + mk3(p2.x, p2.y, p2.z) + mk3(p3.x, p3.y, p3.z)) * 0.25f;
+  reinterpret_cast<float4*>(a.rad)[ray] = out;
+  if (a.hits) a.hits[ray] = h.id;
 }
 
-// mergeWorld (src/core/rc.ts:688-708) on the dense grid: the parents of probe g are the c+1 grid points
-// (g >> 1) + o, o in {0,1}^3 with o = 0 on even axes (linkParents, rc.ts:540-570), those outside the grid absent.
-__global__ void __launch_bounds__(BLK) k_merge(TraceK a, int c) {
+__device__ __forceinline__ f3 rgb_of(float4 x) { return mk3(x.x, x.y, x.z); }
+
+// Parent mean for one (probe g, direction di) of cascade c < N-1 (mergeWorld, src/core/rc.ts:693-706) on the dense
+// grid: the parents of probe g are the c+1 grid points (g >> 1) + o, o in {0,1}^3 with o = 0 on even axes
+// (linkParents, rc.ts:540-570), those outside the grid absent. avg holds cascade c+1's upperAvg at c's directions.
+__device__ __forceinline__ f3 merge_up(const TraceK& a, int c, i3 g, uint32_t di) {
+  const CascK& U = a.c[c + 1];
+  const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
+  int un = U.n;
+  f3 acc = splat3(0.0f);
+  float ws = 0.0f;
+  for (int kk = 0; kk < 8; kk++) {
+    i3 o{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1};
+    if ((o.x && !(g.x & 1)) || (o.y && !(g.y & 1)) || (o.z && !(g.z & 1))) continue;
+    i3 q = shr3(g, 1) + o;
+    if (q.x >= un || q.y >= un || q.z >= un) continue;
+    uint32_t qi = U.n_log2 >= 0 ? (uint32_t)q.x | ((uint32_t)q.y << U.n_log2) | ((uint32_t)q.z << (2 * U.n_log2))
+                                : (uint32_t)q.x + (uint32_t)un * ((uint32_t)q.y + (uint32_t)un * (uint32_t)q.z);
+    float4 x = A[qi];
+    acc = acc + mk3(x.x, x.y, x.z);
+    ws += 1.0f;
+  }
+  return ws > 0.0f ? acc / fmaxf(ws, 1.0f) : splat3(0.0f);
+}
+
+__device__ __forceinline__ float4 merged(const TraceK& a, int c, i3 g, uint32_t di, float4 v) {
+  f3 up = mk3(a.sky[0], a.sky[1], a.sky[2]);
+  if (c + 1 < a.N && v.w > 0.0f) up = merge_up(a, c, g, di);
+  return make_float4(v.x + v.w * up.x, v.y + v.w * up.y, v.z + v.w * up.z, 1.0f);
+}
+
+// Cascade c >= 1: thread per (direction texel dl of cascade c-1's resolution, probe). It merges the 2x2 block of c's
+// directions under dl (a single texel when the resolution is capped, rc.ts:682) and stores the block's average
+// ((p00 + p10 + p01 + p11) * 0.25, upperAvg's order) for cascade c-1's merge.
+__global__ void __launch_bounds__(BLK) k_merge_up(TraceK a, int c) {
   const CascK& C = a.c[c];
+  const CascK& D = a.c[c - 1];
+  uint32_t ndl = (uint32_t)(D.res * D.res);
+  uint64_t t = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (uint64_t)ndl * C.np) return;
+  uint32_t dl, pi;
+  if (C.n_log2 >= 0) {
+    dl = (uint32_t)(t >> (3 * C.n_log2));
+    pi = (uint32_t)t & (C.np - 1u);
+  } else {
+    dl = (uint32_t)(t / C.np);
+    pi = (uint32_t)(t % C.np);
+  }
+  i3 g = probe_xyz(C, pi);
+  float4* rad = reinterpret_cast<float4*>(a.rad) + C.ray_off + pi;
+  uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
+  f3 avg;
+  if (C.res == D.res) {
+    size_t i = (size_t)dl * C.np;
+    float4 m = merged(a, c, g, dl, rad[i]);
+    rad[i] = m;
+    avg = mk3(m.x, m.y, m.z);
+  } else {
+    uint32_t d00 = ((2u * lw) << C.res_log2) + 2u * lu;
+    uint32_t dd[4] = {d00, d00 + 1u, d00 + (uint32_t)C.res, d00 + (uint32_t)C.res + 1u};
+    f3 m[4];
+    for (int j = 0; j < 4; j++) {
+      size_t i = (size_t)dd[j] * C.np;
+      float4 x = merged(a, c, g, dd[j], rad[i]);
+      rad[i] = x;
+      m[j] = mk3(x.x, x.y, x.z);
+    }
+    avg = (m[0] + m[1] + m[2] + m[3]) * 0.25f;
+  }
+  reinterpret_cast<float4*>(a.avg)[C.avg_off + (size_t)dl * C.np + pi] = make_float4(avg.x, avg.y, avg.z, 0.0f);
+}
+
+// Cascade 0: thread per ray.
+__global__ void __launch_bounds__(BLK) k_merge0(TraceK a) {
+  const CascK& C = a.c[0];
   uint64_t r = (uint64_t)blockIdx.x * BLK + threadIdx.x;
   if (r >= C.rays) return;
+  uint32_t di, pi;
+  split_ray(C, (uint32_t)r, di, pi);
   float4* rad = reinterpret_cast<float4*>(a.rad);
-  float4 v = rad[C.ray_off + r];
-  f3 up = mk3(a.sky[0], a.sky[1], a.sky[2]);
-  if (c + 1 < a.N && v.w > 0.0f) {
-    const CascK& U = a.c[c + 1];
-    uint32_t dirs = (uint32_t)(C.res * C.res);
-    uint32_t pi = (uint32_t)(r / dirs), di = (uint32_t)(r % dirs);
-    uint32_t n = (uint32_t)C.n, un = (uint32_t)U.n;
-    i3 g{(int)(pi % n), (int)((pi / n) % n), (int)(pi / (n * n))};
-    uint32_t u0 = di % (uint32_t)C.res, w0 = di / (uint32_t)C.res;
-    f3 acc = splat3(0.0f);
-    float ws = 0.0f;
-    for (int kk = 0; kk < 8; kk++) {
-      i3 o{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1};
-      if ((o.x && !(g.x & 1)) || (o.y && !(g.y & 1)) || (o.z && !(g.z & 1))) continue;
-      i3 q = shr3(g, 1) + o;
-      if (q.x >= (int)un || q.y >= (int)un || q.z >= (int)un) continue;
-      uint32_t qi = (uint32_t)q.x + un * ((uint32_t)q.y + un * (uint32_t)q.z);
-      acc = acc + upper_avg(rad, U, qi, u0, w0, (uint32_t)C.res);
-      ws += 1.0f;
-    }
-    up = ws > 0.0f ? acc / fmaxf(ws, 1.0f) : splat3(0.0f);
-  }
-  rad[C.ray_off + r] = make_float4(v.x + v.w * up.x, v.y + v.w * up.y, v.z + v.w * up.z, 1.0f);
+  rad[C.ray_off + r] = merged(a, 0, probe_xyz(C, pi), di, rad[C.ray_off + r]);
 }
 
-// projectSH (src/core/rc.ts:745-774): 27 f32 per c0 probe, stored as 27 planes of probes0^3 floats.
-__global__ void __launch_bounds__(BLK) k_sh(TraceK a) {
-  const CascK& C = a.c[0];
-  uint32_t np = (uint32_t)C.n * (uint32_t)C.n * (uint32_t)C.n;
-  uint32_t pi = blockIdx.x * BLK + threadIdx.x;
-  if (pi >= np) return;
-  const float4* rad = reinterpret_cast<const float4*>(a.rad);
-  f3 cf[9];
-  for (int q = 0; q < 9; q++) cf[q] = splat3(0.0f);
-  for (int d = 0; d < 16; d++) {
-    float4 x = rad[(uint64_t)pi * 16u + d];
-    f3 r = mk3(x.x, x.y, x.z) * a.omega0[d];
-    f3 n = dir_of(d, 4);
-    float y[9] = {0.282095f,
-                  0.488603f * n.y,
-                  0.488603f * n.z,
-                  0.488603f * n.x,
-                  1.092548f * n.x * n.y,
-                  1.092548f * n.y * n.z,
-                  0.315392f * (3.0f * n.z * n.z - 1.0f),
-                  1.092548f * n.x * n.z,
-                  0.546274f * (n.x * n.x - n.y * n.y)};
-    for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];
-  }
+// shBasis (src/core/rc.ts:745-749) of c0 texel d, from the direction table.
+__device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) {
+  float nx = de.d[0], ny = de.d[1], nz = de.d[2];
+  y[0] = 0.282095f;
+  y[1] = 0.488603f * ny;
+  y[2] = 0.488603f * nz;
+  y[3] = 0.488603f * nx;
+  y[4] = 1.092548f * nx * ny;
+  y[5] = 1.092548f * ny * nz;
+  y[6] = 0.315392f * (3.0f * nz * nz - 1.0f);
+  y[7] = 1.092548f * nx * nz;
+  y[8] = 0.546274f * (nx * nx - ny * ny);
+}
+
+__device__ __forceinline__ void sh_store(const TraceK& a, uint32_t np, uint32_t pi, const f3 cf[9]) {
   for (int q = 0; q < 9; q++) {
     float s = q == 0 ? SH_A0 : (q < 4 ? SH_A1 : SH_A2);
     a.sh[(size_t)(q * 3 + 0) * np + pi] = cf[q].x * s;
     a.sh[(size_t)(q * 3 + 1) * np + pi] = cf[q].y * s;
     a.sh[(size_t)(q * 3 + 2) * np + pi] = cf[q].z * s;
   }
+}
+
+// projectSH (src/core/rc.ts:752-774) of already merged c0 radiance: 27 f32 per probe in 27 planes.
+__global__ void __launch_bounds__(BLK) k_sh(TraceK a) {
+  const CascK& C = a.c[0];
+  uint32_t pi = blockIdx.x * BLK + threadIdx.x;
+  if (pi >= C.np) return;
+  const float4* rad = reinterpret_cast<const float4*>(a.rad);
+  f3 cf[9];
+  for (int q = 0; q < 9; q++) cf[q] = splat3(0.0f);
+  for (uint32_t d = 0; d < 16; d++) {
+    f3 r = rgb_of(rad[C.ray_off + (size_t)d * C.np + pi]) * a.omega0[d];
+    float y[9];
+    sh_basis(a.dirs[C.dir_off + d], y);
+    for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];
+  }
+  sh_store(a, C.np, pi, cf);
 }
 
 int grid_for(uint64_t n) {
@@ -343,12 +431,16 @@ int launch_trace(const TraceK& k, void* stream) {
 }
 
 int launch_merge(const TraceK& k, int c, void* stream) {
-  if (k.c[c].rays) k_merge<<<grid_for(k.c[c].rays), BLK, 0, (hipStream_t)stream>>>(k, c);
+  if (c > 0) {
+    uint64_t n = (uint64_t)k.c[c - 1].res * k.c[c - 1].res * k.c[c].np;
+    k_merge_up<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+  } else if (k.c[0].rays) {
+    k_merge0<<<grid_for(k.c[0].rays), BLK, 0, (hipStream_t)stream>>>(k);
+  }
   return (int)hipGetLastError();
 }
 
 int launch_sh(const TraceK& k, void* stream) {
-  uint64_t np = (uint64_t)k.c[0].n * k.c[0].n * k.c[0].n;
-  k_sh<<<grid_for(np), BLK, 0, (hipStream_t)stream>>>(k);
+  k_sh<<<grid_for(k.c[0].np), BLK, 0, (hipStream_t)stream>>>(k);
   return (int)hipGetLastError();
 }
