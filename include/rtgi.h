@@ -201,6 +201,25 @@ int rtgi_world_read_sh(rtgi_world* w, float* sh);
 /* Frees the world's device memory (replaces VoxWorld.destroy src/core/voxel.ts:923 and Rc.destroy rc.ts:1234). */
 int rtgi_world_destroy(rtgi_world* w);
 
+/* ---- Batched update of many worlds ----
+ * No reference counterpart: the reference updates one world per frame. A batch runs one GI update of each of its
+ * worlds with one launch per stage (clipmap stages, trace, each merge step, SH) whatever their sizes; the sequence
+ * can be captured once as a hipGraph and replayed. Worlds keep their own buffers and parameters; their frames and
+ * scenes are read at each enqueue. A batch holds pointers to its worlds: destroy it before them. */
+typedef struct rtgi_batch rtgi_batch;
+
+int rtgi_batch_create(rtgi_world* const* worlds, int32_t n, rtgi_batch** out);
+/* Capture the launch sequence as a hipGraph on the next enqueue and replay it (on: 1) or launch directly (0). */
+int rtgi_batch_set_graph(rtgi_batch* b, int32_t on);
+/* One update of every world of the batch on `stream` (NULL: the batch's own stream). Asynchronous; the worlds' read
+ * functions wait for it. */
+int rtgi_batch_enqueue(rtgi_batch* b, void* stream);
+int rtgi_batch_sync(rtgi_batch* b);
+/* Per-stage HIP-event timing of direct (not graph) enqueues; ms as for rtgi_world_stage_ms. */
+int rtgi_batch_set_timing(rtgi_batch* b, int32_t on);
+int rtgi_batch_stage_ms(rtgi_batch* b, float ms[4]);
+int rtgi_batch_destroy(rtgi_batch* b);
+
 /* ---- Seeded procedural scenes (inputs of the benchmark configurations; no reference counterpart beyond the Builder
  * idiom of src/core/scenes.ts:176-211 and the rigid posing of src/core/scenes.ts:80-102). ---- */
 #define RTGI_SCENE_ATRIUM 1 /* 20 x 20 x 10 m atrium: walls, 24 pillars, tessellated arches, 16 moving boxes */

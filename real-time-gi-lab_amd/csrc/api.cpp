@@ -67,7 +67,7 @@ struct rtgi_world {
   bool timing = false;
   hipEvent_t ev[5] = {};
   uint32_t last_stages = 0;
-  uint32_t pending_stages = 0;  // stages of the enqueue being recorded
+  hipEvent_t ext_done = nullptr;  // completion of the last batched update that included this world
 };
 
 namespace {
@@ -210,6 +210,13 @@ void free_all(rtgi_world* w) {
   dfree(w, w->sh, np0 * 27);
   dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
+}
+
+// Waits for the world's own stream and for the last batched update that included it.
+hipError_t world_wait(rtgi_world* w) {
+  hipError_t e = hipStreamSynchronize(w->stream);
+  if (e == hipSuccess && w->ext_done) e = hipEventSynchronize(w->ext_done);
+  return e;
 }
 
 int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
@@ -377,7 +384,7 @@ extern "C" int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const fl
   }
   if ((uint64_t)ntris >= 0xffffffffull) return rtgi_fail(RTGI_E_INVALID, "too many triangles");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   int e = RTGI_OK;
   if (ntris > w->cap_tris) {
     dfree(w, w->tri, (size_t)w->cap_tris * 4);
@@ -425,7 +432,7 @@ extern "C" int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_
   if ((uint64_t)first + count > w->nverts) return rtgi_fail(RTGI_E_INVALID, "vertex range out of bounds");
   if (count && (!pos || !nrm)) return rtgi_fail(RTGI_E_INVALID, "null array");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   if (count) {
     HIPCHK(hipMemcpy(w->pos + (size_t)first * 3, pos, (size_t)count * 12, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(w->nrm + (size_t)first * 3, nrm, (size_t)count * 12, hipMemcpyHostToDevice));
@@ -468,8 +475,9 @@ extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* s
   if (stages & ~RTGI_STAGE_ALL) return rtgi_fail(RTGI_E_INVALID, "unknown stage bits");
   HIPCHK(hipSetDevice(w->device));
   hipStream_t s = stream ? (hipStream_t)stream : w->stream;
+  // A batched update that included this world may still be running.
+  if (w->ext_done) HIPCHK(hipStreamWaitEvent(s, w->ext_done, 0));
   const uint32_t order[4] = {RTGI_STAGE_VOX, RTGI_STAGE_TRACE, RTGI_STAGE_MERGE, RTGI_STAGE_SH};
-  w->pending_stages = stages;
   if (w->timing) HIPCHK(hipEventRecord(w->ev[0], s));
   for (int i = 0; i < 4; i++) {
     if (stages & order[i]) {
@@ -509,13 +517,14 @@ extern "C" int rtgi_world_stage_ms(rtgi_world* w, float ms[4]) {
 extern "C" int rtgi_world_sync(rtgi_world* w) {
   if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   return RTGI_OK;
 }
 
 extern "C" int rtgi_world_read_counters(rtgi_world* w, rtgi_counters* out) {
   if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
   HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
   uint32_t h[HDR_WORDS];
   HIPCHK(hipMemcpyAsync(h, w->hdr, sizeof(h), hipMemcpyDeviceToHost, w->stream));
   HIPCHK(hipStreamSynchronize(w->stream));
@@ -561,7 +570,7 @@ extern "C" int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* o
   if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
   if (level < 0 || level >= w->NL) return rtgi_fail(RTGI_E_INVALID, "level out of range");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   uint32_t h[HDR_WORDS];
   HIPCHK(hipMemcpy(h, w->hdr, sizeof(h), hipMemcpyDeviceToHost));
   uint32_t nb = std::min(h[H_MARKED], (uint32_t)w->vp.max_bricks);
@@ -629,7 +638,7 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   if (c < 0 || c >= w->N) return rtgi_fail(RTGI_E_INVALID, "cascade out of range");
   if (hits && !w->hits) return rtgi_fail(RTGI_E_STATE, "hit ids were not recorded (RTGI_RC_RECORD_HITS)");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   size_t n = w->ray_off[c + 1] - w->ray_off[c];
   size_t np = (size_t)w->n[c] * w->n[c] * w->n[c], dirs = (size_t)w->res[c] * w->res[c];
   // Device order is direction-major (ray = di * np + probe); the readback is probe-major.
@@ -651,7 +660,7 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
 extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
   if (!w || !sh) return rtgi_fail(RTGI_E_INVALID, "null argument");
   HIPCHK(hipSetDevice(w->device));
-  HIPCHK(hipStreamSynchronize(w->stream));
+  HIPCHK(world_wait(w));
   size_t np = (size_t)w->n[0] * w->n[0] * w->n[0];
   std::vector<float> planes(np * 27);
   HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
@@ -669,5 +678,224 @@ extern "C" int rtgi_world_destroy(rtgi_world* w) {
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete w;
+  return RTGI_OK;
+}
+
+// ---- Batches ----------------------------------------------------------------------------------------------------
+
+struct rtgi_batch {
+  int device = 0;
+  std::vector<rtgi_world*> worlds;
+  hipStream_t stream = nullptr;
+  TraceK* d_desc = nullptr;
+  TraceK* h_desc[2] = {nullptr, nullptr};  // pinned staging, double-buffered
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint32_t* d_start = nullptr;
+  BatchPlan plan{};
+  std::vector<uint64_t> sig;  // sizes the plan was made for
+  bool use_graph = false, graph_valid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+};
+
+namespace {
+
+void batch_free(rtgi_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int i = 0; i < 2; i++) {
+    if (b->h_desc[i]) (void)hipHostFree(b->h_desc[i]);
+    if (b->copied[i]) (void)hipEventDestroy(b->copied[i]);
+  }
+  for (int i = 0; i < 5; i++)
+    if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_start) (void)hipFree(b->d_start);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// Block counts per launch and world -> prefix arrays on the device (synchronous upload; only when sizes change).
+int batch_plan(rtgi_batch* b, const std::vector<uint64_t>& sig) {
+  const int n = (int)b->worlds.size();
+  std::vector<uint32_t> start((size_t)BL_COUNT * (n + 1), 0);
+  size_t lds = 0;
+  bool all_lds = true;
+  for (int i = 0; i < n; i++) {
+    TraceK t = trace_args(b->worlds[i]);
+    uint32_t g[BL_COUNT];
+    batch_blocks(t, g);
+    if (b->worlds[i]->rp.flags & RTGI_RC_NO_MERGE) {
+      for (int st = 0; st < RTGI_MAX_CASCADES; st++) g[BL_MERGE + st] = 0;
+      g[BL_SH] = 0;
+    }
+    for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1) + i + 1] = start[(size_t)l * (n + 1) + i] + g[l];
+    size_t m = vox_mark_lds_bytes(t.vx);
+    if (m == 0) all_lds = false;
+    lds = std::max(lds, m);
+  }
+  HIPCHK(hipMemcpy(b->d_start, start.data(), start.size() * 4, hipMemcpyHostToDevice));
+  b->plan.w = b->d_desc;
+  b->plan.n = n;
+  for (int l = 0; l < BL_COUNT; l++) {
+    b->plan.start[l] = b->d_start + (size_t)l * (n + 1);
+    b->plan.grid[l] = start[(size_t)l * (n + 1) + n];
+  }
+  // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
+  // says whether it marks in LDS).
+  b->plan.mark_lds = all_lds ? lds : 0;
+  b->sig = sig;
+  b->graph_valid = false;
+  return RTGI_OK;
+}
+
+int batch_launch(rtgi_batch* b, hipStream_t s, bool timed) {
+  int e = 0;
+  if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
+  e = launch_vox_build_batch(b->plan, s);
+  if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
+  if (!e) e = launch_trace_batch(b->plan, s);
+  if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
+  for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(b->plan, st, s);
+  if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
+  if (!e) e = launch_sh_batch(b->plan, s);
+  if (timed) HIPCHK(hipEventRecord(b->ev[4], s));
+  if (e) return rtgi_fail(RTGI_E_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
+  return RTGI_OK;
+}
+
+}  // namespace
+
+extern "C" int rtgi_batch_create(rtgi_world* const* worlds, int32_t n, rtgi_batch** out) {
+  if (!worlds || !out || n < 1) return rtgi_fail(RTGI_E_INVALID, "rtgi_batch_create: bad argument");
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    if (!worlds[i]) return rtgi_fail(RTGI_E_INVALID, "rtgi_batch_create: null world %d", i);
+    if (worlds[i]->device != worlds[0]->device)
+      return rtgi_fail(RTGI_E_INVALID, "rtgi_batch_create: worlds on different devices");
+    for (int j = 0; j < i; j++)
+      if (worlds[j] == worlds[i]) return rtgi_fail(RTGI_E_INVALID, "rtgi_batch_create: world %d repeated", i);
+  }
+  rtgi_batch* b = new (std::nothrow) rtgi_batch();
+  if (!b) return rtgi_fail(RTGI_E_NOMEM, "out of host memory");
+  b->worlds.assign(worlds, worlds + n);
+  b->device = worlds[0]->device;
+  int e = RTGI_OK;
+  if (hipSetDevice(b->device) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipSetDevice failed");
+  if (!e && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "hipStreamCreate failed");
+  if (!e && hipMalloc((void**)&b->d_desc, sizeof(TraceK) * n) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+  if (!e && hipMalloc((void**)&b->d_start, sizeof(uint32_t) * BL_COUNT * (n + 1)) != hipSuccess)
+    e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+  for (int i = 0; i < 2 && !e; i++) {
+    if (hipHostMalloc((void**)&b->h_desc[i], sizeof(TraceK) * n, hipHostMallocDefault) != hipSuccess)
+      e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");
+    if (!e && hipEventCreateWithFlags(&b->copied[i], hipEventDisableTiming) != hipSuccess)
+      e = rtgi_fail(RTGI_E_HIP, "hipEventCreate");
+  }
+  if (!e && hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "hipEventCreate");
+  for (int i = 0; i < 5 && !e; i++)
+    if (hipEventCreate(&b->ev[i]) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipEventCreate");
+  if (e) {
+    batch_free(b);
+    return e;
+  }
+  *out = b;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_set_graph(rtgi_batch* b, int32_t on) {
+  if (!b) return rtgi_fail(RTGI_E_INVALID, "null batch");
+  b->use_graph = on != 0;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_set_timing(rtgi_batch* b, int32_t on) {
+  if (!b) return rtgi_fail(RTGI_E_INVALID, "null batch");
+  b->timing = on != 0;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
+  if (!b) return rtgi_fail(RTGI_E_INVALID, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+  const int n = (int)b->worlds.size();
+  std::vector<uint64_t> sig;
+  sig.reserve((size_t)n * 4);
+  for (int i = 0; i < n; i++) {
+    rtgi_world* w = b->worlds[i];
+    if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "world %d has no scene or no frame", i);
+    HIPCHK(hipStreamSynchronize(w->stream));  // per-world work (uploads, single updates) is done first
+    sig.push_back(w->ntris);
+    sig.push_back(w->ray_off[w->N]);
+    sig.push_back((uint64_t)w->NL << 32 | (uint32_t)w->per);
+    sig.push_back((uint64_t)w->vp.max_bricks << 32 | w->rp.flags);
+  }
+  if (sig != b->sig) {
+    // The previous update may still read the block prefixes being replaced.
+    HIPCHK(hipStreamSynchronize(s));
+    int e = batch_plan(b, sig);
+    if (e) return e;
+  }
+  // Descriptors: the staging buffer being filled was last copied two enqueues ago; wait for that copy.
+  HIPCHK(hipEventSynchronize(b->copied[b->cur]));
+  for (int i = 0; i < n; i++) b->h_desc[b->cur][i] = trace_args(b->worlds[i]);
+  HIPCHK(hipMemcpyAsync(b->d_desc, b->h_desc[b->cur], sizeof(TraceK) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(b->copied[b->cur], s));
+  b->cur ^= 1;
+  if (b->use_graph) {
+    if (!b->graph_valid) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int e = batch_launch(b, s, false);
+      hipError_t ce = hipStreamEndCapture(s, &b->graph);
+      if (e) return e;
+      if (ce != hipSuccess) return rtgi_fail(RTGI_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+      HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_valid = true;
+    }
+    HIPCHK(hipGraphLaunch(b->exec, s));
+  } else {
+    int e = batch_launch(b, s, b->timing);
+    if (e) return e;
+  }
+  HIPCHK(hipEventRecord(b->done, s));
+  for (rtgi_world* w : b->worlds) w->ext_done = b->done;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_sync(rtgi_batch* b) {
+  if (!b) return rtgi_fail(RTGI_E_INVALID, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipEventSynchronize(b->done));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_stage_ms(rtgi_batch* b, float ms[4]) {
+  if (!b || !ms) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  if (!b->timing || b->use_graph) return rtgi_fail(RTGI_E_STATE, "timing needs direct (non-graph) enqueues");
+  HIPCHK(hipEventSynchronize(b->ev[4]));
+  for (int i = 0; i < 4; i++) HIPCHK(hipEventElapsedTime(&ms[i], b->ev[i], b->ev[i + 1]));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_batch_destroy(rtgi_batch* b) {
+  if (!b) return RTGI_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->done) (void)hipEventSynchronize(b->done);
+  for (rtgi_world* w : b->worlds)
+    if (w->ext_done == b->done) w->ext_done = nullptr;
+  batch_free(b);
   return RTGI_OK;
 }

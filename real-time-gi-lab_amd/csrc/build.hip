@@ -33,6 +33,7 @@
 #include <algorithm>
 
 #include "rtgi_internal.h"
+#include "batch.h"
 #include "vmath.h"
 
 namespace {
@@ -187,18 +188,23 @@ __device__ __forceinline__ void bits_flush(const VoxK& k) {
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_clear(VoxK k) {
-  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+__device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+  uint32_t i = lb * BLK + threadIdx.x;
   if (i < HDR_WORDS) k.hdr[i] = 0u;
   if (i < (uint32_t)(k.NL * k.per)) k.ptr[i] = PTR_EMPTY;
+}
+__global__ void __launch_bounds__(BLK) k_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  clear_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level
 // list append sees converged waves).
-__global__ void __launch_bounds__(BLK) k_mark(VoxK k) {
+__device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t* bits = k.lds_marks ? lds_bits : nullptr;
   if (bits) bits_clear(k);
-  for (uint32_t base = blockIdx.x * BLK; base < k.ntris; base += gridDim.x * BLK) {
+  for (uint32_t base = lb * BLK; base < k.ntris; base += nblk * BLK) {
     uint32_t t = base + threadIdx.x;
     bool valid = t < k.ntris;
     Tri T{};
@@ -220,6 +226,11 @@ __global__ void __launch_bounds__(BLK) k_mark(VoxK k) {
   }
   if (bits) bits_flush(k);
 }
+__global__ void __launch_bounds__(BLK) k_mark(VoxK k) { mark_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_mark_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  mark_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
 // One wavefront per listed (triangle, level); entries of all levels are taken in level order.
 __device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hdr_base, int& L, uint32_t& idx) {
@@ -234,14 +245,14 @@ __device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hd
   return false;
 }
 
-__global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) {
+__device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGMARK_L + L];
   if (total == 0) return;  // uniform across the grid
   uint32_t* bits = k.lds_marks ? lds_bits : nullptr;
   if (bits) bits_clear(k);
-  uint32_t wave = (blockIdx.x * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
-  uint32_t waves = (gridDim.x * BLK) >> 6;
+  uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
+  uint32_t waves = (nblk * BLK) >> 6;
   for (uint32_t e = wave; e < total; e += waves) {
     int L;
     uint32_t idx;
@@ -252,9 +263,14 @@ __global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) {
   }
   if (bits) bits_flush(k);
 }
+__global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) { mark_big_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_mark_big_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  mark_big_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
-__global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) {
-  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+__device__ __forceinline__ void mark_parents_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+  uint32_t i = lb * BLK + threadIdx.x;
   uint32_t per = (uint32_t)k.per;
   if (i >= (uint32_t)(k.NL - 1) * per) return;
   if (k.ptr[i] != PTR_MARK) return;
@@ -268,6 +284,11 @@ __global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) {
     uint32_t j = ptr_index(k, L, b);
     if (k.ptr[j] == PTR_EMPTY) k.ptr[j] = PTR_MARK;
   }
+}
+__global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) { mark_parents_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_mark_parents_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  mark_parents_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Block-wide exclusive sum of one value per thread (1024 threads), thread order; *total gets the sum. Every thread
@@ -302,7 +323,7 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t* wsum, u
 // (scanMarks, src/core/voxel.ts:428-448); ranks past max_bricks are dropped (allocBricks, voxel.ts:450-470).
 // Each thread owns 32 consecutive orders (one level, since G^3 is a multiple of 64), read with 8 vector loads in
 // flight; one block scan of the per-thread counts gives every rank. Chunks of 32768 orders.
-__global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) {
+__device__ __forceinline__ void alloc_bricks_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   __shared__ uint32_t wsum[32];
   __shared__ uint32_t lvl[RTGI_MAX_LEVELS];
   if (threadIdx.x < RTGI_MAX_LEVELS) lvl[threadIdx.x] = 0u;
@@ -350,16 +371,26 @@ __global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) {
   if (threadIdx.x == 0) k.hdr[H_MARKED] = base;
   if (threadIdx.x < NL) k.hdr[H_BRICKS_L + threadIdx.x] = lvl[threadIdx.x];
 }
+__global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) { alloc_bricks_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) k_alloc_bricks_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  alloc_bricks_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
 __device__ __forceinline__ uint32_t n_bricks(const VoxK& k) { return min(k.hdr[H_MARKED], (uint32_t)k.max_bricks); }
 
-__global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) {
+__device__ __forceinline__ void init_bricks_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t nb = n_bricks(k);
   uint32_t n = nb * 512u;
-  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+  for (uint32_t i = lb * BLK + threadIdx.x; i < n; i += nblk * BLK) {
     k.rep[i] = 0u;
     if ((i & 511u) < 16u) k.occ[(i >> 9) * 16u + (i & 511u)] = 0u;
   }
+}
+__global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) { init_bricks_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_init_bricks_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  init_bricks_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Voxel updates of a workgroup are first merged in an LDS hash table keyed by (brick slot, bit): each unique voxel
@@ -423,7 +454,7 @@ __device__ void voxelize_range(const VoxK& k, uint32_t* hkey, uint32_t* hval, co
 
 // voxBits (src/core/voxel.ts:511-513) over VOX_CHUNK consecutive triangles per workgroup.
 constexpr int VOX_CHUNK = 4;
-__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
+__device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   __shared__ uint32_t hkey[VH_SIZE];
   __shared__ uint32_t hval[VH_SIZE];
   for (uint32_t i = threadIdx.x; i < VH_SIZE; i += BLK) {
@@ -433,7 +464,7 @@ __global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
   __syncthreads();
   uint32_t drops = k.hdr[H_BRICK_OVF];
   for (int sub = 0; sub < VOX_CHUNK; sub++) {
-    uint32_t t = (blockIdx.x * VOX_CHUNK + sub) * BLK + threadIdx.x;
+    uint32_t t = (lb * VOX_CHUNK + sub) * BLK + threadIdx.x;
     bool valid = t < k.ntris;
     Tri T{};
     if (valid) T = load_tri(k, t);
@@ -455,14 +486,19 @@ __global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) {
   }
   voxel_table_flush(k, hkey, hval);
 }
+__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) { vox_small_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  vox_small_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
 // voxelizeBig (src/core/voxel.ts:515-531): one wavefront per listed (triangle, level), one cell per lane.
-__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) {
+__device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGVOX_L + L];
   uint32_t drops = k.hdr[H_BRICK_OVF];
-  uint32_t wave = (blockIdx.x * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
-  uint32_t waves = (gridDim.x * BLK) >> 6;
+  uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
+  uint32_t waves = (nblk * BLK) >> 6;
   for (uint32_t e = wave; e < total; e += waves) {
     int L;
     uint32_t idx;
@@ -487,17 +523,22 @@ __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) {
     }
   }
 }
+__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  vox_big_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
 // Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
 // children's bits are ORed and their representatives maxed. One wave owns two occupancy words (64 consecutive voxels)
 // and each representative has one writer, so there are no atomics. The result equals the reference's scatter form:
 // a child brick is allocated only if its parent is (markParents and the coarse-first order), and a dropped child
 // contributes nothing in either form.
-__global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) {
+__device__ __forceinline__ void reduce_body(const VoxK& k, int L, uint32_t lb, uint32_t nblk) {
   uint32_t nb = n_bricks(k);
   uint32_t n = nb * 512u;
   const i3 oL = origin(k, L), oF = origin(k, L - 1);
-  for (uint32_t base = blockIdx.x * BLK; base < n; base += gridDim.x * BLK) {
+  for (uint32_t base = lb * BLK; base < n; base += nblk * BLK) {
     uint32_t i = base + threadIdx.x;
     uint32_t ptr = i >> 9, bit = i & 511u;
     bool any = false;
@@ -539,6 +580,11 @@ __global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) {
     }
   }
 }
+__global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) { reduce_body(k, L, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_reduce_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  reduce_body(b.w[w].vx, b.arg, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
 
 __device__ __forceinline__ uint32_t brick_pop(const VoxK& k, uint32_t b) {
   const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)b * 16u);
@@ -553,7 +599,7 @@ __device__ __forceinline__ uint32_t brick_pop(const VoxK& k, uint32_t b) {
 // Attribute slots by prefix sum over bricks in slot order (allocVoxels, src/core/voxel.ts:579-588, with the atomic
 // counter's order fixed to the slot order): rounds of 4096 bricks, four consecutive bricks per thread whose 16
 // occupancy loads are in flight together.
-__global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) {
+__device__ __forceinline__ void alloc_voxels_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   __shared__ uint32_t wsum[32];
   uint32_t nb = n_bricks(k);
   uint32_t first = 0, ovf = 0;
@@ -578,6 +624,11 @@ __global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) {
   }
   if (ovf) atomicAdd(&k.hdr[H_VOXEL_OVF], ovf);
   if (threadIdx.x == 0) k.hdr[H_VOXELS] = first;
+}
+__global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) { alloc_voxels_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(1024) k_alloc_voxels_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  alloc_voxels_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
@@ -619,10 +670,10 @@ __device__ __forceinline__ uint32_t snorm16(float e) {
   return (uint32_t)(int)floorf(0.5f + 32767.0f * fminf(1.0f, fmaxf(-1.0f, e))) & 0xffffu;
 }
 
-__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) {
+__device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t nb = n_bricks(k);
   uint32_t n = nb * 512u;
-  for (uint32_t i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+  for (uint32_t i = lb * BLK + threadIdx.x; i < n; i += nblk * BLK) {
     uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
     const uint32_t* o = k.occ + (size_t)ptr * 16u;
     uint32_t word = o[w];
@@ -663,11 +714,12 @@ __global__ void __launch_bounds__(BLK) k_resolve(VoxK k) {
     k.plane[ai] = dot3(nn, q - p);
   }
 }
-
-int grid_for(uint64_t n) {
-  uint64_t g = (n + BLK - 1) / BLK;
-  return (int)(g < 1 ? 1 : g);
+__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_resolve_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  resolve_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
+
 
 }  // namespace
 
@@ -677,21 +729,69 @@ size_t vox_mark_lds_bytes(const VoxK& k) {
   return bytes <= 64 * 1024 ? bytes : 0;
 }
 
+static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+
+int launch_vox_build_batch(const BatchPlan& p, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (p.grid[BL_CLEAR]) k_clear_b<<<p.grid[BL_CLEAR], BLK, 0, s>>>(bk(p, BL_CLEAR));
+  if (p.grid[BL_MARK]) k_mark_b<<<p.grid[BL_MARK], BLK, p.mark_lds, s>>>(bk(p, BL_MARK));
+  if (p.grid[BL_MARK_BIG]) k_mark_big_b<<<p.grid[BL_MARK_BIG], BLK, p.mark_lds, s>>>(bk(p, BL_MARK_BIG));
+  if (p.grid[BL_MARK_PARENTS]) k_mark_parents_b<<<p.grid[BL_MARK_PARENTS], BLK, 0, s>>>(bk(p, BL_MARK_PARENTS));
+  if (p.grid[BL_ALLOC_BRICKS]) k_alloc_bricks_b<<<p.grid[BL_ALLOC_BRICKS], 1024, 0, s>>>(bk(p, BL_ALLOC_BRICKS));
+  if (p.grid[BL_INIT]) k_init_bricks_b<<<p.grid[BL_INIT], BLK, 0, s>>>(bk(p, BL_INIT));
+  if (p.grid[BL_VOX_SMALL]) k_vox_small_b<<<p.grid[BL_VOX_SMALL], BLK, 0, s>>>(bk(p, BL_VOX_SMALL));
+  if (p.grid[BL_VOX_BIG]) k_vox_big_b<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+  for (int L = 1; L < RTGI_MAX_LEVELS; L++)
+    if (p.grid[BL_REDUCE + L - 1]) k_reduce_b<<<p.grid[BL_REDUCE + L - 1], BLK, 0, s>>>(bk(p, BL_REDUCE + L - 1, L));
+  if (p.grid[BL_ALLOC_VOXELS]) k_alloc_voxels_b<<<p.grid[BL_ALLOC_VOXELS], 1024, 0, s>>>(bk(p, BL_ALLOC_VOXELS));
+  if (p.grid[BL_RESOLVE]) k_resolve_b<<<p.grid[BL_RESOLVE], BLK, 0, s>>>(bk(p, BL_RESOLVE));
+  return (int)hipGetLastError();
+}
+
+static uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
+  const VoxK& k = t.vx;
+  for (int i = 0; i < BL_COUNT; i++) out[i] = 0;
+  uint64_t cells = (uint64_t)k.NL * k.per, pool = (uint64_t)k.max_bricks * 512u;
+  uint32_t persist = std::min<uint32_t>(2048, std::max<uint32_t>(1, cdiv(pool, BLK * 4)));
+  out[BL_CLEAR] = cdiv(std::max<uint64_t>(cells, HDR_WORDS), BLK);
+  out[BL_MARK] = k.ntris ? std::min<uint32_t>(1024, cdiv(k.ntris, BLK * 4)) : 0;
+  out[BL_MARK_BIG] = std::min<uint32_t>(512, std::max<uint32_t>(1, k.ntris / 1024));
+  out[BL_MARK_PARENTS] = k.NL > 1 ? cdiv((uint64_t)(k.NL - 1) * k.per, BLK) : 0;
+  out[BL_ALLOC_BRICKS] = 1;
+  out[BL_INIT] = persist;
+  out[BL_VOX_SMALL] = k.ntris ? cdiv(k.ntris, (uint64_t)BLK * VOX_CHUNK) : 0;
+  out[BL_VOX_BIG] = std::min<uint32_t>(2048, std::max<uint32_t>(16, k.ntris / 256));
+  for (int L = 1; L < k.NL; L++) out[BL_REDUCE + L - 1] = persist;
+  out[BL_ALLOC_VOXELS] = 1;
+  out[BL_RESOLVE] = persist;
+  out[BL_TRACE] = cdiv(t.total_rays, BLK);
+  for (int st = 0; st < t.N; st++) {
+    int c = t.N - 1 - st;
+    uint64_t work = c > 0 ? (uint64_t)t.c[c - 1].res * t.c[c - 1].res * t.c[c].np : t.c[0].rays;
+    out[BL_MERGE + st] = cdiv(work, BLK);
+  }
+  out[BL_SH] = cdiv(t.c[0].np, BLK);
+}
+
 int launch_vox_build(const VoxK& k, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const int persist = 2048;  // 256 CUs x 8 workgroups of 256 threads for the grid-stride stages
+  TraceK t{};
+  t.vx = k;
+  uint32_t g[BL_COUNT];
+  batch_blocks(t, g);
   size_t lds = k.lds_marks ? vox_mark_lds_bytes(k) : 0;
-  int mark_grid = std::min(1024, grid_for(k.ntris));
-  k_clear<<<grid_for((uint64_t)std::max(k.NL * k.per, (int)HDR_WORDS)), BLK, 0, s>>>(k);
-  if (k.ntris) k_mark<<<mark_grid, BLK, lds, s>>>(k);
-  k_mark_big<<<std::min(persist, 512), BLK, lds, s>>>(k);
-  if (k.NL > 1) k_mark_parents<<<grid_for((uint64_t)(k.NL - 1) * k.per), BLK, 0, s>>>(k);
+  k_clear<<<g[BL_CLEAR], BLK, 0, s>>>(k);
+  if (g[BL_MARK]) k_mark<<<g[BL_MARK], BLK, lds, s>>>(k);
+  k_mark_big<<<g[BL_MARK_BIG], BLK, lds, s>>>(k);
+  if (g[BL_MARK_PARENTS]) k_mark_parents<<<g[BL_MARK_PARENTS], BLK, 0, s>>>(k);
   k_alloc_bricks<<<1, 1024, 0, s>>>(k);
-  k_init_bricks<<<persist, BLK, 0, s>>>(k);
-  if (k.ntris) k_vox_small<<<grid_for((k.ntris + VOX_CHUNK - 1) / VOX_CHUNK), BLK, 0, s>>>(k);
-  k_vox_big<<<persist, BLK, 0, s>>>(k);
-  for (int L = 1; L < k.NL; L++) k_reduce<<<persist, BLK, 0, s>>>(k, L);
+  k_init_bricks<<<g[BL_INIT], BLK, 0, s>>>(k);
+  if (g[BL_VOX_SMALL]) k_vox_small<<<g[BL_VOX_SMALL], BLK, 0, s>>>(k);
+  k_vox_big<<<g[BL_VOX_BIG], BLK, 0, s>>>(k);
+  for (int L = 1; L < k.NL; L++) k_reduce<<<g[BL_REDUCE + L - 1], BLK, 0, s>>>(k, L);
   k_alloc_voxels<<<1, 1024, 0, s>>>(k);
-  k_resolve<<<persist, BLK, 0, s>>>(k);
+  k_resolve<<<g[BL_RESOLVE], BLK, 0, s>>>(k);
   return (int)hipGetLastError();
 }

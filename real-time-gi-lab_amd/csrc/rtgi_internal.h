@@ -102,6 +102,15 @@ struct TraceK {
   uint64_t total_rays;
 };
 
+// One batched launch: world descriptors and the block prefix (n + 1 entries) in device memory; arg is the level of a
+// reduce step or the cascade-step of a merge.
+struct BatchK {
+  const TraceK* __restrict__ w;
+  const uint32_t* __restrict__ start;
+  int32_t n;
+  int32_t arg;
+};
+
 // Launchers (each returns a hipError_t as int).
 size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
@@ -109,6 +118,38 @@ int launch_trace(const TraceK& k, void* stream);
 int launch_merge(const TraceK& k, int c, void* stream);
 int launch_sh(const TraceK& k, void* stream);
 
+
+// The launches of one batched update, and for each the per-world block prefix (device memory) and grid size.
+enum BatchLaunch : int {
+  BL_CLEAR = 0,
+  BL_MARK,
+  BL_MARK_BIG,
+  BL_MARK_PARENTS,
+  BL_ALLOC_BRICKS,
+  BL_INIT,
+  BL_VOX_SMALL,
+  BL_VOX_BIG,
+  BL_REDUCE,                               // + level - 1 (levels 1..7)
+  BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
+  BL_RESOLVE,
+  BL_TRACE,
+  BL_MERGE,                                // + step (cascade N-1-step of each world)
+  BL_SH = BL_MERGE + RTGI_MAX_CASCADES,
+  BL_COUNT
+};
+struct BatchPlan {
+  const TraceK* w;
+  int32_t n;
+  const uint32_t* start[BL_COUNT];
+  uint32_t grid[BL_COUNT];
+  size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
+};
+// Blocks a world needs for each launch (the single-world launches use the same counts).
+void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);
+int launch_vox_build_batch(const BatchPlan& p, void* stream);
+int launch_trace_batch(const BatchPlan& p, void* stream);
+int launch_merge_batch(const BatchPlan& p, int step, void* stream);
+int launch_sh_batch(const BatchPlan& p, void* stream);
 
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).
 void dir_entry(int di, int res, DirEntry* out);

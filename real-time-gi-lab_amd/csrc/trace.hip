@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rtgi_internal.h"
+#include "batch.h"
 #include "vmath.h"
 
 namespace {
@@ -269,8 +270,8 @@ __device__ __forceinline__ void split_ray(const CascK& C, uint32_t r, uint32_t& 
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_trace(TraceK a) {
-  uint64_t ray = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+__device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  uint64_t ray = (uint64_t)lb * BLK + threadIdx.x;
   if (ray >= a.total_rays) return;
   int c = cascade_of(a, ray);
   const CascK& C = a.c[c];
@@ -293,6 +294,11 @@ __global__ void __launch_bounds__(BLK) k_trace(TraceK a) {
   }
   reinterpret_cast<float4*>(a.rad)[ray] = out;
   if (a.hits) a.hits[ray] = h.id;
+}
+__global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_trace_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  trace_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ f3 rgb_of(float4 x) { return mk3(x.x, x.y, x.z); }
@@ -329,11 +335,11 @@ __device__ __forceinline__ float4 merged(const TraceK& a, int c, i3 g, uint32_t 
 // Cascade c >= 1: thread per (direction texel dl of cascade c-1's resolution, probe). It merges the 2x2 block of c's
 // directions under dl (a single texel when the resolution is capped, rc.ts:682) and stores the block's average
 // ((p00 + p10 + p01 + p11) * 0.25, upperAvg's order) for cascade c-1's merge.
-__global__ void __launch_bounds__(BLK) k_merge_up(TraceK a, int c) {
+__device__ __forceinline__ void merge_up_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[c];
   const CascK& D = a.c[c - 1];
   uint32_t ndl = (uint32_t)(D.res * D.res);
-  uint64_t t = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
   if (t >= (uint64_t)ndl * C.np) return;
   uint32_t dl, pi;
   if (C.n_log2 >= 0) {
@@ -366,17 +372,19 @@ __global__ void __launch_bounds__(BLK) k_merge_up(TraceK a, int c) {
   }
   reinterpret_cast<float4*>(a.avg)[C.avg_off + (size_t)dl * C.np + pi] = make_float4(avg.x, avg.y, avg.z, 0.0f);
 }
+__global__ void __launch_bounds__(BLK) k_merge_up(TraceK a, int c) { merge_up_body(a, c, blockIdx.x, gridDim.x); }
 
 // Cascade 0: thread per ray.
-__global__ void __launch_bounds__(BLK) k_merge0(TraceK a) {
+__device__ __forceinline__ void merge0_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[0];
-  uint64_t r = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t r = (uint64_t)lb * BLK + threadIdx.x;
   if (r >= C.rays) return;
   uint32_t di, pi;
   split_ray(C, (uint32_t)r, di, pi);
   float4* rad = reinterpret_cast<float4*>(a.rad);
   rad[C.ray_off + r] = merged(a, 0, probe_xyz(C, pi), di, rad[C.ray_off + r]);
 }
+__global__ void __launch_bounds__(BLK) k_merge0(TraceK a) { merge0_body(a, blockIdx.x, gridDim.x); }
 
 // shBasis (src/core/rc.ts:745-749) of c0 texel d, from the direction table.
 __device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) {
@@ -402,9 +410,9 @@ __device__ __forceinline__ void sh_store(const TraceK& a, uint32_t np, uint32_t 
 }
 
 // projectSH (src/core/rc.ts:752-774) of already merged c0 radiance: 27 f32 per probe in 27 planes.
-__global__ void __launch_bounds__(BLK) k_sh(TraceK a) {
+__device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[0];
-  uint32_t pi = blockIdx.x * BLK + threadIdx.x;
+  uint32_t pi = lb * BLK + threadIdx.x;
   if (pi >= C.np) return;
   const float4* rad = reinterpret_cast<const float4*>(a.rad);
   f3 cf[9];
@@ -416,6 +424,23 @@ __global__ void __launch_bounds__(BLK) k_sh(TraceK a) {
     for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];
   }
   sh_store(a, C.np, pi, cf);
+}
+__global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  sh_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
+// Merge step of a batch: each world merges its cascade N-1-step (c > 0 with the 2x2 averages, c = 0 per ray).
+__global__ void __launch_bounds__(BLK) k_merge_step_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  const TraceK& a = b.w[w];
+  uint32_t lb = blockIdx.x - b.start[w], nb = b.start[w + 1] - b.start[w];
+  int c = a.N - 1 - b.arg;
+  if (c > 0)
+    merge_up_body(a, c, lb, nb);
+  else
+    merge0_body(a, lb, nb);
 }
 
 int grid_for(uint64_t n) {
@@ -442,5 +467,23 @@ int launch_merge(const TraceK& k, int c, void* stream) {
 
 int launch_sh(const TraceK& k, void* stream) {
   k_sh<<<grid_for(k.c[0].np), BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
+}
+
+static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+
+int launch_trace_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_TRACE]) k_trace_b<<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+  return (int)hipGetLastError();
+}
+
+int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
+  if (p.grid[BL_MERGE + step])
+    k_merge_step_b<<<p.grid[BL_MERGE + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_MERGE + step, step));
+  return (int)hipGetLastError();
+}
+
+int launch_sh_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_SH]) k_sh_b<<<p.grid[BL_SH], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SH));
   return (int)hipGetLastError();
 }

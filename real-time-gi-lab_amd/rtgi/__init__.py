@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib as _L
 from ._lib import RtgiError, check, lib, ptr
 
-__all__ = ["VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World",
+__all__ = ["VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World", "Batch",
            "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY", "device_count", "oct_solid_angles",
            "lib"]
 
@@ -275,6 +275,50 @@ class World:
         sh = np.zeros((n, 27), np.float32)
         check(lib().rtgi_world_read_sh(self._h, ptr(sh)))
         return sh
+
+
+class Batch:
+    """One GI update of many worlds with one launch per stage (rtgi_batch_*); optionally replayed as a hipGraph."""
+
+    def __init__(self, worlds, graph=False):
+        self.worlds = list(worlds)
+        arr = (C.c_void_p * len(self.worlds))(*[w._h.value for w in self.worlds])
+        h = C.c_void_p()
+        check(lib().rtgi_batch_create(arr, len(self.worlds), C.byref(h)))
+        self._h = h
+        if graph:
+            check(lib().rtgi_batch_set_graph(self._h, 1))
+
+    def encode(self, stream=None):
+        check(lib().rtgi_batch_enqueue(self._h, stream))
+
+    def sync(self):
+        check(lib().rtgi_batch_sync(self._h))
+
+    def set_timing(self, on=True):
+        check(lib().rtgi_batch_set_timing(self._h, 1 if on else 0))
+
+    def stage_ms(self):
+        ms = np.zeros(4, np.float32)
+        check(lib().rtgi_batch_stage_ms(self._h, ptr(ms)))
+        return dict(zip(("vox", "trace", "merge", "sh"), ms.tolist()))
+
+    def destroy(self):
+        if self._h:
+            lib().rtgi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
 
 
 def _atrium_points(seed=1):

@@ -94,6 +94,13 @@ SIGNATURES = {
     "rtgi_world_read_cascade": (C.c_int, [_P, C.c_int32, _P, _P]),
     "rtgi_world_read_sh": (C.c_int, [_P, _P]),
     "rtgi_world_destroy": (C.c_int, [_P]),
+    "rtgi_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    "rtgi_batch_set_graph": (C.c_int, [_P, C.c_int32]),
+    "rtgi_batch_enqueue": (C.c_int, [_P, _P]),
+    "rtgi_batch_sync": (C.c_int, [_P]),
+    "rtgi_batch_set_timing": (C.c_int, [_P, C.c_int32]),
+    "rtgi_batch_stage_ms": (C.c_int, [_P, _P]),
+    "rtgi_batch_destroy": (C.c_int, [_P]),
     "rtgi_scene_generate": (C.c_int, [C.POINTER(SceneParamsC), C.POINTER(C.POINTER(SceneC))]),
     "rtgi_scene_free": (C.c_int, [C.POINTER(SceneC)]),
 }
