@@ -55,6 +55,9 @@ typedef struct rtgi_vox_params {
 /* Flags of rtgi_rc_params.flags. */
 #define RTGI_RC_RECORD_HITS 1u /* also write the first-hit voxel id of every ray (rtgi_world_read_cascade) */
 #define RTGI_RC_NO_MERGE 2u    /* stop after the trace (the cascade buffers hold the traced intervals) */
+#define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
+                                  reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
+                                  pack2x16float, rc.ts:770-771); all arithmetic stays f32 */
 
 /* Radiance-cascade parameters; the subset of RcParams (src/core/rc.ts:42-91) that the world-space voxel tracer uses,
  * with probes on a dense camera-centred grid instead of screen-registered sparse probes (relocation 'none'). */
@@ -188,7 +191,8 @@ int rtgi_world_get_layout(const rtgi_world* w, rtgi_layout* out);
 int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* occ_bits, uint32_t* rep, uint32_t* albedo,
                             uint32_t* normal, float* plane, uint8_t* brick_state);
 
-/* Cascade c after the last update: rgba (4 floats per ray: radiance, transmittance; after the merge a = 1), in ray order
+/* Cascade c after the last update (f16 storage is widened to f32): rgba (4 floats per ray: radiance, transmittance;
+ * after the merge a = 1), in ray order
  * probe-major (probe x + n (y + n z)), direction di = u + res v within a probe; hits (if recorded): first-hit voxel id
  * level << 29 | (x + E (y + E z)) of the window cell, RTGI_NO_HIT for a miss. Either pointer may be NULL. Replaces reading
  * the rad buffer (src/core/rc.ts:1064). Synchronises. */

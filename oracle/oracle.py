@@ -36,7 +36,7 @@ class OrcRc(C.Structure):
                 ("vox_skip", C.c_float), ("vox_level", C.c_int32 * 8), ("b0", C.c_float * 8), ("b1", C.c_float * 8),
                 ("G0", C.c_int32 * 3), ("sun_dir", C.c_float * 3), ("sun_e", C.c_float * 3), ("sky", C.c_float * 3),
                 ("sun_vis", C.c_float), ("n_point", C.c_int32), ("point_pos", (C.c_float * 3) * 8),
-                ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16)]
+                ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -59,6 +59,8 @@ _SIG = {
     "orc_pack4x8unorm": (C.c_uint32, [_P]),
     "orc_pack2x16snorm": (C.c_uint32, [C.c_float, C.c_float]),
     "orc_oct_enc": (None, [_P, _P]),
+    "orc_f2h": (C.c_uint16, [C.c_float]),
+    "orc_h2f": (C.c_float, [C.c_uint16]),
 }
 _lib = None
 
@@ -112,6 +114,10 @@ def oct_enc(n):
     return out
 
 
+def f2h(x):
+    return int(lib().orc_f2h(float(np.float32(x))))
+
+
 def oct_solid_angles(res, samples=1 << 20):
     out = np.zeros(res * res, np.float32)
     lib().orc_oct_solid_angles(res, samples, _p(out))
@@ -144,6 +150,7 @@ class Oracle:
         r.probes0 = rc.probes0
         r.res_cap = rc.resCap
         r.vox_skip = rc.voxSkip
+        r.fp16 = 1 if getattr(rc, "fp16", False) else 0
         eye = np.ascontiguousarray(frame.eye, np.float64)
         lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
                                 _p(eye))

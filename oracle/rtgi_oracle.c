@@ -550,6 +550,48 @@ int orc_canonical(const orc_vox* p, const uint32_t* vox, const uint32_t* rep, in
 /* ------------------------------------------------------------------------------------------------------------ */
 /* Cascades.                                                                                                     */
 
+/* binary16 conversion with round to nearest even (pack2x16float / unpack2x16float, rc.ts:309-310). */
+uint16_t orc_f2h(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u, a = x & 0x7fffffffu;
+  if (a >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (a > 0x7f800000u ? 0x200u : 0u));
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to infinity */
+  if (a >= 0x38800000u) {                                  /* normal */
+    uint32_t r = a - 0x38000000u;
+    r += 0xfffu + ((r >> 13) & 1u);
+    return (uint16_t)(sign | (r >> 13));
+  }
+  if (a < 0x33000000u) return (uint16_t)sign; /* below 2^-25 */
+  uint32_t e = a >> 23, m = (a & 0x7fffffu) | 0x800000u, shift = 126u - e;
+  uint32_t q = m >> shift, rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+  if (rem > half || (rem == half && (q & 1u))) q++;
+  return (uint16_t)(sign | q);
+}
+float orc_h2f(uint16_t h) {
+  uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u, bits;
+  if (e == 0) {
+    if (m == 0) {
+      bits = s;
+    } else {
+      e = 113;
+      while (!(m & 1024u)) {
+        m <<= 1;
+        e--;
+      }
+      bits = s | (e << 23) | ((m & 1023u) << 13);
+    }
+  } else if (e == 31) {
+    bits = s | 0x7f800000u | (m << 13);
+  } else {
+    bits = s | ((e + 112u) << 23) | (m << 13);
+  }
+  float f;
+  memcpy(&f, &bits, 4);
+  return f;
+}
+static float q16(const orc_rc* rc, float x) { return rc->fp16 ? orc_h2f(orc_f2h(x)) : x; }
+
 /* octEncode (frame.ts:97-103) in double; octSolidAngles (frame.ts:105-114). */
 static void octEncodeD(double x, double y, double z, double* u, double* v) {
   double s = fabs(x) + fabs(y) + fabs(z);
@@ -836,6 +878,7 @@ int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t 
     float out[4];
     uint32_t h;
     traceRay(&W, rc, (uint64_t)q, out, &h);
+    for (int k = 0; k < 4; k++) out[k] = q16(rc, out[k]); /* packRad of the trace output (rc.ts:621) */
     if (rgba) memcpy(rgba + (size_t)(q - (int64_t)r0) * 4, out, 16);
     if (hits) hits[q - (int64_t)r0] = h;
   }
@@ -885,9 +928,10 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
         }
         up = ws > 0.0f ? vdivs(acc, fmx(ws, 1.0f)) : V3(0, 0, 0);
       }
-      v[0] = v[0] + v[3] * up.x;
-      v[1] = v[1] + v[3] * up.y;
-      v[2] = v[2] + v[3] * up.z;
+      /* packRad of the merged value (rc.ts:707) */
+      v[0] = q16(rc, v[0] + v[3] * up.x);
+      v[1] = q16(rc, v[1] + v[3] * up.y);
+      v[2] = q16(rc, v[2] + v[3] * up.z);
       v[3] = 1.0f;
     }
   }
@@ -909,9 +953,9 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
       }
       for (int k = 0; k < 9; k++) {
         float a = k == 0 ? a0 : (k < 4 ? a1 : a2);
-        sh[pi * 27 + k * 3] = cf[k].x * a;
-        sh[pi * 27 + k * 3 + 1] = cf[k].y * a;
-        sh[pi * 27 + k * 3 + 2] = cf[k].z * a;
+        sh[pi * 27 + k * 3] = q16(rc, cf[k].x * a); /* pack2x16float (rc.ts:770-771) */
+        sh[pi * 27 + k * 3 + 1] = q16(rc, cf[k].y * a);
+        sh[pi * 27 + k * 3 + 2] = q16(rc, cf[k].z * a);
       }
     }
   }

@@ -34,6 +34,7 @@ typedef struct orc_rc {
   int32_t n_point;
   float point_pos[8][3], point_i[8][3];
   float omega0[16];
+  int32_t fp16; /* precision form: radiance quantised to f16 (RNE) where packRad stores it, SH too */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -52,6 +53,10 @@ int orc_build(const orc_vox* p, const float* pos, const float* nrm, const uint32
 /* Canonical content of level L (the layout of rtgi_world_read_clipmap). Any output may be NULL. */
 int orc_canonical(const orc_vox* p, const uint32_t* vox, const uint32_t* rep, int level, uint32_t* occ_bits,
                   uint32_t* rep_out, uint32_t* albedo, uint32_t* normal, float* plane, uint8_t* brick_state);
+
+/* f32 <-> f16 (IEEE binary16, round to nearest even: pack2x16float / unpack2x16float). */
+uint16_t orc_f2h(float f);
+float orc_h2f(uint16_t h);
 
 /* Number of rays of cascade c and the total. */
 uint64_t orc_rays(const orc_rc* rc, int c);

@@ -106,6 +106,35 @@ void oct_encode_d(double x, double y, double z, double& u, double& v) {
   v = py * 0.5 + 0.5;
 }
 
+// Radiance and SH buffer sizes in floats: f32 rgba / 27 f32 per probe, or halves in the precision form.
+size_t rad_floats(const rtgi_world* w) { return (size_t)w->ray_off[w->N] * ((w->rp.flags & RTGI_RC_FP16) ? 2 : 4); }
+size_t sh_floats(const rtgi_world* w) {
+  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
+  return (w->rp.flags & RTGI_RC_FP16) ? (np0 * 27 + 1) / 2 : np0 * 27;
+}
+float half_to_float(uint16_t h) {
+  uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u, bits;
+  if (e == 0) {
+    if (m == 0) {
+      bits = s;
+    } else {  // subnormal: renormalise
+      e = 127 - 15 + 1;
+      while (!(m & 1024u)) {
+        m <<= 1;
+        e--;
+      }
+      bits = s | (e << 23) | ((m & 1023u) << 13);
+    }
+  } else if (e == 31) {
+    bits = s | 0x7f800000u | (m << 13);
+  } else {
+    bits = s | ((e + 127 - 15) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
 bool is_pow2(double v) {
   int e;
   return std::frexp(v, &e) == 0.5;
@@ -180,6 +209,7 @@ TraceK trace_args(const rtgi_world* w) {
   std::memcpy(t.point_i, w->frame.point_i, sizeof(t.point_i));
   std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
   t.dirs = w->dirs;
+  t.fp16 = (w->rp.flags & RTGI_RC_FP16) ? 1 : 0;
   t.rad = w->rad;
   t.avg = w->avg;
   t.hits = w->hits;
@@ -204,10 +234,9 @@ void free_all(rtgi_world* w) {
   dfree(w, w->pos, (size_t)w->cap_verts * 3);
   dfree(w, w->nrm, (size_t)w->cap_verts * 3);
   dfree(w, w->mat, (size_t)w->cap_mat * 4);
-  dfree(w, w->rad, (size_t)w->ray_off[w->N] * 4);
+  dfree(w, w->rad, rad_floats(w));
   dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
-  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
-  dfree(w, w->sh, np0 * 27);
+  dfree(w, w->sh, sh_floats(w));
   dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
 }
@@ -336,7 +365,6 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
 
   int e = RTGI_OK;
   size_t NLper = (size_t)w->NL * w->per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
-  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
   if (!e) e = dalloc(w, &w->hdr, HDR_WORDS);
   if (!e) e = dalloc(w, &w->ptr, NLper);
   if (!e) e = dalloc(w, &w->occ, mb * 16);
@@ -345,9 +373,9 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->rep, mb * 512);
   if (!e) e = dalloc(w, &w->attr, mv * 2);
   if (!e) e = dalloc(w, &w->plane, mv);
-  if (!e) e = dalloc(w, &w->rad, (size_t)off * 4);
+  if (!e) e = dalloc(w, &w->rad, rad_floats(w));
   if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
-  if (!e) e = dalloc(w, &w->sh, np0 * 27);
+  if (!e) e = dalloc(w, &w->sh, sh_floats(w));
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
   if (!e) {
@@ -642,7 +670,13 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   size_t n = w->ray_off[c + 1] - w->ray_off[c];
   size_t np = (size_t)w->n[c] * w->n[c] * w->n[c], dirs = (size_t)w->res[c] * w->res[c];
   // Device order is direction-major (ray = di * np + probe); the readback is probe-major.
-  if (rgba) {
+  if (rgba && (w->rp.flags & RTGI_RC_FP16)) {
+    std::vector<uint16_t> t(n * 4);
+    HIPCHK(hipMemcpy(t.data(), (const uint16_t*)w->rad + w->ray_off[c] * 4, n * 8, hipMemcpyDeviceToHost));
+    for (size_t d = 0; d < dirs; d++)
+      for (size_t p = 0; p < np; p++)
+        for (int q = 0; q < 4; q++) rgba[(p * dirs + d) * 4 + q] = half_to_float(t[(d * np + p) * 4 + q]);
+  } else if (rgba) {
     std::vector<float> t(n * 4);
     HIPCHK(hipMemcpy(t.data(), w->rad + w->ray_off[c] * 4, n * 16, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < dirs; d++)
@@ -662,10 +696,17 @@ extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(world_wait(w));
   size_t np = (size_t)w->n[0] * w->n[0] * w->n[0];
-  std::vector<float> planes(np * 27);
-  HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
-  for (size_t p = 0; p < np; p++)
-    for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + p];
+  if (w->rp.flags & RTGI_RC_FP16) {
+    std::vector<uint16_t> planes(np * 27);
+    HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 2, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < np; p++)
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = half_to_float(planes[(size_t)q * np + p]);
+  } else {
+    std::vector<float> planes(np * 27);
+    HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < np; p++)
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + p];
+  }
   return RTGI_OK;
 }
 

@@ -95,7 +95,8 @@ struct TraceK {
   float point_i[RTGI_MAX_POINT_LIGHTS][3];
   float omega0[16];
   const DirEntry* dirs;  // all cascades' direction tables
-  float* rad;       // 4 floats per ray; within a cascade direction-major: ray = di * np + probe
+  int32_t fp16;     // rad holds 4 halves per ray and sh halves (RNE at the reference's pack points), else f32
+  float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
   uint32_t* hits;   // per ray (same order), or null
   float* sh;        // [27][probes0^3]

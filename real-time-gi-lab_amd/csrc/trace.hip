@@ -17,6 +17,7 @@
 // The DDA restates the reference's f32 operation sequence exactly (compiled with -ffp-contract=off; IEEE division
 // and square root; x / v becomes x * (1/v) only when v is a power of two, where both are the same value), so
 // first-hit voxels are bit-identical to a CPU restatement of the same WGSL.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "rtgi_internal.h"
@@ -32,6 +33,30 @@ constexpr float PI_F = (float)3.14159265;
 constexpr float SH_A0 = (float)3.14159265;
 constexpr float SH_A1 = (float)(2.0 * 3.14159265 / 3.0);
 constexpr float SH_A2 = (float)(3.14159265 / 4.0);
+
+// Radiance storage: f32 rgba, or (precision form) 4 halves rounded to nearest even, as packRad / unpackRad
+// (src/core/rc.ts:309-310). rad_store returns the value a later read sees (the rounded one in f16 mode).
+__device__ __forceinline__ uint32_t h2(float x, float y) {
+  return (uint32_t)__half_as_ushort(__float2half_rn(x)) | ((uint32_t)__half_as_ushort(__float2half_rn(y)) << 16);
+}
+__device__ __forceinline__ float hlo(uint32_t p) { return __half2float(__ushort_as_half((unsigned short)(p & 0xffffu))); }
+__device__ __forceinline__ float hhi(uint32_t p) { return __half2float(__ushort_as_half((unsigned short)(p >> 16))); }
+__device__ __forceinline__ float4 rad_load(const TraceK& a, size_t i) {
+  if (a.fp16) {
+    uint2 p = reinterpret_cast<const uint2*>(a.rad)[i];
+    return make_float4(hlo(p.x), hhi(p.x), hlo(p.y), hhi(p.y));
+  }
+  return reinterpret_cast<const float4*>(a.rad)[i];
+}
+__device__ __forceinline__ float4 rad_store(const TraceK& a, size_t i, float4 v) {
+  if (a.fp16) {
+    uint2 p = make_uint2(h2(v.x, v.y), h2(v.z, v.w));
+    reinterpret_cast<uint2*>(a.rad)[i] = p;
+    return make_float4(hlo(p.x), hhi(p.x), hlo(p.y), hhi(p.y));
+  }
+  reinterpret_cast<float4*>(a.rad)[i] = v;
+  return v;
+}
 
 struct VoxHit {
   uint32_t brick, bit;
@@ -292,7 +317,7 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
     f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
     out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
-  reinterpret_cast<float4*>(a.rad)[ray] = out;
+  rad_store(a, ray, out);
   if (a.hits) a.hits[ray] = h.id;
 }
 __global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body(a, blockIdx.x, gridDim.x); }
@@ -350,22 +375,20 @@ __device__ __forceinline__ void merge_up_body(const TraceK& a, int c, uint32_t l
     pi = (uint32_t)(t % C.np);
   }
   i3 g = probe_xyz(C, pi);
-  float4* rad = reinterpret_cast<float4*>(a.rad) + C.ray_off + pi;
+  size_t base = C.ray_off + pi;
   uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
   f3 avg;
   if (C.res == D.res) {
-    size_t i = (size_t)dl * C.np;
-    float4 m = merged(a, c, g, dl, rad[i]);
-    rad[i] = m;
+    size_t i = base + (size_t)dl * C.np;
+    float4 m = rad_store(a, i, merged(a, c, g, dl, rad_load(a, i)));
     avg = mk3(m.x, m.y, m.z);
   } else {
     uint32_t d00 = ((2u * lw) << C.res_log2) + 2u * lu;
     uint32_t dd[4] = {d00, d00 + 1u, d00 + (uint32_t)C.res, d00 + (uint32_t)C.res + 1u};
     f3 m[4];
     for (int j = 0; j < 4; j++) {
-      size_t i = (size_t)dd[j] * C.np;
-      float4 x = merged(a, c, g, dd[j], rad[i]);
-      rad[i] = x;
+      size_t i = base + (size_t)dd[j] * C.np;
+      float4 x = rad_store(a, i, merged(a, c, g, dd[j], rad_load(a, i)));
       m[j] = mk3(x.x, x.y, x.z);
     }
     avg = (m[0] + m[1] + m[2] + m[3]) * 0.25f;
@@ -381,8 +404,7 @@ __device__ __forceinline__ void merge0_body(const TraceK& a, uint32_t lb, uint32
   if (r >= C.rays) return;
   uint32_t di, pi;
   split_ray(C, (uint32_t)r, di, pi);
-  float4* rad = reinterpret_cast<float4*>(a.rad);
-  rad[C.ray_off + r] = merged(a, 0, probe_xyz(C, pi), di, rad[C.ray_off + r]);
+  rad_store(a, C.ray_off + r, merged(a, 0, probe_xyz(C, pi), di, rad_load(a, C.ray_off + r)));
 }
 __global__ void __launch_bounds__(BLK) k_merge0(TraceK a) { merge0_body(a, blockIdx.x, gridDim.x); }
 
@@ -400,12 +422,18 @@ __device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) {
   y[8] = 0.546274f * (nx * nx - ny * ny);
 }
 
+// 27 planes of f32, or of halves in the precision form (pack2x16float per value, src/core/rc.ts:770-771).
 __device__ __forceinline__ void sh_store(const TraceK& a, uint32_t np, uint32_t pi, const f3 cf[9]) {
   for (int q = 0; q < 9; q++) {
     float s = q == 0 ? SH_A0 : (q < 4 ? SH_A1 : SH_A2);
-    a.sh[(size_t)(q * 3 + 0) * np + pi] = cf[q].x * s;
-    a.sh[(size_t)(q * 3 + 1) * np + pi] = cf[q].y * s;
-    a.sh[(size_t)(q * 3 + 2) * np + pi] = cf[q].z * s;
+    float v[3] = {cf[q].x * s, cf[q].y * s, cf[q].z * s};
+    for (int ch = 0; ch < 3; ch++) {
+      size_t i = (size_t)(q * 3 + ch) * np + pi;
+      if (a.fp16)
+        reinterpret_cast<unsigned short*>(a.sh)[i] = __half_as_ushort(__float2half_rn(v[ch]));
+      else
+        a.sh[i] = v[ch];
+    }
   }
 }
 
@@ -414,11 +442,10 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
   const CascK& C = a.c[0];
   uint32_t pi = lb * BLK + threadIdx.x;
   if (pi >= C.np) return;
-  const float4* rad = reinterpret_cast<const float4*>(a.rad);
   f3 cf[9];
   for (int q = 0; q < 9; q++) cf[q] = splat3(0.0f);
   for (uint32_t d = 0; d < 16; d++) {
-    f3 r = rgb_of(rad[C.ray_off + (size_t)d * C.np + pi]) * a.omega0[d];
+    f3 r = rgb_of(rad_load(a, C.ray_off + (size_t)d * C.np + pi)) * a.omega0[d];
     float y[9];
     sh_basis(a.dirs[C.dir_off + d], y);
     for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];

@@ -64,6 +64,7 @@ class RcParams:
     resCap: int = 8
     recordHits: bool = False
     noMerge: bool = False
+    fp16: bool = False       # precision form: radiance and SH stored as f16 at the reference's pack points
 
 
 DEFAULT_PARAMS = RcParams()
@@ -152,7 +153,8 @@ class World:
     def __init__(self, vox: VoxParams, rc: RcParams):
         self.vox, self.rc = vox, rc
         vp = _L.VoxParamsC(vox.v0, vox.levels, vox.G, vox.maxBricks, vox.maxVoxels)
-        flags = (_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
+        flags = ((_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
+                 | (_L.RC_FP16 if rc.fp16 else 0))
         rp = _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
                           rc.voxSkip, rc.resCap, flags)
         h = C.c_void_p()

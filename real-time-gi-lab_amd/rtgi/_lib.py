@@ -21,6 +21,7 @@ PTR_MARK = 0xFFFFFFFD
 NO_HIT = 0xFFFFFFFF
 RC_RECORD_HITS = 1
 RC_NO_MERGE = 2
+RC_FP16 = 4
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15
 SCENE_ATRIUM, SCENE_CITY = 1, 2
 

@@ -32,6 +32,8 @@ def make_worlds(seed=0):
                   rtgi.RcParams(cascades=1, probes0=4, s0=0.4), (0.05, 0.0, 0.0)))
     specs.append((random_scene(rng, 200, -4, 4, 0.5), rtgi.VoxParams(0.25, 3, 4, 192, 1 << 16),
                   rtgi.RcParams(cascades=3, probes0=8, s0=0.5, noMerge=True), (0.0, 0.0, 0.0)))
+    specs.append((random_scene(rng, 600, -5, 5, 0.6), rtgi.VoxParams(0.25, 3, 8, 3 * 512, 1 << 16),  # f16 storage
+                  rtgi.RcParams(cascades=4, probes0=16, s0=0.5, fp16=True), (0.3, -0.2, 0.1)))
     worlds = []
     for i, (sc, vox, rc, eye) in enumerate(specs):
         w = rtgi.World(vox, dataclasses.replace(rc, recordHits=True))
@@ -125,7 +127,7 @@ def test_batch_of_cities_matches_solo(require_gpu):
         w.upload_scene(sc)
         w.set_frame(dataclasses.replace(frame, eye=sc.eye))
         worlds.append(w)
-    solo = [snapshot(w) for w in worlds] if False else solo_results(worlds)
+    solo = solo_results(worlds)
     with rtgi.Batch(worlds, graph=True) as b:
         b.encode()
         b.sync()

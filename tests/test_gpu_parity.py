@@ -234,3 +234,26 @@ def test_golden_fixture(require_gpu):
         off += n
     np.testing.assert_allclose(w.read_sh(), g["sh"], rtol=RTOL, atol=ATOL)
     w.destroy()
+
+
+@pytest.mark.parametrize("case", ["tiny", "atrium", "city"])
+def test_fp16_precision_form(require_gpu, case):
+    """Radiance and SH stored as f16 at the reference's pack points: identical to the oracle's quantised values."""
+    if case == "tiny":
+        scene, vox, rc, frame = tiny_case(seed=8, ntris=500)
+    else:
+        scene, vox, rc, frame = rtgi.config(case)
+    rc = dataclasses.replace(rc, fp16=True)
+    say(f"case: f16 storage, {case}")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    rgba, hits = o.trace()
+    m, sh = o.merge_sh(rgba)
+    per_m, per_h = o.split(m), o.split(hits)
+    for c in range(rc.cascades):
+        g, h = w.read_cascade(c, hits=True)
+        assert np.array_equal(h, per_h[c])
+        assert np.array_equal(g, per_m[c]), f"cascade {c}: {np.count_nonzero(g != per_m[c])} values differ"
+    assert np.array_equal(w.read_sh(), sh)
+    w.destroy()

@@ -129,6 +129,18 @@ def test_oct_encode_axes_and_round_trip():
         assert np.allclose(m, n, atol=1e-5)
 
 
+def test_f16_rounding_matches_ieee():
+    """pack2x16float's f32 -> f16 conversion (round to nearest even) against numpy's IEEE binary16 cast."""
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([rng.normal(0, 1, 20000) * np.exp(rng.uniform(-30, 12, 20000)),
+                           [0.0, -0.0, 65504.0, 65519.99, 6e-8, 5.96e-8, 2.98e-8, 2 ** -25, 2 ** -24, 1e-40,
+                            1.0 + 2 ** -11, 1.0 + 3 * 2 ** -11]]).astype(np.float32)
+    with np.errstate(over="ignore"):
+        ref = vals.astype(np.float16).view(np.uint16)
+    got = np.array([orc.f2h(v) for v in vals], np.uint16)
+    assert np.array_equal(got, ref)
+
+
 def test_oct_solid_angles():
     w = orc.oct_solid_angles(4)
     assert abs(w.astype(np.float64).sum() - 4 * np.pi) < 1e-5
