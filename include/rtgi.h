@@ -55,6 +55,7 @@ typedef struct rtgi_vox_params {
 /* Flags of rtgi_rc_params.flags. */
 #define RTGI_RC_RECORD_HITS 1u /* also write the first-hit voxel id of every ray (rtgi_world_read_cascade) */
 #define RTGI_RC_NO_MERGE 2u    /* stop after the trace (the cascade buffers hold the traced intervals) */
+#define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
                                   pack2x16float, rc.ts:770-771); all arithmetic stays f32 */
@@ -197,6 +198,15 @@ int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* occ_bits, ui
  * level << 29 | (x + E (y + E z)) of the window cell, RTGI_NO_HIT for a miss. Either pointer may be NULL. Replaces reading
  * the rad buffer (src/core/rc.ts:1064). Synchronises. */
 int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hits);
+
+/* Tracer statistics of the last update (RTGI_RC_STATS): per cascade the world traces, the DDA loop iterations and
+ * the traces that reached the 1024-iteration cap (the reference's counts[24..32), [32..40), [64..72), rc.ts:232-235). */
+typedef struct rtgi_trace_stats {
+  uint64_t traces[RTGI_MAX_CASCADES];
+  uint64_t steps[RTGI_MAX_CASCADES];
+  uint64_t capped[RTGI_MAX_CASCADES];
+} rtgi_trace_stats;
+int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out);
 
 /* SH irradiance of the c0 probes: 27 floats per probe (9 coefficients x rgb, scaled by {pi, 2pi/3, pi/4}), replacing
  * the sh buffer written by projectSH (src/core/rc.ts:752-774) before its f16 packing. Synchronises. */

@@ -49,7 +49,7 @@ _SIG = {
     "orc_build": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, C.c_uint32, _P, _P, _P]),
     "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
-    "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, C.c_int]),
+    "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int]),
     "orc_merge_sh": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
     "orc_trace_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
                                 C.c_double, C.c_int]),
@@ -208,13 +208,14 @@ class Oracle:
         assert rc == 0
         return dict(occ=occ, rep=rp, albedo=alb, normal=nrm, plane=pl, brick_state=st)
 
-    def trace(self, r0=0, r1=None, threads=0):
+    def trace(self, r0=0, r1=None, threads=0, steps=False):
         total = sum(self.cascade_rays())
         r1 = total if r1 is None else r1
         rgba = np.zeros((r1 - r0, 4), np.float32)
         hits = np.zeros(r1 - r0, np.uint32)
-        lib().orc_trace(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgba), _p(hits), threads)
-        return rgba, hits
+        st = np.zeros(r1 - r0, np.uint32) if steps else None
+        lib().orc_trace(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgba), _p(hits), _p(st), threads)
+        return (rgba, hits, st) if steps else (rgba, hits)
 
     def merge_sh(self, rgba):
         rgba = np.ascontiguousarray(rgba, np.float32).copy()

@@ -740,9 +740,10 @@ static uint32_t hitId(const orc_vox* p, uint32_t L, iv3 c) {
 
 /* traceVox, voxel.ts:121-202. out: rgb, T. */
 static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, float t1, float tSkip, uint32_t L,
-                     float out[4], uint32_t* hid) {
+                     float out[4], uint32_t* hid, uint32_t* steps) {
   const orc_vox* p = W->p;
   *hid = 0xffffffffu;
+  *steps = 0;
   out[0] = out[1] = out[2] = 0.0f;
   out[3] = 1.0f;
   float v = voxSize(p, L);
@@ -765,6 +766,7 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
   v3 tDelta = vabs(vscale(inv, v));
   float tIn = t;
   for (int it = 0; it < 1024; it++) {
+    (*steps)++; /* voxSteps (voxel.ts:143) */
     iv3 b = ishr(c, 3);
     uint32_t ptr = vPtr(W, L, b);
     if (ptr == V_DROP) {
@@ -843,7 +845,7 @@ static int cascadeOf(const orc_rc* rc, uint64_t ray, uint64_t* local) {
 }
 
 /* trace (rc.ts:600-622) -> traceWorld (rc.ts:216-245) with the probe at its grid point, normal 0. */
-static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid) {
+static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid, uint32_t* steps) {
   uint64_t r = 0;
   int c = cascadeOf(rc, ray, &r);
   int res = resOf(rc, c), n = probesOf(rc, c);
@@ -857,7 +859,7 @@ static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4]
   float t0 = rc->b0[c] * rc->s0, t1 = rc->b1[c] * rc->s0;
   uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
   float v = voxSize(W->p, L);
-  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid);
+  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
 }
 
 static void makeView(View* W, const orc_vox* p, const uint32_t* vox) {
@@ -867,7 +869,7 @@ static void makeView(View* W, const orc_vox* p, const uint32_t* vox) {
 }
 
 int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
-              uint32_t* hits, int threads) {
+              uint32_t* hits, uint32_t* steps, int threads) {
   View W;
   makeView(&W, p, vox);
 #ifdef _OPENMP
@@ -876,8 +878,9 @@ int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t 
 #endif
   for (int64_t q = (int64_t)r0; q < (int64_t)r1; q++) {
     float out[4];
-    uint32_t h;
-    traceRay(&W, rc, (uint64_t)q, out, &h);
+    uint32_t h, st;
+    traceRay(&W, rc, (uint64_t)q, out, &h, &st);
+    if (steps) steps[q - (int64_t)r0] = st;
     for (int k = 0; k < 4; k++) out[k] = q16(rc, out[k]); /* packRad of the trace output (rc.ts:621) */
     if (rgba) memcpy(rgba + (size_t)(q - (int64_t)r0) * 4, out, 16);
     if (hits) hits[q - (int64_t)r0] = h;

@@ -60,10 +60,10 @@ float orc_h2f(uint16_t h);
 
 /* Number of rays of cascade c and the total. */
 uint64_t orc_rays(const orc_rc* rc, int c);
-/* Trace of rays [r0, r1) of the whole ray range (cascade-major order) -> rgba, hit ids (either may be NULL).
- * threads: OpenMP threads (<= 0: all). */
+/* Trace of rays [r0, r1) of the whole ray range (cascade-major order) -> rgba, hit ids, DDA loop iterations (the
+ * reference's voxSteps); any may be NULL. threads: OpenMP threads (<= 0: all). */
 int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
-              uint32_t* hits, int threads);
+              uint32_t* hits, uint32_t* steps, int threads);
 /* mergeWorld N-1..0 in place on the full rgba array, then projectSH -> sh (27 floats per c0 probe). */
 int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
 

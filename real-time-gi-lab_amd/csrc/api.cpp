@@ -62,6 +62,7 @@ struct rtgi_world {
   float* sh = nullptr;
   DirEntry* dirs = nullptr;
   float* avg = nullptr;
+  unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
   uint64_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -169,6 +170,7 @@ VoxK vox_args(const rtgi_world* w) {
   k.ntris = w->ntris;
   k.lds_marks = 0;
   k.lds_marks = vox_mark_lds_bytes(k) > 0;
+  k.stats = w->stats;
   return k;
 }
 
@@ -239,6 +241,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->sh, sh_floats(w));
   dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
+  dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
 }
 
 // Waits for the world's own stream and for the last batched update that included it.
@@ -378,6 +381,10 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->sh, sh_floats(w));
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
+  if (!e && (rp->flags & RTGI_RC_STATS)) {
+    e = dalloc(w, &w->stats, 3 * RTGI_MAX_CASCADES);
+    if (!e && hipMemset(w->stats, 0, 3 * RTGI_MAX_CASCADES * 8) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipMemset");
+  }
   if (!e) {
     std::vector<DirEntry> tab(w->dir_off[Nc]);
     for (int c = 0; c < Nc; c++)
@@ -691,6 +698,21 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   return RTGI_OK;
 }
 
+extern "C" int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out) {
+  if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  if (!w->stats) return rtgi_fail(RTGI_E_STATE, "tracer statistics are off (RTGI_RC_STATS)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  unsigned long long v[3 * RTGI_MAX_CASCADES];
+  HIPCHK(hipMemcpy(v, w->stats, sizeof(v), hipMemcpyDeviceToHost));
+  for (int c = 0; c < RTGI_MAX_CASCADES; c++) {
+    out->traces[c] = v[c];
+    out->steps[c] = v[RTGI_MAX_CASCADES + c];
+    out->capped[c] = v[2 * RTGI_MAX_CASCADES + c];
+  }
+  return RTGI_OK;
+}
+
 extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
   if (!w || !sh) return rtgi_fail(RTGI_E_INVALID, "null argument");
   HIPCHK(hipSetDevice(w->device));
@@ -790,6 +812,8 @@ int batch_plan(rtgi_batch* b, const std::vector<uint64_t>& sig) {
   // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
   // says whether it marks in LDS).
   b->plan.mark_lds = all_lds ? lds : 0;
+  b->plan.stats = false;
+  for (int i = 0; i < n; i++) b->plan.stats = b->plan.stats || b->worlds[i]->stats != nullptr;
   b->sig = sig;
   b->graph_valid = false;
   return RTGI_OK;

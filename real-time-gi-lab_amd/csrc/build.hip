@@ -191,6 +191,7 @@ __device__ __forceinline__ void bits_flush(const VoxK& k) {
 __device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t i = lb * BLK + threadIdx.x;
   if (i < HDR_WORDS) k.hdr[i] = 0u;
+  if (k.stats && i < 3u * RTGI_MAX_CASCADES) k.stats[i] = 0ull;
   if (i < (uint32_t)(k.NL * k.per)) k.ptr[i] = PTR_EMPTY;
 }
 __global__ void __launch_bounds__(BLK) k_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
@@ -769,8 +770,7 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   out[BL_TRACE] = cdiv(t.total_rays, BLK);
   for (int st = 0; st < t.N; st++) {
     int c = t.N - 1 - st;
-    uint64_t work = c > 0 ? (uint64_t)t.c[c - 1].res * t.c[c - 1].res * t.c[c].np : t.c[0].rays;
-    out[BL_MERGE + st] = cdiv(work, BLK);
+    out[BL_MERGE + st] = cdiv(merge_threads(t, c), BLK);
   }
   out[BL_SH] = cdiv(t.c[0].np, BLK);
 }

@@ -58,6 +58,7 @@ struct VoxK {
   const float* mat;     // 4 floats per material
   uint32_t ntris;
   int32_t lds_marks;    // brick marks accumulate in an LDS bitmap of levels * G^3 bits (0: global fallback)
+  unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
 };
 
 struct CascK {
@@ -116,6 +117,8 @@ struct BatchK {
 size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
 int launch_trace(const TraceK& k, void* stream);
+// Threads of cascade c's merge launch.
+uint64_t merge_threads(const TraceK& k, int c);
 int launch_merge(const TraceK& k, int c, void* stream);
 int launch_sh(const TraceK& k, void* stream);
 
@@ -144,6 +147,7 @@ struct BatchPlan {
   const uint32_t* start[BL_COUNT];
   uint32_t grid[BL_COUNT];
   size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
+  bool stats;       // some world of the batch counts tracer statistics
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);

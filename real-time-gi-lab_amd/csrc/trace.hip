@@ -154,14 +154,16 @@ struct Hit {
   uint32_t brick, bit;  // occupied voxel (brick slot, bit)
   float tIn;            // entry distance
   uint32_t id;          // level << 29 | window cell
+  uint32_t steps;       // loop iterations (the reference's voxSteps), counted only by trace_vox<true>
 };
 
 // traceVox (src/core/voxel.ts:121-202): hierarchical DDA over level L; empty bricks are skipped whole. d and inv come
 // from the direction table (dd = d with |d| < 1e-9 components set to 1e-9, inv = 1 / dd). Returns the first occupied
 // voxel entered at tIn >= tSkip (shading happens after the loop).
+template <bool ST>
 __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1, float tSkip, int L) {
   const VoxK& k = a.vx;
-  Hit res{false, 0u, 0u, 0.0f, RTGI_NO_HIT};
+  Hit res{false, 0u, 0u, 0.0f, RTGI_NO_HIT, 0u};
   const bool p2 = k.pow2 != 0;
   float v = vox_size(k, L);
   float iv = k.inv_v0 * (1.0f / (float)(1u << L));
@@ -184,7 +186,8 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   float tIn = t;
   const uint32_t* ptrL = k.ptr + (size_t)L * (uint32_t)k.per;
   uint32_t lastb = 0xffffffffu, ptr = 0;
-  for (int it = 0; it < 1024; it++) {
+  int it;
+  for (it = 0; it < 1024; it++) {
     i3 b = shr3(c, 3);
     uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
     if (bi != lastb) {  // the pointer of the brick is read once per brick visited
@@ -194,7 +197,7 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
     if (ptr == PTR_DROP) {
       VoxHit h;
       if (vox_at(k, L, c, h) && tIn >= tSkip) {
-        res = Hit{true, h.brick, h.bit, tIn, hit_id(k, h.level, h.cell)};
+        res = Hit{true, h.brick, h.bit, tIn, hit_id(k, h.level, h.cell), ST ? (uint32_t)it + 1u : 0u};
         return res;
       }
       float tx = fminf(fminf(tMax.x, tMax.y), tMax.z);
@@ -235,7 +238,7 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
     uint32_t occ = k.occ[ptr * 16u + (bit >> 5)];
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
-      res = Hit{true, ptr, bit, tIn, hit_id(k, L, c)};
+      res = Hit{true, ptr, bit, tIn, hit_id(k, L, c), ST ? (uint32_t)it + 1u : 0u};
       return res;
     }
     if (tExit >= tEnd) break;
@@ -252,6 +255,7 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
     }
     if (!inside3(c, nVox)) break;
   }
+  if (ST) res.steps = (uint32_t)min(it + 1, 1024);
   return res;
 }
 
@@ -295,6 +299,7 @@ __device__ __forceinline__ void split_ray(const CascK& C, uint32_t r, uint32_t& 
   }
 }
 
+template <bool ST>
 __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   uint64_t ray = (uint64_t)lb * BLK + threadIdx.x;
   if (ray >= a.total_rays) return;
@@ -311,7 +316,7 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
   float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
   int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
   float v = vox_size(a.vx, L);
-  Hit h = trace_vox(a, o, d, inv, t0, t1, a.vox_skip * v, L);
+  Hit h = trace_vox<ST>(a, o, d, inv, t0, t1, a.vox_skip * v, L);
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (h.hit) {
     f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
@@ -319,81 +324,126 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
   }
   rad_store(a, ray, out);
   if (a.hits) a.hits[ray] = h.id;
+  if (ST && a.vx.stats) {
+    // counts[24 + c], [32 + c], [64 + c] of traceWorld (src/core/rc.ts:232-235), summed per wavefront when all its
+    // lanes trace the same cascade.
+    unsigned long long* S = a.vx.stats;
+    int c0 = __shfl(c, 0);
+    bool full = __ballot(1) == ~0ull;  // the reduction reads every lane: only for a wavefront with no lane returned
+    if (full && __all(c == c0)) {
+      unsigned long long st = h.steps, cap = h.steps >= 1024u ? 1ull : 0ull, n = 1ull;
+      for (int m = 32; m >= 1; m >>= 1) {
+        st += __shfl_xor(st, m);
+        cap += __shfl_xor(cap, m);
+        n += __shfl_xor(n, m);
+      }
+      if (__lane_id() == 0) {
+        atomicAdd(&S[c], n);
+        atomicAdd(&S[RTGI_MAX_CASCADES + c], st);
+        if (cap) atomicAdd(&S[2 * RTGI_MAX_CASCADES + c], cap);
+      }
+    } else {
+      atomicAdd(&S[c], 1ull);
+      atomicAdd(&S[RTGI_MAX_CASCADES + c], (unsigned long long)h.steps);
+      if (h.steps >= 1024u) atomicAdd(&S[2 * RTGI_MAX_CASCADES + c], 1ull);
+    }
+  }
 }
-__global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body(a, blockIdx.x, gridDim.x); }
+// ST: count the tracer statistics (a separate instantiation, so the counting costs the plain trace nothing).
+template <bool ST>
+__global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body<ST>(a, blockIdx.x, gridDim.x); }
+template <bool ST>
 __global__ void __launch_bounds__(BLK) k_trace_b(BatchK b) {
   int w = batch_world(b, blockIdx.x);
-  trace_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  trace_body<ST>(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ f3 rgb_of(float4 x) { return mk3(x.x, x.y, x.z); }
 
-// Parent mean for one (probe g, direction di) of cascade c < N-1 (mergeWorld, src/core/rc.ts:693-706) on the dense
-// grid: the parents of probe g are the c+1 grid points (g >> 1) + o, o in {0,1}^3 with o = 0 on even axes
-// (linkParents, rc.ts:540-570), those outside the grid absent. avg holds cascade c+1's upperAvg at c's directions.
-__device__ __forceinline__ f3 merge_up(const TraceK& a, int c, i3 g, uint32_t di) {
-  const CascK& U = a.c[c + 1];
-  const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
-  int un = U.n;
-  f3 acc = splat3(0.0f);
-  float ws = 0.0f;
+// Parents of probe g of cascade c < N-1 on the dense grid (linkParents, src/core/rc.ts:540-570): the c+1 grid points
+// (g >> 1) + o, o in {0,1}^3 with o = 0 on even axes, those outside the grid absent. Bit kk of the mask is parent
+// o = (kk & 1, kk >> 1 & 1, kk >> 2 & 1). Per axis one or two parents exist, so their count is 1, 2, 4 or 8.
+__device__ __forceinline__ uint32_t parent_mask(const CascK& U, i3 g) {
+  uint32_t m = 0;
   for (int kk = 0; kk < 8; kk++) {
     i3 o{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1};
-    if ((o.x && !(g.x & 1)) || (o.y && !(g.y & 1)) || (o.z && !(g.z & 1))) continue;
     i3 q = shr3(g, 1) + o;
-    if (q.x >= un || q.y >= un || q.z >= un) continue;
-    uint32_t qi = U.n_log2 >= 0 ? (uint32_t)q.x | ((uint32_t)q.y << U.n_log2) | ((uint32_t)q.z << (2 * U.n_log2))
-                                : (uint32_t)q.x + (uint32_t)un * ((uint32_t)q.y + (uint32_t)un * (uint32_t)q.z);
-    float4 x = A[qi];
-    acc = acc + mk3(x.x, x.y, x.z);
-    ws += 1.0f;
+    bool ok = !((o.x && !(g.x & 1)) || (o.y && !(g.y & 1)) || (o.z && !(g.z & 1))) && q.x < U.n && q.y < U.n &&
+              q.z < U.n;
+    m |= (ok ? 1u : 0u) << kk;
   }
-  return ws > 0.0f ? acc / fmaxf(ws, 1.0f) : splat3(0.0f);
+  return m;
+}
+__device__ __forceinline__ uint32_t parent_index(const CascK& U, i3 g, int kk) {
+  i3 q = shr3(g, 1) + i3{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1};
+  return U.n_log2 >= 0 ? (uint32_t)q.x | ((uint32_t)q.y << U.n_log2) | ((uint32_t)q.z << (2 * U.n_log2))
+                       : (uint32_t)q.x + (uint32_t)U.n * ((uint32_t)q.y + (uint32_t)U.n * (uint32_t)q.z);
 }
 
-__device__ __forceinline__ float4 merged(const TraceK& a, int c, i3 g, uint32_t di, float4 v) {
-  f3 up = mk3(a.sky[0], a.sky[1], a.sky[2]);
-  if (c + 1 < a.N && v.w > 0.0f) up = merge_up(a, c, g, di);
+// Parent mean for one (probe g, direction di) of cascade c < N-1 (mergeWorld, src/core/rc.ts:693-706): sum over the
+// existing parents in kk order, divided by their count. avg holds cascade c+1's upperAvg at c's directions. Every
+// parent value is loaded before any is used, so a thread's loads are in flight together. The count is a power of two,
+// for which acc * (1 / n) is the correctly rounded acc / n.
+__device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t di) {
+  const CascK& U = a.c[c + 1];
+  const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
+  uint32_t m = parent_mask(U, g);
+  float4 x[8];
+  for (int kk = 0; kk < 8; kk++)
+    x[kk] = ((m >> kk) & 1u) ? A[parent_index(U, g, kk)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  f3 acc = splat3(0.0f);
+  for (int kk = 0; kk < 8; kk++)
+    if ((m >> kk) & 1u) acc = acc + mk3(x[kk].x, x[kk].y, x[kk].z);
+  return acc * (1.0f / (float)__popc(m));
+}
+
+// rad_c = v.rgb + v.a * up, a := 1, with up = sky at the last cascade or where v.a = 0 (mergeWorld, rc.ts:690-707).
+// The parent mean is read before v.a is known (it is discarded where unused), so the loads do not wait for v.
+__device__ __forceinline__ float4 merged(const TraceK& a, int c, i3 g, uint32_t di, float4 v, f3 um) {
+  f3 up = (c + 1 < a.N && v.w > 0.0f) ? um : mk3(a.sky[0], a.sky[1], a.sky[2]);
   return make_float4(v.x + v.w * up.x, v.y + v.w * up.y, v.z + v.w * up.z, 1.0f);
 }
+__device__ __forceinline__ float4 merge_ray(const TraceK& a, int c, i3 g, uint32_t di, size_t i) {
+  f3 um = c + 1 < a.N ? upper_mean(a, c, g, di) : splat3(0.0f);
+  return rad_store(a, i, merged(a, c, g, di, rad_load(a, i), um));
+}
 
-// Cascade c >= 1: thread per (direction texel dl of cascade c-1's resolution, probe). It merges the 2x2 block of c's
-// directions under dl (a single texel when the resolution is capped, rc.ts:682) and stores the block's average
-// ((p00 + p10 + p01 + p11) * 0.25, upperAvg's order) for cascade c-1's merge.
+// Cascade c >= 1, thread per ray. A wavefront holds 16 probes x the 4 directions of one 2x2 block of c's directions
+// under texel dl of cascade c-1's resolution (lane = 16 j + probe, j = 2x2 position); it merges its rays and lane j = 0
+// stores the block's average ((p00 + p10 + p01 + p11) * 0.25, upperAvg's order, rc.ts:681-688) for cascade c-1's
+// merge. Where the resolution is capped (rc.ts:682) the block is one texel: thread per ray, average = merged value.
 __device__ __forceinline__ void merge_up_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[c];
   const CascK& D = a.c[c - 1];
-  uint32_t ndl = (uint32_t)(D.res * D.res);
   uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
-  if (t >= (uint64_t)ndl * C.np) return;
-  uint32_t dl, pi;
-  if (C.n_log2 >= 0) {
-    dl = (uint32_t)(t >> (3 * C.n_log2));
-    pi = (uint32_t)t & (C.np - 1u);
-  } else {
-    dl = (uint32_t)(t / C.np);
-    pi = (uint32_t)(t % C.np);
-  }
-  i3 g = probe_xyz(C, pi);
-  size_t base = C.ray_off + pi;
-  uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
-  f3 avg;
+  float4* AV = reinterpret_cast<float4*>(a.avg) + C.avg_off;
   if (C.res == D.res) {
-    size_t i = base + (size_t)dl * C.np;
-    float4 m = rad_store(a, i, merged(a, c, g, dl, rad_load(a, i)));
-    avg = mk3(m.x, m.y, m.z);
-  } else {
-    uint32_t d00 = ((2u * lw) << C.res_log2) + 2u * lu;
-    uint32_t dd[4] = {d00, d00 + 1u, d00 + (uint32_t)C.res, d00 + (uint32_t)C.res + 1u};
-    f3 m[4];
-    for (int j = 0; j < 4; j++) {
-      size_t i = base + (size_t)dd[j] * C.np;
-      float4 x = rad_store(a, i, merged(a, c, g, dd[j], rad_load(a, i)));
-      m[j] = mk3(x.x, x.y, x.z);
-    }
-    avg = (m[0] + m[1] + m[2] + m[3]) * 0.25f;
+    if (t >= C.rays) return;
+    uint32_t di, pi;
+    split_ray(C, (uint32_t)t, di, pi);
+    float4 m = merge_ray(a, c, probe_xyz(C, pi), di, C.ray_off + t);
+    AV[(size_t)di * C.np + pi] = make_float4(m.x, m.y, m.z, 0.0f);
+    return;
   }
-  reinterpret_cast<float4*>(a.avg)[C.avg_off + (size_t)dl * C.np + pi] = make_float4(avg.x, avg.y, avg.z, 0.0f);
+  uint32_t npb = ((uint32_t)C.np + 15u) >> 4;
+  uint64_t grp = t >> 6;
+  if (grp >= (uint64_t)D.res * D.res * npb) return;  // uniform over the wavefront
+  uint32_t lane = (uint32_t)t & 63u, j = lane >> 4;
+  uint32_t dl = (uint32_t)(grp / npb), pi = (uint32_t)(grp % npb) * 16u + (lane & 15u);
+  bool ok = pi < (uint32_t)C.np;
+  uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
+  uint32_t di = ((2u * lw + (j >> 1)) << C.res_log2) + 2u * lu + (j & 1u);
+  float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (ok) m = merge_ray(a, c, probe_xyz(C, pi), di, C.ray_off + (size_t)di * C.np + pi);
+  uint32_t l0 = lane & 15u;
+  f3 p00 = mk3(__shfl(m.x, l0), __shfl(m.y, l0), __shfl(m.z, l0));
+  f3 p10 = mk3(__shfl(m.x, l0 + 16), __shfl(m.y, l0 + 16), __shfl(m.z, l0 + 16));
+  f3 p01 = mk3(__shfl(m.x, l0 + 32), __shfl(m.y, l0 + 32), __shfl(m.z, l0 + 32));
+  f3 p11 = mk3(__shfl(m.x, l0 + 48), __shfl(m.y, l0 + 48), __shfl(m.z, l0 + 48));
+  if (ok && j == 0) {
+    f3 avg = (p00 + p10 + p01 + p11) * 0.25f;
+    AV[(size_t)dl * C.np + pi] = make_float4(avg.x, avg.y, avg.z, 0.0f);
+  }
 }
 __global__ void __launch_bounds__(BLK) k_merge_up(TraceK a, int c) { merge_up_body(a, c, blockIdx.x, gridDim.x); }
 
@@ -404,7 +454,7 @@ __device__ __forceinline__ void merge0_body(const TraceK& a, uint32_t lb, uint32
   if (r >= C.rays) return;
   uint32_t di, pi;
   split_ray(C, (uint32_t)r, di, pi);
-  rad_store(a, C.ray_off + r, merged(a, 0, probe_xyz(C, pi), di, rad_load(a, C.ray_off + r)));
+  merge_ray(a, 0, probe_xyz(C, pi), di, C.ray_off + r);
 }
 __global__ void __launch_bounds__(BLK) k_merge0(TraceK a) { merge0_body(a, blockIdx.x, gridDim.x); }
 
@@ -478,14 +528,23 @@ int grid_for(uint64_t n) {
 }  // namespace
 
 int launch_trace(const TraceK& k, void* stream) {
-  if (k.total_rays) k_trace<<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+  if (k.total_rays) {
+    if (k.vx.stats)
+      k_trace<true><<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+    else
+      k_trace<false><<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+  }
   return (int)hipGetLastError();
+}
+
+uint64_t merge_threads(const TraceK& k, int c) {
+  if (c == 0 || k.c[c].res == k.c[c - 1].res) return k.c[c].rays;
+  return (uint64_t)k.c[c - 1].res * k.c[c - 1].res * ((k.c[c].np + 15u) >> 4) * 64u;
 }
 
 int launch_merge(const TraceK& k, int c, void* stream) {
   if (c > 0) {
-    uint64_t n = (uint64_t)k.c[c - 1].res * k.c[c - 1].res * k.c[c].np;
-    k_merge_up<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+    k_merge_up<<<grid_for(merge_threads(k, c)), BLK, 0, (hipStream_t)stream>>>(k, c);
   } else if (k.c[0].rays) {
     k_merge0<<<grid_for(k.c[0].rays), BLK, 0, (hipStream_t)stream>>>(k);
   }
@@ -500,7 +559,12 @@ int launch_sh(const TraceK& k, void* stream) {
 static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
 
 int launch_trace_batch(const BatchPlan& p, void* stream) {
-  if (p.grid[BL_TRACE]) k_trace_b<<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+  if (p.grid[BL_TRACE]) {
+    if (p.stats)
+      k_trace_b<true><<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+    else
+      k_trace_b<false><<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+  }
   return (int)hipGetLastError();
 }
 

@@ -65,6 +65,7 @@ class RcParams:
     recordHits: bool = False
     noMerge: bool = False
     fp16: bool = False       # precision form: radiance and SH stored as f16 at the reference's pack points
+    stats: int = 0           # setStats (src/core/rc.ts:1233): 1 counts traces, DDA steps and step-cap hits
 
 
 DEFAULT_PARAMS = RcParams()
@@ -154,7 +155,7 @@ class World:
         self.vox, self.rc = vox, rc
         vp = _L.VoxParamsC(vox.v0, vox.levels, vox.G, vox.maxBricks, vox.maxVoxels)
         flags = ((_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
-                 | (_L.RC_FP16 if rc.fp16 else 0))
+                 | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0))
         rp = _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
                           rc.voxSkip, rc.resCap, flags)
         h = C.c_void_p()
@@ -271,6 +272,13 @@ class World:
         h = np.zeros(n, np.uint32) if hits else None
         check(lib().rtgi_world_read_cascade(self._h, c, ptr(rgba), ptr(h)))
         return (rgba, h) if hits else rgba
+
+    def read_trace_stats(self):
+        """Per cascade: world traces, DDA steps, traces at the 1024-step cap (rc.ts:232-235)."""
+        st = _L.TraceStatsC()
+        check(lib().rtgi_world_read_trace_stats(self._h, C.byref(st)))
+        N = self.rc.cascades
+        return dict(traces=list(st.traces[:N]), steps=list(st.steps[:N]), capped=list(st.capped[:N]))
 
     def read_sh(self):
         n = self.rc.probes0 ** 3

@@ -22,6 +22,7 @@ NO_HIT = 0xFFFFFFFF
 RC_RECORD_HITS = 1
 RC_NO_MERGE = 2
 RC_FP16 = 4
+RC_STATS = 8
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15
 SCENE_ATRIUM, SCENE_CITY = 1, 2
 
@@ -61,6 +62,11 @@ class LayoutC(C.Structure):
                 ("device_bytes", C.c_uint64)]
 
 
+class TraceStatsC(C.Structure):
+    _fields_ = [("traces", C.c_uint64 * MAX_CASCADES), ("steps", C.c_uint64 * MAX_CASCADES),
+                ("capped", C.c_uint64 * MAX_CASCADES)]
+
+
 class SceneParamsC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("frame", C.c_int32), ("extent", C.c_double),
                 ("buildings", C.c_int32), ("detail", C.c_double)]
@@ -94,6 +100,7 @@ SIGNATURES = {
     "rtgi_world_read_clipmap": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P]),
     "rtgi_world_read_cascade": (C.c_int, [_P, C.c_int32, _P, _P]),
     "rtgi_world_read_sh": (C.c_int, [_P, _P]),
+    "rtgi_world_read_trace_stats": (C.c_int, [_P, C.POINTER(TraceStatsC)]),
     "rtgi_world_destroy": (C.c_int, [_P]),
     "rtgi_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
     "rtgi_batch_set_graph": (C.c_int, [_P, C.c_int32]),

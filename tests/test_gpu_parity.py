@@ -257,3 +257,30 @@ def test_fp16_precision_form(require_gpu, case):
         assert np.array_equal(g, per_m[c]), f"cascade {c}: {np.count_nonzero(g != per_m[c])} values differ"
     assert np.array_equal(w.read_sh(), sh)
     w.destroy()
+
+
+@pytest.mark.parametrize("case", ["tiny", "overflow", "atrium"])
+def test_trace_stats(require_gpu, case):
+    """setStats counters (rc.ts:232-235): traces, DDA loop iterations and step-cap hits per cascade equal the
+    oracle's per-ray voxSteps summed per cascade; a second update starts them from zero again."""
+    if case == "atrium":
+        scene, vox, rc, frame = rtgi.config("atrium")
+    else:
+        scene, vox, rc, frame = tiny_case(seed=4, ntris=500, **({"maxBricks": 7} if case == "overflow" else {}))
+    rc = dataclasses.replace(rc, stats=1)
+    say(f"case: tracer statistics, {case}")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    _, _, steps = o.trace(steps=True)
+    per = o.split(steps)
+    for rep in range(2):
+        st = w.read_trace_stats()
+        for c in range(rc.cascades):
+            s = per[c].astype(np.uint64)
+            assert st["traces"][c] == s.size, (rep, c)
+            assert st["steps"][c] == int(s.sum()), (rep, c, st["steps"][c], int(s.sum()))
+            assert st["capped"][c] == int(np.count_nonzero(s >= 1024)), (rep, c)
+        w.encode()
+        w.sync()
+    w.destroy()
