@@ -215,6 +215,13 @@ int rtgi_world_read_sh(rtgi_world* w, float* sh);
 /* Frees the world's device memory (replaces VoxWorld.destroy src/core/voxel.ts:923 and Rc.destroy rc.ts:1234). */
 int rtgi_world_destroy(rtgi_world* w);
 
+/* Device bytes a world of these parameters holds once a scene of nverts / ntris / nmat is uploaded (what
+ * rtgi_layout.device_bytes then reads), without allocating: the admission test of a memory budget. No reference
+ * counterpart (the reference sizes its GPUBuffers inside createVoxWorld / createRc, voxel.ts:753-767, rc.ts:1061-1072).
+ * Errors as rtgi_world_create. */
+int rtgi_world_estimate_bytes(const rtgi_vox_params* vox, const rtgi_rc_params* rc, uint32_t nverts, uint32_t ntris,
+                              uint32_t nmat, uint64_t* bytes);
+
 /* ---- Batched update of many worlds ----
  * No reference counterpart: the reference updates one world per frame. A batch runs one GI update of each of its
  * worlds with one launch per stage (clipmap stages, trace, each merge step, SH) whatever their sizes; the sequence
@@ -233,6 +240,70 @@ int rtgi_batch_sync(rtgi_batch* b);
 int rtgi_batch_set_timing(rtgi_batch* b, int32_t on);
 int rtgi_batch_stage_ms(rtgi_batch* b, float ms[4]);
 int rtgi_batch_destroy(rtgi_batch* b);
+
+/* ---- Request server: continuous batching of GI updates over resident worlds ----
+ * Replaces the reference's frame loop (src/core/rcrun.ts:300-340: one world, one command buffer per frame, two frames
+ * in flight) for many worlds at once. Worlds are registered with their scene (copied to host memory) and become
+ * resident on the device when a request needs them and their bytes (rtgi_world_estimate_bytes) fit in the memory
+ * budget; idle resident worlds are evicted least recently used first to make room (an evicted world is rebuilt from
+ * its host copy on its next request: every update is a full rebuild, so results do not depend on residency).
+ * Requests carry a frame (viewpoint and lights). A scheduler thread gathers the queued requests whose worlds are idle
+ * and resident or admissible, at most one per world and max_batch per launch, into one batched update (one launch per
+ * stage) and starts it on one of `slots` streams as soon as a slot is free, so requests that arrive while others run
+ * are admitted as slots and memory free up and their work overlaps on the device. Requests of one world run in
+ * submission order. A completion thread waits for each launch, copies the optional per-request outputs, and reports
+ * completions through rtgi_server_poll and the optional callback (called on that thread). */
+typedef struct rtgi_server rtgi_server;
+
+typedef struct rtgi_server_params {
+  uint64_t hbm_budget; /* device bytes for resident worlds (0: 90% of the free device memory at creation) */
+  int32_t max_batch;   /* worlds per batched launch (0: 64) */
+  int32_t slots;       /* batched launches in flight at once (0: 2) */
+} rtgi_server_params;
+
+typedef struct rtgi_request {
+  int32_t world;               /* id from rtgi_server_add_world */
+  uint64_t tag;                /* the caller's id, returned in the completion */
+  rtgi_frame frame;            /* eye and lights of this update */
+  float* sh_out;               /* optional (NULL): receives this update's SH (rtgi_world_read_sh layout) */
+  rtgi_counters* counters_out; /* optional (NULL): receives this update's clipmap counters */
+} rtgi_request;
+
+typedef struct rtgi_completion {
+  uint64_t tag;
+  int32_t world;
+  int32_t status;                /* RTGI_OK, or the RTGI_E_* code of the failed update */
+  int32_t batch_size;            /* worlds in the batched launch that ran it */
+  double t_submit, t_launch, t_done; /* seconds since the server's creation (steady clock) */
+} rtgi_completion;
+
+typedef struct rtgi_server_stats {
+  uint64_t submitted, completed, batches, admissions, evictions;
+  uint64_t resident_bytes, peak_resident_bytes, budget;
+  int32_t worlds, resident_worlds, max_batch_seen, queued;
+} rtgi_server_stats;
+
+typedef void (*rtgi_completion_fn)(void* user, const rtgi_completion* c);
+
+int rtgi_server_create(const rtgi_server_params* p, rtgi_server** out);
+/* Registers a world (parameters and scene, copied). RTGI_E_NOMEM if its bytes exceed the whole budget. */
+int rtgi_server_add_world(rtgi_server* s, const rtgi_vox_params* vox, const rtgi_rc_params* rc, const float* pos,
+                          const float* nrm, uint32_t nverts, const uint32_t* tri, uint32_t ntris, const float* materials,
+                          uint32_t nmat, int32_t* id);
+/* Queues one update. The output pointers must stay valid until the request's completion. */
+int rtgi_server_submit(rtgi_server* s, const rtgi_request* r);
+/* Up to max completions (oldest first) into out, waiting up to wait_ms (0: no wait, < 0: until one is there). */
+int rtgi_server_poll(rtgi_server* s, rtgi_completion* out, int32_t max, int32_t wait_ms, int32_t* n);
+/* Completion callback (NULL: none); completions are still delivered to rtgi_server_poll as well. */
+int rtgi_server_set_callback(rtgi_server* s, rtgi_completion_fn fn, void* user);
+/* Waits until every submitted request has completed. */
+int rtgi_server_drain(rtgi_server* s);
+int rtgi_server_get_stats(rtgi_server* s, rtgi_server_stats* out);
+/* The resident world of id (RTGI_E_STATE if it is not resident); valid until its next eviction. Read it only while
+ * no request for it is queued or running. */
+int rtgi_server_world(rtgi_server* s, int32_t id, rtgi_world** out);
+/* Drains, stops the threads and frees every world. */
+int rtgi_server_destroy(rtgi_server* s);
 
 /* ---- Seeded procedural scenes (inputs of the benchmark configurations; no reference counterpart beyond the Builder
  * idiom of src/core/scenes.ts:176-211 and the rigid posing of src/core/scenes.ts:80-102). ---- */

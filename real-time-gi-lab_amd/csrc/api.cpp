@@ -12,6 +12,7 @@
 
 #include "rtgi.h"
 #include "rtgi_internal.h"
+#include "world.h"
 
 namespace {
 thread_local char g_err[512] = "";
@@ -24,52 +25,6 @@ int rtgi_fail(int code, const char* fmt, ...) {
   va_end(ap);
   return code;
 }
-
-#define HIPCHK(call)                                                                                 \
-  do {                                                                                               \
-    hipError_t e_ = (call);                                                                          \
-    if (e_ != hipSuccess) return rtgi_fail(RTGI_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                                           __FILE__, __LINE__);                                      \
-  } while (0)
-
-struct rtgi_world {
-  int device = 0;
-  rtgi_vox_params vp{};
-  rtgi_rc_params rp{};
-  int NL = 0, G = 0, E = 0, per = 0;
-  int origin[RTGI_MAX_LEVELS][3] = {};
-  int N = 0;
-  int n[RTGI_MAX_CASCADES] = {}, res[RTGI_MAX_CASCADES] = {}, vlev[RTGI_MAX_CASCADES] = {};
-  float b0[RTGI_MAX_CASCADES] = {}, b1[RTGI_MAX_CASCADES] = {};
-  uint64_t ray_off[RTGI_MAX_CASCADES + 1] = {};
-  uint32_t dir_off[RTGI_MAX_CASCADES + 1] = {};
-  uint64_t avg_off[RTGI_MAX_CASCADES + 1] = {};  // [c]: cascade c's 2x2 average; [N]: total
-  int G0[3] = {};
-  float omega0[16] = {};
-  rtgi_frame frame{};
-  bool frame_set = false, scene_set = false;
-  uint32_t ntris = 0, nverts = 0, nmat = 0;
-  uint32_t cap_tris = 0, cap_verts = 0, cap_mat = 0;
-  // device buffers
-  uint32_t *hdr = nullptr, *ptr = nullptr, *occ = nullptr, *battr = nullptr, *bcode = nullptr, *rep = nullptr,
-           *attr = nullptr;
-  float* plane = nullptr;
-  uint32_t *big_vox = nullptr, *big_mark = nullptr;
-  float *pos = nullptr, *nrm = nullptr, *mat = nullptr;
-  uint32_t* tri = nullptr;
-  float* rad = nullptr;
-  uint32_t* hits = nullptr;
-  float* sh = nullptr;
-  DirEntry* dirs = nullptr;
-  float* avg = nullptr;
-  unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
-  uint64_t bytes = 0;
-  hipStream_t stream = nullptr;
-  bool timing = false;
-  hipEvent_t ev[5] = {};
-  uint32_t last_stages = 0;
-  hipEvent_t ext_done = nullptr;  // completion of the last batched update that included this world
-};
 
 namespace {
 
@@ -140,6 +95,8 @@ bool is_pow2(double v) {
   int e;
   return std::frexp(v, &e) == 0.5;
 }
+
+}  // namespace
 
 VoxK vox_args(const rtgi_world* w) {
   VoxK k{};
@@ -220,6 +177,8 @@ TraceK trace_args(const rtgi_world* w) {
   return t;
 }
 
+namespace {
+
 void free_all(rtgi_world* w) {
   size_t NLper = (size_t)w->NL * w->per, mb = (size_t)w->vp.max_bricks, mv = (size_t)w->vp.max_voxels;
   dfree(w, w->hdr, HDR_WORDS);
@@ -244,12 +203,15 @@ void free_all(rtgi_world* w) {
   dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
 }
 
-// Waits for the world's own stream and for the last batched update that included it.
+}  // namespace
+
 hipError_t world_wait(rtgi_world* w) {
   hipError_t e = hipStreamSynchronize(w->stream);
   if (e == hipSuccess && w->ext_done) e = hipEventSynchronize(w->ext_done);
   return e;
 }
+
+namespace {
 
 int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   int e = 0;
@@ -317,9 +279,10 @@ extern "C" int rtgi_oct_solid_angles(int32_t res, int32_t samples, float* out) {
   return RTGI_OK;
 }
 
-extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params* rp, rtgi_world** out) {
-  if (!vp || !rp || !out) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_create: null argument");
-  *out = nullptr;
+namespace {
+
+// Checks the parameters and fills the world's layout (levels, cascades, ray/direction/average offsets).
+int world_layout(rtgi_world* w, const rtgi_vox_params* vp, const rtgi_rc_params* rp) {
   if (vp->levels < 1 || vp->levels > RTGI_MAX_LEVELS) return rtgi_fail(RTGI_E_INVALID, "at most 8 levels");
   if (vp->G < 1 || vp->G > 64 || ((int64_t)vp->G * vp->G * vp->G) % 64 != 0)
     return rtgi_fail(RTGI_E_INVALID, "G must be in 1..64 with G^3 a multiple of 64 (got %d)", vp->G);
@@ -333,12 +296,6 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
     return rtgi_fail(RTGI_E_INVALID, "probes0 (%d) must be a positive multiple of 2^(cascades-1)", rp->probes0);
   if (!(rp->s0 > 0)) return rtgi_fail(RTGI_E_INVALID, "s0 must be positive");
   if (rp->res_cap < 0) return rtgi_fail(RTGI_E_INVALID, "res_cap must be >= 0");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return rtgi_fail(RTGI_E_NODEVICE, "no HIP device visible");
-
-  rtgi_world* w = new (std::nothrow) rtgi_world();
-  if (!w) return rtgi_fail(RTGI_E_NOMEM, "out of host memory");
-  (void)hipGetDevice(&w->device);
   w->vp = *vp;
   w->rp = *rp;
   w->NL = vp->levels;
@@ -364,6 +321,60 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
     w->avg_off[c + 1] = w->avg_off[c] + (c > 0 ? npc * w->res[c - 1] * w->res[c - 1] : 0);
   }
   w->ray_off[Nc] = off;
+  return RTGI_OK;
+}
+
+template <class T>
+uint64_t nbytes(size_t count) {
+  return (uint64_t)std::max<size_t>(count, 1) * sizeof(T);
+}
+
+}  // namespace
+
+uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32_t nverts, uint32_t ntris,
+                     uint32_t nmat) {
+  rtgi_world w;
+  if (world_layout(&w, vp, rp)) return 0;
+  size_t NLper = (size_t)w.NL * w.per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
+  uint64_t b = nbytes<uint32_t>(HDR_WORDS) + nbytes<uint32_t>(NLper) + nbytes<uint32_t>(mb * 16) +
+               nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
+               nbytes<float>(rad_floats(&w)) + nbytes<float>(sh_floats(&w)) +
+               nbytes<DirEntry>(w.dir_off[w.N]) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
+  if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
+  if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
+  if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * ntris);
+  if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
+  if (nmat) b += nbytes<float>((size_t)nmat * 4);
+  return b;
+}
+
+extern "C" int rtgi_world_estimate_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32_t nverts,
+                                         uint32_t ntris, uint32_t nmat, uint64_t* bytes) {
+  if (!vp || !rp || !bytes) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_estimate_bytes: null argument");
+  rtgi_world w;
+  int e = world_layout(&w, vp, rp);
+  if (e) return e;
+  *bytes = world_bytes(vp, rp, nverts, ntris, nmat);
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params* rp, rtgi_world** out) {
+  if (!vp || !rp || !out) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_create: null argument");
+  *out = nullptr;
+  {
+    rtgi_world probe;
+    int e = world_layout(&probe, vp, rp);
+    if (e) return e;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return rtgi_fail(RTGI_E_NODEVICE, "no HIP device visible");
+
+  rtgi_world* w = new (std::nothrow) rtgi_world();
+  if (!w) return rtgi_fail(RTGI_E_NOMEM, "out of host memory");
+  (void)hipGetDevice(&w->device);
+  world_layout(w, vp, rp);
+  int Nc = w->N;
+  uint64_t off = w->ray_off[Nc];
   rtgi_oct_solid_angles(4, 1 << 20, w->omega0);
 
   int e = RTGI_OK;
@@ -765,6 +776,47 @@ struct rtgi_batch {
   hipEvent_t ev[5] = {};
 };
 
+void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p) {
+  for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1)] = 0;
+  size_t lds = 0;
+  bool all_lds = true;
+  p.stats = false;
+  for (int i = 0; i < n; i++) {
+    TraceK t = trace_args(worlds[i]);
+    uint32_t g[BL_COUNT];
+    batch_blocks(t, g);
+    if (worlds[i]->rp.flags & RTGI_RC_NO_MERGE) {
+      for (int st = 0; st < RTGI_MAX_CASCADES; st++) g[BL_MERGE + st] = 0;
+      g[BL_SH] = 0;
+    }
+    for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1) + i + 1] = start[(size_t)l * (n + 1) + i] + g[l];
+    size_t m = vox_mark_lds_bytes(t.vx);
+    if (m == 0) all_lds = false;
+    lds = std::max(lds, m);
+    p.stats = p.stats || worlds[i]->stats != nullptr;
+  }
+  p.n = n;
+  for (int l = 0; l < BL_COUNT; l++) p.grid[l] = start[(size_t)l * (n + 1) + n];
+  // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
+  // says whether it marks in LDS).
+  p.mark_lds = all_lds ? lds : 0;
+}
+
+int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
+  int e = 0;
+  if (ev) HIPCHK(hipEventRecord(ev[0], s));
+  e = launch_vox_build_batch(p, s);
+  if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  if (!e) e = launch_trace_batch(p, s);
+  if (ev) HIPCHK(hipEventRecord(ev[2], s));
+  for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(p, st, s);
+  if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  if (!e) e = launch_sh_batch(p, s);
+  if (ev) HIPCHK(hipEventRecord(ev[4], s));
+  if (e) return rtgi_fail(RTGI_E_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
+  return RTGI_OK;
+}
+
 namespace {
 
 void batch_free(rtgi_batch* b) {
@@ -783,54 +835,16 @@ void batch_free(rtgi_batch* b) {
   delete b;
 }
 
-// Block counts per launch and world -> prefix arrays on the device (synchronous upload; only when sizes change).
+// Block prefix arrays on the device (synchronous upload; only when sizes change).
 int batch_plan(rtgi_batch* b, const std::vector<uint64_t>& sig) {
   const int n = (int)b->worlds.size();
   std::vector<uint32_t> start((size_t)BL_COUNT * (n + 1), 0);
-  size_t lds = 0;
-  bool all_lds = true;
-  for (int i = 0; i < n; i++) {
-    TraceK t = trace_args(b->worlds[i]);
-    uint32_t g[BL_COUNT];
-    batch_blocks(t, g);
-    if (b->worlds[i]->rp.flags & RTGI_RC_NO_MERGE) {
-      for (int st = 0; st < RTGI_MAX_CASCADES; st++) g[BL_MERGE + st] = 0;
-      g[BL_SH] = 0;
-    }
-    for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1) + i + 1] = start[(size_t)l * (n + 1) + i] + g[l];
-    size_t m = vox_mark_lds_bytes(t.vx);
-    if (m == 0) all_lds = false;
-    lds = std::max(lds, m);
-  }
+  plan_blocks(b->worlds.data(), n, start.data(), b->plan);
   HIPCHK(hipMemcpy(b->d_start, start.data(), start.size() * 4, hipMemcpyHostToDevice));
   b->plan.w = b->d_desc;
-  b->plan.n = n;
-  for (int l = 0; l < BL_COUNT; l++) {
-    b->plan.start[l] = b->d_start + (size_t)l * (n + 1);
-    b->plan.grid[l] = start[(size_t)l * (n + 1) + n];
-  }
-  // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
-  // says whether it marks in LDS).
-  b->plan.mark_lds = all_lds ? lds : 0;
-  b->plan.stats = false;
-  for (int i = 0; i < n; i++) b->plan.stats = b->plan.stats || b->worlds[i]->stats != nullptr;
+  for (int l = 0; l < BL_COUNT; l++) b->plan.start[l] = b->d_start + (size_t)l * (n + 1);
   b->sig = sig;
   b->graph_valid = false;
-  return RTGI_OK;
-}
-
-int batch_launch(rtgi_batch* b, hipStream_t s, bool timed) {
-  int e = 0;
-  if (timed) HIPCHK(hipEventRecord(b->ev[0], s));
-  e = launch_vox_build_batch(b->plan, s);
-  if (timed) HIPCHK(hipEventRecord(b->ev[1], s));
-  if (!e) e = launch_trace_batch(b->plan, s);
-  if (timed) HIPCHK(hipEventRecord(b->ev[2], s));
-  for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(b->plan, st, s);
-  if (timed) HIPCHK(hipEventRecord(b->ev[3], s));
-  if (!e) e = launch_sh_batch(b->plan, s);
-  if (timed) HIPCHK(hipEventRecord(b->ev[4], s));
-  if (e) return rtgi_fail(RTGI_E_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
   return RTGI_OK;
 }
 
@@ -922,7 +936,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
       b->exec = nullptr;
       b->graph = nullptr;
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int e = batch_launch(b, s, false);
+      int e = plan_launch(b->plan, s, nullptr);
       hipError_t ce = hipStreamEndCapture(s, &b->graph);
       if (e) return e;
       if (ce != hipSuccess) return rtgi_fail(RTGI_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
@@ -931,7 +945,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
     }
     HIPCHK(hipGraphLaunch(b->exec, s));
   } else {
-    int e = batch_launch(b, s, b->timing);
+    int e = plan_launch(b->plan, s, b->timing ? b->ev : nullptr);
     if (e) return e;
   }
   HIPCHK(hipEventRecord(b->done, s));

@@ -1,0 +1,70 @@
+// Internals shared by the host files of librtgi (api.cpp: worlds and batches, server.cpp: the request server).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "rtgi.h"
+#include "rtgi_internal.h"
+
+#define HIPCHK(call)                                                                                 \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return rtgi_fail(RTGI_E_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                           __FILE__, __LINE__);                                      \
+  } while (0)
+
+struct rtgi_world {
+  int device = 0;
+  rtgi_vox_params vp{};
+  rtgi_rc_params rp{};
+  int NL = 0, G = 0, E = 0, per = 0;
+  int origin[RTGI_MAX_LEVELS][3] = {};
+  int N = 0;
+  int n[RTGI_MAX_CASCADES] = {}, res[RTGI_MAX_CASCADES] = {}, vlev[RTGI_MAX_CASCADES] = {};
+  float b0[RTGI_MAX_CASCADES] = {}, b1[RTGI_MAX_CASCADES] = {};
+  uint64_t ray_off[RTGI_MAX_CASCADES + 1] = {};
+  uint32_t dir_off[RTGI_MAX_CASCADES + 1] = {};
+  uint64_t avg_off[RTGI_MAX_CASCADES + 1] = {};  // [c]: cascade c's 2x2 average; [N]: total
+  int G0[3] = {};
+  float omega0[16] = {};
+  rtgi_frame frame{};
+  bool frame_set = false, scene_set = false;
+  uint32_t ntris = 0, nverts = 0, nmat = 0;
+  uint32_t cap_tris = 0, cap_verts = 0, cap_mat = 0;
+  // device buffers
+  uint32_t *hdr = nullptr, *ptr = nullptr, *occ = nullptr, *battr = nullptr, *bcode = nullptr, *rep = nullptr,
+           *attr = nullptr;
+  float* plane = nullptr;
+  uint32_t *big_vox = nullptr, *big_mark = nullptr;
+  float *pos = nullptr, *nrm = nullptr, *mat = nullptr;
+  uint32_t* tri = nullptr;
+  float* rad = nullptr;
+  uint32_t* hits = nullptr;
+  float* sh = nullptr;
+  DirEntry* dirs = nullptr;
+  float* avg = nullptr;
+  unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
+  uint64_t bytes = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  uint32_t last_stages = 0;
+  hipEvent_t ext_done = nullptr;  // completion of the last batched update that included this world
+};
+
+// Launch arguments of a world (its current frame, buffers and layout).
+VoxK vox_args(const rtgi_world* w);
+TraceK trace_args(const rtgi_world* w);
+// Waits for the world's own stream and for the last batched update that included it.
+hipError_t world_wait(rtgi_world* w);
+// Device bytes a world of these parameters holds once a scene of this size is uploaded (rtgi_world_estimate_bytes).
+uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32_t nverts, uint32_t ntris,
+                     uint32_t nmat);
+
+// Block prefix arrays of a batched launch over worlds[0..n): start holds BL_COUNT * (n + 1) entries (launch-major);
+// fills p.n, p.grid, p.mark_lds and p.stats (the caller sets p.w and p.start to device copies).
+void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p);
+// The stage launches of one batched update on stream s; ev (5 events or null) brackets the four stages.
+int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev);

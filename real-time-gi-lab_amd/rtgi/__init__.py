@@ -22,8 +22,8 @@ from . import _lib as _L
 from ._lib import RtgiError, check, lib, ptr
 
 __all__ = ["VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World", "Batch",
-           "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY", "device_count", "oct_solid_angles",
-           "lib"]
+           "Server", "estimate_bytes", "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY",
+           "device_count", "oct_solid_angles", "lib"]
 
 SUN_E = (9.0, 8.5, 7.8)
 SKY = (0.45, 0.6, 0.9)
@@ -148,16 +148,42 @@ def oct_solid_angles(res, samples=1 << 20):
     return out
 
 
+def _vox_c(vox):
+    return _L.VoxParamsC(vox.v0, vox.levels, vox.G, vox.maxBricks, vox.maxVoxels)
+
+
+def _rc_c(rc):
+    flags = ((_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
+             | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0))
+    return _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
+                        rc.voxSkip, rc.resCap, flags)
+
+
+def _scene_arrays(scene):
+    return (np.ascontiguousarray(scene.pos, np.float32), np.ascontiguousarray(scene.nrm, np.float32),
+            np.ascontiguousarray(scene.tri, np.uint32), np.ascontiguousarray(scene.materials, np.float32))
+
+
+def estimate_bytes(vox: VoxParams, rc: RcParams, nverts, ntris, nmat):
+    """Device bytes of a world with a scene of this size (rtgi_world_estimate_bytes), without allocating."""
+    out = C.c_uint64()
+    check(lib().rtgi_world_estimate_bytes(C.byref(_vox_c(vox)), C.byref(_rc_c(rc)), nverts, ntris, nmat,
+                                          C.byref(out)))
+    return out.value
+
+
+def _counters_dict(c, NL):
+    return dict(bricks=c.bricks, marked=c.marked, voxels=c.voxels, brickOverflow=c.brick_overflow,
+                bigOverflow=c.big_overflow, voxelOverflow=c.voxel_overflow,
+                bricksPerLevel=list(c.bricks_per_level[:NL]), bigPerLevel=list(c.big_per_level[:NL]))
+
+
 class World:
     """One GI world on the current GPU: createVoxWorld + createRc of the reference, backed by an rtgi_world."""
 
     def __init__(self, vox: VoxParams, rc: RcParams):
         self.vox, self.rc = vox, rc
-        vp = _L.VoxParamsC(vox.v0, vox.levels, vox.G, vox.maxBricks, vox.maxVoxels)
-        flags = ((_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
-                 | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0))
-        rp = _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
-                          rc.voxSkip, rc.resCap, flags)
+        vp, rp = _vox_c(vox), _rc_c(rc)
         h = C.c_void_p()
         check(lib().rtgi_world_create(C.byref(vp), C.byref(rp), C.byref(h)))
         self._h = h
@@ -240,10 +266,7 @@ class World:
         """VoxWorld.readCounters (src/core/voxel.ts:908-918)."""
         c = _L.CountersC()
         check(lib().rtgi_world_read_counters(self._h, C.byref(c)))
-        NL = self.vox.levels
-        return dict(bricks=c.bricks, marked=c.marked, voxels=c.voxels, brickOverflow=c.brick_overflow,
-                    bigOverflow=c.big_overflow, voxelOverflow=c.voxel_overflow,
-                    bricksPerLevel=list(c.bricks_per_level[:NL]), bigPerLevel=list(c.big_per_level[:NL]))
+        return _counters_dict(c, self.vox.levels)
 
     def used_bytes(self, c=None):
         """VoxWorld.usedBytes (src/core/voxel.ts:919-922) for this layout: persistent and build-temporary bytes."""
@@ -316,6 +339,86 @@ class Batch:
     def destroy(self):
         if self._h:
             lib().rtgi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+class Server:
+    """Request server (rtgi_server_*): GI-update requests over registered worlds, continuously batched, with the
+    worlds resident on the device under a memory budget (0: 90% of the free device memory)."""
+
+    def __init__(self, budget=0, max_batch=64, slots=2):
+        p = _L.ServerParamsC(budget, max_batch, slots)
+        h = C.c_void_p()
+        check(lib().rtgi_server_create(C.byref(p), C.byref(h)))
+        self._h = h
+        self._worlds = {}   # id -> (VoxParams, RcParams)
+        self._out = {}      # tag -> (sh array, counters struct, world id)
+        self._next = 1
+
+    def add_world(self, scene, vox: VoxParams, rc: RcParams):
+        pos, nrm, tri, mat = _scene_arrays(scene)
+        i = C.c_int32()
+        check(lib().rtgi_server_add_world(self._h, C.byref(_vox_c(vox)), C.byref(_rc_c(rc)), ptr(pos), ptr(nrm),
+                                          pos.shape[0], ptr(tri), tri.shape[0], ptr(mat), mat.shape[0], C.byref(i)))
+        self._worlds[i.value] = (vox, rc)
+        return i.value
+
+    def submit(self, world, frame, sh=False, counters=False, tag=None):
+        """Queues one update of `world` at `frame`; returns its tag. sh / counters: capture those outputs."""
+        if tag is None:
+            tag = self._next
+            self._next += 1
+        vox, rc = self._worlds[world]
+        sh_arr = np.zeros((rc.probes0 ** 3, 27), np.float32) if sh else None
+        cnt = _L.CountersC() if counters else None
+        r = _L.RequestC(world, tag, frame.to_c(), ptr(sh_arr) if sh else None,
+                        C.cast(C.pointer(cnt), C.c_void_p) if counters else None)
+        self._out[tag] = (sh_arr, cnt, world)
+        try:
+            check(lib().rtgi_server_submit(self._h, C.byref(r)))
+        except Exception:
+            del self._out[tag]
+            raise
+        return tag
+
+    def poll(self, max_n=1024, wait_ms=0):
+        """Completed requests (oldest first): dicts with tag, world, status, batch, t_submit/t_launch/t_done (s),
+        latency_ms and the captured sh / counters."""
+        buf = (_L.CompletionC * max_n)()
+        n = C.c_int32()
+        check(lib().rtgi_server_poll(self._h, buf, max_n, wait_ms, C.byref(n)))
+        out = []
+        for c in buf[:n.value]:
+            sh_arr, cnt, world = self._out.pop(c.tag, (None, None, c.world))
+            d = dict(tag=c.tag, world=c.world, status=c.status, batch=c.batch_size, t_submit=c.t_submit,
+                     t_launch=c.t_launch, t_done=c.t_done, latency_ms=(c.t_done - c.t_submit) * 1e3, sh=sh_arr,
+                     counters=_counters_dict(cnt, self._worlds[world][0].levels) if cnt is not None else None)
+            out.append(d)
+        return out
+
+    def drain(self):
+        check(lib().rtgi_server_drain(self._h))
+
+    def stats(self):
+        st = _L.ServerStatsC()
+        check(lib().rtgi_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def destroy(self):
+        if self._h:
+            lib().rtgi_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

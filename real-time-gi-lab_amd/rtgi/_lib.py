@@ -67,6 +67,27 @@ class TraceStatsC(C.Structure):
                 ("capped", C.c_uint64 * MAX_CASCADES)]
 
 
+class ServerParamsC(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("slots", C.c_int32)]
+
+
+class RequestC(C.Structure):
+    _fields_ = [("world", C.c_int32), ("tag", C.c_uint64), ("frame", FrameC), ("sh_out", C.c_void_p),
+                ("counters_out", C.c_void_p)]
+
+
+class CompletionC(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("world", C.c_int32), ("status", C.c_int32), ("batch_size", C.c_int32),
+                ("t_submit", C.c_double), ("t_launch", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("admissions", C.c_uint64), ("evictions", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("budget", C.c_uint64), ("worlds", C.c_int32),
+                ("resident_worlds", C.c_int32), ("max_batch_seen", C.c_int32), ("queued", C.c_int32)]
+
+
 class SceneParamsC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("frame", C.c_int32), ("extent", C.c_double),
                 ("buildings", C.c_int32), ("detail", C.c_double)]
@@ -109,6 +130,18 @@ SIGNATURES = {
     "rtgi_batch_set_timing": (C.c_int, [_P, C.c_int32]),
     "rtgi_batch_stage_ms": (C.c_int, [_P, _P]),
     "rtgi_batch_destroy": (C.c_int, [_P]),
+    "rtgi_world_estimate_bytes": (C.c_int, [C.POINTER(VoxParamsC), C.POINTER(RcParamsC), C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.POINTER(C.c_uint64)]),
+    "rtgi_server_create": (C.c_int, [C.POINTER(ServerParamsC), C.POINTER(_P)]),
+    "rtgi_server_add_world": (C.c_int, [_P, C.POINTER(VoxParamsC), C.POINTER(RcParamsC), _P, _P, C.c_uint32, _P,
+                                        C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_int32)]),
+    "rtgi_server_submit": (C.c_int, [_P, C.POINTER(RequestC)]),
+    "rtgi_server_poll": (C.c_int, [_P, C.POINTER(CompletionC), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+    "rtgi_server_set_callback": (C.c_int, [_P, _P, _P]),
+    "rtgi_server_drain": (C.c_int, [_P]),
+    "rtgi_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStatsC)]),
+    "rtgi_server_world": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "rtgi_server_destroy": (C.c_int, [_P]),
     "rtgi_scene_generate": (C.c_int, [C.POINTER(SceneParamsC), C.POINTER(C.POINTER(SceneC))]),
     "rtgi_scene_free": (C.c_int, [C.POINTER(SceneC)]),
 }

@@ -1,6 +1,7 @@
 """CPU-side checks of the boundary: the library loads, exports exactly the declared C ABI, validates arguments and
 reports errors the reference's way (synchronous failure with a message), and the scene generator is deterministic."""
 import ctypes as C
+import dataclasses
 import os
 import re
 import subprocess
@@ -50,6 +51,23 @@ def test_version_and_errors_without_gpu():
     with pytest.raises(rtgi.RtgiError, match="null"):
         rtgi._lib.check(rtgi.lib().rtgi_world_sync(None))
     assert rtgi.lib().rtgi_last_error() != b""
+
+
+def test_estimate_bytes_and_server_without_gpu():
+    """The byte estimate is pure host arithmetic (the admission test of the server's budget); creating a server needs
+    a device and says so."""
+    vox, rc = rtgi.CONFIGS["city"]["vox"], rtgi.CONFIGS["city"]["rc"]
+    b0 = rtgi.estimate_bytes(vox, rc, 0, 0, 0)
+    b1 = rtgi.estimate_bytes(vox, rc, 1000, 2000, 2000)
+    # Scene buffers: 2 x 12 B per vertex, 16 B + 2 x 4 levels x 4 B per triangle, 16 B per material.
+    assert b1 - b0 == 1000 * 24 + 2000 * (16 + 2 * 4 * 4) + 2000 * 16
+    # Cascades: 8,126,464 rays of f32 rgba; f16 halves them, and the SH planes.
+    f16 = rtgi.estimate_bytes(vox, dataclasses.replace(rc, fp16=True), 0, 0, 0)
+    assert b0 - f16 == 8126464 * 8 + 64 ** 3 * 27 * 2
+    with pytest.raises(rtgi.RtgiError, match="at most 8 levels"):
+        rtgi.estimate_bytes(dataclasses.replace(vox, levels=9), rc, 0, 0, 0)
+    with pytest.raises(rtgi.RtgiError, match="no HIP device"):
+        rtgi.Server()
 
 
 def test_vox_params_rejects_unknown_keys():
