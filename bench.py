@@ -11,6 +11,11 @@ A step updates every world once. The value is the batched path's aggregate GI up
 update is the time from the step's submission to the completion of the batch that holds its world (p50/p99 over all
 updates of the timed steps). Inputs are resident in HBM before timing starts.
 
+Serving (the "serving" field; --serve-seconds 0 skips it): a Poisson stream of update requests (--serve-rate per
+second) over a resident mix of --serve-worlds seeded worlds of different sizes (rtgi.workloads), FP16 storage, through
+the request server (continuous batching under a budget of 128 MiB per world); p50/p99 latency from submission to
+completion, beside the same request trace run serially on the plain FP32 path.
+
 Prints one JSON line. Only --gpus 1 is supported: this path runs on one GPU.
 """
 import argparse
@@ -50,6 +55,97 @@ def pct(x, q):
     return round(float(np.percentile(np.asarray(x) * 1e3, q)), 3)
 
 
+def serve_bench(a):
+    """Serving: a Poisson stream of update requests over a resident mix of worlds (10% tail of 2M-triangle blocks),
+    FP16 storage, admitted under a budget of 128 MiB per world; then a prefix of the same request trace through the
+    plain FP32 path, one update at a time."""
+    import dataclasses
+
+    import rtgi
+    from rtgi.workloads import world_mix
+
+    t0 = time.perf_counter()
+    mix = world_mix(a.serve_worlds, seed=7)
+    budget = 128 * 2 ** 20 * a.serve_worlds
+    srv = rtgi.Server(budget=budget, max_batch=a.serve_batch, slots=2)
+    ids = [srv.add_world(w.scene, w.vox, w.rc) for w in mix]
+    rng = np.random.default_rng(11)
+    for i in ids:  # warm-up: every world once (admission, first build)
+        srv.submit(i, mix[i].frame(), counters=True)
+    srv.drain()
+    warm = srv.poll(4096)
+    overflowing = sum(1 for c in warm if c["counters"]["brickOverflow"] or c["counters"]["voxelOverflow"]
+                      or c["counters"]["bigOverflow"])
+    st0 = srv.stats()
+    log(f"serving setup: {len(mix)} worlds ({sum(w.tail for w in mix)} tail), "
+        f"{sum(w.scene.ntris for w in mix) / 1e6:.1f} M triangles, budget {budget / 2 ** 30:.1f} GiB, "
+        f"{time.perf_counter() - t0:.1f} s")
+    n_req = int(a.serve_rate * a.serve_seconds)
+    gaps = rng.exponential(1.0 / a.serve_rate, n_req)
+    arrive = np.cumsum(gaps)
+    worlds = rng.integers(0, len(mix), n_req)
+    trace = [(int(worlds[k]), mix[int(worlds[k])].frame(rng)) for k in range(n_req)]
+    done = []
+    ts = time.perf_counter()
+    for k, (wi, fr) in enumerate(trace):
+        while True:
+            now = time.perf_counter() - ts
+            if now >= arrive[k]:
+                break
+            if arrive[k] - now > 5e-4:
+                time.sleep(min(arrive[k] - now - 4e-4, 2e-3))
+        srv.submit(ids[wi], fr)
+        if k % 64 == 63:
+            done.extend(srv.poll(4096))
+            log(f"serving: {k + 1}/{n_req} submitted, {len(done)} done") if k % 2048 == 2047 else None
+    srv.drain()
+    done.extend(srv.poll(n_req + 16))
+    st = srv.stats()
+    srv.destroy()
+    lat = [c["t_done"] - c["t_submit"] for c in done]
+    span = max(c["t_done"] for c in done) - min(c["t_submit"] for c in done)
+    value = len(done) / span
+    # Plain FP32 path, serially: the first requests of the same trace, one world update at a time.
+    k_plain = min(n_req, a.serve_plain)
+    plain_worlds, plain_lat = {}, []
+    for wi, fr in trace[:k_plain]:
+        if wi not in plain_worlds:
+            m = mix[wi]
+            w = rtgi.World(m.vox, dataclasses.replace(m.rc, fp16=False))
+            w.upload_scene(m.scene)
+            w.set_frame(fr)
+            w.encode()
+            w.sync()
+            plain_worlds[wi] = w
+    tp = time.perf_counter()
+    for wi, fr in trace[:k_plain]:
+        t = time.perf_counter()
+        w = plain_worlds[wi]
+        w.set_frame(fr)
+        w.encode()
+        w.sync()
+        plain_lat.append(time.perf_counter() - t)
+    plain_value = k_plain / (time.perf_counter() - tp)
+    for w in plain_worlds.values():
+        w.destroy()
+    log(f"serving: {value:.1f} updates/s (offered {a.serve_rate}/s), p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms; "
+        f"plain FP32 serial {plain_value:.1f} updates/s")
+    return {
+        "workload": f"{len(mix)} resident mixed worlds (atrium/city, 10k-500k triangles log-uniform, clipmap edge "
+                    f"32-128 x 4 levels, 3-5 cascades, FP16 storage; every 10th a 2M-triangle 400 m block), Poisson "
+                    f"arrivals at {a.serve_rate:g}/s for {a.serve_seconds:g} s, budget 128 MiB per world",
+        "requests": len(done), "updates_per_s": round(value, 2), "offered_per_s": a.serve_rate,
+        "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": pct(lat, 100)},
+        "mean_batch": round(len(done) / max(1, st["batches"] - st0["batches"]), 2),
+        "admissions": st["admissions"] - st0["admissions"], "evictions": st["evictions"] - st0["evictions"],
+        "peak_resident_gib": round(st["peak_resident_bytes"] / 2 ** 30, 3), "budget_gib": round(budget / 2 ** 30, 3),
+        "triangles_total_m": round(sum(w.scene.ntris for w in mix) / 1e6, 2), "worlds_overflowing_pools": overflowing,
+        "plain_fp32_serial": {"updates_per_s": round(plain_value, 2), "requests": k_plain,
+                              "latency_ms": {"p50": pct(plain_lat, 50), "p99": pct(plain_lat, 99)}},
+        "speedup_vs_plain": round(value / plain_value, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +157,11 @@ def main():
     ap.add_argument("--config", default="city")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
+    ap.add_argument("--serve-worlds", type=int, default=64)
+    ap.add_argument("--serve-batch", type=int, default=64, help="server max worlds per batched launch")
+    ap.add_argument("--serve-plain", type=int, default=400, help="requests of the trace run on the plain path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -158,6 +259,12 @@ def main():
     achieved = alg_bytes / (stage["trace"] * 1e-3) / 1e9
     cnt = worlds[0].read_counters()
 
+    for b in batches:
+        b.destroy()
+    for w in worlds:
+        w.destroy()
+    serving = serve_bench(a) if a.serve_seconds > 0 else None
+
     cpu = None
     if not a.no_cpu:
         sc, vox, rc, frame = first
@@ -195,11 +302,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
-    for b in batches:
-        b.destroy()
-    for w in worlds:
-        w.destroy()
     print(json.dumps(out), flush=True)
 
 

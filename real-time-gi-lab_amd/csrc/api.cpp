@@ -120,6 +120,7 @@ VoxK vox_args(const rtgi_world* w) {
   k.plane = w->plane;
   k.big_vox = w->big_vox;
   k.big_mark = w->big_mark;
+  k.big_cap = big_cap_for(w->ntris);
   k.pos = w->pos;
   k.nrm = w->nrm;
   k.tri = w->tri;
@@ -189,8 +190,8 @@ void free_all(rtgi_world* w) {
   dfree(w, w->rep, mb * 512);
   dfree(w, w->attr, mv * 2);
   dfree(w, w->plane, mv);
-  dfree(w, w->big_vox, (size_t)w->NL * w->cap_tris);
-  dfree(w, w->big_mark, (size_t)w->NL * w->cap_tris);
+  dfree(w, w->big_vox, (size_t)w->NL * big_cap_for(w->cap_tris));
+  dfree(w, w->big_mark, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->tri, (size_t)w->cap_tris * 4);
   dfree(w, w->pos, (size_t)w->cap_verts * 3);
   dfree(w, w->nrm, (size_t)w->cap_verts * 3);
@@ -342,7 +343,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
                nbytes<DirEntry>(w.dir_off[w.N]) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
   if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
   if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
-  if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * ntris);
+  if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
   return b;
@@ -434,12 +435,12 @@ extern "C" int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const fl
   int e = RTGI_OK;
   if (ntris > w->cap_tris) {
     dfree(w, w->tri, (size_t)w->cap_tris * 4);
-    dfree(w, w->big_vox, (size_t)w->NL * w->cap_tris);
-    dfree(w, w->big_mark, (size_t)w->NL * w->cap_tris);
+    dfree(w, w->big_vox, (size_t)w->NL * big_cap_for(w->cap_tris));
+    dfree(w, w->big_mark, (size_t)w->NL * big_cap_for(w->cap_tris));
     w->cap_tris = 0;
     if (!e) e = dalloc(w, &w->tri, (size_t)ntris * 4);
-    if (!e) e = dalloc(w, &w->big_vox, (size_t)w->NL * ntris);
-    if (!e) e = dalloc(w, &w->big_mark, (size_t)w->NL * ntris);
+    if (!e) e = dalloc(w, &w->big_vox, (size_t)w->NL * big_cap_for(ntris));
+    if (!e) e = dalloc(w, &w->big_mark, (size_t)w->NL * big_cap_for(ntris));
     if (e) return e;
     w->cap_tris = ntris;
   }

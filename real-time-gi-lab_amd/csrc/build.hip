@@ -136,14 +136,20 @@ __device__ __forceinline__ uint32_t lane_below(uint64_t m) {
 }
 
 // Wavefront-aggregated append: lanes with `want` get consecutive slots of list[] past *counter; one atomic per wave.
-__device__ __forceinline__ void wave_append(bool want, uint32_t* counter, uint32_t* list, uint32_t value) {
+// The counter counts every append (it may pass cap); entries at or past cap are dropped and counted in *ovf.
+__device__ __forceinline__ void wave_append(bool want, uint32_t* counter, uint32_t* list, uint32_t cap, uint32_t* ovf,
+                                            uint32_t value) {
   uint64_t m = __ballot(want);
   if (m == 0) return;
   int leader = __ffsll((unsigned long long)m) - 1;
   uint32_t base = 0;
   if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
   base = __shfl(base, leader);
-  if (want) list[base + lane_below(m)] = value;
+  uint32_t slot = base + lane_below(m);
+  bool keep = want && slot < cap;
+  if (keep) list[slot] = value;
+  uint64_t lost = __ballot(want && !keep);
+  if (lost && __lane_id() == leader) atomicAdd(ovf, (uint32_t)__popcll(lost));
 }
 
 // Mark of brick pointer i: a bit of the workgroup's LDS bitmap, or (no bitmap) the pointer itself.
@@ -221,7 +227,8 @@ __device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t n
           work = !big;
         }
       }
-      wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.ntris, t);
+      wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.big_cap, k.big_cap, &k.hdr[H_BIG_OVF],
+                  t);
       if (work) mark_range(k, bits, T, L, r, 0, 1);
     }
   }
@@ -236,7 +243,7 @@ __global__ void __launch_bounds__(BLK) k_mark_b(BatchK b) {
 // One wavefront per listed (triangle, level); entries of all levels are taken in level order.
 __device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hdr_base, int& L, uint32_t& idx) {
   for (L = 0; L < k.NL; L++) {
-    uint32_t n = k.hdr[hdr_base + L];
+    uint32_t n = min(k.hdr[hdr_base + L], k.big_cap);
     if (e < n) {
       idx = e;
       return true;
@@ -248,7 +255,7 @@ __device__ __forceinline__ bool big_entry(const VoxK& k, uint32_t e, uint32_t hd
 
 __device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t total = 0;
-  for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGMARK_L + L];
+  for (int L = 0; L < k.NL; L++) total += min(k.hdr[H_BIGMARK_L + L], k.big_cap);
   if (total == 0) return;  // uniform across the grid
   uint32_t* bits = k.lds_marks ? lds_bits : nullptr;
   if (bits) bits_clear(k);
@@ -258,7 +265,7 @@ __device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32
     int L;
     uint32_t idx;
     big_entry(k, e, H_BIGMARK_L, L, idx);
-    uint32_t t = k.big_mark[(size_t)L * k.ntris + idx];
+    uint32_t t = k.big_mark[(size_t)L * k.big_cap + idx];
     Tri T = load_tri(k, t);
     mark_range(k, bits, T, L, vox_range(k, T, L, true), (int)lane, 64);
   }
@@ -481,7 +488,8 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
           work = !big;
         }
       }
-      wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.ntris, t);
+      wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.big_cap, k.big_cap, &k.hdr[H_BIG_OVF],
+                  t);
       if (work) voxelize_range(k, hkey, hval, T, t, L, r, n, drops);
     }
   }
@@ -496,7 +504,7 @@ __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
 // voxelizeBig (src/core/voxel.ts:515-531): one wavefront per listed (triangle, level), one cell per lane.
 __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t total = 0;
-  for (int L = 0; L < k.NL; L++) total += k.hdr[H_BIGVOX_L + L];
+  for (int L = 0; L < k.NL; L++) total += min(k.hdr[H_BIGVOX_L + L], k.big_cap);
   uint32_t drops = k.hdr[H_BRICK_OVF];
   uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
   uint32_t waves = (nblk * BLK) >> 6;
@@ -504,7 +512,7 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
     int L;
     uint32_t idx;
     big_entry(k, e, H_BIGVOX_L, L, idx);
-    uint32_t t = k.big_vox[(size_t)L * k.ntris + idx];
+    uint32_t t = k.big_vox[(size_t)L * k.big_cap + idx];
     Tri T = load_tri(k, t);
     Range r = vox_range(k, T, L, false);
     int n = r.n.x * r.n.y * r.n.z;

@@ -35,6 +35,11 @@ constexpr uint32_t ATTR_NONE = 0xffffffffu;
 //   rep    [max_bricks][512]   representative triangle + 1 per voxel (build temporary, kept for readback)
 //   attr   [max_voxels][2]     pack4x8unorm albedo, pack2x16snorm oct normal
 //   plane  [max_voxels]        representative plane offset from the voxel centre
+// Capacity of a level's big-triangle list (BIG_CAP, src/core/voxel.ts:224); entries past it are dropped and counted
+// in H_BIG_OVF, as the reference does.
+constexpr uint32_t BIG_CAP = 1u << 18;
+inline uint32_t big_cap_for(uint32_t ntris) { return ntris < BIG_CAP ? ntris : BIG_CAP; }
+
 struct VoxK {
   float v0;
   float inv_v0;   // 1 / v0, used only when pow2 (then x / v == x * (1/v) exactly)
@@ -50,13 +55,14 @@ struct VoxK {
   uint32_t* rep;
   uint32_t* attr;
   float* plane;
-  uint32_t* big_vox;   // [levels][ntris]
-  uint32_t* big_mark;  // [levels][ntris]
+  uint32_t* big_vox;   // [levels][big_cap]
+  uint32_t* big_mark;  // [levels][big_cap]
   const float* pos;
   const float* nrm;
   const uint32_t* tri;  // 4 words per triangle
   const float* mat;     // 4 floats per material
   uint32_t ntris;
+  uint32_t big_cap;    // entries per level of each big list: min(ntris, BIG_CAP)
   int32_t lds_marks;    // brick marks accumulate in an LDS bitmap of levels * G^3 bits (0: global fallback)
   unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
 };
