@@ -199,6 +199,14 @@ int rtgi_world_read_clipmap(rtgi_world* w, int32_t level, uint32_t* occ_bits, ui
  * the rad buffer (src/core/rc.ts:1064). Synchronises. */
 int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hits);
 
+/* Ray query over the clipmap of the last update: n rays from origins o along unit directions d (3 floats each)
+ * traced as rays of cascade c (its interval [b_c, b_c+1] s0, level voxLevelFor and skip), shaded as the cascade trace
+ * (traceWorld -> traceVox -> voxShade, src/core/rc.ts:216-222, src/core/voxel.ts:121-202) -> rgba (4 floats: radiance,
+ * transmittance; f16-rounded in the precision form) and hits (first-hit voxel ids, may be NULL). Synchronous. The
+ * reference has no ray query entry point; this one serves error measurements on arbitrary (non-texel) directions. */
+int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float* d, uint32_t n, float* rgba,
+                          uint32_t* hits);
+
 /* Tracer statistics of the last update (RTGI_RC_STATS): per cascade the world traces, the DDA loop iterations and
  * the traces that reached the 1024-iteration cap (the reference's counts[24..32), [32..40), [64..72), rc.ts:232-235). */
 typedef struct rtgi_trace_stats {

@@ -50,6 +50,10 @@ _SIG = {
     "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
     "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int]),
+    "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
+                                 C.c_int]),
+    "orc_trace_rays_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
+                                     _P, C.c_double, C.c_int]),
     "orc_merge_sh": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
     "orc_trace_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
                                 C.c_double, C.c_int]),
@@ -232,6 +236,29 @@ class Oracle:
         tie = np.zeros(r1 - r0, np.uint8)
         lib().orc_trace_f64(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgbt), _p(hits), _p(tie),
                             eps, threads)
+        return rgbt, hits, tie
+
+    def trace_rays(self, c, o, d, threads=0):
+        """Rays from origins o along unit directions d (n x 3 float32) traced as cascade c's rays (f32)."""
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = o.shape[0]
+        rgba = np.zeros((n, 4), np.float32)
+        hits = np.zeros(n, np.uint32)
+        assert lib().orc_trace_rays(C.byref(self.vp), _p(self.buf), C.byref(self.rc), c, _p(o), _p(d), n, _p(rgba),
+                                    _p(hits), threads) == 0
+        return rgba, hits
+
+    def trace_rays_f64(self, c, o, d, eps=1e-6, threads=0):
+        """The same rays in double: rgbt, hit ids, tie flags."""
+        o = np.ascontiguousarray(o, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        n = o.shape[0]
+        rgbt = np.zeros((n, 4), np.float64)
+        hits = np.zeros(n, np.uint32)
+        tie = np.zeros(n, np.uint8)
+        assert lib().orc_trace_rays_f64(C.byref(self.vp), _p(self.buf), C.byref(self.rc), c, _p(o), _p(d), n,
+                                        _p(rgbt), _p(hits), _p(tie), eps, threads) == 0
         return rgbt, hits, tie
 
     def merge_sh_f64(self, rgbt):

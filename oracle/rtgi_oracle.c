@@ -845,6 +845,15 @@ static int cascadeOf(const orc_rc* rc, uint64_t ray, uint64_t* local) {
 }
 
 /* trace (rc.ts:600-622) -> traceWorld (rc.ts:216-245) with the probe at its grid point, normal 0. */
+/* One ray of cascade c from o along d (traceWorld, rc.ts:216-222): interval [b0, b1] s0, level voxLevelFor. */
+static void traceFrom(const View* W, const orc_rc* rc, int c, v3 o, v3 d, float out[4], uint32_t* hid,
+                      uint32_t* steps) {
+  float t0 = rc->b0[c] * rc->s0, t1 = rc->b1[c] * rc->s0;
+  uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
+  float v = voxSize(W->p, L);
+  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
+}
+
 static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid, uint32_t* steps) {
   uint64_t r = 0;
   int c = cascadeOf(rc, ray, &r);
@@ -854,12 +863,7 @@ static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4]
   int di = (int)(r % dirs);
   iv3 g = I3((rc->G0[0] >> c) + (int)(pi % n), (rc->G0[1] >> c) + (int)((pi / n) % n), (rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n)));
   float s = rc->s0 * (float)(1u << c);
-  v3 o = vscale(vf(g), s);
-  v3 d = dirOf(di, res);
-  float t0 = rc->b0[c] * rc->s0, t1 = rc->b1[c] * rc->s0;
-  uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
-  float v = voxSize(W->p, L);
-  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
+  traceFrom(W, rc, c, vscale(vf(g), s), dirOf(di, res), out, hid, steps);
 }
 
 static void makeView(View* W, const orc_vox* p, const uint32_t* vox) {
@@ -884,6 +888,28 @@ int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t 
     for (int k = 0; k < 4; k++) out[k] = q16(rc, out[k]); /* packRad of the trace output (rc.ts:621) */
     if (rgba) memcpy(rgba + (size_t)(q - (int64_t)r0) * 4, out, 16);
     if (hits) hits[q - (int64_t)r0] = h;
+  }
+  (void)threads;
+  return 0;
+}
+
+int orc_trace_rays(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const float* o, const float* d,
+                   uint64_t n, float* rgba, uint32_t* hits, int threads) {
+  if (c < 0 || c >= rc->cascades) return -1;
+  View W;
+  makeView(&W, p, vox);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    float out[4];
+    uint32_t h, st;
+    traceFrom(&W, rc, c, V3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), V3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), out, &h,
+              &st);
+    for (int k = 0; k < 4; k++) out[k] = q16(rc, out[k]);
+    if (rgba) memcpy(rgba + (size_t)i * 4, out, 16);
+    if (hits) hits[i] = h;
   }
   (void)threads;
   return 0;
@@ -1084,6 +1110,33 @@ static void traceVoxD(const View* W, const orc_rc* rc, d3 o, d3 d, double t0, do
   }
 }
 
+/* One ray of cascade c in double: the level (the smallest level >= voxLevel whose window strictly contains the
+ * interval's end), then the traversal. tie: as traceVoxD, or the end lies within eps of a window face. */
+static void traceRayD(const View* W, const orc_rc* rc, int c, d3 o, d3 d, double eps, double out[4], uint32_t* h,
+                      uint8_t* tie) {
+  const orc_vox* p = W->p;
+  double t0 = (double)rc->b0[c] * rc->s0, t1 = (double)rc->b1[c] * rc->s0;
+  d3 e = D3(o.x + d.x * t1, o.y + d.y * t1, o.z + d.z * t1);
+  uint32_t L = (uint32_t)rc->vox_level[c];
+  uint8_t tl = 0;
+  while (L + 1 < (uint32_t)p->levels) {
+    double bs = p->v0 * (double)(1u << L) * 8;
+    double ev[3] = {e.x, e.y, e.z};
+    int in = 1;
+    for (int a = 0; a < 3; a++) {
+      double lo = p->origin[L][a] * bs, hi = lo + p->G * bs;
+      if (near(ev[a], lo, eps * (fabs(lo) + 1)) || near(ev[a], hi, eps * (fabs(hi) + 1))) tl = 1;
+      if (!(ev[a] > lo && ev[a] < hi)) in = 0;
+    }
+    if (in) break;
+    L++;
+  }
+  double v = p->v0 * (double)(1u << L);
+  uint8_t tt;
+  traceVoxD(W, rc, o, d, t0, t1, (double)rc->vox_skip * v, L, eps, out, h, &tt);
+  *tie = (uint8_t)(tt | tl);
+}
+
 int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, double* rgbt,
                   uint32_t* hits, uint8_t* tie, double eps, int threads) {
   View W;
@@ -1101,33 +1154,38 @@ int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint6
     double s = (double)rc->s0 * (double)(1u << c);
     d3 o = D3(((rc->G0[0] >> c) + (int)(pi % n)) * s, ((rc->G0[1] >> c) + (int)((pi / n) % n)) * s,
               ((rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n))) * s);
-    d3 d = dirOfD(di, res);
-    double t0 = (double)rc->b0[c] * rc->s0, t1 = (double)rc->b1[c] * rc->s0;
-    /* Level: the smallest level >= voxLevel whose window strictly contains the interval's end. */
-    d3 e = D3(o.x + d.x * t1, o.y + d.y * t1, o.z + d.z * t1);
-    uint32_t L = (uint32_t)rc->vox_level[c];
-    uint8_t tl = 0;
-    while (L + 1 < (uint32_t)p->levels) {
-      double bs = p->v0 * (double)(1u << L) * 8;
-      double ev[3] = {e.x, e.y, e.z};
-      int in = 1;
-      for (int a = 0; a < 3; a++) {
-        double lo = p->origin[L][a] * bs, hi = lo + p->G * bs;
-        if (near(ev[a], lo, eps * (fabs(lo) + 1)) || near(ev[a], hi, eps * (fabs(hi) + 1))) tl = 1;
-        if (!(ev[a] > lo && ev[a] < hi)) in = 0;
-      }
-      if (in) break;
-      L++;
-    }
-    double v = p->v0 * (double)(1u << L);
     double out[4];
     uint32_t h;
     uint8_t tt;
-    traceVoxD(&W, rc, o, d, t0, t1, (double)rc->vox_skip * v, L, eps, out, &h, &tt);
+    traceRayD(&W, rc, c, o, dirOfD(di, res), eps, out, &h, &tt);
     size_t i = (size_t)(q - (int64_t)r0);
     if (rgbt) memcpy(rgbt + i * 4, out, 32);
     if (hits) hits[i] = h;
-    if (tie) tie[i] = (uint8_t)(tt | tl);
+    if (tie) tie[i] = tt;
+  }
+  (void)threads;
+  return 0;
+}
+
+int orc_trace_rays_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const double* o,
+                       const double* d, uint64_t n, double* rgbt, uint32_t* hits, uint8_t* tie, double eps,
+                       int threads) {
+  if (c < 0 || c >= rc->cascades) return -1;
+  View W;
+  makeView(&W, p, vox);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    double out[4];
+    uint32_t h;
+    uint8_t tt;
+    traceRayD(&W, rc, c, D3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), D3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), eps, out,
+              &h, &tt);
+    if (rgbt) memcpy(rgbt + (size_t)i * 4, out, 32);
+    if (hits) hits[i] = h;
+    if (tie) tie[i] = tt;
   }
   (void)threads;
   return 0;

@@ -64,6 +64,10 @@ uint64_t orc_rays(const orc_rc* rc, int c);
  * reference's voxSteps); any may be NULL. threads: OpenMP threads (<= 0: all). */
 int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
               uint32_t* hits, uint32_t* steps, int threads);
+/* n arbitrary rays traced as rays of cascade c (its interval, level choice and skip): origins o and unit directions d,
+ * 3 floats each -> rgba (packRad-rounded in the precision form) and hit ids; either output may be NULL. */
+int orc_trace_rays(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const float* o, const float* d,
+                   uint64_t n, float* rgba, uint32_t* hits, int threads);
 /* mergeWorld N-1..0 in place on the full rgba array, then projectSH -> sh (27 floats per c0 probe). */
 int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
 
@@ -72,6 +76,10 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
  * (its first-hit voxel is then not fixed by exact arithmetic's neighbourhood). Any output may be NULL. */
 int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, double* rgbt,
                   uint32_t* hits, uint8_t* tie, double eps, int threads);
+/* The same for n arbitrary rays of cascade c (origins and directions in double). */
+int orc_trace_rays_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const double* o,
+                       const double* d, uint64_t n, double* rgbt, uint32_t* hits, uint8_t* tie, double eps,
+                       int threads);
 /* FP64 merge and SH on a full double rgbt array (4 doubles per ray). */
 int orc_merge_sh_f64(const orc_rc* rc, double* rgbt, double* sh);
 

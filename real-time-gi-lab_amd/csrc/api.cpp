@@ -710,6 +710,38 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   return RTGI_OK;
 }
 
+extern "C" int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float* d, uint32_t n, float* rgba,
+                                     uint32_t* hits) {
+  if (!w || (n && (!o || !d || !rgba))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_trace_rays: null argument");
+  if (c < 0 || c >= w->N) return rtgi_fail(RTGI_E_INVALID, "cascade %d out of range", c);
+  if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "no scene or no frame");
+  if (n == 0) return RTGI_OK;
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  float *dod = nullptr, *dout = nullptr;
+  uint32_t* dh = nullptr;
+  int e = RTGI_OK;
+  if (hipMalloc((void**)&dod, (size_t)n * 24) != hipSuccess || hipMalloc((void**)&dout, (size_t)n * 16) != hipSuccess ||
+      hipMalloc((void**)&dh, (size_t)n * 4) != hipSuccess)
+    e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc for %u rays", n);
+  if (!e && (hipMemcpy(dod, o, (size_t)n * 12, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(dod + (size_t)n * 3, d, (size_t)n * 12, hipMemcpyHostToDevice) != hipSuccess))
+    e = rtgi_fail(RTGI_E_HIP, "ray upload failed");
+  if (!e) {
+    int le = launch_trace_rays(trace_args(w), c, dod, dod + (size_t)n * 3, n, dout, dh, w->stream);
+    if (le) e = rtgi_fail(RTGI_E_HIP, "ray query launch failed: %s", hipGetErrorString((hipError_t)le));
+  }
+  if (!e && hipStreamSynchronize(w->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "ray query failed");
+  if (!e && hipMemcpy(rgba, dout, (size_t)n * 16, hipMemcpyDeviceToHost) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "readback failed");
+  if (!e && hits && hipMemcpy(hits, dh, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "readback failed");
+  if (dod) (void)hipFree(dod);
+  if (dout) (void)hipFree(dout);
+  if (dh) (void)hipFree(dh);
+  return e;
+}
+
 extern "C" int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out) {
   if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
   if (!w->stats) return rtgi_fail(RTGI_E_STATE, "tracer statistics are off (RTGI_RC_STATS)");

@@ -123,6 +123,9 @@ struct BatchK {
 size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
 int launch_trace(const TraceK& k, void* stream);
+// n rays (origins o, unit directions d; device arrays of 3 floats each) traced as rays of cascade c -> rgba, hits.
+int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, uint32_t n, float* rgba,
+                      uint32_t* hits, void* stream);
 // Threads of cascade c's merge launch.
 uint64_t merge_threads(const TraceK& k, int c);
 int launch_merge(const TraceK& k, int c, void* stream);

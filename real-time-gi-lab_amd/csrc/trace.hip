@@ -349,6 +349,32 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
     }
   }
 }
+// Ray queries: n rays from origins o along unit directions d (3 floats each) traced as rays of cascade c (its
+// interval, level choice and skip), shaded as the trace does; rgba is rounded as packRad stores it in the precision
+// form. inv = 1 / dd with dd = d where |d| >= 1e-9, else 1e-9 (traceVox, src/core/voxel.ts:128-129).
+__global__ void __launch_bounds__(BLK) k_trace_rays(TraceK a, int c, const float* o, const float* d, uint32_t n,
+                                                    float4* out, uint32_t* hits) {
+  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const CascK& C = a.c[c];
+  f3 O = mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+  f3 D = mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+  f3 dd = mk3(fabsf(D.x) < 1e-9f ? 1e-9f : D.x, fabsf(D.y) < 1e-9f ? 1e-9f : D.y, fabsf(D.z) < 1e-9f ? 1e-9f : D.z);
+  f3 inv = mk3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
+  int L = vox_level_for(a.vx, C.vox_level, O + D * t1);
+  float v = vox_size(a.vx, L);
+  Hit h = trace_vox<false>(a, O, D, inv, t0, t1, a.vox_skip * v, L);
+  float4 r = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (h.hit) {
+    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), D, O + D * h.tIn);
+    r = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
+  }
+  if (a.fp16) r = make_float4(hlo(h2(r.x, r.y)), hhi(h2(r.x, r.y)), hlo(h2(r.z, r.w)), hhi(h2(r.z, r.w)));
+  out[i] = r;
+  hits[i] = h.id;
+}
+
 // ST: count the tracer statistics (a separate instantiation, so the counting costs the plain trace nothing).
 template <bool ST>
 __global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body<ST>(a, blockIdx.x, gridDim.x); }
@@ -540,6 +566,12 @@ int launch_trace(const TraceK& k, void* stream) {
 uint64_t merge_threads(const TraceK& k, int c) {
   if (c == 0 || k.c[c].res == k.c[c - 1].res) return k.c[c].rays;
   return (uint64_t)k.c[c - 1].res * k.c[c - 1].res * ((k.c[c].np + 15u) >> 4) * 64u;
+}
+
+int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, uint32_t n, float* rgba,
+                      uint32_t* hits, void* stream) {
+  if (n) k_trace_rays<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c, o, d, n, (float4*)rgba, hits);
+  return (int)hipGetLastError();
 }
 
 int launch_merge(const TraceK& k, int c, void* stream) {

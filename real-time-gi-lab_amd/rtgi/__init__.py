@@ -296,6 +296,16 @@ class World:
         check(lib().rtgi_world_read_cascade(self._h, c, ptr(rgba), ptr(h)))
         return (rgba, h) if hits else rgba
 
+    def trace_rays(self, c, origins, dirs):
+        """Rays from origins along unit dirs (n x 3) traced as cascade c's rays over the last update's clipmap."""
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = o.shape[0]
+        rgba = np.zeros((n, 4), np.float32)
+        hits = np.zeros(n, np.uint32)
+        check(lib().rtgi_world_trace_rays(self._h, c, ptr(o), ptr(d), n, ptr(rgba), ptr(hits)))
+        return rgba, hits
+
     def read_trace_stats(self):
         """Per cascade: world traces, DDA steps, traces at the 1024-step cap (rc.ts:232-235)."""
         st = _L.TraceStatsC()

@@ -122,6 +122,7 @@ SIGNATURES = {
     "rtgi_world_read_cascade": (C.c_int, [_P, C.c_int32, _P, _P]),
     "rtgi_world_read_sh": (C.c_int, [_P, _P]),
     "rtgi_world_read_trace_stats": (C.c_int, [_P, C.POINTER(TraceStatsC)]),
+    "rtgi_world_trace_rays": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, _P, _P]),
     "rtgi_world_destroy": (C.c_int, [_P]),
     "rtgi_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
     "rtgi_batch_set_graph": (C.c_int, [_P, C.c_int32]),

@@ -334,6 +334,59 @@ def test_trace_matches_fp64_away_from_ties(s0, min_clean, sh_median):
     assert np.median(rel) < sh_median
 
 
+def sphere_rays(o, c, n, seed):
+    """n rays of cascade c: origins at uniformly drawn probe positions, directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    N = o.rc.probes0 >> c
+    g = rng.integers(0, N, (n, 3)) + np.array(o.rc.G0[:3]) // (1 << c)
+    org = (g.astype(np.float32) * np.float32(o.rc.s0 * 2 ** c)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return org, d
+
+
+def test_ray_queries_match_cascade_trace_and_fp64():
+    """orc_trace_rays on a cascade's own probe origins and table directions reproduces orc_trace bit for bit; on
+    random sphere directions its first hits equal the FP64 walk's away from ties, and radiance agrees to 1e-4."""
+    scene, vox, rc, frame = rtgi.config("atrium", s0=0.61)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    rgba, hits = o.trace(r0=0, r1=4096)
+    res = 4
+    pi, di = np.arange(4096) // (res * res), np.arange(4096) % (res * res)
+    n = rc.probes0
+    g = np.stack([pi % n, (pi // n) % n, pi // (n * n)], 1) + np.array(o.rc.G0[:3])
+    org = (g.astype(np.float32) * np.float32(rc.s0)).astype(np.float32)
+    dirs = np.array([orc_dir(int(x), res) for x in di], np.float32)
+    r2, h2 = o.trace_rays(0, org, dirs)
+    assert np.array_equal(h2, hits) and np.array_equal(r2.view(np.uint32), rgba.view(np.uint32))
+    for c in range(rc.cascades):
+        org, d = sphere_rays(o, c, 4000, seed=c)
+        r32, h32 = o.trace_rays(c, org, d)
+        r64, h64, tie = o.trace_rays_f64(c, org.astype(np.float64), d.astype(np.float64), eps=1e-5)
+        clean = tie == 0
+        # Tie flags are conservative: a start on the window's face (rays entering from outside, most of the top
+        # cascade's long intervals) counts as a tie.
+        assert clean.sum() > 500 and np.array_equal(h32[clean], h64[clean])
+        hit = clean & (h32 != 0xFFFFFFFF)
+        err = np.abs(r32[hit, :3] - r64[hit, :3]) / np.maximum(np.abs(r64[hit, :3]), 1e-3)
+        assert hit.sum() > 50 and err.max() < 1e-4
+
+
+def orc_dir(di, res):
+    """dirOf (src/core/rc.ts:291-298) in f32 with the operation order of the device."""
+    f = np.float32
+    u = (f(di % res) + f(0.5)) / f(res)
+    v = (f(di // res) + f(0.5)) / f(res)
+    fx, fy = u * f(2) - f(1), v * f(2) - f(1)
+    nz = f(1) - abs(fx) - abs(fy)
+    t = max(-nz, f(0))
+    fx = fx - t if fx >= 0 else fx + t
+    fy = fy - t if fy >= 0 else fy + t
+    ln = np.sqrt(fx * fx + fy * fy + nz * nz, dtype=np.float32)
+    return [fx / ln, fy / ln, nz / ln]
+
+
 def test_trace_empty_world_and_single_voxel():
     vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
     rc = rtgi.RcParams(cascades=2, probes0=2, s0=0.5)
