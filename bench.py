@@ -101,6 +101,14 @@ def serve_bench(a):
     srv.drain()
     done.extend(srv.poll(n_req + 16))
     st = srv.stats()
+    # Capacity: the same trace submitted at once (closed loop), completions per second until the queue drains.
+    tc = time.perf_counter()
+    for wi, fr in trace[:a.serve_capacity]:
+        srv.submit(ids[wi], fr)
+    srv.drain()
+    cap_value = a.serve_capacity / (time.perf_counter() - tc)
+    cap_done = srv.poll(a.serve_capacity + 16)
+    cap_batch = a.serve_capacity / max(1, srv.stats()["batches"] - st["batches"])
     srv.destroy()
     lat = [c["t_done"] - c["t_submit"] for c in done]
     span = max(c["t_done"] for c in done) - min(c["t_submit"] for c in done)
@@ -129,7 +137,7 @@ def serve_bench(a):
     for w in plain_worlds.values():
         w.destroy()
     log(f"serving: {value:.1f} updates/s (offered {a.serve_rate}/s), p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms; "
-        f"plain FP32 serial {plain_value:.1f} updates/s")
+        f"capacity {cap_value:.1f} updates/s; plain FP32 serial {plain_value:.1f} updates/s")
     return {
         "workload": f"{len(mix)} resident mixed worlds (atrium/city, 10k-500k triangles log-uniform, clipmap edge "
                     f"32-128 x 4 levels, 3-5 cascades, FP16 storage; every 10th a 2M-triangle 400 m block), Poisson "
@@ -142,7 +150,8 @@ def serve_bench(a):
         "triangles_total_m": round(sum(w.scene.ntris for w in mix) / 1e6, 2), "worlds_overflowing_pools": overflowing,
         "plain_fp32_serial": {"updates_per_s": round(plain_value, 2), "requests": k_plain,
                               "latency_ms": {"p50": pct(plain_lat, 50), "p99": pct(plain_lat, 99)}},
-        "speedup_vs_plain": round(value / plain_value, 3),
+        "capacity": {"updates_per_s": round(cap_value, 2), "requests": len(cap_done), "mean_batch": round(cap_batch, 2)},
+        "capacity_vs_plain": round(cap_value / plain_value, 3),
     }
 
 
@@ -162,6 +171,7 @@ def main():
     ap.add_argument("--serve-worlds", type=int, default=64)
     ap.add_argument("--serve-batch", type=int, default=64, help="server max worlds per batched launch")
     ap.add_argument("--serve-plain", type=int, default=400, help="requests of the trace run on the plain path")
+    ap.add_argument("--serve-capacity", type=int, default=4000, help="requests of the closed-loop capacity run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -201,6 +211,7 @@ def main():
         if k == 0:
             first = (sc, vox, rc, frame)
     lay = worlds[0].layout()
+    worlds_tuning = worlds[0].tuning()
     rays = lay["total_rays"]
     log(f"setup: {a.worlds} worlds of '{a.config}' in {time.perf_counter() - t0:.1f} s, {first[0].ntris} triangles "
         f"in world 1, {rays} rays per update, {lay['device_bytes'] / 2 ** 20:.0f} MiB per world")
@@ -254,8 +265,12 @@ def main():
             st[k].append(v)
     tb.destroy()
     stage = {k: float(np.median(v)) for k, v in st.items()}
-    # Roofline of the trace launch (the dominant kernel): algorithmic bytes = 16 B of f32 rgba written per ray.
-    alg_bytes = rays * 16 * len(groups[0])
+    # Roofline of the trace stage (the dominant kernel). Algorithmic bytes per world update: 16 B of f32 rgba written
+    # per ray; fused with the merge (one launch per cascade), also each cascade c >= 1's 2x2 averages (one float4 per
+    # 4 rays) written once and read once by cascade c-1.
+    per_c = lay["rays"]
+    alg_world = 16 * rays + (2 * 16 * sum(r // 4 for r in per_c[1:]) if worlds_tuning["fused"] != 0 else 0)
+    alg_bytes = alg_world * len(groups[0])
     achieved = alg_bytes / (stage["trace"] * 1e-3) / 1e9
     cnt = worlds[0].read_counters()
 
@@ -297,8 +312,11 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 3),
         "mrays_per_s": round(rays * value / 1e6, 1),
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage.items()},
+        "trace_merge_fused": worlds_tuning["fused"] != 0,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
-        "roofline": {"kernel": "k_trace (batched launch)", "bound": "hbm", "achieved": round(achieved, 2),
+        "roofline": {"kernel": ("k_trace_merge_b (5 batched launches, one per cascade)" if worlds_tuning["fused"] != 0
+                                else "k_trace_b (batched launch)"),
+                     "bytes_per_world_update": int(alg_world), "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "cpu_baseline": cpu,

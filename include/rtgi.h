@@ -223,6 +223,23 @@ int rtgi_world_read_sh(rtgi_world* w, float* sh);
 /* Frees the world's device memory (replaces VoxWorld.destroy src/core/voxel.ts:923 and Rc.destroy rc.ts:1234). */
 int rtgi_world_destroy(rtgi_world* w);
 
+/* Launch tuning of a world: choices that change speed and device storage order, never results (the readbacks are
+ * in the same order whatever the tuning). No reference counterpart (the reference's workgroup sizes are constants).
+ *   probe_tile  probes per axis of a trace block: 1 stores probes in grid rows (a wavefront traces 64 probes of a
+ *               row), 2 or 4 in blocks of 2^3 / 4^3 (a wavefront traces a compact block of parallel rays); cascades
+ *               whose probes per axis are not a power of two at least probe_tile keep rows. Default 1.
+ *   fused       1: each cascade is traced and merged in one launch, cascades N-1 .. 0 in sequence (the traced values
+ *               are never stored unmerged; the trace stage then includes the merge and the merge stage is empty);
+ *               0: one trace launch over all cascades, then the merge steps; -1 (default): fused in batches, not in
+ *               single-world updates (a single world's top cascades alone do not fill the GPU). A batch runs fused
+ *               when none of its worlds says 0. */
+typedef struct rtgi_tuning {
+  int32_t probe_tile;
+  int32_t fused;
+} rtgi_tuning;
+int rtgi_world_set_tuning(rtgi_world* w, const rtgi_tuning* t);
+int rtgi_world_get_tuning(const rtgi_world* w, rtgi_tuning* t);
+
 /* Device bytes a world of these parameters holds once a scene of nverts / ntris / nmat is uploaded (what
  * rtgi_layout.device_bytes then reads), without allocating: the admission test of a memory budget. No reference
  * counterpart (the reference sizes its GPUBuffers inside createVoxWorld / createRc, voxel.ts:753-767, rc.ts:1061-1072).

@@ -152,6 +152,7 @@ TraceK trace_args(const rtgi_world* w) {
     int rl = 0;
     while ((1 << rl) < w->res[c]) rl++;
     t.c[c].res_log2 = rl;
+    t.c[c].tile_log2 = probe_tile_log2(w, c);
     t.c[c].rays = w->ray_off[c + 1] - w->ray_off[c];
   }
   for (int a = 0; a < 3; a++) t.G0[a] = w->G0[a];
@@ -170,12 +171,38 @@ TraceK trace_args(const rtgi_world* w) {
   std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
   t.dirs = w->dirs;
   t.fp16 = (w->rp.flags & RTGI_RC_FP16) ? 1 : 0;
+  t.no_merge = (w->rp.flags & RTGI_RC_NO_MERGE) ? 1 : 0;
   t.rad = w->rad;
   t.avg = w->avg;
   t.hits = w->hits;
   t.sh = w->sh;
   t.total_rays = w->ray_off[w->N];
   return t;
+}
+
+namespace {
+
+}  // namespace
+
+int probe_tile_log2(const rtgi_world* w, int c) {
+  int T = 0;
+  while ((1 << (T + 1)) <= w->tuning.probe_tile) T++;
+  int n = w->n[c], nl = 0;
+  while ((1 << nl) < n) nl++;
+  if ((1 << nl) != n || nl < T) return 0;  // rows when n is not a power of two or is smaller than the block
+  return T;
+}
+
+uint32_t probe_index_host(const rtgi_world* w, int c, uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t n = (uint32_t)w->n[c];
+  int T = probe_tile_log2(w, c);
+  if (T == 0) return x + n * (y + n * z);
+  int nl = 0;
+  while ((1u << nl) < n) nl++;
+  int nt = nl - T;
+  uint32_t mT = (1u << T) - 1u;
+  uint32_t t = (x >> T) | ((y >> T) << nt) | ((z >> T) << (2 * nt));
+  return (t << (3 * T)) | (x & mT) | ((y & mT) << T) | ((z & mT) << (2 * T));
 }
 
 namespace {
@@ -218,10 +245,13 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   int e = 0;
   if (stage == RTGI_STAGE_VOX) {
     e = launch_vox_build(vox_args(w), s);
+  } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
+    TraceK t = trace_args(w);
+    for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
     e = launch_trace(trace_args(w), s);
   } else if (stage == RTGI_STAGE_MERGE) {
-    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) {
+    if (!(w->rp.flags & RTGI_RC_NO_MERGE) && w->tuning.fused != 1) {
       TraceK t = trace_args(w);
       for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_merge(t, c, s);
     }
@@ -688,24 +718,29 @@ extern "C" int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, ui
   HIPCHK(world_wait(w));
   size_t n = w->ray_off[c + 1] - w->ray_off[c];
   size_t np = (size_t)w->n[c] * w->n[c] * w->n[c], dirs = (size_t)w->res[c] * w->res[c];
-  // Device order is direction-major (ray = di * np + probe); the readback is probe-major.
+  // Device order is direction-major (ray = di * np + device probe index); the readback is probe-major with probes
+  // in grid order x + n (y + n z).
+  std::vector<uint32_t> dev(np);
+  for (uint32_t z = 0, p = 0; z < (uint32_t)w->n[c]; z++)
+    for (uint32_t y = 0; y < (uint32_t)w->n[c]; y++)
+      for (uint32_t x = 0; x < (uint32_t)w->n[c]; x++, p++) dev[p] = probe_index_host(w, c, x, y, z);
   if (rgba && (w->rp.flags & RTGI_RC_FP16)) {
     std::vector<uint16_t> t(n * 4);
     HIPCHK(hipMemcpy(t.data(), (const uint16_t*)w->rad + w->ray_off[c] * 4, n * 8, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < dirs; d++)
       for (size_t p = 0; p < np; p++)
-        for (int q = 0; q < 4; q++) rgba[(p * dirs + d) * 4 + q] = half_to_float(t[(d * np + p) * 4 + q]);
+        for (int q = 0; q < 4; q++) rgba[(p * dirs + d) * 4 + q] = half_to_float(t[(d * np + dev[p]) * 4 + q]);
   } else if (rgba) {
     std::vector<float> t(n * 4);
     HIPCHK(hipMemcpy(t.data(), w->rad + w->ray_off[c] * 4, n * 16, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < dirs; d++)
-      for (size_t p = 0; p < np; p++) std::memcpy(rgba + (p * dirs + d) * 4, &t[(d * np + p) * 4], 16);
+      for (size_t p = 0; p < np; p++) std::memcpy(rgba + (p * dirs + d) * 4, &t[(d * np + dev[p]) * 4], 16);
   }
   if (hits) {
     std::vector<uint32_t> t(n);
     HIPCHK(hipMemcpy(t.data(), w->hits + w->ray_off[c], n * 4, hipMemcpyDeviceToHost));
     for (size_t d = 0; d < dirs; d++)
-      for (size_t p = 0; p < np; p++) hits[p * dirs + d] = t[d * np + p];
+      for (size_t p = 0; p < np; p++) hits[p * dirs + d] = t[d * np + dev[p]];
   }
   return RTGI_OK;
 }
@@ -762,17 +797,38 @@ extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(world_wait(w));
   size_t np = (size_t)w->n[0] * w->n[0] * w->n[0];
+  std::vector<uint32_t> dev(np);
+  for (uint32_t z = 0, p = 0; z < (uint32_t)w->n[0]; z++)
+    for (uint32_t y = 0; y < (uint32_t)w->n[0]; y++)
+      for (uint32_t x = 0; x < (uint32_t)w->n[0]; x++, p++) dev[p] = probe_index_host(w, 0, x, y, z);
   if (w->rp.flags & RTGI_RC_FP16) {
     std::vector<uint16_t> planes(np * 27);
     HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 2, hipMemcpyDeviceToHost));
     for (size_t p = 0; p < np; p++)
-      for (int q = 0; q < 27; q++) sh[p * 27 + q] = half_to_float(planes[(size_t)q * np + p]);
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = half_to_float(planes[(size_t)q * np + dev[p]]);
   } else {
     std::vector<float> planes(np * 27);
     HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
     for (size_t p = 0; p < np; p++)
-      for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + p];
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + dev[p]];
   }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_set_tuning(rtgi_world* w, const rtgi_tuning* t) {
+  if (!w || !t) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  if (t->probe_tile != 1 && t->probe_tile != 2 && t->probe_tile != 4)
+    return rtgi_fail(RTGI_E_INVALID, "probe_tile must be 1, 2 or 4 (got %d)", t->probe_tile);
+  if (t->fused < -1 || t->fused > 1) return rtgi_fail(RTGI_E_INVALID, "fused must be -1, 0 or 1");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));  // the storage order of an update in flight must not change under it
+  w->tuning = *t;
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_get_tuning(const rtgi_world* w, rtgi_tuning* t) {
+  if (!w || !t) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  *t = w->tuning;
   return RTGI_OK;
 }
 
@@ -814,7 +870,9 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   size_t lds = 0;
   bool all_lds = true;
   p.stats = false;
+  p.fused = n > 0;
   for (int i = 0; i < n; i++) {
+    p.fused = p.fused && worlds[i]->tuning.fused != 0;
     TraceK t = trace_args(worlds[i]);
     uint32_t g[BL_COUNT];
     batch_blocks(t, g);
@@ -840,10 +898,16 @@ int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
   e = launch_vox_build_batch(p, s);
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
-  if (!e) e = launch_trace_batch(p, s);
-  if (ev) HIPCHK(hipEventRecord(ev[2], s));
-  for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(p, st, s);
-  if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  if (p.fused) {
+    for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  } else {
+    if (!e) e = launch_trace_batch(p, s);
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
+    for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(p, st, s);
+    if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  }
   if (!e) e = launch_sh_batch(p, s);
   if (ev) HIPCHK(hipEventRecord(ev[4], s));
   if (e) return rtgi_fail(RTGI_E_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));

@@ -779,6 +779,7 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   for (int st = 0; st < t.N; st++) {
     int c = t.N - 1 - st;
     out[BL_MERGE + st] = cdiv(merge_threads(t, c), BLK);
+    out[BL_TM + st] = cdiv(trace_merge_threads(t, c), BLK);
   }
   out[BL_SH] = cdiv(t.c[0].np, BLK);
 }

@@ -75,6 +75,8 @@ struct CascK {
   uint32_t np;      // probes (n^3)
   int32_t n_log2;   // log2(n) when n is a power of two, else -1 (index arithmetic by shifts)
   int32_t res_log2; // log2(res) (res = 4 << min(c, resCap) is always a power of two)
+  int32_t tile_log2; // probe storage order: 0 rows of the grid (x fastest); k > 0 (n a power of two, n >= 2^k):
+                     // blocks of 2^k probes per axis in row order, x fastest inside a block
   uint32_t dir_off; // first entry of the cascade in the direction table
   uint64_t avg_off; // cascade c >= 1: first entry of its 2x2-direction average (probes x res_{c-1}^2 float4)
   uint64_t ray_off; // first ray of the cascade
@@ -103,6 +105,7 @@ struct TraceK {
   float omega0[16];
   const DirEntry* dirs;  // all cascades' direction tables
   int32_t fp16;     // rad holds 4 halves per ray and sh halves (RNE at the reference's pack points), else f32
+  int32_t no_merge; // trace only: rad keeps the traced values, no averages, no SH
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
   uint32_t* hits;   // per ray (same order), or null
@@ -126,8 +129,11 @@ int launch_trace(const TraceK& k, void* stream);
 // n rays (origins o, unit directions d; device arrays of 3 floats each) traced as rays of cascade c -> rgba, hits.
 int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, uint32_t n, float* rgba,
                       uint32_t* hits, void* stream);
-// Threads of cascade c's merge launch.
+// Threads of cascade c's merge launch, and of its fused trace+merge launch.
 uint64_t merge_threads(const TraceK& k, int c);
+uint64_t trace_merge_threads(const TraceK& k, int c);
+// Trace and merge of cascade c in one launch (cascades N-1 .. 0 in order; c + 1's averages must be complete).
+int launch_trace_merge(const TraceK& k, int c, void* stream);
 int launch_merge(const TraceK& k, int c, void* stream);
 int launch_sh(const TraceK& k, void* stream);
 
@@ -147,7 +153,8 @@ enum BatchLaunch : int {
   BL_RESOLVE,
   BL_TRACE,
   BL_MERGE,                                // + step (cascade N-1-step of each world)
-  BL_SH = BL_MERGE + RTGI_MAX_CASCADES,
+  BL_TM = BL_MERGE + RTGI_MAX_CASCADES,    // + step: trace and merge of cascade N-1-step in one launch
+  BL_SH = BL_TM + RTGI_MAX_CASCADES,
   BL_COUNT
 };
 struct BatchPlan {
@@ -157,12 +164,14 @@ struct BatchPlan {
   uint32_t grid[BL_COUNT];
   size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
   bool stats;       // some world of the batch counts tracer statistics
+  bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);
 int launch_vox_build_batch(const BatchPlan& p, void* stream);
 int launch_trace_batch(const BatchPlan& p, void* stream);
 int launch_merge_batch(const BatchPlan& p, int step, void* stream);
+int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_sh_batch(const BatchPlan& p, void* stream);
 
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).

@@ -279,8 +279,15 @@ __device__ __forceinline__ int cascade_of(const TraceK& a, uint64_t ray) {
   return c;
 }
 
-// Probe grid coordinates of probe index pi (x fastest) of a cascade.
+// Probe grid coordinates of probe index pi of a cascade (storage order: rows, or blocks of 2^tile_log2 per axis; with
+// 4^3 blocks a wavefront of 64 consecutive probes holds one compact block of parallel rays).
 __device__ __forceinline__ i3 probe_xyz(const CascK& C, uint32_t pi) {
+  if (C.tile_log2 > 0) {
+    int T = C.tile_log2, nt = C.n_log2 - T;
+    uint32_t mT = (1u << T) - 1u, mt = (1u << nt) - 1u, t = pi >> (3 * T);
+    return i3{(int)(((t & mt) << T) | (pi & mT)), (int)((((t >> nt) & mt) << T) | ((pi >> T) & mT)),
+              (int)(((t >> (2 * nt)) << T) | ((pi >> (2 * T)) & mT))};
+  }
   if (C.n_log2 >= 0) {
     uint32_t m = (uint32_t)C.n - 1u;
     return i3{(int)(pi & m), (int)((pi >> C.n_log2) & m), (int)(pi >> (2 * C.n_log2))};
@@ -400,10 +407,20 @@ __device__ __forceinline__ uint32_t parent_mask(const CascK& U, i3 g) {
   }
   return m;
 }
+// Probe index of grid point q (the inverse of probe_xyz).
+__device__ __forceinline__ uint32_t probe_index(const CascK& C, i3 q) {
+  if (C.tile_log2 > 0) {
+    int T = C.tile_log2, nt = C.n_log2 - T;
+    uint32_t mT = (1u << T) - 1u;
+    uint32_t t = ((uint32_t)q.x >> T) | (((uint32_t)q.y >> T) << nt) | (((uint32_t)q.z >> T) << (2 * nt));
+    uint32_t r = ((uint32_t)q.x & mT) | (((uint32_t)q.y & mT) << T) | (((uint32_t)q.z & mT) << (2 * T));
+    return (t << (3 * T)) | r;
+  }
+  return C.n_log2 >= 0 ? (uint32_t)q.x | ((uint32_t)q.y << C.n_log2) | ((uint32_t)q.z << (2 * C.n_log2))
+                       : (uint32_t)q.x + (uint32_t)C.n * ((uint32_t)q.y + (uint32_t)C.n * (uint32_t)q.z);
+}
 __device__ __forceinline__ uint32_t parent_index(const CascK& U, i3 g, int kk) {
-  i3 q = shr3(g, 1) + i3{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1};
-  return U.n_log2 >= 0 ? (uint32_t)q.x | ((uint32_t)q.y << U.n_log2) | ((uint32_t)q.z << (2 * U.n_log2))
-                       : (uint32_t)q.x + (uint32_t)U.n * ((uint32_t)q.y + (uint32_t)U.n * (uint32_t)q.z);
+  return probe_index(U, shr3(g, 1) + i3{kk & 1, (kk >> 1) & 1, (kk >> 2) & 1});
 }
 
 // Parent mean for one (probe g, direction di) of cascade c < N-1 (mergeWorld, src/core/rc.ts:693-706): sum over the
@@ -546,6 +563,116 @@ __global__ void __launch_bounds__(BLK) k_merge_step_b(BatchK b) {
     merge0_body(a, lb, nb);
 }
 
+// Trace and merge of cascade c in one pass (cascades are launched N-1 .. 0): each thread traces one ray, rounds it
+// as packRad stores it, merges it with the parent mean of cascade c+1's averages (merged in the previous launch) and
+// stores the merged value once; the traced values are never stored unmerged, which saves the merge pass's read and
+// write of every ray. Where cascade c's directions double those of c-1, a wavefront holds 16 probes x the 4
+// directions of one 2x2 block under a texel of c-1 (lane = 16 j + probe, j the block position, the layout of
+// merge_up_body) and the block's average for c-1's merge comes from the other lanes; otherwise a thread takes one ray
+// (direction-major). Results are identical to the trace followed by the merge steps (the same operations on the same
+// values).
+__device__ __forceinline__ float4 rad_round(const TraceK& a, float4 v) {
+  if (!a.fp16) return v;
+  uint32_t p0 = h2(v.x, v.y), p1 = h2(v.z, v.w);
+  return make_float4(hlo(p0), hhi(p0), hlo(p1), hhi(p1));
+}
+
+template <bool ST>
+__device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
+  const CascK& C = a.c[c];
+  const bool quad = !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
+  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+  uint32_t lane = threadIdx.x & 63u, di = 0, pi = 0, dl = 0, j = 0;
+  bool ok;
+  if (quad) {
+    const CascK& D = a.c[c - 1];
+    uint32_t npb = (C.np + 15u) >> 4;
+    uint64_t grp = t >> 6;
+    if (grp >= (uint64_t)D.res * D.res * npb) return;  // uniform over the wavefront
+    j = lane >> 4;
+    dl = (uint32_t)(grp / npb);
+    pi = (uint32_t)(grp % npb) * 16u + (lane & 15u);
+    ok = pi < C.np;
+    uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
+    di = ((2u * lw + (j >> 1)) << C.res_log2) + 2u * lu + (j & 1u);
+  } else {
+    if ((t & ~63ull) >= C.rays) return;  // uniform over the wavefront
+    ok = t < C.rays;
+    if (ok) split_ray(C, (uint32_t)t, di, pi);
+  }
+  float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t steps = 0;
+  if (ok) {
+    i3 g = i3{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c} + probe_xyz(C, pi);
+    float s = a.s0 * (float)(1u << c);
+    f3 o = tof3(g) * s;
+    const DirEntry& de = a.dirs[C.dir_off + di];
+    f3 d = mk3(de.d[0], de.d[1], de.d[2]);
+    f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
+    float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
+    int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
+    float v = vox_size(a.vx, L);
+    Hit h = trace_vox<ST>(a, o, d, inv, t0, t1, a.vox_skip * v, L);
+    float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (h.hit) {
+      f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
+      out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
+    }
+    size_t ray = C.ray_off + (size_t)di * C.np + pi;
+    if (a.hits) a.hits[ray] = h.id;
+    steps = h.steps;
+    if (a.no_merge) {
+      m = rad_store(a, ray, out);
+    } else {
+      f3 um = c + 1 < a.N ? upper_mean(a, c, probe_xyz(C, pi), di) : splat3(0.0f);
+      m = rad_store(a, ray, merged(a, c, g, di, rad_round(a, out), um));
+    }
+  }
+  if (ST && a.vx.stats) {
+    // counts[24 + c], [32 + c], [64 + c] of traceWorld (src/core/rc.ts:232-235), summed per wavefront (every lane
+    // of a wavefront that did not return reaches this point).
+    unsigned long long st = steps, cap = steps >= 1024u ? 1ull : 0ull, n = ok ? 1ull : 0ull;
+    for (int k = 32; k >= 1; k >>= 1) {
+      st += __shfl_xor(st, k);
+      cap += __shfl_xor(cap, k);
+      n += __shfl_xor(n, k);
+    }
+    if (lane == 0) {
+      unsigned long long* S = a.vx.stats;
+      atomicAdd(&S[c], n);
+      atomicAdd(&S[RTGI_MAX_CASCADES + c], st);
+      if (cap) atomicAdd(&S[2 * RTGI_MAX_CASCADES + c], cap);
+    }
+  }
+  if (a.no_merge || c == 0) return;
+  float4* AV = reinterpret_cast<float4*>(a.avg) + C.avg_off;
+  if (!quad) {  // resolution capped: the average of a one-texel block is the merged value
+    if (ok) AV[(size_t)di * C.np + pi] = make_float4(m.x, m.y, m.z, 0.0f);
+    return;
+  }
+  uint32_t l0 = lane & 15u;
+  f3 p00 = mk3(__shfl(m.x, l0), __shfl(m.y, l0), __shfl(m.z, l0));
+  f3 p10 = mk3(__shfl(m.x, l0 + 16), __shfl(m.y, l0 + 16), __shfl(m.z, l0 + 16));
+  f3 p01 = mk3(__shfl(m.x, l0 + 32), __shfl(m.y, l0 + 32), __shfl(m.z, l0 + 32));
+  f3 p11 = mk3(__shfl(m.x, l0 + 48), __shfl(m.y, l0 + 48), __shfl(m.z, l0 + 48));
+  if (ok && j == 0) {
+    f3 avg = (p00 + p10 + p01 + p11) * 0.25f;
+    AV[(size_t)dl * C.np + pi] = make_float4(avg.x, avg.y, avg.z, 0.0f);
+  }
+}
+// Eight wavefronts per SIMD (at most 64 VGPRs): the trace is latency-bound, and occupancy pays more than registers.
+template <bool ST>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge(TraceK a, int c) {
+  trace_merge_body<ST>(a, c, blockIdx.x, gridDim.x);
+}
+// Step of a batch: each world traces and merges its cascade N-1-step.
+template <bool ST>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  const TraceK& a = b.w[w];
+  trace_merge_body<ST>(a, a.N - 1 - b.arg, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
 int grid_for(uint64_t n) {
   uint64_t g = (n + BLK - 1) / BLK;
   return (int)(g < 1 ? 1 : g);
@@ -571,6 +698,18 @@ uint64_t merge_threads(const TraceK& k, int c) {
 int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, uint32_t n, float* rgba,
                       uint32_t* hits, void* stream) {
   if (n) k_trace_rays<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c, o, d, n, (float4*)rgba, hits);
+  return (int)hipGetLastError();
+}
+
+uint64_t trace_merge_threads(const TraceK& k, int c) { return k.no_merge ? k.c[c].rays : merge_threads(k, c); }
+
+int launch_trace_merge(const TraceK& k, int c, void* stream) {
+  uint64_t n = trace_merge_threads(k, c);
+  if (!n) return 0;
+  if (k.vx.stats)
+    k_trace_merge<true><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+  else
+    k_trace_merge<false><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   return (int)hipGetLastError();
 }
 
@@ -603,6 +742,16 @@ int launch_trace_batch(const BatchPlan& p, void* stream) {
 int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
   if (p.grid[BL_MERGE + step])
     k_merge_step_b<<<p.grid[BL_MERGE + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_MERGE + step, step));
+  return (int)hipGetLastError();
+}
+
+int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
+  if (p.grid[BL_TM + step]) {
+    if (p.stats)
+      k_trace_merge_b<true><<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
+    else
+      k_trace_merge_b<false><<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
+  }
   return (int)hipGetLastError();
 }
 

@@ -52,7 +52,12 @@ struct rtgi_world {
   hipEvent_t ev[5] = {};
   uint32_t last_stages = 0;
   hipEvent_t ext_done = nullptr;  // completion of the last batched update that included this world
+  rtgi_tuning tuning{1, -1};
 };
+
+// log2 of the probe block edge cascade c is stored in (0: rows), and the device index of probe (x, y, z).
+int probe_tile_log2(const rtgi_world* w, int c);
+uint32_t probe_index_host(const rtgi_world* w, int c, uint32_t x, uint32_t y, uint32_t z);
 
 // Launch arguments of a world (its current frame, buffers and layout).
 VoxK vox_args(const rtgi_world* w);

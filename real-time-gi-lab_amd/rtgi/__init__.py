@@ -296,6 +296,19 @@ class World:
         check(lib().rtgi_world_read_cascade(self._h, c, ptr(rgba), ptr(h)))
         return (rgba, h) if hits else rgba
 
+    def set_tuning(self, probe_tile=None, fused=None):
+        """Launch tuning (rtgi_world_set_tuning): speed and storage order only, never results. None keeps a
+        setting."""
+        t = self.tuning()
+        t = _L.TuningC(t["probe_tile"] if probe_tile is None else probe_tile,
+                       t["fused"] if fused is None else int(fused))
+        check(lib().rtgi_world_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self):
+        t = _L.TuningC()
+        check(lib().rtgi_world_get_tuning(self._h, C.byref(t)))
+        return dict(probe_tile=t.probe_tile, fused=t.fused)
+
     def trace_rays(self, c, origins, dirs):
         """Rays from origins along unit dirs (n x 3) traced as cascade c's rays over the last update's clipmap."""
         o = np.ascontiguousarray(origins, np.float32)

@@ -67,6 +67,10 @@ class TraceStatsC(C.Structure):
                 ("capped", C.c_uint64 * MAX_CASCADES)]
 
 
+class TuningC(C.Structure):
+    _fields_ = [("probe_tile", C.c_int32), ("fused", C.c_int32)]
+
+
 class ServerParamsC(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("slots", C.c_int32)]
 
@@ -123,6 +127,8 @@ SIGNATURES = {
     "rtgi_world_read_sh": (C.c_int, [_P, _P]),
     "rtgi_world_read_trace_stats": (C.c_int, [_P, C.POINTER(TraceStatsC)]),
     "rtgi_world_trace_rays": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, _P, _P]),
+    "rtgi_world_set_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
+    "rtgi_world_get_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_destroy": (C.c_int, [_P]),
     "rtgi_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
     "rtgi_batch_set_graph": (C.c_int, [_P, C.c_int32]),

@@ -79,10 +79,13 @@ def solo_results(worlds):
     return res
 
 
-def test_batch_matches_solo_direct(require_gpu):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_batch_matches_solo_direct(require_gpu, fused):
     worlds = make_worlds()
-    say(f"case: batch of {len(worlds)} uneven worlds, direct launches")
+    say(f"case: batch of {len(worlds)} uneven worlds, direct launches, fused={fused}")
     solo = solo_results(worlds)
+    for w in worlds:
+        w.set_tuning(fused=fused, probe_tile=4 if fused else 1)
     with rtgi.Batch(worlds) as b:
         b.encode()
         b.sync()
@@ -118,7 +121,7 @@ def test_batch_graph_replays(require_gpu):
 
 
 def test_batch_of_cities_matches_solo(require_gpu):
-    say("case: batch of 3 city worlds (500k triangles each) vs solo")
+    say("case: batch of 3 city worlds (500k triangles each) vs solo, fused hipGraph")
     worlds = []
     for seed in (1, 2, 3):
         sc, vox, rc, frame = rtgi.config("city")
@@ -128,10 +131,13 @@ def test_batch_of_cities_matches_solo(require_gpu):
         w.set_frame(dataclasses.replace(frame, eye=sc.eye))
         worlds.append(w)
     solo = solo_results(worlds)
+    for w in worlds:
+        w.set_tuning(fused=1)
     with rtgi.Batch(worlds, graph=True) as b:
-        b.encode()
-        b.sync()
-        for i, w in enumerate(worlds):
-            assert_same(snapshot(w), solo[i], f"city {i}")
+        for rep in range(2):
+            b.encode()
+            b.sync()
+            for i, w in enumerate(worlds):
+                assert_same(snapshot(w), solo[i], f"city {i}")
     for w in worlds:
         w.destroy()

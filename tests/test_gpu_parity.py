@@ -284,3 +284,38 @@ def test_trace_stats(require_gpu, case):
         w.encode()
         w.sync()
     w.destroy()
+
+
+@pytest.mark.parametrize("case", ["tiny", "tiny-nomerge", "atrium", "city"])
+def test_tuning_keeps_results(require_gpu, case):
+    """Launch tuning changes no result: probe storage order (probe_tile 1 / 2 / 4) and per-cascade fused trace+merge
+    launches read back cascades, hit ids, SH and tracer statistics bit-identical, plain and FP16 forms."""
+    if case.startswith("tiny"):
+        scene, vox, rc, frame = tiny_case(seed=9, ntris=400)
+        rc = dataclasses.replace(rc, noMerge=case.endswith("nomerge"))
+    else:
+        scene, vox, rc, frame = rtgi.config(case)
+    for fp16 in (False, True):
+        rcf = dataclasses.replace(rc, fp16=fp16, recordHits=True, stats=1)
+        say(f"case: tuning, {case}, fp16={fp16}")
+        w = rtgi.World(vox, rcf)
+        w.upload_scene(scene)
+        w.set_frame(frame)
+        ref = None
+        for tile, fused in ((1, 0), (2, 0), (4, 0), (1, 1), (4, 1)):
+            w.set_tuning(probe_tile=tile, fused=fused)
+            assert w.tuning() == {"probe_tile": tile, "fused": fused}
+            w.encode()
+            w.sync()
+            out = [w.read_cascade(c, hits=True) for c in range(rc.cascades)]
+            if not rc.noMerge:
+                out.append((w.read_sh(), None))
+            stats = w.read_trace_stats()
+            if ref is None:
+                ref = (out, stats)
+                continue
+            assert stats == ref[1], (tile, fused)
+            for (a, ha), (b, hb) in zip(out, ref[0]):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (tile, fused, case)
+                assert ha is None or np.array_equal(ha, hb)
+        w.destroy()
