@@ -6,10 +6,16 @@ cascade merge and SH projection of one world.
 
 Two paths are timed on the same worlds:
   plain   one world at a time: its update is enqueued, then waited for (the single-instance path);
-  batched all W worlds with one launch per stage (worlds in batches of --batch), replayed as a hipGraph.
+  batched all W worlds with one launch per stage per batch, replayed as hipGraphs; the batch width (worlds per
+          launch; batches run on their own streams, concurrently), probe tile and trace/merge fusion come from the
+          autotuner (rtgi.autotune) unless --batch fixes the width.
 A step updates every world once. The value is the batched path's aggregate GI updates per second; the latency of an
 update is the time from the step's submission to the completion of the batch that holds its world (p50/p99 over all
 updates of the timed steps). Inputs are resident in HBM before timing starts.
+
+Batched mix (the "batched_mix" field; --mix-worlds 0 skips it): --mix-worlds seeded worlds of different sizes and
+cascade counts (rtgi.workloads, FP16 storage) updated in batched hipGraph replays with the tuning and batch width the
+autotuner (rtgi.autotune) picks, beside the same worlds' plain FP32 updates one at a time.
 
 Serving (the "serving" field; --serve-seconds 0 skips it): a Poisson stream of update requests (--serve-rate per
 second) over a resident mix of --serve-worlds seeded worlds of different sizes (rtgi.workloads), FP16 storage, through
@@ -53,6 +59,69 @@ def cpu_baseline(scene, vox, rc, frame, repeats):
 
 def pct(x, q):
     return round(float(np.percentile(np.asarray(x) * 1e3, q)), 3)
+
+
+def mix_bench(a):
+    """Batched instances: a seeded mix of worlds of different sizes (FP16 storage), one launch per stage per batch,
+    hipGraph replays, tile / fusion / batch width picked by the autotuner; beside it the same worlds' plain FP32
+    updates one at a time."""
+    import dataclasses
+
+    import rtgi
+    from rtgi.autotune import autotune
+    from rtgi.workloads import world_mix
+
+    mix = world_mix(a.mix_worlds, seed=5, tail_every=0)
+    worlds = []
+    for m in mix:
+        w = rtgi.World(m.vox, m.rc)
+        w.upload_scene(m.scene)
+        w.set_frame(m.frame())
+        worlds.append(w)
+    tune = autotune(worlds, repeats=2, log=log)
+    best = tune["best"]
+    width = best["width"]
+    batches = [rtgi.Batch(worlds[i:i + width], graph=True) for i in range(0, len(worlds), width)]
+    for _ in range(2):
+        for b in batches:
+            b.encode()
+        for b in batches:
+            b.sync()
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        for b in batches:
+            b.encode()
+        for b in batches:
+            b.sync()
+    value = a.steps * len(worlds) / (time.perf_counter() - t)
+    for b in batches:
+        b.destroy()
+    for w in worlds:
+        w.destroy()
+    plain, plain_worlds = [], []
+    for m in mix:
+        w = rtgi.World(m.vox, dataclasses.replace(m.rc, fp16=False))
+        w.upload_scene(m.scene)
+        w.set_frame(m.frame())
+        w.encode()
+        w.sync()
+        plain_worlds.append(w)
+    t = time.perf_counter()
+    for _ in range(2):
+        for w in plain_worlds:
+            w.encode()
+            w.sync()
+    plain_value = 2 * len(plain_worlds) / (time.perf_counter() - t)
+    for w in plain_worlds:
+        w.destroy()
+    log(f"mix: {value:.1f} updates/s batched ({best}), plain FP32 {plain_value:.1f} updates/s")
+    return {
+        "workload": f"{len(mix)} seeded mixed worlds (atrium/city, 10k-500k triangles log-uniform, clipmap edge "
+                    f"32-128 x 4 levels, 3-5 cascades), FP16 storage, batched hipGraph replays",
+        "updates_per_s": round(value, 2), "tuning": best, "autotune_table": tune["table"],
+        "triangles_total_m": round(sum(m.scene.ntris for m in mix) / 1e6, 2),
+        "plain_fp32_serial_updates_per_s": round(plain_value, 2), "speedup_vs_plain": round(value / plain_value, 3),
+    }
 
 
 def serve_bench(a):
@@ -166,6 +235,7 @@ def main():
     ap.add_argument("--config", default="city")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mix-worlds", type=int, default=32, help="batched mixed-world run (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
     ap.add_argument("--serve-worlds", type=int, default=64)
@@ -211,7 +281,6 @@ def main():
         if k == 0:
             first = (sc, vox, rc, frame)
     lay = worlds[0].layout()
-    worlds_tuning = worlds[0].tuning()
     rays = lay["total_rays"]
     log(f"setup: {a.worlds} worlds of '{a.config}' in {time.perf_counter() - t0:.1f} s, {first[0].ntris} triangles "
         f"in world 1, {rays} rays per update, {lay['device_bytes'] / 2 ** 20:.0f} MiB per world")
@@ -231,8 +300,17 @@ def main():
     plain_value = a.plain_steps * len(worlds) / (time.perf_counter() - t_start)
     log(f"plain: {plain_value:.1f} updates/s")
 
-    # Batched path, hipGraph replays.
-    bsz = a.batch if a.batch > 0 else len(worlds)
+    # Batched path, hipGraph replays; batch width, probe tile and fusion from the autotuner unless --batch is given.
+    tune = None
+    if a.batch > 0:
+        bsz = a.batch
+    else:
+        from rtgi.autotune import autotune
+
+        widths = sorted({w for w in (8, 16, 32) if w < len(worlds)} | {len(worlds)})
+        tune = autotune(worlds, tiles=(1, 4), widths=widths, repeats=2, log=log)
+        bsz = tune["best"]["width"]
+    worlds_tuning = worlds[0].tuning()
     groups = [worlds[i:i + bsz] for i in range(0, len(worlds), bsz)]
     batches = [rtgi.Batch(g, graph=True) for g in groups]
     for _ in range(a.warmup):
@@ -278,6 +356,7 @@ def main():
         b.destroy()
     for w in worlds:
         w.destroy()
+    mixed = mix_bench(a) if a.mix_worlds > 0 else None
     serving = serve_bench(a) if a.serve_seconds > 0 else None
 
     cpu = None
@@ -303,7 +382,8 @@ def main():
         "data": "synthetic (seeded procedural city blocks)",
         "config": {"workload": f"{a.worlds} resident '{a.config}' worlds, batched FP32 update (hipGraph), "
                                f"{len(groups)} batch(es) of {bsz}",
-                   "worlds": a.worlds, "batch": bsz, "triangles_world1": int(first[0].ntris),
+                   "worlds": a.worlds, "batch": bsz, "tuning": worlds_tuning,
+                   "triangles_world1": int(first[0].ntris),
                    "clipmap": "4 x 128^3 @ 0.25 m", "probes_c0": "64^3", "cascades": 5,
                    "rays_per_update": int(rays)},
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
@@ -313,6 +393,7 @@ def main():
         "mrays_per_s": round(rays * value / 1e6, 1),
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage.items()},
         "trace_merge_fused": worlds_tuning["fused"] != 0,
+        "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
         "roofline": {"kernel": ("k_trace_merge_b (5 batched launches, one per cascade)" if worlds_tuning["fused"] != 0
                                 else "k_trace_b (batched launch)"),
@@ -320,6 +401,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "cpu_baseline": cpu,
+        "batched_mix": mixed,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -58,6 +58,16 @@ __device__ __forceinline__ float4 rad_store(const TraceK& a, size_t i, float4 v)
   return v;
 }
 
+// Buffer-resource loads for the DDA's per-step reads: the base sits in scalar registers (one per world, uniform over
+// a workgroup) and a lane supplies a 32-bit byte offset, so a read costs no 64-bit address arithmetic; offsets past
+// the resource's size read 0 (never the case here: indexes are in range by construction).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
+}
+
 struct VoxHit {
   uint32_t brick, bit;
   int level;
@@ -184,14 +194,16 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   f3 tMax = ((tof3(c) + sp) * v + lo - o) * inv;
   f3 tDelta = abs3(inv * v);
   float tIn = t;
-  const uint32_t* ptrL = k.ptr + (size_t)L * (uint32_t)k.per;
+  const __amdgpu_buffer_rsrc_t ptr_r = buf_rsrc(k.ptr, (uint32_t)(k.NL * k.per) * 4u);
+  const __amdgpu_buffer_rsrc_t occ_r = buf_rsrc(k.occ, (uint32_t)k.max_bricks * 64u);
+  const uint32_t ptr_base = (uint32_t)L * (uint32_t)k.per * 4u;
   uint32_t lastb = 0xffffffffu, ptr = 0;
   int it;
   for (it = 0; it < 1024; it++) {
     i3 b = shr3(c, 3);
     uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
     if (bi != lastb) {  // the pointer of the brick is read once per brick visited
-      ptr = ptrL[bi];
+      ptr = buf_u32(ptr_r, ptr_base + bi * 4u);
       lastb = bi;
     }
     if (ptr == PTR_DROP) {
@@ -235,7 +247,7 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       continue;
     }
     uint32_t bit = bit_of(c);
-    uint32_t occ = k.occ[ptr * 16u + (bit >> 5)];
+    uint32_t occ = buf_u32(occ_r, ptr * 64u + (bit >> 5) * 4u);
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
       res = Hit{true, ptr, bit, tIn, hit_id(k, L, c), ST ? (uint32_t)it + 1u : 0u};
