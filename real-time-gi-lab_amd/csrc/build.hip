@@ -96,7 +96,7 @@ struct Range {
 };
 __device__ __forceinline__ Range vox_range(const VoxK& k, const Tri& T, int L, bool brick) {
   float cell = brick ? vox_size(k, L) * 8.0f : vox_size(k, L);
-  float inv = k.inv_v0 * (1.0f / (float)(1u << L)) * (brick ? 0.125f : 1.0f);
+  float inv = k.inv_v0 * inv_pow2(L) * (brick ? 0.125f : 1.0f);
   bool p2 = k.pow2 != 0;
   f3 lo = region_lo(k, L);
   f3 mn = min3(min3(T.a, T.b), T.c), mx = max3(max3(T.a, T.b), T.c);

@@ -176,7 +176,7 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   Hit res{false, 0u, 0u, 0.0f, RTGI_NO_HIT, 0u};
   const bool p2 = k.pow2 != 0;
   float v = vox_size(k, L);
-  float iv = k.inv_v0 * (1.0f / (float)(1u << L));
+  float iv = k.inv_v0 * inv_pow2(L);
   int G = k.G;
   f3 lo = tof3(origin(k, L)) * v * 8.0f;
   f3 hi = lo + splat3((float)G * v * 8.0f);
@@ -199,6 +199,8 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   const uint32_t ptr_base = (uint32_t)L * (uint32_t)k.per * 4u;
   uint32_t lastb = 0xffffffffu, ptr = 0;
   int it;
+  // Steps and brick jumps select with conditional moves rather than branches (the same f32 operations: each
+  // candidate sum is the one the branch would form), so lanes on different axes do not split the wavefront.
   for (it = 0; it < 1024; it++) {
     i3 b = shr3(c, 3);
     uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
@@ -206,39 +208,12 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       ptr = buf_u32(ptr_r, ptr_base + bi * 4u);
       lastb = bi;
     }
-    if (ptr == PTR_DROP) {
-      VoxHit h;
-      if (vox_at(k, L, c, h) && tIn >= tSkip) {
-        res = Hit{true, h.brick, h.bit, tIn, hit_id(k, h.level, h.cell), ST ? (uint32_t)it + 1u : 0u};
-        return res;
-      }
-      float tx = fminf(fminf(tMax.x, tMax.y), tMax.z);
-      if (tx >= tEnd) break;
-      tIn = tx;
-      if (tMax.x <= tMax.y && tMax.x <= tMax.z) {
-        c.x += st.x;
-        tMax.x += tDelta.x;
-      } else if (tMax.y <= tMax.z) {
-        c.y += st.y;
-        tMax.y += tDelta.y;
-      } else {
-        c.z += st.z;
-        tMax.z += tDelta.z;
-      }
-      if (!inside3(c, nVox)) break;
-      continue;
-    }
-    if (ptr >= PTR_MARK) {
+    if (ptr >= PTR_MARK && ptr != PTR_DROP) {  // empty or marked-unallocated brick: jump to its exit
       f3 bt = ((tof3(b * 8) + sp * 8.0f) * v + lo - o) * inv;
       float te = fminf(fminf(bt.x, bt.y), bt.z);
       if (te >= tEnd) break;
-      i3 nb = b;
-      if (bt.x <= bt.y && bt.x <= bt.z)
-        nb.x += st.x;
-      else if (bt.y <= bt.z)
-        nb.y += st.y;
-      else
-        nb.z += st.z;
+      bool bx = bt.x <= bt.y && bt.x <= bt.z, by = !bx && bt.y <= bt.z, bz = !bx && !by;
+      i3 nb{b.x + (bx ? st.x : 0), b.y + (by ? st.y : 0), b.z + (bz ? st.z : 0)};
       if (!inside3(nb, G)) break;
       t = te;
       tIn = te;
@@ -246,25 +221,31 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       tMax = ((tof3(c) + sp) * v + lo - o) * inv;
       continue;
     }
-    uint32_t bit = bit_of(c);
-    uint32_t occ = buf_u32(occ_r, ptr * 64u + (bit >> 5) * 4u);
-    float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
-    if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
-      res = Hit{true, ptr, bit, tIn, hit_id(k, L, c), ST ? (uint32_t)it + 1u : 0u};
+    bool occupied;
+    uint32_t hbrick, hbit, hid;
+    if (ptr == PTR_DROP) {  // dropped brick: voxel by voxel, each read at the coarser level that holds it
+      VoxHit h;
+      occupied = vox_at(k, L, c, h);
+      hbrick = h.brick;
+      hbit = h.bit;
+      hid = occupied ? hit_id(k, h.level, h.cell) : 0u;
+    } else {
+      hbrick = ptr;
+      hbit = bit_of(c);
+      occupied = (buf_u32(occ_r, ptr * 64u + (hbit >> 5) * 4u) & (1u << (hbit & 31u))) != 0u;
+      hid = hit_id(k, L, c);
+    }
+    if (occupied && tIn >= tSkip) {
+      res = Hit{true, hbrick, hbit, tIn, hid, ST ? (uint32_t)it + 1u : 0u};
       return res;
     }
+    float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if (tExit >= tEnd) break;
     tIn = tExit;
-    if (tMax.x <= tMax.y && tMax.x <= tMax.z) {
-      c.x += st.x;
-      tMax.x += tDelta.x;
-    } else if (tMax.y <= tMax.z) {
-      c.y += st.y;
-      tMax.y += tDelta.y;
-    } else {
-      c.z += st.z;
-      tMax.z += tDelta.z;
-    }
+    bool sx = tMax.x <= tMax.y && tMax.x <= tMax.z, sy = !sx && tMax.y <= tMax.z, sz = !sx && !sy;
+    c = i3{c.x + (sx ? st.x : 0), c.y + (sy ? st.y : 0), c.z + (sz ? st.z : 0)};
+    f3 nt = tMax + tDelta;
+    tMax = mk3(sx ? nt.x : tMax.x, sy ? nt.y : tMax.y, sz ? nt.z : tMax.z);
     if (!inside3(c, nVox)) break;
   }
   if (ST) res.steps = (uint32_t)min(it + 1, 1024);
@@ -396,9 +377,11 @@ __global__ void __launch_bounds__(BLK) k_trace_rays(TraceK a, int c, const float
 
 // ST: count the tracer statistics (a separate instantiation, so the counting costs the plain trace nothing).
 template <bool ST>
-__global__ void __launch_bounds__(BLK) k_trace(TraceK a) { trace_body<ST>(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace(TraceK a) {
+  trace_body<ST>(a, blockIdx.x, gridDim.x);
+}
 template <bool ST>
-__global__ void __launch_bounds__(BLK) k_trace_b(BatchK b) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_b(BatchK b) {
   int w = batch_world(b, blockIdx.x);
   trace_body<ST>(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }

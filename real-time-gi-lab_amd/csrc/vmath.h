@@ -54,3 +54,6 @@ __device__ __forceinline__ i3 maxi3(i3 a, i3 b) { return i3{max(a.x, b.x), max(a
 __device__ __forceinline__ bool inside3(i3 a, int n) {
   return a.x >= 0 && a.y >= 0 && a.z >= 0 && a.x < n && a.y < n && a.z < n;
 }
+
+// 2^-e for 0 <= e < 126 as an f32 built from its exponent bits: equal to 1.0f / (float)(1u << e), which is exact.
+__device__ __forceinline__ float inv_pow2(int e) { return __int_as_float((127 - e) << 23); }

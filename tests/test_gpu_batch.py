@@ -141,3 +141,22 @@ def test_batch_of_cities_matches_solo(require_gpu):
                 assert_same(snapshot(w), solo[i], f"city {i}")
     for w in worlds:
         w.destroy()
+
+
+def test_autotune_applies_a_candidate_and_keeps_results(require_gpu):
+    from rtgi.autotune import autotune
+
+    worlds = make_worlds(seed=2)[:4]
+    say(f"case: autotune over {len(worlds)} uneven worlds")
+    solo = solo_results(worlds)
+    res = autotune(worlds, widths=(2, 4), repeats=1)
+    assert len(res["table"]) == 3 * 2 * 2 and res["best"] in res["table"]
+    for w in worlds:
+        assert w.tuning() == {"probe_tile": res["best"]["probe_tile"], "fused": res["best"]["fused"]}
+    with rtgi.Batch(worlds, graph=True) as b:
+        b.encode()
+        b.sync()
+        for i, w in enumerate(worlds):
+            assert_same(snapshot(w), solo[i], f"tuned world {i}")
+    for w in worlds:
+        w.destroy()
