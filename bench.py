@@ -350,6 +350,13 @@ def main():
     alg_world = 16 * rays + (2 * 16 * sum(r // 4 for r in per_c[1:]) if worlds_tuning["fused"] != 0 else 0)
     alg_bytes = alg_world * len(groups[0])
     achieved = alg_bytes / (stage["trace"] * 1e-3) / 1e9
+    # HBM traffic of the same stage from the committed FETCH_SIZE / WRITE_SIZE profile of the fused city trace
+    # (copy-calibrated), scaled to this batch width; null for other configurations.
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "trace_traffic_city.json")
+    if a.config == "city" and worlds_tuning["fused"] != 0 and os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = int(json.load(fh)["hbm_bytes_per_world_update"]) * len(groups[0])
     cnt = worlds[0].read_counters()
 
     for b in batches:
@@ -399,7 +406,7 @@ def main():
                                 else "k_trace_b (batched launch)"),
                      "bytes_per_world_update": int(alg_world), "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "algorithmic_bytes": int(alg_bytes), "traffic": traffic},
         "cpu_baseline": cpu,
         "batched_mix": mixed,
         "serving": serving,
