@@ -1,16 +1,22 @@
 // Radiance-cascade probe-interval trace, cascade merge and SH projection on gfx950.
 //
-//   k_trace      one thread per (cascade, direction, probe) ray over all cascades in one launch. Rays of a cascade are
-//                ordered direction-major (ray = di * probes + probe), so a wavefront marches 64 neighbouring probes
-//                along one direction: their DDA steps stay in lockstep and their cell reads and rgba writes coalesce.
-//                Per ray: interval [b_c, b_{c+1}] s0, voxel level from the cone footprint, hierarchical DDA through the
-//                clipmap (traceVox, src/core/voxel.ts:121-202), Lambert hit shading (shadeHitV, src/core/rc.ts:108-116)
-//                -> rgba f32 (radiance, transmittance), optionally the first-hit voxel id.
-//   k_merge      one launch per cascade N-1 .. 0: rad_c += T_c * mean over the grid parents of the upper 2x2
-//                direction average (mergeWorld/upperAvg, src/core/rc.ts:681-708). Each thread of a cascade c >= 1
-//                merges one 2x2 block of directions of one probe and also writes the block's average (upperAvg of
-//                the merged values) once, so the next cascade's merge reads 8 parent values per ray, not 32.
-//   k_sh         c0 probes: solid-angle-weighted L2 SH, cosine-convolved (projectSH, src/core/rc.ts:745-774).
+//   k_trace        one thread per (cascade, direction, probe) ray over all cascades in one launch. Rays of a cascade
+//                  are ordered direction-major (ray = di * probes + probe), so a wavefront marches 64 neighbouring
+//                  probes along one direction: their DDA steps stay in lockstep and their rgba writes coalesce.
+//                  Per ray: interval [b_c, b_{c+1}] s0, voxel level from the cone footprint, hierarchical DDA
+//                  through the clipmap (traceVox, src/core/voxel.ts:121-202), Lambert hit shading (shadeHitV,
+//                  src/core/rc.ts:108-116) -> rgba f32 (radiance, transmittance), optionally the first-hit voxel id.
+//   k_merge_up     cascade c >= 1, launched N-1 .. 1: rad_c += T_c * mean over the grid parents of the upper 2x2
+//                  direction average (mergeWorld/upperAvg, src/core/rc.ts:681-708); a wavefront holds 16 probes x
+//                  one 2x2 block of directions and also stores the block's average once, so the next cascade's
+//                  merge reads 8 parent values per ray, not 32.
+//   k_merge0       cascade 0, thread per ray.
+//   k_trace_merge  the trace and merge of one cascade in one launch (cascades N-1 .. 0): the traced value is merged
+//                  in registers and stored once, which saves the merge's read and write of every ray.
+//   k_sh           c0 probes: solid-angle-weighted L2 SH, cosine-convolved (projectSH, src/core/rc.ts:745-774).
+//   k_trace_rays   ray queries: arbitrary rays traced as a cascade's rays (error measurements).
+//   *_b            the same bodies for a batch of worlds: a launch covers every world's blocks (block -> world by a
+//                  binary search over per-launch block prefixes); per-world parameters come from descriptors.
 // Shading is deferred out of the DDA loop: a lane that hits leaves the loop with its voxel, and the wavefront shades
 // all its hits once after the loop instead of once per loop iteration in which some lane hit.
 //
