@@ -207,6 +207,17 @@ int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hit
 int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float* d, uint32_t n, float* rgba,
                           uint32_t* hits);
 
+/* Gather (gatherWorld, src/core/rc.ts:871-927; evalSH :819-829) of the last update's irradiance field at n surface
+ * points, position pos and unit normal nrm (3 floats each) -> irradiance irr (3 floats each). The c0 probes of the grid
+ * cell around a point are weighted by their trilinear weight + 1e-4 times ((side + 1)^2 / 4 + 0.02), side the cosine
+ * between the normal and the direction to the probe; E = sum w evalSH(probe, n) / sum w (0 with no probe). With
+ * RTGI_GATHER_OCCLUSION, a probe whose segment from the point offset one voxel along the normal crosses an occupied
+ * voxel (segOccluded, rc.ts:845-869) is left out, and when every probe is hidden they are retried from two voxels out
+ * (the reference's vrel gather). The points stand in for the reference's G-buffer pixels (depth and normal textures;
+ * the toon-style receiver operator is out of scope). Synchronous. */
+#define RTGI_GATHER_OCCLUSION 1u
+int rtgi_world_gather(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags, float* irr);
+
 /* Tracer statistics of the last update (RTGI_RC_STATS): per cascade the world traces, the DDA loop iterations and
  * the traces that reached the 1024-iteration cap (the reference's counts[24..32), [32..40), [64..72), rc.ts:232-235). */
 typedef struct rtgi_trace_stats {

@@ -50,6 +50,7 @@ _SIG = {
     "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
     "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int]),
+    "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
     "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                  C.c_int]),
     "orc_trace_rays_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
@@ -260,6 +261,16 @@ class Oracle:
         assert lib().orc_trace_rays_f64(C.byref(self.vp), _p(self.buf), C.byref(self.rc), c, _p(o), _p(d), n,
                                         _p(rgbt), _p(hits), _p(tie), eps, threads) == 0
         return rgbt, hits, tie
+
+    def gather(self, sh, pos, nrm, occlusion=False):
+        """gatherWorld at surface points (n x 3 pos, unit nrm) from sh ([P0][27]) -> irradiance (n x 3)."""
+        sh = np.ascontiguousarray(sh, np.float32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        irr = np.zeros((pos.shape[0], 3), np.float32)
+        lib().orc_gather(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(sh), _p(pos), _p(nrm), pos.shape[0],
+                         1 if occlusion else 0, _p(irr))
+        return irr
 
     def merge_sh_f64(self, rgbt):
         rgbt = np.ascontiguousarray(rgbt, np.float64).copy()

@@ -991,6 +991,104 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
   return 0;
 }
 
+/* evalSH (rc.ts:819-829) of probe pi of sh ([P0][27]): coefficients times shBasis(n) in order, clamped at 0. */
+static v3 evalSH(const float* sh, uint64_t pi, v3 n) {
+  float y[9] = {0.282095f, 0.488603f * n.y, 0.488603f * n.z, 0.488603f * n.x, 1.092548f * n.x * n.y,
+                1.092548f * n.y * n.z, 0.315392f * (3.0f * n.z * n.z - 1.0f), 1.092548f * n.x * n.z,
+                0.546274f * (n.x * n.x - n.y * n.y)};
+  const float* v = sh + pi * 27;
+  v3 e = V3(0, 0, 0);
+  for (int k = 0; k < 9; k++) e = vadd(e, vscale(V3(v[k * 3], v[k * 3 + 1], v[k * 3 + 2]), y[k]));
+  return V3(fmaxf(e.x, 0.0f), fmaxf(e.y, 0.0f), fmaxf(e.z, 0.0f));
+}
+
+/* segOccluded (rc.ts:845-869): min(ceil(|b - a| / v), 16) samples at the midpoints; one in an occupied voxel of
+ * level L (voxAt: a dropped brick read coarser) hides the segment; samples outside the window are empty. */
+static int segOccluded(const View* W, v3 a, v3 b, uint32_t L) {
+  const orc_vox* p = W->p;
+  v3 d = vsub(b, a);
+  float v = voxSize(p, L);
+  float kf = ceilf(sqrtf(vdot(d, d)) / v);
+  uint32_t k = (uint32_t)(kf < 16.0f ? kf : 16.0f);
+  v3 lo = vscale(vscale(vf(org(p, L)), v), 8.0f);
+  int E = p->G * 8;
+  for (uint32_t i = 0; i < k; i++) {
+    v3 q = vadd(a, vscale(d, ((float)i + 0.5f) / (float)k));
+    iv3 c = vi(vfloor(vdivs(vsub(q, lo), v)));
+    if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= E || c.y >= E || c.z >= E) continue;
+    uint32_t br, bit, lev;
+    iv3 cell;
+    if (voxAt(W, L, c, &br, &bit, &lev, &cell)) return 1;
+  }
+  return 0;
+}
+
+/* The c0 probe at corner k of the cell b around x: its gather weight and index (gatherWorld, rc.ts:886-895). */
+static int gatherCorner(const orc_rc* rc, v3 x, v3 n, iv3 b, v3 f, int k, v3* pr, float* w, uint64_t* pi) {
+  iv3 o = I3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+  iv3 g = iadd(b, o);
+  int N = rc->probes0;
+  iv3 gl = isub(g, I3(rc->G0[0], rc->G0[1], rc->G0[2]));
+  if (gl.x < 0 || gl.y < 0 || gl.z < 0 || gl.x >= N || gl.y >= N || gl.z >= N) return 0;
+  v3 tw = V3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
+  *pr = vscale(vf(g), rc->s0);
+  v3 dv = vsub(*pr, x);
+  float side = vdot(n, dv) / fmaxf(sqrtf(vdot(dv, dv)), 1e-5f);
+  *w = (tw.x * tw.y * tw.z + 1e-4f) * ((side + 1.0f) * (side + 1.0f) * 0.25f + 0.02f);
+  *pi = (uint64_t)gl.x + (uint64_t)N * ((uint64_t)gl.y + (uint64_t)N * (uint64_t)gl.z);
+  return 1;
+}
+
+int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const float* sh, const float* pos,
+               const float* nrm, uint64_t n, int occlusion, float* irr) {
+  View W;
+  makeView(&W, p, vox);
+  for (uint64_t i = 0; i < n; i++) {
+    v3 x = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    v3 nn = V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    v3 gp = vdivs(x, rc->s0);
+    v3 fl = vfloor(gp);
+    iv3 b = vi(fl);
+    v3 f = vsub(gp, vf(b));
+    uint32_t Lo = voxLevelFor(p, (uint32_t)rc->vox_level[0], x);
+    float vLo = voxSize(p, Lo);
+    v3 xo1 = vadd(x, vscale(nn, 1.0f * vLo));
+    v3 e = V3(0, 0, 0);
+    float ws = 0.0f;
+    uint32_t hidden = 0;
+    for (int k = 0; k < 8; k++) {
+      v3 pr;
+      float w;
+      uint64_t pi;
+      if (!gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi)) continue;
+      if (occlusion && segOccluded(&W, xo1, pr, Lo)) {
+        hidden |= 1u << k;
+        continue;
+      }
+      e = vadd(e, vscale(evalSH(sh, pi, nn), w));
+      ws += w;
+    }
+    if (ws <= 0.0f && hidden) {
+      v3 xo2 = vadd(x, vscale(nn, 2.0f * vLo));
+      for (int k = 0; k < 8; k++) {
+        if (!((hidden >> k) & 1u)) continue;
+        v3 pr;
+        float w;
+        uint64_t pi;
+        gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi);
+        if (segOccluded(&W, xo2, pr, Lo)) continue;
+        e = vadd(e, vscale(evalSH(sh, pi, nn), w));
+        ws += w;
+      }
+    }
+    v3 out = ws > 0.0f ? vdivs(e, ws) : V3(0, 0, 0);
+    irr[3 * i] = out.x;
+    irr[3 * i + 1] = out.y;
+    irr[3 * i + 2] = out.z;
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------------------------------------------------ */
 /* FP64 oracle: the same algorithm in double precision (directions, ray-voxel traversal, shading, merge, SH), on   */
 /* the same occupancy and attributes.                                                                            */

@@ -71,6 +71,11 @@ int orc_trace_rays(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int 
 /* mergeWorld N-1..0 in place on the full rgba array, then projectSH -> sh (27 floats per c0 probe). */
 int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
 
+/* gatherWorld (rc.ts:871-927) at n surface points (pos, unit nrm: 3 floats each) from sh ([P0][27], probe-major as
+ * orc_merge_sh writes it) -> irr (3 floats each); occlusion: segOccluded tests (the vrel gather). */
+int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const float* sh, const float* pos,
+               const float* nrm, uint64_t n, int occlusion, float* irr);
+
 /* FP64 oracle: trace in double over the same occupancy and attributes; rays [r0, r1). rgb: 3 doubles per ray, hit:
  * 1 for a hit, and tie: 1 when the ray passes within eps (relative) of a voxel edge or corner before its first hit
  * (its first-hit voxel is then not fixed by exact arithmetic's neighbourhood). Any output may be NULL. */

@@ -777,6 +777,35 @@ extern "C" int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, c
   return e;
 }
 
+extern "C" int rtgi_world_gather(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags,
+                                 float* irr) {
+  if (!w || (n && (!pos || !nrm || !irr))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather: null argument");
+  if (flags & ~RTGI_GATHER_OCCLUSION) return rtgi_fail(RTGI_E_INVALID, "unknown gather flags");
+  if (w->rp.flags & RTGI_RC_NO_MERGE) return rtgi_fail(RTGI_E_STATE, "no irradiance field (RTGI_RC_NO_MERGE)");
+  if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "no scene or no frame");
+  if (n == 0) return RTGI_OK;
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  float *din = nullptr, *dout = nullptr;
+  int e = RTGI_OK;
+  if (hipMalloc((void**)&din, (size_t)n * 24) != hipSuccess || hipMalloc((void**)&dout, (size_t)n * 12) != hipSuccess)
+    e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc for %u points", n);
+  if (!e && (hipMemcpy(din, pos, (size_t)n * 12, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(din + (size_t)n * 3, nrm, (size_t)n * 12, hipMemcpyHostToDevice) != hipSuccess))
+    e = rtgi_fail(RTGI_E_HIP, "point upload failed");
+  if (!e) {
+    int le = launch_gather(trace_args(w), din, din + (size_t)n * 3, n, (flags & RTGI_GATHER_OCCLUSION) ? 1 : 0, dout,
+                           w->stream);
+    if (le) e = rtgi_fail(RTGI_E_HIP, "gather launch failed: %s", hipGetErrorString((hipError_t)le));
+  }
+  if (!e && hipStreamSynchronize(w->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "gather failed");
+  if (!e && hipMemcpy(irr, dout, (size_t)n * 12, hipMemcpyDeviceToHost) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "readback failed");
+  if (din) (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  return e;
+}
+
 extern "C" int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out) {
   if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
   if (!w->stats) return rtgi_fail(RTGI_E_STATE, "tracer statistics are off (RTGI_RC_STATS)");

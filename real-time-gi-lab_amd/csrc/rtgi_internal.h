@@ -129,6 +129,8 @@ int launch_trace(const TraceK& k, void* stream);
 // n rays (origins o, unit directions d; device arrays of 3 floats each) traced as rays of cascade c -> rgba, hits.
 int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, uint32_t n, float* rgba,
                       uint32_t* hits, void* stream);
+// Gather of the irradiance field at n surface points (device arrays: 3 floats each) -> irr (3 floats each).
+int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
 // Threads of cascade c's merge launch, and of its fused trace+merge launch.
 uint64_t merge_threads(const TraceK& k, int c);
 uint64_t trace_merge_threads(const TraceK& k, int c);

@@ -503,8 +503,7 @@ __device__ __forceinline__ void merge0_body(const TraceK& a, uint32_t lb, uint32
 __global__ void __launch_bounds__(BLK) k_merge0(TraceK a) { merge0_body(a, blockIdx.x, gridDim.x); }
 
 // shBasis (src/core/rc.ts:745-749) of c0 texel d, from the direction table.
-__device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) {
-  float nx = de.d[0], ny = de.d[1], nz = de.d[2];
+__device__ __forceinline__ void sh_basis3(float nx, float ny, float nz, float y[9]) {
   y[0] = 0.282095f;
   y[1] = 0.488603f * ny;
   y[2] = 0.488603f * nz;
@@ -515,6 +514,7 @@ __device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) {
   y[7] = 1.092548f * nx * nz;
   y[8] = 0.546274f * (nx * nx - ny * ny);
 }
+__device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) { sh_basis3(de.d[0], de.d[1], de.d[2], y); }
 
 // 27 planes of f32, or of halves in the precision form (pack2x16float per value, src/core/rc.ts:770-771).
 __device__ __forceinline__ void sh_store(const TraceK& a, uint32_t np, uint32_t pi, const f3 cf[9]) {
@@ -674,6 +674,120 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
   trace_merge_body<ST>(a, a.N - 1 - b.arg, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
+// ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
+
+// evalSH (src/core/rc.ts:819-829) of c0 probe q (device index): coefficients times shBasis(n) in coefficient order,
+// clamped at 0. The precision form's f16 coefficients are widened, as unpack2x16float does.
+__device__ f3 eval_sh(const TraceK& a, uint32_t q, f3 n) {
+  uint32_t np = a.c[0].np;
+  float y[9];
+  sh_basis3(n.x, n.y, n.z, y);
+  f3 e = splat3(0.0f);
+  for (int k = 0; k < 9; k++) {
+    float v[3];
+    for (int ch = 0; ch < 3; ch++) {
+      size_t j = (size_t)(k * 3 + ch) * np + q;
+      v[ch] = a.fp16 ? __half2float(__ushort_as_half(reinterpret_cast<const unsigned short*>(a.sh)[j])) : a.sh[j];
+    }
+    e = e + mk3(v[0], v[1], v[2]) * y[k];
+  }
+  return mk3(fmaxf(e.x, 0.0f), fmaxf(e.y, 0.0f), fmaxf(e.z, 0.0f));
+}
+
+// segOccluded (src/core/rc.ts:845-869): min(ceil(|b - a| / v), 16) samples at the segment's midpoints; true if one
+// lies in an occupied voxel of level L (a dropped brick read one level coarser). Samples outside the window count
+// as empty.
+__device__ bool seg_occluded(const VoxK& k, f3 pa, f3 pb, int L) {
+  f3 d = pb - pa;
+  float v = vox_size(k, L);
+  float iv = k.inv_v0 * inv_pow2(L);
+  uint32_t n = (uint32_t)fminf(ceilf(sqrtf(dot3(d, d)) / v), 16.0f);
+  f3 lo = tof3(origin(k, L)) * v * 8.0f;
+  int nVox = k.G * 8;
+  for (uint32_t i = 0; i < n; i++) {
+    f3 p = pa + d * (((float)i + 0.5f) / (float)n);
+    i3 c = cell_at(p - lo, v, iv, k.pow2 != 0);
+    if (!inside3(c, nVox)) continue;
+    uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
+    if (ptr == PTR_DROP) {
+      VoxHit h;
+      if (vox_at(k, L, c, h)) return true;
+      continue;
+    }
+    if (ptr >= PTR_MARK) continue;
+    uint32_t bit = bit_of(c);
+    if ((k.occ[ptr * 16u + (bit >> 5)] & (1u << (bit & 31u))) != 0u) return true;
+  }
+  return false;
+}
+
+// The 8 c0 probes around x with their gather weights: trilinear weight + 1e-4 times ((side + 1)^2 / 4 + 0.02),
+// side = cos(n, probe - x) (gatherWorld, rc.ts:886-895). Corner k: o = (k & 1, k >> 1 & 1, k >> 2 & 1); absent
+// (outside the dense grid) -> false.
+__device__ __forceinline__ bool gather_corner(const TraceK& a, f3 x, f3 n, i3 b, f3 f, int k, f3& pr, float& w,
+                                              uint32_t& q) {
+  const CascK& C = a.c[0];
+  i3 o{k & 1, (k >> 1) & 1, (k >> 2) & 1};
+  i3 g = b + o;
+  i3 gl = g - i3{a.G0[0], a.G0[1], a.G0[2]};
+  if (!inside3(gl, C.n)) return false;
+  f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
+  pr = tof3(g) * a.s0;
+  f3 dv = pr - x;
+  float side = dot3(n, dv) / fmaxf(sqrtf(dot3(dv, dv)), 1e-5f);
+  w = (tw.x * tw.y * tw.z + 1e-4f) * ((side + 1.0f) * (side + 1.0f) * 0.25f + 0.02f);
+  q = probe_index(C, gl);
+  return true;
+}
+
+__global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, const float* nrm, uint32_t n, int occl,
+                                                float* irr) {
+  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  f3 x = mk3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  f3 nn = mk3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  f3 gp = x / a.s0;
+  i3 b = f2i3(floor3(gp));
+  f3 f = gp - tof3(b);
+  // Occlusion tests start one voxel off the surface along the normal, at the level of the c0 traces.
+  int Lo = vox_level_for(a.vx, a.c[0].vox_level, x);
+  float vLo = vox_size(a.vx, Lo);
+  f3 xo1 = x + nn * (1.0f * vLo);
+  f3 e = splat3(0.0f);
+  float ws = 0.0f;
+  uint32_t hidden = 0;
+  for (int k = 0; k < 8; k++) {
+    f3 pr;
+    float w;
+    uint32_t q;
+    if (!gather_corner(a, x, nn, b, f, k, pr, w, q)) continue;
+    if (occl && seg_occluded(a.vx, xo1, pr, Lo)) {
+      hidden |= 1u << k;
+      continue;
+    }
+    e = e + eval_sh(a, q, nn) * w;
+    ws += w;
+  }
+  // Every probe hidden (the offset start fell into a neighbouring face): retest from two voxels out.
+  if (ws <= 0.0f && hidden != 0u) {
+    f3 xo2 = x + nn * (2.0f * vLo);
+    for (int k = 0; k < 8; k++) {
+      if (!((hidden >> k) & 1u)) continue;
+      f3 pr;
+      float w;
+      uint32_t q;
+      gather_corner(a, x, nn, b, f, k, pr, w, q);
+      if (seg_occluded(a.vx, xo2, pr, Lo)) continue;
+      e = e + eval_sh(a, q, nn) * w;
+      ws += w;
+    }
+  }
+  f3 out = ws > 0.0f ? e / ws : splat3(0.0f);
+  irr[3 * i] = out.x;
+  irr[3 * i + 1] = out.y;
+  irr[3 * i + 2] = out.z;
+}
+
 int grid_for(uint64_t n) {
   uint64_t g = (n + BLK - 1) / BLK;
   return (int)(g < 1 ? 1 : g);
@@ -711,6 +825,11 @@ int launch_trace_merge(const TraceK& k, int c, void* stream) {
     k_trace_merge<true><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   else
     k_trace_merge<false><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+  return (int)hipGetLastError();
+}
+
+int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream) {
+  if (n) k_gather<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, pos, nrm, n, occl, irr);
   return (int)hipGetLastError();
 }
 

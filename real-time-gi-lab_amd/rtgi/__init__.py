@@ -319,6 +319,15 @@ class World:
         check(lib().rtgi_world_trace_rays(self._h, c, ptr(o), ptr(d), n, ptr(rgba), ptr(hits)))
         return rgba, hits
 
+    def gather(self, pos, nrm, occlusion=False):
+        """gatherWorld (src/core/rc.ts:871-927): irradiance (n x 3) of the last update's field at surface points."""
+        p = np.ascontiguousarray(pos, np.float32)
+        n = np.ascontiguousarray(nrm, np.float32)
+        irr = np.zeros((p.shape[0], 3), np.float32)
+        check(lib().rtgi_world_gather(self._h, ptr(p), ptr(n), p.shape[0], _L.GATHER_OCCLUSION if occlusion else 0,
+                                      ptr(irr)))
+        return irr
+
     def read_trace_stats(self):
         """Per cascade: world traces, DDA steps, traces at the 1024-step cap (rc.ts:232-235)."""
         st = _L.TraceStatsC()

@@ -48,3 +48,22 @@ def random_scene(rng, ntris, lo, hi, size, nmat=None):
     mat = np.ones((nm, 4), np.float32)
     mat[:, :3] = rng.uniform(0.1, 0.9, (nm, 3))
     return rtgi.Scene(pos=pos, nrm=nrm.astype(np.float32), tri=tri, materials=mat)
+
+
+def surface_points(scene, n, seed):
+    """n points on random triangles of the scene (uniform barycentrics) with unit face normals: a stand-in for the
+    G-buffer pixels the reference's gather reads."""
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, scene.tri.shape[0], n)
+    tri = scene.tri[t, :3].astype(np.int64)
+    a, b, c = scene.pos[tri[:, 0]], scene.pos[tri[:, 1]], scene.pos[tri[:, 2]]
+    u = rng.uniform(0, 1, (n, 2))
+    flip = u.sum(1) > 1
+    u[flip] = 1 - u[flip]
+    p = a + (b - a) * u[:, :1] + (c - a) * u[:, 1:]
+    nrm = np.cross(b - a, c - a).astype(np.float64)
+    ln = np.linalg.norm(nrm, axis=1, keepdims=True)
+    ok = ln[:, 0] > 1e-12
+    nrm = np.where(ok[:, None], nrm / np.maximum(ln, 1e-12), np.array([0.0, 1.0, 0.0]))
+    return p.astype(np.float32), nrm.astype(np.float32)
+

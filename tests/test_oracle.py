@@ -387,6 +387,60 @@ def orc_dir(di, res):
     return [fx / ln, fy / ln, nz / ln]
 
 
+def wall_world():
+    """A wall of two triangles in the plane x = 0.35 (occupying the voxels 0.25 <= x < 0.375 at v0 = 0.125)."""
+    pos = np.array([[0.35, -6, -6], [0.35, 6, -6], [0.35, 6, 6], [0.35, -6, 6]], np.float32)
+    nrm = np.tile(np.array([[1.0, 0, 0]], np.float32), (4, 1))
+    tri = np.array([[0, 1, 2, 0], [0, 2, 3, 0]], np.uint32)
+    scene = rtgi.Scene(pos=pos, nrm=nrm, tri=tri, materials=np.array([[0.5, 0.5, 0.5, 1]], np.float32))
+    vox = rtgi.VoxParams(v0=0.125, levels=3, G=8, maxBricks=3 * 512, maxVoxels=1 << 16)
+    rc = rtgi.RcParams(cascades=2, probes0=16, s0=0.5)
+    return scene, vox, rc, rtgi.Frame(eye=(0.0, 0.0, 0.0))
+
+
+def test_gather_closed_forms():
+    """gatherWorld on hand-made SH fields. (1) Every probe holds the same SH: any weighting gives that SH's value,
+    so the result is evalSH of it (L0 only: c * 0.282095 for every normal). (2) A point on a probe: the trilinear
+    weight puts 1 + 1e-4 on that probe and 1e-4 on the others, so the result is within 1e-3 of that probe's value,
+    which pins the corner order and the probe index. (3) Occlusion: probes behind a wall are left out, so a point
+    whose far-side probes are all hidden gets exactly the near side's value."""
+    scene, vox, rc, frame = wall_world()
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    N = rc.probes0
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-3, 3, (500, 3)).astype(np.float32)
+    nrm = rng.normal(size=(500, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    sh = np.zeros((N ** 3, 27), np.float32)
+    sh[:, 0:3] = [2.0, 3.0, 5.0]
+    irr = o.gather(sh, pos, nrm)
+    inside = np.all((pos / rc.s0 >= np.array(o.rc.G0[:3])) & (pos / rc.s0 < np.array(o.rc.G0[:3]) + N - 1), axis=1)
+    np.testing.assert_allclose(irr[inside], np.tile(np.float32(0.282095) * np.array([2, 3, 5], np.float32),
+                                                    (inside.sum(), 1)), rtol=2e-6)
+    # (2) distinct L0 per probe, points on probes.
+    sh2 = np.zeros_like(sh)
+    sh2[:, 0] = np.arange(N ** 3, dtype=np.float32) + 1
+    g = rng.integers(1, N - 1, (200, 3))
+    p2 = ((g + np.array(o.rc.G0[:3])) * rc.s0).astype(np.float32)
+    out = o.gather(sh2, p2, np.tile(np.array([[0, 1, 0]], np.float32), (200, 1)))
+    idx = g[:, 0] + N * (g[:, 1] + N * g[:, 2])
+    np.testing.assert_allclose(out[:, 0], np.float32(0.282095) * (idx + 1), rtol=1e-3)
+    # (3) the wall at x = 0.35: near-side probes (x <= 0) hold 1, far-side ones (x >= 0.5) hold 100.
+    G0 = np.array(o.rc.G0[:3])
+    gx = np.arange(N) + G0[0]
+    sh3 = np.zeros_like(sh)
+    L0 = np.where(gx * rc.s0 <= 0.0, 1.0, 100.0).astype(np.float32)
+    sh3[:, 0] = np.tile(L0, N * N)  # probe index x + N (y + N z): x fastest
+    m = 100
+    p3 = np.stack([np.full(m, 0.1), rng.uniform(-2, 2, m), rng.uniform(-2, 2, m)], 1).astype(np.float32)
+    n3 = np.tile(np.array([[-1.0, 0, 0]], np.float32), (m, 1))
+    occ = o.gather(sh3, p3, n3, occlusion=True)
+    np.testing.assert_allclose(occ[:, 0], np.float32(0.282095), rtol=1e-6)
+    free = o.gather(sh3, p3, n3, occlusion=False)
+    assert np.all(free[:, 0] > 1.5 * 0.282095)
+
+
 def test_trace_empty_world_and_single_voxel():
     vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
     rc = rtgi.RcParams(cascades=2, probes0=2, s0=0.5)
