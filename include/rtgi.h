@@ -55,6 +55,9 @@ typedef struct rtgi_vox_params {
 /* Flags of rtgi_rc_params.flags. */
 #define RTGI_RC_RECORD_HITS 1u /* also write the first-hit voxel id of every ray (rtgi_world_read_cascade) */
 #define RTGI_RC_NO_MERGE 2u    /* stop after the trace (the cascade buffers hold the traced intervals) */
+#define RTGI_RC_RELOCATE 16u   /* voxel relocation 'vrel' (relocateVox, rc.ts:413-472): probes are moved off occupied
+                                * voxels along a normal; merge parents and gather probes hidden by voxels are skipped
+                                * (linkParents rc.ts:559-566, segOccluded rc.ts:845-869) */
 #define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
@@ -69,10 +72,11 @@ typedef struct rtgi_rc_params {
   double k_interval;  /* kInterval: interval [b_c, b_{c+1}] * s0, b_k = k (4^k - 1) / 3 */
   double level_map;   /* levelMap */
   double vox_scale;   /* voxScale */
-  double vox_offset;  /* voxOffset (the probe normal is 0 on the dense grid, so the start offset vanishes) */
+  double vox_offset;  /* voxOffset: traces start voxOffset voxels along the probe normal (0 without relocation) */
   double vox_skip;    /* voxSkip */
   int32_t res_cap;    /* resCap: res_c = 4 << min(c, resCap) */
   uint32_t flags;     /* RTGI_RC_* */
+  double lift;        /* lift (rc.ts:53): relocated probes keep lift * spacing of free space behind them (0.25) */
 } rtgi_rc_params;
 
 /* Per-frame inputs; the part of the Frame uniform (src/core/frame.ts:4-23) the path reads, plus the eye that
@@ -206,6 +210,13 @@ int rtgi_world_read_cascade(rtgi_world* w, int32_t c, float* rgba, uint32_t* hit
  * reference has no ray query entry point; this one serves error measurements on arbitrary (non-texel) directions. */
 int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float* d, uint32_t n, float* rgba,
                           uint32_t* hits);
+
+/* Relocated probes of cascade c after the last update (RTGI_RC_RELOCATE), in grid order x + n (y + n z): positions
+ * and normals (3 floats each; normal 0 where no voxel is near, relocateVox rc.ts:413-472) and the merge parents in use
+ * (bit k: parent (g >> 1) + (k & 1, k >> 1 & 1, k >> 2 & 1) exists and is not hidden by voxels, linkParents
+ * rc.ts:559-566; 0 for the last cascade). Any pointer may be NULL. Replaces reading the probes / parents buffers
+ * (rc.ts:1066-1068). Synchronises. */
+int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, float* nrm, uint32_t* parents);
 
 /* Gather (gatherWorld, src/core/rc.ts:871-927; evalSH :819-829) of the last update's irradiance field at n surface
  * points, position pos and unit normal nrm (3 floats each) -> irradiance irr (3 floats each). The c0 probes of the grid

@@ -36,7 +36,9 @@ class OrcRc(C.Structure):
                 ("vox_skip", C.c_float), ("vox_level", C.c_int32 * 8), ("b0", C.c_float * 8), ("b1", C.c_float * 8),
                 ("G0", C.c_int32 * 3), ("sun_dir", C.c_float * 3), ("sun_e", C.c_float * 3), ("sky", C.c_float * 3),
                 ("sun_vis", C.c_float), ("n_point", C.c_int32), ("point_pos", (C.c_float * 3) * 8),
-                ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32)]
+                ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
+                ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
+                ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8)]
 
 
 _P = C.c_void_p
@@ -50,6 +52,8 @@ _SIG = {
     "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
     "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int]),
+    "orc_relocate": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P]),
+    "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
     "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                  C.c_int]),
@@ -156,6 +160,9 @@ class Oracle:
         r.res_cap = rc.resCap
         r.vox_skip = rc.voxSkip
         r.fp16 = 1 if getattr(rc, "fp16", False) else 0
+        r.vox_offset = rc.voxOffset
+        r.lift = getattr(rc, "lift", 0.25)
+        r.reloc = 1 if getattr(rc, "relocate", False) else 0
         eye = np.ascontiguousarray(frame.eye, np.float64)
         lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
                                 _p(eye))
@@ -192,6 +199,34 @@ class Oracle:
         rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), tri.shape[0], _p(mat), _p(self.buf),
                              _p(self.rep))
         assert rc == 0
+        if self.rc.reloc:
+            self.relocate()
+
+    def relocate(self):
+        """relocateVox for every cascade, then the parent visibility masks (linkParents with voxel relocation);
+        the trace, merge and gather then use them."""
+        N = self.rc.cascades
+        self._pos, self._nrm, self._vis = [], [], []
+        for c in range(N):
+            n = (self.rc.probes0 >> c) ** 3
+            pos = np.zeros((n, 3), np.float32)
+            nrm = np.zeros((n, 3), np.float32)
+            assert lib().orc_relocate(C.byref(self.vp), _p(self.buf), C.byref(self.rc), c, _p(pos), _p(nrm)) == 0
+            self._pos.append(pos)
+            self._nrm.append(nrm)
+        for c in range(N):
+            vis = np.zeros((self.rc.probes0 >> c) ** 3, np.uint32)
+            if c + 1 < N:
+                assert lib().orc_link(C.byref(self.vp), _p(self.buf), C.byref(self.rc), c, _p(self._pos[c]),
+                                      _p(self._pos[c + 1]), _p(vis)) == 0
+            self._vis.append(vis)
+            self.rc.ppos[c] = self._pos[c].ctypes.data
+            self.rc.pnrm[c] = self._nrm[c].ctypes.data
+            self.rc.pvis[c] = vis.ctypes.data
+
+    def probes(self, c):
+        """Relocated positions, normals and parent masks of cascade c (grid order)."""
+        return self._pos[c], self._nrm[c], self._vis[c]
 
     def counters(self):
         b = self.buf

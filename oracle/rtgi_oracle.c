@@ -845,13 +845,21 @@ static int cascadeOf(const orc_rc* rc, uint64_t ray, uint64_t* local) {
 }
 
 /* trace (rc.ts:600-622) -> traceWorld (rc.ts:216-245) with the probe at its grid point, normal 0. */
-/* One ray of cascade c from o along d (traceWorld, rc.ts:216-222): interval [b0, b1] s0, level voxLevelFor. */
+/* One ray of cascade c from o along d (traceWorld, rc.ts:216-222): interval [b0, b1] s0, level voxLevelFor at the
+ * interval's end, start voxOffset voxels along the probe normal n (relocated probes only; 0 on the grid). */
+static void traceFromN(const View* W, const orc_rc* rc, int c, v3 o, v3 n, float out[4], uint32_t* hid,
+                       uint32_t* steps, v3 d);
 static void traceFrom(const View* W, const orc_rc* rc, int c, v3 o, v3 d, float out[4], uint32_t* hid,
                       uint32_t* steps) {
+  traceFromN(W, rc, c, o, V3(0, 0, 0), out, hid, steps, d);
+}
+static void traceFromN(const View* W, const orc_rc* rc, int c, v3 o, v3 n, float out[4], uint32_t* hid,
+                       uint32_t* steps, v3 d) {
   float t0 = rc->b0[c] * rc->s0, t1 = rc->b1[c] * rc->s0;
   uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
   float v = voxSize(W->p, L);
-  traceVox(W, rc, o, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
+  v3 ot = rc->reloc ? vadd(o, vscale(n, rc->vox_offset * v)) : o;
+  traceVox(W, rc, ot, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
 }
 
 static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid, uint32_t* steps) {
@@ -863,6 +871,12 @@ static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4]
   int di = (int)(r % dirs);
   iv3 g = I3((rc->G0[0] >> c) + (int)(pi % n), (rc->G0[1] >> c) + (int)((pi / n) % n), (rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n)));
   float s = rc->s0 * (float)(1u << c);
+  if (rc->reloc) {
+    const float* P = rc->ppos[c] + pi * 3;
+    const float* Nn = rc->pnrm[c] + pi * 3;
+    traceFromN(W, rc, c, V3(P[0], P[1], P[2]), V3(Nn[0], Nn[1], Nn[2]), out, hid, steps, dirOf(di, res));
+    return;
+  }
   traceFrom(W, rc, c, vscale(vf(g), s), dirOf(di, res), out, hid, steps);
 }
 
@@ -939,6 +953,7 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
           if ((o.x > 0 && !(g.x & 1)) || (o.y > 0 && !(g.y & 1)) || (o.z > 0 && !(g.z & 1))) continue;
           iv3 q = iadd(ishr(g, 1), o);
           if (q.x >= un || q.y >= un || q.z >= un) continue;
+          if (rc->reloc && !((rc->pvis[c][pi] >> k) & 1u)) continue; /* hidden by voxels (rc.ts:559-566) */
           uint64_t qi = (uint64_t)q.x + (uint64_t)un * ((uint64_t)q.y + (uint64_t)un * q.z);
           const float* U = rgba + (uoff + qi * (uint64_t)ur * ur) * 4;
           v3 a;
@@ -1031,11 +1046,16 @@ static int gatherCorner(const orc_rc* rc, v3 x, v3 n, iv3 b, v3 f, int k, v3* pr
   iv3 gl = isub(g, I3(rc->G0[0], rc->G0[1], rc->G0[2]));
   if (gl.x < 0 || gl.y < 0 || gl.z < 0 || gl.x >= N || gl.y >= N || gl.z >= N) return 0;
   v3 tw = V3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
-  *pr = vscale(vf(g), rc->s0);
+  *pi = (uint64_t)gl.x + (uint64_t)N * ((uint64_t)gl.y + (uint64_t)N * (uint64_t)gl.z);
+  if (rc->reloc) {
+    const float* P = rc->ppos[0] + *pi * 3;
+    *pr = V3(P[0], P[1], P[2]);
+  } else {
+    *pr = vscale(vf(g), rc->s0);
+  }
   v3 dv = vsub(*pr, x);
   float side = vdot(n, dv) / fmaxf(sqrtf(vdot(dv, dv)), 1e-5f);
   *w = (tw.x * tw.y * tw.z + 1e-4f) * ((side + 1.0f) * (side + 1.0f) * 0.25f + 0.02f);
-  *pi = (uint64_t)gl.x + (uint64_t)N * ((uint64_t)gl.y + (uint64_t)N * (uint64_t)gl.z);
   return 1;
 }
 
@@ -1085,6 +1105,107 @@ int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const fl
     irr[3 * i] = out.x;
     irr[3 * i + 1] = out.y;
     irr[3 * i + 2] = out.z;
+  }
+  return 0;
+}
+
+/* voxCell (rc.ts:379-386): the occupied voxel of level L at p (voxAt: a dropped brick read coarser), or 0 when
+ * the cell is empty or outside the window. */
+static int voxCell(const View* W, uint32_t L, v3 p, uint32_t* br, uint32_t* bit) {
+  const orc_vox* pp = W->p;
+  float v = voxSize(pp, L);
+  iv3 c = vi(vfloor(vdivs(vsub(p, vscale(vscale(vf(org(pp, L)), v), 8.0f)), v)));
+  int E = pp->G * 8;
+  if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= E || c.y >= E || c.z >= E) return 0;
+  uint32_t lev;
+  iv3 cell;
+  return voxAt(W, L, c, br, bit, &lev, &cell);
+}
+/* voxStoredNormal (rc.ts:387-393); a brick without attribute slots gives (0, 0, 1). */
+static v3 storedNormal(const View* W, uint32_t br, uint32_t bit) {
+  uint32_t ai = voxRank(W, br, bit);
+  uint32_t pn = ai == V_EMPTY ? 0u : W->vox[W->attrOff + ai * 2 + 1];
+  float nx = fmaxf((float)(int16_t)(uint16_t)(pn & 0xffffu) / 32767.0f, -1.0f);
+  float ny = fmaxf((float)(int16_t)(uint16_t)(pn >> 16) / 32767.0f, -1.0f);
+  v3 n = V3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
+  float q = fmx(-n.z, 0.0f);
+  n.x += n.x >= 0.0f ? -q : q;
+  n.y += n.y >= 0.0f ? -q : q;
+  return n;
+}
+
+int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm) {
+  View W;
+  makeView(&W, p, vox);
+  int n = probesOf(rc, c);
+  float s = rc->s0 * (float)(1u << c);
+  for (int64_t pi = 0; pi < (int64_t)n * n * n; pi++) {
+    iv3 g = I3((rc->G0[0] >> c) + (int)(pi % n), (rc->G0[1] >> c) + (int)((pi / n) % n),
+               (rc->G0[2] >> c) + (int)(pi / ((int64_t)n * n)));
+    v3 pp = vscale(vf(g), s);
+    uint32_t L = voxLevelFor(p, (uint32_t)rc->vox_level[c], pp);
+    uint32_t Lg = voxLevelFor(p, L + 1 < (uint32_t)p->levels ? L + 1 : (uint32_t)p->levels - 1, pp);
+    float vg = voxSize(p, Lg);
+    /* The 4^3 cube of the next level around the grid point (cubeOff, rc.ts:395). */
+    v3 grad = V3(0, 0, 0), nsum = V3(0, 0, 0);
+    float cnt = 0.0f;
+    for (int k = 0; k < 64; k++) {
+      v3 off = vscale(vsub(V3((float)(k & 3), (float)((k >> 2) & 3), (float)(k >> 4)), V3(1.5f, 1.5f, 1.5f)), vg);
+      uint32_t br, bit;
+      if (!voxCell(&W, Lg, vadd(pp, off), &br, &bit)) continue;
+      grad = vsub(grad, off);
+      cnt += 1.0f;
+      nsum = vadd(nsum, storedNormal(&W, br, bit));
+    }
+    int flag = cnt < 1.0f ? 0 : (sqrtf(vdot(grad, grad)) < 0.1f * cnt * vg ? 1 : 2);
+    v3 dir = flag == 1 ? nsum : grad;
+    if (flag != 0) {
+      if (sqrtf(vdot(dir, dir)) < 1e-4f) flag = 0;
+      else dir = vnormalize(dir);
+    }
+    v3 r = pp, nn = V3(0, 0, 0);
+    if (flag != 0) {
+      nn = dir;
+      float back = rc->lift * s;
+      float stp = fmx(0.5f * voxSize(p, L), 1.5f * s / 16.0f);
+      for (int i = 0; i <= 16; i++) {
+        float di = (float)i * stp;
+        if (di > 1.5f * s) break;
+        v3 q = vadd(pp, vscale(dir, di));
+        uint32_t br, bit;
+        if (!voxCell(&W, L, q, &br, &bit) && !voxCell(&W, L, vsub(q, vscale(dir, back)), &br, &bit)) {
+          r = q;
+          break;
+        }
+      }
+    }
+    memcpy(pos + pi * 3, &r, 12);
+    memcpy(nrm + pi * 3, &nn, 12);
+  }
+  return 0;
+}
+
+int orc_link(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const float* pos_c, const float* pos_u,
+             uint32_t* vis) {
+  if (c + 1 >= rc->cascades) return -1;
+  View W;
+  makeView(&W, p, vox);
+  int n = probesOf(rc, c), un = probesOf(rc, c + 1);
+  for (int64_t pi = 0; pi < (int64_t)n * n * n; pi++) {
+    iv3 g = I3((int)(pi % n), (int)((pi / n) % n), (int)(pi / ((int64_t)n * n)));
+    v3 a = V3(pos_c[pi * 3], pos_c[pi * 3 + 1], pos_c[pi * 3 + 2]);
+    uint32_t Lv = voxLevelFor(p, (uint32_t)rc->vox_level[c], a);
+    uint32_t m = 0;
+    for (int k = 0; k < 8; k++) {
+      iv3 o = I3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+      if ((o.x > 0 && !(g.x & 1)) || (o.y > 0 && !(g.y & 1)) || (o.z > 0 && !(g.z & 1))) continue;
+      iv3 q = iadd(ishr(g, 1), o);
+      if (q.x >= un || q.y >= un || q.z >= un) continue;
+      uint64_t qi = (uint64_t)q.x + (uint64_t)un * ((uint64_t)q.y + (uint64_t)un * q.z);
+      v3 b = V3(pos_u[qi * 3], pos_u[qi * 3 + 1], pos_u[qi * 3 + 2]);
+      if (!segOccluded(&W, a, b, Lv)) m |= 1u << k;
+    }
+    vis[pi] = m;
   }
   return 0;
 }

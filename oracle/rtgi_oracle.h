@@ -35,6 +35,11 @@ typedef struct orc_rc {
   float point_pos[8][3], point_i[8][3];
   float omega0[16];
   int32_t fp16; /* precision form: radiance quantised to f16 (RNE) where packRad stores it, SH too */
+  float vox_offset, lift;
+  int32_t reloc; /* voxel relocation: probes at ppos / pnrm, merge parents filtered by pvis (per cascade, grid order) */
+  const float* ppos[8];     /* 3 floats per probe */
+  const float* pnrm[8];     /* 3 floats per probe */
+  const uint32_t* pvis[8];  /* bit k: parent k in use */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -60,6 +65,12 @@ float orc_h2f(uint16_t h);
 
 /* Number of rays of cascade c and the total. */
 uint64_t orc_rays(const orc_rc* rc, int c);
+/* relocateVox (rc.ts:413-472) for every probe of cascade c (grid order): pos, nrm (3 floats each). */
+int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm);
+/* linkParents with voxel relocation (rc.ts:540-570): per probe of cascade c < N-1, bit k set when parent k exists and
+ * the segment between the relocated positions pos_c -> pos_u is free of occupied voxels at the probe's level. */
+int orc_link(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const float* pos_c, const float* pos_u,
+             uint32_t* vis);
 /* Trace of rays [r0, r1) of the whole ray range (cascade-major order) -> rgba, hit ids, DDA loop iterations (the
  * reference's voxSteps); any may be NULL. threads: OpenMP threads (<= 0: all). */
 int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,

@@ -153,6 +153,7 @@ TraceK trace_args(const rtgi_world* w) {
     while ((1 << rl) < w->res[c]) rl++;
     t.c[c].res_log2 = rl;
     t.c[c].tile_log2 = probe_tile_log2(w, c);
+    t.c[c].probe_off = w->probe_off[c];
     t.c[c].rays = w->ray_off[c + 1] - w->ray_off[c];
   }
   for (int a = 0; a < 3; a++) t.G0[a] = w->G0[a];
@@ -172,6 +173,12 @@ TraceK trace_args(const rtgi_world* w) {
   t.dirs = w->dirs;
   t.fp16 = (w->rp.flags & RTGI_RC_FP16) ? 1 : 0;
   t.no_merge = (w->rp.flags & RTGI_RC_NO_MERGE) ? 1 : 0;
+  t.reloc = (w->rp.flags & RTGI_RC_RELOCATE) ? 1 : 0;
+  t.lift = (float)w->rp.lift;
+  t.ppos = w->ppos;
+  t.pnrm = w->pnrm;
+  t.pvis = w->pvis;
+  t.total_probes = w->probe_off[w->N];
   t.rad = w->rad;
   t.avg = w->avg;
   t.hits = w->hits;
@@ -229,6 +236,10 @@ void free_all(rtgi_world* w) {
   dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
   dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
+  size_t np = (w->rp.flags & RTGI_RC_RELOCATE) ? (size_t)w->probe_off[w->N] : 0;
+  dfree(w, w->ppos, np * 4);
+  dfree(w, w->pnrm, np * 4);
+  dfree(w, w->pvis, np);
 }
 
 }  // namespace
@@ -247,9 +258,12 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
     e = launch_vox_build(vox_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
     TraceK t = trace_args(w);
+    e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
-    e = launch_trace(trace_args(w), s);
+    TraceK t = trace_args(w);
+    e = launch_relocate(t, s);
+    if (!e) e = launch_trace(t, s);
   } else if (stage == RTGI_STAGE_MERGE) {
     if (!(w->rp.flags & RTGI_RC_NO_MERGE) && w->tuning.fused != 1) {
       TraceK t = trace_args(w);
@@ -327,6 +341,7 @@ int world_layout(rtgi_world* w, const rtgi_vox_params* vp, const rtgi_rc_params*
     return rtgi_fail(RTGI_E_INVALID, "probes0 (%d) must be a positive multiple of 2^(cascades-1)", rp->probes0);
   if (!(rp->s0 > 0)) return rtgi_fail(RTGI_E_INVALID, "s0 must be positive");
   if (rp->res_cap < 0) return rtgi_fail(RTGI_E_INVALID, "res_cap must be >= 0");
+  if ((rp->flags & RTGI_RC_RELOCATE) && !(rp->lift >= 0)) return rtgi_fail(RTGI_E_INVALID, "lift must be >= 0");
   w->vp = *vp;
   w->rp = *rp;
   w->NL = vp->levels;
@@ -350,6 +365,7 @@ int world_layout(rtgi_world* w, const rtgi_vox_params* vp, const rtgi_rc_params*
     w->dir_off[c + 1] = w->dir_off[c] + (uint32_t)(w->res[c] * w->res[c]);
     uint64_t npc = (uint64_t)w->n[c] * w->n[c] * w->n[c];
     w->avg_off[c + 1] = w->avg_off[c] + (c > 0 ? npc * w->res[c - 1] * w->res[c - 1] : 0);
+    w->probe_off[c + 1] = w->probe_off[c] + npc;
   }
   w->ray_off[Nc] = off;
   return RTGI_OK;
@@ -373,6 +389,8 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
                nbytes<DirEntry>(w.dir_off[w.N]) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
   if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
   if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
+  if (rp->flags & RTGI_RC_RELOCATE)
+    b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -423,6 +441,12 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->sh, sh_floats(w));
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
+  if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
+    size_t np = (size_t)w->probe_off[Nc];
+    if (!e) e = dalloc(w, &w->ppos, np * 4);
+    if (!e) e = dalloc(w, &w->pnrm, np * 4);
+    if (!e) e = dalloc(w, &w->pvis, np);
+  }
   if (!e && (rp->flags & RTGI_RC_STATS)) {
     e = dalloc(w, &w->stats, 3 * RTGI_MAX_CASCADES);
     if (!e && hipMemset(w->stats, 0, 3 * RTGI_MAX_CASCADES * 8) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipMemset");
@@ -777,6 +801,31 @@ extern "C" int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, c
   return e;
 }
 
+extern "C" int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, float* nrm, uint32_t* parents) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (c < 0 || c >= w->N) return rtgi_fail(RTGI_E_INVALID, "cascade out of range");
+  if (!(w->rp.flags & RTGI_RC_RELOCATE)) return rtgi_fail(RTGI_E_STATE, "probes are not relocated (RTGI_RC_RELOCATE)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  size_t np = (size_t)w->n[c] * w->n[c] * w->n[c];
+  std::vector<float> P(np * 4), N(np * 4);
+  std::vector<uint32_t> V(np);
+  HIPCHK(hipMemcpy(P.data(), w->ppos + w->probe_off[c] * 4, np * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(N.data(), w->pnrm + w->probe_off[c] * 4, np * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(V.data(), w->pvis + w->probe_off[c], np * 4, hipMemcpyDeviceToHost));
+  for (uint32_t z = 0, p = 0; z < (uint32_t)w->n[c]; z++)
+    for (uint32_t y = 0; y < (uint32_t)w->n[c]; y++)
+      for (uint32_t x = 0; x < (uint32_t)w->n[c]; x++, p++) {
+        uint32_t d = probe_index_host(w, c, x, y, z);
+        for (int a = 0; a < 3; a++) {
+          if (pos) pos[p * 3 + a] = P[d * 4 + a];
+          if (nrm) nrm[p * 3 + a] = N[d * 4 + a];
+        }
+        if (parents) parents[p] = c + 1 < w->N ? V[d] : 0u;
+      }
+  return RTGI_OK;
+}
+
 extern "C" int rtgi_world_gather(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags,
                                  float* irr) {
   if (!w || (n && (!pos || !nrm || !irr))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather: null argument");
@@ -927,6 +976,7 @@ int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
   e = launch_vox_build_batch(p, s);
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  if (!e) e = launch_relocate_batch(p, s);
   if (p.fused) {
     for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));

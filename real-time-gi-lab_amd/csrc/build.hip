@@ -775,6 +775,10 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   for (int L = 1; L < k.NL; L++) out[BL_REDUCE + L - 1] = persist;
   out[BL_ALLOC_VOXELS] = 1;
   out[BL_RESOLVE] = persist;
+  if (t.reloc) {
+    out[BL_RELOC] = cdiv(t.total_probes, BLK);
+    out[BL_LINK] = t.N > 1 ? cdiv(t.c[t.N - 1].probe_off, BLK) : 0;
+  }
   out[BL_TRACE] = cdiv(t.total_rays, BLK);
   for (int st = 0; st < t.N; st++) {
     int c = t.N - 1 - st;

@@ -81,6 +81,7 @@ struct CascK {
   uint64_t avg_off; // cascade c >= 1: first entry of its 2x2-direction average (probes x res_{c-1}^2 float4)
   uint64_t ray_off; // first ray of the cascade
   uint64_t rays;
+  uint64_t probe_off; // first probe of the cascade in the relocated-probe arrays
 };
 
 // Direction table entry of an octahedral texel: the normalised direction (dirOf, src/core/rc.ts:308) and the
@@ -106,6 +107,12 @@ struct TraceK {
   const DirEntry* dirs;  // all cascades' direction tables
   int32_t fp16;     // rad holds 4 halves per ray and sh halves (RNE at the reference's pack points), else f32
   int32_t no_merge; // trace only: rad keeps the traced values, no averages, no SH
+  int32_t reloc;    // voxel relocation: probe positions / normals / parent visibility from the arrays below
+  float lift;       // relocated probes keep lift * spacing of free space behind them
+  float* ppos;      // [all cascades' probes] float4: relocated position (xyz) and s0 (w)
+  float* pnrm;      // [all cascades' probes] float4: probe normal (xyz; 0: none)
+  uint32_t* pvis;   // [all cascades' probes] bit k: parent k of cascade c+1 exists and is not hidden by voxels
+  uint64_t total_probes;
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
   uint32_t* hits;   // per ray (same order), or null
@@ -131,6 +138,8 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
                       uint32_t* hits, void* stream);
 // Gather of the irradiance field at n surface points (device arrays: 3 floats each) -> irr (3 floats each).
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
+// Voxel relocation of every cascade's probes, then the parent visibility masks (thread per probe).
+int launch_relocate(const TraceK& k, void* stream);
 // Threads of cascade c's merge launch, and of its fused trace+merge launch.
 uint64_t merge_threads(const TraceK& k, int c);
 uint64_t trace_merge_threads(const TraceK& k, int c);
@@ -153,6 +162,8 @@ enum BatchLaunch : int {
   BL_REDUCE,                               // + level - 1 (levels 1..7)
   BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
   BL_RESOLVE,
+  BL_RELOC,                                // relocation of every cascade's probes
+  BL_LINK,                                 // parent visibility of cascades 0 .. N-2
   BL_TRACE,
   BL_MERGE,                                // + step (cascade N-1-step of each world)
   BL_TM = BL_MERGE + RTGI_MAX_CASCADES,    // + step: trace and merge of cascade N-1-step in one launch
@@ -174,6 +185,7 @@ int launch_vox_build_batch(const BatchPlan& p, void* stream);
 int launch_trace_batch(const BatchPlan& p, void* stream);
 int launch_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);
+int launch_relocate_batch(const BatchPlan& p, void* stream);
 int launch_sh_batch(const BatchPlan& p, void* stream);
 
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).

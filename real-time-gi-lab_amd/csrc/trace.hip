@@ -74,6 +74,9 @@ __device__ __forceinline__ uint32_t buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t b
   return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
 }
 
+__device__ __forceinline__ float4* ppos4(const TraceK& a) { return reinterpret_cast<float4*>(a.ppos); }
+__device__ __forceinline__ float4* pnrm4(const TraceK& a) { return reinterpret_cast<float4*>(a.pnrm); }
+
 struct VoxHit {
   uint32_t brick, bit;
   int level;
@@ -305,6 +308,28 @@ __device__ __forceinline__ void split_ray(const CascK& C, uint32_t r, uint32_t& 
   }
 }
 
+// traceWorld (src/core/rc.ts:216-222) for probe (grid point g, device index pi) of cascade c along d: interval
+// [b0, b1] s0, level voxLevelFor at the interval's end, start voxOffset voxels along the probe normal (relocated
+// probes; the grid points have normal 0), skip voxSkip voxels. ot: the traversal's origin (hits enter at ot + d t).
+template <bool ST>
+__device__ __forceinline__ void trace_probe_ray(const TraceK& a, int c, i3 g, uint32_t pi, f3 d, f3 inv, Hit& h,
+                                                f3& ot) {
+  const CascK& C = a.c[c];
+  f3 o, n = splat3(0.0f);
+  if (a.reloc) {
+    float4 P = ppos4(a)[C.probe_off + pi], N = pnrm4(a)[C.probe_off + pi];
+    o = mk3(P.x, P.y, P.z);
+    n = mk3(N.x, N.y, N.z);
+  } else {
+    o = tof3(g) * (a.s0 * (float)(1u << c));
+  }
+  float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
+  int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
+  float v = vox_size(a.vx, L);
+  ot = a.reloc ? o + n * (a.vox_offset * v) : o;
+  h = trace_vox<ST>(a, ot, d, inv, t0, t1, a.vox_skip * v, L);
+}
+
 template <bool ST>
 __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   uint64_t ray = (uint64_t)lb * BLK + threadIdx.x;
@@ -314,18 +339,15 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
   uint32_t di, pi;
   split_ray(C, (uint32_t)(ray - C.ray_off), di, pi);
   i3 g = i3{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c} + probe_xyz(C, pi);
-  float s = a.s0 * (float)(1u << c);
-  f3 o = tof3(g) * s;
   const DirEntry& de = a.dirs[C.dir_off + di];
   f3 d = mk3(de.d[0], de.d[1], de.d[2]);
   f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
-  float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
-  int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
-  float v = vox_size(a.vx, L);
-  Hit h = trace_vox<ST>(a, o, d, inv, t0, t1, a.vox_skip * v, L);
+  Hit h;
+  f3 ot;
+  trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (h.hit) {
-    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
+    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn);
     out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
   rad_store(a, ray, out);
@@ -428,16 +450,20 @@ __device__ __forceinline__ uint32_t parent_index(const CascK& U, i3 g, int kk) {
 // existing parents in kk order, divided by their count. avg holds cascade c+1's upperAvg at c's directions. Every
 // parent value is loaded before any is used, so a thread's loads are in flight together. The count is a power of two,
 // for which acc * (1 / n) is the correctly rounded acc / n.
-__device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t di) {
+// With relocation, parents hidden by voxels are left out (linkParents, rc.ts:559-566); their count is then any of
+// 0..8, so the mean divides (0 parents: 0, as mergeWorld's select gives).
+__device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t pi, uint32_t di) {
   const CascK& U = a.c[c + 1];
   const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
   uint32_t m = parent_mask(U, g);
+  if (a.reloc) m &= a.pvis[a.c[c].probe_off + pi];
   float4 x[8];
   for (int kk = 0; kk < 8; kk++)
     x[kk] = ((m >> kk) & 1u) ? A[parent_index(U, g, kk)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   f3 acc = splat3(0.0f);
   for (int kk = 0; kk < 8; kk++)
     if ((m >> kk) & 1u) acc = acc + mk3(x[kk].x, x[kk].y, x[kk].z);
+  if (a.reloc) return m ? acc / fmaxf((float)__popc(m), 1.0f) : splat3(0.0f);
   return acc * (1.0f / (float)__popc(m));
 }
 
@@ -447,8 +473,8 @@ __device__ __forceinline__ float4 merged(const TraceK& a, int c, i3 g, uint32_t 
   f3 up = (c + 1 < a.N && v.w > 0.0f) ? um : mk3(a.sky[0], a.sky[1], a.sky[2]);
   return make_float4(v.x + v.w * up.x, v.y + v.w * up.y, v.z + v.w * up.z, 1.0f);
 }
-__device__ __forceinline__ float4 merge_ray(const TraceK& a, int c, i3 g, uint32_t di, size_t i) {
-  f3 um = c + 1 < a.N ? upper_mean(a, c, g, di) : splat3(0.0f);
+__device__ __forceinline__ float4 merge_ray(const TraceK& a, int c, i3 g, uint32_t pi, uint32_t di, size_t i) {
+  f3 um = c + 1 < a.N ? upper_mean(a, c, g, pi, di) : splat3(0.0f);
   return rad_store(a, i, merged(a, c, g, di, rad_load(a, i), um));
 }
 
@@ -465,7 +491,7 @@ __device__ __forceinline__ void merge_up_body(const TraceK& a, int c, uint32_t l
     if (t >= C.rays) return;
     uint32_t di, pi;
     split_ray(C, (uint32_t)t, di, pi);
-    float4 m = merge_ray(a, c, probe_xyz(C, pi), di, C.ray_off + t);
+    float4 m = merge_ray(a, c, probe_xyz(C, pi), pi, di, C.ray_off + t);
     AV[(size_t)di * C.np + pi] = make_float4(m.x, m.y, m.z, 0.0f);
     return;
   }
@@ -478,7 +504,7 @@ __device__ __forceinline__ void merge_up_body(const TraceK& a, int c, uint32_t l
   uint32_t lu = dl & ((uint32_t)D.res - 1u), lw = dl >> D.res_log2;
   uint32_t di = ((2u * lw + (j >> 1)) << C.res_log2) + 2u * lu + (j & 1u);
   float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (ok) m = merge_ray(a, c, probe_xyz(C, pi), di, C.ray_off + (size_t)di * C.np + pi);
+  if (ok) m = merge_ray(a, c, probe_xyz(C, pi), pi, di, C.ray_off + (size_t)di * C.np + pi);
   uint32_t l0 = lane & 15u;
   f3 p00 = mk3(__shfl(m.x, l0), __shfl(m.y, l0), __shfl(m.z, l0));
   f3 p10 = mk3(__shfl(m.x, l0 + 16), __shfl(m.y, l0 + 16), __shfl(m.z, l0 + 16));
@@ -498,7 +524,7 @@ __device__ __forceinline__ void merge0_body(const TraceK& a, uint32_t lb, uint32
   if (r >= C.rays) return;
   uint32_t di, pi;
   split_ray(C, (uint32_t)r, di, pi);
-  merge_ray(a, 0, probe_xyz(C, pi), di, C.ray_off + r);
+  merge_ray(a, 0, probe_xyz(C, pi), pi, di, C.ray_off + r);
 }
 __global__ void __launch_bounds__(BLK) k_merge0(TraceK a) { merge0_body(a, blockIdx.x, gridDim.x); }
 
@@ -605,18 +631,15 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
   uint32_t steps = 0;
   if (ok) {
     i3 g = i3{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c} + probe_xyz(C, pi);
-    float s = a.s0 * (float)(1u << c);
-    f3 o = tof3(g) * s;
     const DirEntry& de = a.dirs[C.dir_off + di];
     f3 d = mk3(de.d[0], de.d[1], de.d[2]);
     f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
-    float t0 = C.b0 * a.s0, t1 = C.b1 * a.s0;
-    int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
-    float v = vox_size(a.vx, L);
-    Hit h = trace_vox<ST>(a, o, d, inv, t0, t1, a.vox_skip * v, L);
+    Hit h;
+    f3 ot;
+    trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
     float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     if (h.hit) {
-      f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, o + d * h.tIn);
+      f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn);
       out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
     }
     size_t ray = C.ray_off + (size_t)di * C.np + pi;
@@ -625,7 +648,7 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
     if (a.no_merge) {
       m = rad_store(a, ray, out);
     } else {
-      f3 um = c + 1 < a.N ? upper_mean(a, c, probe_xyz(C, pi), di) : splat3(0.0f);
+      f3 um = c + 1 < a.N ? upper_mean(a, c, probe_xyz(C, pi), pi, di) : splat3(0.0f);
       m = rad_store(a, ray, merged(a, c, g, di, rad_round(a, out), um));
     }
   }
@@ -732,11 +755,16 @@ __device__ __forceinline__ bool gather_corner(const TraceK& a, f3 x, f3 n, i3 b,
   i3 gl = g - i3{a.G0[0], a.G0[1], a.G0[2]};
   if (!inside3(gl, C.n)) return false;
   f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
-  pr = tof3(g) * a.s0;
+  q = probe_index(C, gl);
+  if (a.reloc) {
+    float4 P = ppos4(a)[C.probe_off + q];
+    pr = mk3(P.x, P.y, P.z);
+  } else {
+    pr = tof3(g) * a.s0;
+  }
   f3 dv = pr - x;
   float side = dot3(n, dv) / fmaxf(sqrtf(dot3(dv, dv)), 1e-5f);
   w = (tw.x * tw.y * tw.z + 1e-4f) * ((side + 1.0f) * (side + 1.0f) * 0.25f + 0.02f);
-  q = probe_index(C, gl);
   return true;
 }
 
@@ -788,6 +816,139 @@ __global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, cons
   irr[3 * i + 2] = out.z;
 }
 
+// ---- Voxel relocation (relocateVox, src/core/rc.ts:413-472) and parent visibility (linkParents, :559-566) ----
+
+// voxCell (rc.ts:379-386): the occupied voxel of level L at p (a dropped brick read coarser), or false when the
+// cell is empty or outside the window.
+__device__ __forceinline__ bool vox_cell(const VoxK& k, int L, f3 p, VoxHit& h) {
+  float v = vox_size(k, L);
+  i3 c = cell_at(p - tof3(origin(k, L)) * v * 8.0f, v, k.inv_v0 * inv_pow2(L), k.pow2 != 0);
+  if (!inside3(c, k.G * 8)) return false;
+  return vox_at(k, L, c, h);
+}
+// voxStoredNormal (rc.ts:387-393): the oct-decoded stored normal (unnormalised); a brick without attribute slots
+// (voxel-pool overflow, where the reference reads past its array) gives (0, 0, 1).
+__device__ __forceinline__ f3 stored_normal(const VoxK& k, const VoxHit& h) {
+  uint32_t ai = vox_rank(k, h.brick, h.bit);
+  uint32_t pn = ai == ATTR_NONE ? 0u : k.attr[ai * 2u + 1u];
+  float nx = snorm16_to_f(pn & 0xffffu), ny = snorm16_to_f(pn >> 16);
+  f3 n = mk3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
+  float q = fmaxf(-n.z, 0.0f);
+  n.x += n.x >= 0.0f ? -q : q;
+  n.y += n.y >= 0.0f ? -q : q;
+  return n;
+}
+
+// Probe (grid point g) of cascade c: the normal points away from the occupied voxels of the next level in the 4^3
+// cube around the grid point (their stored normals decide when that is ambiguous); the probe then moves along it,
+// in steps of max(v/2, 1.5 s/16) up to 1.5 s, to the first point whose voxel and the voxel lift * s behind it are
+// empty. One thread per probe, every sum in the reference's serial order.
+__device__ void relocate_probe(const TraceK& a, int c, uint32_t pi, float4& pos, float4& nrm) {
+  const VoxK& k = a.vx;
+  const CascK& C = a.c[c];
+  i3 g = i3{a.G0[0] >> c, a.G0[1] >> c, a.G0[2] >> c} + probe_xyz(C, pi);
+  float s = a.s0 * (float)(1u << c);
+  f3 p = tof3(g) * s;
+  int L = vox_level_for(k, C.vox_level, p);
+  int Lg = vox_level_for(k, min(L + 1, k.NL - 1), p);
+  float vg = vox_size(k, Lg);
+  f3 grad = splat3(0.0f);
+  float cnt = 0.0f;
+  uint64_t occm = 0;
+  for (int q = 0; q < 64; q++) {
+    f3 off = (mk3((float)(q & 3), (float)((q >> 2) & 3), (float)(q >> 4)) - splat3(1.5f)) * vg;
+    VoxHit h;
+    if (!vox_cell(k, Lg, p + off, h)) continue;
+    grad = grad - off;
+    cnt += 1.0f;
+    occm |= 1ull << q;
+  }
+  int flag = cnt < 1.0f ? 0 : (sqrtf(dot3(grad, grad)) < 0.1f * cnt * vg ? 1 : 2);
+  f3 dir = grad;
+  if (flag == 1) {
+    f3 ns = splat3(0.0f);
+    for (int q = 0; q < 64; q++) {
+      if (!((occm >> q) & 1ull)) continue;
+      f3 off = (mk3((float)(q & 3), (float)((q >> 2) & 3), (float)(q >> 4)) - splat3(1.5f)) * vg;
+      VoxHit h;
+      vox_cell(k, Lg, p + off, h);
+      ns = ns + stored_normal(k, h);
+    }
+    dir = ns;
+  }
+  if (flag != 0) {
+    if (sqrtf(dot3(dir, dir)) < 1e-4f)
+      flag = 0;
+    else
+      dir = normalize3(dir);
+  }
+  if (flag == 0) {
+    pos = make_float4(p.x, p.y, p.z, a.s0);
+    nrm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  float back = a.lift * s;
+  float stp = fmaxf(0.5f * vox_size(k, L), 1.5f * s / 16.0f);
+  f3 r = p;
+  for (int i = 0; i <= 16; i++) {
+    float di = (float)i * stp;
+    if (di > 1.5f * s) break;
+    f3 q = p + dir * di;
+    VoxHit h;
+    if (!vox_cell(k, L, q, h) && !vox_cell(k, L, q - dir * back, h)) {
+      r = q;
+      break;
+    }
+  }
+  pos = make_float4(r.x, r.y, r.z, a.s0);
+  nrm = make_float4(dir.x, dir.y, dir.z, 0.0f);
+}
+
+__device__ __forceinline__ int probe_cascade(const TraceK& a, uint64_t t) {
+  int c = 0;
+  while (c + 1 < a.N && t >= a.c[c + 1].probe_off) c++;
+  return c;
+}
+
+__device__ __forceinline__ void relocate_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+  if (t >= a.total_probes) return;
+  int c = probe_cascade(a, t);
+  relocate_probe(a, c, (uint32_t)(t - a.c[c].probe_off), ppos4(a)[t], pnrm4(a)[t]);
+}
+
+// Parents of probe pi of cascade c < N-1 that exist and whose segment from the probe is free of occupied voxels at
+// the probe's trace level (voxOcc samples of linkParents = segOccluded's).
+__device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+  if (a.N < 2 || t >= a.c[a.N - 1].probe_off) return;
+  int c = probe_cascade(a, t);
+  const CascK& C = a.c[c];
+  const CascK& U = a.c[c + 1];
+  uint32_t pi = (uint32_t)(t - C.probe_off);
+  i3 g = probe_xyz(C, pi);
+  float4 P = ppos4(a)[t];
+  f3 pa = mk3(P.x, P.y, P.z);
+  int Lv = vox_level_for(a.vx, C.vox_level, pa);
+  uint32_t m = parent_mask(U, g), vis = 0;
+  for (int kk = 0; kk < 8; kk++) {
+    if (!((m >> kk) & 1u)) continue;
+    float4 Q = ppos4(a)[U.probe_off + parent_index(U, g, kk)];
+    if (!seg_occluded(a.vx, pa, mk3(Q.x, Q.y, Q.z), Lv)) vis |= 1u << kk;
+  }
+  a.pvis[t] = vis;
+}
+__global__ void __launch_bounds__(BLK) k_relocate(TraceK a) { relocate_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_link(TraceK a) { link_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_relocate_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  relocate_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+__global__ void __launch_bounds__(BLK) k_link_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  link_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
 int grid_for(uint64_t n) {
   uint64_t g = (n + BLK - 1) / BLK;
   return (int)(g < 1 ? 1 : g);
@@ -833,6 +994,13 @@ int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t 
   return (int)hipGetLastError();
 }
 
+int launch_relocate(const TraceK& k, void* stream) {
+  if (!k.reloc || !k.total_probes) return 0;
+  k_relocate<<<grid_for(k.total_probes), BLK, 0, (hipStream_t)stream>>>(k);
+  if (k.N > 1) k_link<<<grid_for(k.c[k.N - 1].probe_off), BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
+}
+
 int launch_merge(const TraceK& k, int c, void* stream) {
   if (c > 0) {
     k_merge_up<<<grid_for(merge_threads(k, c)), BLK, 0, (hipStream_t)stream>>>(k, c);
@@ -848,6 +1016,12 @@ int launch_sh(const TraceK& k, void* stream) {
 }
 
 static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+
+int launch_relocate_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_RELOC]) k_relocate_b<<<p.grid[BL_RELOC], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_RELOC));
+  if (p.grid[BL_LINK]) k_link_b<<<p.grid[BL_LINK], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_LINK));
+  return (int)hipGetLastError();
+}
 
 int launch_trace_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_TRACE]) {

@@ -27,6 +27,7 @@ struct rtgi_world {
   uint64_t ray_off[RTGI_MAX_CASCADES + 1] = {};
   uint32_t dir_off[RTGI_MAX_CASCADES + 1] = {};
   uint64_t avg_off[RTGI_MAX_CASCADES + 1] = {};  // [c]: cascade c's 2x2 average; [N]: total
+  uint64_t probe_off[RTGI_MAX_CASCADES + 1] = {};  // [c]: cascade c's first probe; [N]: total
   int G0[3] = {};
   float omega0[16] = {};
   rtgi_frame frame{};
@@ -46,6 +47,8 @@ struct rtgi_world {
   DirEntry* dirs = nullptr;
   float* avg = nullptr;
   unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
+  float *ppos = nullptr, *pnrm = nullptr;  // relocated probes (RTGI_RC_RELOCATE): float4 each
+  uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
   uint64_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;

@@ -106,7 +106,7 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[0], "G", &x)) vp.G = (int32_t)x;
   if (get_f64(env, argv[0], "maxBricks", &x)) vp.max_bricks = (int32_t)x;
   if (get_f64(env, argv[0], "maxVoxels", &x)) vp.max_voxels = (int32_t)x;
-  rtgi_rc_params rp = {5, 64, 0.2, 1.0, 1.0, 0.25, 1.0, 0.5, 8, 0};
+  rtgi_rc_params rp = {5, 64, 0.2, 1.0, 1.0, 0.25, 1.0, 0.5, 8, 0, 0.25};
   if (get_f64(env, argv[1], "cascades", &x)) rp.cascades = (int32_t)x;
   if (get_f64(env, argv[1], "probes0", &x)) rp.probes0 = (int32_t)x;
   if (get_f64(env, argv[1], "s0", &x)) rp.s0 = x;
@@ -117,6 +117,8 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[1], "voxSkip", &x)) rp.vox_skip = x;
   if (get_f64(env, argv[1], "resCap", &x)) rp.res_cap = (int32_t)x;
   if (get_f64(env, argv[1], "recordHits", &x) && x != 0) rp.flags |= RTGI_RC_RECORD_HITS;
+  if (get_f64(env, argv[1], "lift", &x)) rp.lift = x;
+  if (get_f64(env, argv[1], "relocate", &x) && x != 0) rp.flags |= RTGI_RC_RELOCATE;
   rtgi_world* w = NULL;
   int st = rtgi_world_create(&vp, &rp, &w);
   if (st) return throw_status(env, st);

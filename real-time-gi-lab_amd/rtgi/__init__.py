@@ -66,6 +66,8 @@ class RcParams:
     noMerge: bool = False
     fp16: bool = False       # precision form: radiance and SH stored as f16 at the reference's pack points
     stats: int = 0           # setStats (src/core/rc.ts:1233): 1 counts traces, DDA steps and step-cap hits
+    relocate: bool = False   # relocation 'vrel' (relocateVox, src/core/rc.ts:413-472) with occlusion-aware parents
+    lift: float = 0.25       # relocated probes keep lift * spacing free behind them (src/core/rc.ts:53, 88)
 
 
 DEFAULT_PARAMS = RcParams()
@@ -154,9 +156,10 @@ def _vox_c(vox):
 
 def _rc_c(rc):
     flags = ((_L.RC_RECORD_HITS if rc.recordHits else 0) | (_L.RC_NO_MERGE if rc.noMerge else 0)
-             | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0))
+             | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0)
+             | (_L.RC_RELOCATE if rc.relocate else 0))
     return _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
-                        rc.voxSkip, rc.resCap, flags)
+                        rc.voxSkip, rc.resCap, flags, rc.lift)
 
 
 def _scene_arrays(scene):
@@ -318,6 +321,15 @@ class World:
         hits = np.zeros(n, np.uint32)
         check(lib().rtgi_world_trace_rays(self._h, c, ptr(o), ptr(d), n, ptr(rgba), ptr(hits)))
         return rgba, hits
+
+    def read_probes(self, c):
+        """Relocated probes of cascade c (grid order): positions, normals (n x 3) and parent masks."""
+        n = (self.rc.probes0 >> c) ** 3
+        pos = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        par = np.zeros(n, np.uint32)
+        check(lib().rtgi_world_read_probes(self._h, c, ptr(pos), ptr(nrm), ptr(par)))
+        return pos, nrm, par
 
     def gather(self, pos, nrm, occlusion=False):
         """gatherWorld (src/core/rc.ts:871-927): irradiance (n x 3) of the last update's field at surface points."""

@@ -23,6 +23,7 @@ RC_RECORD_HITS = 1
 RC_NO_MERGE = 2
 RC_FP16 = 4
 RC_STATS = 8
+RC_RELOCATE = 16
 GATHER_OCCLUSION = 1
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15
 SCENE_ATRIUM, SCENE_CITY = 1, 2
@@ -38,7 +39,7 @@ class VoxParamsC(C.Structure):
 class RcParamsC(C.Structure):
     _fields_ = [("cascades", C.c_int32), ("probes0", C.c_int32), ("s0", C.c_double), ("k_interval", C.c_double),
                 ("level_map", C.c_double), ("vox_scale", C.c_double), ("vox_offset", C.c_double),
-                ("vox_skip", C.c_double), ("res_cap", C.c_int32), ("flags", C.c_uint32)]
+                ("vox_skip", C.c_double), ("res_cap", C.c_int32), ("flags", C.c_uint32), ("lift", C.c_double)]
 
 
 class FrameC(C.Structure):
@@ -129,6 +130,7 @@ SIGNATURES = {
     "rtgi_world_read_trace_stats": (C.c_int, [_P, C.POINTER(TraceStatsC)]),
     "rtgi_world_trace_rays": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, _P, _P]),
     "rtgi_world_gather": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
+    "rtgi_world_read_probes": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "rtgi_world_set_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_get_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_destroy": (C.c_int, [_P]),

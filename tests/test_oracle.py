@@ -387,9 +387,9 @@ def orc_dir(di, res):
     return [fx / ln, fy / ln, nz / ln]
 
 
-def wall_world():
-    """A wall of two triangles in the plane x = 0.35 (occupying the voxels 0.25 <= x < 0.375 at v0 = 0.125)."""
-    pos = np.array([[0.35, -6, -6], [0.35, 6, -6], [0.35, 6, 6], [0.35, -6, 6]], np.float32)
+def wall_world(x=0.35):
+    """A wall of two triangles in the plane x (0.35: occupying the voxels 0.25 <= x < 0.375 at v0 = 0.125)."""
+    pos = np.array([[x, -6, -6], [x, 6, -6], [x, 6, 6], [x, -6, 6]], np.float32)
     nrm = np.tile(np.array([[1.0, 0, 0]], np.float32), (4, 1))
     tri = np.array([[0, 1, 2, 0], [0, 2, 3, 0]], np.uint32)
     scene = rtgi.Scene(pos=pos, nrm=nrm, tri=tri, materials=np.array([[0.5, 0.5, 0.5, 1]], np.float32))
@@ -439,6 +439,49 @@ def test_gather_closed_forms():
     np.testing.assert_allclose(occ[:, 0], np.float32(0.282095), rtol=1e-6)
     free = o.gather(sh3, p3, n3, occlusion=False)
     assert np.all(free[:, 0] > 1.5 * 0.282095)
+
+
+def test_relocation_closed_forms():
+    """relocateVox (rc.ts:413-472) worked by hand for a wall in the plane x = 0.52 (in the level-0 voxels
+    0.5 <= x < 0.625 and the level-1 voxels 0.5 <= x < 0.75), c0 spacing 0.5, lift 0.25:
+    - a c0 probe on the grid plane x = 0.5 sits in an occupied voxel. Its 4^3 cube at level 1 has occupied cells
+      only at +0.125, so the normal is -x. Marching by max(v/2, 1.5 s/16) = 0.0625 with 0.125 of free space required
+      behind it, the first free step is the third: x = 0.3125.
+    - a c1 probe at x = 1 (spacing 1, cube at level 2 occupied at -0.25) gets normal +x and stays: its voxel and the
+      one 0.25 behind it are free.
+    - probes with no occupied voxel in their cube keep their grid point and get normal 0.
+    - linkParents: from the relocated x = 0.3125 probes, the parents at x = 1 (behind the wall) are hidden and those
+      at x = 0 are used."""
+    scene, vox, rc, frame = wall_world(x=0.52)
+    rc = dataclasses.replace(rc, relocate=True)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    G0 = np.array(o.rc.G0[:3])
+    n0 = rc.probes0
+    pos0, nrm0, vis0 = o.probes(0)
+    idx = np.arange(n0 ** 3)
+    g = np.stack([idx % n0, (idx // n0) % n0, idx // (n0 * n0)], 1) + G0
+    grid = (g * np.float32(rc.s0)).astype(np.float32)
+    wallcol = (g[:, 0] * rc.s0 == 0.5) & (np.abs(grid[:, 1]) < 3) & (np.abs(grid[:, 2]) < 3)
+    assert wallcol.sum() > 50
+    np.testing.assert_array_equal(pos0[wallcol, 0], np.float32(0.3125))
+    np.testing.assert_array_equal(pos0[wallcol, 1:], grid[wallcol, 1:])
+    np.testing.assert_array_equal(nrm0[wallcol], np.tile(np.array([[-1, 0, 0]], np.float32), (wallcol.sum(), 1)))
+    far = np.abs(grid[:, 0] - 0.52) > 1.6
+    np.testing.assert_array_equal(pos0[far], grid[far])
+    assert not np.any(nrm0[far])
+    pos1, nrm1, _ = o.probes(1)
+    n1 = n0 // 2
+    i1 = np.arange(n1 ** 3)
+    g1 = np.stack([i1 % n1, (i1 // n1) % n1, i1 // (n1 * n1)], 1) + G0 // 2
+    at1 = (g1[:, 0] == 1) & (np.abs(g1[:, 1]) < 3) & (np.abs(g1[:, 2]) < 3)
+    np.testing.assert_array_equal(pos1[at1, 0], np.float32(1.0))
+    np.testing.assert_array_equal(nrm1[at1], np.tile(np.array([[1, 0, 0]], np.float32), (at1.sum(), 1)))
+    # Parents of the relocated wall-column probes: bit k has o.x = k & 1.
+    ox1 = sum(1 << k for k in range(8) if k & 1)
+    inner = wallcol & (np.abs(grid[:, 1]) < 2) & (np.abs(grid[:, 2]) < 2)
+    assert np.all(vis0[inner] & ox1 == 0)
+    assert np.all(vis0[inner] & 1)  # the o = 0 parent (x = 0) is in use
 
 
 def test_trace_empty_world_and_single_voxel():
