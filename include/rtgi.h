@@ -58,6 +58,9 @@ typedef struct rtgi_vox_params {
 #define RTGI_RC_RELOCATE 16u   /* voxel relocation 'vrel' (relocateVox, rc.ts:413-472): probes are moved off occupied
                                 * voxels along a normal; merge parents and gather probes hidden by voxels are skipped
                                 * (linkParents rc.ts:559-566, segOccluded rc.ts:845-869) */
+#define RTGI_RC_SUN_TRACE 32u  /* per-voxel sun visibility (voxSun, voxel.ts:631-673) with the shadow-map taps replaced
+                                * by voxel shadow rays toward the sun through the clipmap levels; hit shading reads
+                                * it instead of the frame's sun_vis */
 #define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH

@@ -38,7 +38,7 @@ class OrcRc(C.Structure):
                 ("sun_vis", C.c_float), ("n_point", C.c_int32), ("point_pos", (C.c_float * 3) * 8),
                 ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
-                ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8)]
+                ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -52,6 +52,7 @@ _SIG = {
     "orc_canonical": (C.c_int, [C.POINTER(OrcVox), _P, _P, C.c_int, _P, _P, _P, _P, _P, _P]),
     "orc_rays": (C.c_uint64, [C.POINTER(OrcRc), C.c_int]),
     "orc_trace": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int]),
+    "orc_sun": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P]),
     "orc_relocate": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P]),
     "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
@@ -163,6 +164,7 @@ class Oracle:
         r.vox_offset = rc.voxOffset
         r.lift = getattr(rc, "lift", 0.25)
         r.reloc = 1 if getattr(rc, "relocate", False) else 0
+        self._sun_trace = bool(getattr(rc, "sunTrace", False))
         eye = np.ascontiguousarray(frame.eye, np.float64)
         lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
                                 _p(eye))
@@ -199,6 +201,10 @@ class Oracle:
         rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), tri.shape[0], _p(mat), _p(self.buf),
                              _p(self.rep))
         assert rc == 0
+        if self._sun_trace:
+            self.sunvis = np.zeros(self.vox.maxVoxels, np.float32)
+            assert lib().orc_sun(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(self.sunvis)) == 0
+            self.rc.sunvis = self.sunvis.ctypes.data
         if self.rc.reloc:
             self.relocate()
 

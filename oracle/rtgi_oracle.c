@@ -721,8 +721,9 @@ static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entr
   v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]);
   float ndl = fmx(vdot(n, sd), 0.0f);
   v3 out = b;
+  float vis = (rc->sunvis && ai != V_EMPTY) ? rc->sunvis[ai] : rc->sun_vis;
   if (ndl > 0.0f)
-    out = vadd(b, vscale(vscale(vmul(vdivs(alb, PI_F), V3(rc->sun_e[0], rc->sun_e[1], rc->sun_e[2])), ndl), rc->sun_vis));
+    out = vadd(b, vscale(vscale(vmul(vdivs(alb, PI_F), V3(rc->sun_e[0], rc->sun_e[1], rc->sun_e[2])), ndl), vis));
   for (int k = 0; k < rc->n_point; k++) {
     v3 l = vsub(V3(rc->point_pos[k][0], rc->point_pos[k][1], rc->point_pos[k][2]), entry);
     float r2 = vdot(l, l);
@@ -1105,6 +1106,76 @@ int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const fl
     irr[3 * i] = out.x;
     irr[3 * i + 1] = out.y;
     irr[3 * i + 2] = out.z;
+  }
+  return 0;
+}
+
+/* A shadow ray from x toward the sun: traced at level L, then from each level's window exit on at the next coarser
+ * level; occupied voxels entered within tSkip are ignored. 1 if it leaves the clipmap unhit. */
+static float sunRay(const View* W, const orc_rc* rc, v3 x, v3 d, uint32_t L, float tSkip) {
+  const orc_vox* p = W->p;
+  v3 dd = V3(fabsf(d.x) < 1e-9f ? 1e-9f : d.x, fabsf(d.y) < 1e-9f ? 1e-9f : d.y, fabsf(d.z) < 1e-9f ? 1e-9f : d.z);
+  v3 inv = V3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  float t = 0.0f;
+  for (uint32_t l = L; l < (uint32_t)p->levels; l++) {
+    float out[4];
+    uint32_t hid, st;
+    traceVox(W, rc, x, d, t, 1e30f, tSkip, l, out, &hid, &st);
+    if (hid != 0xffffffffu) return 0.0f;
+    float v = voxSize(p, l);
+    v3 lo = vscale(vscale(vf(org(p, l)), v), 8.0f);
+    float span = (float)p->G * v * 8.0f;
+    v3 hi = vadd(lo, V3(span, span, span));
+    v3 ta = vmul(vsub(lo, x), inv), tb = vmul(vsub(hi, x), inv);
+    v3 tf = vmax(ta, tb);
+    t = fmx(t, fmn(fmn(tf.x, tf.y), tf.z));
+  }
+  return 1.0f;
+}
+
+int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis) {
+  View W;
+  makeView(&W, p, vox);
+  uint32_t nb = W.vox[0] < (uint32_t)p->max_bricks ? W.vox[0] : (uint32_t)p->max_bricks;
+  v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]);
+  for (uint32_t ptr = 0; ptr < nb; ptr++) {
+    uint32_t base = W.brickOff + ptr * BW;
+    for (uint32_t bit = 0; bit < 512; bit++) {
+      if ((W.vox[base + (bit >> 5)] & (1u << (bit & 31u))) == 0u) continue;
+      uint32_t ai = voxRank(&W, ptr, bit);
+      if (ai == V_EMPTY) continue;
+      uint32_t code = W.vox[base + 17];
+      uint32_t L = code >> 27;
+      iv3 b = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+      iv3 c = iadd(imuls(b, 8), I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+      float v = voxSize(p, L);
+      v3 pc = vadd(vscale(vscale(vf(org(p, L)), v), 8.0f), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+      uint32_t pn = W.vox[W.attrOff + ai * 2 + 1];
+      float ex = fmx((float)(int16_t)(uint16_t)(pn & 0xffffu) / 32767.0f, -1.0f);
+      float ey = fmx((float)(int16_t)(uint16_t)(pn >> 16) / 32767.0f, -1.0f);
+      v3 n = V3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+      float q0 = fmx(-n.z, 0.0f);
+      n.x += n.x >= 0.0f ? -q0 : q0;
+      n.y += n.y >= 0.0f ? -q0 : q0;
+      v3 q = pc;
+      if (vdot(n, n) > 1e-4f) {
+        n = vnormalize(n);
+        float pl;
+        memcpy(&pl, &W.vox[W.plOff + ai], 4);
+        q = vadd(pc, vscale(n, pl));
+      } else {
+        n = V3(0.0f, 1.0f, 0.0f);
+      }
+      v3 ns = vdot(n, sd) < 0.0f ? vneg(n) : n;
+      v3 t1 = vnormalize(vcross(ns, fabsf(ns.y) > 0.9f ? V3(1.0f, 0.0f, 0.0f) : V3(0.0f, 1.0f, 0.0f)));
+      v3 t2 = vcross(ns, t1);
+      float h = 0.25f * v;
+      v3 o = vadd(q, vscale(ns, 0.25f * v + 0.03f));
+      v3 pp = vscale(vadd(t1, t2), h), pm = vscale(vsub(t1, t2), h);
+      float vis = 0.25f * (sunRay(&W, rc, vadd(o, pp), sd, L, v) + sunRay(&W, rc, vadd(o, pm), sd, L, v) +
+                           sunRay(&W, rc, vsub(o, pp), sd, L, v) + sunRay(&W, rc, vsub(o, pm), sd, L, v));
+      sunvis[ai] = vis;
+    }
   }
   return 0;
 }

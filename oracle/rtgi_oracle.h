@@ -40,6 +40,7 @@ typedef struct orc_rc {
   const float* ppos[8];     /* 3 floats per probe */
   const float* pnrm[8];     /* 3 floats per probe */
   const uint32_t* pvis[8];  /* bit k: parent k in use */
+  const float* sunvis;      /* traced sun visibility per attribute slot (orc_sun), or NULL: sun_vis */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -65,6 +66,9 @@ float orc_h2f(uint16_t h);
 
 /* Number of rays of cascade c and the total. */
 uint64_t orc_rays(const orc_rc* rc, int c);
+/* voxSun (voxel.ts:631-673) with voxel shadow rays for the shadow-map taps: the sun visibility of every occupied
+ * voxel with an attribute slot -> sunvis[slot] (max_voxels floats; other entries untouched). */
+int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis);
 /* relocateVox (rc.ts:413-472) for every probe of cascade c (grid order): pos, nrm (3 floats each). */
 int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm);
 /* linkParents with voxel relocation (rc.ts:540-570): per probe of cascade c < N-1, bit k set when parent k exists and

@@ -178,6 +178,7 @@ TraceK trace_args(const rtgi_world* w) {
   t.ppos = w->ppos;
   t.pnrm = w->pnrm;
   t.pvis = w->pvis;
+  t.sunvis = w->sunvis;
   t.total_probes = w->probe_off[w->N];
   t.rad = w->rad;
   t.avg = w->avg;
@@ -240,6 +241,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->ppos, np * 4);
   dfree(w, w->pnrm, np * 4);
   dfree(w, w->pvis, np);
+  dfree(w, w->sunvis, (w->rp.flags & RTGI_RC_SUN_TRACE) ? mv : 0);
 }
 
 }  // namespace
@@ -258,11 +260,13 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
     e = launch_vox_build(vox_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
     TraceK t = trace_args(w);
-    e = launch_relocate(t, s);
+    e = launch_sun(t, s);
+    if (!e) e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
     TraceK t = trace_args(w);
-    e = launch_relocate(t, s);
+    e = launch_sun(t, s);
+    if (!e) e = launch_relocate(t, s);
     if (!e) e = launch_trace(t, s);
   } else if (stage == RTGI_STAGE_MERGE) {
     if (!(w->rp.flags & RTGI_RC_NO_MERGE) && w->tuning.fused != 1) {
@@ -391,6 +395,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
   if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
   if (rp->flags & RTGI_RC_RELOCATE)
     b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
+  if (rp->flags & RTGI_RC_SUN_TRACE) b += nbytes<float>(mv);
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -441,6 +446,7 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->sh, sh_floats(w));
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
+  if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
   if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
     size_t np = (size_t)w->probe_off[Nc];
     if (!e) e = dalloc(w, &w->ppos, np * 4);
@@ -976,6 +982,7 @@ int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
   if (ev) HIPCHK(hipEventRecord(ev[0], s));
   e = launch_vox_build_batch(p, s);
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
+  if (!e) e = launch_sun_batch(p, s);
   if (!e) e = launch_relocate_batch(p, s);
   if (p.fused) {
     for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);

@@ -112,6 +112,7 @@ struct TraceK {
   float* ppos;      // [all cascades' probes] float4: relocated position (xyz) and s0 (w)
   float* pnrm;      // [all cascades' probes] float4: probe normal (xyz; 0: none)
   uint32_t* pvis;   // [all cascades' probes] bit k: parent k of cascade c+1 exists and is not hidden by voxels
+  float* sunvis;    // [max_voxels] traced sun visibility per attribute slot, or null (frame sun_vis)
   uint64_t total_probes;
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
@@ -138,6 +139,8 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
                       uint32_t* hits, void* stream);
 // Gather of the irradiance field at n surface points (device arrays: 3 floats each) -> irr (3 floats each).
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
+// Traced sun visibility of every occupied voxel (thread per brick voxel).
+int launch_sun(const TraceK& k, void* stream);
 // Voxel relocation of every cascade's probes, then the parent visibility masks (thread per probe).
 int launch_relocate(const TraceK& k, void* stream);
 // Threads of cascade c's merge launch, and of its fused trace+merge launch.
@@ -162,6 +165,7 @@ enum BatchLaunch : int {
   BL_REDUCE,                               // + level - 1 (levels 1..7)
   BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
   BL_RESOLVE,
+  BL_SUN,                                  // traced sun visibility of every occupied voxel
   BL_RELOC,                                // relocation of every cascade's probes
   BL_LINK,                                 // parent visibility of cascades 0 .. N-2
   BL_TRACE,
@@ -186,6 +190,7 @@ int launch_trace_batch(const BatchPlan& p, void* stream);
 int launch_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_relocate_batch(const BatchPlan& p, void* stream);
+int launch_sun_batch(const BatchPlan& p, void* stream);
 int launch_sh_batch(const BatchPlan& p, void* stream);
 
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).

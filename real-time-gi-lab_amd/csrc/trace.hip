@@ -149,7 +149,8 @@ __device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry) {
   f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
   float ndl = fmaxf(dot3(n, sd), 0.0f);
   f3 out = b;
-  if (ndl > 0.0f) out = b + alb / PI_F * mk3(a.sun_e[0], a.sun_e[1], a.sun_e[2]) * ndl * a.sun_vis;
+  float vis = (a.sunvis && ai != ATTR_NONE) ? a.sunvis[ai] : a.sun_vis;
+  if (ndl > 0.0f) out = b + alb / PI_F * mk3(a.sun_e[0], a.sun_e[1], a.sun_e[2]) * ndl * vis;
   for (int k = 0; k < a.n_point; k++) {
     f3 l = mk3(a.point_pos[k][0], a.point_pos[k][1], a.point_pos[k][2]) - entry;
     float r2 = dot3(l, l);
@@ -904,6 +905,77 @@ __device__ void relocate_probe(const TraceK& a, int c, uint32_t pi, float4& pos,
   nrm = make_float4(dir.x, dir.y, dir.z, 0.0f);
 }
 
+// ---- Traced sun visibility of the occupied voxels (voxSun, src/core/voxel.ts:631-673) ----
+
+// 1 if a shadow ray from x toward the sun leaves the clipmap without entering an occupied voxel: traced at level L,
+// then from where it leaves each level's window on at the next coarser level; occupied voxels entered within tSkip
+// of x are ignored (the voxel the point lies on).
+__device__ float sun_ray(const TraceK& a, f3 x, f3 d, f3 inv, int L, float tSkip) {
+  const VoxK& k = a.vx;
+  float t = 0.0f;
+  for (int l = L; l < k.NL; l++) {
+    Hit h = trace_vox<false>(a, x, d, inv, t, 1e30f, tSkip, l);
+    if (h.hit) return 0.0f;
+    float v = vox_size(k, l);
+    f3 lo = tof3(origin(k, l)) * v * 8.0f;
+    f3 hi = lo + splat3((float)k.G * v * 8.0f);
+    f3 ta = (lo - x) * inv, tb = (hi - x) * inv;
+    f3 tf = max3(ta, tb);
+    t = fmaxf(t, fminf(fminf(tf.x, tf.y), tf.z));
+  }
+  return 1.0f;
+}
+
+// The voxel's sun visibility: the mean of 2 x 2 shadow rays from points a quarter voxel along two tangents of the
+// representative's plane, (0.25 voxel + 3 cm) off it on its sunward side, as voxSun places its shadow-map taps.
+__device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  const VoxK& k = a.vx;
+  uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
+  f3 dd = mk3(fabsf(sd.x) < 1e-9f ? 1e-9f : sd.x, fabsf(sd.y) < 1e-9f ? 1e-9f : sd.y,
+              fabsf(sd.z) < 1e-9f ? 1e-9f : sd.z);
+  f3 inv = mk3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  for (uint32_t i = lb * BLK + threadIdx.x; i < nb * 512u; i += nblk * BLK) {
+    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
+    uint32_t word = k.occ[ptr * 16u + w];
+    if ((word & (1u << (bit & 31u))) == 0u) continue;
+    uint32_t ai = vox_rank(k, ptr, bit);
+    if (ai == ATTR_NONE) continue;
+    uint32_t code = k.bcode[ptr];
+    int L = (int)(code >> 27);
+    i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c = b * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    float v = vox_size(k, L);
+    f3 pc = tof3(origin(k, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+    uint32_t pn = k.attr[ai * 2u + 1u];
+    float ex = snorm16_to_f(pn & 0xffffu), ey = snorm16_to_f(pn >> 16);
+    f3 n = mk3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+    float q0 = fmaxf(-n.z, 0.0f);
+    n.x += n.x >= 0.0f ? -q0 : q0;
+    n.y += n.y >= 0.0f ? -q0 : q0;
+    f3 q = pc;
+    if (dot3(n, n) > 1e-4f) {
+      n = normalize3(n);
+      q = pc + n * k.plane[ai];
+    } else {
+      n = mk3(0.0f, 1.0f, 0.0f);
+    }
+    f3 ns = dot3(n, sd) < 0.0f ? -n : n;
+    f3 t1 = normalize3(cross3(ns, fabsf(ns.y) > 0.9f ? mk3(1.0f, 0.0f, 0.0f) : mk3(0.0f, 1.0f, 0.0f)));
+    f3 t2 = cross3(ns, t1);
+    float h = 0.25f * v;
+    f3 o = q + ns * (0.25f * v + 0.03f);
+    float vis = 0.25f * (sun_ray(a, o + (t1 + t2) * h, sd, inv, L, v) + sun_ray(a, o + (t1 - t2) * h, sd, inv, L, v) +
+                         sun_ray(a, o - (t1 + t2) * h, sd, inv, L, v) + sun_ray(a, o - (t1 - t2) * h, sd, inv, L, v));
+    a.sunvis[ai] = vis;
+  }
+}
+__global__ void __launch_bounds__(BLK) k_sun(TraceK a) { sun_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  sun_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
 __device__ __forceinline__ int probe_cascade(const TraceK& a, uint64_t t) {
   int c = 0;
   while (c + 1 < a.N && t >= a.c[c + 1].probe_off) c++;
@@ -994,6 +1066,14 @@ int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t 
   return (int)hipGetLastError();
 }
 
+int launch_sun(const TraceK& k, void* stream) {
+  if (!k.sunvis) return 0;
+  uint32_t pool = (uint32_t)k.vx.max_bricks * 512u;
+  uint32_t g = std::min<uint32_t>(2048, std::max<uint32_t>(1, (pool + BLK * 4 - 1) / (BLK * 4)));
+  k_sun<<<g, BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
+}
+
 int launch_relocate(const TraceK& k, void* stream) {
   if (!k.reloc || !k.total_probes) return 0;
   k_relocate<<<grid_for(k.total_probes), BLK, 0, (hipStream_t)stream>>>(k);
@@ -1016,6 +1096,11 @@ int launch_sh(const TraceK& k, void* stream) {
 }
 
 static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+
+int launch_sun_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_SUN]) k_sun_b<<<p.grid[BL_SUN], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SUN));
+  return (int)hipGetLastError();
+}
 
 int launch_relocate_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_RELOC]) k_relocate_b<<<p.grid[BL_RELOC], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_RELOC));

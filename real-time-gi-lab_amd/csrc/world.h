@@ -49,6 +49,7 @@ struct rtgi_world {
   unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
   float *ppos = nullptr, *pnrm = nullptr;  // relocated probes (RTGI_RC_RELOCATE): float4 each
   uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
+  float* sunvis = nullptr;                // traced sun visibility per attribute slot (RTGI_RC_SUN_TRACE)
   uint64_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;

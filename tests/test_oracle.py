@@ -484,6 +484,42 @@ def test_relocation_closed_forms():
     assert np.all(vis0[inner] & 1)  # the o = 0 parent (x = 0) is in use
 
 
+def roof_world():
+    """A floor quad at y = 0 (x, z in [-6, 6]) under a roof quad at y = 2 over x in [-6, 0]; sun straight up."""
+    q = lambda a, b, c, d: [a, b, c, d]
+    pos = np.array(q([-6, 0, -6], [6, 0, -6], [6, 0, 6], [-6, 0, 6]) + q([-6, 2, -6], [0, 2, -6], [0, 2, 6], [-6, 2, 6]),
+                   np.float32)
+    nrm = np.tile(np.array([[0.0, 1, 0]], np.float32), (8, 1))
+    tri = np.array([[0, 1, 2, 0], [0, 2, 3, 0], [4, 5, 6, 0], [4, 6, 7, 0]], np.uint32)
+    scene = rtgi.Scene(pos=pos, nrm=nrm, tri=tri, materials=np.array([[0.5, 0.6, 0.7, 1]], np.float32))
+    vox = rtgi.VoxParams(v0=0.125, levels=3, G=8, maxBricks=3 * 512, maxVoxels=1 << 17)
+    rc = rtgi.RcParams(cascades=2, probes0=16, s0=0.5)
+    return scene, vox, rc, rtgi.Frame(eye=(0.0, 1.0, 0.0), sunDir=(0.0, 1.0, 0.0))
+
+
+def test_traced_sun_visibility_closed_form():
+    """voxSun with voxel shadow rays: floor voxels under the roof see no sun, the others all of it, so a ray that
+    hits the floor under the roof returns 0 and one outside returns exactly the plain shading (visibility 1)."""
+    scene, vox, rc, frame = roof_world()
+    plain = Oracle(vox, rc, frame)
+    plain.build(scene)
+    traced = Oracle(vox, dataclasses.replace(rc, sunTrace=True), frame)
+    traced.build(scene)
+    rng = np.random.default_rng(4)
+    m = 200
+    xs = np.concatenate([rng.uniform(-3.5, -1.5, m), rng.uniform(1.5, 3.5, m)])
+    org = np.stack([xs, np.full(2 * m, 0.3), rng.uniform(-3, 3, 2 * m)], 1).astype(np.float32)
+    d = np.tile(np.array([[0.0, -1.0, 0.0]], np.float32), (2 * m, 1))
+    rp, hp = plain.trace_rays(0, org, d)
+    rt, ht = traced.trace_rays(0, org, d)
+    assert np.array_equal(hp, ht) and np.all(hp != 0xFFFFFFFF)
+    assert np.all(rp[:, :3] > 0)
+    assert np.all(rt[:m, :3] == 0)
+    assert np.array_equal(rt[m:], rp[m:])
+    vis = traced.sunvis
+    assert set(np.unique(vis[vis != 0]).tolist()) <= {0.25, 0.5, 0.75, 1.0}
+
+
 def test_trace_empty_world_and_single_voxel():
     vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
     rc = rtgi.RcParams(cascades=2, probes0=2, s0=0.5)
