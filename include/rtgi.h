@@ -61,6 +61,10 @@ typedef struct rtgi_vox_params {
 #define RTGI_RC_SUN_TRACE 32u  /* per-voxel sun visibility (voxSun, voxel.ts:631-673) with the shadow-map taps replaced
                                 * by voxel shadow rays toward the sun through the clipmap levels; hit shading reads
                                 * it instead of the frame's sun_vis */
+#define RTGI_RC_BOUNCE 64u     /* multi-bounce: hit shading adds albedo / pi times the irradiance at the hit's side of
+                                * the voxel (shadeHitV, voxBounce, voxSide, rc.ts:107-126, 198-202), injected per
+                                * voxel from the previous update's irradiance field (bnInject placement, bounce.ts:
+                                * 224-262; the field is this path's own c0 SH instead of the separate volume) */
 #define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH

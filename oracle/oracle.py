@@ -38,7 +38,8 @@ class OrcRc(C.Structure):
                 ("sun_vis", C.c_float), ("n_point", C.c_int32), ("point_pos", (C.c_float * 3) * 8),
                 ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
-                ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p)]
+                ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p),
+                ("bounce", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -55,6 +56,7 @@ _SIG = {
     "orc_sun": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P]),
     "orc_relocate": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P]),
     "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
+    "orc_bounce": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
     "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                  C.c_int]),
@@ -165,6 +167,7 @@ class Oracle:
         r.lift = getattr(rc, "lift", 0.25)
         r.reloc = 1 if getattr(rc, "relocate", False) else 0
         self._sun_trace = bool(getattr(rc, "sunTrace", False))
+        self._bounce = bool(getattr(rc, "bounce", False))
         eye = np.ascontiguousarray(frame.eye, np.float64)
         lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
                                 _p(eye))
@@ -207,6 +210,17 @@ class Oracle:
             self.rc.sunvis = self.sunvis.ctypes.data
         if self.rc.reloc:
             self.relocate()
+        if self._bounce:
+            self.bounce = np.zeros((self.vox.maxVoxels, 4), np.uint32)
+            self.rc.bounce = self.bounce.ctypes.data
+
+    def inject_bounce(self, prev, sh):
+        """Bounce injection (bnInject placement) into this frame's voxels from the field of the previous frame:
+        prev is that frame's oracle (its c0 grid and probe positions), sh its merged SH ([P0][27], grid order).
+        The trace then adds albedo / pi times the stored irradiance of the side a ray hits."""
+        assert self._bounce and self.buf is not None
+        sh = np.ascontiguousarray(sh, np.float32)
+        assert lib().orc_bounce(C.byref(self.vp), _p(self.buf), C.byref(prev.rc), _p(sh), _p(self.bounce)) == 0
 
     def relocate(self):
         """relocateVox for every cascade, then the parent visibility masks (linkParents with voxel relocation);

@@ -702,7 +702,7 @@ static int voxAt(const View* W, uint32_t L, iv3 c, uint32_t* brick, uint32_t* bi
 
 /* shadeHitV (rc.ts:108-116) at a voxel hit with the voxel's attributes (voxel.ts:150-156, 185-192), no bounce,
  * the frame's sun visibility; plus the point lights of this path (none: the reference shading). */
-static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entry) {
+static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entry, uint32_t br, uint32_t bit) {
   uint32_t pa = 0, pn = 0;
   if (ai != V_EMPTY) {
     pa = W->vox[W->attrOff + ai * 2];
@@ -717,7 +717,22 @@ static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entr
   n0.y += n0.y >= 0.0f ? -q : q;
   v3 n = vdot(n0, n0) > 0.01f ? n0 : vneg(d);
   if (vdot(n, d) > 0.0f) n = vneg(n);
-  v3 b = vscale(vdivs(alb, PI_F), 0.0f);
+  v3 e = V3(0, 0, 0);
+  if (rc->bounce && ai != V_EMPTY) {
+    /* voxBounce / voxSide (rc.ts:122-126, 198-202) at a point a quarter voxel back from the entry (voxShade). */
+    uint32_t code = W->vox[W->brickOff + br * BW + 17];
+    uint32_t L = code >> 27;
+    iv3 bc = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+    iv3 c = iadd(imuls(bc, 8), I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+    float v = voxSize(W->p, L);
+    v3 pc = vadd(vscale(vscale(vf(org(W->p, L)), v), 8.0f), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+    v3 p = vsub(entry, vscale(d, 0.25f * v));
+    float sd = vdot(vsub(p, pc), n0);
+    int back = fabsf(sd) > 0.05f * v ? !(sd > 0.0f) : vdot(n0, d) > 0.0f;
+    const uint32_t* B = rc->bounce + (size_t)ai * 4 + (back ? 2 : 0);
+    e = V3(orc_h2f((uint16_t)(B[0] & 0xffffu)), orc_h2f((uint16_t)(B[0] >> 16)), orc_h2f((uint16_t)(B[1] & 0xffffu)));
+  }
+  v3 b = vmul(vdivs(alb, PI_F), e);
   v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]);
   float ndl = fmx(vdot(n, sd), 0.0f);
   v3 out = b;
@@ -775,7 +790,7 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
       iv3 cell;
       if (voxAt(W, L, c, &br, &bit, &lev, &cell) && tIn >= tSkip) {
         *hid = hitId(p, lev, cell);
-        v3 r = shadeVoxel(W, rc, voxRank(W, br, bit), d, vadd(o, vscale(d, tIn)));
+        v3 r = shadeVoxel(W, rc, voxRank(W, br, bit), d, vadd(o, vscale(d, tIn)), br, bit);
         out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
         return;
       }
@@ -806,7 +821,7 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
     float tExit = fmn(fmn(tMax.x, tMax.y), tMax.z);
     if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
       *hid = hitId(p, L, c);
-      v3 r = shadeVoxel(W, rc, voxRank(W, ptr, bit), d, vadd(o, vscale(d, tIn)));
+      v3 r = shadeVoxel(W, rc, voxRank(W, ptr, bit), d, vadd(o, vscale(d, tIn)), ptr, bit);
       out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
       return;
     }
@@ -1060,52 +1075,94 @@ static int gatherCorner(const orc_rc* rc, v3 x, v3 n, iv3 b, v3 f, int k, v3* pr
   return 1;
 }
 
+/* The irradiance of the field described by rc (c0 grid, probe positions) at x for normal nn. */
+static v3 gatherPoint(const View* W, const orc_rc* rc, const float* sh, v3 x, v3 nn, int occlusion) {
+  const orc_vox* p = W->p;
+  v3 gp = vdivs(x, rc->s0);
+  v3 fl = vfloor(gp);
+  iv3 b = vi(fl);
+  v3 f = vsub(gp, vf(b));
+  uint32_t Lo = voxLevelFor(p, (uint32_t)rc->vox_level[0], x);
+  float vLo = voxSize(p, Lo);
+  v3 xo1 = vadd(x, vscale(nn, 1.0f * vLo));
+  v3 e = V3(0, 0, 0);
+  float ws = 0.0f;
+  uint32_t hidden = 0;
+  for (int k = 0; k < 8; k++) {
+    v3 pr;
+    float w;
+    uint64_t pi;
+    if (!gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi)) continue;
+    if (occlusion && segOccluded(W, xo1, pr, Lo)) {
+      hidden |= 1u << k;
+      continue;
+    }
+    e = vadd(e, vscale(evalSH(sh, pi, nn), w));
+    ws += w;
+  }
+  if (ws <= 0.0f && hidden) {
+    v3 xo2 = vadd(x, vscale(nn, 2.0f * vLo));
+    for (int k = 0; k < 8; k++) {
+      if (!((hidden >> k) & 1u)) continue;
+      v3 pr;
+      float w;
+      uint64_t pi;
+      gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi);
+      if (segOccluded(W, xo2, pr, Lo)) continue;
+      e = vadd(e, vscale(evalSH(sh, pi, nn), w));
+      ws += w;
+    }
+  }
+  return ws > 0.0f ? vdivs(e, ws) : V3(0, 0, 0);
+}
+
 int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const float* sh, const float* pos,
                const float* nrm, uint64_t n, int occlusion, float* irr) {
   View W;
   makeView(&W, p, vox);
   for (uint64_t i = 0; i < n; i++) {
-    v3 x = V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-    v3 nn = V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
-    v3 gp = vdivs(x, rc->s0);
-    v3 fl = vfloor(gp);
-    iv3 b = vi(fl);
-    v3 f = vsub(gp, vf(b));
-    uint32_t Lo = voxLevelFor(p, (uint32_t)rc->vox_level[0], x);
-    float vLo = voxSize(p, Lo);
-    v3 xo1 = vadd(x, vscale(nn, 1.0f * vLo));
-    v3 e = V3(0, 0, 0);
-    float ws = 0.0f;
-    uint32_t hidden = 0;
-    for (int k = 0; k < 8; k++) {
-      v3 pr;
-      float w;
-      uint64_t pi;
-      if (!gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi)) continue;
-      if (occlusion && segOccluded(&W, xo1, pr, Lo)) {
-        hidden |= 1u << k;
-        continue;
-      }
-      e = vadd(e, vscale(evalSH(sh, pi, nn), w));
-      ws += w;
-    }
-    if (ws <= 0.0f && hidden) {
-      v3 xo2 = vadd(x, vscale(nn, 2.0f * vLo));
-      for (int k = 0; k < 8; k++) {
-        if (!((hidden >> k) & 1u)) continue;
-        v3 pr;
-        float w;
-        uint64_t pi;
-        gatherCorner(rc, x, nn, b, f, k, &pr, &w, &pi);
-        if (segOccluded(&W, xo2, pr, Lo)) continue;
-        e = vadd(e, vscale(evalSH(sh, pi, nn), w));
-        ws += w;
-      }
-    }
-    v3 out = ws > 0.0f ? vdivs(e, ws) : V3(0, 0, 0);
+    v3 out = gatherPoint(&W, rc, sh, V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                         V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), occlusion);
     irr[3 * i] = out.x;
     irr[3 * i + 1] = out.y;
     irr[3 * i + 2] = out.z;
+  }
+  return 0;
+}
+
+int orc_bounce(const orc_vox* p, const uint32_t* vox, const orc_rc* field, const float* sh, uint32_t* bounce) {
+  View W;
+  makeView(&W, p, vox);
+  uint32_t nb = W.vox[0] < (uint32_t)p->max_bricks ? W.vox[0] : (uint32_t)p->max_bricks;
+  for (uint32_t ptr = 0; ptr < nb; ptr++) {
+    uint32_t base = W.brickOff + ptr * BW;
+    for (uint32_t bit = 0; bit < 512; bit++) {
+      if ((W.vox[base + (bit >> 5)] & (1u << (bit & 31u))) == 0u) continue;
+      uint32_t ai = voxRank(&W, ptr, bit);
+      if (ai == V_EMPTY) continue;
+      uint32_t code = W.vox[base + 17];
+      uint32_t L = code >> 27;
+      iv3 b = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+      iv3 c = iadd(imuls(b, 8), I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+      float v = voxSize(p, L);
+      v3 pc = vadd(vscale(vscale(vf(org(p, L)), v), 8.0f), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+      uint32_t pn = W.vox[W.attrOff + ai * 2 + 1];
+      float ex = fmx((float)(int16_t)(uint16_t)(pn & 0xffffu) / 32767.0f, -1.0f);
+      float ey = fmx((float)(int16_t)(uint16_t)(pn >> 16) / 32767.0f, -1.0f);
+      v3 n = V3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+      float q0 = fmx(-n.z, 0.0f);
+      n.x += n.x >= 0.0f ? -q0 : q0;
+      n.y += n.y >= 0.0f ? -q0 : q0;
+      v3 nf = vdot(n, n) > 0.01f ? vnormalize(n) : V3(0.0f, 1.0f, 0.0f);
+      float off = L == 0 ? 0.0f : 0.5f * v;
+      v3 ef = gatherPoint(&W, field, sh, vadd(pc, vscale(nf, off)), nf, 1);
+      v3 eb = gatherPoint(&W, field, sh, vadd(pc, vscale(vneg(nf), off)), vneg(nf), 1);
+      uint32_t* o = bounce + (size_t)ai * 4;
+      o[0] = (uint32_t)orc_f2h(ef.x) | ((uint32_t)orc_f2h(ef.y) << 16);
+      o[1] = (uint32_t)orc_f2h(ef.z) | ((uint32_t)orc_f2h(0.0f) << 16);
+      o[2] = (uint32_t)orc_f2h(eb.x) | ((uint32_t)orc_f2h(eb.y) << 16);
+      o[3] = (uint32_t)orc_f2h(eb.z) | ((uint32_t)orc_f2h(0.0f) << 16);
+    }
   }
   return 0;
 }

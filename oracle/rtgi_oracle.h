@@ -41,6 +41,7 @@ typedef struct orc_rc {
   const float* pnrm[8];     /* 3 floats per probe */
   const uint32_t* pvis[8];  /* bit k: parent k in use */
   const float* sunvis;      /* traced sun visibility per attribute slot (orc_sun), or NULL: sun_vis */
+  const uint32_t* bounce;   /* bounce irradiance per attribute slot (orc_bounce: 4 words of halves), or NULL: none */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -69,6 +70,10 @@ uint64_t orc_rays(const orc_rc* rc, int c);
 /* voxSun (voxel.ts:631-673) with voxel shadow rays for the shadow-map taps: the sun visibility of every occupied
  * voxel with an attribute slot -> sunvis[slot] (max_voxels floats; other entries untouched). */
 int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis);
+/* Bounce injection (bnInject placement, bounce.ts:224-262) from the field sh ([P0][27]) whose c0 grid, probe
+ * positions (relocated: field->ppos[0]) and parameters field describes, over the clipmap vox -> bounce (4 words per
+ * attribute slot: front rg, b, back rg, b as halves). */
+int orc_bounce(const orc_vox* p, const uint32_t* vox, const orc_rc* field, const float* sh, uint32_t* bounce);
 /* relocateVox (rc.ts:413-472) for every probe of cascade c (grid order): pos, nrm (3 floats each). */
 int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm);
 /* linkParents with voxel relocation (rc.ts:540-570): per probe of cascade c < N-1, bit k set when parent k exists and

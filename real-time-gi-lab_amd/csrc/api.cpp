@@ -179,6 +179,9 @@ TraceK trace_args(const rtgi_world* w) {
   t.pnrm = w->pnrm;
   t.pvis = w->pvis;
   t.sunvis = w->sunvis;
+  t.bounce = w->bounce;
+  t.bounce_valid = w->sh_valid ? 1 : 0;
+  for (int a = 0; a < 3; a++) t.bG0[a] = w->sh_G0[a];
   t.total_probes = w->probe_off[w->N];
   t.rad = w->rad;
   t.avg = w->avg;
@@ -242,9 +245,15 @@ void free_all(rtgi_world* w) {
   dfree(w, w->pnrm, np * 4);
   dfree(w, w->pvis, np);
   dfree(w, w->sunvis, (w->rp.flags & RTGI_RC_SUN_TRACE) ? mv : 0);
+  dfree(w, w->bounce, (w->rp.flags & RTGI_RC_BOUNCE) ? mv * 4 : 0);
 }
 
 }  // namespace
+
+void mark_field(rtgi_world* w) {
+  w->sh_valid = true;
+  for (int a = 0; a < 3; a++) w->sh_G0[a] = w->G0[a];
+}
 
 hipError_t world_wait(rtgi_world* w) {
   hipError_t e = hipStreamSynchronize(w->stream);
@@ -261,11 +270,13 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
+    if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
+    if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     if (!e) e = launch_trace(t, s);
   } else if (stage == RTGI_STAGE_MERGE) {
@@ -396,6 +407,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
   if (rp->flags & RTGI_RC_RELOCATE)
     b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
   if (rp->flags & RTGI_RC_SUN_TRACE) b += nbytes<float>(mv);
+  if (rp->flags & RTGI_RC_BOUNCE) b += nbytes<uint32_t>(mv * 4);
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -447,6 +459,7 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
   if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
+  if (!e && (rp->flags & RTGI_RC_BOUNCE)) e = dalloc(w, &w->bounce, mv * 4);
   if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
     size_t np = (size_t)w->probe_off[Nc];
     if (!e) e = dalloc(w, &w->ppos, np * 4);
@@ -594,6 +607,7 @@ extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* s
     if (w->timing) HIPCHK(hipEventRecord(w->ev[i + 1], s));
   }
   w->last_stages = stages;
+  if ((stages & RTGI_STAGE_SH) && !(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
   if (s != w->stream) {
     // Later calls on the world's own stream (readback, sync) must see this work.
     HIPCHK(hipEventRecord(w->ev[4], s));
@@ -983,6 +997,7 @@ int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
   e = launch_vox_build_batch(p, s);
   if (ev) HIPCHK(hipEventRecord(ev[1], s));
   if (!e) e = launch_sun_batch(p, s);
+  if (!e) e = launch_bounce_batch(p, s);
   if (!e) e = launch_relocate_batch(p, s);
   if (p.fused) {
     for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);
@@ -1132,7 +1147,10 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
     if (e) return e;
   }
   HIPCHK(hipEventRecord(b->done, s));
-  for (rtgi_world* w : b->worlds) w->ext_done = b->done;
+  for (rtgi_world* w : b->worlds) {
+    w->ext_done = b->done;
+    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
+  }
   return RTGI_OK;
 }
 

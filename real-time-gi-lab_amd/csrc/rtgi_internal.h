@@ -113,6 +113,9 @@ struct TraceK {
   float* pnrm;      // [all cascades' probes] float4: probe normal (xyz; 0: none)
   uint32_t* pvis;   // [all cascades' probes] bit k: parent k of cascade c+1 exists and is not hidden by voxels
   float* sunvis;    // [max_voxels] traced sun visibility per attribute slot, or null (frame sun_vis)
+  uint32_t* bounce; // [max_voxels][4] bounce irradiance per slot: front rg, b, back rg, b (halves), or null
+  int32_t bounce_valid;  // the SH holds a previous update's field (else the injected bounce is 0)
+  int32_t bG0[3];        // c0 grid origin of that field
   uint64_t total_probes;
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
@@ -141,6 +144,8 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
 // Traced sun visibility of every occupied voxel (thread per brick voxel).
 int launch_sun(const TraceK& k, void* stream);
+// Bounce injection of every occupied voxel from the previous update's field (thread per brick voxel).
+int launch_bounce(const TraceK& k, void* stream);
 // Voxel relocation of every cascade's probes, then the parent visibility masks (thread per probe).
 int launch_relocate(const TraceK& k, void* stream);
 // Threads of cascade c's merge launch, and of its fused trace+merge launch.
@@ -166,6 +171,7 @@ enum BatchLaunch : int {
   BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
   BL_RESOLVE,
   BL_SUN,                                  // traced sun visibility of every occupied voxel
+  BL_BOUNCE,                               // bounce injection from the previous field
   BL_RELOC,                                // relocation of every cascade's probes
   BL_LINK,                                 // parent visibility of cascades 0 .. N-2
   BL_TRACE,
@@ -191,6 +197,7 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_relocate_batch(const BatchPlan& p, void* stream);
 int launch_sun_batch(const BatchPlan& p, void* stream);
+int launch_bounce_batch(const BatchPlan& p, void* stream);
 int launch_sh_batch(const BatchPlan& p, void* stream);
 
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).

@@ -243,7 +243,10 @@ void sched_loop(rtgi_server* s) {
     }
     if (!e && hipEventRecord(sl.done, sl.stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipEventRecord");
     if (!e)
-      for (rtgi_world* w : ws) w->ext_done = sl.done;
+      for (rtgi_world* w : ws) {
+        w->ext_done = sl.done;
+        if (!(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
+      }
     lk.lock();
     sl.status = e;
     s->st.batches++;

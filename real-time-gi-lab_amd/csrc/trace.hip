@@ -130,7 +130,7 @@ __device__ __forceinline__ float snorm16_to_f(uint32_t h) {
 // Hit radiance of an occupied voxel: attributes, oct-decoded (unnormalised) normal, shadeHitV with no bounce
 // (src/core/rc.ts:108-116, 136-140; voxel.ts:150-156, 185-192). Point lights are an addition of this path (0 lights:
 // the reference's sun-only shading).
-__device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry) {
+__device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry, uint32_t brick, uint32_t bit) {
   uint32_t pa = 0u, pn = 0u;
   if (ai != ATTR_NONE) {
     pa = a.vx.attr[ai * 2u];
@@ -145,7 +145,24 @@ __device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry) {
   // shadeHitV
   f3 n = dot3(n0, n0) > 0.01f ? n0 : -d;
   if (dot3(n, d) > 0.0f) n = -n;
-  f3 b = alb / PI_F * 0.0f;
+  f3 e = splat3(0.0f);
+  if (a.bounce && ai != ATTR_NONE) {
+    // voxBounce (rc.ts:198-202): the side of the voxel the ray comes from, judged at a point a quarter voxel back
+    // from the entry (voxShade) against the voxel centre (voxSide, rc.ts:122-126).
+    uint32_t code = a.vx.bcode[brick];
+    int L = (int)(code >> 27);
+    i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c = bc * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    float v = vox_size(a.vx, L);
+    f3 pc = tof3(origin(a.vx, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+    f3 p = entry - d * (0.25f * v);
+    float sd = dot3(p - pc, n0);
+    bool back = fabsf(sd) > 0.05f * v ? !(sd > 0.0f) : dot3(n0, d) > 0.0f;
+    const uint32_t* B = a.bounce + (size_t)ai * 4u + (back ? 2u : 0u);
+    uint32_t w0 = B[0], w1 = B[1];
+    e = mk3(hlo(w0), hhi(w0), hlo(w1));
+  }
+  f3 b = alb / PI_F * e;
   f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
   float ndl = fmaxf(dot3(n, sd), 0.0f);
   f3 out = b;
@@ -348,7 +365,7 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
   trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (h.hit) {
-    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn);
+    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn, h.brick, h.bit);
     out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
   rad_store(a, ray, out);
@@ -396,7 +413,7 @@ __global__ void __launch_bounds__(BLK) k_trace_rays(TraceK a, int c, const float
   Hit h = trace_vox<false>(a, O, D, inv, t0, t1, a.vox_skip * v, L);
   float4 r = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (h.hit) {
-    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), D, O + D * h.tIn);
+    f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), D, O + D * h.tIn, h.brick, h.bit);
     r = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
   if (a.fp16) r = make_float4(hlo(h2(r.x, r.y)), hhi(h2(r.x, r.y)), hlo(h2(r.z, r.w)), hhi(h2(r.z, r.w)));
@@ -640,7 +657,7 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
     trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
     float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     if (h.hit) {
-      f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn);
+      f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn, h.brick, h.bit);
       out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
     }
     size_t ray = C.ray_off + (size_t)di * C.np + pi;
@@ -748,12 +765,12 @@ __device__ bool seg_occluded(const VoxK& k, f3 pa, f3 pb, int L) {
 // The 8 c0 probes around x with their gather weights: trilinear weight + 1e-4 times ((side + 1)^2 / 4 + 0.02),
 // side = cos(n, probe - x) (gatherWorld, rc.ts:886-895). Corner k: o = (k & 1, k >> 1 & 1, k >> 2 & 1); absent
 // (outside the dense grid) -> false.
-__device__ __forceinline__ bool gather_corner(const TraceK& a, f3 x, f3 n, i3 b, f3 f, int k, f3& pr, float& w,
-                                              uint32_t& q) {
+__device__ __forceinline__ bool gather_corner(const TraceK& a, const int32_t* G0, f3 x, f3 n, i3 b, f3 f, int k,
+                                              f3& pr, float& w, uint32_t& q) {
   const CascK& C = a.c[0];
   i3 o{k & 1, (k >> 1) & 1, (k >> 2) & 1};
   i3 g = b + o;
-  i3 gl = g - i3{a.G0[0], a.G0[1], a.G0[2]};
+  i3 gl = g - i3{G0[0], G0[1], G0[2]};
   if (!inside3(gl, C.n)) return false;
   f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
   q = probe_index(C, gl);
@@ -769,12 +786,8 @@ __device__ __forceinline__ bool gather_corner(const TraceK& a, f3 x, f3 n, i3 b,
   return true;
 }
 
-__global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, const float* nrm, uint32_t n, int occl,
-                                                float* irr) {
-  uint32_t i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
-  f3 x = mk3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
-  f3 nn = mk3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+// The irradiance of the field (c0 grid at G0) at x for normal nn.
+__device__ f3 gather_point(const TraceK& a, const int32_t* G0, f3 x, f3 nn, bool occl) {
   f3 gp = x / a.s0;
   i3 b = f2i3(floor3(gp));
   f3 f = gp - tof3(b);
@@ -789,7 +802,7 @@ __global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, cons
     f3 pr;
     float w;
     uint32_t q;
-    if (!gather_corner(a, x, nn, b, f, k, pr, w, q)) continue;
+    if (!gather_corner(a, G0, x, nn, b, f, k, pr, w, q)) continue;
     if (occl && seg_occluded(a.vx, xo1, pr, Lo)) {
       hidden |= 1u << k;
       continue;
@@ -805,13 +818,22 @@ __global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, cons
       f3 pr;
       float w;
       uint32_t q;
-      gather_corner(a, x, nn, b, f, k, pr, w, q);
+      gather_corner(a, G0, x, nn, b, f, k, pr, w, q);
       if (seg_occluded(a.vx, xo2, pr, Lo)) continue;
       e = e + eval_sh(a, q, nn) * w;
       ws += w;
     }
   }
-  f3 out = ws > 0.0f ? e / ws : splat3(0.0f);
+  return ws > 0.0f ? e / ws : splat3(0.0f);
+}
+
+__global__ void __launch_bounds__(BLK) k_gather(TraceK a, const float* pos, const float* nrm, uint32_t n, int occl,
+                                                float* irr) {
+  uint32_t i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  f3 x = mk3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  f3 nn = mk3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+  f3 out = gather_point(a, a.G0, x, nn, occl != 0);
   irr[3 * i] = out.x;
   irr[3 * i + 1] = out.y;
   irr[3 * i + 2] = out.z;
@@ -976,6 +998,49 @@ __global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
   sun_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
+// ---- Bounce injection (bnInject, src/core/bounce.ts:224-262) ----
+
+// Irradiance on both sides of every occupied voxel from the previous update's field: side n of a voxel centred at
+// p is read at p (level 0) or half a voxel towards that side (coarser levels: a coarse voxel's centre may lie metres
+// from the surface), by the gather with occlusion tests; stored as halves (pack2x16float). Without a previous field
+// (first update) the bounce is 0.
+__device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  const VoxK& k = a.vx;
+  uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  for (uint32_t i = lb * BLK + threadIdx.x; i < nb * 512u; i += nblk * BLK) {
+    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
+    if ((k.occ[ptr * 16u + w] & (1u << (bit & 31u))) == 0u) continue;
+    uint32_t ai = vox_rank(k, ptr, bit);
+    if (ai == ATTR_NONE) continue;
+    uint4 out = make_uint4(0u, 0u, 0u, 0u);
+    if (a.bounce_valid) {
+      uint32_t code = k.bcode[ptr];
+      int L = (int)(code >> 27);
+      i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+      i3 c = bc * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+      float v = vox_size(k, L);
+      f3 p = tof3(origin(k, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+      uint32_t pn = k.attr[ai * 2u + 1u];
+      float ex = snorm16_to_f(pn & 0xffffu), ey = snorm16_to_f(pn >> 16);
+      f3 n = mk3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+      float q0 = fmaxf(-n.z, 0.0f);
+      n.x += n.x >= 0.0f ? -q0 : q0;
+      n.y += n.y >= 0.0f ? -q0 : q0;
+      f3 nf = dot3(n, n) > 0.01f ? normalize3(n) : mk3(0.0f, 1.0f, 0.0f);
+      float off = L == 0 ? 0.0f : 0.5f * v;
+      f3 ef = gather_point(a, a.bG0, p + nf * off, nf, true);
+      f3 eb = gather_point(a, a.bG0, p + (-nf) * off, -nf, true);
+      out = make_uint4(h2(ef.x, ef.y), h2(ef.z, 0.0f), h2(eb.x, eb.y), h2(eb.z, 0.0f));
+    }
+    reinterpret_cast<uint4*>(a.bounce)[ai] = out;
+  }
+}
+__global__ void __launch_bounds__(BLK) k_bounce(TraceK a) { bounce_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_bounce_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  bounce_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
 __device__ __forceinline__ int probe_cascade(const TraceK& a, uint64_t t) {
   int c = 0;
   while (c + 1 < a.N && t >= a.c[c + 1].probe_off) c++;
@@ -1074,6 +1139,14 @@ int launch_sun(const TraceK& k, void* stream) {
   return (int)hipGetLastError();
 }
 
+int launch_bounce(const TraceK& k, void* stream) {
+  if (!k.bounce) return 0;
+  uint32_t pool = (uint32_t)k.vx.max_bricks * 512u;
+  uint32_t g = std::min<uint32_t>(2048, std::max<uint32_t>(1, (pool + BLK * 4 - 1) / (BLK * 4)));
+  k_bounce<<<g, BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
+}
+
 int launch_relocate(const TraceK& k, void* stream) {
   if (!k.reloc || !k.total_probes) return 0;
   k_relocate<<<grid_for(k.total_probes), BLK, 0, (hipStream_t)stream>>>(k);
@@ -1099,6 +1172,11 @@ static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.
 
 int launch_sun_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_SUN]) k_sun_b<<<p.grid[BL_SUN], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SUN));
+  return (int)hipGetLastError();
+}
+
+int launch_bounce_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_BOUNCE]) k_bounce_b<<<p.grid[BL_BOUNCE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_BOUNCE));
   return (int)hipGetLastError();
 }
 

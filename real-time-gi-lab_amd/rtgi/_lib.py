@@ -25,6 +25,7 @@ RC_FP16 = 4
 RC_STATS = 8
 RC_RELOCATE = 16
 RC_SUN_TRACE = 32
+RC_BOUNCE = 64
 GATHER_OCCLUSION = 1
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15
 SCENE_ATRIUM, SCENE_CITY = 1, 2

@@ -520,6 +520,55 @@ def test_traced_sun_visibility_closed_form():
     assert set(np.unique(vis[vis != 0]).tolist()) <= {0.25, 0.5, 0.75, 1.0}
 
 
+def test_bounce_closed_form():
+    """Multi-bounce (bnInject + voxBounce): the field below puts L0 = 1 on the probes at x <= 0 and 100 on those
+    beyond the wall at x = 0.35. With occlusion each wall voxel's front (+x) side gathers only far-side probes and its
+    back side only near-side ones, so the injected irradiance is f16(0.282095 * 100) in front and f16(0.282095) behind
+    (an L0 field evaluates to c * 0.282095 for every normal; SH words are coefficient-major, rgb innermost). A ray from +x hits the front face, which the sun
+    (direction x < 0) does not light, so it returns exactly albedo / pi times the front value; a ray from -x returns
+    the plain shading plus albedo / pi times the back value."""
+    scene, vox, rc, frame = wall_world()
+    plain = Oracle(vox, rc, frame)
+    plain.build(scene)
+    o = Oracle(vox, dataclasses.replace(rc, bounce=True), frame)
+    o.build(scene)
+    N = rc.probes0
+    gx = np.arange(N) + np.array(o.rc.G0[:3])[0]
+    sh = np.zeros((N ** 3, 27), np.float32)
+    sh[:, 0:3] = np.tile(np.where(gx * rc.s0 <= 0.0, 1.0, 100.0).astype(np.float32), N * N)[:, None]
+    rng = np.random.default_rng(6)
+    m = 200
+    yz = rng.uniform(-1.5, 1.5, (m, 2))
+    dx = np.tile(np.array([[1.0, 0, 0]], np.float32), (m, 1))
+    alb = np.float32(128 / 255) / np.float32(3.14159265)
+    e_front = np.float32(np.float16(np.float32(0.282095) * np.float32(100.0)))
+    e_back = np.float32(np.float16(np.float32(0.282095)))
+    # Cascade 0 rays hit the wall at level 0 (gathers at the voxel centre), cascade 1 rays at level 1 (half a voxel
+    # out along the normal).
+    cases = ((0, 0.6, -0.1), (1, 0.9, -0.4))
+    for k, (c, xf, xb) in enumerate(cases):
+        front = np.concatenate([np.full((m, 1), xf), yz], 1).astype(np.float32)
+        back = np.concatenate([np.full((m, 1), xb), yz], 1).astype(np.float32)
+        pb, ph = plain.trace_rays(c, back, dx)
+        assert np.all(ph != 0xFFFFFFFF) and np.all(ph >> 29 == c) and np.all(pb[:, :3] > 0)
+        if k == 0:
+            # Before injection the bounce term is zero: identical to the plain shading.
+            assert np.array_equal(o.trace_rays(c, back, dx)[0], pb)
+            o.inject_bounce(o, sh)
+        rf, hf = o.trace_rays(c, front, -dx)
+        assert np.all(hf != 0xFFFFFFFF) and np.all(hf >> 29 == c)
+        np.testing.assert_array_equal(rf[:, :3], np.full((m, 3), alb * e_front, np.float32))
+        rb, hb = o.trace_rays(c, back, dx)
+        assert np.array_equal(hb, ph)
+        np.testing.assert_allclose(rb[:, :3], pb[:, :3] + alb * e_back, rtol=1e-6)
+    # Every wall voxel inside the probe grid got both sides (one outside it has no corner: zero on both).
+    h = np.frombuffer(o.bounce[:o.counters()["voxels"]].tobytes(), np.float16).reshape(-1, 8)
+    inside = h[:, 0] != 0
+    assert inside.sum() > 1000
+    assert np.all(h[inside][:, [0, 1, 2]] == np.float16(e_front)) and np.all(h[inside][:, [4, 5, 6]] == np.float16(e_back))
+    assert np.all(h[~inside] == 0) and np.all(h[:, [3, 7]] == 0)
+
+
 def test_trace_empty_world_and_single_voxel():
     vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
     rc = rtgi.RcParams(cascades=2, probes0=2, s0=0.5)
