@@ -61,10 +61,12 @@ typedef struct rtgi_vox_params {
 #define RTGI_RC_SUN_TRACE 32u  /* per-voxel sun visibility (voxSun, voxel.ts:631-673) with the shadow-map taps replaced
                                 * by voxel shadow rays toward the sun through the clipmap levels; hit shading reads
                                 * it instead of the frame's sun_vis */
-#define RTGI_RC_BOUNCE 64u     /* multi-bounce: hit shading adds albedo / pi times the irradiance at the hit's side of
-                                * the voxel (shadeHitV, voxBounce, voxSide, rc.ts:107-126, 198-202), injected per
-                                * voxel from the previous update's irradiance field (bnInject placement, bounce.ts:
-                                * 224-262; the field is this path's own c0 SH instead of the separate volume) */
+#define RTGI_RC_BOUNCE 64u     /* multi-bounce (createBounce with DEFAULT_BOUNCE, bounce.ts:33, 312-379): each update
+                                * first rebuilds the bounce volume, 32 x 16 x 32 ambient-cube probes 1.2 m apart
+                                * around the eye (bnClassify, bnUpdate: 16 rays per probe whose hits read the previous
+                                * update's volume, every 2nd update per probe), then stores its irradiance on both
+                                * sides of every voxel (bnInject); cascade hits add albedo / pi times it (voxBounce,
+                                * voxSide, shadeHitV, rc.ts:107-126, 198-202) */
 #define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
@@ -224,6 +226,12 @@ int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float*
  * rc.ts:559-566; 0 for the last cascade). Any pointer may be NULL. Replaces reading the probes / parents buffers
  * (rc.ts:1066-1068). Synchronises. */
 int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, float* nrm, uint32_t* parents);
+
+/* The bounce volume the last update wrote (RTGI_RC_BOUNCE): texels [32 * 16 * 32 probes, x fastest][6 axes +x, -x,
+ * +y, -y, +z, -z][4 halves: rgb irradiance, reach along the axis (0: inactive probe)] and the probe coordinates of
+ * probe 0 (origin: probe i sits at (origin + (x, y, z)) * 1.2 m). Replaces reading the bounce texture (bounce.ts:
+ * 303-311: six axes stacked along z, the origin in an extra texel). Synchronises. */
+int rtgi_world_read_bounce_volume(rtgi_world* w, uint16_t* texels, int32_t origin[3]);
 
 /* Gather (gatherWorld, src/core/rc.ts:871-927; evalSH :819-829) of the last update's irradiance field at n surface
  * points, position pos and unit normal nrm (3 floats each) -> irradiance irr (3 floats each). The c0 probes of the grid

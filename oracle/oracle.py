@@ -39,7 +39,7 @@ class OrcRc(C.Structure):
                 ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
                 ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p),
-                ("bounce", C.c_void_p)]
+                ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3)]
 
 
 _P = C.c_void_p
@@ -56,7 +56,9 @@ _SIG = {
     "orc_sun": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P]),
     "orc_relocate": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P]),
     "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
-    "orc_bounce": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P]),
+    "orc_bn_origin": (None, [_P, _P]),
+    "orc_bounce_volume": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, C.c_uint32, _P, _P]),
+    "orc_bounce_inject": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, _P]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
     "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                  C.c_int]),
@@ -100,6 +102,18 @@ def _p(a):
 
 def f32(x):
     return np.ascontiguousarray(x, np.float32)
+
+
+BN_DIMS = (32, 16, 32)   # bounce volume probes per axis (x, y, z)
+BN_CELL = np.float32(1.2)
+
+
+def bn_origin(eye):
+    """Probe coordinates of probe 0 of the bounce volume around eye (bnOriginNow)."""
+    e = np.ascontiguousarray(eye, np.float32)
+    o = np.zeros(3, np.int32)
+    lib().orc_bn_origin(_p(e), _p(o))
+    return o
 
 
 def tri_box(c, h, a, b, cc):
@@ -168,6 +182,7 @@ class Oracle:
         r.reloc = 1 if getattr(rc, "relocate", False) else 0
         self._sun_trace = bool(getattr(rc, "sunTrace", False))
         self._bounce = bool(getattr(rc, "bounce", False))
+        self._eye = np.asarray(frame.eye, np.float32)
         eye = np.ascontiguousarray(frame.eye, np.float64)
         lib().orc_cascade_setup(C.byref(r), vox.v0, vox.levels, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale,
                                 _p(eye))
@@ -214,13 +229,28 @@ class Oracle:
             self.bounce = np.zeros((self.vox.maxVoxels, 4), np.uint32)
             self.rc.bounce = self.bounce.ctypes.data
 
-    def inject_bounce(self, prev, sh):
-        """Bounce injection (bnInject placement) into this frame's voxels from the field of the previous frame:
-        prev is that frame's oracle (its c0 grid and probe positions), sh its merged SH ([P0][27], grid order).
-        The trace then adds albedo / pi times the stored irradiance of the side a ray hits."""
+    def bounce_update(self, prev=None):
+        """The bounce volume of this frame (bnClassify + bnUpdate, hits reading the previous frame's volume) and its
+        injection into this frame's voxels (bnInject), as an update with multi-bounce runs them before the trace.
+        prev: the previous frame's (volume, origin, update number) as returned here, or None (first update).
+        Returns this frame's (volume [probes][6][4] halves as uint16, origin, update number)."""
         assert self._bounce and self.buf is not None
-        sh = np.ascontiguousarray(sh, np.float32)
-        assert lib().orc_bounce(C.byref(self.vp), _p(self.buf), C.byref(prev.rc), _p(sh), _p(self.bounce)) == 0
+        org = bn_origin(self._eye)
+        n = 1 if prev is None else prev[2] + 1
+        vol = np.zeros((BN_DIMS[0] * BN_DIMS[1] * BN_DIMS[2], 6, 4), np.uint16)
+        pv, po = (None, None) if prev is None else (np.ascontiguousarray(prev[0], np.uint16),
+                                                     np.ascontiguousarray(prev[1], np.int32))
+        assert lib().orc_bounce_volume(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(pv), _p(po), n, _p(org),
+                                       _p(vol)) == 0
+        self.inject_volume(vol, org)
+        return vol, org, n
+
+    def inject_volume(self, vol, org):
+        """bnInject from a given volume ([probes][6][4] halves as uint16; origin in probe coordinates)."""
+        assert self._bounce and self.buf is not None
+        vol = np.ascontiguousarray(vol, np.uint16)
+        org = np.ascontiguousarray(org, np.int32)
+        assert lib().orc_bounce_inject(C.byref(self.vp), _p(self.buf), _p(vol), _p(org), _p(self.bounce)) == 0
 
     def relocate(self):
         """relocateVox for every cascade, then the parent visibility masks (linkParents with voxel relocation);

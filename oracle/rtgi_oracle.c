@@ -700,6 +700,50 @@ static int voxAt(const View* W, uint32_t L, iv3 c, uint32_t* brick, uint32_t* bi
 
 #define PI_F ((float)3.14159265)
 
+static int bnInside(iv3 c) { return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < ORC_BN_DX && c.y < ORC_BN_DY && c.z < ORC_BN_DZ; }
+static const uint16_t* bnTexel(const uint16_t* vol, iv3 c, int a) {
+  return vol + ((size_t)(c.x + ORC_BN_DX * (c.y + ORC_BN_DY * c.z)) * 6 + (size_t)a) * 4;
+}
+static float bnReach(const uint16_t* vol, iv3 c, int a) { return orc_h2f(bnTexel(vol, c, a)[3]); }
+static v3 bnRgb(const uint16_t* vol, iv3 c, int a) {
+  const uint16_t* t = bnTexel(vol, c, a);
+  return V3(orc_h2f(t[0]), orc_h2f(t[1]), orc_h2f(t[2]));
+}
+/* WGSL smoothstep. */
+static float smoothStep(float lo, float hi, float x) {
+  float t = fmn(fmx((x - lo) / (hi - lo), 0.0f), 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+/* bnRead (bounce.ts:49-75), tol = half a level-0 voxel. */
+static v3 bnRead(const uint16_t* vol, const int32_t org3[3], float tol, v3 q, v3 p, v3 n) {
+  iv3 o = I3(org3[0], org3[1], org3[2]);
+  v3 g = vsub(vdivs(q, ORC_BN_CELL), vf(o));
+  iv3 bb = vi(vfloor(g));
+  v3 fr = vsub(g, vf(bb));
+  v3 n2 = vmul(n, n);
+  v3 e = V3(0, 0, 0);
+  float ws = 0.0f;
+  for (int k = 0; k < 8; k++) {
+    iv3 off = I3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+    iv3 cc = iadd(bb, off);
+    if (!bnInside(cc)) continue;
+    v3 pp = vscale(vf(iadd(o, cc)), ORC_BN_CELL);
+    v3 u = vsub(q, pp);
+    if (!(bnReach(vol, cc, 0) > 0.0f)) continue;
+    if (fabsf(u.x) > bnReach(vol, cc, u.x >= 0.0f ? 0 : 1) + tol || fabsf(u.y) > bnReach(vol, cc, u.y >= 0.0f ? 2 : 3) + tol ||
+        fabsf(u.z) > bnReach(vol, cc, u.z >= 0.0f ? 4 : 5) + tol)
+      continue;
+    v3 t3 = V3(off.x ? fr.x : 1.0f - fr.x, off.y ? fr.y : 1.0f - fr.y, off.z ? fr.z : 1.0f - fr.z);
+    float w = t3.x * t3.y * t3.z * smoothStep(-0.05f, 0.25f, vdot(vsub(pp, p), n) / ORC_BN_CELL);
+    if (w <= 0.0f) continue;
+    v3 A = bnRgb(vol, cc, n.x >= 0.0f ? 0 : 1), B = bnRgb(vol, cc, n.y >= 0.0f ? 2 : 3), C = bnRgb(vol, cc, n.z >= 0.0f ? 4 : 5);
+    v3 sum = vadd(vadd(vscale(A, n2.x), vscale(B, n2.y)), vscale(C, n2.z));
+    e = vadd(e, vscale(sum, w));
+    ws += w;
+  }
+  return ws > 1e-6f ? vdivs(e, ws) : V3(0, 0, 0);
+}
+
 /* shadeHitV (rc.ts:108-116) at a voxel hit with the voxel's attributes (voxel.ts:150-156, 185-192), no bounce,
  * the frame's sun visibility; plus the point lights of this path (none: the reference shading). */
 static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entry, uint32_t br, uint32_t bit) {
@@ -718,7 +762,25 @@ static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entr
   v3 n = vdot(n0, n0) > 0.01f ? n0 : vneg(d);
   if (vdot(n, d) > 0.0f) n = vneg(n);
   v3 e = V3(0, 0, 0);
-  if (rc->bounce && ai != V_EMPTY) {
+  if (rc->bn_prev && ai != V_EMPTY) {
+    /* voxBounce of bounce.ts:96-99: the previous volume read at the shading point, on the side the ray comes from. */
+    uint32_t code = W->vox[W->brickOff + br * BW + 17];
+    uint32_t L = code >> 27;
+    iv3 bc = I3((int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u));
+    iv3 c = iadd(imuls(bc, 8), I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+    float v = voxSize(W->p, L);
+    v3 pc = vadd(vscale(vscale(vf(org(W->p, L)), v), 8.0f), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+    v3 p = vsub(entry, vscale(d, 0.25f * v));
+    v3 nb;
+    if (vdot(n0, n0) <= 0.01f) {
+      nb = vneg(d);
+    } else {
+      float sd = vdot(vsub(p, pc), n0);
+      float side = fabsf(sd) > 0.05f * v ? (sd > 0.0f ? 1.0f : -1.0f) : (vdot(n0, d) > 0.0f ? -1.0f : 1.0f);
+      nb = vscale(n0, side);
+    }
+    e = bnRead(rc->bn_prev, rc->bn_prev_org, 0.5f * W->p->v0, p, p, nb);
+  } else if (rc->bounce && ai != V_EMPTY) {
     /* voxBounce / voxSide (rc.ts:122-126, 198-202) at a point a quarter voxel back from the entry (voxShade). */
     uint32_t code = W->vox[W->brickOff + br * BW + 17];
     uint32_t L = code >> 27;
@@ -755,8 +817,8 @@ static uint32_t hitId(const orc_vox* p, uint32_t L, iv3 c) {
 }
 
 /* traceVox, voxel.ts:121-202. out: rgb, T. */
-static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, float t1, float tSkip, uint32_t L,
-                     float out[4], uint32_t* hid, uint32_t* steps) {
+static void traceVoxT(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, float t1, float tSkip, uint32_t L,
+                      float out[4], uint32_t* hid, uint32_t* steps, float* tin) {
   const orc_vox* p = W->p;
   *hid = 0xffffffffu;
   *steps = 0;
@@ -790,6 +852,7 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
       iv3 cell;
       if (voxAt(W, L, c, &br, &bit, &lev, &cell) && tIn >= tSkip) {
         *hid = hitId(p, lev, cell);
+        if (tin) *tin = tIn;
         v3 r = shadeVoxel(W, rc, voxRank(W, br, bit), d, vadd(o, vscale(d, tIn)), br, bit);
         out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
         return;
@@ -821,6 +884,7 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
     float tExit = fmn(fmn(tMax.x, tMax.y), tMax.z);
     if ((occ & (1u << (bit & 31u))) != 0u && tIn >= tSkip) {
       *hid = hitId(p, L, c);
+      if (tin) *tin = tIn;
       v3 r = shadeVoxel(W, rc, voxRank(W, ptr, bit), d, vadd(o, vscale(d, tIn)), ptr, bit);
       out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = 0.0f;
       return;
@@ -832,6 +896,11 @@ static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, floa
     else { c.z += st.z; tMax.z += tDelta.z; }
     if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= nVox || c.y >= nVox || c.z >= nVox) break;
   }
+}
+
+static void traceVox(const View* W, const orc_rc* rc, v3 o, v3 d, float t0, float t1, float tSkip, uint32_t L,
+                     float out[4], uint32_t* hid, uint32_t* steps) {
+  traceVoxT(W, rc, o, d, t0, t1, tSkip, L, out, hid, steps, NULL);
 }
 
 /* voxInside / voxLevelFor, voxel.ts:108-119. */
@@ -1130,9 +1199,114 @@ int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const fl
   return 0;
 }
 
-int orc_bounce(const orc_vox* p, const uint32_t* vox, const orc_rc* field, const float* sh, uint32_t* bounce) {
+static int voxCell(const View* W, uint32_t L, v3 p, uint32_t* br, uint32_t* bit);
+
+void orc_bn_origin(const float eye[3], int32_t o[3]) {
+  const int dims[3] = {ORC_BN_DX, ORC_BN_DY, ORC_BN_DZ};
+  for (int a = 0; a < 3; a++) o[a] = (int32_t)floorf(eye[a] / ORC_BN_CELL) - dims[a] / 2;
+}
+
+/* sfDir (bounce.ts:155-160), evaluated in double and rounded to f32. */
+static v3 sfDir(int i, int n) {
+  double z = 1.0 - (2.0 * i + 1.0) / n;
+  double r = sqrt(fmax(0.0, 1.0 - z * z));
+  double f = i * 0.618033988749;
+  double phi = 2.0 * 3.14159265 * (f - floor(f));
+  return V3((float)(r * cos(phi)), (float)(r * sin(phi)), (float)z);
+}
+
+/* bnTrace (bounce.ts:164-175): rgb and the hit's entry distance (sky and 1e9 without a hit). */
+static void bnTrace(const View* W, const orc_rc* rc, v3 o, v3 d, float tMax, float out[4]) {
+  const orc_vox* p = W->p;
+  float b0 = 0.0f;
+  for (uint32_t k = 0; k < (uint32_t)p->levels && b0 * ORC_BN_CELL < tMax; k++) {
+    float b1 = b0 * 4.0f + 1.0f;
+    uint32_t L = voxLevelFor(p, k, vadd(o, vscale(vscale(d, b1), ORC_BN_CELL)));
+    float r[4], tin = 0.0f;
+    uint32_t hid, st;
+    traceVoxT(W, rc, o, d, b0 * ORC_BN_CELL, fmn(b1 * ORC_BN_CELL, tMax), 0.0f, L, r, &hid, &st, &tin);
+    if (r[3] < 0.5f) {
+      v3 entry = vadd(o, vscale(d, tin));
+      out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+      out[3] = vdot(vsub(entry, o), d);
+      return;
+    }
+    b0 = b1;
+  }
+  out[0] = rc->sky[0]; out[1] = rc->sky[1]; out[2] = rc->sky[2];
+  out[3] = 1e9f;
+}
+
+int orc_bounce_volume(const orc_vox* p, const uint32_t* vox, const orc_rc* rc0, const uint16_t* prev,
+                      const int32_t prev_org[3], uint32_t frame, const int32_t o3[3], uint16_t* cur) {
   View W;
   makeView(&W, p, vox);
+  orc_rc rc = *rc0;
+  rc.bounce = NULL;
+  rc.bn_prev = prev;
+  for (int a = 0; a < 3; a++) rc.bn_prev_org[a] = prev ? prev_org[a] : 0;
+  /* levelOf (bounce.ts:110) */
+  double lv = nearbyint(log2((double)(ORC_BN_CELL / p->v0)));
+  uint32_t Lc = (uint32_t)fmin(fmax(lv, 0.0), (double)(p->levels - 1));
+  v3 dirs[ORC_BN_RAYS];
+  for (int r = 0; r < ORC_BN_RAYS; r++) dirs[r] = sfDir(r, ORC_BN_RAYS);
+  const int n = ORC_BN_DX * ORC_BN_DY * ORC_BN_DZ;
+  for (int i = 0; i < n; i++) {
+    iv3 c = I3(i % ORC_BN_DX, (i / ORC_BN_DX) % ORC_BN_DY, i / (ORC_BN_DX * ORC_BN_DY));
+    iv3 wp = iadd(I3(o3[0], o3[1], o3[2]), c);
+    v3 pt = vscale(vf(wp), ORC_BN_CELL);
+    uint16_t* out = cur + (size_t)i * 24;
+    int near = 0;
+    uint32_t br, bit;
+    for (int k = 0; k < 8 && !near; k++) {
+      v3 corner = vsub(V3((float)(k & 1), (float)((k >> 1) & 1), (float)(k >> 2)), V3(0.5f, 0.5f, 0.5f));
+      near = voxCell(&W, Lc, vadd(pt, vscale(corner, ORC_BN_CELL)), &br, &bit);
+    }
+    if (!near || voxCell(&W, 0, pt, &br, &bit)) {
+      memset(out, 0, 24 * sizeof(uint16_t));
+      continue;
+    }
+    if (ORC_BN_DIV > 1 && ((uint32_t)(wp.x + wp.y + wp.z + 1048576) + frame) % ORC_BN_DIV != 0u && prev) {
+      iv3 pc = isub(wp, I3(prev_org[0], prev_org[1], prev_org[2]));
+      if (bnInside(pc) && bnReach(prev, pc, 0) > 0.0f) {
+        memcpy(out, bnTexel(prev, pc, 0), 24 * sizeof(uint16_t));
+        continue;
+      }
+    }
+    float rad[ORC_BN_RAYS][4], reach[6];
+    for (int r = 0; r < ORC_BN_RAYS; r++) bnTrace(&W, &rc, pt, dirs[r], 1e9f, rad[r]);
+    for (int a = 0; a < 6; a++) {
+      float sg = (a & 1) ? -1.0f : 1.0f;
+      v3 d = V3(a / 2 == 0 ? sg : 0.0f, a / 2 == 1 ? sg : 0.0f, a / 2 == 2 ? sg : 0.0f);
+      float r[4];
+      bnTrace(&W, &rc, pt, d, 1.5f * ORC_BN_CELL, r);
+      reach[a] = fmx(fmn(r[3], 1.5f * ORC_BN_CELL), 1e-3f);
+    }
+    for (int a = 0; a < 6; a++) {
+      float sg = (a & 1) ? -1.0f : 1.0f;
+      v3 e = V3(0, 0, 0);
+      float ws = 0.0f;
+      for (int r = 0; r < ORC_BN_RAYS; r++) {
+        float comp = a / 2 == 0 ? dirs[r].x : (a / 2 == 1 ? dirs[r].y : dirs[r].z);
+        float w = fmx(sg * comp, 0.0f);
+        e = vadd(e, vscale(V3(rad[r][0], rad[r][1], rad[r][2]), w));
+        ws += w;
+      }
+      float m = fmx(ws, 1e-6f);
+      out[a * 4 + 0] = orc_f2h(PI_F * e.x / m);
+      out[a * 4 + 1] = orc_f2h(PI_F * e.y / m);
+      out[a * 4 + 2] = orc_f2h(PI_F * e.z / m);
+      out[a * 4 + 3] = orc_f2h(reach[a]);
+    }
+  }
+  return 0;
+}
+
+int orc_bounce_inject(const orc_vox* p, const uint32_t* vox, const uint16_t* vol, const int32_t vorg[3],
+                      uint32_t* bounce) {
+  View W;
+  makeView(&W, p, vox);
+  const float tol = 0.5f * p->v0;
   uint32_t nb = W.vox[0] < (uint32_t)p->max_bricks ? W.vox[0] : (uint32_t)p->max_bricks;
   for (uint32_t ptr = 0; ptr < nb; ptr++) {
     uint32_t base = W.brickOff + ptr * BW;
@@ -1155,8 +1329,8 @@ int orc_bounce(const orc_vox* p, const uint32_t* vox, const orc_rc* field, const
       n.y += n.y >= 0.0f ? -q0 : q0;
       v3 nf = vdot(n, n) > 0.01f ? vnormalize(n) : V3(0.0f, 1.0f, 0.0f);
       float off = L == 0 ? 0.0f : 0.5f * v;
-      v3 ef = gatherPoint(&W, field, sh, vadd(pc, vscale(nf, off)), nf, 1);
-      v3 eb = gatherPoint(&W, field, sh, vadd(pc, vscale(vneg(nf), off)), vneg(nf), 1);
+      v3 ef = bnRead(vol, vorg, tol, vadd(pc, vscale(nf, off)), pc, nf);
+      v3 eb = bnRead(vol, vorg, tol, vadd(pc, vscale(vneg(nf), off)), pc, vneg(nf));
       uint32_t* o = bounce + (size_t)ai * 4;
       o[0] = (uint32_t)orc_f2h(ef.x) | ((uint32_t)orc_f2h(ef.y) << 16);
       o[1] = (uint32_t)orc_f2h(ef.z) | ((uint32_t)orc_f2h(0.0f) << 16);

@@ -41,7 +41,9 @@ typedef struct orc_rc {
   const float* pnrm[8];     /* 3 floats per probe */
   const uint32_t* pvis[8];  /* bit k: parent k in use */
   const float* sunvis;      /* traced sun visibility per attribute slot (orc_sun), or NULL: sun_vis */
-  const uint32_t* bounce;   /* bounce irradiance per attribute slot (orc_bounce: 4 words of halves), or NULL: none */
+  const uint32_t* bounce;   /* bounce irradiance per attribute slot (orc_bounce_inject: 4 words of halves), or NULL */
+  const uint16_t* bn_prev;  /* set only inside orc_bounce_volume: hits read this volume (bounce.ts voxBounce) */
+  int32_t bn_prev_org[3];
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -70,10 +72,25 @@ uint64_t orc_rays(const orc_rc* rc, int c);
 /* voxSun (voxel.ts:631-673) with voxel shadow rays for the shadow-map taps: the sun visibility of every occupied
  * voxel with an attribute slot -> sunvis[slot] (max_voxels floats; other entries untouched). */
 int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis);
-/* Bounce injection (bnInject placement, bounce.ts:224-262) from the field sh ([P0][27]) whose c0 grid, probe
- * positions (relocated: field->ppos[0]) and parameters field describes, over the clipmap vox -> bounce (4 words per
- * attribute slot: front rg, b, back rg, b as halves). */
-int orc_bounce(const orc_vox* p, const uint32_t* vox, const orc_rc* field, const float* sh, uint32_t* bounce);
+/* Bounce volume (DEFAULT_BOUNCE, bounce.ts:33): ORC_BN_DX x ORC_BN_DY x ORC_BN_DZ probes ORC_BN_CELL m apart; a volume
+ * is [probes, x fastest][6 axes +x -x +y -y +z -z][4 halves: rgb, reach]. */
+#define ORC_BN_DX 32
+#define ORC_BN_DY 16
+#define ORC_BN_DZ 32
+#define ORC_BN_RAYS 16
+#define ORC_BN_DIV 2
+#define ORC_BN_CELL 1.2f
+/* The volume origin of an eye (bnOriginNow, bounce.ts:103). */
+void orc_bn_origin(const float eye[3], int32_t org[3]);
+/* bnClassify + bnUpdate (bounce.ts:117-144, 182-208): the volume cur (origin org) of update number frame from the
+ * clipmap vox, with rc's frame lighting and per-voxel sun visibility; hits read the previous volume prev (origin
+ * prev_org; NULL: none). */
+int orc_bounce_volume(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const uint16_t* prev,
+                      const int32_t prev_org[3], uint32_t frame, const int32_t org[3], uint16_t* cur);
+/* bnInject (bounce.ts:224-262): the irradiance of volume vol (origin org) on both sides of every occupied voxel ->
+ * bounce (4 words per attribute slot: front rg, b, back rg, b as halves). */
+int orc_bounce_inject(const orc_vox* p, const uint32_t* vox, const uint16_t* vol, const int32_t org[3],
+                      uint32_t* bounce);
 /* relocateVox (rc.ts:413-472) for every probe of cascade c (grid order): pos, nrm (3 floats each). */
 int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm);
 /* linkParents with voxel relocation (rc.ts:540-570): per probe of cascade c < N-1, bit k set when parent k exists and

@@ -180,8 +180,19 @@ TraceK trace_args(const rtgi_world* w) {
   t.pvis = w->pvis;
   t.sunvis = w->sunvis;
   t.bounce = w->bounce;
-  t.bounce_valid = w->sh_valid ? 1 : 0;
-  for (int a = 0; a < 3; a++) t.bG0[a] = w->sh_G0[a];
+  if (w->bounce) {
+    t.bn_cur = w->bnvol[w->bn_par];
+    t.bn_prev = w->bnvol[1 - w->bn_par];
+    for (int a = 0; a < 3; a++) {
+      t.bn_org[a] = w->bn_org[w->bn_par][a];
+      t.bn_prev_org[a] = w->bn_org[1 - w->bn_par][a];
+    }
+    t.bn_frame = w->bn_frame;
+    // levelOf (bounce.ts:110): round(log2(max(size, 1e-4) / v0)) clamped to the levels.
+    double lv = std::nearbyint(std::log2((double)(BN_CELL / (float)w->vp.v0)));
+    t.bn_level = (int32_t)std::min<double>(std::max<double>(lv, 0.0), (double)(w->NL - 1));
+  }
+  t.bn_dir_off = w->dir_off[w->N];
   t.total_probes = w->probe_off[w->N];
   t.rad = w->rad;
   t.avg = w->avg;
@@ -237,7 +248,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->rad, rad_floats(w));
   dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
   dfree(w, w->sh, sh_floats(w));
-  dfree(w, w->dirs, (size_t)w->dir_off[w->N]);
+  dfree(w, w->dirs, (size_t)w->dir_off[w->N] + BN_RAYS);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
   dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
   size_t np = (w->rp.flags & RTGI_RC_RELOCATE) ? (size_t)w->probe_off[w->N] : 0;
@@ -246,13 +257,18 @@ void free_all(rtgi_world* w) {
   dfree(w, w->pvis, np);
   dfree(w, w->sunvis, (w->rp.flags & RTGI_RC_SUN_TRACE) ? mv : 0);
   dfree(w, w->bounce, (w->rp.flags & RTGI_RC_BOUNCE) ? mv * 4 : 0);
+  for (int i = 0; i < 2; i++) dfree(w, w->bnvol[i], (w->rp.flags & RTGI_RC_BOUNCE) ? BN_WORDS : 0);
 }
 
 }  // namespace
 
-void mark_field(rtgi_world* w) {
-  w->sh_valid = true;
-  for (int a = 0; a < 3; a++) w->sh_G0[a] = w->G0[a];
+void bounce_advance(rtgi_world* w) {
+  if (!w->bounce) return;
+  w->bn_par = 1 - w->bn_par;
+  w->bn_frame++;
+  // bnOriginNow (bounce.ts:103): floor(eye / BN_CELL) - dims / 2, eye as the f32 Frame uniform holds it.
+  const int dims[3] = {BN_DX, BN_DY, BN_DZ};
+  for (int a = 0; a < 3; a++) w->bn_org[w->bn_par][a] = (int)std::floor((float)w->frame.eye[a] / BN_CELL) - dims[a] / 2;
 }
 
 hipError_t world_wait(rtgi_world* w) {
@@ -268,12 +284,14 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   if (stage == RTGI_STAGE_VOX) {
     e = launch_vox_build(vox_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
+    bounce_advance(w);
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
     if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
+    bounce_advance(w);
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
     if (!e) e = launch_bounce(t, s);
@@ -307,6 +325,20 @@ void dir_entry(int di, int res, DirEntry* out) {
   for (int a = 0; a < 3; a++) {
     out->d[a] = n[a] / l;
     float dd = std::fabs(out->d[a]) < 1e-9f ? 1e-9f : out->d[a];
+    out->inv[a] = 1.0f / dd;
+  }
+  out->d[3] = out->inv[3] = 0.0f;
+}
+
+void bn_dir_entry(int i, int n, DirEntry* out) {
+  double z = 1.0 - (2.0 * i + 1.0) / n;
+  double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+  double f = i * 0.618033988749;
+  double phi = 2.0 * 3.14159265 * (f - std::floor(f));
+  float d[3] = {(float)(r * std::cos(phi)), (float)(r * std::sin(phi)), (float)z};
+  for (int a = 0; a < 3; a++) {
+    out->d[a] = d[a];
+    float dd = std::fabs(d[a]) < 1e-9f ? 1e-9f : d[a];
     out->inv[a] = 1.0f / dd;
   }
   out->d[3] = out->inv[3] = 0.0f;
@@ -401,13 +433,13 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
   uint64_t b = nbytes<uint32_t>(HDR_WORDS) + nbytes<uint32_t>(NLper) + nbytes<uint32_t>(mb * 16) +
                nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
                nbytes<float>(rad_floats(&w)) + nbytes<float>(sh_floats(&w)) +
-               nbytes<DirEntry>(w.dir_off[w.N]) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
+               nbytes<DirEntry>((size_t)w.dir_off[w.N] + BN_RAYS) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
   if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
   if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
   if (rp->flags & RTGI_RC_RELOCATE)
     b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
   if (rp->flags & RTGI_RC_SUN_TRACE) b += nbytes<float>(mv);
-  if (rp->flags & RTGI_RC_BOUNCE) b += nbytes<uint32_t>(mv * 4);
+  if (rp->flags & RTGI_RC_BOUNCE) b += nbytes<uint32_t>(mv * 4) + 2 * nbytes<uint32_t>(BN_WORDS);
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -456,10 +488,11 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->rad, rad_floats(w));
   if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
   if (!e) e = dalloc(w, &w->sh, sh_floats(w));
-  if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc]);
+  if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc] + BN_RAYS);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
   if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
   if (!e && (rp->flags & RTGI_RC_BOUNCE)) e = dalloc(w, &w->bounce, mv * 4);
+  for (int i = 0; i < 2 && !e && (rp->flags & RTGI_RC_BOUNCE); i++) e = dalloc(w, &w->bnvol[i], BN_WORDS);
   if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
     size_t np = (size_t)w->probe_off[Nc];
     if (!e) e = dalloc(w, &w->ppos, np * 4);
@@ -471,9 +504,10 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
     if (!e && hipMemset(w->stats, 0, 3 * RTGI_MAX_CASCADES * 8) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipMemset");
   }
   if (!e) {
-    std::vector<DirEntry> tab(w->dir_off[Nc]);
+    std::vector<DirEntry> tab((size_t)w->dir_off[Nc] + BN_RAYS);
     for (int c = 0; c < Nc; c++)
       for (int di = 0; di < w->res[c] * w->res[c]; di++) dir_entry(di, w->res[c], &tab[w->dir_off[c] + di]);
+    for (uint32_t i = 0; i < BN_RAYS; i++) bn_dir_entry((int)i, (int)BN_RAYS, &tab[w->dir_off[Nc] + i]);
     if (hipMemcpy(w->dirs, tab.data(), tab.size() * sizeof(DirEntry), hipMemcpyHostToDevice) != hipSuccess)
       e = rtgi_fail(RTGI_E_HIP, "direction table upload failed");
   }
@@ -483,6 +517,9 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
     if (hipEventCreate(&w->ev[i]) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipEventCreate failed");
   if (!e && hipMemsetAsync(w->hdr, 0, HDR_WORDS * 4, w->stream) != hipSuccess)
     e = rtgi_fail(RTGI_E_HIP, "hipMemsetAsync failed");
+  for (int i = 0; i < 2 && !e && w->bnvol[i]; i++)  // no volume before the first update: every probe inactive
+    if (hipMemsetAsync(w->bnvol[i], 0, (size_t)BN_WORDS * 4, w->stream) != hipSuccess)
+      e = rtgi_fail(RTGI_E_HIP, "hipMemsetAsync failed");
   if (!e && hipStreamSynchronize(w->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipStreamSynchronize failed");
   if (e) {
     rtgi_world_destroy(w);
@@ -607,7 +644,6 @@ extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* s
     if (w->timing) HIPCHK(hipEventRecord(w->ev[i + 1], s));
   }
   w->last_stages = stages;
-  if ((stages & RTGI_STAGE_SH) && !(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
   if (s != w->stream) {
     // Later calls on the world's own stream (readback, sync) must see this work.
     HIPCHK(hipEventRecord(w->ev[4], s));
@@ -843,6 +879,16 @@ extern "C" int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, floa
         }
         if (parents) parents[p] = c + 1 < w->N ? V[d] : 0u;
       }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_bounce_volume(rtgi_world* w, uint16_t* texels, int32_t origin[3]) {
+  if (!w || !texels || !origin) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_read_bounce_volume: null argument");
+  if (!w->bounce) return rtgi_fail(RTGI_E_STATE, "no bounce volume (RTGI_RC_BOUNCE)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  HIPCHK(hipMemcpy(texels, w->bnvol[w->bn_par], (size_t)BN_WORDS * 4, hipMemcpyDeviceToHost));
+  for (int a = 0; a < 3; a++) origin[a] = w->bn_org[w->bn_par][a];
   return RTGI_OK;
 }
 
@@ -1123,7 +1169,10 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   }
   // Descriptors: the staging buffer being filled was last copied two enqueues ago; wait for that copy.
   HIPCHK(hipEventSynchronize(b->copied[b->cur]));
-  for (int i = 0; i < n; i++) b->h_desc[b->cur][i] = trace_args(b->worlds[i]);
+  for (int i = 0; i < n; i++) {
+    bounce_advance(b->worlds[i]);
+    b->h_desc[b->cur][i] = trace_args(b->worlds[i]);
+  }
   HIPCHK(hipMemcpyAsync(b->d_desc, b->h_desc[b->cur], sizeof(TraceK) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(b->copied[b->cur], s));
   b->cur ^= 1;
@@ -1149,7 +1198,6 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   HIPCHK(hipEventRecord(b->done, s));
   for (rtgi_world* w : b->worlds) {
     w->ext_done = b->done;
-    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
   }
   return RTGI_OK;
 }

@@ -776,7 +776,10 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   out[BL_ALLOC_VOXELS] = 1;
   out[BL_RESOLVE] = persist;
   if (t.sunvis) out[BL_SUN] = persist;
-  if (t.bounce) out[BL_BOUNCE] = persist;
+  if (t.bounce) {
+    out[BL_BNVOL] = BN_PROBES / BN_PROBES_PER_BLOCK;
+    out[BL_BOUNCE] = persist;
+  }
   if (t.reloc) {
     out[BL_RELOC] = cdiv(t.total_probes, BLK);
     out[BL_LINK] = t.N > 1 ? cdiv(t.c[t.N - 1].probe_off, BLK) : 0;

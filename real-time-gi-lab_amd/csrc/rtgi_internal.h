@@ -67,6 +67,18 @@ struct VoxK {
   unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
 };
 
+// Bounce volume (DEFAULT_BOUNCE, src/core/bounce.ts:33): BN_DX x BN_DY x BN_DZ ambient-cube probes BN_CELL m apart
+// in a window around the eye; each active probe traces BN_RAYS irradiance rays and 6 axis rays, every BN_DIV-th
+// update by its world position (kept in between). A volume is [BN_PROBES][6] texels of 4 halves (rgb irradiance
+// along axis +x, -x, +y, -y, +z, -z; alpha: the free reach along that axis, 0 = inactive probe).
+constexpr int BN_DX = 32, BN_DY = 16, BN_DZ = 32;
+constexpr uint32_t BN_PROBES = (uint32_t)(BN_DX * BN_DY * BN_DZ);
+constexpr uint32_t BN_RAYS = 16;
+constexpr uint32_t BN_DIV = 2;
+constexpr float BN_CELL = 1.2f;
+constexpr uint32_t BN_WORDS = BN_PROBES * 6u * 2u;  // u32 words of one volume
+constexpr uint32_t BN_PROBES_PER_BLOCK = 8;          // 32 lanes per probe in a 256-thread block
+
 struct CascK {
   int32_t n;        // probes per axis
   int32_t res;      // directions per axis
@@ -114,8 +126,13 @@ struct TraceK {
   uint32_t* pvis;   // [all cascades' probes] bit k: parent k of cascade c+1 exists and is not hidden by voxels
   float* sunvis;    // [max_voxels] traced sun visibility per attribute slot, or null (frame sun_vis)
   uint32_t* bounce; // [max_voxels][4] bounce irradiance per slot: front rg, b, back rg, b (halves), or null
-  int32_t bounce_valid;  // the SH holds a previous update's field (else the injected bounce is 0)
-  int32_t bG0[3];        // c0 grid origin of that field
+  uint32_t* bn_cur;        // bounce volume this update writes (BN_WORDS), or null
+  const uint32_t* bn_prev; // the previous update's volume: its hits' bounce term and the kept probes
+  int32_t bn_org[3];       // probe coordinates of probe 0 of bn_cur (floor(eye / BN_CELL) - dims / 2)
+  int32_t bn_prev_org[3];  // the same of bn_prev
+  uint32_t bn_frame;       // updates of the volume so far, this one included
+  int32_t bn_level;        // levelOf(BN_CELL): the clipmap level of the near-surface test
+  uint32_t bn_dir_off;     // first of the BN_RAYS irradiance directions in dirs
   uint64_t total_probes;
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
@@ -144,7 +161,8 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
 // Traced sun visibility of every occupied voxel (thread per brick voxel).
 int launch_sun(const TraceK& k, void* stream);
-// Bounce injection of every occupied voxel from the previous update's field (thread per brick voxel).
+// Bounce volume update (8 probes per block), then bounce injection of every occupied voxel from it (thread per
+// brick voxel).
 int launch_bounce(const TraceK& k, void* stream);
 // Voxel relocation of every cascade's probes, then the parent visibility masks (thread per probe).
 int launch_relocate(const TraceK& k, void* stream);
@@ -171,7 +189,8 @@ enum BatchLaunch : int {
   BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
   BL_RESOLVE,
   BL_SUN,                                  // traced sun visibility of every occupied voxel
-  BL_BOUNCE,                               // bounce injection from the previous field
+  BL_BNVOL,                                // bounce volume update (8 probes per block)
+  BL_BOUNCE,                               // bounce injection from this update's volume
   BL_RELOC,                                // relocation of every cascade's probes
   BL_LINK,                                 // parent visibility of cascades 0 .. N-2
   BL_TRACE,
@@ -200,5 +219,8 @@ int launch_sun_batch(const BatchPlan& p, void* stream);
 int launch_bounce_batch(const BatchPlan& p, void* stream);
 int launch_sh_batch(const BatchPlan& p, void* stream);
 
+// Direction i of the n irradiance rays of a bounce probe (sfDir, src/core/bounce.ts:155-160: a spherical Fibonacci
+// set), evaluated in double and rounded to f32, with the slab-safe reciprocal.
+void bn_dir_entry(int i, int n, DirEntry* out);
 // Host-side f32 restatement of dirOf (bit-identical to the device's: same IEEE operations, no contraction).
 void dir_entry(int di, int res, DirEntry* out);

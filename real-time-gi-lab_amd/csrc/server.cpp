@@ -230,7 +230,10 @@ void sched_loop(rtgi_server* s) {
     BatchPlan plan{};
     if (!e) {
       plan_blocks(ws.data(), n, sl.h_start, plan);
-      for (int i = 0; i < n; i++) sl.h_desc[i] = trace_args(ws[i]);
+      for (int i = 0; i < n; i++) {
+        bounce_advance(ws[i]);
+        sl.h_desc[i] = trace_args(ws[i]);
+      }
       if (hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(TraceK) * n, hipMemcpyHostToDevice, sl.stream) != hipSuccess ||
           hipMemcpyAsync(sl.d_start, sl.h_start, sizeof(uint32_t) * BL_COUNT * (n + 1), hipMemcpyHostToDevice,
                          sl.stream) != hipSuccess)
@@ -243,10 +246,7 @@ void sched_loop(rtgi_server* s) {
     }
     if (!e && hipEventRecord(sl.done, sl.stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipEventRecord");
     if (!e)
-      for (rtgi_world* w : ws) {
-        w->ext_done = sl.done;
-        if (!(w->rp.flags & RTGI_RC_NO_MERGE)) mark_field(w);
-      }
+      for (rtgi_world* w : ws) w->ext_done = sl.done;
     lk.lock();
     sl.status = e;
     s->st.batches++;

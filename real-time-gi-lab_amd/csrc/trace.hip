@@ -998,41 +998,236 @@ __global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
   sun_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
-// ---- Bounce injection (bnInject, src/core/bounce.ts:224-262) ----
+// ---- Bounce volume (src/core/bounce.ts): update (bnClassify / bnUpdate) and injection into the voxels (bnInject) ----
 
-// Irradiance on both sides of every occupied voxel from the previous update's field: side n of a voxel centred at
-// p is read at p (level 0) or half a voxel towards that side (coarser levels: a coarse voxel's centre may lie metres
-// from the surface), by the gather with occlusion tests; stored as halves (pack2x16float). Without a previous field
-// (first update) the bounce is 0.
+static_assert(BLK == 32 * (int)BN_PROBES_PER_BLOCK, "bounce volume blocks are 8 probes x 32 lanes");
+static_assert(BN_RAYS + 6u <= 32u, "a probe's rays fit its 32 lanes");
+static_assert(BN_PROBES % BN_PROBES_PER_BLOCK == 0u, "whole blocks of probes");
+
+__device__ __forceinline__ uint32_t bn_probe(i3 c) { return (uint32_t)(c.x + BN_DX * (c.y + BN_DY * c.z)); }
+__device__ __forceinline__ bool bn_inside(i3 c) {
+  return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < BN_DX && c.y < BN_DY && c.z < BN_DZ;
+}
+__device__ __forceinline__ f3 bn_rgb(uint2 t) { return mk3(hlo(t.x), hhi(t.x), hlo(t.y)); }
+__device__ __forceinline__ float bn_reach(uint2 t) { return hhi(t.y); }
+
+// WGSL smoothstep: t = clamp((x - lo) / (hi - lo), 0, 1); t * t * (3 - 2 t).
+__device__ __forceinline__ float smooth_step(float lo, float hi, float x) {
+  float t = fminf(fmaxf((x - lo) / (hi - lo), 0.0f), 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// bnRead (bounce.ts:49-75): irradiance on the side n of a surface at p, read at q from the volume vol (origin org):
+// trilinear over the 8 probes around q, counting a probe when it is active, q lies within the reach it sees along
+// its axes towards q (plus tol = half a level-0 voxel), and it lies on the side n of p (smoothstep weight).
+__device__ f3 bn_read(const uint32_t* vol, const int32_t* org, float tol, f3 q, f3 p, f3 n) {
+  const uint2* V = reinterpret_cast<const uint2*>(vol);
+  i3 o{org[0], org[1], org[2]};
+  f3 g = mk3(q.x / BN_CELL, q.y / BN_CELL, q.z / BN_CELL) - tof3(o);
+  i3 bb = f2i3(floor3(g));
+  f3 fr = g - tof3(bb);
+  f3 n2 = n * n;
+  f3 e = splat3(0.0f);
+  float ws = 0.0f;
+  for (int k = 0; k < 8; k++) {
+    i3 off{k & 1, (k >> 1) & 1, (k >> 2) & 1};
+    i3 cc = bb + off;
+    if (!bn_inside(cc)) continue;
+    f3 pp = tof3(o + cc) * BN_CELL;
+    f3 u = q - pp;
+    const uint2* t = V + (size_t)bn_probe(cc) * 6u;
+    if (!(bn_reach(t[0]) > 0.0f)) continue;
+    if (fabsf(u.x) > bn_reach(t[u.x >= 0.0f ? 0 : 1]) + tol || fabsf(u.y) > bn_reach(t[u.y >= 0.0f ? 2 : 3]) + tol ||
+        fabsf(u.z) > bn_reach(t[u.z >= 0.0f ? 4 : 5]) + tol)
+      continue;
+    f3 t3 = mk3(off.x ? fr.x : 1.0f - fr.x, off.y ? fr.y : 1.0f - fr.y, off.z ? fr.z : 1.0f - fr.z);
+    float w = t3.x * t3.y * t3.z * smooth_step(-0.05f, 0.25f, dot3(pp - p, n) / BN_CELL);
+    if (w <= 0.0f) continue;
+    f3 A = bn_rgb(t[n.x >= 0.0f ? 0 : 1]), B = bn_rgb(t[n.y >= 0.0f ? 2 : 3]), C = bn_rgb(t[n.z >= 0.0f ? 4 : 5]);
+    f3 s = mk3(n2.x * A.x + n2.y * B.x + n2.z * C.x, n2.x * A.y + n2.y * B.y + n2.z * C.y,
+               n2.x * A.z + n2.y * B.z + n2.z * C.z);
+    e = e + s * w;
+    ws += w;
+  }
+  return ws > 1e-6f ? mk3(e.x / ws, e.y / ws, e.z / ws) : splat3(0.0f);
+}
+
+// A voxel hit of a bounce-volume ray (voxShade with the voxBounce of bounce.ts:96-99): the bounce term reads the
+// previous update's volume at a point a quarter voxel back from the entry, on the side the ray comes from (voxSide,
+// src/core/rc.ts:122-126); a voxel without a normal is read facing the ray. Otherwise the cascades' shading.
+__device__ f3 shade_voxel_vol(const TraceK& a, uint32_t ai, f3 d, f3 entry, uint32_t brick, uint32_t bit) {
+  uint32_t pa = 0u, pn = 0u;
+  if (ai != ATTR_NONE) {
+    pa = a.vx.attr[ai * 2u];
+    pn = a.vx.attr[ai * 2u + 1u];
+  }
+  f3 alb = mk3(unorm8_to_f(pa & 255u), unorm8_to_f((pa >> 8) & 255u), unorm8_to_f((pa >> 16) & 255u));
+  float nx = snorm16_to_f(pn & 0xffffu), ny = snorm16_to_f(pn >> 16);
+  f3 n0 = mk3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
+  float q = fmaxf(-n0.z, 0.0f);
+  n0.x += n0.x >= 0.0f ? -q : q;
+  n0.y += n0.y >= 0.0f ? -q : q;
+  f3 n = dot3(n0, n0) > 0.01f ? n0 : -d;
+  if (dot3(n, d) > 0.0f) n = -n;
+  f3 e = splat3(0.0f);
+  if (ai != ATTR_NONE) {
+    uint32_t code = a.vx.bcode[brick];
+    int L = (int)(code >> 27);
+    i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c = bc * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    float v = vox_size(a.vx, L);
+    f3 pc = tof3(origin(a.vx, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+    f3 p = entry - d * (0.25f * v);
+    f3 nb;
+    if (dot3(n0, n0) <= 0.01f) {
+      nb = -d;
+    } else {
+      float sd = dot3(p - pc, n0);
+      float side = fabsf(sd) > 0.05f * v ? (sd > 0.0f ? 1.0f : -1.0f) : (dot3(n0, d) > 0.0f ? -1.0f : 1.0f);
+      nb = n0 * side;
+    }
+    e = bn_read(a.bn_prev, a.bn_prev_org, 0.5f * a.vx.v0, p, p, nb);
+  }
+  f3 b = alb / PI_F * e;
+  f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
+  float ndl = fmaxf(dot3(n, sd), 0.0f);
+  f3 out = b;
+  float vis = (a.sunvis && ai != ATTR_NONE) ? a.sunvis[ai] : a.sun_vis;
+  if (ndl > 0.0f) out = b + alb / PI_F * mk3(a.sun_e[0], a.sun_e[1], a.sun_e[2]) * ndl * vis;
+  for (int k = 0; k < a.n_point; k++) {
+    f3 l = mk3(a.point_pos[k][0], a.point_pos[k][1], a.point_pos[k][2]) - entry;
+    float r2 = dot3(l, l);
+    float nl = fmaxf(dot3(n, l) / sqrtf(r2), 0.0f);
+    if (nl > 0.0f) out = out + alb / PI_F * mk3(a.point_i[k][0], a.point_i[k][1], a.point_i[k][2]) * (nl / r2);
+  }
+  return out;
+}
+
+// bnTrace (bounce.ts:164-175): segments [b_k, b_k+1] * BN_CELL (b_k+1 = 4 b_k + 1) on voxel level k (or the first
+// level holding the segment's end), nothing skipped, up to tMax. Returns rgb and the entry distance of the hit, or
+// the sky and 1e9. shade = false leaves rgb 0 (the axis rays use only the distance).
+__device__ float4 bn_trace(const TraceK& a, f3 o, f3 d, f3 inv, float tMax, bool shade) {
+  const VoxK& k = a.vx;
+  float b0 = 0.0f;
+  for (int kk = 0; kk < k.NL && b0 * BN_CELL < tMax; kk++) {
+    float b1 = b0 * 4.0f + 1.0f;
+    int L = vox_level_for(k, kk, o + d * b1 * BN_CELL);
+    Hit h = trace_vox<false>(a, o, d, inv, b0 * BN_CELL, fminf(b1 * BN_CELL, tMax), 0.0f, L);
+    if (h.hit) {
+      f3 entry = o + d * h.tIn;
+      f3 rgb = shade ? shade_voxel_vol(a, vox_rank(k, h.brick, h.bit), d, entry, h.brick, h.bit) : splat3(0.0f);
+      return make_float4(rgb.x, rgb.y, rgb.z, dot3(entry - o, d));
+    }
+    b0 = b1;
+  }
+  return make_float4(a.sky[0], a.sky[1], a.sky[2], 1e9f);
+}
+
+// bnClassify + bnUpdate (bounce.ts:117-144, 182-208), 8 probes per 256-thread block, 32 lanes per probe. A probe is
+// active when an occupied voxel of level levelOf(BN_CELL) lies at one of the 8 points half a cell around it and
+// none of level 0 at the probe itself; an inactive probe is cleared. An active probe whose turn it is not (every
+// BN_DIV-th update by world position) keeps the previous volume's texels when that volume holds it active;
+// otherwise lanes [0, BN_RAYS) trace the irradiance rays, lanes [BN_RAYS, BN_RAYS + 6) the axis rays (free reach,
+// at most 1.5 cells), and lanes 0..5 form the ambient cube: axis a holds pi * sum(w r) / sum(w), w = max(a . dir, 0),
+// summed in ray order. Texels are rounded to halves (RNE).
+__device__ __forceinline__ void bnvol_body(const TraceK& a, uint32_t lb) {
+  __shared__ float s_rad[BN_PROBES_PER_BLOCK][BN_RAYS][3];
+  __shared__ float s_reach[BN_PROBES_PER_BLOCK][6];
+  const VoxK& k = a.vx;
+  const uint32_t g = threadIdx.x >> 5, li = threadIdx.x & 31u;
+  const uint32_t i = lb * BN_PROBES_PER_BLOCK + g;
+  i3 c{(int)(i % (uint32_t)BN_DX), (int)((i / (uint32_t)BN_DX) % (uint32_t)BN_DY), (int)(i / (uint32_t)(BN_DX * BN_DY))};
+  i3 wp{a.bn_org[0] + c.x, a.bn_org[1] + c.y, a.bn_org[2] + c.z};
+  f3 p = tof3(wp) * BN_CELL;
+  bool occ = false;
+  VoxHit vh;
+  if (li < 8u) {
+    f3 corner = mk3((float)(li & 1u), (float)((li >> 1) & 1u), (float)(li >> 2)) - splat3(0.5f);
+    occ = vox_cell(k, a.bn_level, p + corner * BN_CELL, vh);
+  } else if (li == 8u) {
+    occ = vox_cell(k, 0, p, vh);
+  }
+  uint32_t gm = (uint32_t)(__ballot(occ) >> (threadIdx.x & 32u));
+  int mode = ((gm & 0xffu) != 0u && ((gm >> 8) & 1u) == 0u) ? 2 : 0;  // 0 clear, 1 keep, 2 trace
+  i3 pc{0, 0, 0};
+  const uint2* prev = reinterpret_cast<const uint2*>(a.bn_prev);
+  if (mode == 2 && BN_DIV > 1u && ((uint32_t)(wp.x + wp.y + wp.z + 1048576) + a.bn_frame) % BN_DIV != 0u) {
+    pc = wp - i3{a.bn_prev_org[0], a.bn_prev_org[1], a.bn_prev_org[2]};
+    if (bn_inside(pc) && bn_reach(prev[(size_t)bn_probe(pc) * 6u]) > 0.0f) mode = 1;
+  }
+  if (mode == 2) {
+    if (li < BN_RAYS) {
+      const DirEntry& de = a.dirs[a.bn_dir_off + li];
+      float4 r = bn_trace(a, p, mk3(de.d[0], de.d[1], de.d[2]), mk3(de.inv[0], de.inv[1], de.inv[2]), 1e9f, true);
+      s_rad[g][li][0] = r.x;
+      s_rad[g][li][1] = r.y;
+      s_rad[g][li][2] = r.z;
+    } else if (li < BN_RAYS + 6u) {
+      uint32_t ax = li - BN_RAYS;
+      float sg = (ax & 1u) ? -1.0f : 1.0f;
+      f3 d = mk3(ax / 2u == 0u ? sg : 0.0f, ax / 2u == 1u ? sg : 0.0f, ax / 2u == 2u ? sg : 0.0f);
+      f3 inv = mk3(1.0f / (d.x == 0.0f ? 1e-9f : d.x), 1.0f / (d.y == 0.0f ? 1e-9f : d.y),
+                   1.0f / (d.z == 0.0f ? 1e-9f : d.z));
+      float4 r = bn_trace(a, p, d, inv, 1.5f * BN_CELL, false);
+      s_reach[g][ax] = fmaxf(fminf(r.w, 1.5f * BN_CELL), 1e-3f);
+    }
+  }
+  __syncthreads();
+  if (li < 6u) {
+    uint2* out = reinterpret_cast<uint2*>(a.bn_cur) + (size_t)i * 6u + li;
+    if (mode == 0) {
+      *out = make_uint2(0u, 0u);
+    } else if (mode == 1) {
+      *out = prev[(size_t)bn_probe(pc) * 6u + li];
+    } else {
+      float sg = (li & 1u) ? -1.0f : 1.0f;
+      f3 e = splat3(0.0f);
+      float ws = 0.0f;
+      for (uint32_t r = 0; r < BN_RAYS; r++) {
+        float w = fmaxf(sg * a.dirs[a.bn_dir_off + r].d[li >> 1], 0.0f);
+        e = e + mk3(s_rad[g][r][0], s_rad[g][r][1], s_rad[g][r][2]) * w;
+        ws += w;
+      }
+      float m = fmaxf(ws, 1e-6f);
+      *out = make_uint2(h2(PI_F * e.x / m, PI_F * e.y / m), h2(PI_F * e.z / m, s_reach[g][li]));
+    }
+  }
+}
+__global__ void __launch_bounds__(BLK) k_bnvol(TraceK a) { bnvol_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(BLK) k_bnvol_b(BatchK b) {
+  int w = batch_world(b, blockIdx.x);
+  bnvol_body(b.w[w], blockIdx.x - b.start[w]);
+}
+
+// bnInject (bounce.ts:224-262): every occupied voxel stores this update's volume irradiance on its two sides (along
+// and against its normal; no normal: +y): level 0 read at the voxel centre, coarser levels half a voxel out towards
+// the side. Four words per attribute slot: front rg, b, back rg, b (halves).
 __device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const VoxK& k = a.vx;
   uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  const float tol = 0.5f * k.v0;
   for (uint32_t i = lb * BLK + threadIdx.x; i < nb * 512u; i += nblk * BLK) {
     uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
     if ((k.occ[ptr * 16u + w] & (1u << (bit & 31u))) == 0u) continue;
     uint32_t ai = vox_rank(k, ptr, bit);
     if (ai == ATTR_NONE) continue;
-    uint4 out = make_uint4(0u, 0u, 0u, 0u);
-    if (a.bounce_valid) {
-      uint32_t code = k.bcode[ptr];
-      int L = (int)(code >> 27);
-      i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
-      i3 c = bc * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
-      float v = vox_size(k, L);
-      f3 p = tof3(origin(k, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
-      uint32_t pn = k.attr[ai * 2u + 1u];
-      float ex = snorm16_to_f(pn & 0xffffu), ey = snorm16_to_f(pn >> 16);
-      f3 n = mk3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
-      float q0 = fmaxf(-n.z, 0.0f);
-      n.x += n.x >= 0.0f ? -q0 : q0;
-      n.y += n.y >= 0.0f ? -q0 : q0;
-      f3 nf = dot3(n, n) > 0.01f ? normalize3(n) : mk3(0.0f, 1.0f, 0.0f);
-      float off = L == 0 ? 0.0f : 0.5f * v;
-      f3 ef = gather_point(a, a.bG0, p + nf * off, nf, true);
-      f3 eb = gather_point(a, a.bG0, p + (-nf) * off, -nf, true);
-      out = make_uint4(h2(ef.x, ef.y), h2(ef.z, 0.0f), h2(eb.x, eb.y), h2(eb.z, 0.0f));
-    }
-    reinterpret_cast<uint4*>(a.bounce)[ai] = out;
+    uint32_t code = k.bcode[ptr];
+    int L = (int)(code >> 27);
+    i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+    i3 c = bc * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    float v = vox_size(k, L);
+    f3 p = tof3(origin(k, L)) * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+    uint32_t pn = k.attr[ai * 2u + 1u];
+    float ex = snorm16_to_f(pn & 0xffffu), ey = snorm16_to_f(pn >> 16);
+    f3 n = mk3(ex, ey, 1.0f - fabsf(ex) - fabsf(ey));
+    float q0 = fmaxf(-n.z, 0.0f);
+    n.x += n.x >= 0.0f ? -q0 : q0;
+    n.y += n.y >= 0.0f ? -q0 : q0;
+    f3 nf = dot3(n, n) > 0.01f ? normalize3(n) : mk3(0.0f, 1.0f, 0.0f);
+    float off = L == 0 ? 0.0f : 0.5f * v;
+    f3 ef = bn_read(a.bn_cur, a.bn_org, tol, p + nf * off, p, nf);
+    f3 eb = bn_read(a.bn_cur, a.bn_org, tol, p + (-nf) * off, p, -nf);
+    reinterpret_cast<uint4*>(a.bounce)[ai] = make_uint4(h2(ef.x, ef.y), h2(ef.z, 0.0f), h2(eb.x, eb.y), h2(eb.z, 0.0f));
   }
 }
 __global__ void __launch_bounds__(BLK) k_bounce(TraceK a) { bounce_body(a, blockIdx.x, gridDim.x); }
@@ -1141,6 +1336,7 @@ int launch_sun(const TraceK& k, void* stream) {
 
 int launch_bounce(const TraceK& k, void* stream) {
   if (!k.bounce) return 0;
+  k_bnvol<<<BN_PROBES / BN_PROBES_PER_BLOCK, BLK, 0, (hipStream_t)stream>>>(k);
   uint32_t pool = (uint32_t)k.vx.max_bricks * 512u;
   uint32_t g = std::min<uint32_t>(2048, std::max<uint32_t>(1, (pool + BLK * 4 - 1) / (BLK * 4)));
   k_bounce<<<g, BLK, 0, (hipStream_t)stream>>>(k);
@@ -1176,6 +1372,7 @@ int launch_sun_batch(const BatchPlan& p, void* stream) {
 }
 
 int launch_bounce_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_BNVOL]) k_bnvol_b<<<p.grid[BL_BNVOL], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_BNVOL));
   if (p.grid[BL_BOUNCE]) k_bounce_b<<<p.grid[BL_BOUNCE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_BOUNCE));
   return (int)hipGetLastError();
 }

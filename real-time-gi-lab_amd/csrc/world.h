@@ -51,8 +51,11 @@ struct rtgi_world {
   uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
   float* sunvis = nullptr;                // traced sun visibility per attribute slot (RTGI_RC_SUN_TRACE)
   uint32_t* bounce = nullptr;             // bounce irradiance per attribute slot, 4 words (RTGI_RC_BOUNCE)
-  bool sh_valid = false;                  // sh holds the field of an enqueued update (bounce source)
-  int sh_G0[3] = {};                      // that field's c0 grid origin
+  uint32_t* bnvol[2] = {};                // the two bounce volumes (RTGI_RC_BOUNCE), BN_WORDS each
+  int bn_par = 1;                         // the volume the last update wrote
+  int bn_org[2][3] = {};                  // origins of the two volumes
+  uint32_t bn_frame = 0;                  // volume updates so far
+  uint32_t bn_dir_off = 0;                // first bounce ray direction in dirs
   uint64_t bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
@@ -69,9 +72,9 @@ uint32_t probe_index_host(const rtgi_world* w, int c, uint32_t x, uint32_t y, ui
 // Launch arguments of a world (its current frame, buffers and layout).
 VoxK vox_args(const rtgi_world* w);
 TraceK trace_args(const rtgi_world* w);
-// Records that the world's sh now holds the field of an enqueued update at its current grid (the bounce source of
-// the next update).
-void mark_field(rtgi_world* w);
+// Advances the bounce volume by one update (the next volume written, its origin from the frame's eye, the update
+// counter); called once for each enqueued update that includes the trace stage, before its launch arguments.
+void bounce_advance(rtgi_world* w);
 // Waits for the world's own stream and for the last batched update that included it.
 hipError_t world_wait(rtgi_world* w);
 // Device bytes a world of these parameters holds once a scene of this size is uploaded (rtgi_world_estimate_bytes).

@@ -334,6 +334,15 @@ class World:
         check(lib().rtgi_world_read_probes(self._h, c, ptr(pos), ptr(nrm), ptr(par)))
         return pos, nrm, par
 
+    def read_bounce_volume(self):
+        """The bounce volume of the last update: texels (z, y, x, axis, rgba) as float16 (axes +x -x +y -y +z -z;
+        alpha: reach, 0 = inactive probe) and the probe coordinates of probe 0."""
+        dx, dy, dz = _L.BN_DIMS
+        t = np.zeros((dz, dy, dx, 6, 4), np.uint16)
+        org = np.zeros(3, np.int32)
+        check(lib().rtgi_world_read_bounce_volume(self._h, ptr(t), ptr(org)))
+        return t.view(np.float16), org
+
     def gather(self, pos, nrm, occlusion=False):
         """gatherWorld (src/core/rc.ts:871-927): irradiance (n x 3) of the last update's field at surface points."""
         p = np.ascontiguousarray(pos, np.float32)

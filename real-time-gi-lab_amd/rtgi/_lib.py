@@ -26,6 +26,7 @@ RC_STATS = 8
 RC_RELOCATE = 16
 RC_SUN_TRACE = 32
 RC_BOUNCE = 64
+BN_DIMS = (32, 16, 32)  # bounce volume probes per axis (x, y, z); 1.2 m apart
 GATHER_OCCLUSION = 1
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15
 SCENE_ATRIUM, SCENE_CITY = 1, 2
@@ -133,6 +134,7 @@ SIGNATURES = {
     "rtgi_world_trace_rays": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, _P, _P]),
     "rtgi_world_gather": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "rtgi_world_read_probes": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "rtgi_world_read_bounce_volume": (C.c_int, [_P, _P, _P]),
     "rtgi_world_set_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_get_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_destroy": (C.c_int, [_P]),

@@ -1,8 +1,9 @@
-"""Multi-bounce (bnInject, src/core/bounce.ts:224-262, read back by voxBounce, src/core/rc.ts:198-202) on the GPU
-against the oracle: the second of two updates with a moved eye injects the first update's field (its c0 grid and,
-with relocation, its probe positions) into the second clipmap's voxels and shades hits with it. Cascades, first-hit
-ids and SH under the plain bars (ids exact, radiance within 1e-4; FP16 exact); and the batched, graph-replayed path
-gives the single-world path's results bit for bit."""
+"""Multi-bounce (createBounce with DEFAULT_BOUNCE, src/core/bounce.ts) on the GPU against the oracle, over three
+updates with a moving eye (the last one at the second's eye, so that every second probe keeps its texels): the bounce
+volume of each update (bnClassify + bnUpdate: hits read the previous volume) bit for bit with its origin, then the
+cascades, first-hit ids and SH of the update, whose hits add the injected volume irradiance (bnInject, voxBounce),
+under the plain bars (ids exact, radiance within 1e-4; FP16 exact). The batched, graph-replayed path gives the
+single-world path's volumes and SH bit for bit."""
 import dataclasses
 
 import numpy as np
@@ -11,6 +12,7 @@ import pytest
 import rtgi
 from conftest import random_scene
 from oracle.oracle import Oracle
+from test_oracle import floor_world
 
 pytestmark = pytest.mark.gpu
 RTOL, ATOL = 1e-4, 1e-6
@@ -32,7 +34,7 @@ def moved(frame, d):
     return dataclasses.replace(frame, eye=tuple(float(a + b) for a, b in zip(frame.eye, d)))
 
 
-def check_frame(w, o, rc, fp16):
+def check_update(w, o, rc, fp16):
     rgba, hits = o.trace()
     m, sh = o.merge_sh(rgba)
     per_m, per_h = o.split(m), o.split(hits)
@@ -49,40 +51,47 @@ def check_frame(w, o, rc, fp16):
         np.testing.assert_allclose(w.read_sh(), sh, rtol=RTOL, atol=ATOL)
 
 
-@pytest.mark.parametrize("case", ["tiny", "atrium", "city"])
+def check_volume(w, state):
+    vol, org = w.read_bounce_volume()
+    assert np.array_equal(org, state[1])
+    assert np.array_equal(vol.view(np.uint16).reshape(-1, 6, 4), state[0])
+    return int(np.count_nonzero(vol[..., 0, 3] > 0))
+
+
+@pytest.mark.parametrize("case", ["tiny", "floor", "atrium", "city"])
 def test_bounce_matches_oracle(require_gpu, case):
-    scene, vox, rc, f1 = tiny() if case == "tiny" else rtgi.config(case)
-    f2 = moved(f1, (3.0 * rc.s0, 0.0, -2.0 * rc.s0))
-    for fp16, reloc in ((False, False), (True, True)):
-        rcf = dataclasses.replace(rc, bounce=True, relocate=reloc, fp16=fp16, recordHits=True)
-        say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, update 1")
+    if case == "tiny":
+        scene, vox, rc, f1 = tiny()
+    elif case == "floor":
+        scene, vox, rc, f1 = floor_world()
+    else:
+        scene, vox, rc, f1 = rtgi.config(case)
+    f2 = moved(f1, (2.5, 0.0, -1.7))
+    frames = [f1, f2, f2]
+    for fp16, reloc, sun in ((False, False, False), (True, True, True)):
+        rcf = dataclasses.replace(rc, bounce=True, relocate=reloc, sunTrace=sun, fp16=fp16, recordHits=True)
         w = rtgi.World(vox, rcf)
         w.upload_scene(scene)
-        w.set_frame(f1)
-        w.encode()
-        w.sync()
-        sh1 = w.read_sh()
-        o1 = Oracle(vox, rcf, f1)
-        o1.build(scene)
-        if case == "tiny":
-            check_frame(w, o1, rc, fp16)  # no previous field: no bounce
-        say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, update 2")
-        w.set_frame(f2)
-        w.encode()
-        w.sync()
-        o2 = Oracle(vox, rcf, f2)
-        o2.build(scene)
-        o2.inject_bounce(o1, sh1)
-        assert np.count_nonzero(o2.bounce) > 0
-        check_frame(w, o2, rc, fp16)
+        state = None
+        for k, f in enumerate(frames):
+            say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, sunTrace={sun}, update {k + 1}")
+            w.set_frame(f)
+            w.encode()
+            w.sync()
+            o = Oracle(vox, rcf, f)
+            o.build(scene)
+            state = o.bounce_update(state)
+            active = check_volume(w, state)
+            assert active > 0
+            check_update(w, o, rc, fp16)
         w.destroy()
 
 
 def test_bounce_batch_matches_single(require_gpu):
     scene, vox, rc, f1 = rtgi.config("atrium")
     rcf = dataclasses.replace(rc, bounce=True, relocate=True, fp16=True)
-    frames = [f1, moved(f1, (2.0, 0.0, 1.0)), moved(f1, (2.5, 0.0, -1.0))]
-    say("case: multi-bounce, atrium, single world vs batch of 2 (graph), 3 updates")
+    frames = [f1, moved(f1, (2.0, 0.0, 1.0)), moved(f1, (2.0, 0.0, 1.0)), moved(f1, (-1.5, 0.0, 0.5))]
+    say("case: multi-bounce, atrium, single world vs batch of 2 (graph), 4 updates")
     single = rtgi.World(vox, rcf)
     pair = [rtgi.World(vox, rcf) for _ in range(2)]
     for w in [single] + pair:
@@ -97,8 +106,11 @@ def test_bounce_batch_matches_single(require_gpu):
             b.encode()
             b.sync()
             ref = single.read_sh()
+            rv, ro = single.read_bounce_volume()
             for w in pair:
                 assert np.array_equal(w.read_sh(), ref)
+                v, og = w.read_bounce_volume()
+                assert np.array_equal(og, ro) and np.array_equal(v.view(np.uint16), rv.view(np.uint16))
     finally:
         b.destroy()
         for w in [single] + pair:

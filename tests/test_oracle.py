@@ -520,31 +520,35 @@ def test_traced_sun_visibility_closed_form():
     assert set(np.unique(vis[vis != 0]).tolist()) <= {0.25, 0.5, 0.75, 1.0}
 
 
-def test_bounce_closed_form():
-    """Multi-bounce (bnInject + voxBounce): the field below puts L0 = 1 on the probes at x <= 0 and 100 on those
-    beyond the wall at x = 0.35. With occlusion each wall voxel's front (+x) side gathers only far-side probes and its
-    back side only near-side ones, so the injected irradiance is f16(0.282095 * 100) in front and f16(0.282095) behind
-    (an L0 field evaluates to c * 0.282095 for every normal; SH words are coefficient-major, rgb innermost). A ray from +x hits the front face, which the sun
-    (direction x < 0) does not light, so it returns exactly albedo / pi times the front value; a ray from -x returns
-    the plain shading plus albedo / pi times the back value."""
+def _volume(org, rgb_of_x, reach=1.8):
+    """A bounce volume (probes x fastest, 6 axes, 4 halves) whose every probe is active with the given reach on all
+    axes and, on all axes, the rgb rgb_of_x(world x of the probe)."""
+    dx, dy, dz = orc.BN_DIMS
+    px = (org[0] + np.arange(dx)) * np.float32(1.2)
+    vol = np.zeros((dz, dy, dx, 6, 4), np.float16)
+    vol[..., :3] = np.asarray([rgb_of_x(x) for x in px], np.float16)[None, None, :, None, :]
+    vol[..., 3] = np.float16(reach)
+    return vol.reshape(dx * dy * dz, 6, 4).view(np.uint16)
+
+
+def test_bounce_inject_closed_form():
+    """bnInject + voxBounce on a hand-made volume: probes at x <= 0 hold 1 on every axis, those beyond the wall at
+    x = 0.35 hold 100, all active with reach 1.8 (> the 1.2 m spacing, so the reach test keeps every probe). bnRead
+    weights a probe by smoothstep(-0.05, 0.25, (probe - p) . n / 1.2): the wall voxels' front (+x) side reads only
+    the far probes and the back side only the near ones, and an axis-aligned normal picks that axis alone, so every
+    voxel stores 100 in front and 1 behind (f16). A cascade ray from +x hits the front face, which the sun (x < 0) does
+    not light: exactly albedo / pi * 100; a ray from -x returns the plain shading plus albedo / pi * 1."""
     scene, vox, rc, frame = wall_world()
     plain = Oracle(vox, rc, frame)
     plain.build(scene)
     o = Oracle(vox, dataclasses.replace(rc, bounce=True), frame)
     o.build(scene)
-    N = rc.probes0
-    gx = np.arange(N) + np.array(o.rc.G0[:3])[0]
-    sh = np.zeros((N ** 3, 27), np.float32)
-    sh[:, 0:3] = np.tile(np.where(gx * rc.s0 <= 0.0, 1.0, 100.0).astype(np.float32), N * N)[:, None]
     rng = np.random.default_rng(6)
     m = 200
     yz = rng.uniform(-1.5, 1.5, (m, 2))
     dx = np.tile(np.array([[1.0, 0, 0]], np.float32), (m, 1))
     alb = np.float32(128 / 255) / np.float32(3.14159265)
-    e_front = np.float32(np.float16(np.float32(0.282095) * np.float32(100.0)))
-    e_back = np.float32(np.float16(np.float32(0.282095)))
-    # Cascade 0 rays hit the wall at level 0 (gathers at the voxel centre), cascade 1 rays at level 1 (half a voxel
-    # out along the normal).
+    # Cascade 0 rays hit the wall at level 0 (read at the voxel centre), cascade 1 rays at level 1 (half a voxel out).
     cases = ((0, 0.6, -0.1), (1, 0.9, -0.4))
     for k, (c, xf, xb) in enumerate(cases):
         front = np.concatenate([np.full((m, 1), xf), yz], 1).astype(np.float32)
@@ -554,19 +558,81 @@ def test_bounce_closed_form():
         if k == 0:
             # Before injection the bounce term is zero: identical to the plain shading.
             assert np.array_equal(o.trace_rays(c, back, dx)[0], pb)
-            o.inject_bounce(o, sh)
+            org = orc.bn_origin(frame.eye)
+            o.inject_volume(_volume(org, lambda x: (1.0,) * 3 if x <= 0 else (100.0,) * 3), org)
         rf, hf = o.trace_rays(c, front, -dx)
         assert np.all(hf != 0xFFFFFFFF) and np.all(hf >> 29 == c)
-        np.testing.assert_array_equal(rf[:, :3], np.full((m, 3), alb * e_front, np.float32))
+        np.testing.assert_array_equal(rf[:, :3], np.full((m, 3), alb * np.float32(100.0), np.float32))
         rb, hb = o.trace_rays(c, back, dx)
         assert np.array_equal(hb, ph)
-        np.testing.assert_allclose(rb[:, :3], pb[:, :3] + alb * e_back, rtol=1e-6)
-    # Every wall voxel inside the probe grid got both sides (one outside it has no corner: zero on both).
+        np.testing.assert_allclose(rb[:, :3], pb[:, :3] + alb, rtol=1e-6)
     h = np.frombuffer(o.bounce[:o.counters()["voxels"]].tobytes(), np.float16).reshape(-1, 8)
-    inside = h[:, 0] != 0
-    assert inside.sum() > 1000
-    assert np.all(h[inside][:, [0, 1, 2]] == np.float16(e_front)) and np.all(h[inside][:, [4, 5, 6]] == np.float16(e_back))
-    assert np.all(h[~inside] == 0) and np.all(h[:, [3, 7]] == 0)
+    assert np.all(h[:, [0, 1, 2]] == 100) and np.all(h[:, [4, 5, 6]] == 1) and np.all(h[:, [3, 7]] == 0)
+
+
+def floor_world(y=0.1, extent=60.0):
+    pos = np.array([[-extent, y, -extent], [extent, y, -extent], [extent, y, extent], [-extent, y, extent]], np.float32)
+    nrm = np.tile(np.array([[0.0, 1, 0]], np.float32), (4, 1))
+    tri = np.array([[0, 2, 1, 0], [0, 3, 2, 0]], np.uint32)
+    scene = rtgi.Scene(pos=pos, nrm=nrm, tri=tri, materials=np.array([[0.6, 0.5, 0.4, 1]], np.float32))
+    vox = rtgi.VoxParams(v0=0.25, levels=4, G=8, maxBricks=4 * 512, maxVoxels=1 << 18)
+    rc = rtgi.RcParams(cascades=2, probes0=8, s0=0.5, bounce=True)
+    return scene, vox, rc, rtgi.Frame(eye=(0.3, 1.0, -0.2), sunDir=(0.2, 1.0, 0.1))
+
+
+def test_bounce_volume_floor_closed_form():
+    """bnClassify + bnUpdate over a floor at y = 0.1 (60 m around): the probes 1.2 m up are the only active ones (a
+    level-2 voxel of the floor lies half a cell below them; those at y = 0 sit in a floor voxel, the others have no
+    surface near). Every downward ray hits the floor, whose hits all shade alike (same albedo and normal at every
+    level, sun visibility 1, no bounce on the first update), and every upward ray sees the sky, so the ambient cube
+    holds pi * floor radiance along -y and pi * sky along +y; the -y reach is the distance down to the first floor
+    voxel (0.95 m to the level-0 voxel top at 0.25 near the eye), the others the 1.8 m cap. The second update keeps
+    the probes whose turn it is not (every 2nd update by x + y + z) from the previous volume, texel for texel, and
+    traces the rest, whose floor hits now add albedo / pi times the first volume's irradiance above the floor."""
+    scene, vox, rc, frame = floor_world()
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    vol, org, n = o.bounce_update(None)
+    assert n == 1 and list(org) == [-16, -8, -17]
+    dx, dy, dz = orc.BN_DIMS
+    V = vol.view(np.float16).reshape(dz, dy, dx, 6, 4).astype(np.float32)
+    iy = 1 - org[1]  # the probe layer at y = 1.2
+    assert np.all(np.delete(V, [iy - 1, iy], axis=1) == 0)
+    # The layer at y = 0 lies in the floor's level-0 voxels inside the level-0 window (inactive); outside it the
+    # level-0 test reads nothing (occAt is false out of the window), so those probes count as free and active.
+    lo0 = np.array([o.vp.origin[0][a] for a in range(3)]) * 8 * vox.v0
+    xw0 = (org[0] + np.arange(dx)) * np.float32(1.2)
+    zw0 = (org[2] + np.arange(dz)) * np.float32(1.2)
+    in0 = ((zw0[:, None] >= lo0[2]) & (zw0[:, None] < lo0[2] + 16) & (xw0[None, :] >= lo0[0]) & (xw0[None, :] < lo0[0] + 16))
+    assert np.array_equal(V[:, iy - 1, :, 0, 3] > 0, ~in0) and in0.sum() > 100
+    A = V[:, iy]                        # (z, x, axis, rgba)
+    assert np.all(A[..., 0, 3] > 0)     # every probe of the layer is active
+    alb = np.array([np.float32(np.round(255 * c) / 255) for c in (0.6, 0.5, 0.4)], np.float32)
+    sd = np.array(frame.sunDir, np.float64)
+    sd = (sd / np.linalg.norm(sd)).astype(np.float32)
+    floor_l = alb / np.float32(3.14159265) * np.asarray(frame.sunE, np.float32) * sd[1]
+    np.testing.assert_allclose(A[..., 3, :3], np.broadcast_to(np.float32(3.14159265) * floor_l, A[..., 3, :3].shape),
+                               rtol=1e-3)
+    np.testing.assert_allclose(A[..., 2, :3], np.broadcast_to(np.float32(3.14159265) * np.asarray(frame.sky, np.float32),
+                                                               A[..., 2, :3].shape), rtol=1e-3)
+    cap = np.float32(np.float16(np.float32(1.5) * np.float32(1.2)))
+    assert np.all(A[..., [0, 1, 2, 4, 5], 3] == cap)
+    xw = (org[0] + np.arange(dx)) * 1.2
+    zw = (org[2] + np.arange(dz)) * 1.2
+    near = (np.abs(zw[:, None] - 0.3) < 5) & (np.abs(xw[None, :] - 0.3) < 5)
+    np.testing.assert_allclose(A[near][:, 3, 3], 0.95, rtol=1e-3)
+    # Second update, same eye: kept probes copy the previous volume; traced ones gain the bounce.
+    vol2, org2, n2 = o.bounce_update((vol, org, n))
+    assert n2 == 2 and np.array_equal(org2, org)
+    V2 = vol2.view(np.float16).reshape(dz, dy, dx, 6, 4).astype(np.float32)
+    A2 = V2[:, iy]
+    wsum = (org[0] + np.arange(dx))[None, :] + (org[1] + iy) + (org[2] + np.arange(dz))[:, None]
+    kept = (wsum + 1048576 + 2) % 2 != 0
+    assert kept.any() and (~kept).any()
+    assert np.array_equal(A2[kept], A[kept])
+    assert np.all(A2[~kept][:, 3, :3] > A[~kept][:, 3, :3])   # floor hits add the bounce
+    assert np.array_equal(A2[~kept][:, :, 3], A[~kept][:, :, 3])  # reach does not depend on light
+    assert np.array_equal(A2[~kept][:, 2, :3], A[~kept][:, 2, :3])  # the sky side is unchanged
 
 
 def test_trace_empty_world_and_single_voxel():
