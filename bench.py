@@ -224,6 +224,64 @@ def serve_bench(a):
     }
 
 
+def full_bench(a, scenes, vox, rc, frame0, tuning, bsz):
+    """The same worlds with the reference's default extras on: voxel relocation with occlusion-aware parents, traced
+    per-voxel sun visibility and the multi-bounce volume. Batched hipGraph updates with the headline's tuning and
+    width, and the plain path one update at a time."""
+    import dataclasses
+
+    import rtgi
+
+    rcf = dataclasses.replace(rc, relocate=True, sunTrace=True, bounce=True)
+    worlds = []
+    for sc in scenes[:a.full_worlds]:
+        w = rtgi.World(vox, rcf)
+        w.upload_scene(sc)
+        w.set_frame(dataclasses.replace(frame0, eye=sc.eye, sunDir=sc.sunDir))
+        w.set_tuning(**tuning)
+        worlds.append(w)
+    for w in worlds:
+        w.encode()
+        w.sync()
+    t = time.perf_counter()
+    for w in worlds:
+        w.encode()
+        w.sync()
+    plain = len(worlds) / (time.perf_counter() - t)
+    groups = [worlds[i:i + bsz] for i in range(0, len(worlds), bsz)]
+    batches = [rtgi.Batch(g, graph=True) for g in groups]
+    for _ in range(2):
+        for b in batches:
+            b.encode()
+        for b in batches:
+            b.sync()
+    steps = max(1, a.steps // 2)
+    t = time.perf_counter()
+    for _ in range(steps):
+        for b in batches:
+            b.encode()
+        for b in batches:
+            b.sync()
+    value = steps * len(worlds) / (time.perf_counter() - t)
+    tb = rtgi.Batch(groups[0])
+    tb.set_timing(True)
+    tb.encode()
+    tb.sync()
+    stage = tb.stage_ms()
+    tb.destroy()
+    active = int(np.count_nonzero(worlds[0].read_bounce_volume()[0][..., 0, 3] > 0))
+    for b in batches:
+        b.destroy()
+    for w in worlds:
+        w.destroy()
+    log(f"full default: {value:.1f} updates/s batched, {plain:.1f} plain")
+    return {"value": round(value, 3), "unit": "GI updates/s", "worlds": len(worlds), "batch": bsz,
+            "plain_fp32": round(plain, 3), "speedup_vs_plain": round(value / plain, 3),
+            "stage_ms_per_batch": {k: round(v, 4) for k, v in stage.items()},
+            "bounce_probes_active_world1": active,
+            "extras": "relocation + occlusion-aware parents, traced sun visibility, bounce volume"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +294,8 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mix-worlds", type=int, default=32, help="batched mixed-world run (0: skip it)")
+    ap.add_argument("--full-worlds", type=int, default=64,
+                    help="run with relocation, traced sun visibility and multi-bounce on (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
     ap.add_argument("--serve-worlds", type=int, default=64)
@@ -268,10 +328,11 @@ def main():
         raise SystemExit("bench.py needs a GPU: no HIP device visible")
 
     base = rtgi.CONFIGS[a.config]
-    worlds, first = [], None
+    worlds, first, scenes = [], None, []
     t0 = time.perf_counter()
     for k in range(a.worlds):
         sc = rtgi.generate_scene(**dict(base["scene"], seed=k + 1))
+        scenes.append(sc)
         _, vox, rc, frame = rtgi.config(a.config) if k == 0 else (None, base["vox"], base["rc"], first[3])
         frame = dataclasses.replace(frame, eye=sc.eye, sunDir=sc.sunDir)
         w = rtgi.World(vox, rc)
@@ -363,6 +424,7 @@ def main():
         b.destroy()
     for w in worlds:
         w.destroy()
+    full = (full_bench(a, scenes, first[1], first[2], first[3], worlds_tuning, bsz) if a.full_worlds > 0 else None)
     mixed = mix_bench(a) if a.mix_worlds > 0 else None
     serving = serve_bench(a) if a.serve_seconds > 0 else None
 
@@ -408,6 +470,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "algorithmic_bytes": int(alg_bytes), "traffic": traffic},
         "cpu_baseline": cpu,
+        "full_default": full,
         "batched_mix": mixed,
         "serving": serving,
     }

@@ -2,7 +2,8 @@
 // JS face of the Node-API addon: the reference host's pass interface over librtgi.
 //   createVoxWorld(scene, params)  <- createVoxWorld (src/core/voxel.ts:746): update(eye), encode(), readCounters(),
 //                                     usedBytes(c), destroy()
-//   createRc(voxWorld, params)     <- createRc (src/core/rc.ts:1019): encode(), readSh(), readCascade(c), destroy()
+//   createRc(voxWorld, params)     <- createRc (src/core/rc.ts:1019): encode(), readSh(), readCascade(c), destroy(),
+//                                     gather(pos, nrm, occlusion) (gatherWorld), readProbes(c), readBounceVolume()
 // One library world backs both: the voxel world owns it, and createRc re-creates it with the cascade parameters
 // (its scene upload is repeated). encode() on either runs the whole update (rebuild, trace, merge, SH): the reference
 // records both passes into one command buffer per frame (src/core/rcrun.ts:283-298).
@@ -11,7 +12,7 @@ const addon = require(path.join(__dirname, 'rtgi_napi.node'));
 
 const DEFAULT_VOX = { v0: 0.2, levels: 5, G: 32, maxBricks: 4608, maxVoxels: 524288 };
 const DEFAULT_PARAMS = { cascades: 5, probes0: 64, s0: 0.2, kInterval: 1, levelMap: 1, voxScale: 0.25, voxOffset: 1,
-  voxSkip: 0.5, resCap: 8, recordHits: 0 };
+  voxSkip: 0.5, resCap: 8, recordHits: 0, relocate: 0, lift: 0.25, sunTrace: 0, bounce: 0, fp16: 0 };
 const SUN_E = [9, 8.5, 7.8];
 const SKY = [0.45, 0.6, 0.9];
 
@@ -56,6 +57,9 @@ function createRc(voxWorld, params) {
     encode: () => addon.encode(voxWorld.handle),
     readSh: () => Promise.resolve().then(() => addon.readSh(voxWorld.handle)),
     readCascade: (c) => Promise.resolve().then(() => addon.readCascade(voxWorld.handle, c)),
+    gather: (pos, nrm, occlusion) => Promise.resolve().then(() => addon.gather(voxWorld.handle, pos, nrm, !!occlusion)),
+    readProbes: (c) => Promise.resolve().then(() => addon.readProbes(voxWorld.handle, c)),
+    readBounceVolume: () => Promise.resolve().then(() => addon.readBounceVolume(voxWorld.handle)),
     destroy: () => {},
   };
 }

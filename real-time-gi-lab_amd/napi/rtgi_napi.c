@@ -90,7 +90,7 @@ static napi_value js_device_count(napi_env env, napi_callback_info info) {
 }
 
 /* createWorld(vox: {v0, levels, G, maxBricks, maxVoxels}, rc: {cascades, probes0, s0, kInterval, levelMap, voxScale,
- * voxOffset, voxSkip, resCap, recordHits}) -> handle */
+ * voxOffset, voxSkip, resCap, recordHits, relocate, lift, sunTrace, bounce, fp16}) -> handle */
 static napi_value js_create_world(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -119,6 +119,9 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[1], "recordHits", &x) && x != 0) rp.flags |= RTGI_RC_RECORD_HITS;
   if (get_f64(env, argv[1], "lift", &x)) rp.lift = x;
   if (get_f64(env, argv[1], "relocate", &x) && x != 0) rp.flags |= RTGI_RC_RELOCATE;
+  if (get_f64(env, argv[1], "sunTrace", &x) && x != 0) rp.flags |= RTGI_RC_SUN_TRACE;
+  if (get_f64(env, argv[1], "bounce", &x) && x != 0) rp.flags |= RTGI_RC_BOUNCE;
+  if (get_f64(env, argv[1], "fp16", &x) && x != 0) rp.flags |= RTGI_RC_FP16;
   rtgi_world* w = NULL;
   int st = rtgi_world_create(&vp, &rp, &w);
   if (st) return throw_status(env, st);
@@ -276,6 +279,86 @@ static napi_value js_read_cascade(napi_env env, napi_callback_info info) {
   return ta;
 }
 
+static napi_value typed(napi_env env, napi_typedarray_type t, size_t n, size_t esize, void** data) {
+  napi_value ab, ta;
+  if (napi_create_arraybuffer(env, n * esize, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, t, n, ab, 0, &ta) != napi_ok) return NULL;
+  return ta;
+}
+
+/* gather(h, pos: Float32Array, nrm: Float32Array, occlusion) -> Float32Array irradiance (3 per point); replaces
+ * gatherWorld (src/core/rc.ts:871-927) over G-buffer pixels */
+static napi_value js_gather(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc >= 3 ? world_of(env, argv[0]) : NULL;
+  size_t np = 0, nn = 0;
+  float* pos = w ? typed_data(env, argv[1], napi_float32_array, &np) : NULL;
+  float* nrm = w ? typed_data(env, argv[2], napi_float32_array, &nn) : NULL;
+  bool occl = false;
+  if (argc == 4) napi_get_value_bool(env, argv[3], &occl);
+  if (!w || !pos || !nrm || np != nn || np % 3) {
+    napi_throw_type_error(env, NULL, "gather(h, Float32Array pos, Float32Array nrm, occlusion)");
+    return NULL;
+  }
+  void* data;
+  napi_value ta = typed(env, napi_float32_array, np, 4, &data);
+  if (!ta) return NULL;
+  int st = rtgi_world_gather(w, pos, nrm, (uint32_t)(np / 3), occl ? RTGI_GATHER_OCCLUSION : 0u, (float*)data);
+  if (st) return throw_status(env, st);
+  return ta;
+}
+
+/* readProbes(h, c) -> {pos: Float32Array, nrm: Float32Array, parents: Uint32Array} (relocated probes of cascade c) */
+static napi_value js_read_probes(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 2 ? world_of(env, argv[0]) : NULL;
+  int32_t c = -1;
+  rtgi_layout lay;
+  if (!w || napi_get_value_int32(env, argv[1], &c) != napi_ok || rtgi_world_get_layout(w, &lay) || c < 0 ||
+      c >= lay.cascades) {
+    napi_throw_type_error(env, NULL, "readProbes(h, c)");
+    return NULL;
+  }
+  size_t n = (size_t)lay.probes[c] * lay.probes[c] * lay.probes[c];
+  void *p, *q, *r;
+  napi_value tp = typed(env, napi_float32_array, n * 3, 4, &p), tq = typed(env, napi_float32_array, n * 3, 4, &q),
+             tr = typed(env, napi_uint32_array, n, 4, &r), o;
+  if (!tp || !tq || !tr) return NULL;
+  int st = rtgi_world_read_probes(w, c, (float*)p, (float*)q, (uint32_t*)r);
+  if (st) return throw_status(env, st);
+  NAPI_CALL(env, napi_create_object(env, &o));
+  napi_set_named_property(env, o, "pos", tp);
+  napi_set_named_property(env, o, "nrm", tq);
+  napi_set_named_property(env, o, "parents", tr);
+  return o;
+}
+
+/* readBounceVolume(h) -> {texels: Uint16Array (halves: probes x 6 axes x rgba), origin: Int32Array(3)} */
+static napi_value js_read_bounce_volume(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 1 ? world_of(env, argv[0]) : NULL;
+  if (!w) {
+    napi_throw_type_error(env, NULL, "readBounceVolume(h)");
+    return NULL;
+  }
+  size_t n = (size_t)32 * 16 * 32 * 6 * 4;
+  void *t, *g;
+  napi_value tt = typed(env, napi_uint16_array, n, 2, &t), tg = typed(env, napi_int32_array, 3, 4, &g), o;
+  if (!tt || !tg) return NULL;
+  int st = rtgi_world_read_bounce_volume(w, (uint16_t*)t, (int32_t*)g);
+  if (st) return throw_status(env, st);
+  NAPI_CALL(env, napi_create_object(env, &o));
+  napi_set_named_property(env, o, "texels", tt);
+  napi_set_named_property(env, o, "origin", tg);
+  return o;
+}
+
 /* generateScene(kind: 'atrium'|'city', seed, frame) -> {pos, nrm, tri, materials, eye, sunDir} */
 static napi_value js_generate_scene(napi_env env, napi_callback_info info) {
   size_t argc = 3;
@@ -337,6 +420,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readCounters", 0, js_read_counters, 0, 0, 0, napi_default, 0},
       {"readCascade", 0, js_read_cascade, 0, 0, 0, napi_default, 0},
       {"readSh", 0, js_read_sh, 0, 0, 0, napi_default, 0},
+      {"gather", 0, js_gather, 0, 0, 0, napi_default, 0},
+      {"readProbes", 0, js_read_probes, 0, 0, 0, napi_default, 0},
+      {"readBounceVolume", 0, js_read_bounce_volume, 0, 0, 0, napi_default, 0},
       {"destroy", 0, js_destroy, 0, 0, 0, napi_default, 0},
       {"generateScene", 0, js_generate_scene, 0, 0, 0, napi_default, 0},
   };

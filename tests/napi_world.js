@@ -3,16 +3,35 @@
 // (createVoxWorld -> createRc -> update -> encode -> readCounters / read SH) and prints the results as JSON.
 const path = require('path');
 const r = require(path.join(__dirname, '..', 'real-time-gi-lab_amd', 'napi', 'rtgi.js'));
-const [kind, seed, v0, levels, G, cascades, probes0, s0] = process.argv.slice(2).map((x, i) => (i ? Number(x) : x));
+const args = process.argv.slice(2);
+const [kind, seed, v0, levels, G, cascades, probes0, s0] = args.slice(0, 8).map((x, i) => (i ? Number(x) : x));
+// 'full': the reference's default extras (voxel relocation, per-voxel sun visibility, multi-bounce), two updates, and
+// the gather / probe / bounce-volume readbacks.
+const full = args[8] === 'full';
 const scene = r.generateScene(kind, seed, 0);
 const vw = r.createVoxWorld(scene, { v0, levels, G, maxBricks: levels * G * G * G, maxVoxels: 1 << 20 },
   { eye: scene.eye, sunDir: scene.sunDir });
-const rc = r.createRc(vw, { cascades, probes0, s0 });
+const rc = r.createRc(vw, full ? { cascades, probes0, s0, relocate: 1, sunTrace: 1, bounce: 1 } : { cascades, probes0, s0 });
 vw.update(scene.eye);
 rc.encode();
-Promise.all([vw.readCounters(), rc.readSh(), rc.readCascade(cascades - 1)]).then(([c, sh, top]) => {
-  let s = 0;
-  for (let i = 0; i < sh.length; i++) s += sh[i];
-  process.stdout.write(JSON.stringify({ counters: c, shSum: s, sh0: Array.from(sh.slice(0, 27)), topLen: top.length }) + '\n');
+if (full) rc.encode();
+const sum = (a) => { let s = 0; for (let i = 0; i < a.length; i++) s += a[i]; return s; };
+// Gather points: an 8 x 8 grid 1 m below the eye, facing up.
+const pos = new Float32Array(64 * 3), nrm = new Float32Array(64 * 3);
+for (let i = 0; i < 64; i++) {
+  pos.set([scene.eye[0] + (i % 8) - 3.5, scene.eye[1] - 1, scene.eye[2] + Math.floor(i / 8) - 3.5], i * 3);
+  nrm.set([0, 1, 0], i * 3);
+}
+const extra = full ? [rc.gather(pos, nrm, true), rc.readProbes(0), rc.readBounceVolume()] : [];
+Promise.all([vw.readCounters(), rc.readSh(), rc.readCascade(cascades - 1), ...extra]).then(([c, sh, top, irr, pr, bv]) => {
+  const out = { counters: c, shSum: sum(sh), sh0: Array.from(sh.slice(0, 27)), topLen: top.length };
+  if (full) {
+    out.irr = Array.from(irr);
+    out.probeSum = sum(pr.pos) + sum(pr.nrm);
+    out.parentsSum = sum(pr.parents);
+    out.texelSum = sum(bv.texels);
+    out.origin = Array.from(bv.origin);
+  }
+  process.stdout.write(JSON.stringify(out) + '\n');
   vw.destroy();
 });

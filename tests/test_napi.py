@@ -65,3 +65,36 @@ def test_addon_update_matches_python_path(require_gpu):
         assert out["counters"][k] == c[k]
     np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
     assert abs(out["shSum"] - float(sh.astype(np.float64).sum())) <= 1e-6 * abs(float(sh.sum()))
+
+
+@pytest.mark.gpu
+def test_addon_full_default_matches_python_path(require_gpu):
+    """The addon with voxel relocation, traced sun visibility and multi-bounce over two updates: SH, gather,
+    relocated probes and the bounce volume equal the Python path's."""
+    print("case: node addon, atrium seed 2, relocation + sun + bounce, 2 updates", flush=True)
+    r = node(os.path.join(ROOT, "tests", "napi_world.js"), "atrium", "2", "0.125", "4", "8", "4", "32", "0.625", "full",
+             timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    scene = rtgi.generate_scene("atrium", seed=2)
+    vox = rtgi.VoxParams(v0=0.125, levels=4, G=8, maxBricks=4 * 8 ** 3, maxVoxels=1 << 20)
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625, relocate=True, sunTrace=True, bounce=True)
+    eye = np.asarray(scene.eye, np.float32)
+    g = np.arange(64)
+    pos = np.stack([eye[0] + (g % 8) - 3.5, np.full(64, eye[1] - 1), eye[2] + g // 8 - 3.5], 1).astype(np.float32)
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (64, 1))
+    with rtgi.World(vox, rc) as w:
+        w.upload_scene(scene)
+        w.set_frame(rtgi.Frame(eye=scene.eye, sunDir=scene.sunDir))
+        w.encode()
+        w.encode()
+        sh = w.read_sh()
+        irr = w.gather(pos, nrm, occlusion=True)
+        pp, pn, par = w.read_probes(0)
+        vol, org = w.read_bounce_volume()
+    np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
+    np.testing.assert_array_equal(np.float32(out["irr"]), irr.ravel())
+    assert abs(out["probeSum"] - (float(pp.astype(np.float64).sum()) + float(pn.astype(np.float64).sum()))) <= 1e-3
+    assert out["parentsSum"] == int(par.astype(np.int64).sum())
+    assert out["texelSum"] == int(vol.view(np.uint16).astype(np.int64).sum())
+    assert out["origin"] == org.tolist()
