@@ -118,6 +118,7 @@ VoxK vox_args(const rtgi_world* w) {
   k.rep = w->rep;
   k.attr = w->attr;
   k.plane = w->plane;
+  k.vslot = w->vslot;
   k.big_vox = w->big_vox;
   k.big_mark = w->big_mark;
   k.big_cap = big_cap_for(w->ntris);
@@ -258,6 +259,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->sunvis, (w->rp.flags & RTGI_RC_SUN_TRACE) ? mv : 0);
   dfree(w, w->bounce, (w->rp.flags & RTGI_RC_BOUNCE) ? mv * 4 : 0);
   for (int i = 0; i < 2; i++) dfree(w, w->bnvol[i], (w->rp.flags & RTGI_RC_BOUNCE) ? BN_WORDS : 0);
+  dfree(w, w->vslot, (w->rp.flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) ? mv : 0);
 }
 
 }  // namespace
@@ -440,6 +442,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
     b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
   if (rp->flags & RTGI_RC_SUN_TRACE) b += nbytes<float>(mv);
   if (rp->flags & RTGI_RC_BOUNCE) b += nbytes<uint32_t>(mv * 4) + 2 * nbytes<uint32_t>(BN_WORDS);
+  if (rp->flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) b += nbytes<uint32_t>(mv);
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -493,6 +496,7 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
   if (!e && (rp->flags & RTGI_RC_BOUNCE)) e = dalloc(w, &w->bounce, mv * 4);
   for (int i = 0; i < 2 && !e && (rp->flags & RTGI_RC_BOUNCE); i++) e = dalloc(w, &w->bnvol[i], BN_WORDS);
+  if (!e && (rp->flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE))) e = dalloc(w, &w->vslot, mv);
   if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
     size_t np = (size_t)w->probe_off[Nc];
     if (!e) e = dalloc(w, &w->ppos, np * 4);

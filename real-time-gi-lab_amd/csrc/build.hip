@@ -692,6 +692,7 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
     uint32_t r = __popc(word & ((1u << (bit & 31u)) - 1u));
     for (uint32_t j = 0; j < w; j++) r += __popc(o[j]);
     uint32_t ai = first + r;
+    if (k.vslot) k.vslot[ai] = (ptr << 9) | bit;
     uint32_t code = k.bcode[ptr];
     int L = (int)(code >> 27);
     i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
@@ -775,10 +776,12 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   for (int L = 1; L < k.NL; L++) out[BL_REDUCE + L - 1] = persist;
   out[BL_ALLOC_VOXELS] = 1;
   out[BL_RESOLVE] = persist;
-  if (t.sunvis) out[BL_SUN] = persist;
+  // Per attribute slot: 4 lanes per voxel for the sun taps, one for the bounce injection (grid-stride loops).
+  const uint64_t slots = (uint64_t)k.max_voxels;
+  if (t.sunvis) out[BL_SUN] = std::min<uint32_t>(8192, std::max<uint32_t>(1, cdiv(slots * 4, BLK)));
   if (t.bounce) {
     out[BL_BNVOL] = BN_PROBES / BN_PROBES_PER_BLOCK;
-    out[BL_BOUNCE] = persist;
+    out[BL_BOUNCE] = std::min<uint32_t>(4096, std::max<uint32_t>(1, cdiv(slots, BLK)));
   }
   if (t.reloc) {
     out[BL_RELOC] = cdiv(t.total_probes, BLK);

@@ -55,6 +55,7 @@ struct VoxK {
   uint32_t* rep;
   uint32_t* attr;
   float* plane;
+  uint32_t* vslot;     // [max_voxels] brick << 9 | bit of each attribute slot (written by the resolve), or null
   uint32_t* big_vox;   // [levels][big_cap]
   uint32_t* big_mark;  // [levels][big_cap]
   const float* pos;

@@ -948,21 +948,34 @@ __device__ float sun_ray(const TraceK& a, f3 x, f3 d, f3 inv, int L, float tSkip
   return 1.0f;
 }
 
+// The voxel at attribute slot ai (slot list written by the resolve), or false when the slot holds no voxel of this
+// update (a slot past the voxel-pool limit keeps an older entry: its brick must be allocated, the bit occupied and
+// the rank equal to the slot).
+__device__ __forceinline__ bool slot_voxel(const VoxK& k, uint32_t ai, uint32_t nb, uint32_t& ptr, uint32_t& bit) {
+  uint32_t e = k.vslot[ai];
+  ptr = e >> 9;
+  bit = e & 511u;
+  if (ptr >= nb) return false;
+  if ((k.occ[ptr * 16u + (bit >> 5)] & (1u << (bit & 31u))) == 0u) return false;
+  return vox_rank(k, ptr, bit) == ai;
+}
+
 // The voxel's sun visibility: the mean of 2 x 2 shadow rays from points a quarter voxel along two tangents of the
 // representative's plane, (0.25 voxel + 3 cm) off it on its sunward side, as voxSun places its shadow-map taps.
+// Four lanes per occupied voxel (dense over the attribute slots), one tap each; the quad sums the taps in tap order.
 __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const VoxK& k = a.vx;
-  uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  const uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  const uint32_t nv = min(k.hdr[H_VOXELS], (uint32_t)k.max_voxels);
   f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
   f3 dd = mk3(fabsf(sd.x) < 1e-9f ? 1e-9f : sd.x, fabsf(sd.y) < 1e-9f ? 1e-9f : sd.y,
               fabsf(sd.z) < 1e-9f ? 1e-9f : sd.z);
   f3 inv = mk3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
-  for (uint32_t i = lb * BLK + threadIdx.x; i < nb * 512u; i += nblk * BLK) {
-    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
-    uint32_t word = k.occ[ptr * 16u + w];
-    if ((word & (1u << (bit & 31u))) == 0u) continue;
-    uint32_t ai = vox_rank(k, ptr, bit);
-    if (ai == ATTR_NONE) continue;
+  // The stride is a multiple of 4 and so is the bound: a voxel's four lanes are always in the loop together.
+  for (uint64_t t = (uint64_t)lb * BLK + threadIdx.x; t < (uint64_t)nv * 4u; t += (uint64_t)nblk * BLK) {
+    const uint32_t ai = (uint32_t)(t >> 2), tap = (uint32_t)(t & 3u);
+    uint32_t ptr, bit;
+    if (!slot_voxel(k, ai, nb, ptr, bit)) continue;  // the same decision for the quad's four lanes
     uint32_t code = k.bcode[ptr];
     int L = (int)(code >> 27);
     i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
@@ -987,9 +1000,11 @@ __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t 
     f3 t2 = cross3(ns, t1);
     float h = 0.25f * v;
     f3 o = q + ns * (0.25f * v + 0.03f);
-    float vis = 0.25f * (sun_ray(a, o + (t1 + t2) * h, sd, inv, L, v) + sun_ray(a, o + (t1 - t2) * h, sd, inv, L, v) +
-                         sun_ray(a, o - (t1 + t2) * h, sd, inv, L, v) + sun_ray(a, o - (t1 - t2) * h, sd, inv, L, v));
-    a.sunvis[ai] = vis;
+    f3 x = tap == 0u ? o + (t1 + t2) * h : tap == 1u ? o + (t1 - t2) * h : tap == 2u ? o - (t1 + t2) * h
+                                                                                    : o - (t1 - t2) * h;
+    float r = sun_ray(a, x, sd, inv, L, v);
+    float r0 = __shfl(r, 0, 4), r1 = __shfl(r, 1, 4), r2 = __shfl(r, 2, 4), r3 = __shfl(r, 3, 4);
+    if (tap == 0u) a.sunvis[ai] = 0.25f * (r0 + r1 + r2 + r3);
   }
 }
 __global__ void __launch_bounds__(BLK) k_sun(TraceK a) { sun_body(a, blockIdx.x, gridDim.x); }
@@ -1204,13 +1219,12 @@ __global__ void __launch_bounds__(BLK) k_bnvol_b(BatchK b) {
 // the side. Four words per attribute slot: front rg, b, back rg, b (halves).
 __device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const VoxK& k = a.vx;
-  uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  const uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
+  const uint32_t nv = min(k.hdr[H_VOXELS], (uint32_t)k.max_voxels);
   const float tol = 0.5f * k.v0;
-  for (uint32_t i = lb * BLK + threadIdx.x; i < nb * 512u; i += nblk * BLK) {
-    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
-    if ((k.occ[ptr * 16u + w] & (1u << (bit & 31u))) == 0u) continue;
-    uint32_t ai = vox_rank(k, ptr, bit);
-    if (ai == ATTR_NONE) continue;
+  for (uint32_t ai = lb * BLK + threadIdx.x; ai < nv; ai += nblk * BLK) {  // dense over the attribute slots
+    uint32_t ptr, bit;
+    if (!slot_voxel(k, ai, nb, ptr, bit)) continue;
     uint32_t code = k.bcode[ptr];
     int L = (int)(code >> 27);
     i3 bc{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
@@ -1328,18 +1342,18 @@ int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t 
 
 int launch_sun(const TraceK& k, void* stream) {
   if (!k.sunvis) return 0;
-  uint32_t pool = (uint32_t)k.vx.max_bricks * 512u;
-  uint32_t g = std::min<uint32_t>(2048, std::max<uint32_t>(1, (pool + BLK * 4 - 1) / (BLK * 4)));
-  k_sun<<<g, BLK, 0, (hipStream_t)stream>>>(k);
+  uint32_t g[BL_COUNT];
+  batch_blocks(k, g);
+  k_sun<<<g[BL_SUN], BLK, 0, (hipStream_t)stream>>>(k);
   return (int)hipGetLastError();
 }
 
 int launch_bounce(const TraceK& k, void* stream) {
   if (!k.bounce) return 0;
-  k_bnvol<<<BN_PROBES / BN_PROBES_PER_BLOCK, BLK, 0, (hipStream_t)stream>>>(k);
-  uint32_t pool = (uint32_t)k.vx.max_bricks * 512u;
-  uint32_t g = std::min<uint32_t>(2048, std::max<uint32_t>(1, (pool + BLK * 4 - 1) / (BLK * 4)));
-  k_bounce<<<g, BLK, 0, (hipStream_t)stream>>>(k);
+  uint32_t g[BL_COUNT];
+  batch_blocks(k, g);
+  k_bnvol<<<g[BL_BNVOL], BLK, 0, (hipStream_t)stream>>>(k);
+  k_bounce<<<g[BL_BOUNCE], BLK, 0, (hipStream_t)stream>>>(k);
   return (int)hipGetLastError();
 }
 

@@ -50,6 +50,7 @@ struct rtgi_world {
   float *ppos = nullptr, *pnrm = nullptr;  // relocated probes (RTGI_RC_RELOCATE): float4 each
   uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
   float* sunvis = nullptr;                // traced sun visibility per attribute slot (RTGI_RC_SUN_TRACE)
+  uint32_t* vslot = nullptr;              // brick << 9 | bit per attribute slot (RTGI_RC_SUN_TRACE or _BOUNCE)
   uint32_t* bounce = nullptr;             // bounce irradiance per attribute slot, 4 words (RTGI_RC_BOUNCE)
   uint32_t* bnvol[2] = {};                // the two bounce volumes (RTGI_RC_BOUNCE), BN_WORDS each
   int bn_par = 1;                         // the volume the last update wrote
