@@ -18,3 +18,13 @@ __device__ __forceinline__ int batch_world(const BatchK& b, uint32_t blk) {
   }
   return lo;
 }
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block i runs on XCD i mod 8), and each XCD has its own L2.
+// xcd_block maps hardware block i to a logical block so that every XCD runs one contiguous logical range: in a
+// batch, a few whole worlds per XCD instead of a slice of every world, so each L2 holds fewer worlds' clipmaps. It is a
+// bijection on [0, nblk), so no work moves between blocks, only where it runs.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t blk, uint32_t nblk) {
+  const uint32_t X = 8u;
+  uint32_t x = blk % X, i = blk / X, q = nblk / X, r = nblk % X;
+  return x < r ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
+}

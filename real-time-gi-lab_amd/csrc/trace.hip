@@ -224,16 +224,19 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   const __amdgpu_buffer_rsrc_t ptr_r = buf_rsrc(k.ptr, (uint32_t)(k.NL * k.per) * 4u);
   const __amdgpu_buffer_rsrc_t occ_r = buf_rsrc(k.occ, (uint32_t)k.max_bricks * 64u);
   const uint32_t ptr_base = (uint32_t)L * (uint32_t)k.per * 4u;
-  uint32_t lastb = 0xffffffffu, ptr = 0;
+  uint32_t ptr = 0;
+  i3 lc{-8, -8, -8};  // a voxel of the brick whose pointer was read last (none yet: every XOR below is negative)
   int it;
   // Steps and brick jumps select with conditional moves rather than branches (the same f32 operations: each
   // candidate sum is the one the branch would form), so lanes on different axes do not split the wavefront.
   for (it = 0; it < 1024; it++) {
     i3 b = shr3(c, 3);
-    uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
-    if (bi != lastb) {  // the pointer of the brick is read once per brick visited
+    // The pointer of a brick is read once per brick visited: c left that brick when some coordinate differs above
+    // its low 3 bits (c >= 0 inside the window).
+    if ((uint32_t)((c.x ^ lc.x) | (c.y ^ lc.y) | (c.z ^ lc.z)) > 7u) {
+      uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
       ptr = buf_u32(ptr_r, ptr_base + bi * 4u);
-      lastb = bi;
+      lc = c;
     }
     if (ptr >= PTR_MARK && ptr != PTR_DROP) {  // empty or marked-unallocated brick: jump to its exit
       f3 bt = ((tof3(b * 8) + sp * 8.0f) * v + lo - o) * inv;
@@ -249,21 +252,23 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       continue;
     }
     bool occupied;
-    uint32_t hbrick, hbit, hid;
+    uint32_t hbrick, hbit;
+    int hlev = L;
+    i3 hcell = c;
     if (ptr == PTR_DROP) {  // dropped brick: voxel by voxel, each read at the coarser level that holds it
       VoxHit h;
       occupied = vox_at(k, L, c, h);
       hbrick = h.brick;
       hbit = h.bit;
-      hid = occupied ? hit_id(k, h.level, h.cell) : 0u;
+      hlev = h.level;
+      hcell = h.cell;
     } else {
       hbrick = ptr;
       hbit = bit_of(c);
       occupied = (buf_u32(occ_r, ptr * 64u + (hbit >> 5) * 4u) & (1u << (hbit & 31u))) != 0u;
-      hid = hit_id(k, L, c);
     }
-    if (occupied && tIn >= tSkip) {
-      res = Hit{true, hbrick, hbit, tIn, hid, ST ? (uint32_t)it + 1u : 0u};
+    if (occupied && tIn >= tSkip) {  // the hit id is formed once, here
+      res = Hit{true, hbrick, hbit, tIn, hit_id(k, hlev, hcell), ST ? (uint32_t)it + 1u : 0u};
       return res;
     }
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
@@ -428,8 +433,9 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 }
 template <bool ST>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  trace_body<ST>(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  trace_body<ST>(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ f3 rgb_of(float4 x) { return mk3(x.x, x.y, x.z); }
@@ -592,15 +598,17 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
 }
 __global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  sh_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  sh_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Merge step of a batch: each world merges its cascade N-1-step (c > 0 with the 2x2 averages, c = 0 per ray).
 __global__ void __launch_bounds__(BLK) k_merge_step_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
-  uint32_t lb = blockIdx.x - b.start[w], nb = b.start[w + 1] - b.start[w];
+  uint32_t lb = blk - b.start[w], nb = b.start[w + 1] - b.start[w];
   int c = a.N - 1 - b.arg;
   if (c > 0)
     merge_up_body(a, c, lb, nb);
@@ -710,9 +718,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 // Step of a batch: each world traces and merges its cascade N-1-step.
 template <bool ST>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
-  trace_merge_body<ST>(a, a.N - 1 - b.arg, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  trace_merge_body<ST>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
@@ -1009,8 +1018,9 @@ __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t 
 }
 __global__ void __launch_bounds__(BLK) k_sun(TraceK a) { sun_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  sun_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  sun_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // ---- Bounce volume (src/core/bounce.ts): update (bnClassify / bnUpdate) and injection into the voxels (bnInject) ----
@@ -1210,8 +1220,9 @@ __device__ __forceinline__ void bnvol_body(const TraceK& a, uint32_t lb) {
 }
 __global__ void __launch_bounds__(BLK) k_bnvol(TraceK a) { bnvol_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(BLK) k_bnvol_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  bnvol_body(b.w[w], blockIdx.x - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  bnvol_body(b.w[w], blk - b.start[w]);
 }
 
 // bnInject (bounce.ts:224-262): every occupied voxel stores this update's volume irradiance on its two sides (along
@@ -1246,8 +1257,9 @@ __device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32
 }
 __global__ void __launch_bounds__(BLK) k_bounce(TraceK a) { bounce_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_bounce_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  bounce_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  bounce_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ int probe_cascade(const TraceK& a, uint64_t t) {
@@ -1287,12 +1299,14 @@ __device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t
 __global__ void __launch_bounds__(BLK) k_relocate(TraceK a) { relocate_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_link(TraceK a) { link_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_relocate_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  relocate_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  relocate_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 __global__ void __launch_bounds__(BLK) k_link_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  link_body(b.w[w], blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  link_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 int grid_for(uint64_t n) {
