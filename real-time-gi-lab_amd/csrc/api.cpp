@@ -114,6 +114,7 @@ VoxK vox_args(const rtgi_world* w) {
   k.ptr = w->ptr;
   k.occ = w->occ;
   k.battr = w->battr;
+  k.wslot = w->wslot;
   k.bcode = w->bcode;
   k.rep = w->rep;
   k.attr = w->attr;
@@ -236,6 +237,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->ptr, NLper);
   dfree(w, w->occ, mb * 16);
   dfree(w, w->battr, mb);
+  dfree(w, w->wslot, mb * 16);
   dfree(w, w->bcode, mb);
   dfree(w, w->rep, mb * 512);
   dfree(w, w->attr, mv * 2);
@@ -433,7 +435,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
   if (world_layout(&w, vp, rp)) return 0;
   size_t NLper = (size_t)w.NL * w.per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
   uint64_t b = nbytes<uint32_t>(HDR_WORDS) + nbytes<uint32_t>(NLper) + nbytes<uint32_t>(mb * 16) +
-               nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
+               nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 16) + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
                nbytes<float>(rad_floats(&w)) + nbytes<float>(sh_floats(&w)) +
                nbytes<DirEntry>((size_t)w.dir_off[w.N] + BN_RAYS) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
   if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
@@ -484,6 +486,7 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->ptr, NLper);
   if (!e) e = dalloc(w, &w->occ, mb * 16);
   if (!e) e = dalloc(w, &w->battr, mb);
+  if (!e) e = dalloc(w, &w->wslot, mb * 16);
   if (!e) e = dalloc(w, &w->bcode, mb);
   if (!e) e = dalloc(w, &w->rep, mb * 512);
   if (!e) e = dalloc(w, &w->attr, mv * 2);

@@ -202,8 +202,9 @@ __device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t 
 }
 __global__ void __launch_bounds__(BLK) k_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  clear_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  clear_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level
@@ -236,8 +237,9 @@ __device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t n
 }
 __global__ void __launch_bounds__(BLK) k_mark(VoxK k) { mark_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  mark_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  mark_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // One wavefront per listed (triangle, level); entries of all levels are taken in level order.
@@ -273,8 +275,9 @@ __device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32
 }
 __global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) { mark_big_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_big_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  mark_big_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  mark_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ void mark_parents_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
@@ -295,8 +298,9 @@ __device__ __forceinline__ void mark_parents_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) { mark_parents_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_parents_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  mark_parents_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  mark_parents_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Block-wide exclusive sum of one value per thread (1024 threads), thread order; *total gets the sum. Every thread
@@ -381,8 +385,9 @@ __device__ __forceinline__ void alloc_bricks_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) { alloc_bricks_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(1024) k_alloc_bricks_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  alloc_bricks_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  alloc_bricks_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ uint32_t n_bricks(const VoxK& k) { return min(k.hdr[H_MARKED], (uint32_t)k.max_bricks); }
@@ -397,8 +402,9 @@ __device__ __forceinline__ void init_bricks_body(const VoxK& k, uint32_t lb, uin
 }
 __global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) { init_bricks_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_init_bricks_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  init_bricks_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  init_bricks_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Voxel updates of a workgroup are first merged in an LDS hash table keyed by (brick slot, bit): each unique voxel
@@ -497,8 +503,9 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
 }
 __global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) { vox_small_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  vox_small_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  vox_small_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // voxelizeBig (src/core/voxel.ts:515-531): one wavefront per listed (triangle, level), one cell per lane.
@@ -534,8 +541,9 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
 }
 __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  vox_big_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
@@ -591,8 +599,9 @@ __device__ __forceinline__ void reduce_body(const VoxK& k, int L, uint32_t lb, u
 }
 __global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) { reduce_body(k, L, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_reduce_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  reduce_body(b.w[w].vx, b.arg, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  reduce_body(b.w[w].vx, b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ uint32_t brick_pop(const VoxK& k, uint32_t b) {
@@ -621,11 +630,29 @@ __device__ __forceinline__ void alloc_voxels_body(const VoxK& k, uint32_t lb, ui
     uint32_t f = first + block_excl_sum(n[0] + n[1] + n[2] + n[3], wsum, &round);
     for (int j = 0; j < 4; j++) {
       if (b + j >= nb) break;
-      if (f + n[j] > (uint32_t)k.max_voxels) {
+      const bool fits = f + n[j] <= (uint32_t)k.max_voxels;
+      if (!fits) {
         k.battr[b + j] = ATTR_NONE;
         ovf++;
       } else {
         k.battr[b + j] = f;
+      }
+      // First slot of each occupancy word (voxRank's prefix, so that a rank is one load and one popcount).
+      const uint4* o = reinterpret_cast<const uint4*>(k.occ + (size_t)(b + j) * 16u);
+      uint4* ws = reinterpret_cast<uint4*>(k.wslot + (size_t)(b + j) * 16u);
+      uint32_t run = f;
+      for (int q = 0; q < 4; q++) {
+        uint4 x = o[q];
+        uint4 y;
+        y.x = fits ? run : ATTR_NONE;
+        run += __popc(x.x);
+        y.y = fits ? run : ATTR_NONE;
+        run += __popc(x.y);
+        y.z = fits ? run : ATTR_NONE;
+        run += __popc(x.z);
+        y.w = fits ? run : ATTR_NONE;
+        run += __popc(x.w);
+        ws[q] = y;
       }
       f += n[j];
     }
@@ -636,8 +663,9 @@ __device__ __forceinline__ void alloc_voxels_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) { alloc_voxels_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(1024) k_alloc_voxels_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  alloc_voxels_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  alloc_voxels_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
@@ -687,11 +715,9 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
     const uint32_t* o = k.occ + (size_t)ptr * 16u;
     uint32_t word = o[w];
     if ((word & (1u << (bit & 31u))) == 0u) continue;
-    uint32_t first = k.battr[ptr];
+    uint32_t first = k.wslot[ptr * 16u + w];
     if (first == ATTR_NONE) continue;
-    uint32_t r = __popc(word & ((1u << (bit & 31u)) - 1u));
-    for (uint32_t j = 0; j < w; j++) r += __popc(o[j]);
-    uint32_t ai = first + r;
+    uint32_t ai = first + __popc(word & ((1u << (bit & 31u)) - 1u));
     if (k.vslot) k.vslot[ai] = (ptr << 9) | bit;
     uint32_t code = k.bcode[ptr];
     int L = (int)(code >> 27);
@@ -726,8 +752,9 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
 }
 __global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_resolve_b(BatchK b) {
-  int w = batch_world(b, blockIdx.x);
-  resolve_body(b.w[w].vx, blockIdx.x - b.start[w], b.start[w + 1] - b.start[w]);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  resolve_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 

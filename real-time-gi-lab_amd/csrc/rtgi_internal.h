@@ -31,6 +31,7 @@ constexpr uint32_t ATTR_NONE = 0xffffffffu;
 //   ptr    [levels][G^3]       brick pointer per window brick (index z-major: (bz G + by) G + bx), or a sentinel
 //   occ    [max_bricks][16]    occupancy words of a brick (bit = x + 8 y + 64 z), 64 B per brick
 //   battr  [max_bricks]        first attribute slot of the brick's voxels (ATTR_NONE: no slots)
+//   wslot  [max_bricks][16]    first attribute slot of each occupancy word (battr + popcounts of the earlier words)
 //   bcode  [max_bricks]        level << 27 | bx << 18 | by << 9 | bz (the reference's brick code word)
 //   rep    [max_bricks][512]   representative triangle + 1 per voxel (build temporary, kept for readback)
 //   attr   [max_voxels][2]     pack4x8unorm albedo, pack2x16snorm oct normal
@@ -51,6 +52,7 @@ struct VoxK {
   uint32_t* ptr;
   uint32_t* occ;
   uint32_t* battr;
+  uint32_t* wslot;     // [max_bricks][16] first attribute slot of each occupancy word (ATTR_NONE: brick has none)
   uint32_t* bcode;
   uint32_t* rep;
   uint32_t* attr;

@@ -111,21 +111,17 @@ __device__ bool vox_at(const VoxK& k, int L, i3 c, VoxHit& h) {
   }
 }
 
-// voxRank (src/core/voxel.ts:81-87): attribute slot of a voxel, ATTR_NONE when its brick got no slots.
+// voxRank (src/core/voxel.ts:81-87): attribute slot of a voxel, ATTR_NONE when its brick got no slots. The per-word
+// first slots (written with the slot allocation) hold the brick's first slot plus the popcounts of the earlier words.
 __device__ __forceinline__ uint32_t vox_rank(const VoxK& k, uint32_t brick, uint32_t bit) {
-  uint32_t first = k.battr[brick];
-  if (first == ATTR_NONE) return ATTR_NONE;
-  const uint32_t* o = k.occ + (size_t)brick * 16u;
   uint32_t w = bit >> 5;
-  uint32_t r = __popc(o[w] & ((1u << (bit & 31u)) - 1u));
-  for (uint32_t i = 0; i < w; i++) r += __popc(o[i]);
-  return first + r;
+  uint32_t first = k.wslot[brick * 16u + w];
+  if (first == ATTR_NONE) return ATTR_NONE;
+  return first + __popc(k.occ[brick * 16u + w] & ((1u << (bit & 31u)) - 1u));
 }
 
-__device__ __forceinline__ float unorm8_to_f(uint32_t b) { return (float)b / 255.0f; }
-__device__ __forceinline__ float snorm16_to_f(uint32_t h) {
-  return fmaxf((float)(int16_t)(uint16_t)h / 32767.0f, -1.0f);
-}
+__device__ __forceinline__ float unorm8_to_f(uint32_t b) { return div255((float)b); }
+__device__ __forceinline__ float snorm16_to_f(uint32_t h) { return fmaxf(div32767((float)(int16_t)(uint16_t)h), -1.0f); }
 
 // Hit radiance of an occupied voxel: attributes, oct-decoded (unnormalised) normal, shadeHitV with no bounce
 // (src/core/rc.ts:108-116, 136-140; voxel.ts:150-156, 185-192). Point lights are an addition of this path (0 lights:
@@ -479,11 +475,27 @@ __device__ __forceinline__ uint32_t parent_index(const CascK& U, i3 g, int kk) {
 __device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t pi, uint32_t di) {
   const CascK& U = a.c[c + 1];
   const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
-  uint32_t m = parent_mask(U, g);
-  if (a.reloc) m &= a.pvis[a.c[c].probe_off + pi];
+  uint32_t m;
   float4 x[8];
-  for (int kk = 0; kk < 8; kk++)
-    x[kk] = ((m >> kk) & 1u) ? A[parent_index(U, g, kk)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (U.tile_log2 == 0 && U.n_log2 >= 0) {
+    // Rows of a power-of-two grid: the parents are the base point g >> 1 plus unit steps, and the mask is the AND of
+    // one pattern per axis (bits with o = 0 on that axis: 0x55, 0x33, 0x0f; all 8 where the axis has a second parent).
+    i3 b = shr3(g, 1);
+    const int nl = U.n_log2;
+    bool hx = (g.x & 1) && b.x + 1 < U.n, hy = (g.y & 1) && b.y + 1 < U.n, hz = (g.z & 1) && b.z + 1 < U.n;
+    m = (hx ? 0xffu : 0x55u) & (hy ? 0xffu : 0x33u) & (hz ? 0xffu : 0x0fu);
+    if (a.reloc) m &= a.pvis[a.c[c].probe_off + pi];
+    uint32_t i0 = (uint32_t)b.x | ((uint32_t)b.y << nl) | ((uint32_t)b.z << (2 * nl));
+    for (int kk = 0; kk < 8; kk++) {
+      uint32_t i = i0 + (uint32_t)(kk & 1) + ((uint32_t)((kk >> 1) & 1) << nl) + ((uint32_t)(kk >> 2) << (2 * nl));
+      x[kk] = ((m >> kk) & 1u) ? A[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  } else {
+    m = parent_mask(U, g);
+    if (a.reloc) m &= a.pvis[a.c[c].probe_off + pi];
+    for (int kk = 0; kk < 8; kk++)
+      x[kk] = ((m >> kk) & 1u) ? A[parent_index(U, g, kk)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
   f3 acc = splat3(0.0f);
   for (int kk = 0; kk < 8; kk++)
     if ((m >> kk) & 1u) acc = acc + mk3(x[kk].x, x[kk].y, x[kk].z);

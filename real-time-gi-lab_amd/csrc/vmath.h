@@ -57,3 +57,17 @@ __device__ __forceinline__ bool inside3(i3 a, int n) {
 
 // 2^-e for 0 <= e < 126 as an f32 built from its exponent bits: equal to 1.0f / (float)(1u << e), which is exact.
 __device__ __forceinline__ float inv_pow2(int e) { return __int_as_float((127 - e) << 23); }
+
+// x / 255 and x / 32767, correctly rounded, for the integer x of an 8-bit unorm (0..255) and of a 16-bit snorm
+// (-32768..32767): a multiply by the rounded reciprocal and one FMA correction step. For these inputs the result
+// equals IEEE division (checked exhaustively in exact rational arithmetic), at 3 instructions instead of about 10.
+__device__ __forceinline__ float div255(float x) {
+  const float r = 1.0f / 255.0f;
+  float q = x * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, 255.0f, x), r, q);
+}
+__device__ __forceinline__ float div32767(float x) {
+  const float r = 1.0f / 32767.0f;
+  float q = x * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, 32767.0f, x), r, q);
+}

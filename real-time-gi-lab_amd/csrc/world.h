@@ -36,7 +36,7 @@ struct rtgi_world {
   uint32_t cap_tris = 0, cap_verts = 0, cap_mat = 0;
   // device buffers
   uint32_t *hdr = nullptr, *ptr = nullptr, *occ = nullptr, *battr = nullptr, *bcode = nullptr, *rep = nullptr,
-           *attr = nullptr;
+           *attr = nullptr, *wslot = nullptr;
   float* plane = nullptr;
   uint32_t *big_vox = nullptr, *big_mark = nullptr;
   float *pos = nullptr, *nrm = nullptr, *mat = nullptr;
