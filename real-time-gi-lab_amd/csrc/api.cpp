@@ -105,6 +105,9 @@ VoxK vox_args(const rtgi_world* w) {
   k.inv_v0 = k.pow2 ? 1.0f / k.v0 : 0.0f;
   k.NL = w->NL;
   k.G = w->G;
+  k.G_log2 = -1;
+  for (int l = 0; l < 16; l++)
+    if ((1 << l) == w->G) k.G_log2 = l;
   k.E = w->E;
   k.per = w->per;
   k.max_bricks = w->vp.max_bricks;

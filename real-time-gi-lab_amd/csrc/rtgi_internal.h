@@ -46,6 +46,7 @@ struct VoxK {
   float inv_v0;   // 1 / v0, used only when pow2 (then x / v == x * (1/v) exactly)
   int32_t pow2;   // v0 is a power of two
   int32_t NL, G, E, per;  // per = G^3
+  int32_t G_log2;         // log2(G) when G is a power of two, else -1 (brick indices by shifts)
   int32_t max_bricks, max_voxels;
   int32_t origin[RTGI_MAX_LEVELS][3];
   uint32_t* hdr;

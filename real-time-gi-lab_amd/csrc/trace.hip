@@ -226,19 +226,28 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   // Steps and brick jumps select with conditional moves rather than branches (the same f32 operations: each
   // candidate sum is the one the branch would form), so lanes on different axes do not split the wavefront.
   for (it = 0; it < 1024; it++) {
-    i3 b = shr3(c, 3);
     // The pointer of a brick is read once per brick visited: c left that brick when some coordinate differs above
     // its low 3 bits (c >= 0 inside the window).
     if ((uint32_t)((c.x ^ lc.x) | (c.y ^ lc.y) | (c.z ^ lc.z)) > 7u) {
-      uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
-      ptr = buf_u32(ptr_r, ptr_base + bi * 4u);
+      const i3 b = shr3(c, 3);
+      uint32_t off;
+      if (k.G_log2 >= 0) {  // (bz G + by) G + bx and the level's table by shifts
+        const int gl = k.G_log2;
+        off = (((uint32_t)L << (3 * gl)) | ((uint32_t)b.z << (2 * gl)) | ((uint32_t)b.y << gl) | (uint32_t)b.x) << 2;
+      } else {
+        off = ptr_base + (((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x) * 4u;
+      }
+      ptr = buf_u32(ptr_r, off);
       lc = c;
     }
     if (ptr >= PTR_MARK && ptr != PTR_DROP) {  // empty or marked-unallocated brick: jump to its exit
+      const i3 b = shr3(c, 3);
       f3 bt = ((tof3(b * 8) + sp * 8.0f) * v + lo - o) * inv;
       float te = fminf(fminf(bt.x, bt.y), bt.z);
       if (te >= tEnd) break;
-      bool bx = bt.x <= bt.y && bt.x <= bt.z, by = !bx && bt.y <= bt.z, bz = !bx && !by;
+      // The reference's axis choice (x if bt.x <= bt.y and bt.x <= bt.z, else y if bt.y <= bt.z, else z) is the
+      // first axis equal to the minimum (values are finite; +0 == -0).
+      bool bx = bt.x == te, by = !bx && bt.y == te, bz = !bx && !by;
       i3 nb{b.x + (bx ? st.x : 0), b.y + (by ? st.y : 0), b.z + (bz ? st.z : 0)};
       if (!inside3(nb, G)) break;
       t = te;
@@ -260,17 +269,21 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       hcell = h.cell;
     } else {
       hbrick = ptr;
-      hbit = bit_of(c);
-      occupied = (buf_u32(occ_r, ptr * 64u + (hbit >> 5) * 4u) & (1u << (hbit & 31u))) != 0u;
+      hbit = 0u;  // formed at a hit
+      // bit = x + 8 y + 64 z of the cell in its brick: word bit >> 5 = (y >> 2 & 1) | z << 1, shift bit & 31.
+      const uint32_t wi = (((uint32_t)c.y >> 2) & 1u) | (((uint32_t)c.z & 7u) << 1);
+      const uint32_t sh = ((uint32_t)c.x & 7u) | (((uint32_t)c.y & 3u) << 3);
+      occupied = ((buf_u32(occ_r, (ptr << 6) | (wi << 2)) >> sh) & 1u) != 0u;
     }
     if (occupied && tIn >= tSkip) {  // the hit id is formed once, here
+      if (ptr != PTR_DROP) hbit = bit_of(c);
       res = Hit{true, hbrick, hbit, tIn, hit_id(k, hlev, hcell), ST ? (uint32_t)it + 1u : 0u};
       return res;
     }
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if (tExit >= tEnd) break;
     tIn = tExit;
-    bool sx = tMax.x <= tMax.y && tMax.x <= tMax.z, sy = !sx && tMax.y <= tMax.z, sz = !sx && !sy;
+    bool sx = tMax.x == tExit, sy = !sx && tMax.y == tExit, sz = !sx && !sy;  // as for the brick jump above
     c = i3{c.x + (sx ? st.x : 0), c.y + (sy ? st.y : 0), c.z + (sz ? st.z : 0)};
     f3 nt = tMax + tDelta;
     tMax = mk3(sx ? nt.x : tMax.x, sy ? nt.y : tMax.y, sz ? nt.z : tMax.z);
