@@ -368,18 +368,24 @@ def main():
     else:
         from rtgi.autotune import autotune
 
-        widths = sorted({w for w in (8, 16, 32) if w < len(worlds)} | {len(worlds)})
-        tune = autotune(worlds, tiles=(1, 4), widths=widths, repeats=2, log=log)
+        # One batch of every world: its stage events then time each launch alone, so the roofline below is per launch.
+        # (Four concurrent batches of 16 were about 1 % faster in throughput, with launches overlapping in time.)
+        tune = autotune(worlds, tiles=(1, 4), widths=(len(worlds),), repeats=2, log=log)
         bsz = tune["best"]["width"]
     worlds_tuning = worlds[0].tuning()
     groups = [worlds[i:i + bsz] for i in range(0, len(worlds), bsz)]
     batches = [rtgi.Batch(g, graph=True) for g in groups]
+    # Stage timing by HIP events recorded on each batch's stream between its per-stage graphs, read after every timed
+    # step: the stage durations of the timed updates themselves.
+    for b in batches:
+        b.set_timing(True)
     for _ in range(a.warmup):
         for b in batches:
             b.encode()
         for b in batches:
             b.sync()
     lat, step_ms = [], []
+    st = {"vox": [], "trace": [], "merge": [], "sh": []}
     t_start = time.perf_counter()
     for s in range(a.steps):
         ts = time.perf_counter()
@@ -389,21 +395,14 @@ def main():
             b.sync()
             lat.extend([time.perf_counter() - ts] * len(g))
         step_ms.append((time.perf_counter() - ts) * 1e3)
+        for b, g in zip(batches, groups):
+            if len(g) == len(groups[0]):
+                for k, v in b.stage_ms().items():
+                    st[k].append(v)
         log(f"step {s + 1}/{a.steps}: {step_ms[-1]:.2f} ms")
     elapsed = time.perf_counter() - t_start
     value = a.steps * len(worlds) / elapsed
-
-    # Per-stage device time of one batched update (HIP events, direct launches on the batch's stream).
-    tb = rtgi.Batch(groups[0])
-    tb.set_timing(True)
-    st = {"vox": [], "trace": [], "merge": [], "sh": []}
-    for _ in range(3):
-        tb.encode()
-        tb.sync()
-        for k, v in tb.stage_ms().items():
-            st[k].append(v)
-    tb.destroy()
-    stage = {k: float(np.median(v)) for k, v in st.items()}
+    stage = {k: float(np.mean(v)) for k, v in st.items()}
     # Roofline of the trace stage (the dominant kernel). Algorithmic bytes per world update: 16 B of f32 rgba written
     # per ray; fused with the merge (one launch per cascade), also each cascade c >= 1's 2x2 averages (one float4 per
     # 4 rays) written once and read once by cascade c-1.
@@ -461,6 +460,7 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 3),
         "mrays_per_s": round(rays * value / 1e6, 1),
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage.items()},
+        "stage_timing": "HIP events between each batch's stage graphs, mean over the timed steps",
         "trace_merge_fused": worlds_tuning["fused"] != 0,
         "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},

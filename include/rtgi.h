@@ -298,7 +298,8 @@ int rtgi_batch_set_graph(rtgi_batch* b, int32_t on);
  * functions wait for it. */
 int rtgi_batch_enqueue(rtgi_batch* b, void* stream);
 int rtgi_batch_sync(rtgi_batch* b);
-/* Per-stage HIP-event timing of direct (not graph) enqueues; ms as for rtgi_world_stage_ms. */
+/* Per-stage HIP-event timing of enqueues, direct or graph replays (with timing on, a graph batch replays one graph
+ * per stage with the events recorded between them); ms as for rtgi_world_stage_ms, of the last enqueue. */
 int rtgi_batch_set_timing(rtgi_batch* b, int32_t on);
 int rtgi_batch_stage_ms(rtgi_batch* b, float ms[4]);
 int rtgi_batch_destroy(rtgi_batch* b);

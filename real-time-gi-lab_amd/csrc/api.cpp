@@ -1012,8 +1012,9 @@ struct rtgi_batch {
   BatchPlan plan{};
   std::vector<uint64_t> sig;  // sizes the plan was made for
   bool use_graph = false, graph_valid = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // One graph of the whole update; with timing on, one per stage, launched with the stage events between them.
+  hipGraph_t graph[4] = {};
+  hipGraphExec_t exec[4] = {};
   hipEvent_t done = nullptr;
   bool timing = false;
   hipEvent_t ev[5] = {};
@@ -1047,35 +1048,45 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   p.mark_lds = all_lds ? lds : 0;
 }
 
-int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
+int plan_launch_stage(const BatchPlan& p, hipStream_t s, int stage) {
   int e = 0;
-  if (ev) HIPCHK(hipEventRecord(ev[0], s));
-  e = launch_vox_build_batch(p, s);
-  if (ev) HIPCHK(hipEventRecord(ev[1], s));
-  if (!e) e = launch_sun_batch(p, s);
-  if (!e) e = launch_bounce_batch(p, s);
-  if (!e) e = launch_relocate_batch(p, s);
-  if (p.fused) {
-    for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    if (ev) HIPCHK(hipEventRecord(ev[3], s));
+  if (stage == 0) {
+    e = launch_vox_build_batch(p, s);
+  } else if (stage == 1) {
+    e = launch_sun_batch(p, s);
+    if (!e) e = launch_bounce_batch(p, s);
+    if (!e) e = launch_relocate_batch(p, s);
+    if (p.fused)
+      for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_trace_merge_batch(p, st, s);
+    else if (!e)
+      e = launch_trace_batch(p, s);
+  } else if (stage == 2) {
+    if (!p.fused)
+      for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(p, st, s);
   } else {
-    if (!e) e = launch_trace_batch(p, s);
-    if (ev) HIPCHK(hipEventRecord(ev[2], s));
-    for (int st = 0; st < RTGI_MAX_CASCADES && !e; st++) e = launch_merge_batch(p, st, s);
-    if (ev) HIPCHK(hipEventRecord(ev[3], s));
+    e = launch_sh_batch(p, s);
   }
-  if (!e) e = launch_sh_batch(p, s);
-  if (ev) HIPCHK(hipEventRecord(ev[4], s));
   if (e) return rtgi_fail(RTGI_E_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
+  return RTGI_OK;
+}
+
+int plan_launch(const BatchPlan& p, hipStream_t s, hipEvent_t* ev) {
+  for (int st = 0; st < 4; st++) {
+    if (ev) HIPCHK(hipEventRecord(ev[st], s));
+    int e = plan_launch_stage(p, s, st);
+    if (e) return e;
+  }
+  if (ev) HIPCHK(hipEventRecord(ev[4], s));
   return RTGI_OK;
 }
 
 namespace {
 
 void batch_free(rtgi_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (int g = 0; g < 4; g++) {
+    if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
+    if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
+  }
   for (int i = 0; i < 2; i++) {
     if (b->h_desc[i]) (void)hipHostFree(b->h_desc[i]);
     if (b->copied[i]) (void)hipEventDestroy(b->copied[i]);
@@ -1151,6 +1162,7 @@ extern "C" int rtgi_batch_set_graph(rtgi_batch* b, int32_t on) {
 
 extern "C" int rtgi_batch_set_timing(rtgi_batch* b, int32_t on) {
   if (!b) return rtgi_fail(RTGI_E_INVALID, "null batch");
+  if (b->timing != (on != 0)) b->graph_valid = false;  // timed replays use one graph per stage
   b->timing = on != 0;
   return RTGI_OK;
 }
@@ -1187,20 +1199,32 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   HIPCHK(hipEventRecord(b->copied[b->cur], s));
   b->cur ^= 1;
   if (b->use_graph) {
+    const int ng = b->timing ? 4 : 1;
     if (!b->graph_valid) {
-      if (b->exec) (void)hipGraphExecDestroy(b->exec);
-      if (b->graph) (void)hipGraphDestroy(b->graph);
-      b->exec = nullptr;
-      b->graph = nullptr;
-      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int e = plan_launch(b->plan, s, nullptr);
-      hipError_t ce = hipStreamEndCapture(s, &b->graph);
-      if (e) return e;
-      if (ce != hipSuccess) return rtgi_fail(RTGI_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
-      HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      for (int g = 0; g < 4; g++) {
+        if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
+        if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
+        b->exec[g] = nullptr;
+        b->graph[g] = nullptr;
+      }
+      for (int g = 0; g < ng; g++) {
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int e = ng == 1 ? plan_launch(b->plan, s, nullptr) : plan_launch_stage(b->plan, s, g);
+        hipError_t ce = hipStreamEndCapture(s, &b->graph[g]);
+        if (e) return e;
+        if (ce != hipSuccess) return rtgi_fail(RTGI_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+        size_t nodes = 0;
+        HIPCHK(hipGraphGetNodes(b->graph[g], nullptr, &nodes));
+        if (nodes) HIPCHK(hipGraphInstantiate(&b->exec[g], b->graph[g], nullptr, nullptr, 0));
+      }
       b->graph_valid = true;
     }
-    HIPCHK(hipGraphLaunch(b->exec, s));
+    // Stage events are recorded between the stage graphs (events recorded inside a graph carry no timestamps).
+    for (int g = 0; g < ng; g++) {
+      if (b->timing) HIPCHK(hipEventRecord(b->ev[g], s));
+      if (b->exec[g]) HIPCHK(hipGraphLaunch(b->exec[g], s));
+    }
+    if (b->timing) HIPCHK(hipEventRecord(b->ev[4], s));
   } else {
     int e = plan_launch(b->plan, s, b->timing ? b->ev : nullptr);
     if (e) return e;
@@ -1221,7 +1245,7 @@ extern "C" int rtgi_batch_sync(rtgi_batch* b) {
 
 extern "C" int rtgi_batch_stage_ms(rtgi_batch* b, float ms[4]) {
   if (!b || !ms) return rtgi_fail(RTGI_E_INVALID, "null argument");
-  if (!b->timing || b->use_graph) return rtgi_fail(RTGI_E_STATE, "timing needs direct (non-graph) enqueues");
+  if (!b->timing) return rtgi_fail(RTGI_E_STATE, "timing is off (rtgi_batch_set_timing)");
   HIPCHK(hipEventSynchronize(b->ev[4]));
   for (int i = 0; i < 4; i++) HIPCHK(hipEventElapsedTime(&ms[i], b->ev[i], b->ev[i + 1]));
   return RTGI_OK;

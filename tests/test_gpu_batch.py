@@ -120,6 +120,30 @@ def test_batch_graph_replays(require_gpu):
         w.destroy()
 
 
+def test_graph_stage_timing_keeps_results(require_gpu):
+    """Per-stage HIP events captured into the graph: every replay reports positive stage times, and the results stay
+    the solo ones; switching the timing off re-captures a graph without the records."""
+    worlds = make_worlds(seed=2)
+    say(f"case: batch of {len(worlds)} uneven worlds, hipGraph replays with captured stage timing")
+    solo = solo_results(worlds)
+    with rtgi.Batch(worlds, graph=True) as b:
+        b.set_timing(True)
+        for rep in range(3):
+            b.encode()
+            b.sync()
+            ms = b.stage_ms()
+            assert ms["vox"] > 0 and ms["trace"] > 0 and ms["sh"] > 0, ms
+            for i, w in enumerate(worlds):
+                assert_same(snapshot(w), solo[i], f"timed replay {rep} world {i}")
+        b.set_timing(False)
+        b.encode()
+        b.sync()
+        for i, w in enumerate(worlds):
+            assert_same(snapshot(w), solo[i], f"untimed replay world {i}")
+    for w in worlds:
+        w.destroy()
+
+
 def test_batch_of_cities_matches_solo(require_gpu):
     say("case: batch of 3 city worlds (500k triangles each) vs solo, fused hipGraph")
     worlds = []
