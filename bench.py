@@ -403,20 +403,26 @@ def main():
     elapsed = time.perf_counter() - t_start
     value = a.steps * len(worlds) / elapsed
     stage = {k: float(np.mean(v)) for k, v in st.items()}
-    # Roofline of the trace stage (the dominant kernel). Algorithmic bytes per world update: 16 B of f32 rgba written
-    # per ray; fused with the merge (one launch per cascade), also each cascade c >= 1's 2x2 averages (one float4 per
-    # 4 rays) written once and read once by cascade c-1.
+    # Roofline of the dominant kernel, the fused trace+merge: one launch is one cascade step of the batch (5 launches
+    # per update). Algorithmic bytes per world update: 16 B of f32 rgba written per ray; fused with the merge, also
+    # each cascade c >= 1's 2x2 averages (one float4 per 4 rays) written once and read once by cascade c-1. Achieved
+    # = algorithmic bytes per launch / mean launch duration (the trace stage's HIP-event time over its 5 launches).
     per_c = lay["rays"]
-    alg_world = 16 * rays + (2 * 16 * sum(r // 4 for r in per_c[1:]) if worlds_tuning["fused"] != 0 else 0)
-    alg_bytes = alg_world * len(groups[0])
-    achieved = alg_bytes / (stage["trace"] * 1e-3) / 1e9
-    # HBM traffic of the same stage from the committed FETCH_SIZE / WRITE_SIZE profile of the fused city trace
-    # (copy-calibrated), scaled to this batch width; null for other configurations.
-    traffic = None
+    fused = worlds_tuning["fused"] != 0
+    alg_world = 16 * rays + (2 * 16 * sum(r // 4 for r in per_c[1:]) if fused else 0)
+    launches = len(per_c) if fused else 1
+    alg_launch = alg_world * len(groups[0]) / launches
+    launch_s = stage["trace"] * 1e-3 / launches
+    achieved = alg_launch / launch_s / 1e9
+    # HBM traffic per launch from the committed FETCH_SIZE / WRITE_SIZE profile of the fused city trace (copy-
+    # calibrated), scaled to this batch width; null for other configurations. VALU issue from the same profile.
+    traffic, valu_busy = None, None
     tf = os.path.join(ROOT, "profiles", "trace_traffic_city.json")
-    if a.config == "city" and worlds_tuning["fused"] != 0 and os.path.exists(tf):
+    if a.config == "city" and fused and os.path.exists(tf):
         with open(tf) as fh:
-            traffic = int(json.load(fh)["hbm_bytes_per_world_update"]) * len(groups[0])
+            prof = json.load(fh)
+        traffic = int(prof["hbm_bytes_per_world_update"] * len(groups[0]) / launches)
+        valu_busy = prof.get("valu_issue_busy")
     cnt = worlds[0].read_counters()
 
     for b in batches:
@@ -464,11 +470,13 @@ def main():
         "trace_merge_fused": worlds_tuning["fused"] != 0,
         "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
-        "roofline": {"kernel": ("k_trace_merge_b (5 batched launches, one per cascade)" if worlds_tuning["fused"] != 0
-                                else "k_trace_b (batched launch)"),
-                     "bytes_per_world_update": int(alg_world), "bound": "hbm", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "algorithmic_bytes": int(alg_bytes), "traffic": traffic},
+        "roofline": {"kernel": ("k_trace_merge_b (one launch per cascade step of the batch)" if fused
+                                else "k_trace_b (one batched launch)"),
+                     "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": int(alg_launch), "launch_ms": round(launch_s * 1e3, 4),
+                     "bytes_per_world_update": int(alg_world),
+                     "valu_issue_busy_profiled": valu_busy},
         "cpu_baseline": cpu,
         "full_default": full,
         "batched_mix": mixed,
