@@ -227,6 +227,23 @@ int rtgi_world_trace_rays(rtgi_world* w, int32_t c, const float* o, const float*
  * (rc.ts:1066-1068). Synchronises. */
 int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, float* nrm, uint32_t* parents);
 
+/* Dynamic coverage layer (createVoxWorld's firstDynamicTri, src/core/voxel.ts:746-747, 822, and VoxParams.dynBricks;
+ * createDynLayer, src/core/dynlayer.ts:650-761). Triangles [first_dynamic_tri, ntris) of the uploaded scene are then
+ * not voxelised into the bricks: every update splats them into a layer of dyn_bricks bricks of coverage and mean
+ * radiance (mark, parents, alloc, clear, splat, resolve), and the cascade trace reads it in front of each ray's
+ * static hit (traceDyn, dynlayer.ts:118-167; traceWorld, rc.ts:224-231). Samples are shaded with the frame's
+ * sun_vis, untextured. first_dynamic_tri >= ntris, the default: no layer (every triangle in the bricks). Allocates;
+ * call between updates. */
+int rtgi_world_set_dynamic(rtgi_world* w, uint32_t first_dynamic_tri, int32_t dyn_bricks);
+/* The layer's counters after the last update (DynLayer.readCounters, dynlayer.ts:743-754): allocated bricks, bricks
+ * dropped for want of the pool, triangles outside every clipmap. Synchronises. */
+int rtgi_world_read_dyn_counters(rtgi_world* w, uint32_t* bricks, uint32_t* brick_overflow, uint32_t* outside);
+/* The raw layer after the last update: hdr [64] header words, ptr [levels][G^3] slots, meta [dyn_bricks] pointer
+ * index of each slot, mask [dyn_bricks][16] voxels with values, data [dyn_bricks][512][13] words (fixed point 2^-16 at
+ * level 0, f32 bits above). Any pointer may be NULL. Synchronises. */
+int rtgi_world_read_dynamic(rtgi_world* w, uint32_t* hdr, uint32_t* ptr, uint32_t* meta, uint32_t* mask,
+                            uint32_t* data);
+
 /* The bounce volume the last update wrote (RTGI_RC_BOUNCE): texels [32 * 16 * 32 probes, x fastest][6 axes +x, -x,
  * +y, -y, +z, -z][4 halves: rgb irradiance, reach along the axis (0: inactive probe)] and the probe coordinates of
  * probe 0 (origin: probe i sits at (origin + (x, y, z)) * 1.2 m). Replaces reading the bounce texture (bounce.ts:
@@ -391,6 +408,8 @@ typedef struct rtgi_scene {
   double bounds_lo[3], bounds_hi[3];
   double eye[3];     /* a default eye for the scene */
   float sun_dir[3];
+  uint32_t first_dynamic_tri;  /* the moving parts are triangles [first_dynamic_tri, ntris) (atrium: the 16 boxes;
+                                * city: ntris, every building moving, rebuilt with the rest) */
 } rtgi_scene;
 
 int rtgi_scene_generate(const rtgi_scene_params* p, rtgi_scene** out);

@@ -39,7 +39,13 @@ class OrcRc(C.Structure):
                 ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
                 ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p),
-                ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3)]
+                ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3), ("dyn", C.c_void_p)]
+
+
+class OrcDyn(C.Structure):
+    _fields_ = [("cap", C.c_int32), ("first_tri", C.c_uint32), ("end_tri", C.c_uint32), ("hdr", C.c_void_p),
+                ("ptr", C.c_void_p), ("bits", C.c_void_p), ("meta", C.c_void_p), ("mask", C.c_void_p),
+                ("data", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -57,6 +63,8 @@ _SIG = {
     "orc_relocate": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P]),
     "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
     "orc_bn_origin": (None, [_P, _P]),
+    "orc_dyn_init": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcDyn)]),
+    "orc_dyn_build": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcRc), _P, _P, _P, _P, C.POINTER(OrcDyn)]),
     "orc_bounce_volume": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, C.c_uint32, _P, _P]),
     "orc_bounce_inject": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, _P]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
@@ -75,6 +83,7 @@ _SIG = {
     "orc_oct_enc": (None, [_P, _P]),
     "orc_f2h": (C.c_uint16, [C.c_float]),
     "orc_h2f": (C.c_float, [C.c_uint16]),
+    "orc_log2": (C.c_float, [C.c_float]),
 }
 _lib = None
 
@@ -102,6 +111,32 @@ def _p(a):
 
 def f32(x):
     return np.ascontiguousarray(x, np.float32)
+
+
+class DynState:
+    """The dynamic coverage layer's state, kept across updates (as the device keeps it): triangles from first_tri on
+    of each scene built with it are dynamic."""
+
+    def __init__(self, vox, cap, first_tri):
+        per = vox.G ** 3
+        self.cap, self.first_tri = int(cap), int(first_tri)
+        self.hdr = np.zeros(64, np.uint32)
+        self.ptr = np.zeros((vox.levels, per), np.uint32)
+        self.bits = np.zeros(2 * vox.levels * per // 32, np.uint32)
+        self.meta = np.zeros(self.cap, np.uint32)
+        self.mask = np.zeros((self.cap, 16), np.uint32)
+        self.data = np.zeros((self.cap, 512, 13), np.uint32)
+        self._vox = vox
+        vp = OrcVox()
+        vp.levels, vp.G = vox.levels, vox.G
+        assert lib().orc_dyn_init(C.byref(vp), C.byref(self.struct(0))) == 0
+
+    def struct(self, end_tri):
+        d = OrcDyn()
+        d.cap, d.first_tri, d.end_tri = self.cap, self.first_tri, int(end_tri)
+        for k in ("hdr", "ptr", "bits", "meta", "mask", "data"):
+            setattr(d, k, getattr(self, k).ctypes.data)
+        return d
 
 
 BN_DIMS = (32, 16, 32)   # bounce volume probes per axis (x, y, z)
@@ -210,15 +245,24 @@ class Oracle:
     def cascade_rays(self):
         return [int(lib().orc_rays(C.byref(self.rc), c)) for c in range(self.rc.cascades)]
 
-    def build(self, scene):
+    def build(self, scene, dyn=None):
+        """The clipmap of the scene; with dyn (a DynState), triangles from dyn.first_tri on are dynamic: they are
+        splatted into the layer (this update of it) instead of the bricks, and the trace reads it."""
         self.buf = np.zeros(self.words, np.uint32)
         self.rep = np.zeros(self.vox.maxBricks * 512, np.uint32)
         self._scene = (f32(scene.pos), f32(scene.nrm), np.ascontiguousarray(scene.tri, np.uint32),
                        f32(scene.materials))
         pos, nrm, tri, mat = self._scene
-        rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), tri.shape[0], _p(mat), _p(self.buf),
+        nstatic = tri.shape[0] if dyn is None else min(tri.shape[0], dyn.first_tri)
+        rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), nstatic, _p(mat), _p(self.buf),
                              _p(self.rep))
         assert rc == 0
+        if dyn is not None:
+            self._dyn_struct = dyn.struct(tri.shape[0])
+            assert lib().orc_dyn_build(C.byref(self.vp), C.byref(self.rc), _p(pos), _p(nrm), _p(tri), _p(mat),
+                                       C.byref(self._dyn_struct)) == 0
+            self.rc.dyn = C.addressof(self._dyn_struct)
+            self.dyn = dyn
         if self._sun_trace:
             self.sunvis = np.zeros(self.vox.maxVoxels, np.float32)
             assert lib().orc_sun(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(self.sunvis)) == 0

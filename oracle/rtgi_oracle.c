@@ -932,6 +932,396 @@ static int cascadeOf(const orc_rc* rc, uint64_t ray, uint64_t* local) {
 /* trace (rc.ts:600-622) -> traceWorld (rc.ts:216-245) with the probe at its grid point, normal 0. */
 /* One ray of cascade c from o along d (traceWorld, rc.ts:216-222): interval [b0, b1] s0, level voxLevelFor at the
  * interval's end, start voxOffset voxels along the probe normal n (relocated probes only; 0 on the grid). */
+/* ---- Dynamic coverage layer (dynlayer.ts): build and read ---- */
+#define DYN_W 13u
+#define DYN_EMPTY_ 0xffffffffu
+#define DYN_DROPPED_ 0xfffffffdu
+#define DYN_FIX_ 65536.0f
+
+static uint32_t dynNW(const orc_vox* p) { return (uint32_t)(p->levels * p->G * p->G * p->G) / 32u; }
+
+int orc_dyn_init(const orc_vox* p, orc_dyn* d) {
+  size_t per = (size_t)p->G * p->G * p->G, NLper = per * (size_t)p->levels;
+  memset(d->hdr, 0, 64 * 4);
+  d->hdr[32] = d->hdr[33] = d->hdr[34] = 0xffffffffu;
+  for (size_t i = 0; i < NLper; i++) d->ptr[i] = DYN_EMPTY_;
+  memset(d->bits, 0, 2 * (NLper / 32) * 4);
+  memset(d->meta, 0, (size_t)d->cap * 4);
+  memset(d->mask, 0, (size_t)d->cap * 16 * 4);
+  memset(d->data, 0, (size_t)d->cap * 512 * DYN_W * 4);
+  return 0;
+}
+
+static v3 lo3(const orc_vox* p, uint32_t L) { return vscale(vscale(vf(org(p, L)), voxSize(p, L)), 8.0f); }
+/* splatLevel (dynlayer.ts:240-247) */
+static int splatLevel(const orc_vox* p, v3 mn, v3 mx) {
+  for (int L = 0; L < p->levels; L++) {
+    float v = voxSize(p, (uint32_t)L);
+    v3 lo = lo3(p, (uint32_t)L);
+    float span = (float)p->G * v * 8.0f;
+    v3 hi = vadd(lo, V3(span, span, span));
+    v3 a = vsub(mn, V3(2.0f * v, 2.0f * v, 2.0f * v)), b = vadd(mx, V3(2.0f * v, 2.0f * v, 2.0f * v));
+    if (a.x > lo.x && a.y > lo.y && a.z > lo.z && b.x < hi.x && b.y < hi.y && b.z < hi.z) return L;
+  }
+  return -1;
+}
+static void dynWant(const orc_vox* p, uint32_t* cur, int L, iv3 b) {
+  int G = p->G;
+  if (b.x < 0 || b.y < 0 || b.z < 0 || b.x >= G || b.y >= G || b.z >= G) return;
+  uint32_t per = (uint32_t)(G * G * G);
+  uint32_t g = (uint32_t)(p->levels - 1 - L) * per + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x;
+  cur[g >> 5] |= 1u << (g & 31u);
+}
+static uint32_t fixq(float x) {
+  float r = rintf(x * DYN_FIX_);
+  return (uint32_t)(r < 4294967040.0f ? r : 4294967040.0f);
+}
+static void splatSample(const orc_vox* p, orc_dyn* D, int L, v3 pt, v3 n, float area, v3 rad) {
+  float v = voxSize(p, (uint32_t)L);
+  v3 g = vsub(vdivs(vsub(pt, lo3(p, (uint32_t)L)), v), V3(0.5f, 0.5f, 0.5f));
+  iv3 c0 = vi(vfloor(g));
+  v3 f = vsub(g, vf(c0));
+  float sv = area / (v * v);
+  v3 pn = V3(fmx(n.x, 0.0f), fmx(n.y, 0.0f), fmx(n.z, 0.0f)), nn = V3(fmx(-n.x, 0.0f), fmx(-n.y, 0.0f), fmx(-n.z, 0.0f));
+  int E = p->G * 8, G = p->G;
+  uint32_t per = (uint32_t)(G * G * G);
+  for (int q = 0; q < 8; q++) {
+    iv3 o = I3(q & 1, (q >> 1) & 1, q >> 2);
+    iv3 c = iadd(c0, o);
+    if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= E || c.y >= E || c.z >= E) continue;
+    v3 tw = V3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
+    float w = tw.x * tw.y * tw.z * sv;
+    if (w <= 0.0f) continue;
+    iv3 b = ishr(c, 3);
+    uint32_t ptr = D->ptr[(uint32_t)L * per + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x];
+    if (ptr >= (uint32_t)D->cap) continue;
+    uint32_t vi_ = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+    D->mask[ptr * 16 + (vi_ >> 5)] |= 1u << (vi_ & 31u);
+    uint32_t* dd = D->data + ((size_t)ptr * 512 + vi_) * DYN_W;
+    float x[10] = {w * pn.x, w * nn.x, w * pn.y, w * nn.y, w * pn.z, w * nn.z, w, w * rad.x, w * rad.y, w * rad.z};
+    for (int k = 0; k < 10; k++) dd[k] += fixq(x[k]);
+  }
+}
+
+int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const float* nrm, const uint32_t* tri,
+                  const float* mat, orc_dyn* D) {
+  const int G = p->G, NL = p->levels;
+  const uint32_t per = (uint32_t)(G * G * G), nw = dynNW(p), cap = (uint32_t)D->cap;
+  uint32_t* h = D->hdr;
+  uint32_t* cur = D->bits + (h[10] & 1u) * nw;
+  uint32_t* old = D->bits + (1u - (h[10] & 1u)) * nw;
+  if (D->end_tri <= D->first_tri) return 0;
+  /* mark */
+  for (uint32_t t = D->first_tri; t < D->end_tri; t++) {
+    const uint32_t* q = tri + (size_t)t * 4;
+    v3 a = V3(pos[q[0] * 3], pos[q[0] * 3 + 1], pos[q[0] * 3 + 2]);
+    v3 b = V3(pos[q[1] * 3], pos[q[1] * 3 + 1], pos[q[1] * 3 + 2]);
+    v3 c = V3(pos[q[2] * 3], pos[q[2] * 3 + 1], pos[q[2] * 3 + 2]);
+    v3 mn = vmin(vmin(a, b), c), mx = vmax(vmax(a, b), c);
+    int Ls = splatLevel(p, mn, mx);
+    if (Ls < 0) {
+      h[38]++;
+      continue;
+    }
+    float v = voxSize(p, (uint32_t)Ls), bs = v * 8.0f;
+    v3 lo = lo3(p, (uint32_t)Ls);
+    iv3 g0 = vi(vfloor(vdivs(vsub(vsub(mn, V3(v, v, v)), lo), bs)));
+    iv3 g1 = vi(vfloor(vdivs(vsub(vadd(mx, V3(v, v, v)), lo), bs)));
+    for (int z = g0.z; z <= g1.z; z++)
+      for (int y = g0.y; y <= g1.y; y++)
+        for (int x = g0.x; x <= g1.x; x++) dynWant(p, cur, Ls, I3(x, y, z));
+    float gv = voxSize(p, (uint32_t)(NL - 1));
+    v3 blo = vfloor(vscale(vsub(mn, V3(gv, gv, gv)), 100.0f));
+    v3 bh = vadd(mx, V3(gv, gv, gv));
+    v3 bhi = V3(ceilf(bh.x * 100.0f), ceilf(bh.y * 100.0f), ceilf(bh.z * 100.0f));
+    uint32_t lo_[3] = {(uint32_t)(f2i(blo.x) + 1073741824), (uint32_t)(f2i(blo.y) + 1073741824), (uint32_t)(f2i(blo.z) + 1073741824)};
+    uint32_t hi_[3] = {(uint32_t)(f2i(bhi.x) + 1073741824), (uint32_t)(f2i(bhi.y) + 1073741824), (uint32_t)(f2i(bhi.z) + 1073741824)};
+    for (int k = 0; k < 3; k++) {
+      if (lo_[k] < h[32 + k]) h[32 + k] = lo_[k];
+      if (hi_[k] > h[35 + k]) h[35 + k] = hi_[k];
+    }
+  }
+  /* parents: bit order is coarse first, so walking words from the finest level up visits a word after every bit
+   * that finer bricks add to it; each wanted brick adds all its ancestors anyway. */
+  for (uint32_t w = per / 32u; w < nw; w++) {
+    uint32_t m = cur[w];
+    while (m) {
+      uint32_t bit = (uint32_t)__builtin_ctz(m);
+      m &= m - 1u;
+      uint32_t g = w * 32u + bit, lr = g / per;
+      int L = NL - 1 - (int)lr;
+      uint32_t r = g - lr * per;
+      iv3 b = I3((int)(r % G), (int)((r / G) % G), (int)(r / (G * G)));
+      while (L + 1 < NL) {
+        iv3 o0 = org(p, (uint32_t)L), o1 = org(p, (uint32_t)(L + 1));
+        b = ishr(isub(imuls(iadd(o0, b), 4), imuls(o1, 8)), 3);
+        L++;
+        dynWant(p, cur, L, b);
+      }
+    }
+  }
+  /* alloc + finalize */
+  uint32_t first = 0;
+  for (uint32_t w = 0; w < nw; w++) {
+    uint32_t cw = cur[w], ow = old[w];
+    uint32_t c = (uint32_t)__builtin_popcount(cw);
+    uint32_t excl = first;
+    if ((w * 32u) % per == 0u) h[16u + (uint32_t)NL - 1u - w * 32u / per] = excl;
+    if (w == nw - 1u) h[0] = excl + c;
+    uint32_t m = cw | ow;
+    while (m) {
+      uint32_t bit = (uint32_t)__builtin_ctz(m);
+      m &= m - 1u;
+      uint32_t g = w * 32u + bit, lr = g / per;
+      uint32_t pi = ((uint32_t)NL - 1u - lr) * per + g - lr * per;
+      if ((cw >> bit) & 1u) {
+        uint32_t rank = excl + (uint32_t)__builtin_popcount(cw & ((1u << bit) - 1u));
+        if (rank < cap) {
+          D->ptr[pi] = rank;
+          D->meta[rank] = pi;
+        } else {
+          D->ptr[pi] = DYN_DROPPED_;
+        }
+      } else {
+        D->ptr[pi] = DYN_EMPTY_;
+      }
+    }
+    old[w] = 0u;
+    first += c;
+  }
+  {
+    uint32_t total = h[0], al = total < cap ? total : cap;
+    h[8] = al;
+    h[1] = total - al;
+    for (int L = 0; L < NL; L++) {
+      uint32_t e = L > 0 ? h[16 + L - 1] : total;
+      uint32_t s0 = h[16 + L] < cap ? h[16 + L] : cap;
+      h[16 + L] = s0;
+      h[24 + L] = (e < cap ? e : cap) - s0;
+    }
+    for (int a = 0; a < 3; a++) {
+      h[2 + a] = h[32 + a];
+      h[32 + a] = 0xffffffffu;
+      h[5 + a] = h[35 + a];
+      h[35 + a] = 0u;
+    }
+    h[9] = h[38];
+    h[38] = 0u;
+    h[10] = h[10] + 1u;
+  }
+  /* clear */
+  memset(D->data, 0, (size_t)h[8] * 512 * DYN_W * 4);
+  memset(D->mask, 0, (size_t)h[8] * 16 * 4);
+  /* splat */
+  v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]), se = V3(rc->sun_e[0], rc->sun_e[1], rc->sun_e[2]);
+  for (uint32_t t = D->first_tri; t < D->end_tri; t++) {
+    const uint32_t* q = tri + (size_t)t * 4;
+    v3 a = V3(pos[q[0] * 3], pos[q[0] * 3 + 1], pos[q[0] * 3 + 2]);
+    v3 b = V3(pos[q[1] * 3], pos[q[1] * 3 + 1], pos[q[1] * 3 + 2]);
+    v3 c = V3(pos[q[2] * 3], pos[q[2] * 3 + 1], pos[q[2] * 3 + 2]);
+    int L = splatLevel(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c));
+    if (L < 0) continue;
+    v3 cr = vcross(vsub(b, a), vsub(c, a));
+    float area = 0.5f * sqrtf(vdot(cr, cr));
+    if (area <= 0.0f) continue;
+    v3 e0 = vsub(b, a), e1 = vsub(c, b), e2 = vsub(a, c);
+    float e = fmx(fmx(sqrtf(vdot(e0, e0)), sqrtf(vdot(e1, e1))), sqrtf(vdot(e2, e2)));
+    float mf = ceilf(e / voxSize(p, (uint32_t)L));
+    mf = mf > 1.0f ? mf : 1.0f;
+    mf = mf < 16.0f ? mf : 16.0f;
+    uint32_t m = (uint32_t)mf;
+    float fm = (float)m, sa = area / (fm * fm);
+    const float* M = mat + (size_t)q[3] * 4;
+    v3 alb = V3(1.0f * M[0], 1.0f * M[1], 1.0f * M[2]);
+    v3 n0 = V3(nrm[q[0] * 3], nrm[q[0] * 3 + 1], nrm[q[0] * 3 + 2]);
+    v3 n1 = V3(nrm[q[1] * 3], nrm[q[1] * 3 + 1], nrm[q[1] * 3 + 2]);
+    v3 n2 = V3(nrm[q[2] * 3], nrm[q[2] * 3 + 1], nrm[q[2] * 3 + 2]);
+    for (uint32_t u = 0; u < m; u++)
+      for (uint32_t v_ = 0; v_ + u < m; v_++)
+        for (uint32_t flip = 0; flip < 2; flip++) {
+          if (flip == 1 && u + v_ + 1 >= m) continue;
+          float off = flip == 1 ? 2.0f / 3.0f : 1.0f / 3.0f;
+          float wb = ((float)u + off) / fm, wc = ((float)v_ + off) / fm, wa = 1.0f - wb - wc;
+          v3 pt = vadd(vadd(vscale(a, wa), vscale(b, wb)), vscale(c, wc));
+          v3 nv = vadd(vadd(vscale(n0, wa), vscale(n1, wb)), vscale(n2, wc));
+          nv = vnormalize(vdot(nv, nv) > 1e-12f ? nv : cr);
+          float ndl = fmx(vdot(nv, sd), 0.0f);
+          v3 rad = V3(0, 0, 0);
+          if (ndl > 0.0f) rad = vscale(vscale(vmul(vdivs(alb, PI_F), se), ndl), rc->sun_vis);
+          splatSample(p, D, L, pt, nv, sa, rad);
+        }
+  }
+  /* resolve, fine to coarse */
+  for (int L = 1; L < NL; L++) {
+    uint32_t cnt = h[24 + L], s0 = h[16 + L];
+    for (uint32_t i = 0; i < cnt * 512u; i++) {
+      uint32_t slot = s0 + (i >> 9), bit = i & 511u;
+      uint32_t pi = D->meta[slot], r = pi % per;
+      iv3 c = iadd(imuls(I3((int)(r % G), (int)((r / G) % G), (int)(r / (G * G))), 8),
+                   I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+      uint32_t own = (D->mask[slot * 16 + (bit >> 5)] >> (bit & 31u)) & 1u;
+      iv3 gc = isub(imuls(iadd(imuls(org(p, (uint32_t)L), 8), c), 2), imuls(org(p, (uint32_t)(L - 1)), 8));
+      uint32_t cptr = cap;
+      int E = G * 8;
+      if (gc.x >= 0 && gc.y >= 0 && gc.z >= 0 && gc.x < E && gc.y < E && gc.z < E) {
+        iv3 b = ishr(gc, 3);
+        cptr = D->ptr[(uint32_t)(L - 1) * per + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x];
+      }
+      uint32_t kids = 0;
+      if (cptr < cap) {
+        for (int q = 0; q < 8; q++) {
+          iv3 l = iadd(I3(gc.x & 7, gc.y & 7, gc.z & 7), I3(q & 1, (q >> 1) & 1, q >> 2));
+          uint32_t vv = (uint32_t)(l.x + 8 * l.y + 64 * l.z);
+          if ((D->mask[cptr * 16 + (vv >> 5)] >> (vv & 31u)) & 1u) kids |= 1u << q;
+        }
+      }
+      if (own == 0u && kids == 0u) continue;
+      uint32_t* dd = D->data + ((size_t)slot * 512 + bit) * DYN_W;
+      float acc[DYN_W];
+      for (uint32_t k = 0; k < DYN_W; k++) acc[k] = (float)dd[k] / DYN_FIX_;
+      for (int q = 0; q < 8; q++) {
+        if (!((kids >> q) & 1u)) continue;
+        iv3 l = iadd(I3(gc.x & 7, gc.y & 7, gc.z & 7), I3(q & 1, (q >> 1) & 1, q >> 2));
+        const uint32_t* cb = D->data + ((size_t)cptr * 512 + (uint32_t)(l.x + 8 * l.y + 64 * l.z)) * DYN_W;
+        for (uint32_t k = 0; k < DYN_W; k++) {
+          uint32_t u = cb[k];
+          float x;
+          if (L == 1) x = (float)u / DYN_FIX_;
+          else memcpy(&x, &u, 4);
+          acc[k] += 0.25f * x;
+        }
+      }
+      for (uint32_t k = 0; k < DYN_W; k++) memcpy(&dd[k], &acc[k], 4);
+      if (own == 0u) D->mask[slot * 16 + (bit >> 5)] |= 1u << (bit & 31u);
+    }
+  }
+  return 0;
+}
+
+/* log2 of a positive normal float for the level blend of traceDyn (dynlayer.ts:140, WGSL log2): x = m 2^e with m in
+ * [sqrt(1/2), sqrt(2)], ln m = 2 atanh((m - 1) / (m + 1)) by its series to s^9, in plain float operations in a fixed
+ * order so that the device's restatement gives the same bits. */
+float orc_log2(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  int e = (int)(u >> 23) - 127;
+  uint32_t mu = (u & 0x007fffffu) | 0x3f800000u;
+  float m;
+  memcpy(&m, &mu, 4);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
+  float p = 1.0f / 9.0f;
+  p = p * s2 + 1.0f / 7.0f;
+  p = p * s2 + 1.0f / 5.0f;
+  p = p * s2 + 1.0f / 3.0f;
+  p = p * s2 + 1.0f;
+  return (float)e + ((2.0f * s) * p) * 1.44269504f;
+}
+
+/* dynFind / dynWord / dynRead (dynlayer.ts:81-116) */
+static uint32_t dynFind(const orc_vox* p, const orc_dyn* D, uint32_t L, v3 pt, uint32_t* lev) {
+  uint32_t l = L;
+  int G = p->G;
+  uint32_t per = (uint32_t)(G * G * G);
+  for (;;) {
+    float v = voxSize(p, l);
+    iv3 c = vi(vfloor(vdivs(vsub(pt, vscale(vscale(vf(org(p, l)), v), 8.0f)), v)));
+    *lev = l;
+    if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= G * 8 || c.y >= G * 8 || c.z >= G * 8) return DYN_EMPTY_;
+    iv3 b = ishr(c, 3);
+    uint32_t ptr = D->ptr[l * per + ((uint32_t)b.z * G + (uint32_t)b.y) * G + (uint32_t)b.x];
+    if (ptr == DYN_DROPPED_ && l + 1u < (uint32_t)p->levels) {
+      l++;
+      continue;
+    }
+    if (ptr >= (uint32_t)D->cap) return DYN_EMPTY_;
+    return (ptr * 512u + (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7))) * DYN_W;
+  }
+}
+static float dynWord(const orc_dyn* D, uint32_t base, uint32_t k, uint32_t L) {
+  uint32_t u = D->data[base + k];
+  if (L == 0u) return (float)u / DYN_FIX_;
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+static void dynRead(const orc_dyn* D, uint32_t base, uint32_t L, v3 d, float out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0f;
+  if (base == DYN_EMPTY_) return;
+  float a = dynWord(D, base, 6, L);
+  if (a <= 0.0f) return;
+  uint32_t sx = d.x > 0.0f ? 1u : 0u, sy = d.y > 0.0f ? 1u : 0u, sz = d.z > 0.0f ? 1u : 0u;
+  float front = fabsf(d.x) * dynWord(D, base, sx, L) + fabsf(d.y) * dynWord(D, base, 2 + sy, L) +
+                fabsf(d.z) * dynWord(D, base, 4 + sz, L);
+  out[0] = dynWord(D, base, 7, L) / a;
+  out[1] = dynWord(D, base, 8, L) / a;
+  out[2] = dynWord(D, base, 9, L) / a;
+  out[3] = fmn(front, 1.0f);
+}
+/* traceDyn (dynlayer.ts:118-160) */
+static void traceDyn(const orc_vox* p, const orc_dyn* D, v3 o, v3 d, float t0, float t1, float kc, float out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0f;
+  if (D->hdr[0] == 0u || t1 <= t0) return;
+  v3 bmin = vscale(vsub(V3((float)D->hdr[2], (float)D->hdr[3], (float)D->hdr[4]), V3(1073741824.0f, 1073741824.0f, 1073741824.0f)), 0.01f);
+  v3 bmax = vscale(vsub(V3((float)D->hdr[5], (float)D->hdr[6], (float)D->hdr[7]), V3(1073741824.0f, 1073741824.0f, 1073741824.0f)), 0.01f);
+  v3 dd = V3(fabsf(d.x) < 1e-9f ? 1e-9f : d.x, fabsf(d.y) < 1e-9f ? 1e-9f : d.y, fabsf(d.z) < 1e-9f ? 1e-9f : d.z);
+  v3 ta = V3((bmin.x - o.x) / dd.x, (bmin.y - o.y) / dd.y, (bmin.z - o.z) / dd.z);
+  v3 tb = V3((bmax.x - o.x) / dd.x, (bmax.y - o.y) / dd.y, (bmax.z - o.z) / dd.z);
+  v3 tn = vmin(ta, tb), tf = vmax(ta, tb);
+  float ba = fmx(fmx(fmx(tn.x, tn.y), tn.z), t0);
+  float bb = fmn(fmn(fmn(tf.x, tf.y), tf.z), t1);
+  if (ba >= bb) return;
+  float v0 = p->v0;
+  int nl = p->levels;
+  v3 rad = V3(0, 0, 0);
+  float cov = 0.0f, t = t0;
+  for (int i = 0; i < 128; i++) {
+    if (t >= bb || cov >= 1.0f) break;
+    float w = fmx(kc * t, v0);
+    float te = fmn(t + 0.5f * w, t1);
+    if (te > ba) {
+      float len = te - t;
+      v3 pt = vadd(o, vscale(d, 0.5f * (t + te)));
+      float lf = fmn(fmx(orc_log2(w / v0), 0.0f), (float)(nl - 1));
+      uint32_t L0 = (uint32_t)floorf(lf);
+      float f = lf - (float)L0;
+      uint32_t Lw = voxLevelFor(p, L0, pt);
+      if (Lw != L0) {
+        L0 = Lw;
+        f = 0.0f;
+      }
+      uint32_t la = 0, lb = 0;
+      float sa[4], sb[4];
+      /* the level comes out of dynFind, so it is called before dynRead reads it */
+      uint32_t fa = dynFind(p, D, L0, pt, &la);
+      dynRead(D, fa, la, d, sa);
+      float aa = sa[3] * len / voxSize(p, la);
+      v3 ra = vscale(V3(sa[0], sa[1], sa[2]), aa);
+      if (f > 0.0f && (int)L0 + 1 < nl) {
+        uint32_t fb = dynFind(p, D, L0 + 1, pt, &lb);
+        dynRead(D, fb, lb, d, sb);
+        float ab = sb[3] * len / voxSize(p, lb);
+        v3 rb = vscale(V3(sb[0], sb[1], sb[2]), ab);
+        ra = vadd(vscale(ra, 1.0f - f), vscale(rb, f));
+        aa = aa * (1.0f - f) + ab * f;
+      }
+      if (aa > 0.0f) {
+        float ac = fmn(aa, 1.0f - cov);
+        rad = vadd(rad, vscale(ra, ac / aa));
+        cov += ac;
+      }
+    }
+    t = te;
+  }
+  out[0] = rad.x;
+  out[1] = rad.y;
+  out[2] = rad.z;
+  out[3] = cov;
+}
+
 static void traceFromN(const View* W, const orc_rc* rc, int c, v3 o, v3 n, float out[4], uint32_t* hid,
                        uint32_t* steps, v3 d);
 static void traceFrom(const View* W, const orc_rc* rc, int c, v3 o, v3 d, float out[4], uint32_t* hid,
@@ -944,7 +1334,19 @@ static void traceFromN(const View* W, const orc_rc* rc, int c, v3 o, v3 n, float
   uint32_t L = voxLevelFor(W->p, (uint32_t)rc->vox_level[c], vadd(o, vscale(d, t1)));
   float v = voxSize(W->p, L);
   v3 ot = rc->reloc ? vadd(o, vscale(n, rc->vox_offset * v)) : o;
-  traceVox(W, rc, ot, d, t0, t1, rc->vox_skip * v, L, out, hid, steps);
+  float tin = 0.0f;
+  traceVoxT(W, rc, ot, d, t0, t1, rc->vox_skip * v, L, out, hid, steps, &tin);
+  if (rc->dyn) {
+    /* the dynamic layer in front of the static hit (traceWorld, rc.ts:224-231) */
+    float tS = out[3] < 0.5f ? vdot(vsub(vadd(ot, vscale(d, tin)), o), d) : t1;
+    float kc = 3.5449077f / (float)resOf(rc, c);
+    float dl[4];
+    traceDyn(W->p, rc->dyn, o, d, fmx(t0, W->p->v0), fmn(t1, tS), kc, dl);
+    if (dl[3] > 0.0f) {
+      for (int k = 0; k < 3; k++) out[k] = dl[k] + (1.0f - dl[3]) * out[k];
+      out[3] = (1.0f - dl[3]) * out[3];
+    }
+  }
 }
 
 static void traceRay(const View* W, const orc_rc* rc, uint64_t ray, float out[4], uint32_t* hid, uint32_t* steps) {

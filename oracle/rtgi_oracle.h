@@ -24,6 +24,14 @@ typedef struct orc_vox {
   int32_t origin[8][3];
 } orc_vox;
 
+/* Dynamic coverage layer state (dynlayer.ts), kept across updates: hdr [64], ptr [levels G^3], bits [2][levels G^3 /
+ * 32], meta [cap], mask [cap][16], data [cap][512][13]; triangles [first_tri, end_tri) are dynamic. */
+typedef struct orc_dyn {
+  int32_t cap;
+  uint32_t first_tri, end_tri;
+  uint32_t *hdr, *ptr, *bits, *meta, *mask, *data;
+} orc_dyn;
+
 typedef struct orc_rc {
   int32_t cascades, probes0, res_cap;
   float s0, vox_skip;
@@ -44,6 +52,7 @@ typedef struct orc_rc {
   const uint32_t* bounce;   /* bounce irradiance per attribute slot (orc_bounce_inject: 4 words of halves), or NULL */
   const uint16_t* bn_prev;  /* set only inside orc_bounce_volume: hits read this volume (bounce.ts voxBounce) */
   int32_t bn_prev_org[3];
+  const orc_dyn* dyn;       /* the dynamic layer the trace reads in front of the static hit, or NULL */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -66,12 +75,20 @@ int orc_canonical(const orc_vox* p, const uint32_t* vox, const uint32_t* rep, in
 /* f32 <-> f16 (IEEE binary16, round to nearest even: pack2x16float / unpack2x16float). */
 uint16_t orc_f2h(float f);
 float orc_h2f(uint16_t h);
+float orc_log2(float x);
 
 /* Number of rays of cascade c and the total. */
 uint64_t orc_rays(const orc_rc* rc, int c);
 /* voxSun (voxel.ts:631-673) with voxel shadow rays for the shadow-map taps: the sun visibility of every occupied
  * voxel with an attribute slot -> sunvis[slot] (max_voxels floats; other entries untouched). */
 int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis);
+/* The dynamic layer's initial state (every pointer empty, no bits, marking bounds at their identities). */
+int orc_dyn_init(const orc_vox* p, orc_dyn* d);
+/* One update of the dynamic layer (mark, parents, alloc + finalize, clear, splat, resolve; dynlayer.ts BUILD_WGSL)
+ * over the clipmap windows of p, shading samples with rc's frame (sun, sun_vis). */
+int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const float* nrm, const uint32_t* tri,
+                  const float* mat, orc_dyn* d);
+
 /* Bounce volume (DEFAULT_BOUNCE, bounce.ts:33): ORC_BN_DX x ORC_BN_DY x ORC_BN_DZ probes ORC_BN_CELL m apart; a volume
  * is [probes, x fastest][6 axes +x -x +y -y +z -z][4 halves: rgb, reach]. */
 #define ORC_BN_DX 32

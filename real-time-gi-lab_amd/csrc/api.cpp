@@ -105,9 +105,6 @@ VoxK vox_args(const rtgi_world* w) {
   k.inv_v0 = k.pow2 ? 1.0f / k.v0 : 0.0f;
   k.NL = w->NL;
   k.G = w->G;
-  k.G_log2 = -1;
-  for (int l = 0; l < 16; l++)
-    if ((1 << l) == w->G) k.G_log2 = l;
   k.E = w->E;
   k.per = w->per;
   k.max_bricks = w->vp.max_bricks;
@@ -130,10 +127,22 @@ VoxK vox_args(const rtgi_world* w) {
   k.nrm = w->nrm;
   k.tri = w->tri;
   k.mat = w->mat;
-  k.ntris = w->ntris;
+  k.ntris = std::min(w->ntris, w->dyn_first);  // static triangles only (voxel.ts:822)
   k.lds_marks = 0;
   k.lds_marks = vox_mark_lds_bytes(k) > 0;
   k.stats = w->stats;
+  if (w->dyn_hdr) {
+    k.dyn.hdr = w->dyn_hdr;
+    k.dyn.ptr = w->dyn_ptr;
+    k.dyn.bits = w->dyn_bits;
+    k.dyn.meta = w->dyn_meta;
+    k.dyn.mask = w->dyn_mask;
+    k.dyn.data = w->dyn_data;
+    k.dyn.cap = w->dyn_cap;
+    k.dyn.nw = (uint32_t)(w->NL * w->per / 32);
+    k.dyn.first_tri = std::min(w->dyn_first, w->ntris);
+    k.dyn.end_tri = w->ntris;
+  }
   return k;
 }
 
@@ -234,6 +243,16 @@ uint32_t probe_index_host(const rtgi_world* w, int c, uint32_t x, uint32_t y, ui
 
 namespace {
 
+void dyn_free(rtgi_world* w) {
+  const size_t NLper = (size_t)w->NL * w->per, cap = (size_t)w->dyn_cap;
+  dfree(w, w->dyn_hdr, DYN_HDR);
+  dfree(w, w->dyn_ptr, NLper);
+  dfree(w, w->dyn_bits, 2 * NLper / 32);
+  dfree(w, w->dyn_meta, cap);
+  dfree(w, w->dyn_mask, cap * 16);
+  dfree(w, w->dyn_data, cap * 512 * DYN_WORDS);
+}
+
 void free_all(rtgi_world* w) {
   size_t NLper = (size_t)w->NL * w->per, mb = (size_t)w->vp.max_bricks, mv = (size_t)w->vp.max_voxels;
   dfree(w, w->hdr, HDR_WORDS);
@@ -265,6 +284,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->bounce, (w->rp.flags & RTGI_RC_BOUNCE) ? mv * 4 : 0);
   for (int i = 0; i < 2; i++) dfree(w, w->bnvol[i], (w->rp.flags & RTGI_RC_BOUNCE) ? BN_WORDS : 0);
   dfree(w, w->vslot, (w->rp.flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) ? mv : 0);
+  dyn_free(w);
 }
 
 }  // namespace
@@ -290,6 +310,8 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   int e = 0;
   if (stage == RTGI_STAGE_VOX) {
     e = launch_vox_build(vox_args(w), s);
+    if (!e && w->dyn_hdr) e = launch_dyn_build(vox_args(w), s);
+    if (!e && w->dyn_hdr) e = launch_dyn_splat(trace_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
     bounce_advance(w);
     TraceK t = trace_args(w);
@@ -892,6 +914,68 @@ extern "C" int rtgi_world_read_probes(rtgi_world* w, int32_t c, float* pos, floa
   return RTGI_OK;
 }
 
+extern "C" int rtgi_world_set_dynamic(rtgi_world* w, uint32_t first_dynamic_tri, int32_t dyn_bricks) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (dyn_bricks < 1 || dyn_bricks > (1 << 20))
+    return rtgi_fail(RTGI_E_INVALID, "dynamic layer: dyn_bricks must be in 1..2^20");
+  if (w->per % 32) return rtgi_fail(RTGI_E_INVALID, "dynamic layer: G^3 must be a multiple of 32");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  dyn_free(w);
+  w->dyn_first = first_dynamic_tri;
+  w->dyn_cap = dyn_bricks;
+  const size_t NLper = (size_t)w->NL * w->per, cap = (size_t)dyn_bricks;
+  int e = dalloc(w, &w->dyn_hdr, DYN_HDR);
+  if (!e) e = dalloc(w, &w->dyn_ptr, NLper);
+  if (!e) e = dalloc(w, &w->dyn_bits, 2 * NLper / 32);
+  if (!e) e = dalloc(w, &w->dyn_meta, cap);
+  if (!e) e = dalloc(w, &w->dyn_mask, cap * 16);
+  if (!e) e = dalloc(w, &w->dyn_data, cap * 512 * DYN_WORDS);
+  if (e) {
+    dyn_free(w);
+    w->dyn_first = 0xffffffffu;
+    return e;
+  }
+  // Initial state (createDynLayer, dynlayer.ts:670-673): every pointer empty, no bits, bounds being marked at
+  // their identities.
+  std::vector<uint32_t> hdr(DYN_HDR, 0u);
+  hdr[32] = hdr[33] = hdr[34] = 0xffffffffu;
+  HIPCHK(hipMemcpyAsync(w->dyn_hdr, hdr.data(), DYN_HDR * 4, hipMemcpyHostToDevice, w->stream));
+  HIPCHK(hipMemsetD32Async((hipDeviceptr_t)w->dyn_ptr, (int)DYN_EMPTY, NLper, w->stream));
+  HIPCHK(hipMemsetAsync(w->dyn_bits, 0, 2 * NLper / 32 * 4, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_dyn_counters(rtgi_world* w, uint32_t* bricks, uint32_t* brick_overflow,
+                                            uint32_t* outside) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (!w->dyn_hdr) return rtgi_fail(RTGI_E_STATE, "no dynamic layer (rtgi_world_set_dynamic)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  uint32_t h[DYN_HDR];
+  HIPCHK(hipMemcpy(h, w->dyn_hdr, sizeof h, hipMemcpyDeviceToHost));
+  if (bricks) *bricks = h[8];
+  if (brick_overflow) *brick_overflow = h[1];
+  if (outside) *outside = h[9];
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_read_dynamic(rtgi_world* w, uint32_t* hdr, uint32_t* ptr, uint32_t* meta, uint32_t* mask,
+                                       uint32_t* data) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (!w->dyn_hdr) return rtgi_fail(RTGI_E_STATE, "no dynamic layer (rtgi_world_set_dynamic)");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  const size_t NLper = (size_t)w->NL * w->per, cap = (size_t)w->dyn_cap;
+  if (hdr) HIPCHK(hipMemcpy(hdr, w->dyn_hdr, DYN_HDR * 4, hipMemcpyDeviceToHost));
+  if (ptr) HIPCHK(hipMemcpy(ptr, w->dyn_ptr, NLper * 4, hipMemcpyDeviceToHost));
+  if (meta) HIPCHK(hipMemcpy(meta, w->dyn_meta, cap * 4, hipMemcpyDeviceToHost));
+  if (mask) HIPCHK(hipMemcpy(mask, w->dyn_mask, cap * 16 * 4, hipMemcpyDeviceToHost));
+  if (data) HIPCHK(hipMemcpy(data, w->dyn_data, cap * 512 * DYN_WORDS * 4, hipMemcpyDeviceToHost));
+  return RTGI_OK;
+}
+
 extern "C" int rtgi_world_read_bounce_volume(rtgi_world* w, uint16_t* texels, int32_t origin[3]) {
   if (!w || !texels || !origin) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_read_bounce_volume: null argument");
   if (!w->bounce) return rtgi_fail(RTGI_E_STATE, "no bounce volume (RTGI_RC_BOUNCE)");
@@ -1025,6 +1109,7 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   size_t lds = 0;
   bool all_lds = true;
   p.stats = false;
+  p.dyn = false;
   p.fused = n > 0;
   for (int i = 0; i < n; i++) {
     p.fused = p.fused && worlds[i]->tuning.fused != 0;
@@ -1040,6 +1125,7 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
     if (m == 0) all_lds = false;
     lds = std::max(lds, m);
     p.stats = p.stats || worlds[i]->stats != nullptr;
+    p.dyn = p.dyn || worlds[i]->dyn_hdr != nullptr;
   }
   p.n = n;
   for (int l = 0; l < BL_COUNT; l++) p.grid[l] = start[(size_t)l * (n + 1) + n];
@@ -1052,6 +1138,7 @@ int plan_launch_stage(const BatchPlan& p, hipStream_t s, int stage) {
   int e = 0;
   if (stage == 0) {
     e = launch_vox_build_batch(p, s);
+    if (!e) e = launch_dyn_build_batch(p, s);
   } else if (stage == 1) {
     e = launch_sun_batch(p, s);
     if (!e) e = launch_bounce_batch(p, s);

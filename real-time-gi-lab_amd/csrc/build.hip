@@ -803,6 +803,16 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   for (int L = 1; L < k.NL; L++) out[BL_REDUCE + L - 1] = persist;
   out[BL_ALLOC_VOXELS] = 1;
   out[BL_RESOLVE] = persist;
+  if (k.dyn.hdr && k.dyn.end_tri > k.dyn.first_tri) {  // the dynamic layer (grid-stride loops)
+    const uint32_t nd = k.dyn.end_tri - k.dyn.first_tri;
+    const uint32_t vox = std::min<uint32_t>(4096, cdiv((uint64_t)k.dyn.cap * 512u, BLK));
+    out[BL_DL_MARK] = std::min<uint32_t>(4096, cdiv(nd, BLK));
+    out[BL_DL_PARENTS] = k.dyn.nw > (uint32_t)k.per / 32u ? cdiv(k.dyn.nw - (uint32_t)k.per / 32u, BLK) : 0;
+    out[BL_DL_ALLOC] = 1;
+    out[BL_DL_CLEAR] = vox;
+    out[BL_DL_SPLAT] = std::min<uint32_t>(4096, cdiv(nd, BLK));
+    for (int L = 1; L < k.NL; L++) out[BL_DL_RESOLVE + L - 1] = vox;
+  }
   // Per attribute slot: 4 lanes per voxel for the sun taps, one for the bounce injection (grid-stride loops).
   const uint64_t slots = (uint64_t)k.max_voxels;
   if (t.sunvis) out[BL_SUN] = std::min<uint32_t>(8192, std::max<uint32_t>(1, cdiv(slots * 4, BLK)));

@@ -41,12 +41,34 @@ constexpr uint32_t ATTR_NONE = 0xffffffffu;
 constexpr uint32_t BIG_CAP = 1u << 18;
 inline uint32_t big_cap_for(uint32_t ntris) { return ntris < BIG_CAP ? ntris : BIG_CAP; }
 
+// Dynamic coverage layer (src/core/dynlayer.ts): dynamic triangles [first_tri, end_tri) are splatted every update
+// into bricks of 8^3 voxels, each voxel 13 words (projected areas on +x -x +y -y +z -z, area, area x radiance rgb,
+// area x albedo rgb: the last three stay 0, as the reference's splat without objects receiving light), fixed point at
+// level 0 and floats above after the resolve. hdr words: 0 wanted bricks, 1 dropped, 2..7 bounds (cm + 2^30),
+// 8 allocated, 9 triangles outside every clipmap, 10 update counter (bitset parity), 16 + L first slot of level L,
+// 24 + L bricks of level L, 32..37 and 38 bounds and outside count being marked.
+constexpr uint32_t DYN_WORDS = 13;
+constexpr uint32_t DYN_HDR = 64;
+constexpr uint32_t DYN_EMPTY = 0xffffffffu;
+constexpr uint32_t DYN_DROPPED = 0xfffffffdu;
+constexpr float DYN_FIX = 65536.0f;
+struct DynK {
+  uint32_t* hdr;    // [DYN_HDR], or null: no layer
+  uint32_t* ptr;    // [levels][G^3] slot of each brick, DYN_EMPTY or DYN_DROPPED
+  uint32_t* bits;   // [2][nw] wanted bricks of this and the previous update (bit g = (levels-1-L) G^3 + brick)
+  uint32_t* meta;   // [cap] pointer index (L G^3 + brick) of each slot
+  uint32_t* mask;   // [cap][16] voxels with a value
+  uint32_t* data;   // [cap][512][DYN_WORDS]
+  int32_t cap;      // brick pool (dynBricks)
+  uint32_t nw;      // levels G^3 / 32
+  uint32_t first_tri, end_tri;
+};
+
 struct VoxK {
   float v0;
   float inv_v0;   // 1 / v0, used only when pow2 (then x / v == x * (1/v) exactly)
   int32_t pow2;   // v0 is a power of two
   int32_t NL, G, E, per;  // per = G^3
-  int32_t G_log2;         // log2(G) when G is a power of two, else -1 (brick indices by shifts)
   int32_t max_bricks, max_voxels;
   int32_t origin[RTGI_MAX_LEVELS][3];
   uint32_t* hdr;
@@ -69,6 +91,7 @@ struct VoxK {
   uint32_t big_cap;    // entries per level of each big list: min(ntris, BIG_CAP)
   int32_t lds_marks;    // brick marks accumulate in an LDS bitmap of levels * G^3 bits (0: global fallback)
   unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
+  DynK dyn;                   // the dynamic layer (dyn.hdr null: none)
 };
 
 // Bounce volume (DEFAULT_BOUNCE, src/core/bounce.ts:33): BN_DX x BN_DY x BN_DZ ambient-cube probes BN_CELL m apart
@@ -192,7 +215,13 @@ enum BatchLaunch : int {
   BL_REDUCE,                               // + level - 1 (levels 1..7)
   BL_ALLOC_VOXELS = BL_REDUCE + RTGI_MAX_LEVELS - 1,
   BL_RESOLVE,
-  BL_SUN,                                  // traced sun visibility of every occupied voxel
+  BL_DL_MARK,                              // dynamic layer: wanted bricks of the dynamic triangles
+  BL_DL_PARENTS,                           //   their ancestors
+  BL_DL_ALLOC,                             //   slots in bit order (one block of 1024), counts, bounds
+  BL_DL_CLEAR,                             //   the allocated bricks' words
+  BL_DL_SPLAT,                             //   fixed-point splats of the dynamic triangles' samples
+  BL_DL_RESOLVE,                           //   + level - 1 (levels 1..7): coarse values from own splats + children
+  BL_SUN = BL_DL_RESOLVE + RTGI_MAX_LEVELS - 1,  // traced sun visibility of every occupied voxel
   BL_BNVOL,                                // bounce volume update (8 probes per block)
   BL_BOUNCE,                               // bounce injection from this update's volume
   BL_RELOC,                                // relocation of every cascade's probes
@@ -211,10 +240,15 @@ struct BatchPlan {
   size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
   bool stats;       // some world of the batch counts tracer statistics
   bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps
+  bool dyn;         // some world has a dynamic layer (the trace reads it)
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);
 int launch_vox_build_batch(const BatchPlan& p, void* stream);
+// The dynamic layer's build (single world / batch), after the voxel build.
+int launch_dyn_build(const VoxK& k, void* stream);
+int launch_dyn_splat(const TraceK& t, void* stream);
+int launch_dyn_build_batch(const BatchPlan& p, void* stream);
 int launch_trace_batch(const BatchPlan& p, void* stream);
 int launch_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);

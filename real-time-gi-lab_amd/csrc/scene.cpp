@@ -157,11 +157,12 @@ struct Builder {
   }
 };
 
-rtgi_scene* finish(Builder& b, Rng& rng, const double eye[3], const float sun[3]) {
+rtgi_scene* finish(Builder& b, Rng& rng, const double eye[3], const float sun[3], uint32_t first_dynamic = ~0u) {
   rtgi_scene* s = (rtgi_scene*)std::calloc(1, sizeof(rtgi_scene));
   if (!s) return nullptr;
   uint32_t nt = (uint32_t)(b.tri.size() / 4), nv = (uint32_t)(b.pos.size() / 3);
   s->ntris = nt;
+  s->first_dynamic_tri = std::min(first_dynamic, nt);
   s->nverts = nv;
   s->nmat = nt;
   s->pos = (float*)std::malloc(sizeof(float) * 3 * (size_t)std::max(nv, 1u));
@@ -236,7 +237,9 @@ rtgi_scene* atrium(const rtgi_scene_params& p) {
   b.box({0, 8.15, 6}, {6.3, 0.15, 0.3});
   b.box({-6, 8.15, 0}, {0.3, 0.15, 5.7});
   b.box({6, 8.15, 0}, {0.3, 0.15, 5.7});
-  // 16 boxes under seeded rigid motion (rotation about y and translation, evaluated at the frame).
+  // 16 boxes under seeded rigid motion (rotation about y and translation, evaluated at the frame): the tail of the
+  // triangle table.
+  const uint32_t first_dynamic = (uint32_t)(b.tri.size() / 4);
   for (int k = 0; k < 16; k++) {
     V3 c{rng.uni(-8, 8), rng.uni(0.4, 3.0), rng.uni(-8, 8)};
     V3 h{rng.uni(0.15, 0.4), rng.uni(0.15, 0.4), rng.uni(0.15, 0.4)};
@@ -251,7 +254,7 @@ rtgi_scene* atrium(const rtgi_scene_params& p) {
   double eye[3] = {0.0, 5.0, 0.0};
   float sun[3] = {-0.3f, 1.0f, 0.25f};
   normalizef(sun);
-  return finish(b, rng, eye, sun);
+  return finish(b, rng, eye, sun, first_dynamic);
 }
 
 // City block: about 500k triangles at detail 1 with 64 buildings on a 200 m block.

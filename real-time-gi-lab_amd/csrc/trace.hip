@@ -226,28 +226,19 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
   // Steps and brick jumps select with conditional moves rather than branches (the same f32 operations: each
   // candidate sum is the one the branch would form), so lanes on different axes do not split the wavefront.
   for (it = 0; it < 1024; it++) {
+    i3 b = shr3(c, 3);
     // The pointer of a brick is read once per brick visited: c left that brick when some coordinate differs above
     // its low 3 bits (c >= 0 inside the window).
     if ((uint32_t)((c.x ^ lc.x) | (c.y ^ lc.y) | (c.z ^ lc.z)) > 7u) {
-      const i3 b = shr3(c, 3);
-      uint32_t off;
-      if (k.G_log2 >= 0) {  // (bz G + by) G + bx and the level's table by shifts
-        const int gl = k.G_log2;
-        off = (((uint32_t)L << (3 * gl)) | ((uint32_t)b.z << (2 * gl)) | ((uint32_t)b.y << gl) | (uint32_t)b.x) << 2;
-      } else {
-        off = ptr_base + (((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x) * 4u;
-      }
-      ptr = buf_u32(ptr_r, off);
+      uint32_t bi = ((uint32_t)b.z * (uint32_t)G + (uint32_t)b.y) * (uint32_t)G + (uint32_t)b.x;
+      ptr = buf_u32(ptr_r, ptr_base + bi * 4u);
       lc = c;
     }
     if (ptr >= PTR_MARK && ptr != PTR_DROP) {  // empty or marked-unallocated brick: jump to its exit
-      const i3 b = shr3(c, 3);
       f3 bt = ((tof3(b * 8) + sp * 8.0f) * v + lo - o) * inv;
       float te = fminf(fminf(bt.x, bt.y), bt.z);
       if (te >= tEnd) break;
-      // The reference's axis choice (x if bt.x <= bt.y and bt.x <= bt.z, else y if bt.y <= bt.z, else z) is the
-      // first axis equal to the minimum (values are finite; +0 == -0).
-      bool bx = bt.x == te, by = !bx && bt.y == te, bz = !bx && !by;
+      bool bx = bt.x <= bt.y && bt.x <= bt.z, by = !bx && bt.y <= bt.z, bz = !bx && !by;
       i3 nb{b.x + (bx ? st.x : 0), b.y + (by ? st.y : 0), b.z + (bz ? st.z : 0)};
       if (!inside3(nb, G)) break;
       t = te;
@@ -269,21 +260,17 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       hcell = h.cell;
     } else {
       hbrick = ptr;
-      hbit = 0u;  // formed at a hit
-      // bit = x + 8 y + 64 z of the cell in its brick: word bit >> 5 = (y >> 2 & 1) | z << 1, shift bit & 31.
-      const uint32_t wi = (((uint32_t)c.y >> 2) & 1u) | (((uint32_t)c.z & 7u) << 1);
-      const uint32_t sh = ((uint32_t)c.x & 7u) | (((uint32_t)c.y & 3u) << 3);
-      occupied = ((buf_u32(occ_r, (ptr << 6) | (wi << 2)) >> sh) & 1u) != 0u;
+      hbit = bit_of(c);
+      occupied = (buf_u32(occ_r, ptr * 64u + (hbit >> 5) * 4u) & (1u << (hbit & 31u))) != 0u;
     }
     if (occupied && tIn >= tSkip) {  // the hit id is formed once, here
-      if (ptr != PTR_DROP) hbit = bit_of(c);
       res = Hit{true, hbrick, hbit, tIn, hit_id(k, hlev, hcell), ST ? (uint32_t)it + 1u : 0u};
       return res;
     }
     float tExit = fminf(fminf(tMax.x, tMax.y), tMax.z);
     if (tExit >= tEnd) break;
     tIn = tExit;
-    bool sx = tMax.x == tExit, sy = !sx && tMax.y == tExit, sz = !sx && !sy;  // as for the brick jump above
+    bool sx = tMax.x <= tMax.y && tMax.x <= tMax.z, sy = !sx && tMax.y <= tMax.z, sz = !sx && !sy;
     c = i3{c.x + (sx ? st.x : 0), c.y + (sy ? st.y : 0), c.z + (sz ? st.z : 0)};
     f3 nt = tMax + tDelta;
     tMax = mk3(sx ? nt.x : tMax.x, sy ? nt.y : tMax.y, sz ? nt.z : tMax.z);
@@ -305,6 +292,112 @@ __device__ __forceinline__ int vox_level_for(const VoxK& k, int L, f3 e) {
     l++;
   }
   return l;
+}
+
+// ---- Dynamic layer read (traceDyn, src/core/dynlayer.ts:104-160) ----
+
+// dynFind: the voxel at p of level L, or of the first coarser level whose brick was not dropped; x = its first data
+// word (DYN_EMPTY: empty brick), y = the level read.
+__device__ __forceinline__ uint2 dyn_find(const VoxK& k, int L, f3 p) {
+  const DynK& D = k.dyn;
+  int l = L;
+  for (;;) {
+    float v = vox_size(k, l);
+    i3 c = cell_at(p - tof3(origin(k, l)) * v * 8.0f, v, k.inv_v0 * inv_pow2(l), k.pow2 != 0);
+    if (!inside3(c, k.G * 8)) return make_uint2(DYN_EMPTY, (uint32_t)l);
+    i3 b = shr3(c, 3);
+    uint32_t ptr = D.ptr[(uint32_t)l * (uint32_t)k.per + ((uint32_t)b.z * k.G + (uint32_t)b.y) * k.G + (uint32_t)b.x];
+    if (ptr == DYN_DROPPED && l + 1 < k.NL) {
+      l++;
+      continue;
+    }
+    if (ptr >= (uint32_t)D.cap) return make_uint2(DYN_EMPTY, (uint32_t)l);
+    return make_uint2((ptr * 512u + bit_of(c)) * DYN_WORDS, (uint32_t)l);
+  }
+}
+// dynWord: level 0 holds fixed point, coarser levels floats.
+__device__ __forceinline__ float dyn_word(const DynK& D, uint32_t base, uint32_t q, uint32_t L) {
+  uint32_t u = D.data[base + q];
+  return L == 0u ? (float)u / DYN_FIX : __uint_as_float(u);
+}
+// dynRead: mean radiance and coverage per voxel length for direction d (the three faces against d).
+__device__ __forceinline__ float4 dyn_read(const DynK& D, uint2 f, f3 d) {
+  if (f.x == DYN_EMPTY) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float a = dyn_word(D, f.x, 6, f.y);
+  if (a <= 0.0f) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t sx = d.x > 0.0f ? 1u : 0u, sy = d.y > 0.0f ? 1u : 0u, sz = d.z > 0.0f ? 1u : 0u;
+  float front = fabsf(d.x) * dyn_word(D, f.x, sx, f.y) + fabsf(d.y) * dyn_word(D, f.x, 2u + sy, f.y) +
+                fabsf(d.z) * dyn_word(D, f.x, 4u + sz, f.y);
+  f3 r = mk3(dyn_word(D, f.x, 7, f.y) / a, dyn_word(D, f.x, 8, f.y) / a, dyn_word(D, f.x, 9, f.y) / a);
+  return make_float4(r.x, r.y, r.z, fminf(front, 1.0f));
+}
+// traceDyn: covered radiance and coverage of [t0, t1] along d from o, marched in steps of half the cone width
+// w = kc t (not below v0), read at the two levels around w and interpolated; coverage adds front to back up to 1.
+__device__ float4 trace_dyn(const TraceK& a, f3 o, f3 d, float t0, float t1, float kc) {
+  const VoxK& k = a.vx;
+  const DynK& D = k.dyn;
+  if (D.hdr[0] == 0u || t1 <= t0) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  f3 bmin = (mk3((float)D.hdr[2], (float)D.hdr[3], (float)D.hdr[4]) - splat3(1073741824.0f)) * 0.01f;
+  f3 bmax = (mk3((float)D.hdr[5], (float)D.hdr[6], (float)D.hdr[7]) - splat3(1073741824.0f)) * 0.01f;
+  f3 dd = mk3(fabsf(d.x) < 1e-9f ? 1e-9f : d.x, fabsf(d.y) < 1e-9f ? 1e-9f : d.y, fabsf(d.z) < 1e-9f ? 1e-9f : d.z);
+  f3 ta = (bmin - o) / dd, tb = (bmax - o) / dd;
+  f3 tn = min3(ta, tb), tf = max3(ta, tb);
+  float ba = fmaxf(fmaxf(fmaxf(tn.x, tn.y), tn.z), t0);
+  float bb = fminf(fminf(fminf(tf.x, tf.y), tf.z), t1);
+  if (ba >= bb) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float v0 = k.v0;
+  const int nl = k.NL;
+  f3 rad = splat3(0.0f);
+  float cov = 0.0f, t = t0;
+  for (int i = 0; i < 128; i++) {
+    if (t >= bb || cov >= 1.0f) break;
+    float w = fmaxf(kc * t, v0);
+    float te = fminf(t + 0.5f * w, t1);
+    if (te > ba) {
+      float len = te - t;
+      f3 p = o + d * (0.5f * (t + te));
+      float lf = fminf(fmaxf(log2_series(w / v0), 0.0f), (float)(nl - 1));
+      int L0 = (int)floorf(lf);
+      float f = lf - (float)L0;
+      int Lw = vox_level_for(k, L0, p);
+      if (Lw != L0) {
+        L0 = Lw;
+        f = 0.0f;
+      }
+      uint2 fa = dyn_find(k, L0, p);
+      float4 sa = dyn_read(D, fa, d);
+      float aa = sa.w * len / vox_size(k, (int)fa.y);
+      f3 ra = mk3(sa.x, sa.y, sa.z) * aa;
+      if (f > 0.0f && L0 + 1 < nl) {
+        uint2 fb = dyn_find(k, L0 + 1, p);
+        float4 sb = dyn_read(D, fb, d);
+        float ab = sb.w * len / vox_size(k, (int)fb.y);
+        f3 rb = mk3(sb.x, sb.y, sb.z) * ab;
+        ra = ra * (1.0f - f) + rb * f;  // mix
+        aa = aa * (1.0f - f) + ab * f;
+      }
+      if (aa > 0.0f) {
+        float ac = fminf(aa, 1.0f - cov);
+        rad = rad + ra * (ac / aa);
+        cov += ac;
+      }
+    }
+    t = te;
+  }
+  return make_float4(rad.x, rad.y, rad.z, cov);
+}
+// The dynamic layer in front of a cascade ray's static result r (traceWorld, src/core/rc.ts:224-231): read from
+// max(t0, v0) to the static hit's entry distance (or t1), at the width of the direction bin's cone.
+__device__ __forceinline__ float4 with_dyn(const TraceK& a, int c, f3 o, f3 d, float t0, float t1, const Hit& h, f3 ot,
+                                           float4 r) {
+  if (!a.vx.dyn.hdr) return r;
+  float tS = h.hit ? dot3(ot + d * h.tIn - o, d) : t1;
+  float kc = 3.5449077f / (float)a.c[c].res;
+  float4 dl = trace_dyn(a, o, d, fmaxf(t0, a.vx.v0), fminf(t1, tS), kc);
+  if (dl.w > 0.0f)
+    r = make_float4(dl.x + (1.0f - dl.w) * r.x, dl.y + (1.0f - dl.w) * r.y, dl.z + (1.0f - dl.w) * r.z,
+                    (1.0f - dl.w) * r.w);
+  return r;
 }
 
 __device__ __forceinline__ int cascade_of(const TraceK& a, uint64_t ray) {
@@ -345,7 +438,7 @@ __device__ __forceinline__ void split_ray(const CascK& C, uint32_t r, uint32_t& 
 // probes; the grid points have normal 0), skip voxSkip voxels. ot: the traversal's origin (hits enter at ot + d t).
 template <bool ST>
 __device__ __forceinline__ void trace_probe_ray(const TraceK& a, int c, i3 g, uint32_t pi, f3 d, f3 inv, Hit& h,
-                                                f3& ot) {
+                                                f3& ot, f3& op) {
   const CascK& C = a.c[c];
   f3 o, n = splat3(0.0f);
   if (a.reloc) {
@@ -359,10 +452,11 @@ __device__ __forceinline__ void trace_probe_ray(const TraceK& a, int c, i3 g, ui
   int L = vox_level_for(a.vx, C.vox_level, o + d * t1);
   float v = vox_size(a.vx, L);
   ot = a.reloc ? o + n * (a.vox_offset * v) : o;
+  op = o;
   h = trace_vox<ST>(a, ot, d, inv, t0, t1, a.vox_skip * v, L);
 }
 
-template <bool ST>
+template <bool ST, bool DY>
 __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   uint64_t ray = (uint64_t)lb * BLK + threadIdx.x;
   if (ray >= a.total_rays) return;
@@ -376,12 +470,14 @@ __device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_
   f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
   Hit h;
   f3 ot;
-  trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
+  f3 op;
+  trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot, op);
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   if (h.hit) {
     f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn, h.brick, h.bit);
     out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
+  if (DY) out = with_dyn(a, c, op, d, C.b0 * a.s0, C.b1 * a.s0, h, ot, out);
   rad_store(a, ray, out);
   if (a.hits) a.hits[ray] = h.id;
   if (ST && a.vx.stats) {
@@ -430,21 +526,23 @@ __global__ void __launch_bounds__(BLK) k_trace_rays(TraceK a, int c, const float
     f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), D, O + D * h.tIn, h.brick, h.bit);
     r = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
   }
+  r = with_dyn(a, c, O, D, t0, t1, h, O, r);
   if (a.fp16) r = make_float4(hlo(h2(r.x, r.y)), hhi(h2(r.x, r.y)), hlo(h2(r.z, r.w)), hhi(h2(r.z, r.w)));
   out[i] = r;
   hits[i] = h.id;
 }
 
 // ST: count the tracer statistics (a separate instantiation, so the counting costs the plain trace nothing).
-template <bool ST>
+// DY: some world has a dynamic layer (a separate instantiation: the read costs registers the plain trace keeps).
+template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace(TraceK a) {
-  trace_body<ST>(a, blockIdx.x, gridDim.x);
+  trace_body<ST, DY>(a, blockIdx.x, gridDim.x);
 }
-template <bool ST>
+template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_b(BatchK b) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   int w = batch_world(b, blk);
-  trace_body<ST>(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  trace_body<ST, DY>(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 __device__ __forceinline__ f3 rgb_of(float4 x) { return mk3(x.x, x.y, x.z); }
@@ -655,7 +753,7 @@ __device__ __forceinline__ float4 rad_round(const TraceK& a, float4 v) {
   return make_float4(hlo(p0), hhi(p0), hlo(p1), hhi(p1));
 }
 
-template <bool ST>
+template <bool ST, bool DY>
 __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[c];
   const bool quad = !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
@@ -687,12 +785,14 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
     f3 inv = mk3(de.inv[0], de.inv[1], de.inv[2]);
     Hit h;
     f3 ot;
-    trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot);
+    f3 op;
+    trace_probe_ray<ST>(a, c, g, pi, d, inv, h, ot, op);
     float4 out = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     if (h.hit) {
       f3 rgb = shade_voxel(a, vox_rank(a.vx, h.brick, h.bit), d, ot + d * h.tIn, h.brick, h.bit);
       out = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
     }
+    if (DY) out = with_dyn(a, c, op, d, C.b0 * a.s0, C.b1 * a.s0, h, ot, out);
     size_t ray = C.ray_off + (size_t)di * C.np + pi;
     if (a.hits) a.hits[ray] = h.id;
     steps = h.steps;
@@ -736,17 +836,17 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
   }
 }
 // Eight wavefronts per SIMD (at most 64 VGPRs): the trace is latency-bound, and occupancy pays more than registers.
-template <bool ST>
+template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge(TraceK a, int c) {
-  trace_merge_body<ST>(a, c, blockIdx.x, gridDim.x);
+  trace_merge_body<ST, DY>(a, c, blockIdx.x, gridDim.x);
 }
 // Step of a batch: each world traces and merges its cascade N-1-step.
-template <bool ST>
+template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
-  trace_merge_body<ST>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  trace_merge_body<ST, DY>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
@@ -1342,11 +1442,12 @@ int grid_for(uint64_t n) {
 }  // namespace
 
 int launch_trace(const TraceK& k, void* stream) {
+  const bool dy = k.vx.dyn.hdr != nullptr;
   if (k.total_rays) {
     if (k.vx.stats)
-      k_trace<true><<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+      (dy ? k_trace<true, true> : k_trace<true, false>)<<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
     else
-      k_trace<false><<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+      (dy ? k_trace<false, true> : k_trace<false, false>)<<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
   }
   return (int)hipGetLastError();
 }
@@ -1365,12 +1466,13 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
 uint64_t trace_merge_threads(const TraceK& k, int c) { return k.no_merge ? k.c[c].rays : merge_threads(k, c); }
 
 int launch_trace_merge(const TraceK& k, int c, void* stream) {
+  const bool dy = k.vx.dyn.hdr != nullptr;
   uint64_t n = trace_merge_threads(k, c);
   if (!n) return 0;
   if (k.vx.stats)
-    k_trace_merge<true><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+    (dy ? k_trace_merge<true, true> : k_trace_merge<true, false>)<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   else
-    k_trace_merge<false><<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+    (dy ? k_trace_merge<false, true> : k_trace_merge<false, false>)<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   return (int)hipGetLastError();
 }
 
@@ -1439,9 +1541,9 @@ int launch_relocate_batch(const BatchPlan& p, void* stream) {
 int launch_trace_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_TRACE]) {
     if (p.stats)
-      k_trace_b<true><<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+      (p.dyn ? k_trace_b<true, true> : k_trace_b<true, false>)<<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
     else
-      k_trace_b<false><<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
+      (p.dyn ? k_trace_b<false, true> : k_trace_b<false, false>)<<<p.grid[BL_TRACE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TRACE));
   }
   return (int)hipGetLastError();
 }
@@ -1455,9 +1557,11 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
   if (p.grid[BL_TM + step]) {
     if (p.stats)
-      k_trace_merge_b<true><<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
+      (p.dyn ? k_trace_merge_b<true, true> : k_trace_merge_b<true, false>)<<<p.grid[BL_TM + step], BLK, 0,
+                                                                             (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
     else
-      k_trace_merge_b<false><<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
+      (p.dyn ? k_trace_merge_b<false, true> : k_trace_merge_b<false, false>)<<<p.grid[BL_TM + step], BLK, 0,
+                                                                               (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
   }
   return (int)hipGetLastError();
 }

@@ -66,6 +66,26 @@ __device__ __forceinline__ float div255(float x) {
   float q = x * r;
   return __builtin_fmaf(__builtin_fmaf(-q, 255.0f, x), r, q);
 }
+// log2 of a positive normal float: x = m 2^e with m in [sqrt(1/2), sqrt(2)], ln m = 2 atanh(s), s = (m - 1) / (m + 1),
+// by the series s + s^3/3 + ... + s^9/9 (|s| < 0.172: the next term is below 2^-30 of the sum). Plain IEEE float
+// operations in a fixed order, so that every conforming implementation gives the same bits (the level blend of the
+// dynamic layer's read depends on it; a library log2f differs between implementations in the last place).
+__device__ __forceinline__ float log2_series(float x) {
+  uint32_t u = __float_as_uint(x);
+  int e = (int)(u >> 23) - 127;
+  float m = __uint_as_float((u & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
+  float p = 1.0f / 9.0f;
+  p = p * s2 + 1.0f / 7.0f;
+  p = p * s2 + 1.0f / 5.0f;
+  p = p * s2 + 1.0f / 3.0f;
+  p = p * s2 + 1.0f;
+  return (float)e + ((2.0f * s) * p) * 1.44269504f;
+}
 __device__ __forceinline__ float div32767(float x) {
   const float r = 1.0f / 32767.0f;
   float q = x * r;

@@ -51,6 +51,11 @@ struct rtgi_world {
   uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
   float* sunvis = nullptr;                // traced sun visibility per attribute slot (RTGI_RC_SUN_TRACE)
   uint32_t* vslot = nullptr;              // brick << 9 | bit per attribute slot (RTGI_RC_SUN_TRACE or _BOUNCE)
+  // Dynamic coverage layer (rtgi_world_set_dynamic): triangles [dyn_first, ntris) go to it instead of the bricks.
+  uint32_t dyn_first = 0xffffffffu;
+  int32_t dyn_cap = 0;
+  uint32_t *dyn_hdr = nullptr, *dyn_ptr = nullptr, *dyn_bits = nullptr, *dyn_meta = nullptr, *dyn_mask = nullptr,
+           *dyn_data = nullptr;
   uint32_t* bounce = nullptr;             // bounce irradiance per attribute slot, 4 words (RTGI_RC_BOUNCE)
   uint32_t* bnvol[2] = {};                // the two bounce volumes (RTGI_RC_BOUNCE), BN_WORDS each
   int bn_par = 1;                         // the volume the last update wrote

@@ -114,6 +114,7 @@ class Scene:
     eye: tuple = (0.0, 0.0, 0.0)
     sunDir: tuple = (-0.3, 1.0, 0.25)
     bounds: tuple = None
+    firstDynamicTri: int = None  # the moving parts are triangles [firstDynamicTri, ntris) (None: none)
 
     @property
     def ntris(self):
@@ -133,7 +134,7 @@ def generate_scene(kind, seed=1, frame=0, extent=0.0, buildings=0, detail=1.0):
                    tri=_L.as_np(s.tri, s.ntris * 4, np.uint32).reshape(-1, 4),
                    materials=_L.as_np(s.materials, s.nmat * 4, np.float32).reshape(-1, 4),
                    eye=tuple(s.eye), sunDir=tuple(s.sun_dir),
-                   bounds=(tuple(s.bounds_lo), tuple(s.bounds_hi)))
+                   bounds=(tuple(s.bounds_lo), tuple(s.bounds_hi)), firstDynamicTri=int(s.first_dynamic_tri))
     finally:
         lib().rtgi_scene_free(out)
     return sc
@@ -333,6 +334,29 @@ class World:
         par = np.zeros(n, np.uint32)
         check(lib().rtgi_world_read_probes(self._h, c, ptr(pos), ptr(nrm), ptr(par)))
         return pos, nrm, par
+
+    def set_dynamic(self, first_dynamic_tri, dyn_bricks=512):
+        """Triangles from first_dynamic_tri on go to the dynamic coverage layer (createVoxWorld's firstDynamicTri,
+        VoxParams.dynBricks; src/core/dynlayer.ts) instead of the bricks."""
+        check(lib().rtgi_world_set_dynamic(self._h, int(first_dynamic_tri), int(dyn_bricks)))
+        self._dyn_bricks = int(dyn_bricks)
+
+    def read_dyn_counters(self):
+        """Allocated bricks, dropped bricks and triangles outside every clipmap of the last update."""
+        v = (C.c_uint32 * 3)()
+        check(lib().rtgi_world_read_dyn_counters(self._h, C.byref(v, 0), C.byref(v, 4), C.byref(v, 8)))
+        return dict(bricks=int(v[0]), brickOverflow=int(v[1]), outside=int(v[2]))
+
+    def read_dynamic(self):
+        """The raw layer: hdr [64], ptr [levels, G^3], meta [cap], mask [cap, 16], data [cap, 512, 13] (uint32)."""
+        cap = self._dyn_bricks
+        per = self.vox.G ** 3
+        out = dict(hdr=np.zeros(64, np.uint32), ptr=np.zeros((self.vox.levels, per), np.uint32),
+                   meta=np.zeros(cap, np.uint32), mask=np.zeros((cap, 16), np.uint32),
+                   data=np.zeros((cap, 512, 13), np.uint32))
+        check(lib().rtgi_world_read_dynamic(self._h, ptr(out["hdr"]), ptr(out["ptr"]), ptr(out["meta"]),
+                                            ptr(out["mask"]), ptr(out["data"])))
+        return out
 
     def read_bounce_volume(self):
         """The bounce volume of the last update: texels (z, y, x, axis, rgba) as float16 (axes +x -x +y -y +z -z;

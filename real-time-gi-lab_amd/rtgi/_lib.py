@@ -106,7 +106,7 @@ class SceneC(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntris", C.c_uint32), ("nmat", C.c_uint32),
                 ("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)), ("tri", C.POINTER(C.c_uint32)),
                 ("materials", C.POINTER(C.c_float)), ("bounds_lo", C.c_double * 3), ("bounds_hi", C.c_double * 3),
-                ("eye", C.c_double * 3), ("sun_dir", C.c_float * 3)]
+                ("eye", C.c_double * 3), ("sun_dir", C.c_float * 3), ("first_dynamic_tri", C.c_uint32)]
 
 
 # Every exported entry point and its signature; tests check that the library exports exactly these.
@@ -135,6 +135,9 @@ SIGNATURES = {
     "rtgi_world_gather": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "rtgi_world_read_probes": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "rtgi_world_read_bounce_volume": (C.c_int, [_P, _P, _P]),
+    "rtgi_world_set_dynamic": (C.c_int, [_P, C.c_uint32, C.c_int32]),
+    "rtgi_world_read_dyn_counters": (C.c_int, [_P, _P, _P, _P]),
+    "rtgi_world_read_dynamic": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "rtgi_world_set_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_get_tuning": (C.c_int, [_P, C.POINTER(TuningC)]),
     "rtgi_world_destroy": (C.c_int, [_P]),

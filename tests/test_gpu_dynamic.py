@@ -1,0 +1,164 @@
+"""The dynamic coverage layer (src/core/dynlayer.ts) on the GPU against the oracle. The atrium's 16 moving boxes (the
+tail of its triangle table) go to the layer instead of the bricks; over updates at successive scene frames (the
+boxes move, the eye too) the layer's words are compared bit for bit: header (counts, per-level slot ranges, bounds,
+frame counter), brick pointers, slot owners, masks and the 13 words of every allocated brick (level 0 fixed-point
+sums, coarser levels float prefilters). The cascades, first-hit ids and SH that read the layer then meet the plain
+bars (ids exact, radiance within 1e-4; FP16 exact). A small hand-made world with one dynamic square and a pool too
+small for it covers the dropped-brick path; the batched, graph-replayed path gives the single-world path's layer and
+SH bit for bit."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import rtgi
+from oracle.oracle import DynState, Oracle
+from test_oracle import dyn_world
+
+pytestmark = pytest.mark.gpu
+RTOL, ATOL = 1e-4, 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def check_layer(w, d):
+    got = w.read_dynamic()
+    n = int(d.hdr[8])
+    assert np.array_equal(got["hdr"], d.hdr)
+    assert np.array_equal(got["ptr"], d.ptr)
+    assert np.array_equal(got["meta"][:n], d.meta[:n])
+    assert np.array_equal(got["mask"][:n], d.mask[:n])
+    assert np.array_equal(got["data"][:n], d.data[:n])
+    c = w.read_dyn_counters()
+    assert c == dict(bricks=n, brickOverflow=int(d.hdr[1]), outside=int(d.hdr[9]))
+    return n
+
+
+def check_update(w, o, rc, fp16):
+    rgba, hits = o.trace()
+    m, sh = o.merge_sh(rgba)
+    per_m, per_h = o.split(m), o.split(hits)
+    for c in range(rc.cascades):
+        g, h = w.read_cascade(c, hits=True)
+        assert np.array_equal(h, per_h[c])
+        if fp16:
+            assert np.array_equal(g, per_m[c])
+        else:
+            np.testing.assert_allclose(g, per_m[c], rtol=RTOL, atol=ATOL)
+    if fp16:
+        assert np.array_equal(w.read_sh(), sh)
+    else:
+        np.testing.assert_allclose(w.read_sh(), sh, rtol=RTOL, atol=ATOL)
+
+
+def atrium_frames(n=3, step=7):
+    """The atrium at scene frames 0, step, 2 step, ... with the eye moving by 0.6 m a frame along x."""
+    _, vox, rc, f0 = rtgi.config("atrium")
+    out = []
+    for k in range(n):
+        scene = rtgi.generate_scene("atrium", seed=1, frame=k * step)
+        out.append((scene, dataclasses.replace(f0, eye=(f0.eye[0] + 0.6 * k, f0.eye[1], f0.eye[2]))))
+    return vox, rc, out
+
+
+@pytest.mark.parametrize("fp16,reloc,sun", [(False, False, False), (True, True, True)])
+def test_dynamic_layer_atrium_matches_oracle(require_gpu, fp16, reloc, sun):
+    vox, rc, frames = atrium_frames()
+    first = frames[0][0].firstDynamicTri
+    assert 0 < first < frames[0][0].ntris
+    rcf = dataclasses.replace(rc, fp16=fp16, relocate=reloc, sunTrace=sun, recordHits=True)
+    w = rtgi.World(vox, rcf)
+    try:
+        w.upload_scene(frames[0][0])
+        w.set_dynamic(first, 512)
+        d = DynState(vox, 512, first)
+        moved = []
+        for k, (scene, f) in enumerate(frames):
+            say(f"case: dynamic layer, atrium, fp16={fp16}, relocate={reloc}, sunTrace={sun}, update {k + 1}")
+            if k:
+                w.update_vertices(0, scene.pos, scene.nrm)
+            w.set_frame(f)
+            w.encode()
+            w.sync()
+            o = Oracle(vox, rcf, f)
+            o.build(scene, dyn=d)
+            n = check_layer(w, d)
+            assert n > 0 and d.hdr[1] == 0
+            moved.append(d.data[:n].copy())
+            check_update(w, o, rc, fp16)
+        assert not np.array_equal(moved[0], moved[-1])
+    finally:
+        w.destroy()
+
+
+def test_dynamic_layer_closed_form_world_and_overflow(require_gpu):
+    """The hand-made world of one dynamic square (area 1 m^2, 12 bricks over 3 levels): the layer, the rays that see
+    it and the cascades match the oracle, with a pool of 64 slots and with 6 (coarse-first allocation, level 0
+    dropped), and after the square moves 2 m."""
+    scene, vox, rc, frame = dyn_world()
+    for cap in (64, 6):
+        rcf = dataclasses.replace(rc, recordHits=True)
+        w = rtgi.World(vox, rcf)
+        try:
+            w.upload_scene(scene)
+            w.set_dynamic(scene.firstDynamicTri, cap)
+            d = DynState(vox, cap, scene.firstDynamicTri)
+            for k, zw in enumerate((0.0, 2.0)):
+                say(f"case: dynamic layer, one square, pool {cap}, z = {zw}")
+                sc = dyn_world(zw=zw)[0]
+                if k:
+                    w.update_vertices(0, sc.pos, sc.nrm)
+                w.set_frame(frame)
+                w.encode()
+                w.sync()
+                o = Oracle(vox, rcf, frame)
+                o.build(sc, dyn=d)
+                n = check_layer(w, d)
+                assert n == min(cap, 12 if zw == 0 else 10)
+                check_update(w, o, rc, False)
+                org = np.array([[0.6125, 0.05 * j - 0.3, zw + 0.02 * j - 0.2] for j in range(16)], np.float32)
+                dirs = np.tile(np.array([[-1.0, 0.0, 0.0]], np.float32), (16, 1))
+                gr, gh = w.trace_rays(0, org, dirs)
+                orr, oh = o.trace_rays(0, org, dirs)
+                assert np.array_equal(gh, oh)
+                np.testing.assert_allclose(gr, orr, rtol=RTOL, atol=ATOL)
+                if cap == 64:
+                    assert np.all(gr[:, 3] < 0.5)
+        finally:
+            w.destroy()
+
+
+def test_dynamic_batch_matches_single(require_gpu):
+    vox, rc, frames = atrium_frames(n=4, step=5)
+    first = frames[0][0].firstDynamicTri
+    rcf = dataclasses.replace(rc, fp16=True, relocate=True, sunTrace=True, bounce=True)
+    say("case: dynamic layer, atrium, single world vs batch of 2 (graph), 4 updates")
+    single = rtgi.World(vox, rcf)
+    pair = [rtgi.World(vox, rcf) for _ in range(2)]
+    for w in [single] + pair:
+        w.upload_scene(frames[0][0])
+        w.set_dynamic(first, 512)
+    b = rtgi.Batch(pair, graph=True)
+    try:
+        for k, (scene, f) in enumerate(frames):
+            for w in [single] + pair:
+                if k:
+                    w.update_vertices(0, scene.pos, scene.nrm)
+                w.set_frame(f)
+            single.encode()
+            single.sync()
+            b.encode()
+            b.sync()
+            ref, lay = single.read_sh(), single.read_dynamic()
+            n = int(lay["hdr"][8])
+            for w in pair:
+                assert np.array_equal(w.read_sh(), ref)
+                got = w.read_dynamic()
+                assert np.array_equal(got["hdr"], lay["hdr"]) and np.array_equal(got["ptr"], lay["ptr"])
+                assert np.array_equal(got["data"][:n], lay["data"][:n])
+    finally:
+        b.destroy()
+        for w in [single] + pair:
+            w.destroy()

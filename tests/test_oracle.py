@@ -16,7 +16,7 @@ from scipy.optimize import linprog
 import rtgi
 from conftest import random_scene
 from oracle import oracle as orc
-from oracle.oracle import Oracle
+from oracle.oracle import DynState, Oracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -633,6 +633,154 @@ def test_bounce_volume_floor_closed_form():
     assert np.all(A2[~kept][:, 3, :3] > A[~kept][:, 3, :3])   # floor hits add the bounce
     assert np.array_equal(A2[~kept][:, :, 3], A[~kept][:, :, 3])  # reach does not depend on light
     assert np.array_equal(A2[~kept][:, 2, :3], A[~kept][:, 2, :3])  # the sky side is unchanged
+
+
+def _quad(c, u, v):
+    c, u, v = (np.asarray(x, np.float64) for x in (c, u, v))
+    return [c - u - v, c + u - v, c + u + v, c - u + v]
+
+
+def dyn_world(xw=0.3125, zw=0.0, xs=-0.6):
+    """A static wall (6 m square in the plane x = xs, facing +x) and, as the scene's dynamic part, a 1 m square in the
+    plane x = xw (0.3125: the centre plane of a column of v0 = 0.125 voxels) facing +x, albedo (0.8, 0.4, 0.2)."""
+    pos = np.array(_quad([xs, 0, 0], [0, 3, 0], [0, 0, 3]) + _quad([xw, 0, zw], [0, 0.5, 0], [0, 0, 0.5]), np.float32)
+    nrm = np.tile(np.array([[1.0, 0, 0]], np.float32), (8, 1))
+    tri = np.array([[0, 1, 2, 0], [0, 2, 3, 0], [4, 5, 6, 1], [4, 6, 7, 1]], np.uint32)
+    mat = np.array([[0.5, 0.5, 0.5, 1], [0.8, 0.4, 0.2, 1]], np.float32)
+    scene = rtgi.Scene(pos=pos, nrm=nrm, tri=tri, materials=mat, firstDynamicTri=2)
+    vox = rtgi.VoxParams(v0=0.125, levels=3, G=8, maxBricks=3 * 512, maxVoxels=1 << 16)
+    rc = rtgi.RcParams(cascades=2, probes0=16, s0=0.5)
+    return scene, vox, rc, rtgi.Frame(eye=(0.0, 0.0, 0.0), sunDir=(1.0, 0.0, 0.0))
+
+
+def _level_sums(d, L, per=512):
+    """Sums over the allocated voxels of level L of each of the 13 words (level 0: fixed point, 1/65536)."""
+    slots = np.nonzero(d.meta[:d.hdr[8]] // per == L)[0]
+    w = d.data[slots]
+    if L == 0:
+        return w.astype(np.float64).sum(axis=(0, 1)) / 65536.0
+    return w.view(np.float32).astype(np.float64).sum(axis=(0, 1))
+
+
+def test_dynamic_layer_closed_forms():
+    """The dynamic coverage layer (dynlayer.ts) for one 1 m square facing +x, lit head-on by the sun.
+    (1) Area: the splat spreads each sample's area over 8 voxels with weights summing to 1, so the level-0 coverage
+        words sum to A / v0^2 = 64 (up to fixed-point rounding), all of it on the +x side word; each coarser level
+        holds a quarter of the level below (own 0 + children / 4), i.e. A / v_L^2.
+    (2) Bricks: the square with a v0 margin covers 1 x 2 x 2 bricks at every level (brick edges at multiples of
+        1, 2, 4 m), allocated coarse level first; its bounds are floor/ceil of (min - 0.5, max + 0.5) * 100.
+    (3) Radiance: every voxel holds albedo / pi * sunE (n . l = 1).
+    (4) A ray hitting the square's front sees colour = coverage * that radiance; from behind it sees nothing.
+    (5) Composition with a static hit behind: colour = c R + (1 - c) S with one c for all three channels.
+    (6) A square behind the static wall changes nothing; the static bricks are those of the static triangles alone."""
+    scene, vox, rc, frame = dyn_world()
+    o = Oracle(vox, rc, frame)
+    d = DynState(vox, 64, scene.firstDynamicTri)
+    o.build(scene, dyn=d)
+    h = d.hdr
+    # fixed-point rounding: at most 1/2^17 per addition, 2 x 12^2 samples x 8 voxels -> 0.0176 on the level-0 sum
+    sums = [_level_sums(d, L) for L in range(3)]
+    np.testing.assert_allclose([sums[0][6], sums[0][0]], [64.0, 64.0], atol=0.0176)
+    for L in range(3):
+        assert np.all(sums[L][1:6] == 0) and np.all(sums[L][10:] == 0)
+        if L > 0:
+            np.testing.assert_allclose(sums[L][[0, 6]], sums[L - 1][[0, 6]] / 4, rtol=1e-6)
+    assert h[0] == 12 and h[8] == 12 and h[1] == 0 and h[9] == 0 and h[10] == 1
+    assert list(h[16:19]) == [8, 4, 0] and list(h[24:27]) == [4, 4, 4]
+    B = 1 << 30
+    assert list(h[2:8]) == [B - 19, B - 100, B - 100, B + 82, B + 100, B + 100]
+    per = vox.G ** 3
+    alloc = d.meta[:h[8]]
+    assert np.all(d.ptr.reshape(-1)[alloc] == np.arange(h[8]))
+    assert np.sum(d.ptr != 0xFFFFFFFF) == 12
+    lev0 = np.nonzero(alloc // per == 0)[0]
+    w = d.data[lev0].astype(np.float64)
+    m = w[..., 6] > 0.01 * 65536
+    R = np.array([0.8, 0.4, 0.2]) / np.pi * np.array(frame.sunE)
+    np.testing.assert_allclose(w[..., 7:10][m] / w[..., 6][m, None], np.tile(R, (m.sum(), 1)), rtol=2e-3)
+
+    rng = np.random.default_rng(5)
+    n = 64
+    yz = rng.uniform(-0.35, 0.35, (n, 2))
+    mx = np.tile(np.array([[-1.0, 0, 0]], np.float32), (n, 1))
+    front = np.stack([np.full(n, 0.6125), yz[:, 0], yz[:, 1]], 1).astype(np.float32)
+    r, hid = o.trace_rays(0, front, mx)
+    cov = 1.0 - r[:, 3]
+    assert np.all(cov > 0.5) and np.all(hid == 0xFFFFFFFF)
+    np.testing.assert_allclose(r[:, :3] / cov[:, None], np.tile(R, (n, 1)), rtol=1e-3)
+    back = front.copy()
+    back[:, 0] = 0.0125
+    rb, _ = o.trace_rays(0, back, -mx)
+    assert np.all(rb == np.array([0, 0, 0, 1], np.float32))
+
+    static_only = dataclasses.replace(scene, tri=scene.tri[:2], firstDynamicTri=None)
+    plain = Oracle(vox, rc, frame)
+    plain.build(static_only)
+    assert np.array_equal(plain.buf, o.buf)
+    far = front.copy()
+    far[:, 0] = 1.0
+    rs, hs = plain.trace_rays(1, far, mx)
+    rd, hd = o.trace_rays(1, far, mx)
+    assert np.array_equal(hs, hd) and np.all(hs != 0xFFFFFFFF)
+    c = (rd[:, 0] - rs[:, 0]) / (R[0] - rs[:, 0])
+    assert np.all((c > 0.5) & (c <= 1.0 + 1e-5))
+    np.testing.assert_allclose(rd[:, :3], c[:, None] * R + (1 - c[:, None]) * rs[:, :3], atol=2e-4)
+    hidden_scene = dyn_world(xw=-1.1875)[0]
+    hidden = Oracle(vox, rc, frame)
+    hidden.build(hidden_scene, dyn=DynState(vox, 64, 2))
+    rh, hh = hidden.trace_rays(1, far, mx)
+    assert np.array_equal(rh, rs) and np.array_equal(hh, hs)
+
+
+def test_log2_series_against_float64():
+    """The level blend's log2 (a fixed sequence of float operations, so that the device gives the same bits) against
+    numpy's float64 log2: within 2^-22 absolute plus one float step of the result over [1, 2^10] (the blend's range:
+    cone width over v0), exact at powers of two, and increasing."""
+    lib = orc.lib()
+    x = np.concatenate([np.geomspace(1.0, 1024.0, 20001), 2.0 ** np.arange(11), np.sqrt(2.0) ** np.arange(21)])
+    x = x.astype(np.float32)
+    got = np.array([lib.orc_log2(float(v)) for v in x], np.float64)
+    np.testing.assert_allclose(got, np.log2(x.astype(np.float64)), rtol=2.0 ** -23, atol=2.0 ** -22)
+    assert np.array_equal(got[20001:20012], np.arange(11.0))
+    s = np.sort(x)
+    g = np.array([lib.orc_log2(float(v)) for v in s])
+    assert np.all(np.diff(g) >= 0)
+
+
+def test_dynamic_layer_moves_and_overflows():
+    """Across updates the layer keeps only this update's bricks: moving the square 2 m along z frees the old bricks
+    (their pointers return to empty) and allocates the new ones, with the same area. With 6 slots for the
+    12 bricks, allocation is coarse level first: level 2 gets its 4, level 1 2 of 4, level 0 none (marked dropped).
+    The splats land on level 0 only, so they are lost with its bricks; the coarser levels, which only gather
+    children, stay empty, and a ray reads nothing (dynlayer.ts: splats into dropped bricks are skipped)."""
+    scene, vox, rc, frame = dyn_world()
+    o = Oracle(vox, rc, frame)
+    d = DynState(vox, 64, 2)
+    o.build(scene, dyn=d)
+    before = set(np.nonzero(d.ptr.reshape(-1) != 0xFFFFFFFF)[0].tolist())
+    o.build(dyn_world(zw=2.0)[0], dyn=d)
+    after = set(np.nonzero(d.ptr.reshape(-1) != 0xFFFFFFFF)[0].tolist())
+    # at z in [1.375, 2.625]: 2 level-0 bricks along z, 2 of level 1, 1 of level 2 (edge at z = 0 and 4)
+    assert len(after) == 10 and d.hdr[0] == 10 and list(d.hdr[24:27]) == [4, 4, 2] and d.hdr[10] == 2
+    assert before != after and set(d.meta[:10].tolist()) == after
+    np.testing.assert_allclose(_level_sums(d, 0)[6], 64.0, atol=0.0176)
+
+    small = Oracle(vox, rc, frame)
+    ds = DynState(vox, 6, 2)
+    small.build(scene, dyn=ds)
+    h = ds.hdr
+    assert h[0] == 12 and h[8] == 6 and h[1] == 6
+    assert list(h[16:19]) == [6, 4, 0] and list(h[24:27]) == [0, 2, 4]
+    per = vox.G ** 3
+    ptr = ds.ptr.reshape(-1)
+    assert np.sum(ptr[:per] == 0xFFFFFFFD) == 4 and np.sum(ptr[per:2 * per] == 0xFFFFFFFD) == 2
+    assert np.sum(ptr[2 * per:] < 6) == 4
+    n = 32
+    yz = np.random.default_rng(6).uniform(-0.35, 0.35, (n, 2))
+    front = np.stack([np.full(n, 0.6125), yz[:, 0], yz[:, 1]], 1).astype(np.float32)
+    r, _ = small.trace_rays(0, front, np.tile(np.array([[-1.0, 0, 0]], np.float32), (n, 1)))
+    assert np.all(r == np.array([0, 0, 0, 1], np.float32))
+    assert np.all(ds.data[:6] == 0) and np.all(ds.mask[:6] == 0)
 
 
 def test_trace_empty_world_and_single_voxel():
