@@ -282,6 +282,59 @@ def full_bench(a, scenes, vox, rc, frame0, tuning, bsz):
             "extras": "relocation + occlusion-aware parents, traced sun visibility, bounce volume"}
 
 
+def dyn_bench(a):
+    """The atrium, whose 16 moving boxes are the tail of its triangle table, with the dynamic coverage layer: each
+    update splats the boxes into the layer (mark, parents, allocate, clear, splat, resolve) and the cascade rays read
+    it in front of their static hits. Against the same worlds with the boxes voxelised with the rest. Batched hipGraph
+    updates of a.dyn_worlds atrium worlds (seeds 1..n), fused per-cascade trace+merge."""
+    import dataclasses
+
+    import rtgi
+
+    _, vox, rc, f0 = rtgi.config("atrium")
+    scenes = [rtgi.generate_scene("atrium", seed=s) for s in range(1, a.dyn_worlds + 1)]
+    out = {}
+    for layer in (False, True):
+        worlds = []
+        for sc in scenes:
+            w = rtgi.World(vox, rc)
+            w.upload_scene(sc)
+            if layer:
+                w.set_dynamic(sc.firstDynamicTri, 512)
+            w.set_frame(dataclasses.replace(f0, eye=sc.eye, sunDir=sc.sunDir))
+            w.set_tuning(fused=1)
+            worlds.append(w)
+        b = rtgi.Batch(worlds, graph=True)
+        for _ in range(2):
+            b.encode()
+            b.sync()
+        steps = max(1, a.steps)
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.encode()
+        b.sync()
+        value = steps * len(worlds) / (time.perf_counter() - t)
+        b.set_timing(True)
+        ms = {}
+        for _ in range(steps):
+            b.encode()
+            b.sync()
+            for k, v in b.stage_ms().items():
+                ms[k] = ms.get(k, 0.0) + v / steps
+        key = "dynamic_layer" if layer else "boxes_in_bricks"
+        out[key] = {"updates_per_s": round(value, 3), "stage_ms_per_batch": {k: round(v, 4) for k, v in ms.items()}}
+        if layer:
+            out[key]["layer_bricks_world1"] = worlds[0].read_dyn_counters()["bricks"]
+        b.destroy()
+        for w in worlds:
+            w.destroy()
+    log(f"dynamic layer: {out['dynamic_layer']['updates_per_s']:.1f} updates/s, boxes in bricks "
+        f"{out['boxes_in_bricks']['updates_per_s']:.1f}")
+    return {"value": out["dynamic_layer"]["updates_per_s"], "unit": "GI updates/s", "worlds": len(scenes),
+            "config": "atrium (55,724 triangles, 1,292 of them the 16 moving boxes), FP32, 512 layer bricks",
+            **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +349,7 @@ def main():
     ap.add_argument("--mix-worlds", type=int, default=32, help="batched mixed-world run (0: skip it)")
     ap.add_argument("--full-worlds", type=int, default=64,
                     help="run with relocation, traced sun visibility and multi-bounce on (0: skip it)")
+    ap.add_argument("--dyn-worlds", type=int, default=16, help="atrium worlds of the dynamic-layer run (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
     ap.add_argument("--serve-worlds", type=int, default=64)
@@ -431,6 +485,7 @@ def main():
         w.destroy()
     full = (full_bench(a, scenes, first[1], first[2], first[3], worlds_tuning, bsz) if a.full_worlds > 0 else None)
     mixed = mix_bench(a) if a.mix_worlds > 0 else None
+    dynamic = dyn_bench(a) if a.dyn_worlds > 0 else None
     serving = serve_bench(a) if a.serve_seconds > 0 else None
 
     cpu = None
@@ -479,6 +534,7 @@ def main():
                      "valu_issue_busy_profiled": valu_busy},
         "cpu_baseline": cpu,
         "full_default": full,
+        "dynamic_layer": dynamic,
         "batched_mix": mixed,
         "serving": serving,
     }

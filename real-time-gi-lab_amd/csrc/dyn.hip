@@ -15,9 +15,8 @@
 //                                                                       (splat, splatTri, splatSample, :392-540)
 //   k_dl_resolve  level L = 1..: own splats + 1/4 of each child with a value, as floats    (resolve, :542-580)
 //
-// The reference's splat also stages a workgroup's samples in a window of workgroup memory; fixed-point sums give
-// the same words either way, so this build adds straight to the layer. Sample shading uses the frame's sun
-// visibility (the reference taps its raster shadow map) and untextured albedo (baseColorFactor).
+// Sample shading uses the frame's sun visibility (the reference taps its raster shadow map) and untextured albedo
+// (baseColorFactor).
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
@@ -66,35 +65,79 @@ __device__ __forceinline__ void want(const VoxK& k, uint32_t* cur, int L, i3 b) 
   atomicOr(&cur[g >> 5], 1u << (g & 31u));
 }
 
-__device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+  for (int s = 32; s > 0; s >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, s));
+  return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+  for (int s = 32; s > 0; s >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, s));
+  return x;
+}
+
+// The block's rounds are uniform (i0 < n), so the wavefront reductions of the bounds see every lane; one lane per
+// wavefront then takes the min / max into the header (atomic min and max commute, so the result is the per-triangle
+// atomics' result). The wanted bits gather in a bitset in LDS (when the world's bitset fits MARK_LDS words) and
+// are ORed into the layer's bitset once per block: the triangles of a block want the same few bricks.
+constexpr uint32_t MARK_LDS = 4096;
+__device__ __forceinline__ void mark_body(const VoxK& k, uint32_t* lbits, uint32_t lb, uint32_t nblk) {
   const DynK& D = k.dyn;
-  uint32_t* cur = cur_bits(D);
+  const bool in_lds = D.nw <= MARK_LDS;
+  uint32_t* cur = in_lds ? lbits : cur_bits(D);
   const uint32_t n = D.end_tri - D.first_tri;
-  for (uint32_t i = lb * BLK + threadIdx.x; i < n; i += nblk * BLK) {
-    Tri T = load_tri(k, D.first_tri + i);
-    f3 mn = min3v(min3v(T.a, T.b), T.c), mx = max3v(max3v(T.a, T.b), T.c);
-    int Ls = splat_level(k, mn, mx);
-    if (Ls < 0) {
-      atomicAdd(&D.hdr[38], 1u);
-      continue;
+  if (in_lds) {
+    for (uint32_t w = threadIdx.x; w < D.nw; w += BLK) lbits[w] = 0u;
+    __syncthreads();
+  }
+  for (uint32_t i0 = lb * BLK; i0 < n; i0 += nblk * BLK) {
+    const uint32_t i = i0 + threadIdx.x;
+    uint32_t lo3[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi3[3] = {0u, 0u, 0u};
+    uint32_t outside = 0u;
+    if (i < n) {
+      Tri T = load_tri(k, D.first_tri + i);
+      f3 mn = min3v(min3v(T.a, T.b), T.c), mx = max3v(max3v(T.a, T.b), T.c);
+      int Ls = splat_level(k, mn, mx);
+      if (Ls < 0) {
+        outside = 1u;
+      } else {
+        float v = vox_size(k, Ls), bs = v * 8.0f;
+        f3 lo = region_lo(k, Ls);
+        f3 q0 = (mn - splat3(v) - lo) / bs, q1 = (mx + splat3(v) - lo) / bs;
+        i3 g0 = f2i3(floor3(q0)), g1 = f2i3(floor3(q1));
+        for (int z = g0.z; z <= g1.z; z++)
+          for (int y = g0.y; y <= g1.y; y++)
+            for (int x = g0.x; x <= g1.x; x++) want(k, cur, Ls, i3{x, y, z});
+        // Bounds of everything a read can see: the box grown by the coarsest voxel (cm, offset by 2^30).
+        float gv = vox_size(k, k.NL - 1);
+        f3 blo = floor3((mn - splat3(gv)) * 100.0f), bhi = mx + splat3(gv);
+        bhi = mk3(ceilf(bhi.x * 100.0f), ceilf(bhi.y * 100.0f), ceilf(bhi.z * 100.0f));
+        lo3[0] = (uint32_t)(f2i(blo.x) + 1073741824);
+        lo3[1] = (uint32_t)(f2i(blo.y) + 1073741824);
+        lo3[2] = (uint32_t)(f2i(blo.z) + 1073741824);
+        hi3[0] = (uint32_t)(f2i(bhi.x) + 1073741824);
+        hi3[1] = (uint32_t)(f2i(bhi.y) + 1073741824);
+        hi3[2] = (uint32_t)(f2i(bhi.z) + 1073741824);
+      }
     }
-    float v = vox_size(k, Ls), bs = v * 8.0f;
-    f3 lo = region_lo(k, Ls);
-    f3 q0 = (mn - splat3(v) - lo) / bs, q1 = (mx + splat3(v) - lo) / bs;
-    i3 g0 = f2i3(floor3(q0)), g1 = f2i3(floor3(q1));
-    for (int z = g0.z; z <= g1.z; z++)
-      for (int y = g0.y; y <= g1.y; y++)
-        for (int x = g0.x; x <= g1.x; x++) want(k, cur, Ls, i3{x, y, z});
-    // Bounds of everything a read can see: the box grown by the coarsest voxel (cm, offset by 2^30).
-    float gv = vox_size(k, k.NL - 1);
-    f3 blo = floor3((mn - splat3(gv)) * 100.0f), bhi = mx + splat3(gv);
-    bhi = mk3(ceilf(bhi.x * 100.0f), ceilf(bhi.y * 100.0f), ceilf(bhi.z * 100.0f));
-    atomicMin(&D.hdr[32], (uint32_t)(f2i(blo.x) + 1073741824));
-    atomicMin(&D.hdr[33], (uint32_t)(f2i(blo.y) + 1073741824));
-    atomicMin(&D.hdr[34], (uint32_t)(f2i(blo.z) + 1073741824));
-    atomicMax(&D.hdr[35], (uint32_t)(f2i(bhi.x) + 1073741824));
-    atomicMax(&D.hdr[36], (uint32_t)(f2i(bhi.y) + 1073741824));
-    atomicMax(&D.hdr[37], (uint32_t)(f2i(bhi.z) + 1073741824));
+    const uint64_t out_mask = __ballot(outside != 0u);
+    for (int a = 0; a < 3; a++) {
+      lo3[a] = wave_min(lo3[a]);
+      hi3[a] = wave_max(hi3[a]);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+      if (out_mask) atomicAdd(&D.hdr[38], (uint32_t)__popcll(out_mask));
+      if (lo3[0] != 0xffffffffu) {
+        for (int a = 0; a < 3; a++) {
+          atomicMin(&D.hdr[32 + a], lo3[a]);
+          atomicMax(&D.hdr[35 + a], hi3[a]);
+        }
+      }
+    }
+  }
+  if (in_lds) {
+    __syncthreads();
+    uint32_t* g = cur_bits(D);
+    for (uint32_t w = threadIdx.x; w < D.nw; w += BLK)
+      if (lbits[w]) atomicOr(&g[w], lbits[w]);
   }
 }
 
@@ -216,23 +259,47 @@ __device__ __forceinline__ void alloc_body(const VoxK& k) {
 __device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   const DynK& D = k.dyn;
   const uint32_t al = D.hdr[8];
-  for (uint32_t i = lb * BLK + threadIdx.x; i < al * 512u; i += nblk * BLK) {
-    uint32_t* d = D.data + (size_t)i * DYN_WORDS;
-    for (uint32_t q = 0; q < DYN_WORDS; q++) d[q] = 0u;
-    if ((i & 511u) < 16u) D.mask[(i >> 9) * 16u + (i & 511u)] = 0u;
-  }
+  // The allocated slots are [0, al): one contiguous range of 512 x 13 words (whole 16-byte units) per slot.
+  uint4* d4 = reinterpret_cast<uint4*>(D.data);
+  const uint32_t nq = al * (512u * DYN_WORDS / 4u);
+  for (uint32_t i = lb * BLK + threadIdx.x; i < nq; i += nblk * BLK) d4[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (uint32_t i = lb * BLK + threadIdx.x; i < al * 16u; i += nblk * BLK) D.mask[i] = 0u;
 }
 
 // u32(round(x * FIX)) of WGSL for x >= 0 (round half to even; saturating as WGSL's conversion).
 __device__ __forceinline__ uint32_t fix(float x) { return (uint32_t)fminf(rintf(x * DYN_FIX), 4294967040.0f); }
-__device__ __forceinline__ void add_fix(uint32_t* d, uint32_t q, float x) {
-  uint32_t v = fix(x);
-  if (v > 0u) atomicAdd(&d[q], v);
+
+// The splat window (dynlayer.ts:398-540): a workgroup of SPLAT_TPW consecutive dynamic triangles (one wavefront,
+// SPLAT_LPT lanes per triangle)
+// accumulates its samples' fixed-point adds in workgroup memory over the voxel box of its triangles at their finest
+// splat level, then adds each touched voxel's words to the layer once. Neighbouring triangles hit the same voxels,
+// so this replaces contended device atomics with LDS atomics. Corners outside the window (other levels, or a box
+// larger than SPLAT_WIN voxels) add straight to the layer. Fixed-point sums are exact, so the words are the same.
+constexpr uint32_t SPLAT_WG = 64;
+constexpr uint32_t SPLAT_LPT = 4;                     // lanes per triangle, over its samples
+constexpr uint32_t SPLAT_TPW = SPLAT_WG / SPLAT_LPT;  // triangles per workgroup
+constexpr uint32_t SPLAT_WIN = 512;  // voxels: 512 x 10 words = 20 KiB of LDS
+constexpr uint32_t SPLAT_W = 10;     // words splatted (the albedo words 10..12 stay 0)
+
+struct SplatWin {
+  uint32_t acc[SPLAT_WIN * SPLAT_W];
+  uint32_t touched[SPLAT_WIN / 32];
+};
+
+__device__ __forceinline__ int wave_min_i(int x) {
+  for (int s = 32; s > 0; s >>= 1) x = min(x, __shfl_xor(x, s));
+  return x;
+}
+__device__ __forceinline__ int wave_max_i(int x) {
+  for (int s = 32; s > 0; s >>= 1) x = max(x, __shfl_xor(x, s));
+  return x;
 }
 
 // splatSample (dynlayer.ts:404-438): trilinear weights over the 8 voxel centres nearest to p (level L), each adding
-// its projected areas (front and back per axis), area and area x radiance in units of v^2.
-__device__ void splat_sample(const VoxK& k, int L, f3 p, f3 n, float area, f3 rad) {
+// its projected areas (front and back per axis), area and area x radiance in units of v^2; into the window when the
+// corner lies in it (level wl, voxels wo .. wo + wd - 1), else into the layer.
+__device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, int L, f3 p, f3 n, float area,
+                             f3 rad) {
   const DynK& D = k.dyn;
   float v = vox_size(k, L);
   f3 g = (p - region_lo(k, L)) / v - splat3(0.5f);
@@ -249,50 +316,88 @@ __device__ void splat_sample(const VoxK& k, int L, f3 p, f3 n, float area, f3 ra
     f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
     float w = tw.x * tw.y * tw.z * s;
     if (w <= 0.0f) continue;
+    const uint32_t x[SPLAT_W] = {fix(w * pn.x), fix(w * nn.x), fix(w * pn.y), fix(w * nn.y), fix(w * pn.z),
+                                 fix(w * nn.z), fix(w),        fix(w * rad.x), fix(w * rad.y), fix(w * rad.z)};
+    i3 r = c - wo;
+    if (L == wl && r.x >= 0 && r.y >= 0 && r.z >= 0 && r.x < wd.x && r.y < wd.y && r.z < wd.z) {
+      uint32_t j = (uint32_t)(r.x + wd.x * (r.y + wd.y * r.z));
+      atomicOr(&W.touched[j >> 5], 1u << (j & 31u));
+      for (uint32_t q2 = 0; q2 < SPLAT_W; q2++)
+        if (x[q2]) atomicAdd(&W.acc[j * SPLAT_W + q2], x[q2]);
+      continue;
+    }
     i3 b = shr3(c, 3);
     uint32_t ptr = D.ptr[(uint32_t)L * (uint32_t)k.per + ((uint32_t)b.z * k.G + (uint32_t)b.y) * k.G + (uint32_t)b.x];
     if (ptr >= (uint32_t)D.cap) continue;
     uint32_t vi = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
     atomicOr(&D.mask[ptr * 16u + (vi >> 5)], 1u << (vi & 31u));
     uint32_t* d = D.data + ((size_t)ptr * 512u + vi) * DYN_WORDS;
-    add_fix(d, 0, w * pn.x);
-    add_fix(d, 1, w * nn.x);
-    add_fix(d, 2, w * pn.y);
-    add_fix(d, 3, w * nn.y);
-    add_fix(d, 4, w * pn.z);
-    add_fix(d, 5, w * nn.z);
-    add_fix(d, 6, w);
-    add_fix(d, 7, w * rad.x);
-    add_fix(d, 8, w * rad.y);
-    add_fix(d, 9, w * rad.z);
+    for (uint32_t q2 = 0; q2 < SPLAT_W; q2++)
+      if (x[q2]) atomicAdd(&d[q2], x[q2]);
   }
 }
 
 // splatTri (dynlayer.ts:461-492): m^2 equal sub-triangles (m = ceil(longest edge / voxel) in 1..16), each a sample at
-// its centroid with its area; shadeSample (:440-452) with untextured albedo and the frame's sun visibility.
-__device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, uint32_t lb, uint32_t nblk) {
+// its centroid with its area; shadeSample (:440-452) with untextured albedo and the frame's sun visibility. The rounds
+// of the loop are uniform over the wavefront (i0 < n), so the window reductions see every lane.
+__device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, SplatWin& W, uint32_t lb, uint32_t nblk) {
   const DynK& D = k.dyn;
-  const uint32_t n = D.end_tri - D.first_tri;
+  const uint32_t n = D.end_tri - D.first_tri, lane = threadIdx.x, sub = lane % SPLAT_LPT;
   const f3 sd = mk3(t.sun_dir[0], t.sun_dir[1], t.sun_dir[2]), se = mk3(t.sun_e[0], t.sun_e[1], t.sun_e[2]);
-  for (uint32_t i = lb * BLK + threadIdx.x; i < n; i += nblk * BLK) {
-    Tri T = load_tri(k, D.first_tri + i);
-    int L = splat_level(k, min3v(min3v(T.a, T.b), T.c), max3v(max3v(T.a, T.b), T.c));
-    if (L < 0) continue;
-    f3 cr = cross3(T.b - T.a, T.c - T.a);
-    float area = 0.5f * sqrtf(dot3(cr, cr));
-    if (area <= 0.0f) continue;
-    f3 e0 = T.b - T.a, e1 = T.c - T.b, e2 = T.a - T.c;
-    float e = fmaxf(fmaxf(sqrtf(dot3(e0, e0)), sqrtf(dot3(e1, e1))), sqrtf(dot3(e2, e2)));
-    uint32_t m = (uint32_t)fminf(fmaxf(ceilf(e / vox_size(k, L)), 1.0f), 16.0f);
-    float fm = (float)m;
-    float sa = area / (fm * fm);
-    const float4 mat = reinterpret_cast<const float4*>(k.mat)[T.m];
-    const f3 alb = mk3(1.0f * mat.x, 1.0f * mat.y, 1.0f * mat.z);
-    const f3 n0 = ldp(k.nrm, T.i0), n1 = ldp(k.nrm, T.i1), n2 = ldp(k.nrm, T.i2);
-    for (uint32_t u = 0; u < m; u++) {
-      for (uint32_t v = 0; v + u < m; v++) {
-        for (uint32_t flip = 0; flip < 2u; flip++) {
-          if (flip == 1u && u + v + 1u >= m) continue;
+  for (uint32_t i0 = lb * SPLAT_TPW; i0 < n; i0 += nblk * SPLAT_TPW) {
+    const uint32_t i = i0 + lane / SPLAT_LPT;
+    Tri T{};
+    int L = -1;
+    f3 mn = splat3(0.0f), mx = splat3(0.0f);
+    if (i < n) {
+      T = load_tri(k, D.first_tri + i);
+      mn = min3v(min3v(T.a, T.b), T.c);
+      mx = max3v(max3v(T.a, T.b), T.c);
+      L = splat_level(k, mn, mx);
+    }
+    // The window: the box of the triangles at the finest splat level among them, the voxels their samples'
+    // trilinear corners can reach (floor(x / v - 0.5) .. + 1 over each triangle's box).
+    const int wl = wave_min_i(L < 0 ? 0x7fffffff : L);
+    i3 c0{0x7fffffff, 0x7fffffff, 0x7fffffff}, c1{-0x7fffffff, -0x7fffffff, -0x7fffffff};
+    if (L >= 0 && L == wl) {
+      float v = vox_size(k, wl);
+      f3 lo = region_lo(k, wl);
+      c0 = f2i3(floor3((mn - lo) / v - splat3(0.5f)));
+      c1 = f2i3(floor3((mx - lo) / v - splat3(0.5f))) + i3{1, 1, 1};
+    }
+    const i3 wo{wave_min_i(c0.x), wave_min_i(c0.y), wave_min_i(c0.z)};
+    const i3 whi{wave_max_i(c1.x), wave_max_i(c1.y), wave_max_i(c1.z)};
+    i3 wd{0, 0, 0};
+    if (wl != 0x7fffffff) {
+      const i3 e = whi - wo + i3{1, 1, 1};
+      if ((uint32_t)e.x * (uint32_t)e.y <= SPLAT_WIN)
+        wd = i3{e.x, e.y, min(e.z, (int)(SPLAT_WIN / ((uint32_t)e.x * (uint32_t)e.y)))};
+    }
+    const uint32_t vol = (uint32_t)(wd.x * wd.y * wd.z);
+    for (uint32_t j = lane; j < vol * SPLAT_W; j += SPLAT_WG) W.acc[j] = 0u;
+    for (uint32_t j = lane; j < SPLAT_WIN / 32; j += SPLAT_WG) W.touched[j] = 0u;
+    __syncthreads();
+    if (L >= 0) {
+      f3 cr = cross3(T.b - T.a, T.c - T.a);
+      float area = 0.5f * sqrtf(dot3(cr, cr));
+      if (area > 0.0f) {
+        f3 e0 = T.b - T.a, e1 = T.c - T.b, e2 = T.a - T.c;
+        float e = fmaxf(fmaxf(sqrtf(dot3(e0, e0)), sqrtf(dot3(e1, e1))), sqrtf(dot3(e2, e2)));
+        uint32_t m = (uint32_t)fminf(fmaxf(ceilf(e / vox_size(k, L)), 1.0f), 16.0f);
+        float fm = (float)m;
+        float sa = area / (fm * fm);
+        const float4 mat = reinterpret_cast<const float4*>(k.mat)[T.m];
+        const f3 alb = mk3(1.0f * mat.x, 1.0f * mat.y, 1.0f * mat.z);
+        const f3 n0 = ldp(k.nrm, T.i0), n1 = ldp(k.nrm, T.i1), n2 = ldp(k.nrm, T.i2);
+        for (uint32_t s = sub; s < m * m; s += SPLAT_LPT) {
+          // Sample s in the order u, then v < m - u, then flip (flip 1 only while u + v + 1 < m): row u holds
+          // 2 (m - u) - 1 samples, alternating flip 0 and 1 from v = 0.
+          uint32_t u = 0u, r = s;
+          while (r >= 2u * (m - u) - 1u) {
+            r -= 2u * (m - u) - 1u;
+            u++;
+          }
+          const uint32_t v = r >> 1, flip = r & 1u;
           float off = flip == 1u ? 2.0f / 3.0f : 1.0f / 3.0f;
           float wb = ((float)u + off) / fm, wc = ((float)v + off) / fm, wa = 1.0f - wb - wc;
           f3 p = T.a * wa + T.b * wb + T.c * wc;
@@ -301,10 +406,29 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, uint3
           float ndl = fmaxf(dot3(nv, sd), 0.0f);
           f3 rad = splat3(0.0f);
           if (ndl > 0.0f) rad = alb / PI_F * se * ndl * t.sun_vis;
-          splat_sample(k, L, p, nv, sa, rad);
+          splat_sample(k, W, wl, wo, wd, L, p, nv, sa, rad);
         }
       }
     }
+    __syncthreads();
+    // Add the window to the layer: every touched voxel (a corner with weight > 0 reached it, as a direct add would
+    // have set its mask bit) of a claimed brick.
+    for (uint32_t j = lane; j < vol; j += SPLAT_WG) {
+      if (!((W.touched[j >> 5] >> (j & 31u)) & 1u)) continue;
+      const i3 c = wo + i3{(int)(j % (uint32_t)wd.x), (int)((j / (uint32_t)wd.x) % (uint32_t)wd.y),
+                           (int)(j / (uint32_t)(wd.x * wd.y))};
+      const i3 b = shr3(c, 3);
+      uint32_t ptr = D.ptr[(uint32_t)wl * (uint32_t)k.per + ((uint32_t)b.z * k.G + (uint32_t)b.y) * k.G + (uint32_t)b.x];
+      if (ptr >= (uint32_t)D.cap) continue;
+      uint32_t vi = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+      atomicOr(&D.mask[ptr * 16u + (vi >> 5)], 1u << (vi & 31u));
+      uint32_t* d = D.data + ((size_t)ptr * 512u + vi) * DYN_WORDS;
+      for (uint32_t q = 0; q < SPLAT_W; q++) {
+        const uint32_t x = W.acc[j * SPLAT_W + q];
+        if (x) atomicAdd(&d[q], x);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -354,11 +478,17 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, int L, uint32_t lb, 
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_dl_mark(VoxK k) { mark_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_dl_mark(VoxK k) {
+  __shared__ uint32_t lbits[MARK_LDS];
+  mark_body(k, lbits, blockIdx.x, gridDim.x);
+}
 __global__ void __launch_bounds__(BLK) k_dl_parents(VoxK k) { parents_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(1024) k_dl_alloc(VoxK k) { alloc_body(k); }
 __global__ void __launch_bounds__(BLK) k_dl_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(BLK) k_dl_splat(TraceK t) { splat_body(t.vx, t, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(SPLAT_WG) k_dl_splat(TraceK t) {
+  __shared__ SplatWin W;
+  splat_body(t.vx, t, W, blockIdx.x, gridDim.x);
+}
 __global__ void __launch_bounds__(BLK) k_dl_resolve(VoxK k, int L) { resolve_body(k, L, blockIdx.x, gridDim.x); }
 
 #define DL_BATCHED(name, body)                                              \
@@ -367,17 +497,23 @@ __global__ void __launch_bounds__(BLK) k_dl_resolve(VoxK k, int L) { resolve_bod
     int w = batch_world(b, blk);                                            \
     body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);         \
   }
-DL_BATCHED(k_dl_mark_b, mark_body)
+__global__ void __launch_bounds__(BLK) k_dl_mark_b(BatchK b) {
+  __shared__ uint32_t lbits[MARK_LDS];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  int w = batch_world(b, blk);
+  mark_body(b.w[w].vx, lbits, blk - b.start[w], b.start[w + 1] - b.start[w]);
+}
 DL_BATCHED(k_dl_parents_b, parents_body)
 DL_BATCHED(k_dl_clear_b, clear_body)
 __global__ void __launch_bounds__(1024) k_dl_alloc_b(BatchK b) {
   int w = batch_world(b, xcd_block(blockIdx.x, gridDim.x));
   alloc_body(b.w[w].vx);
 }
-__global__ void __launch_bounds__(BLK) k_dl_splat_b(BatchK b) {
+__global__ void __launch_bounds__(SPLAT_WG) k_dl_splat_b(BatchK b) {
+  __shared__ SplatWin W;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   int w = batch_world(b, blk);
-  splat_body(b.w[w].vx, b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  splat_body(b.w[w].vx, b.w[w], W, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 __global__ void __launch_bounds__(BLK) k_dl_resolve_b(BatchK b) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -410,7 +546,7 @@ int launch_dyn_splat(const TraceK& t, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   uint32_t g[BL_COUNT];
   batch_blocks(t, g);
-  k_dl_splat<<<g[BL_DL_SPLAT], BLK, 0, s>>>(t);
+  k_dl_splat<<<g[BL_DL_SPLAT], SPLAT_WG, 0, s>>>(t);
   for (int L = 1; L < k.NL; L++) k_dl_resolve<<<g[BL_DL_RESOLVE + L - 1], BLK, 0, s>>>(k, L);
   return (int)hipGetLastError();
 }
@@ -421,7 +557,7 @@ int launch_dyn_build_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_DL_PARENTS]) k_dl_parents_b<<<p.grid[BL_DL_PARENTS], BLK, 0, s>>>(dbk(p, BL_DL_PARENTS));
   if (p.grid[BL_DL_ALLOC]) k_dl_alloc_b<<<p.grid[BL_DL_ALLOC], 1024, 0, s>>>(dbk(p, BL_DL_ALLOC));
   if (p.grid[BL_DL_CLEAR]) k_dl_clear_b<<<p.grid[BL_DL_CLEAR], BLK, 0, s>>>(dbk(p, BL_DL_CLEAR));
-  if (p.grid[BL_DL_SPLAT]) k_dl_splat_b<<<p.grid[BL_DL_SPLAT], BLK, 0, s>>>(dbk(p, BL_DL_SPLAT));
+  if (p.grid[BL_DL_SPLAT]) k_dl_splat_b<<<p.grid[BL_DL_SPLAT], SPLAT_WG, 0, s>>>(dbk(p, BL_DL_SPLAT));
   for (int L = 1; L < RTGI_MAX_LEVELS; L++)
     if (p.grid[BL_DL_RESOLVE + L - 1])
       k_dl_resolve_b<<<p.grid[BL_DL_RESOLVE + L - 1], BLK, 0, s>>>(dbk(p, BL_DL_RESOLVE + L - 1, L));

@@ -356,7 +356,8 @@ __device__ float4 trace_dyn(const TraceK& a, f3 o, f3 d, float t0, float t1, flo
     if (te > ba) {
       float len = te - t;
       f3 p = o + d * (0.5f * (t + te));
-      float lf = fminf(fmaxf(log2_series(w / v0), 0.0f), (float)(nl - 1));
+      // w = v0 gives log2(1) = 0 exactly: the series is skipped there
+      float lf = w > v0 ? fminf(fmaxf(log2_series(w / v0), 0.0f), (float)(nl - 1)) : 0.0f;
       int L0 = (int)floorf(lf);
       float f = lf - (float)L0;
       int Lw = vox_level_for(k, L0, p);
@@ -366,12 +367,12 @@ __device__ float4 trace_dyn(const TraceK& a, f3 o, f3 d, float t0, float t1, flo
       }
       uint2 fa = dyn_find(k, L0, p);
       float4 sa = dyn_read(D, fa, d);
-      float aa = sa.w * len / vox_size(k, (int)fa.y);
+      float aa = sa.w > 0.0f ? sa.w * len / vox_size(k, (int)fa.y) : 0.0f;  // 0 x len / v = 0: no division
       f3 ra = mk3(sa.x, sa.y, sa.z) * aa;
       if (f > 0.0f && L0 + 1 < nl) {
         uint2 fb = dyn_find(k, L0 + 1, p);
         float4 sb = dyn_read(D, fb, d);
-        float ab = sb.w * len / vox_size(k, (int)fb.y);
+        float ab = sb.w > 0.0f ? sb.w * len / vox_size(k, (int)fb.y) : 0.0f;
         f3 rb = mk3(sb.x, sb.y, sb.z) * ab;
         ra = ra * (1.0f - f) + rb * f;  // mix
         aa = aa * (1.0f - f) + ab * f;

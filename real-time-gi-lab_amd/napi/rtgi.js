@@ -1,7 +1,10 @@
 'use strict';
 // JS face of the Node-API addon: the reference host's pass interface over librtgi.
-//   createVoxWorld(scene, params)  <- createVoxWorld (src/core/voxel.ts:746): update(eye), encode(), readCounters(),
-//                                     usedBytes(c), destroy()
+//   createVoxWorld(scene, params, frame, firstDynamicTri)
+//                                  <- createVoxWorld (src/core/voxel.ts:746): update(eye), encode(), readCounters(),
+//                                     usedBytes(c), destroy(); with firstDynamicTri below the triangle count, the
+//                                     triangles from it on go to the dynamic coverage layer (dynBricks bricks;
+//                                     src/core/dynlayer.ts) and dyn.readCounters() reads its counters
 //   createRc(voxWorld, params)     <- createRc (src/core/rc.ts:1019): encode(), readSh(), readCascade(c), destroy(),
 //                                     gather(pos, nrm, occlusion) (gatherWorld), readProbes(c), readBounceVolume()
 // One library world backs both: the voxel world owns it, and createRc re-creates it with the cascade parameters
@@ -10,7 +13,7 @@
 const path = require('path');
 const addon = require(path.join(__dirname, 'rtgi_napi.node'));
 
-const DEFAULT_VOX = { v0: 0.2, levels: 5, G: 32, maxBricks: 4608, maxVoxels: 524288 };
+const DEFAULT_VOX = { v0: 0.2, levels: 5, G: 32, maxBricks: 4608, maxVoxels: 524288, dynBricks: 512 };
 const DEFAULT_PARAMS = { cascades: 5, probes0: 64, s0: 0.2, kInterval: 1, levelMap: 1, voxScale: 0.25, voxOffset: 1,
   voxSkip: 0.5, resCap: 8, recordHits: 0, relocate: 0, lift: 0.25, sunTrace: 0, bounce: 0, fp16: 0 };
 const SUN_E = [9, 8.5, 7.8];
@@ -23,7 +26,9 @@ function checked(defaults, o, what) {
 function voxParams(o) { return checked(DEFAULT_VOX, o, 'voxel'); }
 function rcParams(o) { return checked(DEFAULT_PARAMS, o, 'cascade'); }
 
-function createVoxWorld(scene, params, frame) {
+function createVoxWorld(scene, params, frame, firstDynamicTri) {
+  const ntris = scene.tri.length / 4;
+  const first = firstDynamicTri === undefined ? ntris : firstDynamicTri;
   const vw = {
     params: voxParams(params || {}),
     rcParams: { cascades: 1, probes0: 1, s0: 1 },
@@ -34,8 +39,10 @@ function createVoxWorld(scene, params, frame) {
     if (vw.handle) addon.destroy(vw.handle);
     vw.handle = addon.createWorld(vw.params, vw.rcParams);
     addon.uploadScene(vw.handle, scene.pos, scene.nrm, scene.tri, scene.materials);
+    if (first < ntris) addon.setDynamic(vw.handle, first, vw.params.dynBricks);
     addon.setFrame(vw.handle, vw.frame);
   };
+  vw.dyn = first < ntris ? { readCounters: () => Promise.resolve().then(() => addon.readDynCounters(vw.handle)) } : null;
   vw.update = (eye) => { vw.frame = Object.assign({}, vw.frame, { eye }); addon.setFrame(vw.handle, vw.frame); };
   vw.encode = () => addon.encode(vw.handle);
   vw.readCounters = () => Promise.resolve().then(() => addon.readCounters(vw.handle));

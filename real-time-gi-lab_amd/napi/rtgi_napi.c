@@ -359,7 +359,44 @@ static napi_value js_read_bounce_volume(napi_env env, napi_callback_info info) {
   return o;
 }
 
-/* generateScene(kind: 'atrium'|'city', seed, frame) -> {pos, nrm, tri, materials, eye, sunDir} */
+/* setDynamic(h, firstDynamicTri, dynBricks): triangles from firstDynamicTri on go to the dynamic coverage layer */
+static napi_value js_set_dynamic(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 3 ? world_of(env, argv[0]) : NULL;
+  double first = 0, bricks = 0;
+  if (!w || napi_get_value_double(env, argv[1], &first) != napi_ok ||
+      napi_get_value_double(env, argv[2], &bricks) != napi_ok) {
+    napi_throw_type_error(env, NULL, "setDynamic(h, firstDynamicTri, dynBricks)");
+    return NULL;
+  }
+  int st = rtgi_world_set_dynamic(w, (uint32_t)first, (int32_t)bricks);
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* readDynCounters(h) -> {bricks, brickOverflow, outside} (DynLayer.readCounters) */
+static napi_value js_read_dyn_counters(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], o;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 1 ? world_of(env, argv[0]) : NULL;
+  if (!w) {
+    napi_throw_type_error(env, NULL, "readDynCounters(h)");
+    return NULL;
+  }
+  uint32_t b = 0, ov = 0, out = 0;
+  int st = rtgi_world_read_dyn_counters(w, &b, &ov, &out);
+  if (st) return throw_status(env, st);
+  NAPI_CALL(env, napi_create_object(env, &o));
+  napi_set_named_property(env, o, "bricks", make_u32(env, b));
+  napi_set_named_property(env, o, "brickOverflow", make_u32(env, ov));
+  napi_set_named_property(env, o, "outside", make_u32(env, out));
+  return o;
+}
+
+/* generateScene(kind: 'atrium'|'city', seed, frame) -> {pos, nrm, tri, materials, eye, sunDir, firstDynamicTri} */
 static napi_value js_generate_scene(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -404,6 +441,7 @@ static napi_value js_generate_scene(napi_env env, napi_callback_info info) {
     napi_set_element(env, arr, (uint32_t)a, e);
   }
   napi_set_named_property(env, o, "sunDir", arr);
+  napi_set_named_property(env, o, "firstDynamicTri", make_u32(env, s->first_dynamic_tri));
   rtgi_scene_free(s);
   return o;
 }
@@ -423,6 +461,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"gather", 0, js_gather, 0, 0, 0, napi_default, 0},
       {"readProbes", 0, js_read_probes, 0, 0, 0, napi_default, 0},
       {"readBounceVolume", 0, js_read_bounce_volume, 0, 0, 0, napi_default, 0},
+      {"setDynamic", 0, js_set_dynamic, 0, 0, 0, napi_default, 0},
+      {"readDynCounters", 0, js_read_dyn_counters, 0, 0, 0, napi_default, 0},
       {"destroy", 0, js_destroy, 0, 0, 0, napi_default, 0},
       {"generateScene", 0, js_generate_scene, 0, 0, 0, napi_default, 0},
   };

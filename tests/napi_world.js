@@ -7,10 +7,11 @@ const args = process.argv.slice(2);
 const [kind, seed, v0, levels, G, cascades, probes0, s0] = args.slice(0, 8).map((x, i) => (i ? Number(x) : x));
 // 'full': the reference's default extras (voxel relocation, per-voxel sun visibility, multi-bounce), two updates, and
 // the gather / probe / bounce-volume readbacks.
-const full = args[8] === 'full';
+// 'dynamic': the scene's moving part (triangles from firstDynamicTri on) goes to the dynamic coverage layer.
+const full = args[8] === 'full', dynamic = args[8] === 'dynamic';
 const scene = r.generateScene(kind, seed, 0);
 const vw = r.createVoxWorld(scene, { v0, levels, G, maxBricks: levels * G * G * G, maxVoxels: 1 << 20 },
-  { eye: scene.eye, sunDir: scene.sunDir });
+  { eye: scene.eye, sunDir: scene.sunDir }, dynamic ? scene.firstDynamicTri : undefined);
 const rc = r.createRc(vw, full ? { cascades, probes0, s0, relocate: 1, sunTrace: 1, bounce: 1 } : { cascades, probes0, s0 });
 vw.update(scene.eye);
 rc.encode();
@@ -22,9 +23,11 @@ for (let i = 0; i < 64; i++) {
   pos.set([scene.eye[0] + (i % 8) - 3.5, scene.eye[1] - 1, scene.eye[2] + Math.floor(i / 8) - 3.5], i * 3);
   nrm.set([0, 1, 0], i * 3);
 }
-const extra = full ? [rc.gather(pos, nrm, true), rc.readProbes(0), rc.readBounceVolume()] : [];
+const extra = full ? [rc.gather(pos, nrm, true), rc.readProbes(0), rc.readBounceVolume()]
+  : dynamic ? [vw.dyn.readCounters()] : [];
 Promise.all([vw.readCounters(), rc.readSh(), rc.readCascade(cascades - 1), ...extra]).then(([c, sh, top, irr, pr, bv]) => {
   const out = { counters: c, shSum: sum(sh), sh0: Array.from(sh.slice(0, 27)), topLen: top.length };
+  if (dynamic) out.dyn = irr;
   if (full) {
     out.irr = Array.from(irr);
     out.probeSum = sum(pr.pos) + sum(pr.nrm);
