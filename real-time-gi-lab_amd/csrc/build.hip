@@ -441,29 +441,16 @@ __device__ __forceinline__ void voxel_table_flush(const VoxK& k, uint32_t* hkey,
   }
 }
 
-// voxelizeOne (src/core/voxel.ts:476-490) over the up to 8 cells of one small (triangle, level) range; alpha test
-// omitted (the materials of this path are untextured and opaque). The brick pointer is read once per brick.
-__device__ void voxelize_range(const VoxK& k, uint32_t* hkey, uint32_t* hval, const Tri& T, uint32_t t, int L,
-                               const Range& r, int n, uint32_t drops) {
+// voxelizeOne (src/core/voxel.ts:476-490) of cell q of the small range r (n cells) of triangle t at level L.
+__device__ __forceinline__ void voxelize_cell(const VoxK& k, uint32_t* hkey, uint32_t* hval, f3 A, f3 B, f3 Cc,
+                                              uint32_t t, int L, const Range& r, int q, int n, uint32_t drops) {
+  i3 c = cell_of(r, q);
+  if (finer_built(k, L, c, drops)) return;
   float v = vox_size(k, L);
-  f3 lo = region_lo(k, L);
-  f3 h = splat3(0.5f * v * 1.0001f);
-  const uint32_t* ptrL = k.ptr + (size_t)L * (uint32_t)k.per;
-  uint32_t lastb = 0xffffffffu, ptr = PTR_EMPTY;
-  for (int q = 0; q < n; q++) {
-    i3 c = cell_of(r, q);
-    if (finer_built(k, L, c, drops)) continue;
-    f3 p = lo + (tof3(c) + splat3(0.5f)) * v;
-    if (n != 1 && !tri_box(p, h, T.a, T.b, T.c)) continue;
-    i3 b = shr3(c, 3);
-    uint32_t bi = ((uint32_t)b.z * (uint32_t)k.G + (uint32_t)b.y) * (uint32_t)k.G + (uint32_t)b.x;
-    if (bi != lastb) {
-      ptr = ptrL[bi];
-      lastb = bi;
-    }
-    if (ptr >= PTR_MARK) continue;
-    voxel_update(k, hkey, hval, ptr, (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7)), t);
-  }
+  if (n != 1 && !tri_box(region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v, splat3(0.5f * v * 1.0001f), A, B, Cc)) return;
+  uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
+  if (ptr >= PTR_MARK) return;
+  voxel_update(k, hkey, hval, ptr, (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7)), t);
 }
 
 // voxBits (src/core/voxel.ts:511-513) over VOX_CHUNK consecutive triangles per workgroup.
@@ -496,7 +483,33 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
       }
       wave_append(big, &k.hdr[H_BIGVOX_L + L], k.big_vox + (size_t)L * k.big_cap, k.big_cap, &k.hdr[H_BIG_OVF],
                   t);
-      if (work) voxelize_range(k, hkey, hval, T, t, L, r, n, drops);
+      // The wavefront's small ranges of this level laid end to end (a scan of the cell counts), the lanes taking the
+      // cells in turn: each finds its triangle by binary search over the scan and reads it by shuffles. One range per
+      // lane would run the wavefront as long as its largest range (up to 8 cells) with the other lanes idle.
+      const int cnt = work ? n : 0;
+      int inc = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(inc, o);
+        if ((int)(threadIdx.x & 63u) >= o) inc += y;
+      }
+      const int start = inc - cnt, N = __shfl(inc, 63);
+      for (int j0 = 0; j0 < N; j0 += 64) {
+        const int j = j0 + (int)(threadIdx.x & 63u);
+        int own = 0;  // the last lane whose range starts at or before cell j (empty ranges share the next start)
+        for (int step = 32; step > 0; step >>= 1) {
+          int cand = own + step;
+          if (__shfl(start, cand) <= j) own = cand;
+        }
+        const int q = j - __shfl(start, own), no = __shfl(cnt, own);
+        const uint32_t to = (uint32_t)__shfl((int)t, own);
+        const f3 A = mk3(__shfl(T.a.x, own), __shfl(T.a.y, own), __shfl(T.a.z, own));
+        const f3 B = mk3(__shfl(T.b.x, own), __shfl(T.b.y, own), __shfl(T.b.z, own));
+        const f3 Cc = mk3(__shfl(T.c.x, own), __shfl(T.c.y, own), __shfl(T.c.z, own));
+        Range ro;
+        ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
+        ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
+        if (j < N) voxelize_cell(k, hkey, hval, A, B, Cc, to, L, ro, q, no, drops);
+      }
     }
   }
   voxel_table_flush(k, hkey, hval);
@@ -508,34 +521,64 @@ __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
   vox_small_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
-// voxelizeBig (src/core/voxel.ts:515-531): one wavefront per listed (triangle, level), one cell per lane.
+// voxelizeBig (src/core/voxel.ts:515-531) over the listed (triangle, level) entries, 64 to a wavefront: lane i loads
+// entry i, a wavefront scan of the cell counts lays the 64 ranges end to end, and the lanes take the cells in turn,
+// each finding its entry by binary search over the scan and its triangle by shuffles. A listed range has more than 8
+// cells but often far fewer than 64, so one entry per wavefront left most lanes idle. The OR of the bit and the max of
+// the representative commute, so the result does not depend on the schedule.
 __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += min(k.hdr[H_BIGVOX_L + L], k.big_cap);
   uint32_t drops = k.hdr[H_BRICK_OVF];
-  uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
-  uint32_t waves = (nblk * BLK) >> 6;
-  for (uint32_t e = wave; e < total; e += waves) {
-    int L;
-    uint32_t idx;
-    big_entry(k, e, H_BIGVOX_L, L, idx);
-    uint32_t t = k.big_vox[(size_t)L * k.big_cap + idx];
-    Tri T = load_tri(k, t);
-    Range r = vox_range(k, T, L, false);
-    int n = r.n.x * r.n.y * r.n.z;
-    float v = vox_size(k, L);
-    f3 lo = region_lo(k, L);
-    f3 h = splat3(0.5f * v * 1.0001f);
-    for (int q = (int)lane; q < n; q += 64) {
-      i3 c = cell_of(r, q);
-      if (finer_built(k, L, c, drops)) continue;
-      f3 p = lo + (tof3(c) + splat3(0.5f)) * v;
-      if (!tri_box(p, h, T.a, T.b, T.c)) continue;
-      uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
+  const uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
+  const uint32_t waves = (nblk * BLK) >> 6;
+  for (uint32_t base = wave * 64u; base < total; base += waves * 64u) {  // the same rounds for the whole wavefront
+    const uint32_t e = base + lane;
+    int L = 0, n = 0;
+    uint32_t t = 0u;
+    Tri T{};
+    Range r{};
+    if (e < total) {
+      uint32_t idx;
+      big_entry(k, e, H_BIGVOX_L, L, idx);
+      t = k.big_vox[(size_t)L * k.big_cap + idx];
+      T = load_tri(k, t);
+      r = vox_range(k, T, L, false);
+      n = r.n.x * r.n.y * r.n.z;
+    }
+    int inc = n;  // inclusive scan of the cell counts; listed entries have n > 8, so the starts increase strictly
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(inc, o);
+      if ((int)lane >= o) inc += y;
+    }
+    const int start = inc - n, N = __shfl(inc, 63);
+    for (int j0 = 0; j0 < N; j0 += 64) {
+      const int j = j0 + (int)lane;
+      int own = 0;  // the last entry starting at or before cell j
+      for (int step = 32; step > 0; step >>= 1) {
+        int cand = own + step;
+        if (__shfl(start, cand) <= j) own = cand;
+      }
+      const int q = j - __shfl(start, own);
+      const int Lo = __shfl(L, own);
+      const uint32_t to = (uint32_t)__shfl((int)t, own);
+      const f3 A = mk3(__shfl(T.a.x, own), __shfl(T.a.y, own), __shfl(T.a.z, own));
+      const f3 B = mk3(__shfl(T.b.x, own), __shfl(T.b.y, own), __shfl(T.b.z, own));
+      const f3 Cc = mk3(__shfl(T.c.x, own), __shfl(T.c.y, own), __shfl(T.c.z, own));
+      Range ro;
+      ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
+      ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
+      if (j >= N) continue;
+      i3 c = cell_of(ro, q);
+      if (finer_built(k, Lo, c, drops)) continue;
+      float v = vox_size(k, Lo);
+      f3 p = region_lo(k, Lo) + (tof3(c) + splat3(0.5f)) * v;
+      if (!tri_box(p, splat3(0.5f * v * 1.0001f), A, B, Cc)) continue;
+      uint32_t ptr = k.ptr[ptr_index(k, Lo, shr3(c, 3))];
       if (ptr >= PTR_MARK) continue;
       uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
       atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
-      atomicMax(&k.rep[(size_t)ptr * 512u + bit], t + 1u);
+      atomicMax(&k.rep[(size_t)ptr * 512u + bit], to + 1u);
     }
   }
 }
