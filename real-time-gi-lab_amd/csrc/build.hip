@@ -750,47 +750,87 @@ __device__ __forceinline__ uint32_t snorm16(float e) {
   return (uint32_t)(int)floorf(0.5f + 32767.0f * fminf(1.0f, fmaxf(-1.0f, e))) & 0xffffu;
 }
 
-__device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
-  uint32_t nb = n_bricks(k);
-  uint32_t n = nb * 512u;
-  for (uint32_t i = lb * BLK + threadIdx.x; i < n; i += nblk * BLK) {
-    uint32_t ptr = i >> 9, bit = i & 511u, w = bit >> 5;
-    const uint32_t* o = k.occ + (size_t)ptr * 16u;
-    uint32_t word = o[w];
-    if ((word & (1u << (bit & 31u))) == 0u) continue;
-    uint32_t first = k.wslot[ptr * 16u + w];
-    if (first == ATTR_NONE) continue;
-    uint32_t ai = first + __popc(word & ((1u << (bit & 31u)) - 1u));
-    if (k.vslot) k.vslot[ai] = (ptr << 9) | bit;
-    uint32_t code = k.bcode[ptr];
-    int L = (int)(code >> 27);
-    i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
-    i3 c = b * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
-    f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * vox_size(k, L);
-    uint32_t rp = k.rep[ptr * 512u + bit];
-    if (rp == 0u) continue;  // every occupied voxel has a representative; kept as a guard on the load below
-    Tri T = load_tri(k, rp - 1u);
-    // surfaceAt with wantAttr (src/core/voxel.ts:325-336): white texture, so the albedo is baseColorFactor.
-    f3 wb = closest_bary(p, T.a, T.b, T.c);
-    f3 nn = ldp(k.nrm, T.i0) * wb.x + ldp(k.nrm, T.i1) * wb.y + ldp(k.nrm, T.i2) * wb.z;
-    if (!(dot3(nn, nn) > 1e-12f)) nn = cross3(T.b - T.a, T.c - T.a);
-    nn = normalize3(nn);
-    f3 q = T.a * wb.x + T.b * wb.y + T.c * wb.z;
-    const float4 m = reinterpret_cast<const float4*>(k.mat)[T.m];
-    float alb[3] = {1.0f * m.x, 1.0f * m.y, 1.0f * m.z};
-    uint32_t pa = unorm8(alb[0]) | (unorm8(alb[1]) << 8) | (unorm8(alb[2]) << 16) | (unorm8(0.0f) << 24);
-    // octEnc (src/core/voxel.ts:584-588).
-    float s = fabsf(nn.x) + fabsf(nn.y) + fabsf(nn.z);
-    float px = nn.x / s, py = nn.y / s;
-    if (nn.z < 0.0f) {
-      float ox = px, oy = py;
-      px = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
-      py = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+// resolveAttr (src/core/voxel.ts:596-622) of the occupied voxel (ptr, bit) with attribute slot ai.
+__device__ __forceinline__ void resolve_voxel(const VoxK& k, uint32_t ptr, uint32_t bit, uint32_t ai) {
+  if (k.vslot) k.vslot[ai] = (ptr << 9) | bit;
+  uint32_t code = k.bcode[ptr];
+  int L = (int)(code >> 27);
+  i3 b{(int)((code >> 18) & 511u), (int)((code >> 9) & 511u), (int)(code & 511u)};
+  i3 c = b * 8 + i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+  f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * vox_size(k, L);
+  uint32_t rp = k.rep[ptr * 512u + bit];
+  if (rp == 0u) return;  // every occupied voxel has a representative; kept as a guard on the load below
+  Tri T = load_tri(k, rp - 1u);
+  // surfaceAt with wantAttr (src/core/voxel.ts:325-336): white texture, so the albedo is baseColorFactor.
+  f3 wb = closest_bary(p, T.a, T.b, T.c);
+  f3 nn = ldp(k.nrm, T.i0) * wb.x + ldp(k.nrm, T.i1) * wb.y + ldp(k.nrm, T.i2) * wb.z;
+  if (!(dot3(nn, nn) > 1e-12f)) nn = cross3(T.b - T.a, T.c - T.a);
+  nn = normalize3(nn);
+  f3 q = T.a * wb.x + T.b * wb.y + T.c * wb.z;
+  const float4 m = reinterpret_cast<const float4*>(k.mat)[T.m];
+  float alb[3] = {1.0f * m.x, 1.0f * m.y, 1.0f * m.z};
+  uint32_t pa = unorm8(alb[0]) | (unorm8(alb[1]) << 8) | (unorm8(alb[2]) << 16) | (unorm8(0.0f) << 24);
+  // octEnc (src/core/voxel.ts:584-588).
+  float s = fabsf(nn.x) + fabsf(nn.y) + fabsf(nn.z);
+  float px = nn.x / s, py = nn.y / s;
+  if (nn.z < 0.0f) {
+    float ox = px, oy = py;
+    px = (1.0f - fabsf(oy)) * (ox >= 0.0f ? 1.0f : -1.0f);
+    py = (1.0f - fabsf(ox)) * (oy >= 0.0f ? 1.0f : -1.0f);
+  }
+  uint32_t pn = snorm16(px) | (snorm16(py) << 16);
+  k.attr[ai * 2u] = pa;
+  k.attr[ai * 2u + 1u] = pn;
+  k.plane[ai] = dot3(nn, q - p);
+}
+
+// Position of the n-th (from 0) set bit of x, which has more than n set bits.
+__device__ __forceinline__ uint32_t nth_set_bit(uint32_t x, uint32_t n) {
+  uint32_t p = 0u;
+  for (uint32_t s = 16u; s > 0u; s >>= 1) {
+    uint32_t c = (uint32_t)__popc(x & ((1u << s) - 1u));
+    if (n >= c) {
+      n -= c;
+      x >>= s;
+      p += s;
     }
-    uint32_t pn = snorm16(px) | (snorm16(py) << 16);
-    k.attr[ai * 2u] = pa;
-    k.attr[ai * 2u + 1u] = pn;
-    k.plane[ai] = dot3(nn, q - p);
+  }
+  return p;
+}
+
+// The occupied voxels of 4 bricks (64 occupancy words, one per lane) per round of a wavefront, laid end to end by a
+// scan of the words' popcounts: the lanes take the voxels in turn, each finding its word by binary search over the
+// scan and its bit by a popcount search. One lane per voxel slot of the bricks kept about a quarter of the lanes busy.
+__device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+  const uint32_t nb = n_bricks(k);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = (lb * BLK + threadIdx.x) >> 6, waves = (nblk * BLK) >> 6;
+  for (uint32_t b0 = wave * 4u; b0 < nb; b0 += waves * 4u) {  // the same rounds for the whole wavefront
+    uint32_t word = 0u, first = ATTR_NONE;
+    if (b0 + (lane >> 4) < nb) {
+      word = k.occ[(size_t)b0 * 16u + lane];
+      first = k.wslot[(size_t)b0 * 16u + lane];
+    }
+    const int cnt = first == ATTR_NONE ? 0 : __popc(word);
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(inc, o);
+      if ((int)lane >= o) inc += y;
+    }
+    const int start = inc - cnt, N = __shfl(inc, 63);
+    for (int j0 = 0; j0 < N; j0 += 64) {
+      const int j = j0 + (int)lane;
+      int own = 0;  // the last word whose voxels start at or before j
+      for (int step = 32; step > 0; step >>= 1) {
+        int cand = own + step;
+        if (__shfl(start, cand) <= j) own = cand;
+      }
+      const uint32_t wd = (uint32_t)__shfl((int)word, own), fs = (uint32_t)__shfl((int)first, own);
+      const uint32_t n = (uint32_t)(j - __shfl(start, own));
+      if (j >= N) continue;
+      const uint32_t bit = ((uint32_t)own & 15u) * 32u + nth_set_bit(wd, n);
+      resolve_voxel(k, b0 + ((uint32_t)own >> 4), bit, fs + n);
+    }
   }
 }
 __global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x); }
@@ -845,7 +885,7 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   out[BL_VOX_BIG] = std::min<uint32_t>(2048, std::max<uint32_t>(16, k.ntris / 256));
   for (int L = 1; L < k.NL; L++) out[BL_REDUCE + L - 1] = persist;
   out[BL_ALLOC_VOXELS] = 1;
-  out[BL_RESOLVE] = persist;
+  out[BL_RESOLVE] = std::min<uint32_t>(2048, std::max<uint32_t>(1, cdiv((uint64_t)k.max_bricks, 16)));  // 4 bricks a wave
   if (k.dyn.hdr && k.dyn.end_tri > k.dyn.first_tri) {  // the dynamic layer (grid-stride loops)
     const uint32_t nd = k.dyn.end_tri - k.dyn.first_tri;
     const uint32_t vox = std::min<uint32_t>(4096, cdiv((uint64_t)k.dyn.cap * 512u, BLK));
