@@ -162,3 +162,37 @@ def test_dynamic_batch_matches_single(require_gpu):
         b.destroy()
         for w in [single] + pair:
             w.destroy()
+
+
+def test_dynamic_and_plain_worlds_in_one_batch(require_gpu):
+    """A batch holding a world with the layer and one without runs the layer-reading trace for both: the plain
+    world's update equals its solo update bit for bit, and so does the layer world's."""
+    vox, rc, frames = atrium_frames(n=2, step=9)
+    first = frames[0][0].firstDynamicTri
+    say("case: dynamic layer, atrium, batch of a layer world and a plain world vs solo, 2 updates")
+    solo_dyn, solo_plain = rtgi.World(vox, rc), rtgi.World(vox, rc)
+    b_dyn, b_plain = rtgi.World(vox, rc), rtgi.World(vox, rc)
+    for w in (solo_dyn, solo_plain, b_dyn, b_plain):
+        w.upload_scene(frames[0][0])
+    solo_dyn.set_dynamic(first, 512)
+    b_dyn.set_dynamic(first, 512)
+    b = rtgi.Batch([b_plain, b_dyn], graph=True)
+    try:
+        for k, (scene, f) in enumerate(frames):
+            for w in (solo_dyn, solo_plain, b_dyn, b_plain):
+                if k:
+                    w.update_vertices(0, scene.pos, scene.nrm)
+                w.set_frame(f)
+            for w in (solo_dyn, solo_plain):
+                w.encode()
+                w.sync()
+            b.encode()
+            b.sync()
+            assert np.array_equal(b_plain.read_sh(), solo_plain.read_sh())
+            assert np.array_equal(b_dyn.read_sh(), solo_dyn.read_sh())
+            assert not np.array_equal(solo_dyn.read_sh(), solo_plain.read_sh())
+            assert b_plain.read_counters() == solo_plain.read_counters()
+    finally:
+        b.destroy()
+        for w in (solo_dyn, solo_plain, b_dyn, b_plain):
+            w.destroy()
