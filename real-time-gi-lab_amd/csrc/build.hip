@@ -455,6 +455,9 @@ __device__ __forceinline__ void voxelize_cell(const VoxK& k, uint32_t* hkey, uin
 
 // voxBits (src/core/voxel.ts:511-513) over VOX_CHUNK consecutive triangles per workgroup.
 constexpr int VOX_CHUNK = 4;
+// Batches of at least this many worlds fill the GPU on their own: their voxel passes pack work into fewer, fuller
+// wavefronts.
+constexpr int WIDE_BATCH = 8;
 __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   __shared__ uint32_t hkey[VH_SIZE];
   __shared__ uint32_t hval[VH_SIZE];
@@ -521,19 +524,27 @@ __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
   vox_small_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
-// voxelizeBig (src/core/voxel.ts:515-531) over the listed (triangle, level) entries, 64 to a wavefront: lane i loads
-// entry i, a wavefront scan of the cell counts lays the 64 ranges end to end, and the lanes take the cells in turn,
-// each finding its entry by binary search over the scan and its triangle by shuffles. A listed range has more than 8
-// cells but often far fewer than 64, so one entry per wavefront left most lanes idle. The OR of the bit and the max of
-// the representative commute, so the result does not depend on the schedule.
-__device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+// voxelizeBig (src/core/voxel.ts:515-531) over the listed (triangle, level) entries, `chunk` (a power of two up to
+// 64) to a wavefront: lane i loads entry i, a wavefront scan of the cell counts lays the ranges end to end, and the
+// lanes take the cells in turn, each finding its entry by binary search over the scan and its triangle by shuffles.
+// A listed range has more than 8 cells but often far fewer than 64, so one entry per wavefront left most lanes idle.
+// With many worlds in a launch the chunk is 64 (fewest wavefronts, fullest lanes); one world alone takes the smallest
+// chunk that its launch's wavefronts cover, keeping the GPU busy. The OR of the bit and the max of the
+// representative commute, so the result does not depend on the schedule.
+__device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += min(k.hdr[H_BIGVOX_L + L], k.big_cap);
   uint32_t drops = k.hdr[H_BRICK_OVF];
   const uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
   const uint32_t waves = (nblk * BLK) >> 6;
-  for (uint32_t base = wave * 64u; base < total; base += waves * 64u) {  // the same rounds for the whole wavefront
-    const uint32_t e = base + lane;
+  uint32_t chunk = 64u;
+  if (!wide) {
+    const uint32_t per = (total + waves - 1u) / waves;
+    chunk = 1u;
+    while (chunk < per && chunk < 64u) chunk <<= 1;
+  }
+  for (uint32_t base = wave * chunk; base < total; base += waves * chunk) {  // the same rounds for the whole wavefront
+    const uint32_t e = lane < chunk ? base + lane : total;
     int L = 0, n = 0;
     uint32_t t = 0u;
     Tri T{};
@@ -582,11 +593,11 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
     }
   }
 }
-__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x, false); }
 __global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   int w = batch_world(b, blk);
-  vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
 
 // Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
@@ -798,16 +809,18 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint32_t x, uint32_t n) {
   return p;
 }
 
-// The occupied voxels of 4 bricks (64 occupancy words, one per lane) per round of a wavefront, laid end to end by a
-// scan of the words' popcounts: the lanes take the voxels in turn, each finding its word by binary search over the
+// The occupied voxels of R bricks (16 R occupancy words, one per lane) per round of a wavefront, laid end to end by
+// a scan of the words' popcounts: the lanes take the voxels in turn, each finding its word by binary search over the
 // scan and its bit by a popcount search. One lane per voxel slot of the bricks kept about a quarter of the lanes busy.
-__device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
+// R = 4 with many worlds in a launch (fullest lanes); 1 for one world alone, which then spreads over more wavefronts.
+__device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
   const uint32_t nb = n_bricks(k);
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = (lb * BLK + threadIdx.x) >> 6, waves = (nblk * BLK) >> 6;
-  for (uint32_t b0 = wave * 4u; b0 < nb; b0 += waves * 4u) {  // the same rounds for the whole wavefront
+  const uint32_t R = wide ? 4u : 1u;
+  for (uint32_t b0 = wave * R; b0 < nb; b0 += waves * R) {  // the same rounds for the whole wavefront
     uint32_t word = 0u, first = ATTR_NONE;
-    if (b0 + (lane >> 4) < nb) {
+    if ((lane >> 4) < R && b0 + (lane >> 4) < nb) {
       word = k.occ[(size_t)b0 * 16u + lane];
       first = k.wslot[(size_t)b0 * 16u + lane];
     }
@@ -833,11 +846,11 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
     }
   }
 }
-__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x, false); }
 __global__ void __launch_bounds__(BLK) k_resolve_b(BatchK b) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   int w = batch_world(b, blk);
-  resolve_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  resolve_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
 
 
