@@ -1128,7 +1128,17 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
     p.dyn = p.dyn || worlds[i]->dyn_hdr != nullptr;
   }
   p.n = n;
-  for (int l = 0; l < BL_COUNT; l++) p.grid[l] = start[(size_t)l * (n + 1) + n];
+  for (int l = 0; l < BL_COUNT; l++) {
+    p.grid[l] = start[(size_t)l * (n + 1) + n];
+    // About equal block counts (within 25 %) stand for about equal cost per world: then the XCD remap.
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (int i = 0; i < n; i++) {
+      uint32_t g = start[(size_t)l * (n + 1) + i + 1] - start[(size_t)l * (n + 1) + i];
+      lo = std::min(lo, g);
+      hi = std::max(hi, g);
+    }
+    p.xcd[l] = n > 0 && lo > 0 && (uint64_t)hi * 4u <= (uint64_t)lo * 5u;
+  }
   // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
   // says whether it marks in LDS).
   p.mark_lds = all_lds ? lds : 0;

@@ -28,3 +28,10 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t blk, uint32_t nblk) {
   uint32_t x = blk % X, i = blk / X, q = nblk / X, r = nblk % X;
   return x < r ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
 }
+// The logical block of this workgroup in a batched launch. The remap pays when the worlds of the launch cost about
+// the same per block (a uniform batch: +8 % for 16 city worlds); with worlds of unequal cost, a contiguous range per
+// XCD leaves the XCD holding the costliest worlds running alone at the end (-8 % for a mixed batch), so those
+// launches keep the hardware order.
+__device__ __forceinline__ uint32_t batch_block(const BatchK& b) {
+  return b.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+}

@@ -202,7 +202,7 @@ __device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t 
 }
 __global__ void __launch_bounds__(BLK) k_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   clear_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -237,7 +237,7 @@ __device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t n
 }
 __global__ void __launch_bounds__(BLK) k_mark(VoxK k) { mark_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   mark_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -275,7 +275,7 @@ __device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32
 }
 __global__ void __launch_bounds__(BLK) k_mark_big(VoxK k) { mark_big_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_big_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   mark_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -298,7 +298,7 @@ __device__ __forceinline__ void mark_parents_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(BLK) k_mark_parents(VoxK k) { mark_parents_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_mark_parents_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   mark_parents_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -385,7 +385,7 @@ __device__ __forceinline__ void alloc_bricks_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(1024) k_alloc_bricks(VoxK k) { alloc_bricks_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(1024) k_alloc_bricks_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   alloc_bricks_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -402,7 +402,7 @@ __device__ __forceinline__ void init_bricks_body(const VoxK& k, uint32_t lb, uin
 }
 __global__ void __launch_bounds__(BLK) k_init_bricks(VoxK k) { init_bricks_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_init_bricks_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   init_bricks_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -519,7 +519,7 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
 }
 __global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) { vox_small_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   vox_small_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -595,7 +595,7 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
 }
 __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x, false); }
 __global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
@@ -653,7 +653,7 @@ __device__ __forceinline__ void reduce_body(const VoxK& k, int L, uint32_t lb, u
 }
 __global__ void __launch_bounds__(BLK) k_reduce(VoxK k, int L) { reduce_body(k, L, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_reduce_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   reduce_body(b.w[w].vx, b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -717,7 +717,7 @@ __device__ __forceinline__ void alloc_voxels_body(const VoxK& k, uint32_t lb, ui
 }
 __global__ void __launch_bounds__(1024) k_alloc_voxels(VoxK k) { alloc_voxels_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(1024) k_alloc_voxels_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   alloc_voxels_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -848,7 +848,7 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
 }
 __global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x, false); }
 __global__ void __launch_bounds__(BLK) k_resolve_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   resolve_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
@@ -862,7 +862,7 @@ size_t vox_mark_lds_bytes(const VoxK& k) {
   return bytes <= 64 * 1024 ? bytes : 0;
 }
 
-static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0}; }
 
 int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   hipStream_t s = (hipStream_t)stream;

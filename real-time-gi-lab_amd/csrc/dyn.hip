@@ -493,37 +493,37 @@ __global__ void __launch_bounds__(BLK) k_dl_resolve(VoxK k, int L) { resolve_bod
 
 #define DL_BATCHED(name, body)                                              \
   __global__ void __launch_bounds__(BLK) name(BatchK b) {                   \
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);                  \
+    const uint32_t blk = batch_block(b);                  \
     int w = batch_world(b, blk);                                            \
     body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);         \
   }
 __global__ void __launch_bounds__(BLK) k_dl_mark_b(BatchK b) {
   __shared__ uint32_t lbits[MARK_LDS];
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   mark_body(b.w[w].vx, lbits, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 DL_BATCHED(k_dl_parents_b, parents_body)
 DL_BATCHED(k_dl_clear_b, clear_body)
 __global__ void __launch_bounds__(1024) k_dl_alloc_b(BatchK b) {
-  int w = batch_world(b, xcd_block(blockIdx.x, gridDim.x));
+  int w = batch_world(b, batch_block(b));
   alloc_body(b.w[w].vx);
 }
 __global__ void __launch_bounds__(SPLAT_WG) k_dl_splat_b(BatchK b) {
   __shared__ SplatWin W;
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   splat_body(b.w[w].vx, b.w[w], W, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 __global__ void __launch_bounds__(BLK) k_dl_resolve_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   resolve_body(b.w[w].vx, b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 }  // namespace
 
-static BatchK dbk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+static BatchK dbk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0}; }
 
 int launch_dyn_build(const VoxK& k, void* stream) {
   if (!k.dyn.hdr || k.dyn.end_tri <= k.dyn.first_tri) return 0;

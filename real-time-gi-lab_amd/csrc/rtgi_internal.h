@@ -175,6 +175,7 @@ struct BatchK {
   const uint32_t* __restrict__ start;
   int32_t n;
   int32_t arg;
+  int32_t xcd;  // remap blocks so that each XCD runs one contiguous range (worlds of about equal block counts)
 };
 
 // Launchers (each returns a hipError_t as int).
@@ -241,6 +242,7 @@ struct BatchPlan {
   bool stats;       // some world of the batch counts tracer statistics
   bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps
   bool dyn;         // some world has a dynamic layer (the trace reads it)
+  bool xcd[BL_COUNT];  // the launch's worlds have about equal block counts: contiguous ranges per XCD pay
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);

@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 }
 template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   trace_body<ST, DY>(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -722,14 +722,14 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
 }
 __global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   sh_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Merge step of a batch: each world merges its cascade N-1-step (c > 0 with the 2x2 averages, c = 0 per ray).
 __global__ void __launch_bounds__(BLK) k_merge_step_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
   uint32_t lb = blk - b.start[w], nb = b.start[w + 1] - b.start[w];
@@ -844,7 +844,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 // Step of a batch: each world traces and merges its cascade N-1-step.
 template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
   trace_merge_body<ST, DY>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
@@ -1144,7 +1144,7 @@ __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t 
 }
 __global__ void __launch_bounds__(BLK) k_sun(TraceK a) { sun_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   sun_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -1346,7 +1346,7 @@ __device__ __forceinline__ void bnvol_body(const TraceK& a, uint32_t lb) {
 }
 __global__ void __launch_bounds__(BLK) k_bnvol(TraceK a) { bnvol_body(a, blockIdx.x); }
 __global__ void __launch_bounds__(BLK) k_bnvol_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   bnvol_body(b.w[w], blk - b.start[w]);
 }
@@ -1383,7 +1383,7 @@ __device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32
 }
 __global__ void __launch_bounds__(BLK) k_bounce(TraceK a) { bounce_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_bounce_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   bounce_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -1425,12 +1425,12 @@ __device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t
 __global__ void __launch_bounds__(BLK) k_relocate(TraceK a) { relocate_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_link(TraceK a) { link_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_relocate_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   relocate_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 __global__ void __launch_bounds__(BLK) k_link_b(BatchK b) {
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   link_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
@@ -1520,7 +1520,7 @@ int launch_sh(const TraceK& k, void* stream) {
   return (int)hipGetLastError();
 }
 
-static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg}; }
+static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0}; }
 
 int launch_sun_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_SUN]) k_sun_b<<<p.grid[BL_SUN], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SUN));
