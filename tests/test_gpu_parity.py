@@ -191,6 +191,25 @@ def test_city_parity_and_determinism(require_gpu):
     w.destroy()
 
 
+@pytest.mark.parametrize("case", ["atrium", "city"])
+def test_reference_default_voxel_params(require_gpu, case):
+    """The reference's DEFAULT_VOX (v0 0.2, 5 levels of 256^3, 4,608 bricks, 524,288 voxels; voxel.ts:46): on the
+    atrium every level fits; on the city 10,488 bricks are wanted, so the finest 5,880 are dropped and their rays
+    read the coarser levels (voxAt's fallback). Clipmap and cascades at the parity bars."""
+    scene, _, _, frame = rtgi.config(case)
+    vox = rtgi.DEFAULT_VOX
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625)
+    say(f"case: {case} {scene.ntris} triangles at the reference's default voxel parameters")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    check_clipmap(w, o, vox)
+    if case == "city":
+        assert o.counters()["brickOverflow"] > 0
+    check_cascades(w, o, rc)
+    w.destroy()
+
+
 def test_moving_frames(require_gpu):
     """Full rebuild per frame: posed vertices of frame 3 uploaded into a world built at frame 0 match a fresh oracle."""
     scene0 = rtgi.generate_scene("atrium", seed=2, frame=0)
