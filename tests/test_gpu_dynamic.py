@@ -196,3 +196,31 @@ def test_dynamic_and_plain_worlds_in_one_batch(require_gpu):
         b.destroy()
         for w in (solo_dyn, solo_plain, b_dyn, b_plain):
             w.destroy()
+
+
+def test_dynamic_layer_reference_default_voxel_params(require_gpu):
+    """The layer at the reference's DEFAULT_VOX (5 levels of 256^3): its wanted-brick bitset (5 x 32^3 bits) is too
+    large for the mark kernel's LDS copy, so the marks go straight to the layer's bitset. Words and cascades as above
+    over two updates of the moving atrium."""
+    _, _, frames = atrium_frames(n=2, step=11)
+    vox = rtgi.DEFAULT_VOX
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625, recordHits=True)
+    first = frames[0][0].firstDynamicTri
+    w = rtgi.World(vox, rc)
+    try:
+        w.upload_scene(frames[0][0])
+        w.set_dynamic(first, 512)
+        d = DynState(vox, 512, first)
+        for k, (scene, f) in enumerate(frames):
+            say(f"case: dynamic layer, atrium at the default voxel parameters, update {k + 1}")
+            if k:
+                w.update_vertices(0, scene.pos, scene.nrm)
+            w.set_frame(f)
+            w.encode()
+            w.sync()
+            o = Oracle(vox, rc, f)
+            o.build(scene, dyn=d)
+            assert check_layer(w, d) > 0
+            check_update(w, o, rc, False)
+    finally:
+        w.destroy()
