@@ -130,6 +130,18 @@ __device__ __forceinline__ bool finer_built(const VoxK& k, int L, i3 c, uint32_t
 __device__ __forceinline__ i3 cell_of(const Range& r, int q) {
   return i3{r.lo.x + q % r.n.x, r.lo.y + (q / r.n.x) % r.n.y, r.lo.z + q / (r.n.x * r.n.y)};
 }
+// cell_of for a small range (at most 8 cells, so q < 8 and every extent <= 8): q / n as (q m) >> 12 with
+// m = 4096 / n + 1, which is exact for q < 64 and n <= 8 (checked exhaustively), in place of the divisions.
+struct SmallDiv {
+  uint32_t mx, my;
+};
+__device__ __forceinline__ SmallDiv small_div(const Range& r) {
+  return SmallDiv{4096u / (uint32_t)r.n.x + 1u, 4096u / (uint32_t)r.n.y + 1u};
+}
+__device__ __forceinline__ i3 cell_of_small(const Range& r, SmallDiv m, int q) {
+  const uint32_t t = ((uint32_t)q * m.mx) >> 12, z = (t * m.my) >> 12;
+  return i3{r.lo.x + q - (int)t * r.n.x, r.lo.y + (int)(t - z * (uint32_t)r.n.y), r.lo.z + (int)z};
+}
 
 __device__ __forceinline__ uint32_t lane_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -160,14 +172,17 @@ __device__ __forceinline__ void mark_brick(const VoxK& k, uint32_t* bits, uint32
     k.ptr[i] = PTR_MARK;  // every writer stores the same value, and nothing else writes the table in this phase
 }
 
-// markRange (src/core/voxel.ts:348-357).
+// markRange (src/core/voxel.ts:348-357); SMALL: a range of at most 8 bricks (a single brick needs no division).
+template <bool SMALL>
 __device__ void mark_range(const VoxK& k, uint32_t* bits, const Tri& T, int L, const Range& r, int start, int stride) {
   float bs = vox_size(k, L) * 8.0f;
   f3 lo = region_lo(k, L);
   int n = r.n.x * r.n.y * r.n.z;
   f3 h = splat3(0.5f * bs * 1.0001f);
+  SmallDiv m{1u, 1u};
+  if (SMALL && n > 1) m = small_div(r);
   for (int j = start; j < n; j += stride) {
-    i3 b = cell_of(r, j);
+    i3 b = SMALL ? (n == 1 ? r.lo : cell_of_small(r, m, j)) : cell_of(r, j);
     if (finer_holds(k, L, b * 8) && finer_holds(k, L, b * 8 + 7)) continue;
     if (n == 1 || tri_box(lo + (tof3(b) + splat3(0.5f)) * bs, h, T.a, T.b, T.c)) mark_brick(k, bits, ptr_index(k, L, b));
   }
@@ -230,7 +245,7 @@ __device__ __forceinline__ void mark_body(const VoxK& k, uint32_t lb, uint32_t n
       }
       wave_append(big, &k.hdr[H_BIGMARK_L + L], k.big_mark + (size_t)L * k.big_cap, k.big_cap, &k.hdr[H_BIG_OVF],
                   t);
-      if (work) mark_range(k, bits, T, L, r, 0, 1);
+      if (work) mark_range<true>(k, bits, T, L, r, 0, 1);
     }
   }
   if (bits) bits_flush(k);
@@ -269,7 +284,7 @@ __device__ __forceinline__ void mark_big_body(const VoxK& k, uint32_t lb, uint32
     big_entry(k, e, H_BIGMARK_L, L, idx);
     uint32_t t = k.big_mark[(size_t)L * k.big_cap + idx];
     Tri T = load_tri(k, t);
-    mark_range(k, bits, T, L, vox_range(k, T, L, true), (int)lane, 64);
+    mark_range<false>(k, bits, T, L, vox_range(k, T, L, true), (int)lane, 64);
   }
   if (bits) bits_flush(k);
 }
@@ -443,8 +458,9 @@ __device__ __forceinline__ void voxel_table_flush(const VoxK& k, uint32_t* hkey,
 
 // voxelizeOne (src/core/voxel.ts:476-490) of cell q of the small range r (n cells) of triangle t at level L.
 __device__ __forceinline__ void voxelize_cell(const VoxK& k, uint32_t* hkey, uint32_t* hval, f3 A, f3 B, f3 Cc,
-                                              uint32_t t, int L, const Range& r, int q, int n, uint32_t drops) {
-  i3 c = cell_of(r, q);
+                                              uint32_t t, int L, const Range& r, SmallDiv m, int q, int n,
+                                              uint32_t drops) {
+  i3 c = n == 1 ? r.lo : cell_of_small(r, m, q);
   if (finer_built(k, L, c, drops)) return;
   float v = vox_size(k, L);
   if (n != 1 && !tri_box(region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v, splat3(0.5f * v * 1.0001f), A, B, Cc)) return;
@@ -490,6 +506,7 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
       // cells in turn: each finds its triangle by binary search over the scan and reads it by shuffles. One range per
       // lane would run the wavefront as long as its largest range (up to 8 cells) with the other lanes idle.
       const int cnt = work ? n : 0;
+      const SmallDiv md = cnt > 1 ? small_div(r) : SmallDiv{1u, 1u};
       int inc = cnt;
       for (int o = 1; o < 64; o <<= 1) {
         int y = __shfl_up(inc, o);
@@ -511,7 +528,8 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
         Range ro;
         ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
         ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
-        if (j < N) voxelize_cell(k, hkey, hval, A, B, Cc, to, L, ro, q, no, drops);
+        const SmallDiv mo{(uint32_t)__shfl((int)md.mx, own), (uint32_t)__shfl((int)md.my, own)};
+        if (j < N) voxelize_cell(k, hkey, hval, A, B, Cc, to, L, ro, mo, q, no, drops);
       }
     }
   }

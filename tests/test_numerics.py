@@ -50,3 +50,16 @@ def test_unorm8_division_is_exact():
 
 def test_snorm16_division_is_exact():
     assert fast_quotient_mismatches(32767, range(-32768, 32768)) == []
+
+
+def test_small_range_cell_decomposition():
+    """The voxelisation and brick marking split the cell index q of a small range (at most 8 cells, extents nx, ny
+    <= 8) into x, y, z with q / n = (q * m) >> 12, m = 4096 // n + 1, in place of divisions (build.hip
+    cell_of_small). Exhaustive over every extent and every q < 64 (a small range has q < 8)."""
+    for nx in range(1, 9):
+        for ny in range(1, 9):
+            mx, my = 4096 // nx + 1, 4096 // ny + 1
+            for q in range(64):
+                t = (q * mx) >> 12
+                z = (t * my) >> 12
+                assert (q - t * nx, t - z * ny, z) == (q % nx, (q // nx) % ny, q // (nx * ny))
