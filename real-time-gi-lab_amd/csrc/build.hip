@@ -13,15 +13,18 @@
 //   k_alloc_bricks one workgroup, one scan: ordered allocation,         (countMarks/scanMarks/allocBricks,
 //                  coarse levels first, finest dropped first               voxel.ts:405-470)
 //   k_init_bricks  zero occupancy and representatives of the pool         (clearKeys, voxel.ts:471-474)
-//   k_vox_small    per triangle and level: SAT per voxel; bit and         (voxBits, voxel.ts:476-512)
+//   k_vox_small    per triangle and level: SAT per voxel (the wavefront's (voxBits, voxel.ts:476-512)
+//                  cells laid end to end over its lanes); bit and
 //                  triangle + 1 merged per workgroup in an LDS hash table,
 //                  then one atomicOr / atomicMax per unique voxel
-//   k_vox_big      one wavefront per listed (triangle, level)             (voxBitsBig, voxel.ts:514-534)
+//   k_vox_big      listed (triangle, level) entries, up to 64 per         (voxBitsBig, voxel.ts:514-534)
+//                  wavefront, their cells laid end to end over its lanes
 //   k_reduce(L)    coarse level L from L-1, gathered per parent voxel     (reduce, voxel.ts:540-577)
 //                  (single writer per word: no atomics)
 //   k_alloc_voxels one workgroup, scan rounds: attribute slots by         (allocVoxels, voxel.ts:579-588)
 //                  prefix sum in slot order
-//   k_resolve      per (brick, voxel): albedo, oct normal, plane          (resolveAttr, voxel.ts:590-620)
+//   k_resolve      per occupied voxel (laid end to end over the lanes     (resolveAttr, voxel.ts:590-620)
+//                  by popcount scans): albedo, oct normal, plane
 //
 // Device-scope atomics bypass the XCD-local L2 on this part, so the build avoids them where the result does not
 // need them (marks, reduce); the voxel updates that remain return nothing, so no thread waits on them.

@@ -3,15 +3,17 @@
 // coarse levels; the cascade trace reads it in front of the static hit (trace_dyn in trace.hip).
 //
 //   k_dl_mark     per dynamic triangle: splat level (the finest level whose window holds its box grown by two
-//                 voxels), the bricks its box grown by one voxel touches there (bits of this update's bitset), the
-//                 bounds of everything a read can see                                   (mark, dynlayer.ts:255-279)
+//                 voxels), the bricks its box grown by one voxel touches there (bits of this update's bitset, via
+//                 an LDS copy when it fits), the bounds of everything a read can see, reduced per wavefront
+//                                                                                       (mark, dynlayer.ts:255-279)
 //   k_dl_parents  per bitset word below the coarsest level: each wanted brick wants its ancestors (parents, :281-303)
 //   k_dl_alloc    one workgroup: slots in bit order (coarse levels first, then brick index), dropped past the pool;
 //                 bricks wanted last update and not now emptied; per-level ranges, bounds published, parity flipped
 //                                                                       (count, alloc, finalize, :318-384)
 //   k_dl_clear    the allocated bricks' words and voxel masks                                  (clear, :385-390)
 //   k_dl_splat    per dynamic triangle: m^2 sub-triangle samples shaded like a voxel hit, added with trilinear
-//                 weights over the 8 nearest voxel centres as fixed-point sums (exact: order-independent)
+//                 weights over the 8 nearest voxel centres as fixed-point sums (exact: order-independent); 16
+//                 triangles per workgroup, 4 lanes each, through an LDS window of voxels
 //                                                                       (splat, splatTri, splatSample, :392-540)
 //   k_dl_resolve  level L = 1..: own splats + 1/4 of each child with a value, as floats    (resolve, :542-580)
 //
