@@ -191,6 +191,25 @@ def test_city_parity_and_determinism(require_gpu):
     w.destroy()
 
 
+def test_maximum_levels_and_cascades(require_gpu):
+    """The largest clipmap and cascade counts the reference accepts (8 levels, voxel.ts:749; 8 cascades,
+    rc.ts:1022): 128^3 c0 probes (the smallest grid that halves 7 times), directions capped at 16^2 from c2 on,
+    59.9M rays; a random scene spread over all 8 levels. Clipmap and cascades at the parity bars."""
+    rng = np.random.default_rng(8)
+    scene = random_scene(rng, 400, -5.0, 5.0, 0.6)
+    vox = rtgi.VoxParams(v0=0.05, levels=8, G=4, maxBricks=8 * 64, maxVoxels=1 << 16)
+    rc = rtgi.RcParams(cascades=8, probes0=128, s0=0.05, resCap=2)
+    frame = rtgi.Frame(eye=(0.3, 0.2, -0.1), sunDir=(0.3, 1.0, -0.2))
+    say("case: 8 levels x 32^3, 8 cascades, 128^3 c0 probes, 59.9M rays")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    assert all(x > 0 for x in o.counters()["bricksPerLevel"])
+    check_clipmap(w, o, vox)
+    check_cascades(w, o, rc)
+    w.destroy()
+
+
 @pytest.mark.parametrize("case", ["atrium", "city"])
 def test_reference_default_voxel_params(require_gpu, case):
     """The reference's DEFAULT_VOX (v0 0.2, 5 levels of 256^3, 4,608 bricks, 524,288 voxels; voxel.ts:46): on the
