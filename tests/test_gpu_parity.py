@@ -117,6 +117,33 @@ def test_voxel_pool_overflow(require_gpu):
     w.destroy()
 
 
+def test_degenerate_and_far_triangles(require_gpu):
+    """Ragged geometry: a third of the triangles degenerate (a repeated vertex, or three collinear vertices: zero
+    area and a zero face normal), some far outside every window, some with zero vertex normals (the resolve falls
+    back to the face normal), plus axis-aligned slivers exactly on voxel faces. Clipmap and cascades at the bars."""
+    rng = np.random.default_rng(9)
+    scene, vox, rc, frame = tiny_case(seed=9, ntris=600)
+    pos = scene.pos.reshape(-1, 3, 3).copy()
+    nrm = scene.nrm.reshape(-1, 3, 3).copy()
+    n = pos.shape[0]
+    pos[0:100, 2] = pos[0:100, 1]                                              # a repeated vertex
+    pos[100:200, 2] = 0.25 * pos[100:200, 0] + 0.75 * pos[100:200, 1]          # collinear (approximately in f32)
+    pos[200:250] += np.float32(1e4)                                            # outside every window
+    nrm[250:300] = 0.0                                                         # no vertex normals
+    g = rng.integers(-16, 16, (100, 3)).astype(np.float32) * np.float32(0.25)  # slivers on voxel faces
+    pos[300:400, 0] = g
+    pos[300:400, 1] = g + np.array([0.25, 0.0, 0.0], np.float32)
+    pos[300:400, 2] = g + np.array([0.0, 0.25, 0.0], np.float32)
+    scene = dataclasses.replace(scene, pos=pos.reshape(-1, 3), nrm=nrm.reshape(-1, 3))
+    say("case: degenerate, far and axis-aligned triangles, 600 triangles")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    check_clipmap(w, o, vox)
+    check_cascades(w, o, rc)
+    w.destroy()
+
+
 def test_empty_scene(require_gpu):
     scene, vox, rc, frame = tiny_case(seed=5, ntris=1)
     scene = dataclasses.replace(scene, tri=scene.tri[:0])
