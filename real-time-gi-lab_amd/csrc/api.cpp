@@ -1270,7 +1270,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : b->stream;
   const int n = (int)b->worlds.size();
   std::vector<uint64_t> sig;
-  sig.reserve((size_t)n * 4);
+  sig.reserve((size_t)n * 6);
   for (int i = 0; i < n; i++) {
     rtgi_world* w = b->worlds[i];
     if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "world %d has no scene or no frame", i);
@@ -1279,6 +1279,9 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
     sig.push_back(w->ray_off[w->N]);
     sig.push_back((uint64_t)w->NL << 32 | (uint32_t)w->per);
     sig.push_back((uint64_t)w->vp.max_bricks << 32 | w->rp.flags);
+    // The launch set also follows the tuning and the dynamic layer, either of which may change between enqueues.
+    sig.push_back((uint64_t)(uint32_t)w->tuning.probe_tile << 32 | (uint32_t)w->tuning.fused);
+    sig.push_back(w->dyn_hdr ? ((uint64_t)w->dyn_first << 32 | (uint32_t)w->dyn_cap) : ~0ull);
   }
   if (sig != b->sig) {
     // The previous update may still read the block prefixes being replaced.

@@ -224,3 +224,35 @@ def test_dynamic_layer_reference_default_voxel_params(require_gpu):
             check_update(w, o, rc, False)
     finally:
         w.destroy()
+
+
+def test_set_dynamic_and_tuning_after_batch_creation(require_gpu):
+    """A batch re-plans when a world's dynamic layer or tuning changes after the batch was made: the layer is then
+    built and read in the batched update too, equal to the single-world update."""
+    vox, rc, frames = atrium_frames(n=1)
+    scene, f = frames[0]
+    first = scene.firstDynamicTri
+    say("case: dynamic layer and tuning set after the batch was created (graph)")
+    single, bw = rtgi.World(vox, rc), rtgi.World(vox, rc)
+    for w in (single, bw):
+        w.upload_scene(scene)
+        w.set_frame(f)
+    b = rtgi.Batch([bw], graph=True)
+    try:
+        b.encode()
+        b.sync()
+        plain = bw.read_sh()
+        for w in (single, bw):
+            w.set_dynamic(first, 512)
+        bw.set_tuning(fused=0)
+        single.encode()
+        single.sync()
+        b.encode()
+        b.sync()
+        assert np.array_equal(bw.read_sh(), single.read_sh())
+        assert not np.array_equal(bw.read_sh(), plain)
+        assert bw.read_dyn_counters() == single.read_dyn_counters()
+    finally:
+        b.destroy()
+        for w in (single, bw):
+            w.destroy()
