@@ -67,6 +67,10 @@ typedef struct rtgi_vox_params {
                                 * update's volume, every 2nd update per probe), then stores its irradiance on both
                                 * sides of every voxel (bnInject); cascade hits add albedo / pi times it (voxBounce,
                                 * voxSide, shadeHitV, rc.ts:107-126, 198-202) */
+#define RTGI_RC_OBJECTS 128u /* token obj (rc.ts:79-81, not in the reference's defaults): with RTGI_RC_BOUNCE and a dynamic
+                              * layer (rtgi_world_set_dynamic), the layer also splats albedo, its voxels receive the
+                              * previous bounce volume's light (dynInject, dynlayer.ts:166-190, dynE of bounce.ts:267-298)
+                              * and the volume's irradiance rays read it (bnTraceDyn, bounce.ts:173-178) */
 #define RTGI_RC_STATS 8u       /* count traces, DDA steps and step-cap hits per cascade (setStats(1), rc.ts:232-235) */
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH

@@ -39,7 +39,8 @@ class OrcRc(C.Structure):
                 ("point_i", (C.c_float * 3) * 8), ("omega0", C.c_float * 16), ("fp16", C.c_int32),
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
                 ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p),
-                ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3), ("dyn", C.c_void_p)]
+                ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3), ("dyn", C.c_void_p),
+                ("objects", C.c_int32)]
 
 
 class OrcDyn(C.Structure):
@@ -64,6 +65,7 @@ _SIG = {
     "orc_link": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, _P]),
     "orc_bn_origin": (None, [_P, _P]),
     "orc_dyn_init": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcDyn)]),
+    "orc_dyn_inject": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcDyn), _P, _P]),
     "orc_dyn_build": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcRc), _P, _P, _P, _P, C.POINTER(OrcDyn)]),
     "orc_bounce_volume": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, C.c_uint32, _P, _P]),
     "orc_bounce_inject": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, _P]),
@@ -215,6 +217,7 @@ class Oracle:
         r.vox_offset = rc.voxOffset
         r.lift = getattr(rc, "lift", 0.25)
         r.reloc = 1 if getattr(rc, "relocate", False) else 0
+        r.objects = 1 if getattr(rc, "objects", False) else 0
         self._sun_trace = bool(getattr(rc, "sunTrace", False))
         self._bounce = bool(getattr(rc, "bounce", False))
         self._eye = np.asarray(frame.eye, np.float32)
@@ -284,6 +287,9 @@ class Oracle:
         vol = np.zeros((BN_DIMS[0] * BN_DIMS[1] * BN_DIMS[2], 6, 4), np.uint16)
         pv, po = (None, None) if prev is None else (np.ascontiguousarray(prev[0], np.uint16),
                                                      np.ascontiguousarray(prev[1], np.int32))
+        if self.rc.objects and self.rc.dyn and pv is not None:
+            # dynInject: the dynamic layer receives the previous volume's light before this volume's rays read it
+            assert lib().orc_dyn_inject(C.byref(self.vp), C.byref(self._dyn_struct), _p(pv), _p(po)) == 0
         assert lib().orc_bounce_volume(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(pv), _p(po), n, _p(org),
                                        _p(vol)) == 0
         self.inject_volume(vol, org)

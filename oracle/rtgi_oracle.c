@@ -744,6 +744,51 @@ static v3 bnRead(const uint16_t* vol, const int32_t org3[3], float tol, v3 q, v3
   return ws > 1e-6f ? vdivs(e, ws) : V3(0, 0, 0);
 }
 
+/* dynE of the bounce volume (bounce.ts:267-298): the irradiance a dynamic voxel at p with the projected areas P
+ * receives: per axis face, the mean over the probes around p that are active, see p within their reach (+ tol) and
+ * lie on that face's side (smoothstep weight) of their texel on that axis; the faces weighted by P. */
+static v3 dynE(const uint16_t* vol, const int32_t org3[3], float tol, v3 p, const float P[6]) {
+  iv3 o = I3(org3[0], org3[1], org3[2]);
+  v3 g = vsub(vdivs(p, ORC_BN_CELL), vf(o));
+  iv3 bb = vi(vfloor(g));
+  v3 fr = vsub(g, vf(bb));
+  v3 e[6];
+  float ws[6];
+  for (int a = 0; a < 6; a++) {
+    e[a] = V3(0, 0, 0);
+    ws[a] = 0.0f;
+  }
+  for (int k = 0; k < 8; k++) {
+    iv3 off = I3(k & 1, (k >> 1) & 1, (k >> 2) & 1);
+    iv3 cc = iadd(bb, off);
+    if (!bnInside(cc)) continue;
+    v3 pp = vscale(vf(iadd(o, cc)), ORC_BN_CELL);
+    v3 u = vsub(p, pp);
+    if (!(bnReach(vol, cc, 0) > 0.0f)) continue;
+    if (fabsf(u.x) > bnReach(vol, cc, u.x >= 0.0f ? 0 : 1) + tol || fabsf(u.y) > bnReach(vol, cc, u.y >= 0.0f ? 2 : 3) + tol ||
+        fabsf(u.z) > bnReach(vol, cc, u.z >= 0.0f ? 4 : 5) + tol)
+      continue;
+    v3 t3 = V3(off.x ? fr.x : 1.0f - fr.x, off.y ? fr.y : 1.0f - fr.y, off.z ? fr.z : 1.0f - fr.z);
+    float wt = t3.x * t3.y * t3.z;
+    for (int a = 0; a < 6; a++) {
+      if (P[a] <= 0.0f) continue;
+      float sg = (a & 1) ? -1.0f : 1.0f;
+      v3 n = V3(a / 2 == 0 ? sg : 0.0f, a / 2 == 1 ? sg : 0.0f, a / 2 == 2 ? sg : 0.0f);
+      float w = wt * smoothStep(-0.05f, 0.25f, vdot(vsub(pp, p), n) / ORC_BN_CELL);
+      if (w <= 0.0f) continue;
+      e[a] = vadd(e[a], vscale(bnRgb(vol, cc, a), w));
+      ws[a] += w;
+    }
+  }
+  v3 s = V3(0, 0, 0);
+  float ps = 0.0f;
+  for (int a = 0; a < 6; a++) {
+    if (ws[a] > 1e-6f) s = vadd(s, vdivs(vscale(e[a], P[a]), ws[a]));
+    ps += P[a];
+  }
+  return vdivs(s, fmx(ps, 1e-9f));
+}
+
 /* shadeHitV (rc.ts:108-116) at a voxel hit with the voxel's attributes (voxel.ts:150-156, 185-192), no bounce,
  * the frame's sun visibility; plus the point lights of this path (none: the reference shading). */
 static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entry, uint32_t br, uint32_t bit) {
@@ -976,7 +1021,7 @@ static uint32_t fixq(float x) {
   float r = rintf(x * DYN_FIX_);
   return (uint32_t)(r < 4294967040.0f ? r : 4294967040.0f);
 }
-static void splatSample(const orc_vox* p, orc_dyn* D, int L, v3 pt, v3 n, float area, v3 rad) {
+static void splatSample(const orc_vox* p, orc_dyn* D, int L, v3 pt, v3 n, float area, v3 rad, v3 alb, int albw) {
   float v = voxSize(p, (uint32_t)L);
   v3 g = vsub(vdivs(vsub(pt, lo3(p, (uint32_t)L)), v), V3(0.5f, 0.5f, 0.5f));
   iv3 c0 = vi(vfloor(g));
@@ -998,8 +1043,9 @@ static void splatSample(const orc_vox* p, orc_dyn* D, int L, v3 pt, v3 n, float 
     uint32_t vi_ = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
     D->mask[ptr * 16 + (vi_ >> 5)] |= 1u << (vi_ & 31u);
     uint32_t* dd = D->data + ((size_t)ptr * 512 + vi_) * DYN_W;
-    float x[10] = {w * pn.x, w * nn.x, w * pn.y, w * nn.y, w * pn.z, w * nn.z, w, w * rad.x, w * rad.y, w * rad.z};
-    for (int k = 0; k < 10; k++) dd[k] += fixq(x[k]);
+    float x[13] = {w * pn.x, w * nn.x, w * pn.y, w * nn.y, w * pn.z, w * nn.z, w, w * rad.x, w * rad.y, w * rad.z,
+                   w * alb.x, w * alb.y, w * alb.z};
+    for (int k = 0; k < (albw ? 13 : 10); k++) dd[k] += fixq(x[k]);
   }
 }
 
@@ -1148,7 +1194,7 @@ int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const fl
           float ndl = fmx(vdot(nv, sd), 0.0f);
           v3 rad = V3(0, 0, 0);
           if (ndl > 0.0f) rad = vscale(vscale(vmul(vdivs(alb, PI_F), se), ndl), rc->sun_vis);
-          splatSample(p, D, L, pt, nv, sa, rad);
+          splatSample(p, D, L, pt, nv, sa, rad, alb, rc->objects);
         }
   }
   /* resolve, fine to coarse */
@@ -1261,6 +1307,44 @@ static void dynRead(const orc_dyn* D, uint32_t base, uint32_t L, v3 d, float out
   out[2] = dynWord(D, base, 9, L) / a;
   out[3] = fmn(front, 1.0f);
 }
+/* dynInject (dynlayer.ts:166-190) with the bounce volume's dynE: each voxel of the allocated bricks with coverage
+ * adds its area x albedo / pi x the irradiance of the previous volume at its centre, weighted over its six projected
+ * areas; level 0 in fixed point (round half to even), the coarser levels as floats. */
+int orc_dyn_inject(const orc_vox* p, const orc_dyn* D, const uint16_t* vol, const int32_t vorg[3]) {
+  const int G = p->G;
+  const uint32_t per = (uint32_t)(G * G * G), n = D->hdr[8];
+  const float tol = 0.5f * p->v0;
+  for (uint32_t slot = 0; slot < n; slot++) {
+    const uint32_t pi = D->meta[slot], L = pi / per, r = pi % per;
+    for (uint32_t bit = 0; bit < 512u; bit++) {
+      const uint32_t base = (slot * 512u + bit) * DYN_W;
+      if (dynWord(D, base, 6, L) <= 0.0f) continue;
+      float P[6];
+      for (uint32_t k = 0; k < 6; k++) P[k] = dynWord(D, base, k, L);
+      iv3 c = iadd(imuls(I3((int)(r % G), (int)((r / G) % G), (int)(r / (G * G))), 8),
+                   I3((int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)));
+      float v = voxSize(p, L);
+      v3 pc = vadd(vscale(vscale(vf(org(p, L)), v), 8.0f), vscale(vadd(vf(c), V3(0.5f, 0.5f, 0.5f)), v));
+      v3 E = dynE(vol, vorg, tol, pc, P);
+      v3 alb = V3(dynWord(D, base, 10, L), dynWord(D, base, 11, L), dynWord(D, base, 12, L));
+      v3 add = vmul(vdivs(alb, PI_F), E);
+      float ad[3] = {add.x, add.y, add.z};
+      for (uint32_t k = 0; k < 3; k++) {
+        uint32_t* w = D->data + base + 7 + k;
+        if (L == 0) {
+          *w += (uint32_t)rintf(ad[k] * DYN_FIX_);
+        } else {
+          float x;
+          memcpy(&x, w, 4);
+          x += ad[k];
+          memcpy(w, &x, 4);
+        }
+      }
+    }
+  }
+  return 0;
+}
+
 /* traceDyn (dynlayer.ts:118-160) */
 static void traceDyn(const orc_vox* p, const orc_dyn* D, v3 o, v3 d, float t0, float t1, float kc, float out[4]) {
   out[0] = out[1] = out[2] = out[3] = 0.0f;
@@ -1676,7 +1760,15 @@ int orc_bounce_volume(const orc_vox* p, const uint32_t* vox, const orc_rc* rc0, 
       }
     }
     float rad[ORC_BN_RAYS][4], reach[6];
-    for (int r = 0; r < ORC_BN_RAYS; r++) bnTrace(&W, &rc, pt, dirs[r], 1e9f, rad[r]);
+    for (int r = 0; r < ORC_BN_RAYS; r++) {
+      bnTrace(&W, &rc, pt, dirs[r], 1e9f, rad[r]);
+      if (rc.objects && rc.dyn) {
+        /* bnTraceDyn (bounce.ts:173-178): the layer in front of the static result, read with one ray's cone */
+        float dl[4];
+        traceDyn(p, rc.dyn, pt, dirs[r], 0.0f, fmn(rad[r][3], 1e4f), sqrtf(4.0f * PI_F / (float)ORC_BN_RAYS), dl);
+        for (int k = 0; k < 3; k++) rad[r][k] = dl[k] + (1.0f - dl[3]) * rad[r][k];
+      }
+    }
     for (int a = 0; a < 6; a++) {
       float sg = (a & 1) ? -1.0f : 1.0f;
       v3 d = V3(a / 2 == 0 ? sg : 0.0f, a / 2 == 1 ? sg : 0.0f, a / 2 == 2 ? sg : 0.0f);

@@ -53,6 +53,8 @@ typedef struct orc_rc {
   const uint16_t* bn_prev;  /* set only inside orc_bounce_volume: hits read this volume (bounce.ts voxBounce) */
   int32_t bn_prev_org[3];
   const orc_dyn* dyn;       /* the dynamic layer the trace reads in front of the static hit, or NULL */
+  int32_t objects;          /* token obj (rc.ts:79-81): the layer splats albedo, receives the bounce volume's light
+                             * (dynInject) and blocks the bounce volume's irradiance rays (BN_DYN) */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -84,6 +86,9 @@ uint64_t orc_rays(const orc_rc* rc, int c);
 int orc_sun(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, float* sunvis);
 /* The dynamic layer's initial state (every pointer empty, no bits, marking bounds at their identities). */
 int orc_dyn_init(const orc_vox* p, orc_dyn* d);
+/* dynInject with the bounce volume (dynlayer.ts:166-190, dynE of bounce.ts:267-298): every voxel of the layer's
+ * allocated bricks gains albedo / pi x the irradiance the previous volume `vol` (origin vorg) gives it. */
+int orc_dyn_inject(const orc_vox* p, const orc_dyn* d, const uint16_t* vol, const int32_t vorg[3]);
 /* One update of the dynamic layer (mark, parents, alloc + finalize, clear, splat, resolve; dynlayer.ts BUILD_WGSL)
  * over the clipmap windows of p, shading samples with rc's frame (sun, sun_vis). */
 int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const float* nrm, const uint32_t* tri,

@@ -186,6 +186,8 @@ TraceK trace_args(const rtgi_world* w) {
   std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
   t.dirs = w->dirs;
   t.fp16 = (w->rp.flags & RTGI_RC_FP16) ? 1 : 0;
+  t.objects = ((w->rp.flags & RTGI_RC_OBJECTS) && (w->rp.flags & RTGI_RC_BOUNCE) && w->dyn_hdr &&
+               w->dyn_first < w->ntris) ? 1 : 0;
   t.no_merge = (w->rp.flags & RTGI_RC_NO_MERGE) ? 1 : 0;
   t.reloc = (w->rp.flags & RTGI_RC_RELOCATE) ? 1 : 0;
   t.lift = (float)w->rp.lift;
@@ -316,6 +318,7 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
     bounce_advance(w);
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
+    if (!e) e = launch_dyn_inject(t, s);
     if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
@@ -323,6 +326,7 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
     bounce_advance(w);
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
+    if (!e) e = launch_dyn_inject(t, s);
     if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     if (!e) e = launch_trace(t, s);
@@ -1151,6 +1155,7 @@ int plan_launch_stage(const BatchPlan& p, hipStream_t s, int stage) {
     if (!e) e = launch_dyn_build_batch(p, s);
   } else if (stage == 1) {
     e = launch_sun_batch(p, s);
+    if (!e) e = launch_dyn_inject_batch(p, s);
     if (!e) e = launch_bounce_batch(p, s);
     if (!e) e = launch_relocate_batch(p, s);
     if (p.fused)

@@ -929,6 +929,7 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
     out[BL_DL_CLEAR] = vox;
     out[BL_DL_SPLAT] = std::min<uint32_t>(8192, cdiv(nd, 16));  // workgroups of 16 triangles (one wavefront)
     for (int L = 1; L < k.NL; L++) out[BL_DL_RESOLVE + L - 1] = vox;
+    if (t.objects) out[BL_DL_INJECT] = vox;
   }
   // Per attribute slot: 4 lanes per voxel for the sun taps, one for the bounce injection (grid-stride loops).
   const uint64_t slots = (uint64_t)k.max_voxels;

@@ -280,8 +280,8 @@ __device__ __forceinline__ uint32_t fix(float x) { return (uint32_t)fminf(rintf(
 constexpr uint32_t SPLAT_WG = 64;
 constexpr uint32_t SPLAT_LPT = 4;                     // lanes per triangle, over its samples
 constexpr uint32_t SPLAT_TPW = SPLAT_WG / SPLAT_LPT;  // triangles per workgroup
-constexpr uint32_t SPLAT_WIN = 512;  // voxels: 512 x 10 words = 20 KiB of LDS
-constexpr uint32_t SPLAT_W = 10;     // words splatted (the albedo words 10..12 stay 0)
+constexpr uint32_t SPLAT_WIN = 512;  // voxels: 512 x 13 words = 26 KiB of LDS
+constexpr uint32_t SPLAT_W = 13;     // words splatted: 10, or 13 with the albedo words (token obj)
 
 struct SplatWin {
   uint32_t acc[SPLAT_WIN * SPLAT_W];
@@ -301,7 +301,7 @@ __device__ __forceinline__ int wave_max_i(int x) {
 // its projected areas (front and back per axis), area and area x radiance in units of v^2; into the window when the
 // corner lies in it (level wl, voxels wo .. wo + wd - 1), else into the layer.
 __device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, int L, f3 p, f3 n, float area,
-                             f3 rad) {
+                             f3 rad, f3 alb, uint32_t nw) {
   const DynK& D = k.dyn;
   float v = vox_size(k, L);
   f3 g = (p - region_lo(k, L)) / v - splat3(0.5f);
@@ -318,13 +318,14 @@ __device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, i
     f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
     float w = tw.x * tw.y * tw.z * s;
     if (w <= 0.0f) continue;
-    const uint32_t x[SPLAT_W] = {fix(w * pn.x), fix(w * nn.x), fix(w * pn.y), fix(w * nn.y), fix(w * pn.z),
-                                 fix(w * nn.z), fix(w),        fix(w * rad.x), fix(w * rad.y), fix(w * rad.z)};
+    const uint32_t x[SPLAT_W] = {fix(w * pn.x),  fix(w * nn.x),  fix(w * pn.y),  fix(w * nn.y), fix(w * pn.z),
+                                 fix(w * nn.z),  fix(w),         fix(w * rad.x), fix(w * rad.y), fix(w * rad.z),
+                                 fix(w * alb.x), fix(w * alb.y), fix(w * alb.z)};
     i3 r = c - wo;
     if (L == wl && r.x >= 0 && r.y >= 0 && r.z >= 0 && r.x < wd.x && r.y < wd.y && r.z < wd.z) {
       uint32_t j = (uint32_t)(r.x + wd.x * (r.y + wd.y * r.z));
       atomicOr(&W.touched[j >> 5], 1u << (j & 31u));
-      for (uint32_t q2 = 0; q2 < SPLAT_W; q2++)
+      for (uint32_t q2 = 0; q2 < nw; q2++)
         if (x[q2]) atomicAdd(&W.acc[j * SPLAT_W + q2], x[q2]);
       continue;
     }
@@ -334,7 +335,7 @@ __device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, i
     uint32_t vi = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
     atomicOr(&D.mask[ptr * 16u + (vi >> 5)], 1u << (vi & 31u));
     uint32_t* d = D.data + ((size_t)ptr * 512u + vi) * DYN_WORDS;
-    for (uint32_t q2 = 0; q2 < SPLAT_W; q2++)
+    for (uint32_t q2 = 0; q2 < nw; q2++)
       if (x[q2]) atomicAdd(&d[q2], x[q2]);
   }
 }
@@ -345,6 +346,7 @@ __device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, i
 __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, SplatWin& W, uint32_t lb, uint32_t nblk) {
   const DynK& D = k.dyn;
   const uint32_t n = D.end_tri - D.first_tri, lane = threadIdx.x, sub = lane % SPLAT_LPT;
+  const uint32_t nw = t.objects ? 13u : 10u;  // the albedo words with token obj (DYN_ALB), else they stay 0
   const f3 sd = mk3(t.sun_dir[0], t.sun_dir[1], t.sun_dir[2]), se = mk3(t.sun_e[0], t.sun_e[1], t.sun_e[2]);
   for (uint32_t i0 = lb * SPLAT_TPW; i0 < n; i0 += nblk * SPLAT_TPW) {
     const uint32_t i = i0 + lane / SPLAT_LPT;
@@ -376,7 +378,7 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, Splat
         wd = i3{e.x, e.y, min(e.z, (int)(SPLAT_WIN / ((uint32_t)e.x * (uint32_t)e.y)))};
     }
     const uint32_t vol = (uint32_t)(wd.x * wd.y * wd.z);
-    for (uint32_t j = lane; j < vol * SPLAT_W; j += SPLAT_WG) W.acc[j] = 0u;
+    for (uint32_t j = lane; j < vol * SPLAT_W; j += SPLAT_WG) W.acc[j] = 0u;  // all 13 words: the flush reads nw
     for (uint32_t j = lane; j < SPLAT_WIN / 32; j += SPLAT_WG) W.touched[j] = 0u;
     __syncthreads();
     if (L >= 0) {
@@ -408,7 +410,7 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, Splat
           float ndl = fmaxf(dot3(nv, sd), 0.0f);
           f3 rad = splat3(0.0f);
           if (ndl > 0.0f) rad = alb / PI_F * se * ndl * t.sun_vis;
-          splat_sample(k, W, wl, wo, wd, L, p, nv, sa, rad);
+          splat_sample(k, W, wl, wo, wd, L, p, nv, sa, rad, alb, nw);
         }
       }
     }
@@ -425,7 +427,7 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, Splat
       uint32_t vi = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
       atomicOr(&D.mask[ptr * 16u + (vi >> 5)], 1u << (vi & 31u));
       uint32_t* d = D.data + ((size_t)ptr * 512u + vi) * DYN_WORDS;
-      for (uint32_t q = 0; q < SPLAT_W; q++) {
+      for (uint32_t q = 0; q < nw; q++) {
         const uint32_t x = W.acc[j * SPLAT_W + q];
         if (x) atomicAdd(&d[q], x);
       }

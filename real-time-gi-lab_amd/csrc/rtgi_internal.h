@@ -160,6 +160,7 @@ struct TraceK {
   uint32_t bn_frame;       // updates of the volume so far, this one included
   int32_t bn_level;        // levelOf(BN_CELL): the clipmap level of the near-surface test
   uint32_t bn_dir_off;     // first of the BN_RAYS irradiance directions in dirs
+  int32_t objects;         // token obj with bounce and a dynamic layer: albedo splats, dynInject, bounce rays read it
   uint64_t total_probes;
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
@@ -221,6 +222,7 @@ enum BatchLaunch : int {
   BL_DL_ALLOC,                             //   slots in bit order (one block of 1024), counts, bounds
   BL_DL_CLEAR,                             //   the allocated bricks' words
   BL_DL_SPLAT,                             //   fixed-point splats of the dynamic triangles' samples
+  BL_DL_INJECT,                            //   the bounce volume's light into the layer (token obj)
   BL_DL_RESOLVE,                           //   + level - 1 (levels 1..7): coarse values from own splats + children
   BL_SUN = BL_DL_RESOLVE + RTGI_MAX_LEVELS - 1,  // traced sun visibility of every occupied voxel
   BL_BNVOL,                                // bounce volume update (8 probes per block)
@@ -251,6 +253,9 @@ int launch_vox_build_batch(const BatchPlan& p, void* stream);
 int launch_dyn_build(const VoxK& k, void* stream);
 int launch_dyn_splat(const TraceK& t, void* stream);
 int launch_dyn_build_batch(const BatchPlan& p, void* stream);
+// dynInject (token obj): after the layer's build, before the bounce volume update that reads the layer.
+int launch_dyn_inject(const TraceK& t, void* stream);
+int launch_dyn_inject_batch(const BatchPlan& p, void* stream);
 int launch_trace_batch(const BatchPlan& p, void* stream);
 int launch_merge_batch(const BatchPlan& p, int step, void* stream);
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream);

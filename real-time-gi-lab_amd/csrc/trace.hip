@@ -1203,6 +1203,97 @@ __device__ f3 bn_read(const uint32_t* vol, const int32_t* org, float tol, f3 q, 
   return ws > 1e-6f ? mk3(e.x / ws, e.y / ws, e.z / ws) : splat3(0.0f);
 }
 
+// dynE (bounce.ts:267-298): the irradiance a dynamic voxel at p with the projected areas P (+x, -x, +y, -y, +z, -z)
+// receives from the volume vol: per face a, the mean over the 8 probes around p that are active, see p within
+// their reach (+ tol) and lie on that face's side of p (smoothstep weight) of the probe's texel a; the faces weighted
+// by P.
+__device__ f3 dyn_e(const uint32_t* vol, const int32_t* org, float tol, f3 p, const float P[6]) {
+  const uint2* V = reinterpret_cast<const uint2*>(vol);
+  i3 o{org[0], org[1], org[2]};
+  f3 g = mk3(p.x / BN_CELL, p.y / BN_CELL, p.z / BN_CELL) - tof3(o);
+  i3 bb = f2i3(floor3(g));
+  f3 fr = g - tof3(bb);
+  f3 e[6];
+  float ws[6];
+  for (int q = 0; q < 6; q++) {
+    e[q] = splat3(0.0f);
+    ws[q] = 0.0f;
+  }
+  for (int k = 0; k < 8; k++) {
+    i3 off{k & 1, (k >> 1) & 1, (k >> 2) & 1};
+    i3 cc = bb + off;
+    if (!bn_inside(cc)) continue;
+    f3 pp = tof3(o + cc) * BN_CELL;
+    f3 u = p - pp;
+    const uint2* t = V + (size_t)bn_probe(cc) * 6u;
+    if (!(bn_reach(t[0]) > 0.0f)) continue;
+    if (fabsf(u.x) > bn_reach(t[u.x >= 0.0f ? 0 : 1]) + tol || fabsf(u.y) > bn_reach(t[u.y >= 0.0f ? 2 : 3]) + tol ||
+        fabsf(u.z) > bn_reach(t[u.z >= 0.0f ? 4 : 5]) + tol)
+      continue;
+    f3 t3 = mk3(off.x ? fr.x : 1.0f - fr.x, off.y ? fr.y : 1.0f - fr.y, off.z ? fr.z : 1.0f - fr.z);
+    const float wt = t3.x * t3.y * t3.z;
+    for (int q = 0; q < 6; q++) {
+      if (P[q] <= 0.0f) continue;
+      const float sg = (q & 1) ? -1.0f : 1.0f;
+      f3 n = mk3(q / 2 == 0 ? sg : 0.0f, q / 2 == 1 ? sg : 0.0f, q / 2 == 2 ? sg : 0.0f);
+      float w = wt * smooth_step(-0.05f, 0.25f, dot3(pp - p, n) / BN_CELL);
+      if (w <= 0.0f) continue;
+      e[q] = e[q] + bn_rgb(t[q]) * w;
+      ws[q] += w;
+    }
+  }
+  f3 s = splat3(0.0f);
+  float ps = 0.0f;
+  for (int q = 0; q < 6; q++) {
+    if (ws[q] > 1e-6f) {
+      f3 m = e[q] * P[q];
+      s = s + mk3(m.x / ws[q], m.y / ws[q], m.z / ws[q]);
+    }
+    ps += P[q];
+  }
+  const float pm = fmaxf(ps, 1e-9f);
+  return mk3(s.x / pm, s.y / pm, s.z / pm);
+}
+
+// dynInject (dynlayer.ts:166-190) with dynE (token obj): every voxel of the layer's allocated bricks with coverage
+// adds its albedo / pi x the previous bounce volume's irradiance at its centre (the albedo words already hold the
+// area weight) to its radiance words; level 0 in fixed point (round half to even), the coarser levels as floats.
+// One thread per voxel: each voxel's words are its own.
+__device__ __forceinline__ void dyn_inject_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+  if (!a.objects) return;
+  const VoxK& k = a.vx;
+  const DynK& D = k.dyn;
+  const uint32_t n = D.hdr[8], G = (uint32_t)k.G, G3 = (uint32_t)k.per;
+  const float tol = 0.5f * k.v0;
+  for (uint32_t i = lb * BLK + threadIdx.x; i < n * 512u; i += nblk * BLK) {
+    const uint32_t slot = i >> 9, bit = i & 511u;
+    const uint32_t pi = D.meta[slot], L = pi / G3, r = pi % G3;
+    uint32_t* d = D.data + ((size_t)slot * 512u + bit) * DYN_WORDS;
+    float wv[DYN_WORDS];
+    for (uint32_t q = 0; q < DYN_WORDS; q++) wv[q] = L == 0u ? (float)d[q] / DYN_FIX : __uint_as_float(d[q]);
+    if (wv[6] <= 0.0f) continue;
+    i3 c = i3{(int)(r % G), (int)((r / G) % G), (int)(r / (G * G))} * 8 +
+           i3{(int)(bit & 7u), (int)((bit >> 3) & 7u), (int)(bit >> 6)};
+    const float v = k.v0 * (float)(1u << L);
+    f3 o = mk3((float)k.origin[L][0], (float)k.origin[L][1], (float)k.origin[L][2]);
+    f3 pc = o * v * 8.0f + (tof3(c) + splat3(0.5f)) * v;
+    f3 E = dyn_e(a.bn_prev, a.bn_prev_org, tol, pc, wv);
+    const float add[3] = {wv[10] / PI_F * E.x, wv[11] / PI_F * E.y, wv[12] / PI_F * E.z};
+    for (uint32_t q = 0; q < 3; q++) {
+      if (L == 0u)
+        d[7u + q] += (uint32_t)rintf(add[q] * DYN_FIX);
+      else
+        d[7u + q] = __float_as_uint(wv[7u + q] + add[q]);
+    }
+  }
+}
+__global__ void __launch_bounds__(BLK) k_dl_inject(TraceK a) { dyn_inject_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_dl_inject_b(BatchK b) {
+  const uint32_t blk = batch_block(b);
+  int w = batch_world(b, blk);
+  dyn_inject_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+}
+
 // A voxel hit of a bounce-volume ray (voxShade with the voxBounce of bounce.ts:96-99): the bounce term reads the
 // previous update's volume at a point a quarter voxel back from the entry, on the side the ray comes from (voxSide,
 // src/core/rc.ts:122-126); a voxel without a normal is read facing the ray. Otherwise the cascades' shading.
@@ -1309,7 +1400,14 @@ __device__ __forceinline__ void bnvol_body(const TraceK& a, uint32_t lb) {
   if (mode == 2) {
     if (li < BN_RAYS) {
       const DirEntry& de = a.dirs[a.bn_dir_off + li];
-      float4 r = bn_trace(a, p, mk3(de.d[0], de.d[1], de.d[2]), mk3(de.inv[0], de.inv[1], de.inv[2]), 1e9f, true);
+      const f3 dir = mk3(de.d[0], de.d[1], de.d[2]);
+      float4 r = bn_trace(a, p, dir, mk3(de.inv[0], de.inv[1], de.inv[2]), 1e9f, true);
+      if (a.objects) {  // bnTraceDyn (bounce.ts:173-178): the layer in front of the static result, one ray's cone
+        float4 dl = trace_dyn(a, p, dir, 0.0f, fminf(r.w, 1e4f), sqrtf(4.0f * PI_F / (float)BN_RAYS));
+        r.x = dl.x + (1.0f - dl.w) * r.x;
+        r.y = dl.y + (1.0f - dl.w) * r.y;
+        r.z = dl.z + (1.0f - dl.w) * r.z;
+      }
       s_rad[g][li][0] = r.x;
       s_rad[g][li][1] = r.y;
       s_rad[g][li][2] = r.z;
@@ -1499,6 +1597,14 @@ int launch_bounce(const TraceK& k, void* stream) {
   return (int)hipGetLastError();
 }
 
+int launch_dyn_inject(const TraceK& k, void* stream) {
+  if (!k.objects) return 0;
+  uint32_t g[BL_COUNT];
+  batch_blocks(k, g);
+  if (g[BL_DL_INJECT]) k_dl_inject<<<g[BL_DL_INJECT], BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
+}
+
 int launch_relocate(const TraceK& k, void* stream) {
   if (!k.reloc || !k.total_probes) return 0;
   k_relocate<<<grid_for(k.total_probes), BLK, 0, (hipStream_t)stream>>>(k);
@@ -1530,6 +1636,11 @@ int launch_sun_batch(const BatchPlan& p, void* stream) {
 int launch_bounce_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_BNVOL]) k_bnvol_b<<<p.grid[BL_BNVOL], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_BNVOL));
   if (p.grid[BL_BOUNCE]) k_bounce_b<<<p.grid[BL_BOUNCE], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_BOUNCE));
+  return (int)hipGetLastError();
+}
+
+int launch_dyn_inject_batch(const BatchPlan& p, void* stream) {
+  if (p.grid[BL_DL_INJECT]) k_dl_inject_b<<<p.grid[BL_DL_INJECT], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_DL_INJECT));
   return (int)hipGetLastError();
 }
 

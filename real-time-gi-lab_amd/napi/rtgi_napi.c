@@ -90,7 +90,7 @@ static napi_value js_device_count(napi_env env, napi_callback_info info) {
 }
 
 /* createWorld(vox: {v0, levels, G, maxBricks, maxVoxels}, rc: {cascades, probes0, s0, kInterval, levelMap, voxScale,
- * voxOffset, voxSkip, resCap, recordHits, relocate, lift, sunTrace, bounce, fp16}) -> handle */
+ * voxOffset, voxSkip, resCap, recordHits, relocate, lift, sunTrace, bounce, objects, fp16}) -> handle */
 static napi_value js_create_world(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -121,6 +121,7 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[1], "relocate", &x) && x != 0) rp.flags |= RTGI_RC_RELOCATE;
   if (get_f64(env, argv[1], "sunTrace", &x) && x != 0) rp.flags |= RTGI_RC_SUN_TRACE;
   if (get_f64(env, argv[1], "bounce", &x) && x != 0) rp.flags |= RTGI_RC_BOUNCE;
+  if (get_f64(env, argv[1], "objects", &x) && x != 0) rp.flags |= RTGI_RC_OBJECTS;
   if (get_f64(env, argv[1], "fp16", &x) && x != 0) rp.flags |= RTGI_RC_FP16;
   rtgi_world* w = NULL;
   int st = rtgi_world_create(&vp, &rp, &w);

@@ -256,3 +256,59 @@ def test_set_dynamic_and_tuning_after_batch_creation(require_gpu):
         b.destroy()
         for w in (single, bw):
             w.destroy()
+
+
+@pytest.mark.parametrize("fp16", [False, True])
+def test_objects_token_matches_oracle(require_gpu, fp16):
+    """Token obj (rc.ts:79-81) with the bounce volume: the layer also splats albedo (words 10-12), receives the
+    previous volume's light (dynInject, dynlayer.ts:166-190, with dynE, bounce.ts:267-298) and the volume's
+    irradiance rays read it (bnTraceDyn, bounce.ts:173-178). Over three updates of the moving atrium: the layer's 13
+    words and the bounce volume bit for bit, the cascades and SH to the plain bars. The FP16 case also runs a batch
+    of two worlds through a graph and meets the single world bit for bit."""
+    vox, rc, frames = atrium_frames()
+    first = frames[0][0].firstDynamicTri
+    rcf = dataclasses.replace(rc, fp16=fp16, bounce=True, objects=True, recordHits=True)
+    worlds = [rtgi.World(vox, rcf) for _ in range(3 if fp16 else 1)]
+    for w in worlds:
+        w.upload_scene(frames[0][0])
+        w.set_dynamic(first, 512)
+    w = worlds[0]
+    b = rtgi.Batch(worlds[1:], graph=True) if fp16 else None
+    d = DynState(vox, 512, first)
+    state = None
+    try:
+        for k, (scene, f) in enumerate(frames):
+            say(f"case: objects token, atrium, fp16={fp16}, update {k + 1}")
+            for x in worlds:
+                if k:
+                    x.update_vertices(0, scene.pos, scene.nrm)
+                x.set_frame(f)
+            w.encode()
+            w.sync()
+            o = Oracle(vox, rcf, f)
+            o.build(scene, dyn=d)
+            before = d.data[:int(d.hdr[8]), :, 7:10].copy()
+            state = o.bounce_update(state)
+            n = check_layer(w, d)
+            assert n > 0
+            # from the second update on the previous volume lights the boxes: dynInject raised their radiance words
+            assert k == 0 or (np.any(d.data[:n, :, 7:10] != before) and np.all(d.data[:n, :, 7:10] >= before))
+            vol, org = w.read_bounce_volume()
+            assert np.array_equal(org, state[1])
+            assert np.array_equal(vol.view(np.uint16).reshape(-1, 6, 4), state[0])
+            check_update(w, o, rc, fp16)
+            if b is not None:
+                b.encode()
+                b.sync()
+                lay = w.read_dynamic()
+                for x in worlds[1:]:
+                    got = x.read_dynamic()
+                    assert np.array_equal(got["hdr"], lay["hdr"])
+                    assert np.array_equal(got["data"][:n], lay["data"][:n])
+                    assert np.array_equal(x.read_bounce_volume()[0], vol)
+                    assert np.array_equal(x.read_sh(), w.read_sh())
+    finally:
+        if b is not None:
+            b.destroy()
+        for x in worlds:
+            x.destroy()

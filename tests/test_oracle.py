@@ -732,6 +732,58 @@ def test_dynamic_layer_closed_forms():
     assert np.array_equal(rh, rs) and np.array_equal(hh, hs)
 
 
+def test_objects_inject_closed_form():
+    """Token obj (dynInject with the bounce volume's dynE): on a hand-made previous volume whose every probe is active
+    (reach 1.8) with the same rgb E0 on every axis, the dynamic square (facing +x) reads its +x face only, and only the
+    probes in front of it (smoothstep of their offset along +x) count: E = E0. Every voxel of the layer, at every
+    level, gains area x albedo / pi x E0 on its radiance words, so its mean radiance rises by albedo / pi x E0; the
+    albedo words hold area x albedo. Without dynamic triangles the token changes nothing: the same bounce volume bit
+    for bit and an empty layer."""
+    scene, vox, rc, frame = dyn_world()
+    rcb = dataclasses.replace(rc, bounce=True, objects=True)
+    E0 = np.array([10.0, 20.0, 30.0], np.float32)
+    alb = np.array([0.8, 0.4, 0.2], np.float32)
+    o = Oracle(vox, rcb, frame)
+    d = DynState(vox, 64, scene.firstDynamicTri)
+    o.build(scene, dyn=d)
+    before = d.data.copy()
+    n, per = int(d.hdr[8]), vox.G ** 3
+    org = orc.bn_origin(frame.eye)
+    o.bounce_update((_volume(org, lambda x: tuple(E0.tolist())), org, 1))
+    for s in range(n):
+        L = int(d.meta[s]) // per
+        b, a = before[s], d.data[s]
+        if L == 0:
+            w6 = b[:, 6].astype(np.float64)
+            m = w6 > 0.1 * 65536
+            albw = b[:, 10:13][m].astype(np.float64) / w6[m][:, None]
+            tol = 0.5 / w6[m][:, None]  # the injected add is rounded to 1/65536
+            gain = (a[m][:, 7:10].astype(np.float64) - b[m][:, 7:10]) / w6[m][:, None]
+        else:
+            fb, fa = b.view(np.float32).astype(np.float64), a.view(np.float32).astype(np.float64)
+            w6 = fb[:, 6]
+            m = w6 > 0.1
+            albw = fb[m][:, 10:13] / w6[m][:, None]
+            tol = 0.0
+            gain = (fa[m][:, 7:10] - fb[m][:, 7:10]) / w6[m][:, None]
+        assert m.any()
+        # the albedo words: area x albedo (fixed-point rounding of the level-0 sums shows in the 4th digit)
+        np.testing.assert_allclose(albw, np.tile(alb, (m.sum(), 1)), rtol=1e-3, atol=0)
+        # the gain: albedo-area / pi x E0, per voxel
+        want = albw / np.float32(3.14159265) * E0
+        assert np.all(np.abs(gain - want) <= tol + 1e-5 * want), L
+    # No dynamic triangles: identical to the plain bounce volume.
+    plain = Oracle(vox, dataclasses.replace(rc, bounce=True), frame)
+    plain.build(scene)
+    obj = Oracle(vox, rcb, frame)
+    d0 = DynState(vox, 64, scene.ntris)
+    obj.build(scene, dyn=d0)
+    assert d0.hdr[8] == 0
+    v1 = plain.bounce_update(None)
+    v2 = obj.bounce_update(None)
+    assert np.array_equal(v1[0], v2[0])
+
+
 def test_log2_series_against_float64():
     """The level blend's log2 (a fixed sequence of float operations, so that the device gives the same bits) against
     numpy's float64 log2: within 2^-22 absolute plus one float step of the result over [1, 2^10] (the blend's range:
