@@ -285,8 +285,9 @@ def full_bench(a, scenes, vox, rc, frame0, tuning, bsz):
 def dyn_bench(a):
     """The atrium, whose 16 moving boxes are the tail of its triangle table, with the dynamic coverage layer: each
     update splats the boxes into the layer (mark, parents, allocate, clear, splat, resolve) and the cascade rays read
-    it in front of their static hits. Against the same worlds with the boxes voxelised with the rest. Batched hipGraph
-    updates of a.dyn_worlds atrium worlds (seeds 1..n), fused per-cascade trace+merge."""
+    it in front of their static hits. Against the same worlds with the boxes voxelised with the rest; and, with the
+    bounce volume on, with and without token obj (the boxes also receive the volume's light and block and reflect its
+    rays). Batched hipGraph updates of a.dyn_worlds atrium worlds (seeds 1..n), fused per-cascade trace+merge."""
     import dataclasses
 
     import rtgi
@@ -294,10 +295,13 @@ def dyn_bench(a):
     _, vox, rc, f0 = rtgi.config("atrium")
     scenes = [rtgi.generate_scene("atrium", seed=s) for s in range(1, a.dyn_worlds + 1)]
     out = {}
-    for layer in (False, True):
+    variants = (("boxes_in_bricks", False, rc), ("dynamic_layer", True, rc),
+                ("layer_bounce", True, dataclasses.replace(rc, bounce=True)),
+                ("layer_bounce_objects", True, dataclasses.replace(rc, bounce=True, objects=True)))
+    for key, layer, rcv in variants:
         worlds = []
         for sc in scenes:
-            w = rtgi.World(vox, rc)
+            w = rtgi.World(vox, rcv)
             w.upload_scene(sc)
             if layer:
                 w.set_dynamic(sc.firstDynamicTri, 512)
@@ -323,7 +327,6 @@ def dyn_bench(a):
             b.sync()
             for k, v in b.stage_ms().items():
                 ms[k] = ms.get(k, 0.0) + v / ts
-        key = "dynamic_layer" if layer else "boxes_in_bricks"
         out[key] = {"updates_per_s": round(value, 3), "stage_ms_per_batch": {k: round(v, 4) for k, v in ms.items()}}
         if layer:
             out[key]["layer_bricks_world1"] = worlds[0].read_dyn_counters()["bricks"]
@@ -331,7 +334,8 @@ def dyn_bench(a):
         for w in worlds:
             w.destroy()
     log(f"dynamic layer: {out['dynamic_layer']['updates_per_s']:.1f} updates/s, boxes in bricks "
-        f"{out['boxes_in_bricks']['updates_per_s']:.1f}")
+        f"{out['boxes_in_bricks']['updates_per_s']:.1f}; with bounce {out['layer_bounce']['updates_per_s']:.1f}, "
+        f"+ token obj {out['layer_bounce_objects']['updates_per_s']:.1f}")
     return {"value": out["dynamic_layer"]["updates_per_s"], "unit": "GI updates/s", "worlds": len(scenes),
             "config": "atrium (55,724 triangles, 1,292 of them the 16 moving boxes), FP32, 512 layer bricks",
             **out}
