@@ -21,7 +21,7 @@ import numpy as np
 from . import _lib as _L
 from ._lib import RtgiError, check, lib, ptr
 
-__all__ = ["VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World", "Batch",
+__all__ = ["pack_irradiance", "write_irradiance", "read_irradiance", "VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World", "Batch",
            "Server", "estimate_bytes", "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY",
            "device_count", "oct_solid_angles", "lib"]
 
@@ -153,6 +153,36 @@ def oct_solid_angles(res, samples=1 << 20):
     out = np.zeros(res * res, np.float32)
     check(lib().rtgi_oct_solid_angles(res, samples, ptr(out)))
     return out
+
+
+def pack_irradiance(rgb, tag):
+    """packIrradiance (src/runner/report.ts:54-59): per pixel r, g, b and the tag (0 invalid, 1 photoreal, >= 2 toon)
+    as one float32 array [n, 4], rows top to bottom: the layout of the reference's .irr.f32 dumps (--dump-irr), which
+    its comparators read back (src/native/compare_runs.ts:36-45). rgb: [n, 3] (a gather's output), tag: [n]."""
+    rgb = np.asarray(rgb, np.float32).reshape(-1, 3)
+    tag = np.asarray(tag).reshape(-1)
+    if tag.shape[0] != rgb.shape[0]:
+        raise ValueError(f"pack_irradiance: {rgb.shape[0]} rgb pixels but {tag.shape[0]} tags")
+    out = np.empty((rgb.shape[0], 4), np.float32)
+    out[:, :3] = rgb
+    out[:, 3] = tag
+    return out
+
+
+def write_irradiance(path, rgb, tag):
+    """pack_irradiance written as a .irr.f32 file: raw little-endian float32, no header."""
+    pack_irradiance(rgb, tag).astype("<f4").tofile(path)
+
+
+def read_irradiance(path):
+    """A .irr.f32 file back as (rgb [n, 3] float32, tag [n] uint8, valid [n] bool: tag > 0), as the reference's
+    comparators unpack it (compare_runs.ts:38-42)."""
+    v = np.fromfile(path, "<f4")
+    if v.size % 4:
+        raise ValueError(f"read_irradiance: {path} holds {v.size} floats, not a multiple of 4")
+    v = v.reshape(-1, 4)
+    tag = v[:, 3].astype(np.uint8)
+    return v[:, :3].astype(np.float32), tag, tag > 0
 
 
 def _vox_c(vox):

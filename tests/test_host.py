@@ -90,3 +90,22 @@ def test_scene_generator(kind, lo, hi):
     c = rtgi.generate_scene(kind, seed=3, frame=5)
     assert not np.array_equal(a.pos, c.pos)  # moving parts are posed per frame
     assert np.array_equal(a.tri, c.tri)
+
+
+def test_irradiance_dump_layout(tmp_path):
+    """The .irr.f32 dump (packIrradiance, src/runner/report.ts:54-59): 16 bytes per pixel, r g b tag as little-endian
+    float32, in pixel order; read back as the reference's comparator unpacks it (tag > 0 valid)."""
+    rgb = np.array([[0.5, 1.25, -0.0], [3.0, 0.0, 7.5], [1e-3, 2e-3, 3e-3]], np.float32)
+    tag = np.array([1, 0, 2], np.uint8)
+    f = tmp_path / "city.static.irr.f32"
+    rtgi.write_irradiance(f, rgb, tag)
+    raw = f.read_bytes()
+    assert len(raw) == 3 * 16
+    import struct
+    assert struct.unpack("<4f", raw[:16]) == (0.5, 1.25, -0.0, 1.0)
+    assert struct.unpack("<4f", raw[16:32]) == (3.0, 0.0, 7.5, 0.0)
+    got_rgb, got_tag, valid = rtgi.read_irradiance(f)
+    assert np.array_equal(got_rgb, rgb) and np.array_equal(got_tag, tag)
+    assert valid.tolist() == [True, False, True]
+    with pytest.raises(ValueError):
+        rtgi.pack_irradiance(rgb, tag[:2])
