@@ -51,8 +51,9 @@ __device__ __forceinline__ i3 clampi3(i3 a, i3 lo, i3 hi) {
   return i3{min(max(a.x, lo.x), hi.x), min(max(a.y, lo.y), hi.y), min(max(a.z, lo.z), hi.z)};
 }
 __device__ __forceinline__ i3 maxi3(i3 a, i3 b) { return i3{max(a.x, b.x), max(a.y, b.y), max(a.z, b.z)}; }
+// 0 <= a < n per component (n >= 0): one unsigned compare each, a negative component wrapping above n.
 __device__ __forceinline__ bool inside3(i3 a, int n) {
-  return a.x >= 0 && a.y >= 0 && a.z >= 0 && a.x < n && a.y < n && a.z < n;
+  return (uint32_t)a.x < (uint32_t)n && (uint32_t)a.y < (uint32_t)n && (uint32_t)a.z < (uint32_t)n;
 }
 
 // 2^-e for 0 <= e < 126 as an f32 built from its exponent bits: equal to 1.0f / (float)(1u << e), which is exact.
