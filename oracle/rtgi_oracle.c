@@ -1332,7 +1332,7 @@ int orc_dyn_inject(const orc_vox* p, const orc_dyn* D, const uint16_t* vol, cons
       for (uint32_t k = 0; k < 3; k++) {
         uint32_t* w = D->data + base + 7 + k;
         if (L == 0) {
-          *w += (uint32_t)rintf(ad[k] * DYN_FIX_);
+          *w += (uint32_t)fmn(rintf(ad[k] * DYN_FIX_), 4294967040.0f); /* clamped as the splat's words are */
         } else {
           float x;
           memcpy(&x, w, 4);

@@ -1281,7 +1281,7 @@ __device__ __forceinline__ void dyn_inject_body(const TraceK& a, uint32_t lb, ui
     const float add[3] = {wv[10] / PI_F * E.x, wv[11] / PI_F * E.y, wv[12] / PI_F * E.z};
     for (uint32_t q = 0; q < 3; q++) {
       if (L == 0u)
-        d[7u + q] += (uint32_t)rintf(add[q] * DYN_FIX);
+        d[7u + q] += (uint32_t)fminf(rintf(add[q] * DYN_FIX), 4294967040.0f);  // clamped as the splat's words are
       else
         d[7u + q] = __float_as_uint(wv[7u + q] + add[q]);
     }
