@@ -312,3 +312,48 @@ def test_objects_token_matches_oracle(require_gpu, fp16):
             b.destroy()
         for x in worlds:
             x.destroy()
+
+
+def test_objects_and_plain_bounce_worlds_in_one_batch(require_gpu):
+    """A batch of a token-obj world, a bounce world without the token and a token-obj world without a layer (the
+    token then does nothing): the inject launch has blocks for the first world only, and every world's update equals
+    its solo update bit for bit (SH, bounce volume, layer)."""
+    vox, rc, frames = atrium_frames(n=3, step=6)
+    first = frames[0][0].firstDynamicTri
+    rb = dataclasses.replace(rc, fp16=True, bounce=True)
+    ro = dataclasses.replace(rb, objects=True)
+    say("case: objects token, batch of [obj + layer, bounce + layer, obj without layer] vs solo, 3 updates")
+    kinds = [(ro, True), (rb, True), (ro, False)]
+    solo = [rtgi.World(vox, r) for r, _ in kinds]
+    inb = [rtgi.World(vox, r) for r, _ in kinds]
+    for ws in (solo, inb):
+        for w, (_, layer) in zip(ws, kinds):
+            w.upload_scene(frames[0][0])
+            if layer:
+                w.set_dynamic(first, 512)
+    b = rtgi.Batch(inb, graph=True)
+    try:
+        for k, (scene, f) in enumerate(frames):
+            for w in solo + inb:
+                if k:
+                    w.update_vertices(0, scene.pos, scene.nrm)
+                w.set_frame(f)
+            for w in solo:
+                w.encode()
+                w.sync()
+            b.encode()
+            b.sync()
+            for s, w, (_, layer) in zip(solo, inb, kinds):
+                assert np.array_equal(w.read_sh(), s.read_sh())
+                assert np.array_equal(w.read_bounce_volume()[0], s.read_bounce_volume()[0])
+                if layer:
+                    got, ref = w.read_dynamic(), s.read_dynamic()
+                    n = int(ref["hdr"][8])
+                    assert np.array_equal(got["hdr"], ref["hdr"]) and np.array_equal(got["data"][:n], ref["data"][:n])
+            if k:
+                # the token changes the lit layer world's update, not the one without a layer
+                assert not np.array_equal(solo[0].read_sh(), solo[1].read_sh())
+    finally:
+        b.destroy()
+        for w in solo + inb:
+            w.destroy()
