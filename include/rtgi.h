@@ -75,6 +75,13 @@ typedef struct rtgi_vox_params {
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
                                   pack2x16float, rc.ts:770-771); all arithmetic stays f32 */
+#define RTGI_RC_NO_HISTORY 256u /* turns the c0 SH history off. By default (the reference's world probes: HIST_ON =
+                                 * !screen, rc.ts:1071, 1112) each update's c0 SH is blended with the SH the probe at
+                                 * the same grid point had in the previous update: sh = prev * 0.5 + new * 0.5
+                                 * (histBlend, mix with HIST_W = 0.5, rc.ts:193-194, 784-814); probes whose grid point
+                                 * the previous update did not hold (the first update, the grid moved with the eye)
+                                 * keep their own value. The blended SH is what the next update reads, stored as the SH
+                                 * is (f16 in the precision form, as the reference's pack2x16float history). */
 
 /* Radiance-cascade parameters; the subset of RcParams (src/core/rc.ts:42-91) that the world-space voxel tracer uses,
  * with probes on a dense camera-centred grid instead of screen-registered sparse probes (relocation 'none'). */
@@ -274,8 +281,10 @@ typedef struct rtgi_trace_stats {
 } rtgi_trace_stats;
 int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out);
 
-/* SH irradiance of the c0 probes: 27 floats per probe (9 coefficients x rgb, scaled by {pi, 2pi/3, pi/4}), replacing
- * the sh buffer written by projectSH (src/core/rc.ts:752-774) before its f16 packing. Synchronises. */
+/* SH irradiance of the c0 probes in grid order x + n (y + n z): 27 floats per probe (9 coefficients x rgb, scaled by
+ * {pi, 2pi/3, pi/4}), blended with the previous update's SH unless RTGI_RC_NO_HISTORY (histBlend, rc.ts:784-814);
+ * replaces reading the sh buffer written by projectSH (src/core/rc.ts:752-774; f16 storage widened to f32).
+ * Synchronises. */
 int rtgi_world_read_sh(rtgi_world* w, float* sh);
 
 /* Frees the world's device memory (replaces VoxWorld.destroy src/core/voxel.ts:923 and Rc.destroy rc.ts:1234). */
@@ -330,7 +339,9 @@ int rtgi_batch_destroy(rtgi_batch* b);
  * in flight) for many worlds at once. Worlds are registered with their scene (copied to host memory) and become
  * resident on the device when a request needs them and their bytes (rtgi_world_estimate_bytes) fit in the memory
  * budget; idle resident worlds are evicted least recently used first to make room (an evicted world is rebuilt from
- * its host copy on its next request: every update is a full rebuild, so results do not depend on residency).
+ * its host copy on its next request; every update is a full rebuild, and the state one update hands the next, the c0
+ * SH history and the bounce volume, is copied to host memory at eviction and back at admission, so results do not
+ * depend on residency). A request's frame is checked at submission as rtgi_world_set_frame checks it.
  * Requests carry a frame (viewpoint and lights). A scheduler thread gathers the queued requests whose worlds are idle
  * and resident or admissible, at most one per world and max_batch per launch, into one batched update (one launch per
  * stage) and starts it on one of `slots` streams as soon as a slot is free, so requests that arrive while others run

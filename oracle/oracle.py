@@ -43,6 +43,12 @@ class OrcRc(C.Structure):
                 ("objects", C.c_int32)]
 
 
+class OrcHist(C.Structure):
+    _fields_ = [("cap", C.c_int32), ("H", C.c_uint32), ("limit", C.c_int32), ("par", C.c_int32), ("frames", C.c_uint32),
+                ("misses", C.c_uint64), ("hash", C.c_void_p), ("grid", C.c_void_p), ("sh", C.c_void_p),
+                ("G0", (C.c_int32 * 3) * 2), ("n", C.c_int32)]
+
+
 class OrcDyn(C.Structure):
     _fields_ = [("cap", C.c_int32), ("first_tri", C.c_uint32), ("end_tri", C.c_uint32), ("hdr", C.c_void_p),
                 ("ptr", C.c_void_p), ("bits", C.c_void_p), ("meta", C.c_void_p), ("mask", C.c_void_p),
@@ -75,6 +81,8 @@ _SIG = {
     "orc_trace_rays_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                      _P, C.c_double, C.c_int]),
     "orc_merge_sh": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
+    "orc_merge_sh_hist": (C.c_int, [C.POINTER(OrcRc), _P, _P, C.POINTER(OrcHist)]),
+    "orc_hist_init": (C.c_int, [C.POINTER(OrcHist), C.c_int32]),
     "orc_trace_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
                                 C.c_double, C.c_int]),
     "orc_merge_sh_f64": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
@@ -139,6 +147,32 @@ class DynState:
         for k in ("hdr", "ptr", "bits", "meta", "mask", "data"):
             setattr(d, k, getattr(self, k).ctypes.data)
         return d
+
+
+class History:
+    """The c0 SH history of one world's world probes (histBlend / clearHist, rc.ts:775-818), kept across updates as
+    the device keeps it: each merge_sh(..., hist=) of an update blends the c0 SH with the previous update's SH of the
+    probe at the same grid point (mix(prev, new, 0.5)) and records this update's SH.
+
+    limit: slots a hash search visits. The reference's 64 (limit=64) misses a few probes of dense grids of 64^3 and
+    more (a long linear-probing cluster; which ones depends on the GPU's insertion order); limit=0 searches the whole
+    table, so every probe whose grid point the previous update held finds its history. The device path keys the
+    history by grid point directly, which is the limit=0 rule."""
+
+    def __init__(self, probes0, limit=0):
+        cap = int(probes0) ** 3
+        self.s = OrcHist()
+        assert lib().orc_hist_init(C.byref(self.s), cap) == 0
+        self.s.limit = int(limit)
+        self.hash = np.zeros(2 * self.s.H, np.uint32)
+        self.grid = np.zeros((2, cap, 4), np.int32)
+        self.sh = np.zeros((2, cap, 32), np.float32)
+        self.s.hash, self.s.grid, self.s.sh = self.hash.ctypes.data, self.grid.ctypes.data, self.sh.ctypes.data
+
+    @property
+    def misses(self):
+        """Probes whose grid point the previous update held but the reference's 64-slot hash search missed."""
+        return int(self.s.misses)
 
 
 BN_DIMS = (32, 16, 32)   # bounce volume probes per axis (x, y, z)
@@ -357,11 +391,16 @@ class Oracle:
         lib().orc_trace(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgba), _p(hits), _p(st), threads)
         return (rgba, hits, st) if steps else (rgba, hits)
 
-    def merge_sh(self, rgba):
+    def merge_sh(self, rgba, hist=None):
+        """mergeWorld and projectSH; with hist (a History), the c0 SH history blend of this update, which advances
+        hist. Without it, the SH of a first update (no history yet)."""
         rgba = np.ascontiguousarray(rgba, np.float32).copy()
         n = self.rc.probes0 ** 3
         sh = np.zeros((n, 27), np.float32)
-        lib().orc_merge_sh(C.byref(self.rc), _p(rgba), _p(sh))
+        if hist is None:
+            lib().orc_merge_sh(C.byref(self.rc), _p(rgba), _p(sh))
+        else:
+            assert lib().orc_merge_sh_hist(C.byref(self.rc), _p(rgba), _p(sh), C.byref(hist.s)) == 0
         return rgba, sh
 
     def trace_f64(self, r0=0, r1=None, eps=1e-6, threads=0):

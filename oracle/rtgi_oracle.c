@@ -1500,8 +1500,90 @@ int orc_trace_rays(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int 
   return 0;
 }
 
-/* upperAvg / mergeWorld (rc.ts:681-708) with the dense grid's parents, then projectSH (rc.ts:745-774). */
-int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
+/* hash32 (rc.ts:248-250). */
+static uint32_t hash32(uint32_t x) {
+  uint32_t h = x;
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+/* histHash (rc.ts:781-783): the key of a grid point and LOD (u32 wrap of negative coordinates). */
+static uint32_t histHash(int gx, int gy, int gz, int lod) {
+  return hash32(((uint32_t)gx * 73856093u) ^ ((uint32_t)gy * 19349663u) ^ ((uint32_t)gz * 83492791u) ^
+                ((uint32_t)lod * 2654435761u));
+}
+
+int orc_hist_init(orc_hist* h, int32_t cap) {
+  uint32_t H = 1;
+  while (H < (uint32_t)cap * 2u) H <<= 1; /* hashSize = nextPow2(cap * 2) (rc.ts:1029) */
+  h->cap = cap;
+  h->H = H;
+  h->par = 0;
+  h->frames = 0;
+  h->misses = 0;
+  h->limit = 64;
+  return 0;
+}
+
+/* histBlend (rc.ts:784-814) of probe pi (grid point g, LOD 0) into v (27 coefficients; the reference's v[27..31] are
+ * 0 and stay 0): look the key up in the previous parity's hash (linear probing, at most h->limit slots (the
+ * reference's 64; 0: no limit), stop at an empty slot); when found, v = mix(prev, v, HIST_W) = prev * (1 - 0.5) + v * 0.5 per coefficient (WGSL mix); then record
+ * the probe's grid point and insert it into the current parity's hash (compare-exchange on empty slots, at most
+ * h->limit).
+ * The stored SH (sh_out, written by the caller after packing) is what the next frame reads. Returns 1 when a probe
+ * of the previous frame held the key but the 64-slot search missed it (counted, to show the dense lookup's
+ * equivalence). */
+static int histBlend(orc_hist* h, uint64_t pi, const int32_t g[3], float v[27]) {
+  const uint32_t H = h->H, cur = (uint32_t)h->par, prv = 1u - cur;
+  const uint32_t h0 = histHash(g[0], g[1], g[2], 0) & (H - 1u);
+  const uint32_t lim = h->limit > 0 ? (uint32_t)h->limit : H;
+  uint32_t hh = h0;
+  int64_t found = -1;
+  for (uint32_t i = 0; i < lim; i++) {
+    uint32_t sl = h->hash[prv * H + hh];
+    if (sl == 0u) break;
+    uint64_t q = sl - 1u;
+    const int32_t* o = h->grid + ((uint64_t)prv * h->cap + q) * 4;
+    if (o[0] == g[0] && o[1] == g[1] && o[2] == g[2] && o[3] == 0) {
+      found = (int64_t)q;
+      break;
+    }
+    hh = (hh + 1u) & (H - 1u);
+  }
+  int missed = 0;
+  if (found >= 0) {
+    const float* f = h->sh + ((uint64_t)prv * h->cap + (uint64_t)found) * 32;
+    for (int k = 0; k < 27; k++) v[k] = f[k] * (1.0f - 0.5f) + v[k] * 0.5f;
+  } else if (h->frames > 0) {
+    /* did the previous frame's dense grid hold this point? */
+    int in = 1;
+    for (int a = 0; a < 3; a++) in = in && g[a] >= h->G0[prv][a] && g[a] < h->G0[prv][a] + h->n;
+    missed = in;
+  }
+  int32_t* og = h->grid + ((uint64_t)cur * h->cap + pi) * 4;
+  og[0] = g[0];
+  og[1] = g[1];
+  og[2] = g[2];
+  og[3] = 0;
+  hh = h0;
+  for (uint32_t i = 0; i < lim; i++) {
+    if (h->hash[cur * H + hh] == 0u) {
+      h->hash[cur * H + hh] = (uint32_t)pi + 1u;
+      break;
+    }
+    hh = (hh + 1u) & (H - 1u);
+  }
+  return missed;
+}
+
+int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) { return orc_merge_sh_hist(rc, rgba, sh, NULL); }
+
+/* upperAvg / mergeWorld (rc.ts:681-708) with the dense grid's parents, then projectSH (rc.ts:745-774) with the c0
+ * history (clearHist + histBlend, rc.ts:775-818, 1213) when hist is given. */
+int orc_merge_sh_hist(const orc_rc* rc, float* rgba, float* sh, orc_hist* hist) {
   int Nc = rc->cascades;
   for (int c = Nc - 1; c >= 0; c--) {
     int res = resOf(rc, c), n = probesOf(rc, c);
@@ -1553,6 +1635,12 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
   if (sh) {
     int n = probesOf(rc, 0);
     uint64_t np = (uint64_t)n * n * n;
+    if (hist) {
+      if ((uint64_t)hist->cap != np) return -1;
+      /* encode: the parity flips (rc.ts:1170); clearHist empties the new parity's hash (rc.ts:815-818, 1213) */
+      hist->par = 1 - hist->par;
+      memset(hist->hash + (size_t)hist->par * hist->H, 0, (size_t)hist->H * sizeof(uint32_t));
+    }
     const float a0 = (float)3.14159265, a1 = (float)(2.0 * 3.14159265 / 3.0), a2 = (float)(3.14159265 / 4.0);
     for (uint64_t pi = 0; pi < np; pi++) {
       v3 cf[9];
@@ -1566,12 +1654,26 @@ int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh) {
                       0.546274f * (nn.x * nn.x - nn.y * nn.y)};
         for (int k = 0; k < 9; k++) cf[k] = vadd(cf[k], vscale(r, y[k]));
       }
+      float v[27];
       for (int k = 0; k < 9; k++) {
         float a = k == 0 ? a0 : (k < 4 ? a1 : a2);
-        sh[pi * 27 + k * 3] = q16(rc, cf[k].x * a); /* pack2x16float (rc.ts:770-771) */
-        sh[pi * 27 + k * 3 + 1] = q16(rc, cf[k].y * a);
-        sh[pi * 27 + k * 3 + 2] = q16(rc, cf[k].z * a);
+        v[k * 3] = cf[k].x * a;
+        v[k * 3 + 1] = cf[k].y * a;
+        v[k * 3 + 2] = cf[k].z * a;
       }
+      if (hist) {
+        int32_t g[3] = {rc->G0[0] + (int32_t)(pi % n), rc->G0[1] + (int32_t)((pi / n) % n),
+                        rc->G0[2] + (int32_t)(pi / ((uint64_t)n * n))};
+        hist->misses += (uint64_t)histBlend(hist, pi, g, v);
+      }
+      for (int k = 0; k < 27; k++) sh[pi * 27 + k] = q16(rc, v[k]); /* pack2x16float (rc.ts:770-771) */
+      /* the history holds the packed words of this frame's SH (rc.ts:772): their unpacked values */
+      if (hist) memcpy(hist->sh + ((uint64_t)hist->par * hist->cap + pi) * 32, sh + pi * 27, 27 * sizeof(float));
+    }
+    if (hist) {
+      for (int a = 0; a < 3; a++) hist->G0[hist->par][a] = rc->G0[a];
+      hist->n = n;
+      hist->frames++;
     }
   }
   return 0;

@@ -130,6 +130,32 @@ int orc_trace_rays(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int 
 /* mergeWorld N-1..0 in place on the full rgba array, then projectSH -> sh (27 floats per c0 probe). */
 int orc_merge_sh(const orc_rc* rc, float* rgba, float* sh);
 
+/* The c0 SH history of world probes (phase 29, rc.ts:775-818, on whenever probes are not screen probes: HIST_ON =
+ * !screen, rc.ts:1071): two parities of a linear-probing hash (H = nextPow2(2 cap) slots of probe + 1), each probe's
+ * grid point and LOD (4 words) and its stored SH. The caller allocates hash [2][H], grid [2][cap][4] and sh
+ * [2][cap][32] zeroed (a fresh history: no previous frame). sh holds the unpacked values of the stored SH (the
+ * reference stores the pack2x16float words; their unpacked values are the same information), so the FP32 form keeps
+ * f32 history. limit: the slots a search or insertion visits (the reference's 64; 0: the whole table, which finds
+ * every key the previous frame stored). G0, n and frames record the previous frames' dense grids, only to count
+ * misses: probes whose grid point the previous frame held but the limited search did not find. */
+typedef struct orc_hist {
+  int32_t cap;
+  uint32_t H;
+  int32_t limit;
+  int32_t par; /* the parity written by the last frame (CI.y) */
+  uint32_t frames;
+  uint64_t misses;
+  uint32_t* hash;
+  int32_t* grid;
+  float* sh;
+  int32_t G0[2][3];
+  int32_t n;
+} orc_hist;
+/* Sets cap, H and limit = 64 (and zeroes the counters); the arrays are the caller's. */
+int orc_hist_init(orc_hist* h, int32_t cap);
+/* orc_merge_sh whose SH projection blends with the history and advances it (hist NULL: orc_merge_sh). */
+int orc_merge_sh_hist(const orc_rc* rc, float* rgba, float* sh, orc_hist* hist);
+
 /* gatherWorld (rc.ts:871-927) at n surface points (pos, unit nrm: 3 floats each) from sh ([P0][27], probe-major as
  * orc_merge_sh writes it) -> irr (3 floats each); occlusion: segOccluded tests (the vrel gather). */
 int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const float* sh, const float* pos,

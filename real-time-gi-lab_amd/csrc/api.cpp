@@ -64,10 +64,6 @@ void oct_encode_d(double x, double y, double z, double& u, double& v) {
 
 // Radiance and SH buffer sizes in floats: f32 rgba / 27 f32 per probe, or halves in the precision form.
 size_t rad_floats(const rtgi_world* w) { return (size_t)w->ray_off[w->N] * ((w->rp.flags & RTGI_RC_FP16) ? 2 : 4); }
-size_t sh_floats(const rtgi_world* w) {
-  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
-  return (w->rp.flags & RTGI_RC_FP16) ? (np0 * 27 + 1) / 2 : np0 * 27;
-}
 float half_to_float(uint16_t h) {
   uint32_t s = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u, bits;
   if (e == 0) {
@@ -97,6 +93,14 @@ bool is_pow2(double v) {
 }
 
 }  // namespace
+
+size_t sh_floats_of(const rtgi_world* w) {
+  size_t np0 = (size_t)w->n[0] * w->n[0] * w->n[0];
+  return (w->rp.flags & RTGI_RC_FP16) ? (np0 * 27 + 1) / 2 : np0 * 27;
+}
+int sh_buffers(const rtgi_rc_params* rp) {
+  return (rp->flags & (RTGI_RC_NO_HISTORY | RTGI_RC_NO_MERGE)) ? 1 : 2;
+}
 
 VoxK vox_args(const rtgi_world* w) {
   VoxK k{};
@@ -213,7 +217,10 @@ TraceK trace_args(const rtgi_world* w) {
   t.rad = w->rad;
   t.avg = w->avg;
   t.hits = w->hits;
-  t.sh = w->sh;
+  t.sh = w->shb[w->sh_par];
+  t.sh_prev = w->hist_prev ? w->shb[1 - w->sh_par] : nullptr;
+  for (int a = 0; a < 3; a++) t.hist_d[a] = w->G0[a] - w->hist_G0[a];
+  t.sh_fuse = 0;
   t.total_rays = w->ray_off[w->N];
   return t;
 }
@@ -224,7 +231,7 @@ namespace {
 
 int probe_tile_log2(const rtgi_world* w, int c) {
   int T = 0;
-  while ((1 << (T + 1)) <= w->tuning.probe_tile) T++;
+  while ((1 << (T + 1)) <= w->data_tile) T++;
   int n = w->n[c], nl = 0;
   while ((1 << nl) < n) nl++;
   if ((1 << nl) != n || nl < T) return 0;  // rows when n is not a power of two or is smaller than the block
@@ -274,7 +281,7 @@ void free_all(rtgi_world* w) {
   dfree(w, w->mat, (size_t)w->cap_mat * 4);
   dfree(w, w->rad, rad_floats(w));
   dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
-  dfree(w, w->sh, sh_floats(w));
+  for (int i = 0; i < 2; i++) dfree(w, w->shb[i], i < sh_buffers(&w->rp) ? sh_floats_of(w) : 0);
   dfree(w, w->dirs, (size_t)w->dir_off[w->N] + BN_RAYS);
   dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
   dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
@@ -290,6 +297,18 @@ void free_all(rtgi_world* w) {
 }
 
 }  // namespace
+
+void sh_advance(rtgi_world* w) {
+  if (w->rp.flags & RTGI_RC_NO_MERGE) return;
+  const bool hist = sh_buffers(&w->rp) == 2;
+  w->hist_prev = hist && w->sh_written;
+  if (w->hist_prev) {
+    w->sh_par = 1 - w->sh_par;
+    for (int a = 0; a < 3; a++) w->hist_G0[a] = w->sh_G0[a];
+  }
+  for (int a = 0; a < 3; a++) w->sh_G0[a] = w->G0[a];
+  w->sh_written = true;
+}
 
 void bounce_advance(rtgi_world* w) {
   if (!w->bounce) return;
@@ -315,14 +334,19 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
     if (!e && w->dyn_hdr) e = launch_dyn_build(vox_args(w), s);
     if (!e && w->dyn_hdr) e = launch_dyn_splat(trace_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
+    // Fused launches also project the c0 SH in the trace stage (the SH stage is then empty).
+    trace_begin(w);
     bounce_advance(w);
+    sh_advance(w);
     TraceK t = trace_args(w);
+    t.sh_fuse = 1;
     e = launch_sun(t, s);
     if (!e) e = launch_dyn_inject(t, s);
     if (!e) e = launch_bounce(t, s);
     if (!e) e = launch_relocate(t, s);
     for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_trace_merge(t, c, s);
   } else if (stage == RTGI_STAGE_TRACE) {
+    trace_begin(w);
     bounce_advance(w);
     TraceK t = trace_args(w);
     e = launch_sun(t, s);
@@ -336,7 +360,10 @@ int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
       for (int c = w->N - 1; c >= 0 && e == 0; c--) e = launch_merge(t, c, s);
     }
   } else if (stage == RTGI_STAGE_SH) {
-    if (!(w->rp.flags & RTGI_RC_NO_MERGE)) e = launch_sh(trace_args(w), s);
+    if (!(w->rp.flags & RTGI_RC_NO_MERGE) && w->tuning.fused != 1) {
+      sh_advance(w);
+      e = launch_sh(trace_args(w), s);
+    }
   }
   if (e != 0) return rtgi_fail(RTGI_E_HIP, "kernel launch failed: %s", hipGetErrorString((hipError_t)e));
   return RTGI_OK;
@@ -441,7 +468,11 @@ int world_layout(rtgi_world* w, const rtgi_vox_params* vp, const rtgi_rc_params*
     double L = std::nearbyint(std::log2(v / vp->v0));
     w->vlev[c] = (int)std::min(std::max(L, 0.0), (double)(vp->levels - 1));
     w->ray_off[c] = off;
-    off += (uint64_t)w->n[c] * w->n[c] * w->n[c] * w->res[c] * w->res[c];
+    const uint64_t rays_c = (uint64_t)w->n[c] * w->n[c] * w->n[c] * w->res[c] * w->res[c];
+    if (rays_c >= (1ull << 32))  // a cascade's rays are indexed in 32 bits
+      return rtgi_fail(RTGI_E_INVALID, "cascade %d has %llu rays (n^3 res^2 must stay below 2^32)", c,
+                       (unsigned long long)rays_c);
+    off += rays_c;
     w->dir_off[c + 1] = w->dir_off[c] + (uint32_t)(w->res[c] * w->res[c]);
     uint64_t npc = (uint64_t)w->n[c] * w->n[c] * w->n[c];
     w->avg_off[c + 1] = w->avg_off[c] + (c > 0 ? npc * w->res[c - 1] * w->res[c - 1] : 0);
@@ -465,7 +496,7 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
   size_t NLper = (size_t)w.NL * w.per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
   uint64_t b = nbytes<uint32_t>(HDR_WORDS) + nbytes<uint32_t>(NLper) + nbytes<uint32_t>(mb * 16) +
                nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 16) + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
-               nbytes<float>(rad_floats(&w)) + nbytes<float>(sh_floats(&w)) +
+               nbytes<float>(rad_floats(&w)) + (uint64_t)sh_buffers(rp) * nbytes<float>(sh_floats_of(&w)) +
                nbytes<DirEntry>((size_t)w.dir_off[w.N] + BN_RAYS) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
   if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
   if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
@@ -522,7 +553,7 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   if (!e) e = dalloc(w, &w->plane, mv);
   if (!e) e = dalloc(w, &w->rad, rad_floats(w));
   if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
-  if (!e) e = dalloc(w, &w->sh, sh_floats(w));
+  for (int i = 0; i < sh_buffers(rp) && !e; i++) e = dalloc(w, &w->shb[i], sh_floats_of(w));
   if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc] + BN_RAYS);
   if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
   if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
@@ -633,30 +664,40 @@ extern "C" int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_
   return RTGI_OK;
 }
 
-extern "C" int rtgi_world_set_frame(rtgi_world* w, const rtgi_frame* f) {
-  if (!w || !f) return rtgi_fail(RTGI_E_INVALID, "null argument");
+int frame_origins(const rtgi_vox_params* vp, const rtgi_rc_params* rp, const rtgi_frame* f,
+                  int origin[RTGI_MAX_LEVELS][3], int G0[3]) {
   if (f->n_point < 0 || f->n_point > RTGI_MAX_POINT_LIGHTS) return rtgi_fail(RTGI_E_INVALID, "0..8 point lights");
   for (int a = 0; a < 3; a++)
     if (!std::isfinite(f->eye[a])) return rtgi_fail(RTGI_E_INVALID, "eye must be finite");
-  w->frame = *f;
   // Clipmap windows centred on the eye: floor(eye / bs) - G/2 (writeUniform, src/core/voxel.ts:872-875).
-  for (int L = 0; L < w->NL; L++) {
-    double bs = w->vp.v0 * std::exp2((double)L) * 8;
+  for (int L = 0; L < vp->levels; L++) {
+    double bs = vp->v0 * std::exp2((double)L) * 8;
     for (int a = 0; a < 3; a++) {
-      double o = std::floor(f->eye[a] / bs) - w->G / 2;
+      double o = std::floor(f->eye[a] / bs) - vp->G / 2;
       if (std::fabs(o) > 1e6) return rtgi_fail(RTGI_E_INVALID, "eye too far from the origin");
-      w->origin[L][a] = (int)o;
+      origin[L][a] = (int)o;
     }
   }
   // Dense probe grid, snapped to the coarsest cascade's spacing so that every cascade's grid nests in the next.
-  int top = w->N - 1;
-  double st = w->rp.s0 * std::exp2((double)top);
-  int ntop = w->n[top];
+  int top = rp->cascades - 1;
+  double st = rp->s0 * std::exp2((double)top);
+  int ntop = rp->probes0 >> top;
   for (int a = 0; a < 3; a++) {
     double g = std::floor(f->eye[a] / st) - ntop / 2;
     if (std::fabs(g) > 1e6) return rtgi_fail(RTGI_E_INVALID, "eye too far from the origin");
-    w->G0[a] = (int)g * (1 << top);
+    G0[a] = (int)g * (1 << top);
   }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_set_frame(rtgi_world* w, const rtgi_frame* f) {
+  if (!w || !f) return rtgi_fail(RTGI_E_INVALID, "null argument");
+  int origin[RTGI_MAX_LEVELS][3] = {}, G0[3];
+  int e = frame_origins(&w->vp, &w->rp, f, origin, G0);
+  if (e) return e;  // the world keeps its previous frame
+  w->frame = *f;
+  std::memcpy(w->origin, origin, sizeof(origin));
+  for (int a = 0; a < 3; a++) w->G0[a] = G0[a];
   w->frame_set = true;
   return RTGI_OK;
 }
@@ -1039,20 +1080,17 @@ extern "C" int rtgi_world_read_sh(rtgi_world* w, float* sh) {
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(world_wait(w));
   size_t np = (size_t)w->n[0] * w->n[0] * w->n[0];
-  std::vector<uint32_t> dev(np);
-  for (uint32_t z = 0, p = 0; z < (uint32_t)w->n[0]; z++)
-    for (uint32_t y = 0; y < (uint32_t)w->n[0]; y++)
-      for (uint32_t x = 0; x < (uint32_t)w->n[0]; x++, p++) dev[p] = probe_index_host(w, 0, x, y, z);
+  const float* src = w->shb[w->sh_par];  // planes [27][np] in grid-row order
   if (w->rp.flags & RTGI_RC_FP16) {
     std::vector<uint16_t> planes(np * 27);
-    HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 2, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(planes.data(), src, np * 27 * 2, hipMemcpyDeviceToHost));
     for (size_t p = 0; p < np; p++)
-      for (int q = 0; q < 27; q++) sh[p * 27 + q] = half_to_float(planes[(size_t)q * np + dev[p]]);
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = half_to_float(planes[(size_t)q * np + p]);
   } else {
     std::vector<float> planes(np * 27);
-    HIPCHK(hipMemcpy(planes.data(), w->sh, np * 27 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(planes.data(), src, np * 27 * 4, hipMemcpyDeviceToHost));
     for (size_t p = 0; p < np; p++)
-      for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + dev[p]];
+      for (int q = 0; q < 27; q++) sh[p * 27 + q] = planes[(size_t)q * np + p];
   }
   return RTGI_OK;
 }
@@ -1086,6 +1124,49 @@ extern "C" int rtgi_world_destroy(rtgi_world* w) {
   return RTGI_OK;
 }
 
+int world_save_state(rtgi_world* w, WorldState* st) {
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  *st = WorldState{};
+  if (w->sh_written) {
+    st->sh.resize(sh_floats_of(w) * sizeof(float));
+    HIPCHK(hipMemcpy(st->sh.data(), w->shb[w->sh_par], st->sh.size(), hipMemcpyDeviceToHost));
+    for (int a = 0; a < 3; a++) st->sh_G0[a] = w->sh_G0[a];
+  }
+  if (w->bounce) {
+    for (int i = 0; i < 2; i++) {
+      st->bnvol[i].resize(BN_WORDS);
+      HIPCHK(hipMemcpy(st->bnvol[i].data(), w->bnvol[i], (size_t)BN_WORDS * 4, hipMemcpyDeviceToHost));
+    }
+    st->bn_par = w->bn_par;
+    std::memcpy(st->bn_org, w->bn_org, sizeof(st->bn_org));
+    st->bn_frame = w->bn_frame;
+  }
+  st->valid = true;
+  return RTGI_OK;
+}
+
+int world_restore_state(rtgi_world* w, const WorldState& st) {
+  if (!st.valid) return RTGI_OK;
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  if (!st.sh.empty()) {
+    if (st.sh.size() != sh_floats_of(w) * sizeof(float)) return rtgi_fail(RTGI_E_STATE, "saved SH of another shape");
+    w->sh_par = 0;
+    HIPCHK(hipMemcpy(w->shb[0], st.sh.data(), st.sh.size(), hipMemcpyHostToDevice));
+    w->sh_written = true;
+    for (int a = 0; a < 3; a++) w->sh_G0[a] = st.sh_G0[a];
+  }
+  if (w->bounce && !st.bnvol[0].empty()) {
+    for (int i = 0; i < 2; i++)
+      HIPCHK(hipMemcpy(w->bnvol[i], st.bnvol[i].data(), (size_t)BN_WORDS * 4, hipMemcpyHostToDevice));
+    w->bn_par = st.bn_par;
+    std::memcpy(w->bn_org, st.bn_org, sizeof(w->bn_org));
+    w->bn_frame = st.bn_frame;
+  }
+  return RTGI_OK;
+}
+
 // ---- Batches ----------------------------------------------------------------------------------------------------
 
 struct rtgi_batch {
@@ -1115,15 +1196,17 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   p.stats = false;
   p.dyn = false;
   p.fused = n > 0;
+  for (int i = 0; i < n; i++) p.fused = p.fused && worlds[i]->tuning.fused != 0;
   for (int i = 0; i < n; i++) {
-    p.fused = p.fused && worlds[i]->tuning.fused != 0;
     TraceK t = trace_args(worlds[i]);
+    t.sh_fuse = p.fused ? 1 : 0;
     uint32_t g[BL_COUNT];
     batch_blocks(t, g);
     if (worlds[i]->rp.flags & RTGI_RC_NO_MERGE) {
       for (int st = 0; st < RTGI_MAX_CASCADES; st++) g[BL_MERGE + st] = 0;
       g[BL_SH] = 0;
     }
+    if (p.fused) g[BL_SH] = 0;  // the c0 trace+merge launch projects the SH
     for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1) + i + 1] = start[(size_t)l * (n + 1) + i] + g[l];
     size_t m = vox_mark_lds_bytes(t.vx);
     if (m == 0) all_lds = false;
@@ -1146,6 +1229,14 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   // One dynamic LDS size for the whole launch: the largest bitmap when every world's fits (each world's descriptor
   // says whether it marks in LDS).
   p.mark_lds = all_lds ? lds : 0;
+}
+
+void batch_world_begin(rtgi_world* w, const BatchPlan& p, TraceK* out) {
+  trace_begin(w);
+  bounce_advance(w);
+  sh_advance(w);
+  *out = trace_args(w);
+  out->sh_fuse = p.fused ? 1 : 0;
 }
 
 int plan_launch_stage(const BatchPlan& p, hipStream_t s, int stage) {
@@ -1296,10 +1387,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   }
   // Descriptors: the staging buffer being filled was last copied two enqueues ago; wait for that copy.
   HIPCHK(hipEventSynchronize(b->copied[b->cur]));
-  for (int i = 0; i < n; i++) {
-    bounce_advance(b->worlds[i]);
-    b->h_desc[b->cur][i] = trace_args(b->worlds[i]);
-  }
+  for (int i = 0; i < n; i++) batch_world_begin(b->worlds[i], b->plan, &b->h_desc[b->cur][i]);
   HIPCHK(hipMemcpyAsync(b->d_desc, b->h_desc[b->cur], sizeof(TraceK) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(b->copied[b->cur], s));
   b->cur ^= 1;

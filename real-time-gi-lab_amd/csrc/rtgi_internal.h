@@ -165,7 +165,10 @@ struct TraceK {
   float* rad;       // 4 floats (or halves) per ray; within a cascade direction-major: ray = di * np + probe
   float* avg;       // merged cascade c >= 1 averaged over 2x2 directions at cascade c-1's resolution (upperAvg)
   uint32_t* hits;   // per ray (same order), or null
-  float* sh;        // [27][probes0^3]
+  float* sh;        // [27][probes0^3] in grid-row order (f32, or halves in the precision form)
+  const float* sh_prev;  // the previous update's SH (same layout) the c0 SH blends with, or null (no history)
+  int32_t hist_d[3];     // c0 grid origin of this update minus that of sh_prev: a probe's row there is at local + d
+  int32_t sh_fuse;       // the c0 trace+merge launch also projects the SH (16 probes x 16 directions per block)
   uint64_t total_rays;
 };
 
@@ -195,7 +198,7 @@ int launch_sun(const TraceK& k, void* stream);
 int launch_bounce(const TraceK& k, void* stream);
 // Voxel relocation of every cascade's probes, then the parent visibility masks (thread per probe).
 int launch_relocate(const TraceK& k, void* stream);
-// Threads of cascade c's merge launch, and of its fused trace+merge launch.
+// Threads of cascade c's merge launch, and of its fused trace+merge launch (c = 0 with sh_fuse: 256 per 16 probes).
 uint64_t merge_threads(const TraceK& k, int c);
 uint64_t trace_merge_threads(const TraceK& k, int c);
 // Trace and merge of cascade c in one launch (cascades N-1 .. 0 in order; c + 1's averages must be complete).

@@ -41,6 +41,7 @@ struct SrvWorld {
   int queued = 0;         // its requests in the queue
   uint64_t skip_pass = 0; // gather pass in which one of its requests was passed over
   int blocked = 0;        // consecutive passes in which it could not be admitted
+  WorldState saved;       // temporal state (SH history, bounce volume) kept over an eviction
 };
 
 struct Req {
@@ -96,13 +97,15 @@ void slot_free(Slot& s) {
   s = Slot{};
 }
 
-// Creates the device world of sw from its host copy (called without the lock).
+// Creates the device world of sw from its host copy and restores the temporal state it had when it was evicted
+// (called without the lock).
 int make_resident(SrvWorld& sw) {
   rtgi_world* w = nullptr;
   int e = rtgi_world_create(&sw.vp, &sw.rp, &w);
   if (!e)
     e = rtgi_world_upload_scene(w, sw.pos.data(), sw.nrm.data(), sw.nverts, sw.tri.data(), sw.ntris, sw.mat.data(),
                                 sw.nmat);
+  if (!e) e = world_restore_state(w, sw.saved);
   if (e) {
     if (w) rtgi_world_destroy(w);
     return e;
@@ -112,9 +115,10 @@ int make_resident(SrvWorld& sw) {
 }
 
 // Makes room for `need` bytes by evicting idle resident worlds: first those with no queued request, each group least
-// recently used first. Returns whether it fits. Caller holds the lock; the evicted worlds are handed back to be
-// freed without it.
-bool make_room(rtgi_server* s, uint64_t need, std::vector<rtgi_world*>& victims) {
+// recently used first. Returns whether it fits. Caller holds the lock; the evicted worlds are handed back to have
+// their temporal state saved and be freed without it.
+using Victim = std::pair<SrvWorld*, rtgi_world*>;
+bool make_room(rtgi_server* s, uint64_t need, std::vector<Victim>& victims) {
   uint64_t idle = 0;
   for (auto& x : s->worlds)
     if (x->w && !x->busy) idle += x->bytes;
@@ -128,7 +132,7 @@ bool make_room(rtgi_server* s, uint64_t need, std::vector<rtgi_world*>& victims)
         v = x.get();
     }
     if (!v) return false;
-    victims.push_back(v->w);
+    victims.push_back(Victim(v, v->w));
     v->w = nullptr;
     s->st.resident_bytes -= v->bytes;
     s->st.resident_worlds--;
@@ -166,7 +170,7 @@ void sched_loop(rtgi_server* s) {
         continue;
       }
       if (!w->w) {
-        std::vector<rtgi_world*> victims;
+        std::vector<Victim> victims;
         if (admit_closed || !make_room(s, w->bytes, victims)) {
           w->skip_pass = pass;
           // Memory held by running launches: after a few tries, start nothing behind it until they end, so that
@@ -181,7 +185,11 @@ void sched_loop(rtgi_server* s) {
         s->st.resident_bytes += w->bytes;
         s->st.peak_resident_bytes = std::max(s->st.peak_resident_bytes, s->st.resident_bytes);
         lk.unlock();
-        for (rtgi_world* v : victims) rtgi_world_destroy(v);
+        for (const Victim& v : victims) {
+          // An evicted world keeps its history: a failed save drops it (its next update starts a new history).
+          if (world_save_state(v.second, &v.first->saved)) v.first->saved = WorldState{};
+          rtgi_world_destroy(v.second);
+        }
         int e = make_resident(*w);
         lk.lock();
         w->busy = false;
@@ -219,21 +227,44 @@ void sched_loop(rtgi_server* s) {
     lk.unlock();
 
     // Launch: frames, descriptors and block prefixes through the slot's pinned buffers, then one launch per stage.
+    // A request whose frame its world refuses (submit checks frames, so this does not happen in practice) completes
+    // alone with the error; the others launch.
     int e = RTGI_OK;
+    std::vector<rtgi_world*> ws;
+    std::vector<rtgi_completion> refused;
+    for (size_t i = 0; i < pick.size();) {
+      rtgi_world* w = pw[i]->w;
+      int fe = rtgi_world_set_frame(w, &pick[i].r.frame);
+      if (fe) {
+        refused.push_back(rtgi_completion{pick[i].r.tag, pick[i].r.world, fe, 0, pick[i].t_submit, sl.t_launch,
+                                          s->now()});
+        pick.erase(pick.begin() + (long)i);
+        pw.erase(pw.begin() + (long)i);
+        continue;
+      }
+      if (hipStreamSynchronize(w->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "world stream");
+      ws.push_back(w);
+      i++;
+    }
     const int n = (int)pick.size();
-    std::vector<rtgi_world*> ws(n);
-    for (int i = 0; i < n && !e; i++) {
-      ws[i] = pw[i]->w;
-      e = rtgi_world_set_frame(ws[i], &pick[i].r.frame);
-      if (!e && hipStreamSynchronize(ws[i]->stream) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "world stream");
+    if (!refused.empty()) {
+      lk.lock();
+      for (size_t i = 0; i < refused.size(); i++) s->worlds[refused[i].world]->busy = false;
+      for (const rtgi_completion& c : refused) s->done.push_back(c);
+      s->st.completed += refused.size();
+      sl.reqs = pick;
+      s->cv_poll.notify_all();
+      if (n == 0) {
+        sl.busy = false;
+        s->cv_sched.notify_all();
+        continue;
+      }
+      lk.unlock();
     }
     BatchPlan plan{};
     if (!e) {
       plan_blocks(ws.data(), n, sl.h_start, plan);
-      for (int i = 0; i < n; i++) {
-        bounce_advance(ws[i]);
-        sl.h_desc[i] = trace_args(ws[i]);
-      }
+      for (int i = 0; i < n; i++) batch_world_begin(ws[i], plan, &sl.h_desc[i]);
       if (hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(TraceK) * n, hipMemcpyHostToDevice, sl.stream) != hipSuccess ||
           hipMemcpyAsync(sl.d_start, sl.h_start, sizeof(uint32_t) * BL_COUNT * (n + 1), hipMemcpyHostToDevice,
                          sl.stream) != hipSuccess)
@@ -297,14 +328,6 @@ void reap_loop(rtgi_server* s) {
       lk.lock();
     }
   }
-}
-
-int frame_check(const rtgi_frame& f) {
-  if (f.n_point < 0 || f.n_point > RTGI_MAX_POINT_LIGHTS) return rtgi_fail(RTGI_E_INVALID, "0..8 point lights");
-  for (int a = 0; a < 3; a++)
-    if (!(f.eye[a] == f.eye[a]) || f.eye[a] > 1e300 || f.eye[a] < -1e300)
-      return rtgi_fail(RTGI_E_INVALID, "eye must be finite");
-  return RTGI_OK;
 }
 
 }  // namespace
@@ -398,10 +421,14 @@ extern "C" int rtgi_server_add_world(rtgi_server* s, const rtgi_vox_params* vp, 
 
 extern "C" int rtgi_server_submit(rtgi_server* s, const rtgi_request* r) {
   if (!s || !r) return rtgi_fail(RTGI_E_INVALID, "rtgi_server_submit: null argument");
-  int e = frame_check(r->frame);
-  if (e) return e;
   std::lock_guard<std::mutex> lk(s->mu);
   if (r->world < 0 || r->world >= (int32_t)s->worlds.size()) return rtgi_fail(RTGI_E_INVALID, "unknown world id");
+  {  // the frame check of rtgi_world_set_frame, so that a request's frame cannot fail its batch later
+    int origin[RTGI_MAX_LEVELS][3], G0[3];
+    const SrvWorld& sw = *s->worlds[r->world];
+    int e = frame_origins(&sw.vp, &sw.rp, &r->frame, origin, G0);
+    if (e) return e;
+  }
   if (s->stop) return rtgi_fail(RTGI_E_STATE, "server is stopping");
   Req q;
   q.r = *r;

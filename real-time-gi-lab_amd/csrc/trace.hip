@@ -690,26 +690,47 @@ __device__ __forceinline__ void sh_basis3(float nx, float ny, float nz, float y[
 }
 __device__ __forceinline__ void sh_basis(const DirEntry& de, float y[9]) { sh_basis3(de.d[0], de.d[1], de.d[2], y); }
 
-// 27 planes of f32, or of halves in the precision form (pack2x16float per value, src/core/rc.ts:770-771).
-__device__ __forceinline__ void sh_store(const TraceK& a, uint32_t np, uint32_t pi, const f3 cf[9]) {
-  for (int q = 0; q < 9; q++) {
-    float s = q == 0 ? SH_A0 : (q < 4 ? SH_A1 : SH_A2);
-    float v[3] = {cf[q].x * s, cf[q].y * s, cf[q].z * s};
-    for (int ch = 0; ch < 3; ch++) {
-      size_t i = (size_t)(q * 3 + ch) * np + pi;
-      if (a.fp16)
-        reinterpret_cast<unsigned short*>(a.sh)[i] = __half_as_ushort(__float2half_rn(v[ch]));
-      else
-        a.sh[i] = v[ch];
-    }
-  }
+// Grid-row index x + n (y + n z) of local grid point q of a cascade with n probes per axis.
+__device__ __forceinline__ uint32_t grid_row(const CascK& C, i3 q) {
+  return (uint32_t)q.x + (uint32_t)C.n * ((uint32_t)q.y + (uint32_t)C.n * (uint32_t)q.z);
 }
 
-// projectSH (src/core/rc.ts:752-774) of already merged c0 radiance: 27 f32 per probe in 27 planes.
+// Row of the c0 history that probe gl (local grid point of this update) blends with: the same world grid point in
+// the previous update's grid (histBlend's key, rc.ts:784-800; the dense grid holds every point once, so the lookup is
+// the index itself), or ~0u when that grid did not hold it or there is no history.
+__device__ __forceinline__ uint32_t hist_row(const TraceK& a, i3 gl) {
+  if (!a.sh_prev) return ~0u;
+  i3 q = gl + i3{a.hist_d[0], a.hist_d[1], a.hist_d[2]};
+  return inside3(q, a.c[0].n) ? grid_row(a.c[0], q) : ~0u;
+}
+
+// One SH value (plane k = 3 * coefficient + channel) of probe row: the scaled coefficient v, blended with the history
+// value at hrow (mix(prev, v, HIST_W = 0.5) = prev * (1 - 0.5) + v * 0.5, rc.ts:801-804), stored as f32 or as a half
+// (pack2x16float per value, rc.ts:770-771).
+__device__ __forceinline__ void sh_put(const TraceK& a, uint32_t np, uint32_t row, uint32_t hrow, int k, float v) {
+  const size_t i = (size_t)k * np + row;
+  if (hrow != ~0u) {
+    const size_t j = (size_t)k * np + hrow;
+    float p = a.fp16 ? __half2float(__ushort_as_half(reinterpret_cast<const unsigned short*>(a.sh_prev)[j]))
+                     : a.sh_prev[j];
+    v = p * (1.0f - 0.5f) + v * 0.5f;
+  }
+  if (a.fp16)
+    reinterpret_cast<unsigned short*>(a.sh)[i] = __half_as_ushort(__float2half_rn(v));
+  else
+    a.sh[i] = v;
+}
+__device__ __forceinline__ float sh_scale(int q) { return q == 0 ? SH_A0 : (q < 4 ? SH_A1 : SH_A2); }
+
+// projectSH (src/core/rc.ts:752-774) of already merged c0 radiance, thread per probe in grid-row order: 27 values per
+// probe in 27 planes, blended with the history (histBlend, rc.ts:767).
 __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
   const CascK& C = a.c[0];
-  uint32_t pi = lb * BLK + threadIdx.x;
-  if (pi >= C.np) return;
+  uint32_t row = lb * BLK + threadIdx.x;
+  if (row >= C.np) return;
+  const uint32_t n = (uint32_t)C.n;
+  const i3 gl{(int)(row % n), (int)((row / n) % n), (int)(row / (n * n))};
+  const uint32_t pi = probe_index(C, gl);  // device order of the radiance
   f3 cf[9];
   for (int q = 0; q < 9; q++) cf[q] = splat3(0.0f);
   for (uint32_t d = 0; d < 16; d++) {
@@ -718,7 +739,13 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
     sh_basis(a.dirs[C.dir_off + d], y);
     for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];
   }
-  sh_store(a, C.np, pi, cf);
+  const uint32_t hrow = hist_row(a, gl);
+  for (int q = 0; q < 9; q++) {
+    const float s = sh_scale(q);
+    sh_put(a, C.np, row, hrow, 3 * q, cf[q].x * s);
+    sh_put(a, C.np, row, hrow, 3 * q + 1, cf[q].y * s);
+    sh_put(a, C.np, row, hrow, 3 * q + 2, cf[q].z * s);
+  }
 }
 __global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
@@ -754,14 +781,27 @@ __device__ __forceinline__ float4 rad_round(const TraceK& a, float4 v) {
   return make_float4(hlo(p0), hhi(p0), hlo(p1), hhi(p1));
 }
 
+// With sh_fuse, cascade 0's launch also projects the SH: a block holds 16 probes x all 16 c0 directions (wavefront w
+// = 2x2 block (w & 1, w >> 1) of the 4x4 texels, lane = 16 j + probe as above), stages the merged radiance in LDS and
+// forms each probe's 27 SH values in the direction order of projectSH, so the c0 radiance is not read back.
 template <bool ST, bool DY>
 __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
+  __shared__ float s_rad[16][16][3];  // [probe][direction][rgb]
+  __shared__ float s_y[16][9];        // shBasis of the 16 c0 directions
   const CascK& C = a.c[c];
   const bool quad = !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
+  const bool sh0 = c == 0 && a.sh_fuse && !a.no_merge;
   uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
   uint32_t lane = threadIdx.x & 63u, di = 0, pi = 0, dl = 0, j = 0;
   bool ok;
-  if (quad) {
+  if (sh0) {
+    const uint32_t wv = threadIdx.x >> 6;
+    j = lane >> 4;
+    pi = lb * 16u + (lane & 15u);
+    ok = pi < C.np;
+    di = ((2u * (wv >> 1) + (j >> 1)) << 2) + 2u * (wv & 1u) + (j & 1u);
+    if (threadIdx.x < 16u) sh_basis(a.dirs[C.dir_off + threadIdx.x], s_y[threadIdx.x]);
+  } else if (quad) {
     const CascK& D = a.c[c - 1];
     uint32_t npb = (C.np + 15u) >> 4;
     uint64_t grp = t >> 6;
@@ -820,6 +860,24 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
       if (cap) atomicAdd(&S[2 * RTGI_MAX_CASCADES + c], cap);
     }
   }
+  if (sh0) {
+    // projectSH of the block's 16 probes: value k = 3 q + channel of probe p per thread (432 per block), summed over
+    // the directions in order as sh_body does (the same operations on the same stored values).
+    s_rad[lane & 15u][di][0] = m.x;
+    s_rad[lane & 15u][di][1] = m.y;
+    s_rad[lane & 15u][di][2] = m.z;
+    __syncthreads();
+    for (uint32_t o = threadIdx.x; o < 16u * 27u; o += BLK) {
+      const uint32_t p = o / 27u, k = o % 27u, q = k / 3u, ch = k % 3u;
+      const uint32_t pp = lb * 16u + p;
+      if (pp >= C.np) continue;
+      float cf = 0.0f;
+      for (uint32_t d = 0; d < 16; d++) cf = cf + (s_rad[p][d][ch] * a.omega0[d]) * s_y[d][q];
+      const i3 gl = probe_xyz(C, pp);
+      sh_put(a, C.np, grid_row(C, gl), hist_row(a, gl), (int)k, cf * sh_scale((int)q));
+    }
+    return;
+  }
   if (a.no_merge || c == 0) return;
   float4* AV = reinterpret_cast<float4*>(a.avg) + C.avg_off;
   if (!quad) {  // resolution capped: the average of a one-texel block is the merged value
@@ -852,7 +910,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 
 // ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
 
-// evalSH (src/core/rc.ts:819-829) of c0 probe q (device index): coefficients times shBasis(n) in coefficient order,
+// evalSH (src/core/rc.ts:819-829) of c0 probe row q (grid order): coefficients times shBasis(n) in coefficient order,
 // clamped at 0. The precision form's f16 coefficients are widened, as unpack2x16float does.
 __device__ f3 eval_sh(const TraceK& a, uint32_t q, f3 n) {
   uint32_t np = a.c[0].np;
@@ -899,7 +957,7 @@ __device__ bool seg_occluded(const VoxK& k, f3 pa, f3 pb, int L) {
 
 // The 8 c0 probes around x with their gather weights: trilinear weight + 1e-4 times ((side + 1)^2 / 4 + 0.02),
 // side = cos(n, probe - x) (gatherWorld, rc.ts:886-895). Corner k: o = (k & 1, k >> 1 & 1, k >> 2 & 1); absent
-// (outside the dense grid) -> false.
+// (outside the dense grid) -> false. q: the probe's SH row (grid order).
 __device__ __forceinline__ bool gather_corner(const TraceK& a, const int32_t* G0, f3 x, f3 n, i3 b, f3 f, int k,
                                               f3& pr, float& w, uint32_t& q) {
   const CascK& C = a.c[0];
@@ -908,9 +966,9 @@ __device__ __forceinline__ bool gather_corner(const TraceK& a, const int32_t* G0
   i3 gl = g - i3{G0[0], G0[1], G0[2]};
   if (!inside3(gl, C.n)) return false;
   f3 tw = mk3(o.x ? f.x : 1.0f - f.x, o.y ? f.y : 1.0f - f.y, o.z ? f.z : 1.0f - f.z);
-  q = probe_index(C, gl);
+  q = grid_row(C, gl);
   if (a.reloc) {
-    float4 P = ppos4(a)[C.probe_off + q];
+    float4 P = ppos4(a)[C.probe_off + probe_index(C, gl)];
     pr = mk3(P.x, P.y, P.z);
   } else {
     pr = tof3(g) * a.s0;
@@ -1562,7 +1620,11 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
   return (int)hipGetLastError();
 }
 
-uint64_t trace_merge_threads(const TraceK& k, int c) { return k.no_merge ? k.c[c].rays : merge_threads(k, c); }
+uint64_t trace_merge_threads(const TraceK& k, int c) {
+  if (k.no_merge) return k.c[c].rays;
+  if (c == 0 && k.sh_fuse) return (uint64_t)((k.c[0].np + 15u) >> 4) * BLK;  // 16 probes x 16 directions per block
+  return merge_threads(k, c);
+}
 
 int launch_trace_merge(const TraceK& k, int c, void* stream) {
   const bool dy = k.vx.dyn.hdr != nullptr;

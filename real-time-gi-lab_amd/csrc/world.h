@@ -43,7 +43,15 @@ struct rtgi_world {
   uint32_t* tri = nullptr;
   float* rad = nullptr;
   uint32_t* hits = nullptr;
-  float* sh = nullptr;
+  // c0 SH in grid-row order, [27][P0] f32 or halves. Two buffers with the history on (the last update's SH is the
+  // next one's history), one with RTGI_RC_NO_HISTORY.
+  float* shb[2] = {};
+  int sh_par = 0;            // the buffer of the last SH written
+  bool sh_written = false;   // an SH has been written (a history exists)
+  bool hist_prev = false;    // the SH being written blends with shb[1 - sh_par]
+  int sh_G0[3] = {};         // c0 grid origin of the SH in shb[sh_par]
+  int hist_G0[3] = {};       // c0 grid origin of the history it blends with
+  int32_t data_tile = 1;     // probe tile the radiance, averages and probe arrays were last written in
   DirEntry* dirs = nullptr;
   float* avg = nullptr;
   unsigned long long* stats = nullptr;  // 3 x 8 tracer counters (RTGI_RC_STATS)
@@ -75,18 +83,46 @@ struct rtgi_world {
 int probe_tile_log2(const rtgi_world* w, int c);
 uint32_t probe_index_host(const rtgi_world* w, int c, uint32_t x, uint32_t y, uint32_t z);
 
-// Launch arguments of a world (its current frame, buffers and layout).
+// Launch arguments of a world (its current frame, buffers and layout; probes in the order of data_tile).
 VoxK vox_args(const rtgi_world* w);
 TraceK trace_args(const rtgi_world* w);
+// Marks the start of an update's trace: its radiance and probe arrays are written in the tuning's probe order.
+inline void trace_begin(rtgi_world* w) { w->data_tile = w->tuning.probe_tile; }
 // Advances the bounce volume by one update (the next volume written, its origin from the frame's eye, the update
 // counter); called once for each enqueued update that includes the trace stage, before its launch arguments.
 void bounce_advance(rtgi_world* w);
+// Advances the c0 SH history by one update (the next SH buffer written, the one it blends with and their grid
+// origins); called once for each enqueued update that projects SH, before its launch arguments.
+void sh_advance(rtgi_world* w);
+// Buffers of the c0 SH (1, or 2 with the history on), and floats per buffer.
+int sh_buffers(const rtgi_rc_params* rp);
+size_t sh_floats_of(const rtgi_world* w);
 // Waits for the world's own stream and for the last batched update that included it.
 hipError_t world_wait(rtgi_world* w);
 // Device bytes a world of these parameters holds once a scene of this size is uploaded (rtgi_world_estimate_bytes).
 uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32_t nverts, uint32_t ntris,
                      uint32_t nmat);
 
+// Window origins and c0 grid origin of a frame for these parameters (writeUniform, src/core/voxel.ts:872-875, and the
+// dense probe grid), or RTGI_E_INVALID for a frame no world of them accepts (bad light count, eye not finite or too
+// far from the origin). rtgi_world_set_frame and the server's submit share it.
+int frame_origins(const rtgi_vox_params* vp, const rtgi_rc_params* rp, const rtgi_frame* f,
+                  int origin[RTGI_MAX_LEVELS][3], int G0[3]);
+// Starts world w's part of a batched update with plan p: advances its per-update state (trace order, bounce volume,
+// SH history) and writes its launch descriptor.
+void batch_world_begin(rtgi_world* w, const BatchPlan& p, TraceK* out);
+// Temporal state a world carries from one update to the next (the SH history and the bounce volume); the server
+// saves it when it evicts a world and restores it when the world is admitted again.
+struct WorldState {
+  bool valid = false;
+  std::vector<uint8_t> sh;  // the last SH (device bytes), empty if none was written
+  int sh_G0[3] = {};
+  std::vector<uint32_t> bnvol[2];
+  int bn_par = 1, bn_org[2][3] = {};
+  uint32_t bn_frame = 0;
+};
+int world_save_state(rtgi_world* w, WorldState* st);
+int world_restore_state(rtgi_world* w, const WorldState& st);
 // Block prefix arrays of a batched launch over worlds[0..n): start holds BL_COUNT * (n + 1) entries (launch-major);
 // fills p.n, p.grid, p.mark_lds and p.stats (the caller sets p.w and p.start to device copies).
 void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p);

@@ -34,6 +34,11 @@ static int get_f64(napi_env env, napi_value obj, const char* key, double* out) {
   napi_value v;
   if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return 0;
   if (napi_get_named_property(env, obj, key, &v) != napi_ok) return 0;
+  bool b;
+  if (napi_get_value_bool(env, v, &b) == napi_ok) {  /* the reference's switches are booleans (RcParams, rc.ts:42-91) */
+    *out = b ? 1.0 : 0.0;
+    return 1;
+  }
   return napi_get_value_double(env, v, out) == napi_ok;
 }
 
@@ -123,6 +128,8 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[1], "bounce", &x) && x != 0) rp.flags |= RTGI_RC_BOUNCE;
   if (get_f64(env, argv[1], "objects", &x) && x != 0) rp.flags |= RTGI_RC_OBJECTS;
   if (get_f64(env, argv[1], "fp16", &x) && x != 0) rp.flags |= RTGI_RC_FP16;
+  /* history: false (the reference's screen-probe setting, HIST_ON = !screen, rc.ts:1071) turns the c0 history off */
+  if (get_f64(env, argv[1], "history", &x) && x == 0) rp.flags |= RTGI_RC_NO_HISTORY;
   rtgi_world* w = NULL;
   int st = rtgi_world_create(&vp, &rp, &w);
   if (st) return throw_status(env, st);

@@ -15,7 +15,7 @@ def say(msg):
     print(msg, flush=True)
 
 
-def make_worlds(seed=0):
+def make_specs(seed=0):
     """Uneven worlds: different clipmap sizes, levels, cascade counts, direction caps, pools and scenes."""
     rng = np.random.default_rng(seed)
     specs = []
@@ -34,8 +34,12 @@ def make_worlds(seed=0):
                   rtgi.RcParams(cascades=3, probes0=8, s0=0.5, noMerge=True), (0.0, 0.0, 0.0)))
     specs.append((random_scene(rng, 600, -5, 5, 0.6), rtgi.VoxParams(0.25, 3, 8, 3 * 512, 1 << 16),  # f16 storage
                   rtgi.RcParams(cascades=4, probes0=16, s0=0.5, fp16=True), (0.3, -0.2, 0.1)))
+    return specs
+
+
+def make_worlds(seed=0):
     worlds = []
-    for i, (sc, vox, rc, eye) in enumerate(specs):
+    for i, (sc, vox, rc, eye) in enumerate(make_specs(seed)):
         w = rtgi.World(vox, dataclasses.replace(rc, recordHits=True))
         w.upload_scene(sc)
         w.set_frame(rtgi.Frame(eye=eye, sunDir=(0.2 + 0.1 * i, 1.0, -0.3),
@@ -108,14 +112,22 @@ def test_batch_graph_replays(require_gpu):
             # Eager work between replays: a solo update of one world (same inputs, same results).
             worlds[rep].encode()
             worlds[rep].sync()
-        # A frame change between replays reaches the graph through the descriptors.
+        # A frame change between replays reaches the graph through the descriptors. Repeated updates at one frame
+        # keep the SH (the c0 history blends equal values), so a twin world updated once at the first eye and once
+        # at the moved one holds the moved world's result, history included.
         worlds[0].update((0.9, -0.4, 0.6))
         b.encode()
         b.sync()
         moved = snapshot(worlds[0])
-    worlds[0].encode()
-    worlds[0].sync()
-    assert_same(snapshot(worlds[0]), moved, "moved eye")
+    sc, vox, rc, eye = make_specs(seed=1)[0]
+    twin = rtgi.World(vox, dataclasses.replace(rc, recordHits=True))
+    twin.upload_scene(sc)
+    for e in (eye, (0.9, -0.4, 0.6)):
+        twin.set_frame(rtgi.Frame(eye=e, sunDir=(0.2, 1.0, -0.3)))
+        twin.encode()
+        twin.sync()
+    assert_same(snapshot(twin), moved, "moved eye")
+    twin.destroy()
     for w in worlds:
         w.destroy()
 

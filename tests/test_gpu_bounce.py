@@ -11,7 +11,7 @@ import pytest
 
 import rtgi
 from conftest import random_scene
-from oracle.oracle import Oracle
+from oracle.oracle import History, Oracle
 from test_oracle import floor_world
 
 pytestmark = pytest.mark.gpu
@@ -34,9 +34,9 @@ def moved(frame, d):
     return dataclasses.replace(frame, eye=tuple(float(a + b) for a, b in zip(frame.eye, d)))
 
 
-def check_update(w, o, rc, fp16):
+def check_update(w, o, rc, fp16, hist):
     rgba, hits = o.trace()
-    m, sh = o.merge_sh(rgba)
+    m, sh = o.merge_sh(rgba, hist=hist)
     per_m, per_h = o.split(m), o.split(hits)
     for c in range(rc.cascades):
         g, h = w.read_cascade(c, hits=True)
@@ -73,6 +73,7 @@ def test_bounce_matches_oracle(require_gpu, case):
         w = rtgi.World(vox, rcf)
         w.upload_scene(scene)
         state = None
+        hist = History(rc.probes0)
         for k, f in enumerate(frames):
             say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, sunTrace={sun}, update {k + 1}")
             w.set_frame(f)
@@ -83,7 +84,7 @@ def test_bounce_matches_oracle(require_gpu, case):
             state = o.bounce_update(state)
             active = check_volume(w, state)
             assert active > 0
-            check_update(w, o, rc, fp16)
+            check_update(w, o, rc, fp16, hist)
         w.destroy()
 
 

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import rtgi
-from oracle.oracle import DynState, Oracle
+from oracle.oracle import DynState, History, Oracle
 from test_oracle import dyn_world
 
 pytestmark = pytest.mark.gpu
@@ -36,9 +36,11 @@ def check_layer(w, d):
     return n
 
 
-def check_update(w, o, rc, fp16):
+def check_update(w, o, rc, fp16, hist=None):
+    """The update's cascades, hit ids and SH against the oracle's; hist: the oracle's c0 SH history of the world's
+    earlier updates (advanced here)."""
     rgba, hits = o.trace()
-    m, sh = o.merge_sh(rgba)
+    m, sh = o.merge_sh(rgba, hist=hist)
     per_m, per_h = o.split(m), o.split(hits)
     for c in range(rc.cascades):
         g, h = w.read_cascade(c, hits=True)
@@ -74,6 +76,7 @@ def test_dynamic_layer_atrium_matches_oracle(require_gpu, fp16, reloc, sun):
         w.upload_scene(frames[0][0])
         w.set_dynamic(first, 512)
         d = DynState(vox, 512, first)
+        hist = History(rc.probes0)
         moved = []
         for k, (scene, f) in enumerate(frames):
             say(f"case: dynamic layer, atrium, fp16={fp16}, relocate={reloc}, sunTrace={sun}, update {k + 1}")
@@ -87,7 +90,7 @@ def test_dynamic_layer_atrium_matches_oracle(require_gpu, fp16, reloc, sun):
             n = check_layer(w, d)
             assert n > 0 and d.hdr[1] == 0
             moved.append(d.data[:n].copy())
-            check_update(w, o, rc, fp16)
+            check_update(w, o, rc, fp16, hist)
         assert not np.array_equal(moved[0], moved[-1])
     finally:
         w.destroy()
@@ -105,6 +108,7 @@ def test_dynamic_layer_closed_form_world_and_overflow(require_gpu):
             w.upload_scene(scene)
             w.set_dynamic(scene.firstDynamicTri, cap)
             d = DynState(vox, cap, scene.firstDynamicTri)
+            hist = History(rc.probes0)
             for k, zw in enumerate((0.0, 2.0)):
                 say(f"case: dynamic layer, one square, pool {cap}, z = {zw}")
                 sc = dyn_world(zw=zw)[0]
@@ -117,7 +121,7 @@ def test_dynamic_layer_closed_form_world_and_overflow(require_gpu):
                 o.build(sc, dyn=d)
                 n = check_layer(w, d)
                 assert n == min(cap, 12 if zw == 0 else 10)
-                check_update(w, o, rc, False)
+                check_update(w, o, rc, False, hist)
                 org = np.array([[0.6125, 0.05 * j - 0.3, zw + 0.02 * j - 0.2] for j in range(16)], np.float32)
                 dirs = np.tile(np.array([[-1.0, 0.0, 0.0]], np.float32), (16, 1))
                 gr, gh = w.trace_rays(0, org, dirs)
@@ -211,6 +215,7 @@ def test_dynamic_layer_reference_default_voxel_params(require_gpu):
         w.upload_scene(frames[0][0])
         w.set_dynamic(first, 512)
         d = DynState(vox, 512, first)
+        hist = History(rc.probes0)
         for k, (scene, f) in enumerate(frames):
             say(f"case: dynamic layer, atrium at the default voxel parameters, update {k + 1}")
             if k:
@@ -221,7 +226,7 @@ def test_dynamic_layer_reference_default_voxel_params(require_gpu):
             o = Oracle(vox, rc, f)
             o.build(scene, dyn=d)
             assert check_layer(w, d) > 0
-            check_update(w, o, rc, False)
+            check_update(w, o, rc, False, hist)
     finally:
         w.destroy()
 
@@ -239,9 +244,13 @@ def test_set_dynamic_and_tuning_after_batch_creation(require_gpu):
         w.set_frame(f)
     b = rtgi.Batch([bw], graph=True)
     try:
+        # Both worlds run the same sequence of updates (the second blends with the first's SH history).
+        single.encode()
+        single.sync()
         b.encode()
         b.sync()
         plain = bw.read_sh()
+        assert np.array_equal(plain, single.read_sh())
         for w in (single, bw):
             w.set_dynamic(first, 512)
         bw.set_tuning(fused=0)
@@ -275,6 +284,7 @@ def test_objects_token_matches_oracle(require_gpu, fp16):
     w = worlds[0]
     b = rtgi.Batch(worlds[1:], graph=True) if fp16 else None
     d = DynState(vox, 512, first)
+    hist = History(rc.probes0)
     state = None
     try:
         for k, (scene, f) in enumerate(frames):
@@ -296,7 +306,7 @@ def test_objects_token_matches_oracle(require_gpu, fp16):
             vol, org = w.read_bounce_volume()
             assert np.array_equal(org, state[1])
             assert np.array_equal(vol.view(np.uint16).reshape(-1, 6, 4), state[0])
-            check_update(w, o, rc, fp16)
+            check_update(w, o, rc, fp16, hist)
             if b is not None:
                 b.encode()
                 b.sync()

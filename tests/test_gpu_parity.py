@@ -10,7 +10,7 @@ import pytest
 
 import rtgi
 from conftest import random_scene
-from oracle.oracle import Oracle
+from oracle.oracle import History, Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,9 @@ def check_clipmap(w, o, vox):
         assert np.array_equal(g["plane"].view(np.uint32), c["plane"].view(np.uint32)), f"level {L} plane"
 
 
-def check_cascades(w, o, rc, merged=True):
+def check_cascades(w, o, rc, merged=True, hist=None):
+    """Cascades, hit ids and SH against the oracle; hist: the oracle's c0 SH history of the world's earlier updates
+    (advanced here), None for a world's first update."""
     rgba, hits = o.trace()
     per_rgba, per_hits = o.split(rgba), o.split(hits)
     if not merged:
@@ -50,7 +52,7 @@ def check_cascades(w, o, rc, merged=True):
             assert np.array_equal(h, per_hits[c]), f"cascade {c}: {np.count_nonzero(h != per_hits[c])} hit ids differ"
             np.testing.assert_allclose(g, per_rgba[c], rtol=RTOL, atol=ATOL)
         return
-    m, sh = o.merge_sh(rgba)
+    m, sh = o.merge_sh(rgba, hist=hist)
     per_m = o.split(m)
     for c in range(rc.cascades):
         g, h = w.read_cascade(c, hits=True)
@@ -263,6 +265,10 @@ def test_moving_frames(require_gpu):
     _, vox, rc, frame = rtgi.config("atrium")
     say("case: atrium seed 2, frame 0 then vertices of frame 3")
     w = run_world(scene0, vox, rc, frame)
+    hist = History(rc.probes0)  # the second update's SH blends with the first's
+    o0 = Oracle(vox, rc, frame)
+    o0.build(scene0)
+    o0.merge_sh(o0.trace()[0], hist=hist)
     w.update_vertices(0, scene3.pos, scene3.nrm)
     w.update((0.5, 5.0, -0.5))
     w.encode()
@@ -271,7 +277,7 @@ def test_moving_frames(require_gpu):
     o = Oracle(vox, rc, frame3)
     o.build(scene3)
     check_clipmap(w, o, vox)
-    check_cascades(w, o, rc)
+    check_cascades(w, o, rc, hist=hist)
     w.destroy()
 
 
@@ -384,3 +390,27 @@ def test_tuning_keeps_results(require_gpu, case):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (tile, fused, case)
                 assert ha is None or np.array_equal(ha, hb)
         w.destroy()
+
+
+def test_readback_after_set_tuning(require_gpu):
+    """The probe order can change between an update and its readbacks: cascades, hit ids, SH, relocated probes and the
+    gather keep reading that update's values (the buffers remember the order they were written in)."""
+    from conftest import surface_points
+    scene, vox, rc, frame = tiny_case(seed=3)
+    rcf = dataclasses.replace(rc, recordHits=True, relocate=True)
+    say("case: readbacks after set_tuning(probe_tile=2), tiny scene with relocation")
+    w = run_world(scene, vox, rcf, frame)
+    pos, nrm = surface_points(scene, 500, seed=4)
+
+    def reads():
+        out = [w.read_cascade(c, hits=True) for c in range(rc.cascades)]
+        out += [(w.read_sh(), None), (w.gather(pos, nrm), None)]
+        out += [(np.concatenate(w.read_probes(c)[:2], 1), w.read_probes(c)[2]) for c in range(rc.cascades)]
+        return out
+
+    before = reads()
+    w.set_tuning(probe_tile=2)
+    for (a, ha), (b, hb) in zip(reads(), before):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert ha is None or np.array_equal(ha, hb)
+    w.destroy()

@@ -1,6 +1,7 @@
 """Request server (rtgi_server_*): continuous batching over resident worlds under a memory budget. Each request's
-result (SH and clipmap counters) is bit-identical to the same update run alone on a plain world, whether it ran in a
-mix of uneven worlds, in any batch, with its world evicted and rebuilt in between or not."""
+result (SH and clipmap counters) is bit-identical to the same update run on a plain world of its own that runs its
+world's requests in submission order (the c0 SH history and the bounce volume carry from one update of a world to the
+next), whether it ran in a mix of uneven worlds, in any batch, with its world evicted and rebuilt in between or not."""
 import dataclasses
 import time
 
@@ -27,6 +28,9 @@ def world_specs(seed=0):
                       rtgi.RcParams(cascades=3, probes0=8 * (1 + i % 2), s0=0.5, fp16=(i == 4))))
     at = rtgi.generate_scene("atrium", seed=5)
     specs.append((at, rtgi.VoxParams(0.125, 4, 8, 2048, 1 << 20), rtgi.RcParams(cascades=4, probes0=32, s0=0.625)))
+    # multi-bounce: the bounce volume of each update feeds the next one's hits
+    specs.append((random_scene(rng, 800, -4, 4, 0.6), rtgi.VoxParams(0.25, 3, 8, 3 * 512, 1 << 16),
+                  rtgi.RcParams(cascades=3, probes0=16, s0=0.5, bounce=True)))
     return specs
 
 
@@ -38,14 +42,25 @@ def frames(rng, k):
     return out
 
 
-def alone(spec, frame):
-    sc, vox, rc = spec
-    with rtgi.World(vox, rc) as w:
-        w.upload_scene(sc)
-        w.set_frame(frame)
-        w.encode()
-        w.sync()
-        return w.read_sh(), w.read_counters()
+def sequential(specs, plan):
+    """Each request's SH and counters from a plain world of its own per server world, updated through that world's
+    requests in submission order."""
+    worlds, out = {}, []
+    try:
+        for wi, fr in plan:
+            if wi not in worlds:
+                sc, vox, rc = specs[wi]
+                worlds[wi] = rtgi.World(vox, rc)
+                worlds[wi].upload_scene(sc)
+            w = worlds[wi]
+            w.set_frame(fr)
+            w.encode()
+            w.sync()
+            out.append((w.read_sh(), w.read_counters()))
+    finally:
+        for w in worlds.values():
+            w.destroy()
+    return out
 
 
 def test_estimate_matches_allocation(require_gpu):
@@ -58,15 +73,17 @@ def test_estimate_matches_allocation(require_gpu):
 
 @pytest.mark.parametrize("budget_worlds", [0, 3])
 def test_server_mix_equals_alone(require_gpu, budget_worlds):
-    """Requests over 7 uneven worlds, submitted in bursts and spaced arrivals, several per world with different
-    frames. budget_worlds = 3: the budget holds about 3 of them, so worlds are evicted and rebuilt between updates."""
+    """Requests over 8 uneven worlds (one with multi-bounce), submitted in bursts and spaced arrivals, several per
+    world with different frames. budget_worlds = 3: the budget holds about 3 of them, so worlds are evicted and
+    rebuilt between updates, and their history and bounce volume must survive that."""
     specs = world_specs()
     rng = np.random.default_rng(1)
     plan = []  # (world, frame)
-    for k, fr in enumerate(frames(rng, 28)):
+    for k, fr in enumerate(frames(rng, 32)):
         plan.append((int(rng.integers(0, len(specs))), fr))
+    assert sum(1 for wi, _ in plan if wi == len(specs) - 1) >= 2  # the bounce world runs more than once
     say(f"case: server, {len(specs)} worlds, {len(plan)} requests, budget for {budget_worlds or 'all'} worlds")
-    ref = [alone(specs[wi], fr) for wi, fr in plan]
+    ref = sequential(specs, plan)
     sizes = [rtgi.estimate_bytes(vox, rc, sc.pos.shape[0], sc.tri.shape[0], sc.materials.shape[0])
              for sc, vox, rc in specs]
     budget = 0 if budget_worlds == 0 else int(sum(sorted(sizes)[-budget_worlds:]))
@@ -108,3 +125,18 @@ def test_server_errors(require_gpu):
             srv.submit(0, rtgi.Frame(eye=(0.0, 0.0, 0.0)))
         with pytest.raises(rtgi.RtgiError, match="unknown world"):
             rtgi._lib.check(rtgi.lib().rtgi_server_submit(srv._h, rtgi._lib.RequestC(5, 1, rtgi.Frame(eye=(0.0, 0.0, 0.0)).to_c())))
+
+
+def test_server_refuses_a_far_eye_at_submission(require_gpu):
+    """A frame the world would refuse (its window origin past 1e6 bricks) is refused by submit, so it cannot fail the
+    batch it would have joined; the other requests complete normally."""
+    specs = world_specs()[:3]
+    with rtgi.Server(max_batch=4, slots=1) as srv:
+        ids = [srv.add_world(sc, vox, rc) for sc, vox, rc in specs]
+        t0 = srv.submit(ids[0], rtgi.Frame(eye=(0.1, 0.0, 0.0)), sh=True)
+        with pytest.raises(rtgi.RtgiError, match="too far"):
+            srv.submit(ids[1], rtgi.Frame(eye=(1e9, 0.0, 0.0)))
+        t2 = srv.submit(ids[2], rtgi.Frame(eye=(0.0, 0.2, 0.0)), sh=True)
+        srv.drain()
+        got = {c["tag"]: c for c in srv.poll(16)}
+    assert set(got) == {t0, t2} and all(c["status"] == 0 for c in got.values())

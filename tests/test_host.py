@@ -61,11 +61,17 @@ def test_estimate_bytes_and_server_without_gpu():
     b1 = rtgi.estimate_bytes(vox, rc, 1000, 2000, 2000)
     # Scene buffers: 2 x 12 B per vertex, 16 B + 2 x 4 levels x 4 B per triangle, 16 B per material.
     assert b1 - b0 == 1000 * 24 + 2000 * (16 + 2 * 4 * 4) + 2000 * 16
-    # Cascades: 8,126,464 rays of f32 rgba; f16 halves them, and the SH planes.
+    # Cascades: 8,126,464 rays of f32 rgba; f16 halves them, and the SH planes (two with the c0 history: this
+    # update's and the previous one's).
     f16 = rtgi.estimate_bytes(vox, dataclasses.replace(rc, fp16=True), 0, 0, 0)
-    assert b0 - f16 == 8126464 * 8 + 64 ** 3 * 27 * 2
+    assert b0 - f16 == 8126464 * 8 + 2 * 64 ** 3 * 27 * 2
+    nohist = rtgi.estimate_bytes(vox, dataclasses.replace(rc, history=False), 0, 0, 0)
+    assert b0 - nohist == 64 ** 3 * 27 * 4
     with pytest.raises(rtgi.RtgiError, match="at most 8 levels"):
         rtgi.estimate_bytes(dataclasses.replace(vox, levels=9), rc, 0, 0, 0)
+    # A cascade's rays are indexed in 32 bits: 1024^3 c0 probes x 16 directions is refused before any allocation.
+    with pytest.raises(rtgi.RtgiError, match="below 2\\^32"):
+        rtgi.estimate_bytes(vox, dataclasses.replace(rc, probes0=1024, fp16=True), 0, 0, 0)
     with pytest.raises(rtgi.RtgiError, match="no HIP device"):
         rtgi.Server()
 

@@ -965,3 +965,80 @@ def test_golden_world_regression():
     m, sh = o.merge_sh(rgba)
     np.testing.assert_array_equal(m, g["merged"])
     np.testing.assert_array_equal(sh, g["sh"])
+
+
+def _hist_rgba(o, frame_no):
+    """Per-ray radiance of a one-cascade dense grid that depends on the probe's world grid point, the direction and
+    the frame: a wrong history row shows as a wrong value."""
+    n = o.rc.probes0
+    g0 = np.array(o.rc.G0[:3])
+    pi = np.arange(n ** 3)
+    g = np.stack([pi % n, (pi // n) % n, pi // (n * n)], 1) + g0
+    d = np.arange(16)
+    L = (1.0 + 0.05 * g[:, 0] - 0.03 * g[:, 1] + 0.02 * g[:, 2])[:, None] + 0.1 * d[None, :] + frame_no
+    rgba = np.zeros((n ** 3 * 16, 4), np.float32)
+    rgba[:, 0] = L.ravel()
+    rgba[:, 1] = 0.5 * L.ravel()
+    rgba[:, 2] = 2.0 - 0.01 * L.ravel()
+    return rgba, g
+
+
+def test_history_blend_closed_form():
+    """histBlend (rc.ts:784-814): each probe's SH is prev * 0.5 + new * 0.5 of the previous update's stored SH at the
+    same world grid point, the frame's own SH where the previous grid did not hold that point; the parity flips per
+    update, so the third update reads the second's blended values. The expectation indexes the plain per-update SH by
+    world grid point in numpy (not through the oracle's hash)."""
+    vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
+    rc = rtgi.RcParams(cascades=1, probes0=8, s0=0.5)
+    eyes = [(0.1, 0.2, 0.3), (1.1, 0.2, -0.2), (1.1, 0.2, -0.2), (-0.9, 0.7, -0.2)]  # shifts (2, 0, -1), 0, (-4, 1, 0)
+    for limit in (0, 64):
+        hist = orc.History(rc.probes0, limit=limit)
+        prev = None  # {world grid point: stored SH}
+        shifted = kept = 0
+        for f, eye in enumerate(eyes):
+            o = Oracle(vox, rc, rtgi.Frame(eye=eye, sky=(0.0, 0.0, 0.0)))
+            rgba, g = _hist_rgba(o, f)
+            _, plain = o.merge_sh(rgba)
+            _, got = o.merge_sh(rgba, hist=hist)
+            exp = plain.copy()
+            if prev is not None:
+                for p in range(g.shape[0]):
+                    h = prev.get(tuple(g[p]))
+                    if h is not None:
+                        exp[p] = h * np.float32(0.5) + plain[p] * np.float32(0.5)
+                        kept += 1
+                    else:
+                        shifted += 1
+            np.testing.assert_array_equal(got, exp, err_msg=f"update {f + 1}, limit {limit}")
+            prev = {tuple(g[p]): got[p] for p in range(g.shape[0])}
+        assert kept > 0 and shifted > 0 and hist.misses == 0
+    # A constant field keeps its SH through the blend; the history of a moved grid is exactly the overlap's.
+    hist = orc.History(rc.probes0)
+    for eye in eyes:
+        o = Oracle(vox, rc, rtgi.Frame(eye=eye, sky=(0.0, 0.0, 0.0)))
+        rgba = np.zeros((8 ** 3 * 16, 4), np.float32)
+        rgba[:, :3] = 1.5
+        _, got = o.merge_sh(rgba, hist=hist)
+        assert np.all(np.abs(got[:, 0] - np.pi * 0.282095 * 4 * np.pi * 1.5) < 1e-4)
+
+
+def test_history_hash_limit_on_dense_grids():
+    """The reference's 64-slot linear-probing search (rc.ts:791-800, hashSize = nextPow2(2 cap)) finds every probe of
+    dense grids up to 32^3 but misses a few at 64^3 (long clusters at load 1/2); the whole-table search (the device's
+    grid-point index) misses none. Both blend every probe they find identically."""
+    vox = rtgi.VoxParams(v0=0.25, levels=2, G=2, maxBricks=16, maxVoxels=1024)
+    for n, ref_misses in ((16, 0), (64, None)):
+        rc = rtgi.RcParams(cascades=1, probes0=n, s0=0.5)
+        h64, h0 = orc.History(n, limit=64), orc.History(n, limit=0)
+        for f, eye in enumerate([(0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (2.0, 0.5, -1.0)]):
+            o = Oracle(vox, rc, rtgi.Frame(eye=eye, sky=(0.0, 0.0, 0.0)))
+            rgba, _ = _hist_rgba(o, f)
+            _, a = o.merge_sh(rgba, hist=h64)
+            _, b = o.merge_sh(rgba, hist=h0)
+            differ = np.count_nonzero(np.any(a != b, axis=1))
+            assert differ <= h64.misses  # only the probes the limited search missed differ
+        assert h0.misses == 0
+        if ref_misses is not None:
+            assert h64.misses == ref_misses
+        else:
+            assert 0 < h64.misses < 64  # a handful of 262,144 probes per update
