@@ -1199,14 +1199,12 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   for (int i = 0; i < n; i++) p.fused = p.fused && worlds[i]->tuning.fused != 0;
   for (int i = 0; i < n; i++) {
     TraceK t = trace_args(worlds[i]);
-    t.sh_fuse = p.fused ? 1 : 0;
     uint32_t g[BL_COUNT];
     batch_blocks(t, g);
     if (worlds[i]->rp.flags & RTGI_RC_NO_MERGE) {
       for (int st = 0; st < RTGI_MAX_CASCADES; st++) g[BL_MERGE + st] = 0;
       g[BL_SH] = 0;
     }
-    if (p.fused) g[BL_SH] = 0;  // the c0 trace+merge launch projects the SH
     for (int l = 0; l < BL_COUNT; l++) start[(size_t)l * (n + 1) + i + 1] = start[(size_t)l * (n + 1) + i] + g[l];
     size_t m = vox_mark_lds_bytes(t.vx);
     if (m == 0) all_lds = false;
@@ -1236,7 +1234,6 @@ void batch_world_begin(rtgi_world* w, const BatchPlan& p, TraceK* out) {
   bounce_advance(w);
   sh_advance(w);
   *out = trace_args(w);
-  out->sh_fuse = p.fused ? 1 : 0;
 }
 
 int plan_launch_stage(const BatchPlan& p, hipStream_t s, int stage) {

@@ -707,14 +707,12 @@ __device__ __forceinline__ uint32_t hist_row(const TraceK& a, i3 gl) {
 // One SH value (plane k = 3 * coefficient + channel) of probe row: the scaled coefficient v, blended with the history
 // value at hrow (mix(prev, v, HIST_W = 0.5) = prev * (1 - 0.5) + v * 0.5, rc.ts:801-804), stored as f32 or as a half
 // (pack2x16float per value, rc.ts:770-771).
+__device__ __forceinline__ float sh_get(const float* sh, int fp16, size_t j) {
+  return fp16 ? __half2float(__ushort_as_half(reinterpret_cast<const unsigned short*>(sh)[j])) : sh[j];
+}
 __device__ __forceinline__ void sh_put(const TraceK& a, uint32_t np, uint32_t row, uint32_t hrow, int k, float v) {
   const size_t i = (size_t)k * np + row;
-  if (hrow != ~0u) {
-    const size_t j = (size_t)k * np + hrow;
-    float p = a.fp16 ? __half2float(__ushort_as_half(reinterpret_cast<const unsigned short*>(a.sh_prev)[j]))
-                     : a.sh_prev[j];
-    v = p * (1.0f - 0.5f) + v * 0.5f;
-  }
+  if (hrow != ~0u) v = sh_get(a.sh_prev, a.fp16, (size_t)k * np + hrow) * (1.0f - 0.5f) + v * 0.5f;
   if (a.fp16)
     reinterpret_cast<unsigned short*>(a.sh)[i] = __half_as_ushort(__float2half_rn(v));
   else
@@ -739,13 +737,22 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
     sh_basis(a.dirs[C.dir_off + d], y);
     for (int q = 0; q < 9; q++) cf[q] = cf[q] + r * y[q];
   }
-  const uint32_t hrow = hist_row(a, gl);
+  float v[27];
   for (int q = 0; q < 9; q++) {
     const float s = sh_scale(q);
-    sh_put(a, C.np, row, hrow, 3 * q, cf[q].x * s);
-    sh_put(a, C.np, row, hrow, 3 * q + 1, cf[q].y * s);
-    sh_put(a, C.np, row, hrow, 3 * q + 2, cf[q].z * s);
+    v[3 * q] = cf[q].x * s;
+    v[3 * q + 1] = cf[q].y * s;
+    v[3 * q + 2] = cf[q].z * s;
   }
+  // The history's 27 values are read before any store (the stores might alias them as far as the compiler knows),
+  // so the reads are in flight together.
+  const uint32_t hrow = hist_row(a, gl);
+  if (hrow != ~0u) {
+    float p[27];
+    for (int k = 0; k < 27; k++) p[k] = sh_get(a.sh_prev, a.fp16, (size_t)k * C.np + hrow);
+    for (int k = 0; k < 27; k++) v[k] = p[k] * (1.0f - 0.5f) + v[k] * 0.5f;
+  }
+  for (int k = 0; k < 27; k++) sh_put(a, C.np, row, ~0u, k, v[k]);
 }
 __global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
@@ -781,16 +788,49 @@ __device__ __forceinline__ float4 rad_round(const TraceK& a, float4 v) {
   return make_float4(hlo(p0), hhi(p0), hlo(p1), hhi(p1));
 }
 
+// projectSH of a c0 block of 16 probes x 16 directions (sh_fuse). Before the trace, sh0_setup stages the basis and
+// solid angles of the 16 directions in LDS; after it, each thread gives its merged radiance (p = probe of the block,
+// di = direction) and the block forms value k = 3 q + channel of probe p per thread (432 per block), summed over the
+// directions in order as sh_body does (the same operations on the same stored values).
+struct Sh0Lds {
+  float rad[16][16][3];  // [probe][direction][rgb]
+  float y[16][9];        // shBasis of the 16 c0 directions
+  float om[16];          // their solid angles
+};
+__device__ __forceinline__ void sh0_setup(const TraceK& a, Sh0Lds& s) {
+  if (threadIdx.x < 16u) {
+    sh_basis(a.dirs[a.c[0].dir_off + threadIdx.x], s.y[threadIdx.x]);
+    s.om[threadIdx.x] = a.omega0[threadIdx.x];
+  }
+}
+__device__ __forceinline__ void sh0_block(const TraceK& a, Sh0Lds& s, uint32_t lb, uint32_t p, uint32_t di, float mx,
+                                          float my, float mz) {
+  const CascK& C = a.c[0];
+  s.rad[p][di][0] = mx;
+  s.rad[p][di][1] = my;
+  s.rad[p][di][2] = mz;
+  __syncthreads();
+#pragma unroll 1
+  for (uint32_t o = threadIdx.x; o < 16u * 27u; o += BLK) {
+    const uint32_t pb = o / 27u, k = o % 27u, q = k / 3u, ch = k % 3u;
+    const uint32_t pp = lb * 16u + pb;
+    if (pp >= C.np) continue;
+    float cf = 0.0f;
+    for (uint32_t d = 0; d < 16; d++) cf = cf + (s.rad[pb][d][ch] * s.om[d]) * s.y[d][q];
+    const i3 gl = probe_xyz(C, pp);
+    sh_put(a, C.np, grid_row(C, gl), hist_row(a, gl), (int)k, cf * sh_scale((int)q));
+  }
+}
+
 // With sh_fuse, cascade 0's launch also projects the SH: a block holds 16 probes x all 16 c0 directions (wavefront w
-// = 2x2 block (w & 1, w >> 1) of the 4x4 texels, lane = 16 j + probe as above), stages the merged radiance in LDS and
-// forms each probe's 27 SH values in the direction order of projectSH, so the c0 radiance is not read back.
-template <bool ST, bool DY>
+// = 2x2 block (w & 1, w >> 1) of the 4x4 texels, lane = 16 j + probe as above) and ends with sh0_block, so the c0
+// radiance is not read back. SH: the launch holds some world's cascade 0 (a separate instantiation).
+template <bool ST, bool DY, bool SH>
 __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
-  __shared__ float s_rad[16][16][3];  // [probe][direction][rgb]
-  __shared__ float s_y[16][9];        // shBasis of the 16 c0 directions
+  __shared__ Sh0Lds s_sh[1];  // referenced only by the SH instantiation
   const CascK& C = a.c[c];
-  const bool quad = !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
-  const bool sh0 = c == 0 && a.sh_fuse && !a.no_merge;
+  const bool quad = !SH && !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
+  const bool sh0 = SH;  // the SH instantiation runs cascade 0 of worlds with sh_fuse only
   uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
   uint32_t lane = threadIdx.x & 63u, di = 0, pi = 0, dl = 0, j = 0;
   bool ok;
@@ -800,7 +840,7 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
     pi = lb * 16u + (lane & 15u);
     ok = pi < C.np;
     di = ((2u * (wv >> 1) + (j >> 1)) << 2) + 2u * (wv & 1u) + (j & 1u);
-    if (threadIdx.x < 16u) sh_basis(a.dirs[C.dir_off + threadIdx.x], s_y[threadIdx.x]);
+    if (SH) sh0_setup(a, s_sh[0]);
   } else if (quad) {
     const CascK& D = a.c[c - 1];
     uint32_t npb = (C.np + 15u) >> 4;
@@ -860,22 +900,8 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
       if (cap) atomicAdd(&S[2 * RTGI_MAX_CASCADES + c], cap);
     }
   }
-  if (sh0) {
-    // projectSH of the block's 16 probes: value k = 3 q + channel of probe p per thread (432 per block), summed over
-    // the directions in order as sh_body does (the same operations on the same stored values).
-    s_rad[lane & 15u][di][0] = m.x;
-    s_rad[lane & 15u][di][1] = m.y;
-    s_rad[lane & 15u][di][2] = m.z;
-    __syncthreads();
-    for (uint32_t o = threadIdx.x; o < 16u * 27u; o += BLK) {
-      const uint32_t p = o / 27u, k = o % 27u, q = k / 3u, ch = k % 3u;
-      const uint32_t pp = lb * 16u + p;
-      if (pp >= C.np) continue;
-      float cf = 0.0f;
-      for (uint32_t d = 0; d < 16; d++) cf = cf + (s_rad[p][d][ch] * a.omega0[d]) * s_y[d][q];
-      const i3 gl = probe_xyz(C, pp);
-      sh_put(a, C.np, grid_row(C, gl), hist_row(a, gl), (int)k, cf * sh_scale((int)q));
-    }
+  if (SH && sh0) {
+    sh0_block(a, s_sh[0], lb, lane & 15u, di, m.x, m.y, m.z);
     return;
   }
   if (a.no_merge || c == 0) return;
@@ -895,17 +921,18 @@ __device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_
   }
 }
 // Eight wavefronts per SIMD (at most 64 VGPRs): the trace is latency-bound, and occupancy pays more than registers.
-template <bool ST, bool DY>
+template <bool ST, bool DY, bool SH>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge(TraceK a, int c) {
-  trace_merge_body<ST, DY>(a, c, blockIdx.x, gridDim.x);
+  trace_merge_body<ST, DY, SH>(a, c, blockIdx.x, gridDim.x);
 }
-// Step of a batch: each world traces and merges its cascade N-1-step.
+// Step of a batch: each world traces and merges its cascade N-1-step. Batches project the SH in their own launch
+// (k_sh_b): with the projection, this kernel needs 80 VGPRs and spills at 8 wavefronts per SIMD.
 template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   const TraceK& a = b.w[w];
-  trace_merge_body<ST, DY>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  trace_merge_body<ST, DY, false>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
@@ -1626,14 +1653,25 @@ uint64_t trace_merge_threads(const TraceK& k, int c) {
   return merge_threads(k, c);
 }
 
+using TraceMergeFn = void (*)(TraceK, int);
+using TraceMergeBatchFn = void (*)(BatchK);
+template <bool SH>
+TraceMergeFn trace_merge_fn(bool st, bool dy) {
+  return st ? (dy ? k_trace_merge<true, true, SH> : k_trace_merge<true, false, SH>)
+            : (dy ? k_trace_merge<false, true, SH> : k_trace_merge<false, false, SH>);
+}
+TraceMergeBatchFn trace_merge_batch_fn(bool st, bool dy) {
+  return st ? (dy ? k_trace_merge_b<true, true> : k_trace_merge_b<true, false>)
+            : (dy ? k_trace_merge_b<false, true> : k_trace_merge_b<false, false>);
+}
+
 int launch_trace_merge(const TraceK& k, int c, void* stream) {
   const bool dy = k.vx.dyn.hdr != nullptr;
   uint64_t n = trace_merge_threads(k, c);
   if (!n) return 0;
-  if (k.vx.stats)
-    (dy ? k_trace_merge<true, true> : k_trace_merge<true, false>)<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
-  else
-    (dy ? k_trace_merge<false, true> : k_trace_merge<false, false>)<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
+  const bool sh = c == 0 && k.sh_fuse && !k.no_merge;
+  TraceMergeFn f = sh ? trace_merge_fn<true>(k.vx.stats, dy) : trace_merge_fn<false>(k.vx.stats, dy);
+  f<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   return (int)hipGetLastError();
 }
 
@@ -1729,14 +1767,8 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
 }
 
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
-  if (p.grid[BL_TM + step]) {
-    if (p.stats)
-      (p.dyn ? k_trace_merge_b<true, true> : k_trace_merge_b<true, false>)<<<p.grid[BL_TM + step], BLK, 0,
-                                                                             (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
-    else
-      (p.dyn ? k_trace_merge_b<false, true> : k_trace_merge_b<false, false>)<<<p.grid[BL_TM + step], BLK, 0,
-                                                                               (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
-  }
+  if (p.grid[BL_TM + step])
+    trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
   return (int)hipGetLastError();
 }
 
