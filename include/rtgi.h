@@ -272,6 +272,16 @@ int rtgi_world_read_bounce_volume(rtgi_world* w, uint16_t* texels, int32_t origi
 #define RTGI_GATHER_OCCLUSION 1u
 int rtgi_world_gather(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags, float* irr);
 
+/* rtgi_world_gather for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. It allocates nothing: the points and the irradiance are
+ * staged in a fixed device array of RTGI_GATHER_SCRATCH_POINTS points (9 floats each, 9 MiB) that each GPU the library
+ * runs on holds until the process ends; n above that is refused (RTGI_E_INVALID). Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (from several threads) race. Same arguments, results and errors as
+ * rtgi_world_gather. Synchronous. */
+#define RTGI_GATHER_SCRATCH_POINTS 262144u
+int rtgi_world_gather_noalloc(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags,
+                              float* irr);
+
 /* Tracer statistics of the last update (RTGI_RC_STATS): per cascade the world traces, the DDA loop iterations and
  * the traces that reached the 1024-iteration cap (the reference's counts[24..32), [32..40), [64..72), rc.ts:232-235). */
 typedef struct rtgi_trace_stats {

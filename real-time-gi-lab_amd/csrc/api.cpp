@@ -1060,6 +1060,32 @@ extern "C" int rtgi_world_gather(rtgi_world* w, const float* pos, const float* n
   return e;
 }
 
+extern "C" int rtgi_world_gather_noalloc(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags,
+                                         float* irr) {
+  if (!w || (n && (!pos || !nrm || !irr))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather_noalloc: null argument");
+  if (flags & ~RTGI_GATHER_OCCLUSION) return rtgi_fail(RTGI_E_INVALID, "unknown gather flags");
+  if (n > RTGI_GATHER_SCRATCH_POINTS)
+    return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather_noalloc: at most %u points per call (got %u)",
+                     RTGI_GATHER_SCRATCH_POINTS, n);
+  if (w->rp.flags & RTGI_RC_NO_MERGE) return rtgi_fail(RTGI_E_STATE, "no irradiance field (RTGI_RC_NO_MERGE)");
+  if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "no scene or no frame");
+  if (n == 0) return RTGI_OK;
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  float* sp = nullptr;  // this device's copy of the staging array
+  if (int se = gather_scratch(w->device, &sp))
+    return rtgi_fail(RTGI_E_HIP, "gather staging address: %s", hipGetErrorString((hipError_t)se));
+  const size_t M = RTGI_GATHER_SCRATCH_POINTS;
+  float *dpos = sp, *dnrm = sp + 3 * M, *dirr = sp + 6 * M;
+  HIPCHK(hipMemcpyAsync(dpos, pos, (size_t)n * 12, hipMemcpyHostToDevice, w->stream));
+  HIPCHK(hipMemcpyAsync(dnrm, nrm, (size_t)n * 12, hipMemcpyHostToDevice, w->stream));
+  int le = launch_gather(trace_args(w), dpos, dnrm, n, (flags & RTGI_GATHER_OCCLUSION) ? 1 : 0, dirr, w->stream);
+  if (le) return rtgi_fail(RTGI_E_HIP, "gather launch failed: %s", hipGetErrorString((hipError_t)le));
+  HIPCHK(hipMemcpyAsync(irr, dirr, (size_t)n * 12, hipMemcpyDeviceToHost, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  return RTGI_OK;
+}
+
 extern "C" int rtgi_world_read_trace_stats(rtgi_world* w, rtgi_trace_stats* out) {
   if (!w || !out) return rtgi_fail(RTGI_E_INVALID, "null argument");
   if (!w->stats) return rtgi_fail(RTGI_E_STATE, "tracer statistics are off (RTGI_RC_STATS)");

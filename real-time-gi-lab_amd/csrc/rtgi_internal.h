@@ -191,6 +191,11 @@ int launch_trace_rays(const TraceK& k, int c, const float* o, const float* d, ui
                       uint32_t* hits, void* stream);
 // Gather of the irradiance field at n surface points (device arrays: 3 floats each) -> irr (3 floats each).
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream);
+// Device address (on `device`) of the allocation-free gather's staging array: RTGI_GATHER_SCRATCH_POINTS x 9 floats
+// (positions, normals, irradiance). Looked up once per device (hipGetSymbolAddress with the device current); returns
+// a hipError_t as int.
+constexpr int RTGI_SCRATCH_MAX_DEVICES = 64;
+int gather_scratch(int device, float** out);
 // Traced sun visibility of every occupied voxel (thread per brick voxel).
 int launch_sun(const TraceK& k, void* stream);
 // Bounce volume update (8 probes per block), then bounce injection of every occupied voxel from it (thread per

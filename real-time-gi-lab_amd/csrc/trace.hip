@@ -26,6 +26,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "rtgi_internal.h"
 #include "batch.h"
 #include "vmath.h"
@@ -1673,6 +1675,30 @@ int launch_trace_merge(const TraceK& k, int c, void* stream) {
   TraceMergeFn f = sh ? trace_merge_fn<true>(k.vx.stats, dy) : trace_merge_fn<false>(k.vx.stats, dy);
   f<<<grid_for(n), BLK, 0, (hipStream_t)stream>>>(k, c);
   return (int)hipGetLastError();
+}
+
+// Staging of the allocation-free gather: [pos 3 x max][nrm 3 x max][irr 3 x max] floats. Each GPU holds its own copy
+// from the library's load to the end of the process; every call on a GPU shares it (calls that overlap on one GPU
+// race). In host code `scratch` only names the symbol: kernels get the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[(size_t)RTGI_GATHER_SCRATCH_POINTS * 9];
+
+int gather_scratch(int device, float** out) {
+  static std::mutex mu;
+  static float* ptr[RTGI_SCRATCH_MAX_DEVICES] = {};
+  if (device < 0 || device >= RTGI_SCRATCH_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);  // the symbol's address on this device
+    float* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return (int)e;
+    ptr[device] = p;
+  }
+  *out = ptr[device];
+  return 0;
 }
 
 int launch_gather(const TraceK& k, const float* pos, const float* nrm, uint32_t n, int occl, float* irr, void* stream) {
