@@ -224,6 +224,85 @@ def serve_bench(a):
     }
 
 
+def serve_binding_bench(a):
+    """Serving under a binding budget: the serving leg's mix of worlds with a device budget of half their bytes, so
+    requests for worlds that are not resident evict idle ones (their SH history and bounce state are kept in host
+    memory) and rebuild them on admission. Poisson arrivals at the same rate, half as long."""
+    import rtgi
+    from rtgi.workloads import world_mix
+
+    mix = world_mix(a.serve_worlds, seed=7)
+    sizes = [rtgi.estimate_bytes(w.vox, w.rc, w.scene.pos.shape[0], w.scene.tri.shape[0], w.scene.materials.shape[0])
+             for w in mix]
+    budget = max(int(0.5 * sum(sizes)), max(sizes))
+    srv = rtgi.Server(budget=budget, max_batch=a.serve_batch, slots=2)
+    ids = [srv.add_world(w.scene, w.vox, w.rc) for w in mix]
+    rng = np.random.default_rng(13)
+    n_req = int(a.serve_rate * a.serve_seconds / 2)
+    arrive = np.cumsum(rng.exponential(1.0 / a.serve_rate, n_req))
+    worlds = rng.integers(0, len(mix), n_req)
+    st0 = srv.stats()
+    done = []
+    ts = time.perf_counter()
+    for k in range(n_req):
+        while time.perf_counter() - ts < arrive[k]:
+            gap = arrive[k] - (time.perf_counter() - ts)
+            if gap > 5e-4:
+                time.sleep(min(gap - 4e-4, 2e-3))
+        srv.submit(ids[int(worlds[k])], mix[int(worlds[k])].frame(rng))
+        if k % 64 == 63:
+            done.extend(srv.poll(4096))
+    srv.drain()
+    done.extend(srv.poll(n_req + 16))
+    st = srv.stats()
+    srv.destroy()
+    lat = [c["t_done"] - c["t_submit"] for c in done]
+    span = max(c["t_done"] for c in done) - min(c["t_submit"] for c in done)
+    fails = sum(1 for c in done if c["status"] != 0)
+    log(f"serving, binding budget: {len(done) / span:.1f} updates/s, p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms, "
+        f"{st['admissions'] - st0['admissions']} admissions, {st['evictions'] - st0['evictions']} evictions")
+    return {"budget_gib": round(budget / 2 ** 30, 3), "resident_bytes_all_gib": round(sum(sizes) / 2 ** 30, 3),
+            "requests": len(done), "failed": fails, "updates_per_s": round(len(done) / span, 2),
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": pct(lat, 100)},
+            "admissions": st["admissions"] - st0["admissions"], "evictions": st["evictions"] - st0["evictions"],
+            "peak_resident_gib": round(st["peak_resident_bytes"] / 2 ** 30, 3)}
+
+
+def width_bench(a, worlds, widths=(8, 16, 32, 64)):
+    """Latency against throughput at the headline configuration: the resident worlds updated in batches of width W,
+    one batch at a time (a batch is launched when the previous one has completed, as a server with one slot launches
+    them). An update's latency is its batch's time from launch to completion on the otherwise idle GPU (host clock:
+    enqueue of the graph replay to the end of the wait); throughput counts every update over the whole run."""
+    import rtgi
+
+    out = []
+    for W in widths:
+        if W > len(worlds):
+            continue
+        batches = [rtgi.Batch(worlds[i:i + W], graph=True) for i in range(0, len(worlds), W)]
+        for b in batches:  # capture and warm up
+            b.encode()
+            b.sync()
+        lat, n = [], 0
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            for b in batches:
+                t0 = time.perf_counter()
+                b.encode()
+                b.sync()
+                lat.extend([time.perf_counter() - t0] * len(b.worlds))
+                n += len(b.worlds)
+        rate = n / (time.perf_counter() - t)
+        for b in batches:
+            b.destroy()
+        out.append({"width": W, "updates_per_s": round(rate, 2), "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)}})
+        log(f"width {W}: {rate:.1f} updates/s, p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms")
+    ok = [r for r in out if r["latency_ms"]["p99"] <= 4.0]
+    best = max(ok, key=lambda r: r["width"]) if ok else None
+    return {"workload": f"{len(worlds)} resident worlds of the headline, batches of W run one at a time (hipGraph)",
+            "curve": out, "widest_p99_le_4ms": best}
+
+
 def full_bench(a, scenes, vox, rc, frame0, tuning, bsz):
     """The same worlds with the reference's default extras on: voxel relocation with occlusion-aware parents, traced
     per-voxel sun visibility and the multi-bounce volume. Batched hipGraph updates with the headline's tuning and
@@ -356,6 +435,7 @@ def main():
     ap.add_argument("--full-worlds", type=int, default=64,
                     help="run with relocation, traced sun visibility and multi-bounce on (0: skip it)")
     ap.add_argument("--dyn-worlds", type=int, default=16, help="atrium worlds of the dynamic-layer run (0: skip)")
+    ap.add_argument("--no-widths", dest="widths", action="store_false", help="skip the latency-vs-width curve")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
     ap.add_argument("--serve-worlds", type=int, default=64)
@@ -463,36 +543,45 @@ def main():
     elapsed = time.perf_counter() - t_start
     value = a.steps * len(worlds) / elapsed
     stage = {k: float(np.mean(v)) for k, v in st.items()}
-    # Roofline of the dominant kernel, the fused trace+merge: one launch is one cascade step of the batch (5 launches
-    # per update). Algorithmic bytes per world update: 16 B of f32 rgba written per ray; fused with the merge, also
-    # each cascade c >= 1's 2x2 averages (one float4 per 4 rays) written once and read once by cascade c-1. Achieved
-    # = algorithmic bytes per launch / mean launch duration (the trace stage's HIP-event time over its 5 launches).
+    # Roofline of the dominant kernel, the trace (fused with the merge: one launch per cascade step of the batch, 5
+    # per update). Algorithmic bytes per world update are the trace's and merge's of the per-unit figures: 16 B of f32
+    # rgba written per ray, and per merge level c < N-1 (2 R_c + R_{c+1}) x 16 B (own radiance read and written, the
+    # level above read once), 2 R_{N-1} x 16 B for the last. Achieved = those bytes per launch / the mean launch
+    # duration (the trace stage's HIP-event time over its launches). The fused kernel itself needs fewer bytes: the
+    # traced value is merged in registers, so it writes 16 B per ray plus each level c >= 1's 2x2 averages (one float4
+    # per 4 rays, written and read once): frac_fused_bytes relates those to the same time.
     per_c = lay["rays"]
     fused = worlds_tuning["fused"] != 0
-    alg_world = 16 * rays + (2 * 16 * sum(r // 4 for r in per_c[1:]) if fused else 0)
+    merge_world = 16 * (sum(2 * per_c[c] + per_c[c + 1] for c in range(len(per_c) - 1)) + 2 * per_c[-1])
+    alg_world = 16 * rays + merge_world
+    fused_world = 16 * rays + 2 * 16 * sum(r // 4 for r in per_c[1:])
     launches = len(per_c) if fused else 1
     alg_launch = alg_world * len(groups[0]) / launches
-    launch_s = stage["trace"] * 1e-3 / launches
+    launch_s = (stage["trace"] + (0.0 if fused else stage["merge"])) * 1e-3 / launches
     achieved = alg_launch / launch_s / 1e9
-    # HBM traffic per launch from the committed FETCH_SIZE / WRITE_SIZE profile of the fused city trace (copy-
-    # calibrated), scaled to this batch width; null for other configurations. VALU issue from the same profile.
-    traffic, valu_busy = None, None
+    achieved_fused = fused_world * len(groups[0]) / launches / launch_s / 1e9
+    # HBM traffic per launch, L2 hit rate, VALU issue and active lanes from the committed counter profile of the
+    # fused city trace (FETCH_SIZE / WRITE_SIZE copy-calibrated), scaled to this batch width; null for other
+    # configurations.
+    traffic, prof = None, {}
     tf = os.path.join(ROOT, "profiles", "trace_traffic_city.json")
     if a.config == "city" and fused and os.path.exists(tf):
         with open(tf) as fh:
             prof = json.load(fh)
         traffic = int(prof["hbm_bytes_per_world_update"] * len(groups[0]) / launches)
-        valu_busy = prof.get("valu_issue_busy")
     cnt = worlds[0].read_counters()
 
     for b in batches:
         b.destroy()
+    widths = width_bench(a, worlds) if a.widths else None
     for w in worlds:
         w.destroy()
     full = (full_bench(a, scenes, first[1], first[2], first[3], worlds_tuning, bsz) if a.full_worlds > 0 else None)
     mixed = mix_bench(a) if a.mix_worlds > 0 else None
     dynamic = dyn_bench(a) if a.dyn_worlds > 0 else None
     serving = serve_bench(a) if a.serve_seconds > 0 else None
+    if serving is not None:
+        serving["binding_budget"] = serve_binding_bench(a)
 
     cpu = None
     if not a.no_cpu:
@@ -532,13 +621,22 @@ def main():
         "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
         "roofline": {"kernel": ("k_trace_merge_b (one launch per cascade step of the batch)" if fused
-                                else "k_trace_b (one batched launch)"),
+                                else "k_trace_b + k_merge_step_b"),
                      "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes_per_launch": int(alg_launch), "launch_ms": round(launch_s * 1e3, 4),
                      "bytes_per_world_update": int(alg_world),
-                     "valu_issue_busy_profiled": valu_busy},
+                     "fused_kernel_bytes_per_world_update": int(fused_world),
+                     "frac_fused_bytes": round(achieved_fused / HBM_PEAK_GBS, 5),
+                     "limiter": "valu" if prof.get("valu_issue_busy", 0) > 0.9 else None,
+                     "valu_issue_busy_profiled": prof.get("valu_issue_busy"),
+                     "lanes_per_valu_profiled": prof.get("lanes_per_valu"),
+                     "l2_hit_profiled": prof.get("l2_hit"),
+                     "limiter_note": "VALU issue is busy for about every cycle of the launch while HBM moves well "
+                                     "below peak: the DDA is bound by vector instruction issue; frac is against the "
+                                     "HBM roofline because nothing on the path is an MFMA contraction"},
         "cpu_baseline": cpu,
+        "latency_vs_width": widths,
         "full_default": full,
         "dynamic_layer": dynamic,
         "batched_mix": mixed,
