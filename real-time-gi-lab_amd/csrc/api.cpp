@@ -262,38 +262,63 @@ void dyn_free(rtgi_world* w) {
   dfree(w, w->dyn_data, cap * 512 * DYN_WORDS);
 }
 
-void free_all(rtgi_world* w) {
+// The world's fixed buffers live in one device allocation (the arena): one hipMalloc and one hipFree per world,
+// each buffer 256-byte aligned. carve_fixed lays them out in a fixed order; with base == null it only adds up the
+// bytes (the estimate and the arena's size).
+constexpr size_t ARENA_ALIGN = 256;
+struct Carver {
+  char* base = nullptr;
+  size_t off = 0;
+};
+template <class T>
+void carve(Carver& cv, T** p, size_t count, bool on = true) {
+  if (!on) return;
+  size_t b = (std::max<size_t>(count, 1) * sizeof(T) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN;
+  if (cv.base) *p = reinterpret_cast<T*>(cv.base + cv.off);
+  cv.off += b;
+}
+void carve_fixed(rtgi_world* w, Carver& cv) {
+  const rtgi_rc_params* rp = &w->rp;
+  const int Nc = w->N;
   size_t NLper = (size_t)w->NL * w->per, mb = (size_t)w->vp.max_bricks, mv = (size_t)w->vp.max_voxels;
-  dfree(w, w->hdr, HDR_WORDS);
-  dfree(w, w->ptr, NLper);
-  dfree(w, w->occ, mb * 16);
-  dfree(w, w->battr, mb);
-  dfree(w, w->wslot, mb * 16);
-  dfree(w, w->bcode, mb);
-  dfree(w, w->rep, mb * 512);
-  dfree(w, w->attr, mv * 2);
-  dfree(w, w->plane, mv);
+  size_t np = (size_t)w->probe_off[Nc];
+  carve(cv, &w->hdr, HDR_WORDS);
+  carve(cv, &w->ptr, NLper);
+  carve(cv, &w->occ, mb * 16);
+  carve(cv, &w->battr, mb);
+  carve(cv, &w->wslot, mb * 16);
+  carve(cv, &w->bcode, mb);
+  carve(cv, &w->rep, mb * 512);
+  carve(cv, &w->attr, mv * 2);
+  carve(cv, &w->plane, mv);
+  carve(cv, &w->rad, rad_floats(w));
+  carve(cv, &w->hits, (size_t)w->ray_off[Nc], (rp->flags & RTGI_RC_RECORD_HITS) != 0);
+  for (int i = 0; i < sh_buffers(rp); i++) carve(cv, &w->shb[i], sh_floats_of(w));
+  carve(cv, &w->dirs, (size_t)w->dir_off[Nc] + BN_RAYS);
+  carve(cv, &w->avg, (size_t)w->avg_off[Nc] * 4);
+  carve(cv, &w->sunvis, mv, (rp->flags & RTGI_RC_SUN_TRACE) != 0);
+  carve(cv, &w->bounce, mv * 4, (rp->flags & RTGI_RC_BOUNCE) != 0);
+  for (int i = 0; i < 2; i++) carve(cv, &w->bnvol[i], BN_WORDS, (rp->flags & RTGI_RC_BOUNCE) != 0);
+  carve(cv, &w->vslot, mv, (rp->flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) != 0);
+  carve(cv, &w->ppos, np * 4, (rp->flags & RTGI_RC_RELOCATE) != 0);
+  carve(cv, &w->pnrm, np * 4, (rp->flags & RTGI_RC_RELOCATE) != 0);
+  carve(cv, &w->pvis, np, (rp->flags & RTGI_RC_RELOCATE) != 0);
+  carve(cv, &w->stats, 3 * RTGI_MAX_CASCADES, (rp->flags & RTGI_RC_STATS) != 0);
+}
+
+void free_all(rtgi_world* w) {
   dfree(w, w->big_vox, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->big_mark, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->tri, (size_t)w->cap_tris * 4);
   dfree(w, w->pos, (size_t)w->cap_verts * 3);
   dfree(w, w->nrm, (size_t)w->cap_verts * 3);
   dfree(w, w->mat, (size_t)w->cap_mat * 4);
-  dfree(w, w->rad, rad_floats(w));
-  dfree(w, w->hits, (w->rp.flags & RTGI_RC_RECORD_HITS) ? (size_t)w->ray_off[w->N] : 0);
-  for (int i = 0; i < 2; i++) dfree(w, w->shb[i], i < sh_buffers(&w->rp) ? sh_floats_of(w) : 0);
-  dfree(w, w->dirs, (size_t)w->dir_off[w->N] + BN_RAYS);
-  dfree(w, w->avg, (size_t)w->avg_off[w->N] * 4);
-  dfree(w, w->stats, (w->rp.flags & RTGI_RC_STATS) ? 3 * RTGI_MAX_CASCADES : 0);
-  size_t np = (w->rp.flags & RTGI_RC_RELOCATE) ? (size_t)w->probe_off[w->N] : 0;
-  dfree(w, w->ppos, np * 4);
-  dfree(w, w->pnrm, np * 4);
-  dfree(w, w->pvis, np);
-  dfree(w, w->sunvis, (w->rp.flags & RTGI_RC_SUN_TRACE) ? mv : 0);
-  dfree(w, w->bounce, (w->rp.flags & RTGI_RC_BOUNCE) ? mv * 4 : 0);
-  for (int i = 0; i < 2; i++) dfree(w, w->bnvol[i], (w->rp.flags & RTGI_RC_BOUNCE) ? BN_WORDS : 0);
-  dfree(w, w->vslot, (w->rp.flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) ? mv : 0);
   dyn_free(w);
+  if (w->arena) {
+    (void)hipFree(w->arena);
+    w->bytes -= w->arena_bytes;
+    w->arena = nullptr;
+  }
 }
 
 }  // namespace
@@ -493,18 +518,9 @@ uint64_t world_bytes(const rtgi_vox_params* vp, const rtgi_rc_params* rp, uint32
                      uint32_t nmat) {
   rtgi_world w;
   if (world_layout(&w, vp, rp)) return 0;
-  size_t NLper = (size_t)w.NL * w.per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
-  uint64_t b = nbytes<uint32_t>(HDR_WORDS) + nbytes<uint32_t>(NLper) + nbytes<uint32_t>(mb * 16) +
-               nbytes<uint32_t>(mb) * 2 + nbytes<uint32_t>(mb * 16) + nbytes<uint32_t>(mb * 512) + nbytes<uint32_t>(mv * 2) + nbytes<float>(mv) +
-               nbytes<float>(rad_floats(&w)) + (uint64_t)sh_buffers(rp) * nbytes<float>(sh_floats_of(&w)) +
-               nbytes<DirEntry>((size_t)w.dir_off[w.N] + BN_RAYS) + nbytes<float>((size_t)w.avg_off[w.N] * 4);
-  if (rp->flags & RTGI_RC_RECORD_HITS) b += nbytes<uint32_t>((size_t)w.ray_off[w.N]);
-  if (rp->flags & RTGI_RC_STATS) b += nbytes<unsigned long long>(3 * RTGI_MAX_CASCADES);
-  if (rp->flags & RTGI_RC_RELOCATE)
-    b += 2 * nbytes<float>((size_t)w.probe_off[w.N] * 4) + nbytes<uint32_t>((size_t)w.probe_off[w.N]);
-  if (rp->flags & RTGI_RC_SUN_TRACE) b += nbytes<float>(mv);
-  if (rp->flags & RTGI_RC_BOUNCE) b += nbytes<uint32_t>(mv * 4) + 2 * nbytes<uint32_t>(BN_WORDS);
-  if (rp->flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE)) b += nbytes<uint32_t>(mv);
+  Carver cv;
+  carve_fixed(&w, cv);
+  uint64_t b = cv.off;
   if (ntris) b += nbytes<uint32_t>((size_t)ntris * 4) + 2 * nbytes<uint32_t>((size_t)w.NL * big_cap_for(ntris));
   if (nverts) b += 2 * nbytes<float>((size_t)nverts * 3);
   if (nmat) b += nbytes<float>((size_t)nmat * 4);
@@ -537,39 +553,34 @@ extern "C" int rtgi_world_create(const rtgi_vox_params* vp, const rtgi_rc_params
   (void)hipGetDevice(&w->device);
   world_layout(w, vp, rp);
   int Nc = w->N;
-  uint64_t off = w->ray_off[Nc];
-  rtgi_oct_solid_angles(4, 1 << 20, w->omega0);
+  {
+    // octSolidAngles(4) is a constant (a 2^20-point count, about 50 ms of host time): computed once per process.
+    static const std::vector<float> omega0 = [] {
+      std::vector<float> o(16);
+      rtgi_oct_solid_angles(4, 1 << 20, o.data());
+      return o;
+    }();
+    std::memcpy(w->omega0, omega0.data(), sizeof(w->omega0));
+  }
 
   int e = RTGI_OK;
-  size_t NLper = (size_t)w->NL * w->per, mb = (size_t)vp->max_bricks, mv = (size_t)vp->max_voxels;
-  if (!e) e = dalloc(w, &w->hdr, HDR_WORDS);
-  if (!e) e = dalloc(w, &w->ptr, NLper);
-  if (!e) e = dalloc(w, &w->occ, mb * 16);
-  if (!e) e = dalloc(w, &w->battr, mb);
-  if (!e) e = dalloc(w, &w->wslot, mb * 16);
-  if (!e) e = dalloc(w, &w->bcode, mb);
-  if (!e) e = dalloc(w, &w->rep, mb * 512);
-  if (!e) e = dalloc(w, &w->attr, mv * 2);
-  if (!e) e = dalloc(w, &w->plane, mv);
-  if (!e) e = dalloc(w, &w->rad, rad_floats(w));
-  if (!e && (rp->flags & RTGI_RC_RECORD_HITS)) e = dalloc(w, &w->hits, (size_t)off);
-  for (int i = 0; i < sh_buffers(rp) && !e; i++) e = dalloc(w, &w->shb[i], sh_floats_of(w));
-  if (!e) e = dalloc(w, &w->dirs, (size_t)w->dir_off[Nc] + BN_RAYS);
-  if (!e) e = dalloc(w, &w->avg, (size_t)w->avg_off[Nc] * 4);
-  if (!e && (rp->flags & RTGI_RC_SUN_TRACE)) e = dalloc(w, &w->sunvis, mv);
-  if (!e && (rp->flags & RTGI_RC_BOUNCE)) e = dalloc(w, &w->bounce, mv * 4);
-  for (int i = 0; i < 2 && !e && (rp->flags & RTGI_RC_BOUNCE); i++) e = dalloc(w, &w->bnvol[i], BN_WORDS);
-  if (!e && (rp->flags & (RTGI_RC_SUN_TRACE | RTGI_RC_BOUNCE))) e = dalloc(w, &w->vslot, mv);
-  if (!e && (rp->flags & RTGI_RC_RELOCATE)) {
-    size_t np = (size_t)w->probe_off[Nc];
-    if (!e) e = dalloc(w, &w->ppos, np * 4);
-    if (!e) e = dalloc(w, &w->pnrm, np * 4);
-    if (!e) e = dalloc(w, &w->pvis, np);
+  {
+    Carver cv;
+    carve_fixed(w, cv);
+    hipError_t he = hipMalloc((void**)&w->arena, cv.off);
+    if (he != hipSuccess) {
+      w->arena = nullptr;
+      e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc(%zu bytes): %s", cv.off, hipGetErrorString(he));
+    } else {
+      w->arena_bytes = cv.off;
+      w->bytes += cv.off;
+      cv = Carver{};
+      cv.base = w->arena;
+      carve_fixed(w, cv);
+    }
   }
-  if (!e && (rp->flags & RTGI_RC_STATS)) {
-    e = dalloc(w, &w->stats, 3 * RTGI_MAX_CASCADES);
-    if (!e && hipMemset(w->stats, 0, 3 * RTGI_MAX_CASCADES * 8) != hipSuccess) e = rtgi_fail(RTGI_E_HIP, "hipMemset");
-  }
+  if (!e && w->stats && hipMemset(w->stats, 0, 3 * RTGI_MAX_CASCADES * 8) != hipSuccess)
+    e = rtgi_fail(RTGI_E_HIP, "hipMemset");
   if (!e) {
     std::vector<DirEntry> tab((size_t)w->dir_off[Nc] + BN_RAYS);
     for (int c = 0; c < Nc; c++)

@@ -28,11 +28,32 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Host copy of a registered world's scene in pinned memory: an admission uploads it by DMA at full link speed.
+template <class T>
+struct PinnedVec {
+  T* p = nullptr;
+  PinnedVec() = default;
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  ~PinnedVec() {
+    if (p) (void)hipHostFree(p);
+  }
+  bool assign(const T* src, size_t n) {
+    if (hipHostMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    if (n) std::memcpy(p, src, n * sizeof(T));
+    return true;
+  }
+  T* data() { return p; }
+};
+
 struct SrvWorld {
   rtgi_vox_params vp{};
   rtgi_rc_params rp{};
-  std::vector<float> pos, nrm, mat;
-  std::vector<uint32_t> tri;
+  PinnedVec<float> pos, nrm, mat;
+  PinnedVec<uint32_t> tri;
   uint32_t nverts = 0, ntris = 0, nmat = 0;
   uint64_t bytes = 0;     // device bytes once resident
   rtgi_world* w = nullptr;  // resident world, or null
@@ -404,10 +425,9 @@ extern "C" int rtgi_server_add_world(rtgi_server* s, const rtgi_vox_params* vp, 
   if (!w) return rtgi_fail(RTGI_E_NOMEM, "out of host memory");
   w->vp = *vp;
   w->rp = *rp;
-  w->pos.assign(pos, pos + (size_t)nverts * 3);
-  w->nrm.assign(nrm, nrm + (size_t)nverts * 3);
-  w->tri.assign(tri, tri + (size_t)ntris * 4);
-  w->mat.assign(materials, materials + (size_t)nmat * 4);
+  if (!w->pos.assign(pos, (size_t)nverts * 3) || !w->nrm.assign(nrm, (size_t)nverts * 3) ||
+      !w->tri.assign(tri, (size_t)ntris * 4) || !w->mat.assign(materials, (size_t)nmat * 4))
+    return rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc for the scene copy failed");
   w->nverts = nverts;
   w->ntris = ntris;
   w->nmat = nmat;

@@ -71,6 +71,8 @@ struct rtgi_world {
   uint32_t bn_frame = 0;                  // volume updates so far
   uint32_t bn_dir_off = 0;                // first bounce ray direction in dirs
   uint64_t bytes = 0;
+  char* arena = nullptr;     // one device allocation holding every fixed buffer above (scene and layer excluded)
+  size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
   hipEvent_t ev[5] = {};
