@@ -86,6 +86,8 @@ _SIG = {
     "orc_trace_f64": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
                                 C.c_double, C.c_int]),
     "orc_merge_sh_f64": (C.c_int, [C.POINTER(OrcRc), _P, _P]),
+    "orc_trace_f64_take": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
+                                     C.c_double, C.c_int]),
     "orc_tri_box": (C.c_int, [_P, _P, _P, _P, _P]),
     "orc_closest_bary": (None, [_P, _P, _P, _P, _P]),
     "orc_pack4x8unorm": (C.c_uint32, [_P]),
@@ -412,6 +414,18 @@ class Oracle:
         lib().orc_trace_f64(C.byref(self.vp), _p(self.buf), C.byref(self.rc), r0, r1, _p(rgbt), _p(hits), _p(tie),
                             eps, threads)
         return rgbt, hits, tie
+
+    def trace_f64_take(self, hits32, eps=1e-6, threads=0):
+        """The FP64 trace of every ray, taking the f32 walk's first-hit voxel (hits32, as trace() returns them) on tie
+        rays and on rays whose FP64 first hit differs: rgbt (double) and the rays taken so."""
+        n = sum(self.cascade_rays())
+        hits32 = np.ascontiguousarray(hits32, np.uint32)
+        assert hits32.shape[0] == n
+        rgbt = np.zeros((n, 4), np.float64)
+        taken = np.zeros(n, np.uint8)
+        assert lib().orc_trace_f64_take(C.byref(self.vp), _p(self.buf), C.byref(self.rc), 0, n, _p(hits32), _p(rgbt),
+                                        _p(taken), eps, threads) == 0
+        return rgbt, taken
 
     def trace_rays(self, c, o, d, threads=0):
         """Rays from origins o along unit directions d (n x 3 float32) traced as cascade c's rays (f32)."""

@@ -2284,6 +2284,68 @@ int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint6
   return 0;
 }
 
+/* The radiance in double of a ray that enters the voxel of hit id `id` (level << 29 | window cell): entered where the
+ * ray crosses into that voxel's box (the larger of the three slab entries), shaded in double; a miss (NO_HIT) returns
+ * (0, 0, 0, 1). */
+static void shadeIdD(const View* W, const orc_rc* rc, d3 o, d3 d, uint32_t id, double out[4]) {
+  out[0] = out[1] = out[2] = 0;
+  out[3] = 1;
+  if (id == 0xffffffffu) return;
+  const orc_vox* p = W->p;
+  uint32_t L = id >> 29, idx = id & ((1u << 29) - 1u), E = (uint32_t)p->G * 8u;
+  iv3 cell = I3((int)(idx % E), (int)((idx / E) % E), (int)(idx / (E * E)));
+  double v = p->v0 * (double)(1u << L);
+  double lo[3] = {(p->origin[L][0] * 8.0 + cell.x) * v, (p->origin[L][1] * 8.0 + cell.y) * v,
+                  (p->origin[L][2] * 8.0 + cell.z) * v};
+  double ov[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z}, t = -1e300;
+  for (int a = 0; a < 3; a++) {
+    double dd = fabs(dv[a]) < 1e-9 ? 1e-9 : dv[a];
+    double ta = (lo[a] - ov[a]) / dd, tb = (lo[a] + v - ov[a]) / dd;
+    double tn = ta < tb ? ta : tb;
+    if (tn > t) t = tn;
+  }
+  uint32_t br, bit, lev;
+  iv3 cc;
+  uint32_t ai = voxAt(W, L, cell, &br, &bit, &lev, &cc) ? voxRank(W, br, bit) : V_EMPTY;
+  d3 r = shadeVoxelD(W, rc, ai, d, D3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t));
+  out[0] = r.x;
+  out[1] = r.y;
+  out[2] = r.z;
+  out[3] = 0;
+}
+
+int orc_trace_f64_take(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1,
+                       const uint32_t* hits32, double* rgbt, uint8_t* taken, double eps, int threads) {
+  View W;
+  makeView(&W, p, vox);
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads)
+#endif
+  for (int64_t q = (int64_t)r0; q < (int64_t)r1; q++) {
+    uint64_t r = 0;
+    int c = cascadeOf(rc, (uint64_t)q, &r);
+    int res = resOf(rc, c), n = probesOf(rc, c);
+    uint64_t dirs = (uint64_t)res * res, pi = r / dirs;
+    int di = (int)(r % dirs);
+    double s = (double)rc->s0 * (double)(1u << c);
+    d3 o = D3(((rc->G0[0] >> c) + (int)(pi % n)) * s, ((rc->G0[1] >> c) + (int)((pi / n) % n)) * s,
+              ((rc->G0[2] >> c) + (int)(pi / ((uint64_t)n * n))) * s);
+    d3 d = dirOfD(di, res);
+    double out[4];
+    uint32_t h;
+    uint8_t tt;
+    traceRayD(&W, rc, c, o, d, eps, out, &h, &tt);
+    size_t i = (size_t)(q - (int64_t)r0);
+    uint8_t take = (uint8_t)(tt || h != hits32[i]);
+    if (take) shadeIdD(&W, rc, o, d, hits32[i], out);
+    memcpy(rgbt + i * 4, out, 32);
+    if (taken) taken[i] = take;
+  }
+  (void)threads;
+  return 0;
+}
+
 int orc_trace_rays_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const double* o,
                        const double* d, uint64_t n, double* rgbt, uint32_t* hits, uint8_t* tie, double eps,
                        int threads) {

@@ -166,6 +166,12 @@ int orc_gather(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, const fl
  * (its first-hit voxel is then not fixed by exact arithmetic's neighbourhood). Any output may be NULL. */
 int orc_trace_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, double* rgbt,
                   uint32_t* hits, uint8_t* tie, double eps, int threads);
+/* orc_trace_f64 that takes the f32 walk's first-hit voxel (hits32, as orc_trace returns it for the same rays) on every
+ * ray whose walks may disagree: tie rays, and rays whose FP64 first hit differs. Those rays are shaded in double at
+ * the f32 walk's voxel (entered where the ray crosses into its box; a miss gives (0, 0, 0, 1)), so the result's error
+ * against the f32 path is the arithmetic's alone. taken (may be NULL): 1 for such rays. */
+int orc_trace_f64_take(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1,
+                       const uint32_t* hits32, double* rgbt, uint8_t* taken, double eps, int threads);
 /* The same for n arbitrary rays of cascade c (origins and directions in double). */
 int orc_trace_rays_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, const double* o,
                        const double* d, uint64_t n, double* rgbt, uint32_t* hits, uint8_t* tie, double eps,

@@ -392,6 +392,19 @@ def test_tuning_keeps_results(require_gpu, case):
         w.destroy()
 
 
+def test_tilted_triangle_attributes(require_gpu):
+    """The tilted triangle of the resolveAttr closed form (tests/test_oracle.py): occupancy, normals, plane offsets
+    and albedo bit for bit against the oracle, whose values that test pins."""
+    from test_oracle import tilted_triangle_world
+    scene, vox, rc, frame = tilted_triangle_world()
+    say("case: one tilted triangle with three vertex normals")
+    w = run_world(scene, vox, rc, frame)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    check_clipmap(w, o, vox)
+    w.destroy()
+
+
 def test_readback_after_set_tuning(require_gpu):
     """The probe order can change between an update and its readbacks: cascades, hit ids, SH, relocated probes and the
     gather keep reading that update's values (the buffers remember the order they were written in)."""

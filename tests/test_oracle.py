@@ -1042,3 +1042,84 @@ def test_history_hash_limit_on_dense_grids():
             assert h64.misses == ref_misses
         else:
             assert 0 < h64.misses < 64  # a handful of 262,144 probes per update
+
+
+def test_fp64_trace_taking_the_f32_voxel():
+    """orc_trace_f64_take: rays whose walks agree keep the FP64 walk's result; the others are shaded in double at the
+    f32 walk's voxel. The irradiance it gives then differs from the f32 path by the arithmetic alone (well below
+    1e-4 relative), while the plain FP64 walk differs on tie rays by whole voxels."""
+    scene, vox, rc, frame = rtgi.config("atrium")
+    rc = dataclasses.replace(rc, probes0=8, cascades=2)
+    frame = dataclasses.replace(frame, eye=(0.3, 4.0, -0.2))
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    rgba, hits = o.trace()
+    rgbt, hit64, tie = o.trace_f64(eps=1e-6)
+    take, taken = o.trace_f64_take(hits, eps=1e-6)
+    keep = taken == 0
+    assert np.array_equal(take[keep], rgbt[keep])
+    assert np.all(taken[(tie != 0) | (hit64 != hits)] == 1)
+    # where the two walks chose the same voxel the recomputed entry changes only the point-light terms' last bits
+    same = (taken == 1) & (hit64 == hits)
+    np.testing.assert_allclose(take[same], rgbt[same], rtol=1e-9, atol=1e-12)
+    # where they differ, the taken ray follows the f32 walk: a miss there is (0, 0, 0, 1)
+    miss = (taken == 1) & (hits == 0xFFFFFFFF)
+    assert np.all(take[miss] == np.array([0, 0, 0, 1.0]))
+    _, sh32 = o.merge_sh(rgba)
+    _, sh64 = o.merge_sh_f64(take)
+    rel = np.abs(sh32 - sh64).max(axis=1) / np.maximum(np.abs(sh64).max(axis=1), 1e-6)
+    assert rel.max() < 1e-5, rel.max()
+    assert (hit64 != hits).sum() > 0  # the case is not vacuous: some rays do pick another voxel in double
+
+
+def tilted_triangle_world():
+    """One large tilted triangle with three different vertex normals over a one-level 32^3 window."""
+    a = np.array([-9.0, -3.0, -8.0])
+    b = np.array([10.0, -1.0, -6.0])
+    c = np.array([-2.0, 5.0, 9.0])
+    na, nb, nc = (np.array(v) / np.linalg.norm(v) for v in ([0.2, 0.9, -0.3], [-0.4, 0.8, 0.1], [0.1, 0.7, 0.6]))
+    scene = rtgi.Scene(pos=np.array([a, b, c], np.float32), nrm=np.array([na, nb, nc], np.float32),
+                       tri=np.array([[0, 1, 2, 0]], np.uint32), materials=np.array([[0.3, 0.6, 0.9, 1.0]], np.float32))
+    vox = rtgi.VoxParams(v0=0.25, levels=1, G=4, maxBricks=64, maxVoxels=1 << 15)
+    return scene, vox, rtgi.RcParams(cascades=1, probes0=4, s0=1.0), rtgi.Frame(eye=(0.0, 0.0, 0.0))
+
+
+def test_resolve_attr_tilted_triangle_closed_form():
+    """resolveAttr (voxel.ts:590-620) on one large tilted triangle with three different vertex normals, worked in
+    double: for every level-0 voxel whose centre projects inside the triangle, the closest point is the centre's
+    projection onto the plane, the stored normal is octEnc(normalize(sum of the barycentric-weighted vertex
+    normals)) and the plane offset is that normal's dot with (projection - centre). The voxel's albedo is the
+    material's, packed as pack4x8unorm."""
+    scene, vox, rc, frame = tilted_triangle_world()
+    a, b, c = scene.pos.astype(np.float64)
+    na, nb, nc = scene.nrm.astype(np.float64)
+    o = Oracle(vox, rc, frame)
+    o.build(scene)
+    cm = o.clipmap(0)
+    E = vox.G * 8
+    org = orc.origins(vox.v0, 1, vox.G, frame.eye)[0] * 8
+    A, B, Cc = (np.asarray(x, np.float32).astype(np.float64) for x in (a, b, c))
+    NA, NB, NC = (np.asarray(x, np.float32).astype(np.float64) for x in (na, nb, nc))
+    nf = np.cross(B - A, Cc - A)
+    nf /= np.linalg.norm(nf)
+    T = np.stack([B - A, Cc - A], 1)
+    idx = np.nonzero(np.unpackbits(cm["occ"].view(np.uint8), bitorder="little")[:E ** 3])[0]
+    checked = 0
+    for i in idx:
+        cell = np.array([i % E, (i // E) % E, i // (E * E)])
+        p = (org + cell + 0.5) * vox.v0
+        q = p - np.dot(p - A, nf) * nf
+        st = np.linalg.lstsq(T, q - A, rcond=None)[0]
+        w = np.array([1 - st.sum(), st[0], st[1]])
+        if w.min() < 0.02:
+            continue  # the closest point is on an edge: not the plane projection
+        n = w[0] * NA + w[1] * NB + w[2] * NC
+        n /= np.linalg.norm(n)
+        code = int(cm["normal"][i])
+        sx, sy = [max(np.int16(np.uint16(v)) / 32767.0, -1.0) for v in (code & 0xFFFF, code >> 16)]
+        got = oct_decode(sx * 0.5 + 0.5, sy * 0.5 + 0.5)
+        assert np.degrees(np.arccos(min(1.0, float(np.dot(got, n))))) < 0.01, (i, got, n)
+        assert abs(float(cm["plane"][i]) - np.dot(n, q - p)) < 2e-5, (i, cm["plane"][i], np.dot(n, q - p))
+        assert int(cm["albedo"][i]) == orc.pack4x8unorm([0.3, 0.6, 0.9, 0.0])
+        checked += 1
+    assert checked > 500
