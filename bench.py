@@ -1,8 +1,9 @@
 """GI-update benchmark on one MI355X.
 
 Workload: W resident city-block worlds (about 500k triangles each, 4 x 128^3 clipmap at 0.25 m, 64^3 c0 probes,
-5 cascades, FP32), seeds 1..W. One update = voxel rebuild from every triangle, probe-ray trace of all cascades,
-cascade merge and SH projection of one world.
+5 cascades, FP32), seeds 1..W, whose 64 buildings each turn and move every update (--static: they stay put). One
+update = the buildings posed at the next frame (poses sent per world, rest meshes posed on the device), voxel rebuild
+from every triangle, probe-ray trace of all cascades, cascade merge and SH projection of one world.
 
 Two paths are timed on the same worlds:
   plain   one world at a time: its update is enqueued, then waited for (the single-instance path);
@@ -268,11 +269,35 @@ def serve_binding_bench(a):
             "peak_resident_gib": round(st["peak_resident_bytes"] / 2 ** 30, 3)}
 
 
-def width_bench(a, worlds, widths=(8, 16, 32, 64)):
+class Motion:
+    """The worlds' moving objects (every city building) re-posed per update, as the reference's runner poses each
+    frame (src/core/rcrun.ts:194-217): the poses of every frame are evaluated before timing (the runner also poses
+    ahead of its timed frames, rcrun.ts:200-203), and each update sends its world's poses (rtgi_world_set_pose); the
+    device then poses the rest mesh before the voxel build."""
+
+    def __init__(self, worlds, scenes, frames):
+        for w, sc in zip(worlds, scenes):
+            w.set_scene_objects(sc)
+        self.poses = {id(w): [sc.poses_at(f) for f in range(frames)] for w, sc in zip(worlds, scenes)}
+        self.worlds = list(worlds)
+        self.frames = frames
+        self.frame = 0
+        self.objects = int(sum(len(sc.objects) for sc in scenes))
+
+    def advance(self, worlds=None):
+        """Sends the next frame's poses to the given worlds (all by default)."""
+        f = self.frame % self.frames
+        self.frame += 1
+        for w in self.worlds if worlds is None else worlds:
+            w.set_pose(self.poses[id(w)][f])
+
+
+def width_bench(a, worlds, widths=(8, 16, 32, 64), motion=None):
     """Latency against throughput at the headline configuration: the resident worlds updated in batches of width W,
     one batch at a time (a batch is launched when the previous one has completed, as a server with one slot launches
     them). An update's latency is its batch's time from launch to completion on the otherwise idle GPU (host clock:
-    enqueue of the graph replay to the end of the wait); throughput counts every update over the whole run."""
+    from sending the batch's poses (motion) and enqueueing the graph replay to the end of the wait); throughput counts
+    every update over the whole run."""
     import rtgi
 
     out = []
@@ -288,6 +313,8 @@ def width_bench(a, worlds, widths=(8, 16, 32, 64)):
         for _ in range(a.steps):
             for b in batches:
                 t0 = time.perf_counter()
+                if motion:
+                    motion.advance(b.worlds)
                 b.encode()
                 b.sync()
                 lat.extend([time.perf_counter() - t0] * len(b.worlds))
@@ -436,6 +463,8 @@ def main():
                     help="run with relocation, traced sun visibility and multi-bounce on (0: skip it)")
     ap.add_argument("--dyn-worlds", type=int, default=16, help="atrium worlds of the dynamic-layer run (0: skip)")
     ap.add_argument("--no-widths", dest="widths", action="store_false", help="skip the latency-vs-width curve")
+    ap.add_argument("--static", dest="motion", action="store_false",
+                    help="headline without motion (the buildings stay at the pose of frame 0)")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="mean Poisson request rate (1/s)")
     ap.add_argument("--serve-worlds", type=int, default=64)
@@ -481,23 +510,32 @@ def main():
         worlds.append(w)
         if k == 0:
             first = (sc, vox, rc, frame)
+    # Moving buildings: every world's objects re-posed on the device by each update (plain and batched alike).
+    motion = Motion(worlds, scenes, 64) if a.motion else None
     lay = worlds[0].layout()
     rays = lay["total_rays"]
     log(f"setup: {a.worlds} worlds of '{a.config}' in {time.perf_counter() - t0:.1f} s, {first[0].ntris} triangles "
         f"in world 1, {rays} rays per update, {lay['device_bytes'] / 2 ** 20:.0f} MiB per world")
 
     # Plain path: one world at a time.
+    if motion:
+        motion.advance()
     for w in worlds:
         w.encode()
         w.sync()
     plain_lat = []
     t_start = time.perf_counter()
     for s in range(a.plain_steps):
+        f = motion.frame if motion else 0
         for w in worlds:
             t = time.perf_counter()
+            if motion:
+                w.set_pose(motion.poses[id(w)][f % motion.frames])
             w.encode()
             w.sync()
             plain_lat.append(time.perf_counter() - t)
+        if motion:
+            motion.frame += 1
     plain_value = a.plain_steps * len(worlds) / (time.perf_counter() - t_start)
     log(f"plain: {plain_value:.1f} updates/s")
 
@@ -529,6 +567,8 @@ def main():
     t_start = time.perf_counter()
     for s in range(a.steps):
         ts = time.perf_counter()
+        if motion:
+            motion.advance()
         for b in batches:
             b.encode()
         for b, g in zip(batches, groups):
@@ -573,7 +613,7 @@ def main():
 
     for b in batches:
         b.destroy()
-    widths = width_bench(a, worlds) if a.widths else None
+    widths = width_bench(a, worlds, motion=motion) if a.widths else None
     for w in worlds:
         w.destroy()
     full = (full_bench(a, scenes, first[1], first[2], first[3], worlds_tuning, bsz) if a.full_worlds > 0 else None)
@@ -609,7 +649,10 @@ def main():
                    "worlds": a.worlds, "batch": bsz, "tuning": worlds_tuning,
                    "triangles_world1": int(first[0].ntris),
                    "clipmap": "4 x 128^3 @ 0.25 m", "probes_c0": "64^3", "cascades": 5,
-                   "rays_per_update": int(rays)},
+                   "rays_per_update": int(rays),
+                   "motion": (f"{motion.objects} buildings ({motion.objects // len(worlds)} per world) turned and "
+                              f"moved every update: poses sent per update, rest meshes posed on the device"
+                              if motion else None)},
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
         "plain_fp32": {"value": round(plain_value, 3), "unit": "GI updates/s",
                        "latency_ms": {"p50": pct(plain_lat, 50), "p99": pct(plain_lat, 99)}},

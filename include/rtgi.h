@@ -178,6 +178,38 @@ int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const float* nrm, u
 /* Replaces the posed-vertex writeBuffer (src/core/rcrun.ts:213-214): vertices [first, first+count). */
 int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_t count, const float* pos, const float* nrm);
 
+/* ---- Moving objects, posed on the device (DynamicSpec and posedVertices, src/core/scenes.ts:41-53, 80-102) ---- */
+
+/* A rigid object: vertices [first_vertex, first_vertex + vertex_count) turning about the vertical axis through pivot
+ * (DynamicObject, scenes.ts:41; its height serves only the wave and bend deformations, which are not built). */
+typedef struct rtgi_object {
+  uint32_t first_vertex, vertex_count;
+  double pivot[3];
+} rtgi_object;
+
+/* Pose of one object (DynamicPose.poses[i], scenes.ts:47): c = cos(rotY) and s = sin(rotY) as the host evaluates
+ * them (posedVertices, scenes.ts:89), then the translation offset. */
+typedef struct rtgi_pose {
+  double c, s;
+  double offset[3];
+} rtgi_pose;
+
+/* Registers n moving objects (n = 0: none) with their rest mesh, which stays resident on the device: rest_pos and
+ * rest_nrm hold 3 floats per vertex of [objects[0].first_vertex, objects[n-1].first_vertex +
+ * objects[n-1].vertex_count), the scene.positions / scene.normals that posedVertices reads. Objects come in ascending
+ * vertex order without overlap, inside the uploaded scene; vertices between two objects are left alone. No pose
+ * applies until rtgi_world_set_pose; a new scene upload removes the objects. Synchronises; allocates. */
+int rtgi_world_set_objects(rtgi_world* w, const rtgi_object* objects, uint32_t n, const float* rest_pos,
+                           const float* rest_nrm);
+
+/* Poses of the n registered objects for the following updates: from then on every update begins by writing the
+ * objects' positions and normals from the rest mesh (posedVertices, scenes.ts:80-102: x = rest - pivot in double,
+ * pivot + (c x + s z, y, -s x + c z) + offset and the normal turned the same way, each value rounded to float once).
+ * Replaces posedVertices and the writeBuffer of positions and normals per frame (src/core/rcrun.ts:194-217); vertices
+ * written by rtgi_world_update_vertices inside an object are posed over. Asynchronous on the world's stream (the
+ * poses are copied at once); waits for a batched update of the world still in flight. */
+int rtgi_world_set_pose(rtgi_world* w, const rtgi_pose* poses);
+
 /* Replaces VoxWorld.update(eye) (src/core/voxel.ts:896, writeUniform 868-892) and the Frame uniform fields the path reads
  * (packFrame src/core/frame.ts:76-94: sunDir, sunE, sky). Also places the dense probe grid around the eye. */
 int rtgi_world_set_frame(rtgi_world* w, const rtgi_frame* f);
@@ -424,6 +456,13 @@ typedef struct rtgi_scene_params {
   double detail;       /* tessellation step scale (1: default triangle count) */
 } rtgi_scene_params;
 
+/* Motion schedule of a generated object (its DynamicSpec.at, scenes.ts:52): at frame f, rotY = rot0 + w f and
+ * offset = amp sin(wt f + ph). */
+typedef struct rtgi_motion {
+  double rot0, w, wt, ph;
+  double amp[3];
+} rtgi_motion;
+
 typedef struct rtgi_scene {
   uint32_t nverts, ntris, nmat;
   float* pos;        /* nverts * 3 */
@@ -435,10 +474,17 @@ typedef struct rtgi_scene {
   float sun_dir[3];
   uint32_t first_dynamic_tri;  /* the moving parts are triangles [first_dynamic_tri, ntris) (atrium: the 16 boxes;
                                 * city: ntris, every building moving, rebuilt with the rest) */
+  uint32_t nobj;               /* moving objects (atrium: the 16 boxes; city: the buildings), ascending, contiguous */
+  rtgi_object* objects;        /* nobj */
+  rtgi_motion* motion;         /* nobj: each object's schedule */
+  float* rest_pos;             /* the objects' vertices at rest, [objects[0].first_vertex, end of the last) x 3; pos and */
+  float* rest_nrm;             /* nrm hold them posed at the frame (rtgi_motion_pose_at, then posedVertices) */
 } rtgi_scene;
 
 int rtgi_scene_generate(const rtgi_scene_params* p, rtgi_scene** out);
 int rtgi_scene_free(rtgi_scene* s);
+/* Poses of n scheduled objects at a frame (c and s by the C library's cos and sin). */
+int rtgi_motion_pose_at(const rtgi_motion* m, uint32_t n, double frame, rtgi_pose* out);
 
 #ifdef __cplusplus
 }

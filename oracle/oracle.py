@@ -16,6 +16,8 @@ citations are in rtgi_oracle.c); the reference ships no golden vectors or tests 
   mergeWorld (rc.ts:681-708)         equal-weight parent mean = trilinear interpolation on the dense grid
   projectSH (rc.ts:745-774)          constant radiance -> only L0, = pi * 0.282095 * 4 pi * L; sum(omega) = 4 pi
   octSolidAngles (frame.ts:105-114)  sum = 4 pi, 8-fold symmetry of the 4x4 texels, library value agrees
+  posedVertices (scenes.ts:80-102)   exact rational evaluation on dyadic inputs (every double operation exact);
+                                     quarter turns and the identity in closed form; distances to the pivot kept
 """
 import ctypes as C
 import os
@@ -96,6 +98,7 @@ _SIG = {
     "orc_f2h": (C.c_uint16, [C.c_float]),
     "orc_h2f": (C.c_float, [C.c_uint16]),
     "orc_log2": (C.c_float, [C.c_float]),
+    "orc_pose": (None, [C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 _lib = None
 
@@ -215,6 +218,28 @@ def oct_enc(n):
 
 def f2h(x):
     return int(lib().orc_f2h(float(np.float32(x))))
+
+
+def pose(first, count, pivot, poses, rest_pos, rest_nrm, pos, nrm):
+    """posedVertices of objects (first, count, pivot (n, 3)) at poses (n, 5: c, s, offset) from the rest mesh of their
+    span; returns posed copies of the whole-scene arrays pos / nrm."""
+    first = np.ascontiguousarray(first, np.uint32)
+    count = np.ascontiguousarray(count, np.uint32)
+    pivot = np.ascontiguousarray(pivot, np.float64).reshape(-1, 3)
+    poses = np.ascontiguousarray(poses, np.float64).reshape(-1, 5)
+    rp = np.ascontiguousarray(rest_pos, np.float32)
+    rn = np.ascontiguousarray(rest_nrm, np.float32)
+    out_p = np.array(pos, np.float32, order="C")
+    out_n = np.array(nrm, np.float32, order="C")
+    lib().orc_pose(len(first), _p(first), _p(count), _p(pivot), _p(poses), _p(rp), _p(rn), _p(out_p), _p(out_n))
+    return out_p, out_n
+
+
+def pose_scene(scene, poses):
+    """pose() with a generated scene's objects and rest mesh, over its vertex arrays."""
+    o = scene.objects
+    return pose(o["first_vertex"], o["vertex_count"], o["pivot"], poses, scene.rest_pos, scene.rest_nrm, scene.pos,
+                scene.nrm)
 
 
 def oct_solid_angles(res, samples=1 << 20):

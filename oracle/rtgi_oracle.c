@@ -2431,3 +2431,29 @@ int orc_merge_sh_f64(const orc_rc* rc, double* rgbt, double* sh) {
   }
   return 0;
 }
+
+/* posedVertices, scenes.ts:80-102: per vertex x = rest - pivot in double (JavaScript numbers), the turn about +y
+ * (c, s) and the offset, each component stored into a Float32Array (rounded once). Deformations (scenes.ts:56-78)
+ * are not restated. */
+void orc_pose(uint32_t n, const uint32_t* first, const uint32_t* count, const double* pivot, const double* pose,
+              const float* rest_pos, const float* rest_nrm, float* pos, float* nrm) {
+  if (!n) return;
+  const uint32_t f0 = first[0];
+  for (uint32_t i = 0; i < n; i++) {
+    const double* pv = pivot + 3 * i;
+    const double c = pose[5 * i], s = pose[5 * i + 1];
+    const double* off = pose + 5 * i + 2;
+    for (uint32_t v = first[i]; v < first[i] + count[i]; v++) {
+      const float* r = rest_pos + (size_t)(v - f0) * 3;
+      const float* rn = rest_nrm + (size_t)(v - f0) * 3;
+      double x = r[0] - pv[0], y = r[1] - pv[1], z = r[2] - pv[2];
+      double nx = rn[0], ny = rn[1], nz = rn[2];
+      pos[(size_t)v * 3] = (float)(pv[0] + c * x + s * z + off[0]);
+      pos[(size_t)v * 3 + 1] = (float)(pv[1] + y + off[1]);
+      pos[(size_t)v * 3 + 2] = (float)(pv[2] - s * x + c * z + off[2]);
+      nrm[(size_t)v * 3] = (float)(c * nx + s * nz);
+      nrm[(size_t)v * 3 + 1] = (float)ny;
+      nrm[(size_t)v * 3 + 2] = (float)(-s * nx + c * nz);
+    }
+  }
+}

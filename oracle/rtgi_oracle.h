@@ -179,6 +179,12 @@ int orc_trace_rays_f64(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, 
 /* FP64 merge and SH on a full double rgbt array (4 doubles per ray). */
 int orc_merge_sh_f64(const orc_rc* rc, double* rgbt, double* sh);
 
+/* posedVertices (src/core/scenes.ts:80-102) of n rigid objects: object i covers vertices [first[i], first[i] +
+ * count[i]) with pivot pivot[3i..3i+2] and pose pose[5i..5i+4] = (c, s, offset xyz); rest_pos / rest_nrm hold the
+ * vertices from first[0] on (3 floats each). Writes the posed vertices into the whole-scene arrays pos / nrm. */
+void orc_pose(uint32_t n, const uint32_t* first, const uint32_t* count, const double* pivot, const double* pose,
+              const float* rest_pos, const float* rest_nrm, float* pos, float* nrm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,6 +134,11 @@ VoxK vox_args(const rtgi_world* w) {
   k.ntris = std::min(w->ntris, w->dyn_first);  // static triangles only (voxel.ts:822)
   k.lds_marks = 0;
   k.lds_marks = vox_mark_lds_bytes(k) > 0;
+  k.pose = w->d_pose;
+  k.rest = w->rest;
+  k.npose = w->pose_set && w->pose_span ? (uint32_t)w->objs.size() : 0u;
+  k.pose_first = w->pose_first;
+  k.pose_span = w->pose_span;
   k.stats = w->stats;
   if (w->dyn_hdr) {
     k.dyn.hdr = w->dyn_hdr;
@@ -306,7 +311,20 @@ void carve_fixed(rtgi_world* w, Carver& cv) {
   carve(cv, &w->stats, 3 * RTGI_MAX_CASCADES, (rp->flags & RTGI_RC_STATS) != 0);
 }
 
+void objects_free(rtgi_world* w) {
+  dfree(w, w->rest, (size_t)w->pose_span * 6);
+  dfree(w, w->d_pose, w->objs.size());
+  for (int i = 0; i < 2; i++) {
+    if (w->h_pose[i]) (void)hipHostFree(w->h_pose[i]);
+    w->h_pose[i] = nullptr;
+  }
+  w->objs.clear();
+  w->pose_first = w->pose_span = 0;
+  w->pose_set = false;
+}
+
 void free_all(rtgi_world* w) {
+  objects_free(w);
   dfree(w, w->big_vox, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->big_mark, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->tri, (size_t)w->cap_tris * 4);
@@ -355,7 +373,8 @@ namespace {
 int stage_launch(rtgi_world* w, uint32_t stage, hipStream_t s) {
   int e = 0;
   if (stage == RTGI_STAGE_VOX) {
-    e = launch_vox_build(vox_args(w), s);
+    e = launch_pose(vox_args(w), s);
+    if (!e) e = launch_vox_build(vox_args(w), s);
     if (!e && w->dyn_hdr) e = launch_dyn_build(vox_args(w), s);
     if (!e && w->dyn_hdr) e = launch_dyn_splat(trace_args(w), s);
   } else if (stage == RTGI_STAGE_TRACE && w->tuning.fused == 1) {
@@ -620,6 +639,7 @@ extern "C" int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const fl
   if ((uint64_t)ntris >= 0xffffffffull) return rtgi_fail(RTGI_E_INVALID, "too many triangles");
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(world_wait(w));
+  objects_free(w);  // the objects belonged to the previous scene
   int e = RTGI_OK;
   if (ntris > w->cap_tris) {
     dfree(w, w->tri, (size_t)w->cap_tris * 4);
@@ -675,6 +695,79 @@ extern "C" int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_
   return RTGI_OK;
 }
 
+extern "C" int rtgi_world_set_objects(rtgi_world* w, const rtgi_object* objects, uint32_t n, const float* rest_pos,
+                                      const float* rest_nrm) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (n && (!objects || !rest_pos || !rest_nrm)) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_set_objects: null array");
+  if (n && !w->scene_set) return rtgi_fail(RTGI_E_STATE, "no scene uploaded");
+  uint64_t end = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const rtgi_object& o = objects[i];
+    if (i && o.first_vertex < end) return rtgi_fail(RTGI_E_INVALID, "object %u overlaps or precedes object %u", i, i - 1);
+    end = (uint64_t)o.first_vertex + o.vertex_count;
+    if (end > w->nverts) return rtgi_fail(RTGI_E_INVALID, "object %u: vertices out of range", i);
+    for (int a = 0; a < 3; a++)
+      if (!std::isfinite(o.pivot[a])) return rtgi_fail(RTGI_E_INVALID, "object %u: pivot not finite", i);
+  }
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  objects_free(w);
+  if (!n) return RTGI_OK;
+  const uint32_t first = objects[0].first_vertex, span = (uint32_t)(end - first);
+  int e = dalloc(w, &w->rest, (size_t)span * 6);
+  if (!e) e = dalloc(w, &w->d_pose, n);
+  for (int i = 0; i < 2 && !e; i++) {
+    if (hipHostMalloc((void**)&w->h_pose[i], sizeof(PoseRec) * n, hipHostMallocDefault) != hipSuccess)
+      e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");
+    if (!e && !w->pose_copied[i] && hipEventCreateWithFlags(&w->pose_copied[i], hipEventDisableTiming) != hipSuccess)
+      e = rtgi_fail(RTGI_E_HIP, "hipEventCreate");
+  }
+  w->objs.assign(objects, objects + n);
+  w->pose_first = first;
+  w->pose_span = span;
+  if (!e && span) {
+    if (hipMemcpy(w->rest, rest_pos, (size_t)span * 12, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(w->rest + (size_t)span * 3, rest_nrm, (size_t)span * 12, hipMemcpyHostToDevice) != hipSuccess)
+      e = rtgi_fail(RTGI_E_HIP, "rest mesh upload failed");
+  }
+  if (e) objects_free(w);
+  return e;
+}
+
+extern "C" int rtgi_world_set_pose(rtgi_world* w, const rtgi_pose* poses) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (w->objs.empty()) return rtgi_fail(RTGI_E_STATE, "no objects registered");
+  if (!poses) return rtgi_fail(RTGI_E_INVALID, "null poses");
+  const size_t n = w->objs.size();
+  for (size_t i = 0; i < n; i++) {
+    const rtgi_pose& q = poses[i];
+    if (!std::isfinite(q.c) || !std::isfinite(q.s) || !std::isfinite(q.offset[0]) || !std::isfinite(q.offset[1]) ||
+        !std::isfinite(q.offset[2]))
+      return rtgi_fail(RTGI_E_INVALID, "pose %zu not finite", i);
+  }
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(hipEventSynchronize(w->pose_copied[w->pose_cur]));
+  PoseRec* h = w->h_pose[w->pose_cur];
+  for (size_t i = 0; i < n; i++) {
+    const rtgi_object& o = w->objs[i];
+    for (int a = 0; a < 3; a++) {
+      h[i].pivot[a] = o.pivot[a];
+      h[i].off[a] = poses[i].offset[a];
+    }
+    h[i].c = poses[i].c;
+    h[i].s = poses[i].s;
+    h[i].first = o.first_vertex;
+    h[i].count = o.vertex_count;
+  }
+  // A batched update of the world may still read the records being replaced.
+  if (w->ext_done) HIPCHK(hipStreamWaitEvent(w->stream, w->ext_done, 0));
+  HIPCHK(hipMemcpyAsync(w->d_pose, h, sizeof(PoseRec) * n, hipMemcpyHostToDevice, w->stream));
+  HIPCHK(hipEventRecord(w->pose_copied[w->pose_cur], w->stream));
+  w->pose_cur ^= 1;
+  w->pose_set = true;
+  return RTGI_OK;
+}
+
 int frame_origins(const rtgi_vox_params* vp, const rtgi_rc_params* rp, const rtgi_frame* f,
                   int origin[RTGI_MAX_LEVELS][3], int G0[3]) {
   if (f->n_point < 0 || f->n_point > RTGI_MAX_POINT_LIGHTS) return rtgi_fail(RTGI_E_INVALID, "0..8 point lights");
@@ -720,8 +813,9 @@ extern "C" int rtgi_world_enqueue_stages(rtgi_world* w, uint32_t stages, void* s
   if (stages & ~RTGI_STAGE_ALL) return rtgi_fail(RTGI_E_INVALID, "unknown stage bits");
   HIPCHK(hipSetDevice(w->device));
   hipStream_t s = stream ? (hipStream_t)stream : w->stream;
-  // A batched update that included this world may still be running.
+  // A batched update that included this world may still be running; the last pose copy runs on the world's stream.
   if (w->ext_done) HIPCHK(hipStreamWaitEvent(s, w->ext_done, 0));
+  if (s != w->stream && w->pose_set) HIPCHK(hipStreamWaitEvent(s, w->pose_copied[w->pose_cur ^ 1], 0));
   const uint32_t order[4] = {RTGI_STAGE_VOX, RTGI_STAGE_TRACE, RTGI_STAGE_MERGE, RTGI_STAGE_SH};
   if (w->timing) HIPCHK(hipEventRecord(w->ev[0], s));
   for (int i = 0; i < 4; i++) {
@@ -1156,6 +1250,8 @@ extern "C" int rtgi_world_destroy(rtgi_world* w) {
   free_all(w);
   for (int i = 0; i < 5; i++)
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+  for (int i = 0; i < 2; i++)
+    if (w->pose_copied[i]) (void)hipEventDestroy(w->pose_copied[i]);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete w;
   return RTGI_OK;
@@ -1400,7 +1496,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : b->stream;
   const int n = (int)b->worlds.size();
   std::vector<uint64_t> sig;
-  sig.reserve((size_t)n * 6);
+  sig.reserve((size_t)n * 7);
   for (int i = 0; i < n; i++) {
     rtgi_world* w = b->worlds[i];
     if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "world %d has no scene or no frame", i);
@@ -1412,6 +1508,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
     // The launch set also follows the tuning and the dynamic layer, either of which may change between enqueues.
     sig.push_back((uint64_t)(uint32_t)w->tuning.probe_tile << 32 | (uint32_t)w->tuning.fused);
     sig.push_back(w->dyn_hdr ? ((uint64_t)w->dyn_first << 32 | (uint32_t)w->dyn_cap) : ~0ull);
+    sig.push_back(w->pose_set ? ((uint64_t)w->objs.size() << 32 | w->pose_span) : 0ull);
   }
   if (sig != b->sig) {
     // The previous update may still read the block prefixes being replaced.

@@ -218,6 +218,44 @@ __device__ __forceinline__ void clear_body(const VoxK& k, uint32_t lb, uint32_t 
   if (k.stats && i < 3u * RTGI_MAX_CASCADES) k.stats[i] = 0ull;
   if (i < (uint32_t)(k.NL * k.per)) k.ptr[i] = PTR_EMPTY;
 }
+// posedVertices (src/core/scenes.ts:80-102), thread per vertex of the objects' span: its object by binary search of
+// the ascending firsts (vertices between objects are left alone), x = rest - pivot in double, the turn about +y and
+// the offset, each value rounded to float once. The reference evaluates the same expressions in JavaScript doubles
+// on the host; the same IEEE double operations in the same order (no contraction) give the same floats.
+__device__ __forceinline__ void pose_body(const VoxK& k, uint32_t lb) {
+  const uint32_t i = lb * BLK + threadIdx.x;
+  if (i >= k.pose_span) return;
+  const uint32_t v = k.pose_first + i;
+  uint32_t lo = 0, hi = k.npose - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (k.pose[mid].first <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const PoseRec& o = k.pose[lo];
+  if (v < o.first || v - o.first >= o.count) return;
+  const float* rp = k.rest + (size_t)i * 3;
+  const float* rn = k.rest + (size_t)k.pose_span * 3 + (size_t)i * 3;
+  const double x = (double)rp[0] - o.pivot[0], y = (double)rp[1] - o.pivot[1], z = (double)rp[2] - o.pivot[2];
+  const double nx = rn[0], ny = rn[1], nz = rn[2];
+  float* P = const_cast<float*>(k.pos) + (size_t)v * 3;
+  float* N = const_cast<float*>(k.nrm) + (size_t)v * 3;
+  P[0] = (float)(o.pivot[0] + o.c * x + o.s * z + o.off[0]);
+  P[1] = (float)(o.pivot[1] + y + o.off[1]);
+  P[2] = (float)(o.pivot[2] - o.s * x + o.c * z + o.off[2]);
+  N[0] = (float)(o.c * nx + o.s * nz);
+  N[1] = (float)ny;
+  N[2] = (float)(-o.s * nx + o.c * nz);
+}
+__global__ void __launch_bounds__(BLK) k_pose(VoxK k) { pose_body(k, blockIdx.x); }
+__global__ void __launch_bounds__(BLK) k_pose_b(BatchK b) {
+  const uint32_t blk = batch_block(b);
+  int w = batch_world(b, blk);
+  pose_body(b.w[w].vx, blk - b.start[w]);
+}
+
 __global__ void __launch_bounds__(BLK) k_clear(VoxK k) { clear_body(k, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
   const uint32_t blk = batch_block(b);
@@ -887,6 +925,7 @@ static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.
 
 int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
+  if (p.grid[BL_POSE]) k_pose_b<<<p.grid[BL_POSE], BLK, 0, s>>>(bk(p, BL_POSE));
   if (p.grid[BL_CLEAR]) k_clear_b<<<p.grid[BL_CLEAR], BLK, 0, s>>>(bk(p, BL_CLEAR));
   if (p.grid[BL_MARK]) k_mark_b<<<p.grid[BL_MARK], BLK, p.mark_lds, s>>>(bk(p, BL_MARK));
   if (p.grid[BL_MARK_BIG]) k_mark_big_b<<<p.grid[BL_MARK_BIG], BLK, p.mark_lds, s>>>(bk(p, BL_MARK_BIG));
@@ -909,6 +948,7 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   for (int i = 0; i < BL_COUNT; i++) out[i] = 0;
   uint64_t cells = (uint64_t)k.NL * k.per, pool = (uint64_t)k.max_bricks * 512u;
   uint32_t persist = std::min<uint32_t>(2048, std::max<uint32_t>(1, cdiv(pool, BLK * 4)));
+  out[BL_POSE] = k.npose ? cdiv(k.pose_span, BLK) : 0;
   out[BL_CLEAR] = cdiv(std::max<uint64_t>(cells, HDR_WORDS), BLK);
   out[BL_MARK] = k.ntris ? std::min<uint32_t>(1024, cdiv(k.ntris, BLK * 4)) : 0;
   out[BL_MARK_BIG] = std::min<uint32_t>(512, std::max<uint32_t>(1, k.ntris / 1024));
@@ -949,6 +989,11 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
     out[BL_TM + st] = cdiv(trace_merge_threads(t, c), BLK);
   }
   out[BL_SH] = cdiv(t.c[0].np, BLK);
+}
+
+int launch_pose(const VoxK& k, void* stream) {
+  if (k.npose) k_pose<<<cdiv(k.pose_span, BLK), BLK, 0, (hipStream_t)stream>>>(k);
+  return (int)hipGetLastError();
 }
 
 int launch_vox_build(const VoxK& k, void* stream) {

@@ -90,6 +90,11 @@ struct VoxK {
   uint32_t ntris;
   uint32_t big_cap;    // entries per level of each big list: min(ntris, BIG_CAP)
   int32_t lds_marks;    // brick marks accumulate in an LDS bitmap of levels * G^3 bits (0: global fallback)
+  // Moving objects posed at the start of the update (rtgi_world_set_pose; npose 0: none): pose[npose] in ascending
+  // vertex order; rest holds the span's rest positions then its rest normals (3 floats per vertex each).
+  const struct PoseRec* pose;
+  const float* rest;
+  uint32_t npose, pose_first, pose_span;
   unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
   DynK dyn;                   // the dynamic layer (dyn.hdr null: none)
 };
@@ -105,6 +110,14 @@ constexpr uint32_t BN_DIV = 2;
 constexpr float BN_CELL = 1.2f;
 constexpr uint32_t BN_WORDS = BN_PROBES * 6u * 2u;  // u32 words of one volume
 constexpr uint32_t BN_PROBES_PER_BLOCK = 8;          // 32 lanes per probe in a 256-thread block
+
+// One posed object: vertices [first, first + count), pivot, cos / sin of the turn about +y, translation.
+struct PoseRec {
+  double pivot[3];
+  double c, s;
+  double off[3];
+  uint32_t first, count;
+};
 
 struct CascK {
   int32_t n;        // probes per axis
@@ -183,6 +196,8 @@ struct BatchK {
 };
 
 // Launchers (each returns a hipError_t as int).
+// posedVertices of the world's objects into its vertex buffers (before the voxel build that reads them).
+int launch_pose(const VoxK& k, void* stream);
 size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
 int launch_trace(const TraceK& k, void* stream);
@@ -214,7 +229,8 @@ int launch_sh(const TraceK& k, void* stream);
 
 // The launches of one batched update, and for each the per-world block prefix (device memory) and grid size.
 enum BatchLaunch : int {
-  BL_CLEAR = 0,
+  BL_POSE = 0,                             // posed vertices of the moving objects
+  BL_CLEAR,
   BL_MARK,
   BL_MARK_BIG,
   BL_MARK_PARENTS,
@@ -256,7 +272,7 @@ struct BatchPlan {
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);
-int launch_vox_build_batch(const BatchPlan& p, void* stream);
+int launch_vox_build_batch(const BatchPlan& p, void* stream);  // with the pose first
 // The dynamic layer's build (single world / batch), after the voxel build.
 int launch_dyn_build(const VoxK& k, void* stream);
 int launch_dyn_splat(const TraceK& t, void* stream);

@@ -6,8 +6,10 @@
 // baseColorFactor is the triangle's albedo, uniform in [0.1, 0.9] per channel (the white default texture makes the
 // voxel albedo equal to baseColorFactor, src/core/scene_gpu.ts:64-68). Quads follow the Builder idiom of
 // src/core/scenes.ts:176-199 (outward faces, two triangles (0,1,2), (0,2,3) per quad), subdivided into a grid so that
-// triangle counts reach the configured sizes. Moving parts use the rigid pose of posedVertices
-// (src/core/scenes.ts:80-102): p = pivot + rotY(p - pivot) + offset.
+// triangle counts reach the configured sizes. Moving parts are objects (DynamicObject, src/core/scenes.ts:41) built
+// at rest, each with a seeded schedule (DynamicSpec.at); the scene's positions and normals are the rest mesh posed at
+// the requested frame by posedVertices (src/core/scenes.ts:80-102): p = pivot + rotY(p - pivot) + offset, from the
+// float rest data in double, rounded to float once.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -48,32 +50,26 @@ struct Rng {
   double uni(double a, double b) { return a + (b - a) * uni(); }
 };
 
-struct Pose {
-  V3 pivot;
-  double rot;  // about +y
-  V3 offset;
-  V3 apply(V3 p) const {
-    double c = std::cos(rot), s = std::sin(rot);
-    V3 q = p - pivot;
-    return V3{pivot.x + c * q.x + s * q.z + offset.x, pivot.y + q.y + offset.y, pivot.z - s * q.x + c * q.z + offset.z};
-  }
-  V3 applyN(V3 n) const {
-    double c = std::cos(rot), s = std::sin(rot);
-    return V3{c * n.x + s * n.z, n.y, -s * n.x + c * n.z};
-  }
-};
-
 struct Builder {
   std::vector<float> pos, nrm;
   std::vector<uint32_t> tri;
+  std::vector<rtgi_object> objects;
+  std::vector<rtgi_motion> motion;
   double step = 0.5;       // target edge of a sub-quad (m)
-  const Pose* pose = nullptr;
+
+  // The vertices made between begin_object and end_object form one object turning about pivot on its schedule.
+  void begin_object(V3 pivot, const rtgi_motion& m) {
+    rtgi_object o{};
+    o.first_vertex = (uint32_t)(pos.size() / 3);
+    o.pivot[0] = pivot.x;
+    o.pivot[1] = pivot.y;
+    o.pivot[2] = pivot.z;
+    objects.push_back(o);
+    motion.push_back(m);
+  }
+  void end_object() { objects.back().vertex_count = (uint32_t)(pos.size() / 3) - objects.back().first_vertex; }
 
   uint32_t vert(V3 p, V3 n) {
-    if (pose) {
-      p = pose->apply(p);
-      n = pose->applyN(n);
-    }
     uint32_t i = (uint32_t)(pos.size() / 3);
     pos.push_back((float)p.x);
     pos.push_back((float)p.y);
@@ -157,24 +153,65 @@ struct Builder {
   }
 };
 
-rtgi_scene* finish(Builder& b, Rng& rng, const double eye[3], const float sun[3], uint32_t first_dynamic = ~0u) {
+// posedVertices (src/core/scenes.ts:80-102) of objects [0, n) from rest data starting at vertex `first` into pos / nrm
+// (whole-scene arrays), in double with the reference's operation order, each value rounded to float once.
+void pose_objects(const rtgi_object* obj, uint32_t n, const rtgi_pose* ps, const float* rest_pos,
+                  const float* rest_nrm, uint32_t first, float* pos, float* nrm) {
+  for (uint32_t i = 0; i < n; i++) {
+    const rtgi_object& o = obj[i];
+    const double c = ps[i].c, s = ps[i].s;
+    const double* off = ps[i].offset;
+    for (uint32_t v = o.first_vertex; v < o.first_vertex + o.vertex_count; v++) {
+      const float* rp = rest_pos + (size_t)(v - first) * 3;
+      const float* rn = rest_nrm + (size_t)(v - first) * 3;
+      double x = (double)rp[0] - o.pivot[0], y = (double)rp[1] - o.pivot[1], z = (double)rp[2] - o.pivot[2];
+      double nx = rn[0], ny = rn[1], nz = rn[2];
+      pos[(size_t)v * 3 + 0] = (float)(o.pivot[0] + c * x + s * z + off[0]);
+      pos[(size_t)v * 3 + 1] = (float)(o.pivot[1] + y + off[1]);
+      pos[(size_t)v * 3 + 2] = (float)(o.pivot[2] - s * x + c * z + off[2]);
+      nrm[(size_t)v * 3 + 0] = (float)(c * nx + s * nz);
+      nrm[(size_t)v * 3 + 1] = (float)ny;
+      nrm[(size_t)v * 3 + 2] = (float)(-s * nx + c * nz);
+    }
+  }
+}
+
+rtgi_scene* finish(Builder& b, Rng& rng, const double eye[3], const float sun[3], double frame,
+                   uint32_t first_dynamic = ~0u) {
   rtgi_scene* s = (rtgi_scene*)std::calloc(1, sizeof(rtgi_scene));
   if (!s) return nullptr;
   uint32_t nt = (uint32_t)(b.tri.size() / 4), nv = (uint32_t)(b.pos.size() / 3);
+  const uint32_t no = (uint32_t)b.objects.size();
+  const uint32_t rfirst = no ? b.objects[0].first_vertex : 0u;
+  const uint32_t rcount = no ? b.objects[no - 1].first_vertex + b.objects[no - 1].vertex_count - rfirst : 0u;
   s->ntris = nt;
   s->first_dynamic_tri = std::min(first_dynamic, nt);
   s->nverts = nv;
   s->nmat = nt;
+  s->nobj = no;
   s->pos = (float*)std::malloc(sizeof(float) * 3 * (size_t)std::max(nv, 1u));
   s->nrm = (float*)std::malloc(sizeof(float) * 3 * (size_t)std::max(nv, 1u));
   s->tri = (uint32_t*)std::malloc(sizeof(uint32_t) * 4 * (size_t)std::max(nt, 1u));
   s->materials = (float*)std::malloc(sizeof(float) * 4 * (size_t)std::max(nt, 1u));
-  if (!s->pos || !s->nrm || !s->tri || !s->materials) {
+  s->objects = (rtgi_object*)std::malloc(sizeof(rtgi_object) * std::max(no, 1u));
+  s->motion = (rtgi_motion*)std::malloc(sizeof(rtgi_motion) * std::max(no, 1u));
+  s->rest_pos = (float*)std::malloc(sizeof(float) * 3 * (size_t)std::max(rcount, 1u));
+  s->rest_nrm = (float*)std::malloc(sizeof(float) * 3 * (size_t)std::max(rcount, 1u));
+  std::vector<rtgi_pose> ps(no);
+  if (!s->pos || !s->nrm || !s->tri || !s->materials || !s->objects || !s->motion || !s->rest_pos || !s->rest_nrm) {
     rtgi_scene_free(s);
     return nullptr;
   }
   std::memcpy(s->pos, b.pos.data(), sizeof(float) * b.pos.size());
   std::memcpy(s->nrm, b.nrm.data(), sizeof(float) * b.nrm.size());
+  if (no) {
+    std::memcpy(s->objects, b.objects.data(), sizeof(rtgi_object) * no);
+    std::memcpy(s->motion, b.motion.data(), sizeof(rtgi_motion) * no);
+    std::memcpy(s->rest_pos, b.pos.data() + (size_t)rfirst * 3, sizeof(float) * 3 * (size_t)rcount);
+    std::memcpy(s->rest_nrm, b.nrm.data() + (size_t)rfirst * 3, sizeof(float) * 3 * (size_t)rcount);
+    rtgi_motion_pose_at(s->motion, no, frame, ps.data());
+    pose_objects(s->objects, no, ps.data(), s->rest_pos, s->rest_nrm, rfirst, s->pos, s->nrm);
+  }
   for (uint32_t t = 0; t < nt; t++) {
     s->tri[4 * t + 0] = b.tri[4 * t + 0];
     s->tri[4 * t + 1] = b.tri[4 * t + 1];
@@ -237,8 +274,7 @@ rtgi_scene* atrium(const rtgi_scene_params& p) {
   b.box({0, 8.15, 6}, {6.3, 0.15, 0.3});
   b.box({-6, 8.15, 0}, {0.3, 0.15, 5.7});
   b.box({6, 8.15, 0}, {0.3, 0.15, 5.7});
-  // 16 boxes under seeded rigid motion (rotation about y and translation, evaluated at the frame): the tail of the
-  // triangle table.
+  // 16 boxes under seeded rigid motion (rotation about y and translation): the tail of the triangle table.
   const uint32_t first_dynamic = (uint32_t)(b.tri.size() / 4);
   for (int k = 0; k < 16; k++) {
     V3 c{rng.uni(-8, 8), rng.uni(0.4, 3.0), rng.uni(-8, 8)};
@@ -246,15 +282,14 @@ rtgi_scene* atrium(const rtgi_scene_params& p) {
     double w = rng.uni(-0.05, 0.05), ph = rng.uni(0, 2 * M_PI);
     V3 amp{rng.uni(-1, 1), rng.uni(-0.3, 0.3), rng.uni(-1, 1)};
     double wt = rng.uni(0.01, 0.04);
-    Pose pose{c, w * p.frame + ph, amp * std::sin(wt * p.frame + ph)};
-    b.pose = &pose;
+    b.begin_object(c, rtgi_motion{ph, w, wt, ph, {amp.x, amp.y, amp.z}});
     b.box(c, h, b.step * 0.5);
-    b.pose = nullptr;
+    b.end_object();
   }
   double eye[3] = {0.0, 5.0, 0.0};
   float sun[3] = {-0.3f, 1.0f, 0.25f};
   normalizef(sun);
-  return finish(b, rng, eye, sun, first_dynamic);
+  return finish(b, rng, eye, sun, p.frame, first_dynamic);
 }
 
 // City block: about 500k triangles at detail 1 with 64 buildings on a 200 m block.
@@ -269,20 +304,24 @@ rtgi_scene* city(const rtgi_scene_params& p) {
   b.step = 0.71 / det;
   // Ground, tessellated coarsely.
   b.quad({-ext / 2, 0, ext / 2}, {ext / 2, 0, ext / 2}, {ext / 2, 0, -ext / 2}, {-ext / 2, 0, -ext / 2}, 2.0 / det);
+  // The buildings, one object each (contiguous vertices, as posedVertices expects of its objects).
   for (int k = 0; k < nb; k++) {
     int gx = k % side, gz = k / side;
     V3 c{-ext / 2 + (gx + 0.5) * lot, 0, -ext / 2 + (gz + 0.5) * lot};
     double wx = rng.uni(0.35, 0.7) * lot, wz = rng.uni(0.35, 0.7) * lot, hgt = rng.uni(8, 40);
     double rot0 = rng.uni(-0.3, 0.3), w = rng.uni(-0.01, 0.01), ph = rng.uni(0, 2 * M_PI);
     V3 amp{rng.uni(-0.5, 0.5), 0, rng.uni(-0.5, 0.5)};
-    Pose pose{c, rot0 + w * p.frame, amp * std::sin(0.05 * p.frame + ph)};
-    b.pose = &pose;
+    b.begin_object(c, rtgi_motion{rot0, w, 0.05, ph, {amp.x, amp.y, amp.z}});
     b.box({c.x, hgt / 2, c.z}, {wx / 2, hgt / 2, wz / 2});
     // A roof block and ledges.
     b.box({c.x, hgt + 1.0, c.z}, {wx / 4, 1.0, wz / 4}, b.step * 2);
     int nled = (int)(hgt / 4);
     for (int l = 1; l < nled; l++) b.box({c.x, 4.0 * l, c.z + wz / 2 + 0.15}, {wx / 2, 0.1, 0.15}, b.step * 2);
-    b.pose = nullptr;
+    b.end_object();
+  }
+  for (int k = 0; k < nb; k++) {
+    int gx = k % side, gz = k / side;
+    V3 c{-ext / 2 + (gx + 0.5) * lot, 0, -ext / 2 + (gz + 0.5) * lot};
     // Street furniture along the lot edge: lamp posts and benches.
     for (int f = 0; f < 6; f++) {
       double t = rng.uni(-0.45, 0.45) * lot;
@@ -296,7 +335,7 @@ rtgi_scene* city(const rtgi_scene_params& p) {
   double eye[3] = {1.0, 32.0, 1.0};  // above the roofs: the probe grid spans y 0..63 m
   float sun[3] = {-0.4f, 1.0f, 0.3f};
   normalizef(sun);
-  return finish(b, rng, eye, sun);
+  return finish(b, rng, eye, sun, p.frame);
 }
 
 }  // namespace
@@ -326,6 +365,21 @@ extern "C" int rtgi_scene_free(rtgi_scene* s) {
   std::free(s->nrm);
   std::free(s->tri);
   std::free(s->materials);
+  std::free(s->objects);
+  std::free(s->motion);
+  std::free(s->rest_pos);
+  std::free(s->rest_nrm);
   std::free(s);
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_motion_pose_at(const rtgi_motion* m, uint32_t n, double frame, rtgi_pose* out) {
+  if (n && (!m || !out)) return rtgi_fail(RTGI_E_INVALID, "rtgi_motion_pose_at: null array");
+  for (uint32_t i = 0; i < n; i++) {
+    const double rot = m[i].rot0 + m[i].w * frame, a = std::sin(m[i].wt * frame + m[i].ph);
+    out[i].c = std::cos(rot);
+    out[i].s = std::sin(rot);
+    for (int k = 0; k < 3; k++) out[i].offset[k] = m[i].amp[k] * a;
+  }
   return RTGI_OK;
 }

@@ -70,6 +70,17 @@ struct rtgi_world {
   int bn_org[2][3] = {};                  // origins of the two volumes
   uint32_t bn_frame = 0;                  // volume updates so far
   uint32_t bn_dir_off = 0;                // first bounce ray direction in dirs
+  // Moving objects (rtgi_world_set_objects, rtgi_world_set_pose): the rest mesh of their vertex span (positions,
+  // then normals), their pose records on the device and two pinned staging copies (the one being filled was last
+  // copied two poses ago).
+  std::vector<rtgi_object> objs;
+  uint32_t pose_first = 0, pose_span = 0;
+  float* rest = nullptr;
+  PoseRec* d_pose = nullptr;
+  PoseRec* h_pose[2] = {};
+  hipEvent_t pose_copied[2] = {};
+  int pose_cur = 0;
+  bool pose_set = false;  // every update poses the objects first
   uint64_t bytes = 0;
   char* arena = nullptr;     // one device allocation holding every fixed buffer above (scene and layer excluded)
   size_t arena_bytes = 0;

@@ -7,6 +7,12 @@
 //                                     src/core/dynlayer.ts) and dyn.readCounters() reads its counters
 //   createRc(voxWorld, params)     <- createRc (src/core/rc.ts:1019): encode(), readSh(), readCascade(c), destroy(),
 //                                     gather(pos, nrm, occlusion) (gatherWorld), readProbes(c), readBounceVolume()
+//   voxWorld.setDynamicSpec(scene, spec), voxWorld.pose(dynamicPose)
+//                                  <- posedVertices + the positions / normals writeBuffer of the runner's pose(t)
+//                                     (src/core/scenes.ts:80-102, src/core/rcrun.ts:206-217): the rest mesh stays on
+//                                     the device and each update poses the objects there; cos / sin of rotY are
+//                                     taken here with Math.cos / Math.sin, as posedVertices takes them
+//   dynamicSpec(scene)             <- the generated scene's DynamicSpec: objects and at(frame)
 // One library world backs both: the voxel world owns it, and createRc re-creates it with the cascade parameters
 // (its scene upload is repeated). encode() on either runs the whole update (rebuild, trace, merge, SH): the reference
 // records both passes into one command buffer per frame (src/core/rcrun.ts:283-298).
@@ -53,8 +59,41 @@ function createVoxWorld(scene, params, frame, firstDynamicTri) {
       buildTemp: c.bricks * 512 * 4 };
   };
   vw.destroy = () => { if (vw.handle) addon.destroy(vw.handle); vw.handle = null; };
+  // Moving objects survive a rebuild (createRc re-creates the world): the spec and the last pose are replayed.
+  let objects = null, lastPose = null;
+  // sc: the scene whose positions / normals are the rest mesh (SceneData), or a generated scene (its restPos /
+  // restNrm are the objects' span at rest).
+  vw.setDynamicSpec = (sc, spec) => {
+    const o = spec.objects, f0 = o.length ? o[0].firstVertex : 0;
+    const end = o.length ? o[o.length - 1].firstVertex + o[o.length - 1].vertexCount : 0;
+    const rest = sc.restPos ? [sc.restPos, sc.restNrm] : [sc.pos.slice(f0 * 3, end * 3), sc.nrm.slice(f0 * 3, end * 3)];
+    objects = { list: o, rest };
+    addon.setObjects(vw.handle, o, rest[0], rest[1]);
+  };
+  vw.pose = (dp) => {
+    lastPose = dp.poses.map((p) => ({ c: Math.cos(p.rotY), s: Math.sin(p.rotY), offset: p.offset }));
+    addon.setPose(vw.handle, lastPose);
+    return lastPose;
+  };
+  const rebuild = vw.rebuild;
+  vw.rebuild = () => {
+    rebuild();
+    if (objects) addon.setObjects(vw.handle, objects.list, objects.rest[0], objects.rest[1]);
+    if (objects && lastPose) addon.setPose(vw.handle, lastPose);
+  };
   vw.rebuild();
   return vw;
+}
+
+// DynamicSpec of a generated scene (src/core/scenes.ts:48-53): its objects, and at(frame) with rotY = rot0 + w f and
+// offset = amp sin(wt f + ph); the scene's vertices are its rest mesh posed at the frame it was generated for, so a
+// world that poses them takes the rest mesh (restPos / restNrm) as the objects' span.
+function dynamicSpec(scene) {
+  return {
+    objects: scene.objects,
+    at: (f) => ({ poses: scene.motion.map((m) => ({ rotY: m.rot0 + m.w * f,
+      offset: m.amp.map((a) => a * Math.sin(m.wt * f + m.ph)) })) }),
+  };
 }
 
 function createRc(voxWorld, params) {
@@ -73,4 +112,4 @@ function createRc(voxWorld, params) {
 }
 
 module.exports = { addon, DEFAULT_VOX, DEFAULT_PARAMS, SUN_E, SKY, voxParams, rcParams, createVoxWorld, createRc,
-  generateScene: addon.generateScene, deviceCount: addon.deviceCount, version: addon.version };
+  dynamicSpec, generateScene: addon.generateScene, deviceCount: addon.deviceCount, version: addon.version };

@@ -10,6 +10,7 @@
 #include <node_api.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "rtgi.h"
@@ -384,6 +385,78 @@ static napi_value js_set_dynamic(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setObjects(h, objects: [{firstVertex, vertexCount, pivot: [3]}], restPos: Float32Array, restNrm: Float32Array):
+ * the DynamicSpec's objects (src/core/scenes.ts:41) and the rest mesh of their vertex span */
+static napi_value js_set_objects(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4], e;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 4 ? world_of(env, argv[0]) : NULL;
+  uint32_t n = 0;
+  size_t np = 0, nn = 0;
+  float* rp = argc == 4 ? typed_data(env, argv[2], napi_float32_array, &np) : NULL;
+  float* rn = argc == 4 ? typed_data(env, argv[3], napi_float32_array, &nn) : NULL;
+  if (!w || napi_get_array_length(env, argv[1], &n) != napi_ok || (n && (!rp || !rn || np != nn || np % 3))) {
+    napi_throw_type_error(env, NULL, "setObjects(h, [{firstVertex, vertexCount, pivot}], Float32Array restPos, "
+                                     "Float32Array restNrm)");
+    return NULL;
+  }
+  rtgi_object* objs = (rtgi_object*)calloc(n ? n : 1, sizeof(rtgi_object));
+  if (!objs) {
+    napi_throw_error(env, NULL, "rtgi addon: out of memory");
+    return NULL;
+  }
+  int ok = 1;
+  for (uint32_t i = 0; i < n && ok; i++) {
+    double f = 0, c = 0;
+    ok = napi_get_element(env, argv[1], i, &e) == napi_ok && get_f64(env, e, "firstVertex", &f) &&
+         get_f64(env, e, "vertexCount", &c) && get_vec(env, e, "pivot", objs[i].pivot, 3);
+    objs[i].first_vertex = (uint32_t)f;
+    objs[i].vertex_count = (uint32_t)c;
+  }
+  /* the rest arrays must cover the span the objects name */
+  if (ok && n && (uint64_t)objs[n - 1].first_vertex + objs[n - 1].vertex_count - objs[0].first_vertex > np / 3) ok = 0;
+  int st = ok ? rtgi_world_set_objects(w, objs, n, rp, rn) : 0;
+  free(objs);
+  if (!ok) {
+    napi_throw_type_error(env, NULL, "setObjects: bad object record or rest arrays shorter than the objects' span");
+    return NULL;
+  }
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
+/* setPose(h, poses: [{c, s, offset: [3]}]): each object's cos(rotY), sin(rotY) and offset, as posedVertices forms
+ * them (src/core/scenes.ts:87-89) */
+static napi_value js_set_pose(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2], e;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc == 2 ? world_of(env, argv[0]) : NULL;
+  uint32_t n = 0;
+  if (!w || napi_get_array_length(env, argv[1], &n) != napi_ok) {
+    napi_throw_type_error(env, NULL, "setPose(h, [{c, s, offset}])");
+    return NULL;
+  }
+  rtgi_pose* ps = (rtgi_pose*)calloc(n ? n : 1, sizeof(rtgi_pose));
+  if (!ps) {
+    napi_throw_error(env, NULL, "rtgi addon: out of memory");
+    return NULL;
+  }
+  int ok = 1;
+  for (uint32_t i = 0; i < n && ok; i++)
+    ok = napi_get_element(env, argv[1], i, &e) == napi_ok && get_f64(env, e, "c", &ps[i].c) &&
+         get_f64(env, e, "s", &ps[i].s) && get_vec(env, e, "offset", ps[i].offset, 3);
+  int st = ok ? rtgi_world_set_pose(w, ps) : 0;
+  free(ps);
+  if (!ok) {
+    napi_throw_type_error(env, NULL, "setPose: each pose needs c, s and offset [3]");
+    return NULL;
+  }
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
 /* readDynCounters(h) -> {bricks, brickOverflow, outside} (DynLayer.readCounters) */
 static napi_value js_read_dyn_counters(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -404,7 +477,8 @@ static napi_value js_read_dyn_counters(napi_env env, napi_callback_info info) {
   return o;
 }
 
-/* generateScene(kind: 'atrium'|'city', seed, frame) -> {pos, nrm, tri, materials, eye, sunDir, firstDynamicTri} */
+/* generateScene(kind: 'atrium'|'city', seed, frame) -> {pos, nrm, tri, materials, eye, sunDir, firstDynamicTri,
+ * objects: [{firstVertex, vertexCount, pivot}], motion: [{rot0, w, wt, ph, amp}], restPos, restNrm} */
 static napi_value js_generate_scene(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
@@ -450,6 +524,49 @@ static napi_value js_generate_scene(napi_env env, napi_callback_info info) {
   }
   napi_set_named_property(env, o, "sunDir", arr);
   napi_set_named_property(env, o, "firstDynamicTri", make_u32(env, s->first_dynamic_tri));
+  napi_value objs, mot, rec, pv;
+  napi_create_array_with_length(env, s->nobj, &objs);
+  napi_create_array_with_length(env, s->nobj, &mot);
+  for (uint32_t i = 0; i < s->nobj; i++) {
+    napi_create_object(env, &rec);
+    napi_set_named_property(env, rec, "firstVertex", make_u32(env, s->objects[i].first_vertex));
+    napi_set_named_property(env, rec, "vertexCount", make_u32(env, s->objects[i].vertex_count));
+    napi_create_array_with_length(env, 3, &pv);
+    for (int a = 0; a < 3; a++) {
+      napi_create_double(env, s->objects[i].pivot[a], &e);
+      napi_set_element(env, pv, (uint32_t)a, e);
+    }
+    napi_set_named_property(env, rec, "pivot", pv);
+    napi_set_element(env, objs, i, rec);
+    const rtgi_motion* m = &s->motion[i];
+    napi_create_object(env, &rec);
+    const char* keys[4] = {"rot0", "w", "wt", "ph"};
+    const double vals[4] = {m->rot0, m->w, m->wt, m->ph};
+    for (int k = 0; k < 4; k++) {
+      napi_create_double(env, vals[k], &e);
+      napi_set_named_property(env, rec, keys[k], e);
+    }
+    napi_create_array_with_length(env, 3, &pv);
+    for (int a = 0; a < 3; a++) {
+      napi_create_double(env, m->amp[a], &e);
+      napi_set_element(env, pv, (uint32_t)a, e);
+    }
+    napi_set_named_property(env, rec, "amp", pv);
+    napi_set_element(env, mot, i, rec);
+  }
+  napi_set_named_property(env, o, "objects", objs);
+  napi_set_named_property(env, o, "motion", mot);
+  const size_t span = s->nobj ? (size_t)s->objects[s->nobj - 1].first_vertex + s->objects[s->nobj - 1].vertex_count -
+                                    s->objects[0].first_vertex
+                              : 0;
+  const char* rk[2] = {"restPos", "restNrm"};
+  float* rs[2] = {s->rest_pos, s->rest_nrm};
+  for (int i = 0; i < 2; i++) {
+    if (napi_create_arraybuffer(env, span * 12, &data, &ab) != napi_ok) break;
+    if (span) memcpy(data, rs[i], span * 12);
+    napi_create_typedarray(env, napi_float32_array, span * 3, ab, 0, &ta);
+    napi_set_named_property(env, o, rk[i], ta);
+  }
   rtgi_scene_free(s);
   return o;
 }
@@ -470,6 +587,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readProbes", 0, js_read_probes, 0, 0, 0, napi_default, 0},
       {"readBounceVolume", 0, js_read_bounce_volume, 0, 0, 0, napi_default, 0},
       {"setDynamic", 0, js_set_dynamic, 0, 0, 0, napi_default, 0},
+      {"setObjects", 0, js_set_objects, 0, 0, 0, napi_default, 0},
+      {"setPose", 0, js_set_pose, 0, 0, 0, napi_default, 0},
       {"readDynCounters", 0, js_read_dyn_counters, 0, 0, 0, napi_default, 0},
       {"destroy", 0, js_destroy, 0, 0, 0, napi_default, 0},
       {"generateScene", 0, js_generate_scene, 0, 0, 0, napi_default, 0},

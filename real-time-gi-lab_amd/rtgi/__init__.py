@@ -118,10 +118,28 @@ class Scene:
     sunDir: tuple = (-0.3, 1.0, 0.25)
     bounds: tuple = None
     firstDynamicTri: int = None  # the moving parts are triangles [firstDynamicTri, ntris) (None: none)
+    # Moving objects (DynamicSpec, src/core/scenes.ts:41-53): rtgi_object records, each one's motion schedule
+    # (rot0, w, wt, ph, amp xyz) and the rest mesh of their vertex span; pos / nrm hold it posed at the frame.
+    objects: np.ndarray = None
+    motion: np.ndarray = None
+    rest_pos: np.ndarray = None
+    rest_nrm: np.ndarray = None
 
     @property
     def ntris(self):
         return int(self.tri.shape[0])
+
+    def poses_at(self, frame):
+        """Poses (c, s, offset xyz per object; DynamicSpec.at) of the scene's objects at a frame."""
+        return motion_poses(self.motion, frame)
+
+
+def motion_poses(motion, frame):
+    """rtgi_motion_pose_at: (n, 5) doubles (c = cos(rotY), s = sin(rotY), offset) of n motion schedules at a frame."""
+    m = np.ascontiguousarray(motion, np.float64).reshape(-1, 7)
+    out = np.zeros((m.shape[0], 5), np.float64)
+    check(lib().rtgi_motion_pose_at(ptr(m), m.shape[0], float(frame), ptr(out)))
+    return out
 
 
 def generate_scene(kind, seed=1, frame=0, extent=0.0, buildings=0, detail=1.0):
@@ -132,12 +150,18 @@ def generate_scene(kind, seed=1, frame=0, extent=0.0, buildings=0, detail=1.0):
     check(lib().rtgi_scene_generate(C.byref(p), C.byref(out)))
     s = out.contents
     try:
+        objs = np.frombuffer(C.string_at(s.objects, s.nobj * _L.OBJECT_DTYPE.itemsize),
+                             _L.OBJECT_DTYPE).copy() if s.nobj else np.zeros(0, _L.OBJECT_DTYPE)
+        span = int(objs["first_vertex"][-1] + objs["vertex_count"][-1] - objs["first_vertex"][0]) if s.nobj else 0
         sc = Scene(pos=_L.as_np(s.pos, s.nverts * 3, np.float32).reshape(-1, 3),
                    nrm=_L.as_np(s.nrm, s.nverts * 3, np.float32).reshape(-1, 3),
                    tri=_L.as_np(s.tri, s.ntris * 4, np.uint32).reshape(-1, 4),
                    materials=_L.as_np(s.materials, s.nmat * 4, np.float32).reshape(-1, 4),
                    eye=tuple(s.eye), sunDir=tuple(s.sun_dir),
-                   bounds=(tuple(s.bounds_lo), tuple(s.bounds_hi)), firstDynamicTri=int(s.first_dynamic_tri))
+                   bounds=(tuple(s.bounds_lo), tuple(s.bounds_hi)), firstDynamicTri=int(s.first_dynamic_tri),
+                   objects=objs, motion=_L.as_np(s.motion, s.nobj * 7, np.float64).reshape(-1, 7),
+                   rest_pos=_L.as_np(s.rest_pos, span * 3, np.float32).reshape(-1, 3),
+                   rest_nrm=_L.as_np(s.rest_nrm, span * 3, np.float32).reshape(-1, 3))
     finally:
         lib().rtgi_scene_free(out)
     return sc
@@ -257,11 +281,34 @@ class World:
         check(lib().rtgi_world_upload_scene(self._h, ptr(pos), ptr(nrm), pos.shape[0], ptr(tri), tri.shape[0],
                                             ptr(mat), mat.shape[0]))
         self.scene = scene
+        self._nobj = 0  # a new scene has no objects until set_objects
 
     def update_vertices(self, first, pos, nrm):
         pos = np.ascontiguousarray(pos, np.float32)
         nrm = np.ascontiguousarray(nrm, np.float32)
         check(lib().rtgi_world_update_vertices(self._h, first, pos.shape[0], ptr(pos), ptr(nrm)))
+
+    def set_objects(self, objects, rest_pos, rest_nrm):
+        """rtgi_world_set_objects: moving objects (OBJECT_DTYPE records) and the rest mesh of their vertex span, posed
+        on the device by every update once set_pose has been called. An empty array removes them."""
+        objs = np.ascontiguousarray(objects, _L.OBJECT_DTYPE)
+        rp = np.ascontiguousarray(rest_pos, np.float32)
+        rn = np.ascontiguousarray(rest_nrm, np.float32)
+        check(lib().rtgi_world_set_objects(self._h, objs.ctypes.data if len(objs) else None, len(objs),
+                                           ptr(rp), ptr(rn)))
+        self._nobj = len(objs)
+
+    def set_scene_objects(self, scene):
+        """set_objects with a generated scene's objects and rest mesh."""
+        self.set_objects(scene.objects, scene.rest_pos, scene.rest_nrm)
+
+    def set_pose(self, poses):
+        """rtgi_world_set_pose: (n, 5) doubles (c, s, offset xyz) for the n registered objects, from the next update."""
+        q = np.ascontiguousarray(poses, np.float64).reshape(-1, 5)
+        n = getattr(self, "_nobj", 0)
+        if n and q.shape[0] != n:  # (without objects the library refuses the call)
+            raise ValueError(f"{q.shape[0]} poses for {n} objects")
+        check(lib().rtgi_world_set_pose(self._h, ptr(q)))
 
     def set_frame(self, frame: Frame):
         self.frame = frame

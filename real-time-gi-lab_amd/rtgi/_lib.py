@@ -104,11 +104,18 @@ class SceneParamsC(C.Structure):
                 ("buildings", C.c_int32), ("detail", C.c_double)]
 
 
+# rtgi_object, rtgi_pose and rtgi_motion as numpy layouts: objects are a structured array, poses (n, 5) doubles
+# (c, s, offset) and motion schedules (n, 7) doubles (rot0, w, wt, ph, amp).
+OBJECT_DTYPE = np.dtype([("first_vertex", "<u4"), ("vertex_count", "<u4"), ("pivot", "<f8", (3,))])
+
+
 class SceneC(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntris", C.c_uint32), ("nmat", C.c_uint32),
                 ("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)), ("tri", C.POINTER(C.c_uint32)),
                 ("materials", C.POINTER(C.c_float)), ("bounds_lo", C.c_double * 3), ("bounds_hi", C.c_double * 3),
-                ("eye", C.c_double * 3), ("sun_dir", C.c_float * 3), ("first_dynamic_tri", C.c_uint32)]
+                ("eye", C.c_double * 3), ("sun_dir", C.c_float * 3), ("first_dynamic_tri", C.c_uint32),
+                ("nobj", C.c_uint32), ("objects", C.c_void_p), ("motion", C.POINTER(C.c_double)),
+                ("rest_pos", C.POINTER(C.c_float)), ("rest_nrm", C.POINTER(C.c_float))]
 
 
 # Every exported entry point and its signature; tests check that the library exports exactly these.
@@ -121,6 +128,8 @@ SIGNATURES = {
     "rtgi_world_create": (C.c_int, [C.POINTER(VoxParamsC), C.POINTER(RcParamsC), C.POINTER(_P)]),
     "rtgi_world_upload_scene": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32]),
     "rtgi_world_update_vertices": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "rtgi_world_set_objects": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "rtgi_world_set_pose": (C.c_int, [_P, _P]),
     "rtgi_world_set_frame": (C.c_int, [_P, C.POINTER(FrameC)]),
     "rtgi_world_enqueue": (C.c_int, [_P, _P]),
     "rtgi_world_enqueue_stages": (C.c_int, [_P, C.c_uint32, _P]),
@@ -165,6 +174,7 @@ SIGNATURES = {
     "rtgi_server_destroy": (C.c_int, [_P]),
     "rtgi_scene_generate": (C.c_int, [C.POINTER(SceneParamsC), C.POINTER(C.POINTER(SceneC))]),
     "rtgi_scene_free": (C.c_int, [C.POINTER(SceneC)]),
+    "rtgi_motion_pose_at": (C.c_int, [_P, C.c_uint32, C.c_double, _P]),
 }
 
 _lib = None

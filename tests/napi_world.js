@@ -8,13 +8,25 @@ const [kind, seed, v0, levels, G, cascades, probes0, s0] = args.slice(0, 8).map(
 // 'full': the reference's default extras (voxel relocation, per-voxel sun visibility, multi-bounce), two updates, and
 // the gather / probe / bounce-volume readbacks.
 // 'dynamic': the scene's moving part (triangles from firstDynamicTri on) goes to the dynamic coverage layer.
-const full = args[8] === 'full', dynamic = args[8] === 'dynamic';
+// 'motion': the scene's objects posed on the device at frames 3 and 11 (the runner's pose(t)); the poses sent are
+// printed so that the other side can send the same.
+const full = args[8] === 'full', dynamic = args[8] === 'dynamic', motion = args[8] === 'motion';
 const scene = r.generateScene(kind, seed, 0);
 const vw = r.createVoxWorld(scene, { v0, levels, G, maxBricks: levels * G * G * G, maxVoxels: 1 << 20 },
   { eye: scene.eye, sunDir: scene.sunDir }, dynamic ? scene.firstDynamicTri : undefined);
 const rc = r.createRc(vw, full ? { cascades, probes0, s0, relocate: 1, sunTrace: 1, bounce: 1 } : { cascades, probes0, s0 });
 vw.update(scene.eye);
-rc.encode();
+const sent = [];
+if (motion) {
+  const spec = r.dynamicSpec(scene);
+  vw.setDynamicSpec(scene, spec);
+  for (const f of [3, 11]) {
+    sent.push(vw.pose(spec.at(f)));
+    rc.encode();
+  }
+} else {
+  rc.encode();
+}
 if (full) rc.encode();
 const sum = (a) => { let s = 0; for (let i = 0; i < a.length; i++) s += a[i]; return s; };
 // Gather points: an 8 x 8 grid 1 m below the eye, facing up.
@@ -28,6 +40,7 @@ const extra = full ? [rc.gather(pos, nrm, true), rc.readProbes(0), rc.readBounce
 Promise.all([vw.readCounters(), rc.readSh(), rc.readCascade(cascades - 1), ...extra]).then(([c, sh, top, irr, pr, bv]) => {
   const out = { counters: c, shSum: sum(sh), sh0: Array.from(sh.slice(0, 27)), topLen: top.length };
   if (dynamic) out.dyn = irr;
+  if (motion) out.poses = sent.map((ps) => ps.map((p) => [p.c, p.s, ...p.offset]));
   if (full) {
     out.irr = Array.from(irr);
     out.probeSum = sum(pr.pos) + sum(pr.nrm);

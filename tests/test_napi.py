@@ -94,6 +94,37 @@ def test_addon_dynamic_layer_matches_python_path(require_gpu):
 
 
 @pytest.mark.gpu
+def test_addon_motion_matches_python_path(require_gpu):
+    """The runner's per-frame pose through the addon (dynamicSpec(scene).at(f), Math.cos / Math.sin of rotY, the
+    objects posed on the device) over two frames: counters and SH equal the Python path's sent the same poses."""
+    print("case: node addon, atrium seed 5, 16 objects posed at frames 3 and 11", flush=True)
+    r = node(os.path.join(ROOT, "tests", "napi_world.js"), "atrium", "5", "0.125", "4", "8", "4", "32", "0.625",
+             "motion", timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    scene = rtgi.generate_scene("atrium", seed=5)
+    vox = rtgi.VoxParams(v0=0.125, levels=4, G=8, maxBricks=4 * 8 ** 3, maxVoxels=1 << 20)
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625)
+    poses = [np.array(p, np.float64) for p in out["poses"]]
+    assert len(poses) == 2 and poses[0].shape == (16, 5)
+    # Math.cos / Math.sin and the C library agree to an ulp or two.
+    np.testing.assert_allclose(poses[1], scene.poses_at(11), rtol=0, atol=1e-15)
+    with rtgi.World(vox, rc) as w:
+        w.upload_scene(scene)
+        w.set_scene_objects(scene)
+        w.set_frame(rtgi.Frame(eye=scene.eye, sunDir=scene.sunDir))
+        for p in poses:
+            w.set_pose(p)
+            w.encode()
+        c = w.read_counters()
+        sh = w.read_sh()
+    for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow", "bricksPerLevel"):
+        assert out["counters"][k] == c[k]
+    np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
+    assert abs(out["shSum"] - float(sh.astype(np.float64).sum())) <= 1e-6 * abs(float(sh.sum()))
+
+
+@pytest.mark.gpu
 def test_addon_full_default_matches_python_path(require_gpu):
     """The addon with voxel relocation, traced sun visibility and multi-bounce over two updates: SH, gather,
     relocated probes and the bounce volume equal the Python path's."""

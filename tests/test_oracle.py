@@ -1123,3 +1123,92 @@ def test_resolve_attr_tilted_triangle_closed_form():
         assert int(cm["albedo"][i]) == orc.pack4x8unorm([0.3, 0.6, 0.9, 0.0])
         checked += 1
     assert checked > 500
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# posedVertices (scenes.ts:80-102).
+
+def _dyadic(rng, n, scale, bits):
+    return rng.integers(-scale * 2 ** bits, scale * 2 ** bits + 1, size=n) / 2.0 ** bits
+
+
+def test_pose_exact_on_dyadic_inputs():
+    """Dyadic rest positions, pivots, cos/sin stand-ins and offsets with few bits make every double operation of
+    posedVertices exact, so each output is the float32 rounding of the exact rational value of
+    (pivot + c x + s z + off, pivot + y + off, pivot - s x + c z + off) and (c nx + s nz, ny, -s nx + c nz): computed
+    here with fractions, independent of operation order. A wrong sign, a swapped c and s or a wrong axis fails."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(17)
+    n_obj, per = 5, 7
+    first = np.arange(n_obj, dtype=np.uint32) * per + 3  # objects start at vertex 3
+    count = np.full(n_obj, per, np.uint32)
+    pivot = _dyadic(rng, n_obj * 3, 40, 2).reshape(-1, 3)
+    poses = np.stack([_dyadic(rng, n_obj, 1, 5), _dyadic(rng, n_obj, 1, 5)] +
+                     [_dyadic(rng, n_obj, 3, 3) for _ in range(3)], axis=1)
+    nv = 3 + n_obj * per + 2
+    rest_p = _dyadic(rng, n_obj * per * 3, 50, 6).reshape(-1, 3).astype(np.float32)
+    rest_n = _dyadic(rng, n_obj * per * 3, 1, 6).reshape(-1, 3).astype(np.float32)
+    base_p = rng.normal(size=(nv, 3)).astype(np.float32)
+    base_n = rng.normal(size=(nv, 3)).astype(np.float32)
+    pos, nrm = orc.pose(first, count, pivot, poses, rest_p, rest_n, base_p, base_n)
+    assert np.array_equal(pos[:3], base_p[:3]) and np.array_equal(pos[-2:], base_p[-2:])  # outside the objects
+    assert np.array_equal(nrm[:3], base_n[:3]) and np.array_equal(nrm[-2:], base_n[-2:])
+    for i in range(n_obj):
+        c, s = Fr(poses[i, 0]), Fr(poses[i, 1])
+        off = [Fr(v) for v in poses[i, 2:]]
+        pv = [Fr(v) for v in pivot[i]]
+        for v in range(int(first[i]), int(first[i] + count[i])):
+            r = [Fr(float(t)) for t in rest_p[v - 3]]
+            q = [Fr(float(t)) for t in rest_n[v - 3]]
+            x, y, z = r[0] - pv[0], r[1] - pv[1], r[2] - pv[2]
+            want_p = [pv[0] + c * x + s * z + off[0], pv[1] + y + off[1], pv[2] - s * x + c * z + off[2]]
+            want_n = [c * q[0] + s * q[2], q[1], -s * q[0] + c * q[2]]
+            for a in range(3):
+                assert pos[v, a] == np.float32(float(want_p[a])), (i, v, a)
+                assert nrm[v, a] == np.float32(float(want_n[a])), (i, v, a)
+
+
+def test_pose_quarter_turns_identity_and_rigidity():
+    """Closed forms: c = 1, s = 0 with zero offset and a dyadic pivot gives the rest mesh back; c = 0, s = 1 maps
+    (x, y, z) - pivot to (z, y, -x); any true rotation keeps every vertex's distance to the pivot (plus the offset)
+    and unit normals unit, to float rounding."""
+    rng = np.random.default_rng(3)
+    n = 40
+    rest_p = (rng.normal(size=(n, 3)) * 8).astype(np.float32)
+    rest_n = rng.normal(size=(n, 3))
+    rest_n = (rest_n / np.linalg.norm(rest_n, axis=1, keepdims=True)).astype(np.float32)
+    first, count, pivot = [0], [n], np.array([[1.5, -2.25, 4.0]])
+    z = np.zeros((n, 3), np.float32)
+    p, q = orc.pose(first, count, pivot, [[1.0, 0.0, 0.0, 0.0, 0.0]], rest_p, rest_n, z, z)
+    assert np.array_equal(p, rest_p) and np.array_equal(q, rest_n)
+    p, q = orc.pose(first, count, pivot, [[0.0, 1.0, 0.0, 0.0, 0.0]], rest_p, rest_n, z, z)
+    rel = rest_p.astype(np.float64) - pivot[0]
+    want = np.stack([pivot[0, 0] + rel[:, 2], pivot[0, 1] + rel[:, 1], pivot[0, 2] - rel[:, 0]], axis=1)
+    np.testing.assert_allclose(p, want, rtol=0, atol=1e-5)
+    np.testing.assert_array_equal(q, np.stack([rest_n[:, 2], rest_n[:, 1], -rest_n[:, 0]], axis=1))
+    for ang in rng.uniform(-np.pi, np.pi, size=6):
+        off = rng.normal(size=3)
+        p, q = orc.pose(first, count, pivot, [[np.cos(ang), np.sin(ang), *off]], rest_p, rest_n, z, z)
+        d0 = np.linalg.norm(rel, axis=1)
+        d1 = np.linalg.norm(p.astype(np.float64) - pivot[0] - off, axis=1)
+        np.testing.assert_allclose(d1, d0, rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(np.linalg.norm(q.astype(np.float64), axis=1), 1.0, atol=1e-6)
+        assert np.array_equal(p[:, 1], (np.float64(pivot[0, 1]) + rel[:, 1] + off[1]).astype(np.float32))
+
+
+@pytest.mark.parametrize("kind", ["atrium", "city"])
+def test_generated_scene_is_its_rest_mesh_posed(kind):
+    """A generated scene at frame f is its rest mesh posed by the oracle at the scene's own schedule for f; the rest
+    mesh does not depend on f, and only the objects' vertices move."""
+    s0 = rtgi.generate_scene(kind, seed=4, frame=0)
+    s7 = rtgi.generate_scene(kind, seed=4, frame=7)
+    assert len(s0.objects) == (16 if kind == "atrium" else 64)
+    assert np.array_equal(s0.rest_pos, s7.rest_pos) and np.array_equal(s0.objects, s7.objects)
+    o = s0.objects
+    assert np.all(o["first_vertex"][1:] == o["first_vertex"][:-1] + o["vertex_count"][:-1])  # contiguous
+    for s, f in ((s0, 0), (s7, 7)):
+        p, n = orc.pose_scene(s, s.poses_at(f))
+        assert np.array_equal(p, s.pos) and np.array_equal(n, s.nrm)
+    moved = np.any(s0.pos != s7.pos, axis=1)
+    lo, hi = int(o["first_vertex"][0]), int(o["first_vertex"][-1] + o["vertex_count"][-1])
+    assert moved[lo:hi].mean() > 0.9 and not moved[:lo].any() and not moved[hi:].any()
