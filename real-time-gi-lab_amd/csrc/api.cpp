@@ -1311,6 +1311,7 @@ struct rtgi_batch {
   hipEvent_t copied[2] = {nullptr, nullptr};
   int cur = 0;
   uint32_t* d_start = nullptr;
+  uint32_t* d_queue = nullptr;  // the work-queue counter of the batched build
   BatchPlan plan{};
   std::vector<uint64_t> sig;  // sizes the plan was made for
   bool use_graph = false, graph_valid = false;
@@ -1418,6 +1419,7 @@ void batch_free(rtgi_batch* b) {
     if (b->ev[i]) (void)hipEventDestroy(b->ev[i]);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_start) (void)hipFree(b->d_start);
+  if (b->d_queue) (void)hipFree(b->d_queue);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1430,6 +1432,7 @@ int batch_plan(rtgi_batch* b, const std::vector<uint64_t>& sig) {
   plan_blocks(b->worlds.data(), n, start.data(), b->plan);
   HIPCHK(hipMemcpy(b->d_start, start.data(), start.size() * 4, hipMemcpyHostToDevice));
   b->plan.w = b->d_desc;
+  b->plan.queue = b->d_queue;
   for (int l = 0; l < BL_COUNT; l++) b->plan.start[l] = b->d_start + (size_t)l * (n + 1);
   b->sig = sig;
   b->graph_valid = false;
@@ -1459,6 +1462,7 @@ extern "C" int rtgi_batch_create(rtgi_world* const* worlds, int32_t n, rtgi_batc
   if (!e && hipMalloc((void**)&b->d_desc, sizeof(TraceK) * n) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
   if (!e && hipMalloc((void**)&b->d_start, sizeof(uint32_t) * BL_COUNT * (n + 1)) != hipSuccess)
     e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+  if (!e && hipMalloc((void**)&b->d_queue, sizeof(uint32_t)) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
   for (int i = 0; i < 2 && !e; i++) {
     if (hipHostMalloc((void**)&b->h_desc[i], sizeof(TraceK) * n, hipHostMallocDefault) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");

@@ -261,6 +261,7 @@ __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   clear_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  if (b.queue && blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0u;  // the work queue of this update's build
 }
 
 // markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level
@@ -590,10 +591,69 @@ __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
 // With many worlds in a launch the chunk is 64 (fewest wavefronts, fullest lanes); one world alone takes the smallest
 // chunk that its launch's wavefronts cover, keeping the GPU busy. The OR of the bit and the max of the
 // representative commute, so the result does not depend on the schedule.
-__device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
+// Entries [base, base + chunk) of the listed (triangle, level) entries (up to 64, one per lane): a wavefront scan of
+// their cell counts lays the ranges end to end and the lanes take the cells in turn, each finding its entry by binary
+// search over the scan and its triangle by shuffles. Called with the same arguments by the whole wavefront.
+__device__ __forceinline__ void vox_big_chunk(const VoxK& k, uint32_t base, uint32_t chunk, uint32_t total,
+                                              uint32_t lane, uint32_t drops) {
+  const uint32_t e = lane < chunk ? base + lane : total;
+  int L = 0, n = 0;
+  uint32_t t = 0u;
+  Tri T{};
+  Range r{};
+  if (e < total) {
+    uint32_t idx;
+    big_entry(k, e, H_BIGVOX_L, L, idx);
+    t = k.big_vox[(size_t)L * k.big_cap + idx];
+    T = load_tri(k, t);
+    r = vox_range(k, T, L, false);
+    n = r.n.x * r.n.y * r.n.z;
+  }
+  int inc = n;  // inclusive scan of the cell counts; listed entries have n > 8, so the starts increase strictly
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(inc, o);
+    if ((int)lane >= o) inc += y;
+  }
+  const int start = inc - n, N = __shfl(inc, 63);
+  for (int j0 = 0; j0 < N; j0 += 64) {
+    const int j = j0 + (int)lane;
+    int own = 0;  // the last entry starting at or before cell j
+    for (int step = 32; step > 0; step >>= 1) {
+      int cand = own + step;
+      if (__shfl(start, cand) <= j) own = cand;
+    }
+    const int q = j - __shfl(start, own);
+    const int Lo = __shfl(L, own);
+    const uint32_t to = (uint32_t)__shfl((int)t, own);
+    const f3 A = mk3(__shfl(T.a.x, own), __shfl(T.a.y, own), __shfl(T.a.z, own));
+    const f3 B = mk3(__shfl(T.b.x, own), __shfl(T.b.y, own), __shfl(T.b.z, own));
+    const f3 Cc = mk3(__shfl(T.c.x, own), __shfl(T.c.y, own), __shfl(T.c.z, own));
+    Range ro;
+    ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
+    ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
+    if (j >= N) continue;
+    i3 c = cell_of(ro, q);
+    if (finer_built(k, Lo, c, drops)) continue;
+    float v = vox_size(k, Lo);
+    f3 p = region_lo(k, Lo) + (tof3(c) + splat3(0.5f)) * v;
+    if (!tri_box(p, splat3(0.5f * v * 1.0001f), A, B, Cc)) continue;
+    uint32_t ptr = k.ptr[ptr_index(k, Lo, shr3(c, 3))];
+    if (ptr >= PTR_MARK) continue;
+    uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+    atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
+    atomicMax(&k.rep[(size_t)ptr * 512u + bit], to + 1u);
+  }
+}
+
+__device__ __forceinline__ uint32_t big_total(const VoxK& k) {
   uint32_t total = 0;
   for (int L = 0; L < k.NL; L++) total += min(k.hdr[H_BIGVOX_L + L], k.big_cap);
-  uint32_t drops = k.hdr[H_BRICK_OVF];
+  return total;
+}
+
+__device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
+  const uint32_t total = big_total(k);
+  const uint32_t drops = k.hdr[H_BRICK_OVF];
   const uint32_t wave = (lb * BLK + threadIdx.x) >> 6, lane = threadIdx.x & 63u;
   const uint32_t waves = (nblk * BLK) >> 6;
   uint32_t chunk = 64u;
@@ -602,54 +662,58 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
     chunk = 1u;
     while (chunk < per && chunk < 64u) chunk <<= 1;
   }
-  for (uint32_t base = wave * chunk; base < total; base += waves * chunk) {  // the same rounds for the whole wavefront
-    const uint32_t e = lane < chunk ? base + lane : total;
-    int L = 0, n = 0;
-    uint32_t t = 0u;
-    Tri T{};
-    Range r{};
-    if (e < total) {
-      uint32_t idx;
-      big_entry(k, e, H_BIGVOX_L, L, idx);
-      t = k.big_vox[(size_t)L * k.big_cap + idx];
-      T = load_tri(k, t);
-      r = vox_range(k, T, L, false);
-      n = r.n.x * r.n.y * r.n.z;
-    }
-    int inc = n;  // inclusive scan of the cell counts; listed entries have n > 8, so the starts increase strictly
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(inc, o);
-      if ((int)lane >= o) inc += y;
-    }
-    const int start = inc - n, N = __shfl(inc, 63);
-    for (int j0 = 0; j0 < N; j0 += 64) {
-      const int j = j0 + (int)lane;
-      int own = 0;  // the last entry starting at or before cell j
-      for (int step = 32; step > 0; step >>= 1) {
-        int cand = own + step;
-        if (__shfl(start, cand) <= j) own = cand;
+  for (uint32_t base = wave * chunk; base < total; base += waves * chunk)  // the same rounds for the whole wavefront
+    vox_big_chunk(k, base, chunk, total, lane, drops);
+}
+
+// voxelizeBig over every world of a batch from one work queue: a world's listed entries are only counted on the
+// device (k_vox_small appends them), and their cells per entry vary by two orders of magnitude, so a split of the
+// blocks among the worlds fixed on the host leaves some wavefronts with many times the average work. Here each
+// wavefront takes chunks of VQ_CHUNK entries of any world from the batch's counter (one device-scope atomic per
+// chunk) until all are taken; chunk g belongs to the world whose prefix of chunk counts (built per workgroup in LDS
+// from the worlds' list lengths) covers g. The OR of the bit and the max of the representative commute, so the
+// result does not depend on which wavefront takes which chunk.
+constexpr uint32_t VQ_CHUNK = 16;
+constexpr int VQ_MAXW = 1024;  // worlds per batch the queue serves (larger batches use the static split)
+__device__ __forceinline__ void vox_big_queue(const BatchK& b) {
+  __shared__ uint32_t s_pre[VQ_MAXW + 1];  // chunks of worlds [0, w)
+  __shared__ uint32_t s_tot[VQ_MAXW];      // listed entries of world w
+  const uint32_t lane = threadIdx.x & 63u;
+  if (threadIdx.x < 64) {
+    uint32_t carry = 0;
+    for (int w0 = 0; w0 < b.n; w0 += 64) {
+      const int w = w0 + (int)lane;
+      const uint32_t tot = w < b.n ? big_total(b.w[w].vx) : 0u;
+      uint32_t inc = (tot + VQ_CHUNK - 1u) / VQ_CHUNK;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o);
+        if ((int)lane >= o) inc += y;
       }
-      const int q = j - __shfl(start, own);
-      const int Lo = __shfl(L, own);
-      const uint32_t to = (uint32_t)__shfl((int)t, own);
-      const f3 A = mk3(__shfl(T.a.x, own), __shfl(T.a.y, own), __shfl(T.a.z, own));
-      const f3 B = mk3(__shfl(T.b.x, own), __shfl(T.b.y, own), __shfl(T.b.z, own));
-      const f3 Cc = mk3(__shfl(T.c.x, own), __shfl(T.c.y, own), __shfl(T.c.z, own));
-      Range ro;
-      ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
-      ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
-      if (j >= N) continue;
-      i3 c = cell_of(ro, q);
-      if (finer_built(k, Lo, c, drops)) continue;
-      float v = vox_size(k, Lo);
-      f3 p = region_lo(k, Lo) + (tof3(c) + splat3(0.5f)) * v;
-      if (!tri_box(p, splat3(0.5f * v * 1.0001f), A, B, Cc)) continue;
-      uint32_t ptr = k.ptr[ptr_index(k, Lo, shr3(c, 3))];
-      if (ptr >= PTR_MARK) continue;
-      uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
-      atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
-      atomicMax(&k.rep[(size_t)ptr * 512u + bit], to + 1u);
+      if (w < b.n) {
+        s_tot[w] = tot;
+        s_pre[w + 1] = carry + inc;
+      }
+      carry += __shfl(inc, 63);
     }
+    if (lane == 0) s_pre[0] = 0u;
+  }
+  __syncthreads();
+  const uint32_t chunks = s_pre[b.n];
+  for (;;) {
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(b.queue, 1u);
+    g = (uint32_t)__shfl((int)g, 0);
+    if (g >= chunks) break;  // uniform over the wavefront: every wavefront ends here once the chunks are taken
+    int lo = 0, hi = b.n;  // the last world whose first chunk is at or before g
+    while (hi - lo > 1) {
+      const int m = (lo + hi) >> 1;
+      if (s_pre[m] <= g)
+        lo = m;
+      else
+        hi = m;
+    }
+    const VoxK& k = b.w[lo].vx;
+    vox_big_chunk(k, (g - s_pre[lo]) * VQ_CHUNK, VQ_CHUNK, s_tot[lo], lane, k.hdr[H_BRICK_OVF]);
   }
 }
 __global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x, false); }
@@ -658,6 +722,7 @@ __global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
   int w = batch_world(b, blk);
   vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
+__global__ void __launch_bounds__(BLK) k_vox_big_q(BatchK b) { vox_big_queue(b); }
 
 // Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
 // children's bits are ORed and their representatives maxed. One wave owns two occupancy words (64 consecutive voxels)
@@ -921,7 +986,9 @@ size_t vox_mark_lds_bytes(const VoxK& k) {
   return bytes <= 64 * 1024 ? bytes : 0;
 }
 
-static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0}; }
+static BatchK bk(const BatchPlan& p, int l, int arg = 0) {
+  return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0, p.queue};
+}
 
 int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -933,7 +1000,12 @@ int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_ALLOC_BRICKS]) k_alloc_bricks_b<<<p.grid[BL_ALLOC_BRICKS], 1024, 0, s>>>(bk(p, BL_ALLOC_BRICKS));
   if (p.grid[BL_INIT]) k_init_bricks_b<<<p.grid[BL_INIT], BLK, 0, s>>>(bk(p, BL_INIT));
   if (p.grid[BL_VOX_SMALL]) k_vox_small_b<<<p.grid[BL_VOX_SMALL], BLK, 0, s>>>(bk(p, BL_VOX_SMALL));
-  if (p.grid[BL_VOX_BIG]) k_vox_big_b<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+  if (p.grid[BL_VOX_BIG]) {
+    if (p.queue && p.n <= VQ_MAXW)  // persistent wavefronts over the batch's work queue (zeroed by k_clear_b)
+      k_vox_big_q<<<std::min<uint32_t>(p.grid[BL_VOX_BIG], 2048), BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+    else
+      k_vox_big_b<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+  }
   for (int L = 1; L < RTGI_MAX_LEVELS; L++)
     if (p.grid[BL_REDUCE + L - 1]) k_reduce_b<<<p.grid[BL_REDUCE + L - 1], BLK, 0, s>>>(bk(p, BL_REDUCE + L - 1, L));
   if (p.grid[BL_ALLOC_VOXELS]) k_alloc_voxels_b<<<p.grid[BL_ALLOC_VOXELS], 1024, 0, s>>>(bk(p, BL_ALLOC_VOXELS));

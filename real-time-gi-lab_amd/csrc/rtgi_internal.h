@@ -193,6 +193,7 @@ struct BatchK {
   int32_t n;
   int32_t arg;
   int32_t xcd;  // remap blocks so that each XCD runs one contiguous range (worlds of about equal block counts)
+  uint32_t* queue;  // the batch's work-queue counter (zeroed by k_clear_b, taken by k_vox_big_q), or null
 };
 
 // Launchers (each returns a hipError_t as int).
@@ -264,6 +265,7 @@ struct BatchPlan {
   int32_t n;
   const uint32_t* start[BL_COUNT];
   uint32_t grid[BL_COUNT];
+  uint32_t* queue;  // one device word per batch for the work-queue launches (null: static prefixes only)
   size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
   bool stats;       // some world of the batch counts tracer statistics
   bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps

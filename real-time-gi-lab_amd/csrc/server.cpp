@@ -75,6 +75,7 @@ struct Slot {
   hipEvent_t done = nullptr;
   TraceK* d_desc = nullptr;
   uint32_t* d_start = nullptr;
+  uint32_t* d_queue = nullptr;  // the slot's work-queue counter
   TraceK* h_desc = nullptr;    // pinned
   uint32_t* h_start = nullptr;  // pinned
   bool busy = false;
@@ -113,6 +114,7 @@ void slot_free(Slot& s) {
   if (s.h_start) (void)hipHostFree(s.h_start);
   if (s.d_desc) (void)hipFree(s.d_desc);
   if (s.d_start) (void)hipFree(s.d_start);
+  if (s.d_queue) (void)hipFree(s.d_queue);
   if (s.done) (void)hipEventDestroy(s.done);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s = Slot{};
@@ -293,6 +295,7 @@ void sched_loop(rtgi_server* s) {
     }
     if (!e) {
       plan.w = sl.d_desc;
+      plan.queue = sl.d_queue;
       for (int l = 0; l < BL_COUNT; l++) plan.start[l] = sl.d_start + (size_t)l * (n + 1);
       e = plan_launch(plan, sl.stream, nullptr);
     }
@@ -387,6 +390,7 @@ extern "C" int rtgi_server_create(const rtgi_server_params* p, rtgi_server** out
       e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
     if (!e && hipMalloc((void**)&sl.d_start, sizeof(uint32_t) * BL_COUNT * (q.max_batch + 1)) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+    if (!e && hipMalloc((void**)&sl.d_queue, sizeof(uint32_t)) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
     if (!e && hipHostMalloc((void**)&sl.h_desc, sizeof(TraceK) * q.max_batch, hipHostMallocDefault) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");
     if (!e && hipHostMalloc((void**)&sl.h_start, sizeof(uint32_t) * BL_COUNT * (q.max_batch + 1),
