@@ -1462,7 +1462,8 @@ extern "C" int rtgi_batch_create(rtgi_world* const* worlds, int32_t n, rtgi_batc
   if (!e && hipMalloc((void**)&b->d_desc, sizeof(TraceK) * n) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
   if (!e && hipMalloc((void**)&b->d_start, sizeof(uint32_t) * BL_COUNT * (n + 1)) != hipSuccess)
     e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
-  if (!e && hipMalloc((void**)&b->d_queue, sizeof(uint32_t)) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+  if (!e && hipMalloc((void**)&b->d_queue, sizeof(uint32_t) * RTGI_QUEUE_WORDS) != hipSuccess)
+    e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
   for (int i = 0; i < 2 && !e; i++) {
     if (hipHostMalloc((void**)&b->h_desc[i], sizeof(TraceK) * n, hipHostMallocDefault) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   clear_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
-  if (b.queue && blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0u;  // the work queue of this update's build
+  if (b.queue && blockIdx.x == 0 && threadIdx.x < VQ_LANES) b.queue[threadIdx.x * VQ_STRIDE] = 0u;  // the build's queue
 }
 
 // markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level
@@ -669,12 +669,15 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
 // voxelizeBig over every world of a batch from one work queue: a world's listed entries are only counted on the
 // device (k_vox_small appends them), and their cells per entry vary by two orders of magnitude, so a split of the
 // blocks among the worlds fixed on the host leaves some wavefronts with many times the average work. Here each
-// wavefront takes chunks of VQ_CHUNK entries of any world from the batch's counter (one device-scope atomic per
-// chunk) until all are taken; chunk g belongs to the world whose prefix of chunk counts (built per workgroup in LDS
-// from the worlds' list lengths) covers g. The OR of the bit and the max of the representative commute, so the
-// result does not depend on which wavefront takes which chunk.
+// wavefront takes chunks of VQ_CHUNK entries of any world until all are taken; chunk g belongs to the world whose
+// prefix of chunk counts (built per workgroup in LDS from the worlds' list lengths) covers g. The chunks are dealt to
+// VQ_LANES counters (chunk g to counter g mod VQ_LANES, taken by the workgroups of that residue, one device-scope
+// atomic per chunk): device-scope atomics on one address serialise at about 10 ns each, which a single counter
+// made the launch's limit on a batch of 64 cities. The OR of the bit and the max of the representative commute, so
+// the result does not depend on which wavefront takes which chunk.
 constexpr uint32_t VQ_CHUNK = 16;
 constexpr int VQ_MAXW = 1024;  // worlds per batch the queue serves (larger batches use the static split)
+// (the launch has at least VQ_LANES workgroups, so every class of chunks has wavefronts to take it)
 __device__ __forceinline__ void vox_big_queue(const BatchK& b) {
   __shared__ uint32_t s_pre[VQ_MAXW + 1];  // chunks of worlds [0, w)
   __shared__ uint32_t s_tot[VQ_MAXW];      // listed entries of world w
@@ -698,12 +701,13 @@ __device__ __forceinline__ void vox_big_queue(const BatchK& b) {
     if (lane == 0) s_pre[0] = 0u;
   }
   __syncthreads();
-  const uint32_t chunks = s_pre[b.n];
+  const uint32_t chunks = s_pre[b.n], cls = blockIdx.x % VQ_LANES;
+  uint32_t* const counter = b.queue + cls * VQ_STRIDE;
   for (;;) {
-    uint32_t g = 0;
-    if (lane == 0) g = atomicAdd(b.queue, 1u);
-    g = (uint32_t)__shfl((int)g, 0);
-    if (g >= chunks) break;  // uniform over the wavefront: every wavefront ends here once the chunks are taken
+    uint32_t i = 0;
+    if (lane == 0) i = atomicAdd(counter, 1u);
+    const uint32_t g = cls + VQ_LANES * (uint32_t)__shfl((int)i, 0);
+    if (g >= chunks) break;  // uniform over the wavefront: every wavefront ends here once its class is taken
     int lo = 0, hi = b.n;  // the last world whose first chunk is at or before g
     while (hi - lo > 1) {
       const int m = (lo + hi) >> 1;
@@ -1002,7 +1006,8 @@ int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_VOX_SMALL]) k_vox_small_b<<<p.grid[BL_VOX_SMALL], BLK, 0, s>>>(bk(p, BL_VOX_SMALL));
   if (p.grid[BL_VOX_BIG]) {
     if (p.queue && p.n <= VQ_MAXW)  // persistent wavefronts over the batch's work queue (zeroed by k_clear_b)
-      k_vox_big_q<<<std::min<uint32_t>(p.grid[BL_VOX_BIG], 2048), BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+      k_vox_big_q<<<std::max<uint32_t>(VQ_LANES, std::min<uint32_t>(p.grid[BL_VOX_BIG], 2048)), BLK, 0, s>>>(
+          bk(p, BL_VOX_BIG));
     else
       k_vox_big_b<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
   }

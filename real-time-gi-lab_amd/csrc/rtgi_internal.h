@@ -260,12 +260,16 @@ enum BatchLaunch : int {
   BL_SH = BL_TM + RTGI_MAX_CASCADES,
   BL_COUNT
 };
+// The work-queue counters of a batch: VQ_LANES counters VQ_STRIDE words (1 KiB) apart.
+constexpr uint32_t VQ_LANES = 8;
+constexpr uint32_t VQ_STRIDE = 256;
+constexpr size_t RTGI_QUEUE_WORDS = (size_t)VQ_LANES * VQ_STRIDE;
 struct BatchPlan {
   const TraceK* w;
   int32_t n;
   const uint32_t* start[BL_COUNT];
   uint32_t grid[BL_COUNT];
-  uint32_t* queue;  // one device word per batch for the work-queue launches (null: static prefixes only)
+  uint32_t* queue;  // RTGI_QUEUE_WORDS device words per batch for the work-queue launches (null: static only)
   size_t mark_lds;  // dynamic LDS of the mark kernels (the largest bitmap of the batch; 0: global marks)
   bool stats;       // some world of the batch counts tracer statistics
   bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps

@@ -390,7 +390,8 @@ extern "C" int rtgi_server_create(const rtgi_server_params* p, rtgi_server** out
       e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
     if (!e && hipMalloc((void**)&sl.d_start, sizeof(uint32_t) * BL_COUNT * (q.max_batch + 1)) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
-    if (!e && hipMalloc((void**)&sl.d_queue, sizeof(uint32_t)) != hipSuccess) e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
+    if (!e && hipMalloc((void**)&sl.d_queue, sizeof(uint32_t) * RTGI_QUEUE_WORDS) != hipSuccess)
+      e = rtgi_fail(RTGI_E_NOMEM, "hipMalloc");
     if (!e && hipHostMalloc((void**)&sl.h_desc, sizeof(TraceK) * q.max_batch, hipHostMallocDefault) != hipSuccess)
       e = rtgi_fail(RTGI_E_NOMEM, "hipHostMalloc");
     if (!e && hipHostMalloc((void**)&sl.h_start, sizeof(uint32_t) * BL_COUNT * (q.max_batch + 1),
