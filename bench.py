@@ -23,6 +23,9 @@ second) over a resident mix of --serve-worlds seeded worlds of different sizes (
 the request server (continuous batching under a budget of 128 MiB per world); p50/p99 latency from submission to
 completion, beside the same request trace run serially on the plain FP32 path.
 
+Gather at frame rate (the "gather_1080p" field; --gather-frames 0 skips it): 1920 x 1080 surface points in device
+memory gathered from world 1's field each frame (rtgi_world_gather_async), with and without the occlusion tests.
+
 Prints one JSON line. Only --gpus 1 is supported: this path runs on one GPU.
 """
 import argparse
@@ -292,6 +295,66 @@ class Motion:
             w.set_pose(self.poses[id(w)][f])
 
 
+def gather_bench(a, w, scene):
+    """The gather at frame rate (gatherWorld per frame, rc.ts:1218-1220): a 1920 x 1080 G-buffer's worth of surface
+    points (uniform on the scene's triangles inside the c0 probe grid, unit face normals) in device memory, in Morton
+    order of their c0 cell (neighbouring pixels of a G-buffer are neighbouring surface points), gathered from the
+    world's last field with rtgi_world_gather_async, without and with the occlusion tests. HIP events on the gather's
+    stream time --gather-frames frames of each mode, the modes alternating twice (the second round is reported)."""
+    from rtgi.hip import DeviceArray, Event, Stream
+
+    n = 1920 * 1080
+    rng = np.random.default_rng(11)
+    eye = np.asarray(w.frame.eye)
+    tri = scene.tri[:, :3].astype(np.int64)
+    cen = (scene.pos[tri[:, 0]] + scene.pos[tri[:, 1]] + scene.pos[tri[:, 2]]) / 3
+    inside = np.all(np.abs(cen - eye) < 30.0, axis=1)
+    t = rng.choice(np.nonzero(inside)[0], n)
+    A, B, Cc = scene.pos[tri[t, 0]], scene.pos[tri[t, 1]], scene.pos[tri[t, 2]]
+    u = rng.uniform(0, 1, (n, 2))
+    u[u.sum(1) > 1] = 1 - u[u.sum(1) > 1]
+    pos = (A + (B - A) * u[:, :1] + (Cc - A) * u[:, 1:]).astype(np.float32)
+    nr = np.cross(B - A, Cc - A).astype(np.float64)
+    nr = (nr / np.maximum(np.linalg.norm(nr, axis=1, keepdims=True), 1e-12)).astype(np.float32)
+    cell = np.clip(np.floor(pos - pos.min(0)).astype(np.int64), 0, 1023)  # 1 m c0 cells
+    key = np.zeros(n, np.int64)
+    for bit in range(10):
+        for ax in range(3):
+            key |= ((cell[:, ax] >> bit) & 1) << (3 * bit + ax)
+    order = np.argsort(key, kind="stable")
+    pos, nr = np.ascontiguousarray(pos[order]), np.ascontiguousarray(nr[order])
+    dp, dn = DeviceArray.from_numpy(pos), DeviceArray.from_numpy(nr)
+    di = DeviceArray((n, 3))
+    s = Stream()
+    out = {"workload": f"{n} surface points (a 1920 x 1080 G-buffer) inside the c0 probe grid of world 1, device "
+                       "arrays, rtgi_world_gather_async", "points": n,
+           "algorithmic_bytes_per_point": 36, "bytes_note": "positions and normals read, irradiance written (SH reads "
+                                                            "of the 8 surrounding probes are shared and mostly L2)"}
+    for occl in (False, True, False, True):
+        for _ in range(2):
+            w.gather_async(dp.ptr, dn.ptr, n, di.ptr, occlusion=occl, stream=s.handle)
+        s.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(s)
+        for _ in range(a.gather_frames):
+            w.gather_async(dp.ptr, dn.ptr, n, di.ptr, occlusion=occl, stream=s.handle)
+        e1.record(s)
+        s.synchronize()
+        ms = e0.elapsed_ms(e1) / a.gather_frames
+        key = "occlusion" if occl else "plain"
+        irr = di.numpy()
+        out[key] = {"ms_per_frame": round(ms, 4), "mpoints_per_s": round(n / ms / 1e3, 1),
+                    "gbps_algorithmic": round(36 * n / ms / 1e6, 1),
+                    "nonzero_fraction": round(float((np.abs(irr).sum(1) > 0).mean()), 4)}
+        log(f"gather {key}: {ms:.3f} ms per 1080p frame")
+        e0.destroy()
+        e1.destroy()
+    s.destroy()
+    for d in (dp, dn, di):
+        d.free()
+    return out
+
+
 def width_bench(a, worlds, widths=(8, 16, 32, 64), motion=None):
     """Latency against throughput at the headline configuration: the resident worlds updated in batches of width W,
     one batch at a time (a batch is launched when the previous one has completed, as a server with one slot launches
@@ -463,6 +526,7 @@ def main():
                     help="run with relocation, traced sun visibility and multi-bounce on (0: skip it)")
     ap.add_argument("--dyn-worlds", type=int, default=16, help="atrium worlds of the dynamic-layer run (0: skip)")
     ap.add_argument("--no-widths", dest="widths", action="store_false", help="skip the latency-vs-width curve")
+    ap.add_argument("--gather-frames", type=int, default=20, help="1080p gather frames per mode (0 skips the leg)")
     ap.add_argument("--static", dest="motion", action="store_false",
                     help="headline without motion (the buildings stay at the pose of frame 0)")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving run length (0: skip serving)")
@@ -613,6 +677,7 @@ def main():
 
     for b in batches:
         b.destroy()
+    gather = gather_bench(a, worlds[0], scenes[0]) if a.gather_frames > 0 else None
     widths = width_bench(a, worlds, motion=motion) if a.widths else None
     for w in worlds:
         w.destroy()
@@ -680,6 +745,7 @@ def main():
                                      "HBM roofline because nothing on the path is an MFMA contraction"},
         "cpu_baseline": cpu,
         "latency_vs_width": widths,
+        "gather_1080p": gather,
         "full_default": full,
         "dynamic_layer": dynamic,
         "batched_mix": mixed,

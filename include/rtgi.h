@@ -304,6 +304,14 @@ int rtgi_world_read_bounce_volume(rtgi_world* w, uint16_t* texels, int32_t origi
 #define RTGI_GATHER_OCCLUSION 1u
 int rtgi_world_gather(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags, float* irr);
 
+/* The gather at frame rate, on device arrays: the caller's G-buffer positions and normals (3 floats per point each)
+ * in, 3 floats of irradiance per point out, all in device memory. Same computation, results and errors as
+ * rtgi_world_gather, enqueued on `stream` (NULL: the world's own) after the world's last update and before its next
+ * one, without allocation or synchronisation. Replaces gatherWorld's dispatch per frame (rc.ts:1218-1220) where the
+ * G-buffer already lives on the GPU. The arrays must stay valid until the stream has run the gather. */
+int rtgi_world_gather_async(rtgi_world* w, const float* d_pos, const float* d_nrm, uint32_t n, uint32_t flags,
+                            float* d_irr, void* stream);
+
 /* rtgi_world_gather for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc). No counterpart in the reference. It allocates nothing: the points and the irradiance are
  * staged in a fixed device array of RTGI_GATHER_SCRATCH_POINTS points (9 floats each, 9 MiB) that each GPU the library

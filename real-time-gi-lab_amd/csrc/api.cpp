@@ -1165,6 +1165,30 @@ extern "C" int rtgi_world_gather(rtgi_world* w, const float* pos, const float* n
   return e;
 }
 
+extern "C" int rtgi_world_gather_async(rtgi_world* w, const float* d_pos, const float* d_nrm, uint32_t n,
+                                       uint32_t flags, float* d_irr, void* stream) {
+  if (!w || (n && (!d_pos || !d_nrm || !d_irr))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather_async: null argument");
+  if (flags & ~RTGI_GATHER_OCCLUSION) return rtgi_fail(RTGI_E_INVALID, "unknown gather flags");
+  if (w->rp.flags & RTGI_RC_NO_MERGE) return rtgi_fail(RTGI_E_STATE, "no irradiance field (RTGI_RC_NO_MERGE)");
+  if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "no scene or no frame");
+  if (n == 0) return RTGI_OK;
+  HIPCHK(hipSetDevice(w->device));
+  hipStream_t s = stream ? (hipStream_t)stream : w->stream;
+  if (s != w->stream) {  // after the world's own work and its last batched update
+    if (!w->order_ev) HIPCHK(hipEventCreateWithFlags(&w->order_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(w->order_ev, w->stream));
+    HIPCHK(hipStreamWaitEvent(s, w->order_ev, 0));
+  }
+  if (w->ext_done) HIPCHK(hipStreamWaitEvent(s, w->ext_done, 0));
+  int le = launch_gather(trace_args(w), d_pos, d_nrm, n, (flags & RTGI_GATHER_OCCLUSION) ? 1 : 0, d_irr, s);
+  if (le) return rtgi_fail(RTGI_E_HIP, "gather launch failed: %s", hipGetErrorString((hipError_t)le));
+  if (s != w->stream) {  // the world's next update (which rewrites the field) waits for the gather
+    HIPCHK(hipEventRecord(w->order_ev, s));
+    HIPCHK(hipStreamWaitEvent(w->stream, w->order_ev, 0));
+  }
+  return RTGI_OK;
+}
+
 extern "C" int rtgi_world_gather_noalloc(rtgi_world* w, const float* pos, const float* nrm, uint32_t n, uint32_t flags,
                                          float* irr) {
   if (!w || (n && (!pos || !nrm || !irr))) return rtgi_fail(RTGI_E_INVALID, "rtgi_world_gather_noalloc: null argument");
@@ -1252,6 +1276,7 @@ extern "C" int rtgi_world_destroy(rtgi_world* w) {
     if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
   for (int i = 0; i < 2; i++)
     if (w->pose_copied[i]) (void)hipEventDestroy(w->pose_copied[i]);
+  if (w->order_ev) (void)hipEventDestroy(w->order_ev);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete w;
   return RTGI_OK;

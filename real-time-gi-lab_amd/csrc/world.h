@@ -89,6 +89,7 @@ struct rtgi_world {
   hipEvent_t ev[5] = {};
   uint32_t last_stages = 0;
   hipEvent_t ext_done = nullptr;  // completion of the last batched update that included this world
+  hipEvent_t order_ev = nullptr;  // orders work on a caller's stream after the world's own (asynchronous gather)
   rtgi_tuning tuning{1, -1};
 };
 

@@ -457,6 +457,14 @@ class World:
                                       ptr(irr)))
         return irr
 
+    def gather_async(self, d_pos, d_nrm, n, d_irr, occlusion=False, stream=None):
+        """rtgi_world_gather_async: the gather on device arrays (addresses of n x 3 float32 positions, normals and
+        irradiance, e.g. torch tensors' data_ptr()), enqueued on a HIP stream (None: the world's own), no allocation
+        and no wait."""
+        check(lib().rtgi_world_gather_async(self._h, C.c_void_p(int(d_pos)), C.c_void_p(int(d_nrm)), int(n),
+                                            _L.GATHER_OCCLUSION if occlusion else 0, C.c_void_p(int(d_irr)),
+                                            C.c_void_p(int(stream)) if stream else None))
+
     def read_trace_stats(self):
         """Per cascade: world traces, DDA steps, traces at the 1024-step cap (rc.ts:232-235)."""
         st = _L.TraceStatsC()

@@ -145,6 +145,7 @@ SIGNATURES = {
     "rtgi_world_trace_rays": (C.c_int, [_P, C.c_int32, _P, _P, C.c_uint32, _P, _P]),
     "rtgi_world_gather": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "rtgi_world_gather_noalloc": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
+    "rtgi_world_gather_async": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     "rtgi_world_read_probes": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "rtgi_world_read_bounce_volume": (C.c_int, [_P, _P, _P]),
     "rtgi_world_set_dynamic": (C.c_int, [_P, C.c_uint32, C.c_int32]),

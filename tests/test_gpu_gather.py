@@ -75,3 +75,41 @@ def test_gather_wall_occlusion(require_gpu):
     sh_far[far] = 1e6
     np.testing.assert_allclose(o.gather(sh_far, p, n, occlusion=True), g, rtol=1e-6)
     w.destroy()
+
+
+def test_gather_async_on_device_arrays(require_gpu):
+    """rtgi_world_gather_async on device arrays and on a stream of the caller's equals the synchronous gather bit for
+    bit, with and without occlusion; a following update waits for it (the field it read is the old one)."""
+    from rtgi.hip import DeviceArray, Stream
+
+    scene, vox, rc, frame = rtgi.config("atrium")
+    pos, nrm = surface_points(scene, 50000, seed=3)
+    say("case: asynchronous gather on device arrays, atrium, 50,000 points")
+    w = rtgi.World(vox, rc)
+    try:
+        w.upload_scene(scene)
+        w.set_frame(frame)
+        w.encode()
+        w.sync()
+        dp, dn = DeviceArray.from_numpy(pos), DeviceArray.from_numpy(nrm)
+        s = Stream()
+        for occl in (False, True):
+            want = w.gather(pos, nrm, occlusion=occl)
+            di = DeviceArray.from_numpy(np.full((pos.shape[0], 3), -1.0, np.float32))
+            w.gather_async(dp.ptr, dn.ptr, pos.shape[0], di.ptr, occlusion=occl, stream=s.handle)
+            # the next update is ordered after the gather, so the gather read this update's field
+            w.update((frame.eye[0] + 3.0, frame.eye[1], frame.eye[2]))
+            w.encode()
+            s.synchronize()
+            got = di.numpy()
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), occl
+            w.update(frame.eye)  # the next round gathers from a field of the original eye again
+            w.encode()
+            w.sync()
+        di = DeviceArray.from_numpy(np.zeros((pos.shape[0], 3), np.float32))
+        w.gather_async(dp.ptr, dn.ptr, pos.shape[0], di.ptr)  # the world's own stream
+        w.sync()
+        assert np.count_nonzero(di.numpy()) > 0
+        s.destroy()
+    finally:
+        w.destroy()
