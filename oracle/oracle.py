@@ -42,7 +42,7 @@ class OrcRc(C.Structure):
                 ("vox_offset", C.c_float), ("lift", C.c_float), ("reloc", C.c_int32), ("ppos", C.c_void_p * 8),
                 ("pnrm", C.c_void_p * 8), ("pvis", C.c_void_p * 8), ("sunvis", C.c_void_p),
                 ("bounce", C.c_void_p), ("bn_prev", C.c_void_p), ("bn_prev_org", C.c_int32 * 3), ("dyn", C.c_void_p),
-                ("objects", C.c_int32)]
+                ("objects", C.c_int32), ("bf16", C.c_int32)]
 
 
 class OrcHist(C.Structure):
@@ -76,7 +76,7 @@ _SIG = {
     "orc_dyn_inject": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcDyn), _P, _P]),
     "orc_dyn_build": (C.c_int, [C.POINTER(OrcVox), C.POINTER(OrcRc), _P, _P, _P, _P, C.POINTER(OrcDyn)]),
     "orc_bounce_volume": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, C.c_uint32, _P, _P]),
-    "orc_bounce_inject": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, _P]),
+    "orc_bounce_inject": (C.c_int, [C.POINTER(OrcVox), _P, _P, _P, _P, C.c_int]),
     "orc_gather": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), _P, _P, _P, C.c_uint64, C.c_int, _P]),
     "orc_trace_rays": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_int, _P, _P, C.c_uint64, _P, _P,
                                  C.c_int]),
@@ -98,6 +98,8 @@ _SIG = {
     "orc_f2h": (C.c_uint16, [C.c_float]),
     "orc_h2f": (C.c_float, [C.c_uint16]),
     "orc_log2": (C.c_float, [C.c_float]),
+    "orc_f2bf": (C.c_uint16, [C.c_float]),
+    "orc_bf2f": (C.c_float, [C.c_uint16]),
     "orc_pose": (None, [C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 _lib = None
@@ -279,6 +281,7 @@ class Oracle:
         r.lift = getattr(rc, "lift", 0.25)
         r.reloc = 1 if getattr(rc, "relocate", False) else 0
         r.objects = 1 if getattr(rc, "objects", False) else 0
+        r.bf16 = 1 if getattr(rc, "bf16", False) else 0
         self._sun_trace = bool(getattr(rc, "sunTrace", False))
         self._bounce = bool(getattr(rc, "bounce", False))
         self._eye = np.asarray(frame.eye, np.float32)
@@ -361,7 +364,8 @@ class Oracle:
         assert self._bounce and self.buf is not None
         vol = np.ascontiguousarray(vol, np.uint16)
         org = np.ascontiguousarray(org, np.int32)
-        assert lib().orc_bounce_inject(C.byref(self.vp), _p(self.buf), _p(vol), _p(org), _p(self.bounce)) == 0
+        assert lib().orc_bounce_inject(C.byref(self.vp), _p(self.buf), _p(vol), _p(org), _p(self.bounce),
+                                       self.rc.bf16) == 0
 
     def relocate(self):
         """relocateVox for every cascade, then the parent visibility masks (linkParents with voxel relocation);

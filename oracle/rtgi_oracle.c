@@ -91,11 +91,31 @@ static void layout(const orc_vox* p, uint32_t* ptrOff, uint32_t* brickOff, uint3
   *plOff = *attrOff + (uint32_t)p->max_voxels * 2u;
 }
 
-size_t orc_vox_words(const orc_vox* p) {
+/* After the reference's buffer: the precision form's bf16 albedo, 2 words per attribute slot (r | g << 16,
+ * b | toon << 16), written by every build and read when orc_rc.bf16 is set. */
+static uint32_t albOff(const orc_vox* p) {
   uint32_t a, b, c, d;
   layout(p, &a, &b, &c, &d);
-  return (size_t)d + (size_t)p->max_voxels;
+  return d + (uint32_t)p->max_voxels;
 }
+
+size_t orc_vox_words(const orc_vox* p) { return (size_t)albOff(p) + 2u * (size_t)p->max_voxels; }
+
+/* f32 -> bf16, round to nearest even (the top 16 bits of the float, rounded); NaN stays a quiet NaN. */
+static uint16_t f2bf(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+static float bf2f(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+uint16_t orc_f2bf(float f) { return f2bf(f); }
+float orc_bf2f(uint16_t h) { return bf2f(h); }
 
 void orc_origins(double v0, int levels, int G, const double eye[3], int32_t out[8][3]) {
   for (int L = 0; L < levels; L++) {
@@ -458,6 +478,9 @@ static void resolveAttr(Ctx* X) {
     octEnc(n, &ox, &oy);
     X->vox[X->attrOff + ai * 2] = pack4x8unorm(alb);
     X->vox[X->attrOff + ai * 2 + 1] = pack2x16snorm(ox, oy);
+    const uint32_t ao = albOff(X->p);
+    X->vox[ao + ai * 2] = (uint32_t)f2bf(alb[0]) | ((uint32_t)f2bf(alb[1]) << 16);
+    X->vox[ao + ai * 2 + 1] = (uint32_t)f2bf(alb[2]) | ((uint32_t)f2bf(alb[3]) << 16);
     float pl = vdot(n, vsub(q, p));
     memcpy(&X->vox[X->plOff + ai], &pl, 4);
   }
@@ -662,7 +685,7 @@ static v3 dirOf(int di, int res) {
   return vnormalize(n);
 }
 
-typedef struct { const orc_vox* p; const uint32_t* vox; uint32_t ptrOff, brickOff, attrOff, plOff; } View;
+typedef struct { const orc_vox* p; const uint32_t* vox; uint32_t ptrOff, brickOff, attrOff, plOff, alOff; } View;
 
 static uint32_t vPtr(const View* W, uint32_t L, iv3 b) {
   uint32_t G = (uint32_t)W->p->G;
@@ -798,6 +821,10 @@ static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entr
     pn = W->vox[W->attrOff + ai * 2 + 1];
   }
   v3 alb = V3((float)(pa & 255u) / 255.0f, (float)((pa >> 8) & 255u) / 255.0f, (float)((pa >> 16) & 255u) / 255.0f);
+  if (rc->bf16) { /* precision form: the albedo as bf16 */
+    uint32_t w0 = ai != V_EMPTY ? W->vox[W->alOff + ai * 2] : 0u, w1 = ai != V_EMPTY ? W->vox[W->alOff + ai * 2 + 1] : 0u;
+    alb = V3(bf2f((uint16_t)(w0 & 0xffffu)), bf2f((uint16_t)(w0 >> 16)), bf2f((uint16_t)(w1 & 0xffffu)));
+  }
   float nx = fmx((float)(int16_t)(uint16_t)(pn & 0xffffu) / 32767.0f, -1.0f);
   float ny = fmx((float)(int16_t)(uint16_t)(pn >> 16) / 32767.0f, -1.0f);
   v3 n0 = V3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
@@ -837,7 +864,10 @@ static v3 shadeVoxel(const View* W, const orc_rc* rc, uint32_t ai, v3 d, v3 entr
     float sd = vdot(vsub(p, pc), n0);
     int back = fabsf(sd) > 0.05f * v ? !(sd > 0.0f) : vdot(n0, d) > 0.0f;
     const uint32_t* B = rc->bounce + (size_t)ai * 4 + (back ? 2 : 0);
-    e = V3(orc_h2f((uint16_t)(B[0] & 0xffffu)), orc_h2f((uint16_t)(B[0] >> 16)), orc_h2f((uint16_t)(B[1] & 0xffffu)));
+    if (rc->bf16)
+      e = V3(bf2f((uint16_t)(B[0] & 0xffffu)), bf2f((uint16_t)(B[0] >> 16)), bf2f((uint16_t)(B[1] & 0xffffu)));
+    else
+      e = V3(orc_h2f((uint16_t)(B[0] & 0xffffu)), orc_h2f((uint16_t)(B[0] >> 16)), orc_h2f((uint16_t)(B[1] & 0xffffu)));
   }
   v3 b = vmul(vdivs(alb, PI_F), e);
   v3 sd = V3(rc->sun_dir[0], rc->sun_dir[1], rc->sun_dir[2]);
@@ -1455,6 +1485,7 @@ static void makeView(View* W, const orc_vox* p, const uint32_t* vox) {
   W->p = p;
   W->vox = vox;
   layout(p, &W->ptrOff, &W->brickOff, &W->attrOff, &W->plOff);
+  W->alOff = albOff(p);
 }
 
 int orc_trace(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, uint64_t r0, uint64_t r1, float* rgba,
@@ -1899,7 +1930,7 @@ int orc_bounce_volume(const orc_vox* p, const uint32_t* vox, const orc_rc* rc0, 
 }
 
 int orc_bounce_inject(const orc_vox* p, const uint32_t* vox, const uint16_t* vol, const int32_t vorg[3],
-                      uint32_t* bounce) {
+                      uint32_t* bounce, int bf16) {
   View W;
   makeView(&W, p, vox);
   const float tol = 0.5f * p->v0;
@@ -1928,10 +1959,11 @@ int orc_bounce_inject(const orc_vox* p, const uint32_t* vox, const uint16_t* vol
       v3 ef = bnRead(vol, vorg, tol, vadd(pc, vscale(nf, off)), pc, nf);
       v3 eb = bnRead(vol, vorg, tol, vadd(pc, vscale(vneg(nf), off)), pc, vneg(nf));
       uint32_t* o = bounce + (size_t)ai * 4;
-      o[0] = (uint32_t)orc_f2h(ef.x) | ((uint32_t)orc_f2h(ef.y) << 16);
-      o[1] = (uint32_t)orc_f2h(ef.z) | ((uint32_t)orc_f2h(0.0f) << 16);
-      o[2] = (uint32_t)orc_f2h(eb.x) | ((uint32_t)orc_f2h(eb.y) << 16);
-      o[3] = (uint32_t)orc_f2h(eb.z) | ((uint32_t)orc_f2h(0.0f) << 16);
+      uint16_t (*enc)(float) = bf16 ? f2bf : orc_f2h; /* precision form: the injected light as bf16 */
+      o[0] = (uint32_t)enc(ef.x) | ((uint32_t)enc(ef.y) << 16);
+      o[1] = (uint32_t)enc(ef.z) | ((uint32_t)enc(0.0f) << 16);
+      o[2] = (uint32_t)enc(eb.x) | ((uint32_t)enc(eb.y) << 16);
+      o[3] = (uint32_t)enc(eb.z) | ((uint32_t)enc(0.0f) << 16);
     }
   }
   return 0;
@@ -2134,6 +2166,10 @@ static d3 shadeVoxelD(const View* W, const orc_rc* rc, uint32_t ai, d3 d, d3 ent
     pn = W->vox[W->attrOff + ai * 2 + 1];
   }
   d3 alb = D3((pa & 255u) / 255.0, ((pa >> 8) & 255u) / 255.0, ((pa >> 16) & 255u) / 255.0);
+  if (rc->bf16) { /* the stored bf16 albedo, widened exactly */
+    uint32_t w0 = ai != V_EMPTY ? W->vox[W->alOff + ai * 2] : 0u, w1 = ai != V_EMPTY ? W->vox[W->alOff + ai * 2 + 1] : 0u;
+    alb = D3(bf2f((uint16_t)(w0 & 0xffffu)), bf2f((uint16_t)(w0 >> 16)), bf2f((uint16_t)(w1 & 0xffffu)));
+  }
   double nx = fmax((int16_t)(uint16_t)(pn & 0xffffu) / 32767.0, -1.0), ny = fmax((int16_t)(uint16_t)(pn >> 16) / 32767.0, -1.0);
   d3 n = D3(nx, ny, 1 - fabs(nx) - fabs(ny));
   double q = n.z < 0 ? -n.z : 0;

@@ -55,6 +55,7 @@ typedef struct orc_rc {
   const orc_dyn* dyn;       /* the dynamic layer the trace reads in front of the static hit, or NULL */
   int32_t objects;          /* token obj (rc.ts:79-81): the layer splats albedo, receives the bounce volume's light
                              * (dynInject) and blocks the bounce volume's irradiance rays (BN_DYN) */
+  int32_t bf16;             /* precision form: voxel albedo and injected bounce light read as bf16 (RNE) */
 } orc_rc;
 
 /* Words of the reference-layout clipmap buffer for these parameters. */
@@ -111,8 +112,11 @@ int orc_bounce_volume(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, c
                       const int32_t prev_org[3], uint32_t frame, const int32_t org[3], uint16_t* cur);
 /* bnInject (bounce.ts:224-262): the irradiance of volume vol (origin org) on both sides of every occupied voxel ->
  * bounce (4 words per attribute slot: front rg, b, back rg, b as halves). */
+/* f32 <-> bf16 (round to nearest even; NaN stays NaN), the precision form's albedo and bounce-light storage. */
+uint16_t orc_f2bf(float f);
+float orc_bf2f(uint16_t h);
 int orc_bounce_inject(const orc_vox* p, const uint32_t* vox, const uint16_t* vol, const int32_t org[3],
-                      uint32_t* bounce);
+                      uint32_t* bounce, int bf16);
 /* relocateVox (rc.ts:413-472) for every probe of cascade c (grid order): pos, nrm (3 floats each). */
 int orc_relocate(const orc_vox* p, const uint32_t* vox, const orc_rc* rc, int c, float* pos, float* nrm);
 /* linkParents with voxel relocation (rc.ts:540-570): per probe of cascade c < N-1, bit k set when parent k exists and

@@ -124,6 +124,7 @@ VoxK vox_args(const rtgi_world* w) {
   k.attr = w->attr;
   k.plane = w->plane;
   k.vslot = w->vslot;
+  k.alb16 = w->alb16;
   k.big_vox = w->big_vox;
   k.big_mark = w->big_mark;
   k.big_cap = big_cap_for(w->ntris);
@@ -195,6 +196,7 @@ TraceK trace_args(const rtgi_world* w) {
   std::memcpy(t.omega0, w->omega0, sizeof(t.omega0));
   t.dirs = w->dirs;
   t.fp16 = (w->rp.flags & RTGI_RC_FP16) ? 1 : 0;
+  t.bf16 = (w->rp.flags & RTGI_RC_BF16) ? 1 : 0;
   t.objects = ((w->rp.flags & RTGI_RC_OBJECTS) && (w->rp.flags & RTGI_RC_BOUNCE) && w->dyn_hdr &&
                w->dyn_first < w->ntris) ? 1 : 0;
   t.no_merge = (w->rp.flags & RTGI_RC_NO_MERGE) ? 1 : 0;
@@ -309,6 +311,7 @@ void carve_fixed(rtgi_world* w, Carver& cv) {
   carve(cv, &w->pnrm, np * 4, (rp->flags & RTGI_RC_RELOCATE) != 0);
   carve(cv, &w->pvis, np, (rp->flags & RTGI_RC_RELOCATE) != 0);
   carve(cv, &w->stats, 3 * RTGI_MAX_CASCADES, (rp->flags & RTGI_RC_STATS) != 0);
+  carve(cv, &w->alb16, mv * 2, (rp->flags & RTGI_RC_BF16) != 0);
 }
 
 void objects_free(rtgi_world* w) {

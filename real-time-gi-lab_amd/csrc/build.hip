@@ -920,6 +920,7 @@ __device__ __forceinline__ void resolve_voxel(const VoxK& k, uint32_t ptr, uint3
   uint32_t pn = snorm16(px) | (snorm16(py) << 16);
   k.attr[ai * 2u] = pa;
   k.attr[ai * 2u + 1u] = pn;
+  if (k.alb16) *reinterpret_cast<uint2*>(k.alb16 + ai * 2u) = make_uint2(f2bf(alb[0]) | (f2bf(alb[1]) << 16), f2bf(alb[2]) | (f2bf(0.0f) << 16));
   k.plane[ai] = dot3(nn, q - p);
 }
 

@@ -24,6 +24,7 @@ enum : uint32_t {
 
 constexpr uint32_t PTR_EMPTY = RTGI_PTR_EMPTY;
 constexpr uint32_t PTR_DROP = RTGI_PTR_DROP;
+
 constexpr uint32_t PTR_MARK = RTGI_PTR_MARK;
 constexpr uint32_t ATTR_NONE = 0xffffffffu;
 
@@ -81,6 +82,7 @@ struct VoxK {
   uint32_t* attr;
   float* plane;
   uint32_t* vslot;     // [max_voxels] brick << 9 | bit of each attribute slot (written by the resolve), or null
+  uint32_t* alb16;     // [max_voxels][2] bf16 albedo of each attribute slot (precision form), or null
   uint32_t* big_vox;   // [levels][big_cap]
   uint32_t* big_mark;  // [levels][big_cap]
   const float* pos;
@@ -158,6 +160,7 @@ struct TraceK {
   float omega0[16];
   const DirEntry* dirs;  // all cascades' direction tables
   int32_t fp16;     // rad holds 4 halves per ray and sh halves (RNE at the reference's pack points), else f32
+  int32_t bf16;     // bounce words hold bf16 (precision form), else halves
   int32_t no_merge; // trace only: rad keeps the traced values, no averages, no SH
   int32_t reloc;    // voxel relocation: probe positions / normals / parent visibility from the arrays below
   float lift;       // relocated probes keep lift * spacing of free space behind them
@@ -165,7 +168,7 @@ struct TraceK {
   float* pnrm;      // [all cascades' probes] float4: probe normal (xyz; 0: none)
   uint32_t* pvis;   // [all cascades' probes] bit k: parent k of cascade c+1 exists and is not hidden by voxels
   float* sunvis;    // [max_voxels] traced sun visibility per attribute slot, or null (frame sun_vis)
-  uint32_t* bounce; // [max_voxels][4] bounce irradiance per slot: front rg, b, back rg, b (halves), or null
+  uint32_t* bounce; // [max_voxels][4] bounce irradiance per slot: front rg, b, back rg, b (halves or bf16), or null
   uint32_t* bn_cur;        // bounce volume this update writes (BN_WORDS), or null
   const uint32_t* bn_prev; // the previous update's volume: its hits' bounce term and the kept probes
   int32_t bn_org[3];       // probe coordinates of probe 0 of bn_cur (floor(eye / BN_CELL) - dims / 2)

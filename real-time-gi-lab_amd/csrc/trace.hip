@@ -135,6 +135,10 @@ __device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry, uint32_t
     pn = a.vx.attr[ai * 2u + 1u];
   }
   f3 alb = mk3(unorm8_to_f(pa & 255u), unorm8_to_f((pa >> 8) & 255u), unorm8_to_f((pa >> 16) & 255u));
+  if (a.vx.alb16) {  // precision form: the bf16 albedo
+    const uint2 q = ai != ATTR_NONE ? reinterpret_cast<const uint2*>(a.vx.alb16)[ai] : make_uint2(0u, 0u);
+    alb = mk3(bf2f(q.x), bf2f(q.x >> 16), bf2f(q.y));
+  }
   float nx = snorm16_to_f(pn & 0xffffu), ny = snorm16_to_f(pn >> 16);
   f3 n0 = mk3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
   float q = fmaxf(-n0.z, 0.0f);
@@ -158,7 +162,7 @@ __device__ f3 shade_voxel(const TraceK& a, uint32_t ai, f3 d, f3 entry, uint32_t
     bool back = fabsf(sd) > 0.05f * v ? !(sd > 0.0f) : dot3(n0, d) > 0.0f;
     const uint32_t* B = a.bounce + (size_t)ai * 4u + (back ? 2u : 0u);
     uint32_t w0 = B[0], w1 = B[1];
-    e = mk3(hlo(w0), hhi(w0), hlo(w1));
+    e = a.bf16 ? mk3(bf2f(w0), bf2f(w0 >> 16), bf2f(w1)) : mk3(hlo(w0), hhi(w0), hlo(w1));
   }
   f3 b = alb / PI_F * e;
   f3 sd = mk3(a.sun_dir[0], a.sun_dir[1], a.sun_dir[2]);
@@ -1391,6 +1395,10 @@ __device__ f3 shade_voxel_vol(const TraceK& a, uint32_t ai, f3 d, f3 entry, uint
     pn = a.vx.attr[ai * 2u + 1u];
   }
   f3 alb = mk3(unorm8_to_f(pa & 255u), unorm8_to_f((pa >> 8) & 255u), unorm8_to_f((pa >> 16) & 255u));
+  if (a.vx.alb16) {  // precision form: the bf16 albedo
+    const uint2 q = ai != ATTR_NONE ? reinterpret_cast<const uint2*>(a.vx.alb16)[ai] : make_uint2(0u, 0u);
+    alb = mk3(bf2f(q.x), bf2f(q.x >> 16), bf2f(q.y));
+  }
   float nx = snorm16_to_f(pn & 0xffffu), ny = snorm16_to_f(pn >> 16);
   f3 n0 = mk3(nx, ny, 1.0f - fabsf(nx) - fabsf(ny));
   float q = fmaxf(-n0.z, 0.0f);
@@ -1563,7 +1571,11 @@ __device__ __forceinline__ void bounce_body(const TraceK& a, uint32_t lb, uint32
     float off = L == 0 ? 0.0f : 0.5f * v;
     f3 ef = bn_read(a.bn_cur, a.bn_org, tol, p + nf * off, p, nf);
     f3 eb = bn_read(a.bn_cur, a.bn_org, tol, p + (-nf) * off, p, -nf);
-    reinterpret_cast<uint4*>(a.bounce)[ai] = make_uint4(h2(ef.x, ef.y), h2(ef.z, 0.0f), h2(eb.x, eb.y), h2(eb.z, 0.0f));
+    if (a.bf16)  // precision form: the injected light as bf16
+      reinterpret_cast<uint4*>(a.bounce)[ai] = make_uint4(f2bf(ef.x) | (f2bf(ef.y) << 16), f2bf(ef.z) | (f2bf(0.0f) << 16),
+                                                          f2bf(eb.x) | (f2bf(eb.y) << 16), f2bf(eb.z) | (f2bf(0.0f) << 16));
+    else
+      reinterpret_cast<uint4*>(a.bounce)[ai] = make_uint4(h2(ef.x, ef.y), h2(ef.z, 0.0f), h2(eb.x, eb.y), h2(eb.z, 0.0f));
   }
 }
 __global__ void __launch_bounds__(BLK) k_bounce(TraceK a) { bounce_body(a, blockIdx.x, gridDim.x); }

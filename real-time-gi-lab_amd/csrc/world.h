@@ -59,6 +59,7 @@ struct rtgi_world {
   uint32_t* pvis = nullptr;               // parent visibility masks (RTGI_RC_RELOCATE)
   float* sunvis = nullptr;                // traced sun visibility per attribute slot (RTGI_RC_SUN_TRACE)
   uint32_t* vslot = nullptr;              // brick << 9 | bit per attribute slot (RTGI_RC_SUN_TRACE or _BOUNCE)
+  uint32_t* alb16 = nullptr;              // bf16 albedo per attribute slot: r | g << 16, b | toon << 16 (RTGI_RC_BF16)
   // Dynamic coverage layer (rtgi_world_set_dynamic): triangles [dyn_first, ntris) go to it instead of the bricks.
   uint32_t dyn_first = 0xffffffffu;
   int32_t dyn_cap = 0;

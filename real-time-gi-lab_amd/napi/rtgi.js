@@ -22,7 +22,7 @@ const addon = require(path.join(__dirname, 'rtgi_napi.node'));
 const DEFAULT_VOX = { v0: 0.2, levels: 5, G: 32, maxBricks: 4608, maxVoxels: 524288, dynBricks: 512 };
 const DEFAULT_PARAMS = { cascades: 5, probes0: 64, s0: 0.2, kInterval: 1, levelMap: 1, voxScale: 0.25, voxOffset: 1,
   voxSkip: 0.5, resCap: 8, recordHits: 0, relocate: 0, lift: 0.25, sunTrace: 0, bounce: 0, objects: 0, fp16: 0,
-  history: 1 };
+  bf16: 0, history: 1 };
 const SUN_E = [9, 8.5, 7.8];
 const SKY = [0.45, 0.6, 0.9];
 

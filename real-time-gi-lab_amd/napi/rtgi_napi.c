@@ -96,7 +96,8 @@ static napi_value js_device_count(napi_env env, napi_callback_info info) {
 }
 
 /* createWorld(vox: {v0, levels, G, maxBricks, maxVoxels}, rc: {cascades, probes0, s0, kInterval, levelMap, voxScale,
- * voxOffset, voxSkip, resCap, recordHits, relocate, lift, sunTrace, bounce, objects, fp16}) -> handle */
+ * voxOffset, voxSkip, resCap, recordHits, relocate, lift, sunTrace, bounce, objects, fp16, bf16, history})
+ * -> handle */
 static napi_value js_create_world(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -129,6 +130,7 @@ static napi_value js_create_world(napi_env env, napi_callback_info info) {
   if (get_f64(env, argv[1], "bounce", &x) && x != 0) rp.flags |= RTGI_RC_BOUNCE;
   if (get_f64(env, argv[1], "objects", &x) && x != 0) rp.flags |= RTGI_RC_OBJECTS;
   if (get_f64(env, argv[1], "fp16", &x) && x != 0) rp.flags |= RTGI_RC_FP16;
+  if (get_f64(env, argv[1], "bf16", &x) && x != 0) rp.flags |= RTGI_RC_BF16;
   /* history: false (the reference's screen-probe setting, HIST_ON = !screen, rc.ts:1071) turns the c0 history off */
   if (get_f64(env, argv[1], "history", &x) && x == 0) rp.flags |= RTGI_RC_NO_HISTORY;
   rtgi_world* w = NULL;

@@ -73,6 +73,7 @@ class RcParams:
     objects: bool = False    # token obj (src/core/rc.ts:79-81): with bounce and a dynamic layer, the moving objects
                              # receive the bounce volume's light (dynInject) and block its irradiance rays (BN_DYN)
     history: bool = True     # c0 SH history of world probes (HIST_ON = !screen, src/core/rc.ts:1071; histBlend :784)
+    bf16: bool = False       # precision form: voxel albedo and injected bounce light stored as bf16
 
 
 DEFAULT_PARAMS = RcParams()
@@ -219,7 +220,7 @@ def _rc_c(rc):
              | (_L.RC_FP16 if rc.fp16 else 0) | (_L.RC_STATS if rc.stats else 0)
              | (_L.RC_RELOCATE if rc.relocate else 0) | (_L.RC_SUN_TRACE if rc.sunTrace else 0)
              | (_L.RC_BOUNCE if rc.bounce else 0) | (_L.RC_OBJECTS if rc.objects else 0)
-             | (0 if rc.history else _L.RC_NO_HISTORY))
+             | (0 if rc.history else _L.RC_NO_HISTORY) | (_L.RC_BF16 if rc.bf16 else 0))
     return _L.RcParamsC(rc.cascades, rc.probes0, rc.s0, rc.kInterval, rc.levelMap, rc.voxScale, rc.voxOffset,
                         rc.voxSkip, rc.resCap, flags, rc.lift)
 

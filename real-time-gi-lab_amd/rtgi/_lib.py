@@ -28,6 +28,7 @@ RC_SUN_TRACE = 32
 RC_BOUNCE = 64
 RC_OBJECTS = 128
 RC_NO_HISTORY = 256
+RC_BF16 = 512
 BN_DIMS = (32, 16, 32)  # bounce volume probes per axis (x, y, z); 1.2 m apart
 GATHER_OCCLUSION = 1
 STAGE_VOX, STAGE_TRACE, STAGE_MERGE, STAGE_SH, STAGE_ALL = 1, 2, 4, 8, 15

@@ -68,14 +68,17 @@ def test_bounce_matches_oracle(require_gpu, case):
         scene, vox, rc, f1 = rtgi.config(case)
     f2 = moved(f1, (2.5, 0.0, -1.7))
     frames = [f1, f2, f2]
-    for fp16, reloc, sun in ((False, False, False), (True, True, True)):
-        rcf = dataclasses.replace(rc, bounce=True, relocate=reloc, sunTrace=sun, fp16=fp16, recordHits=True)
+    # the last form keeps the albedo and the injected bounce light as bf16 (the precision form of BASELINE cfg3)
+    for fp16, reloc, sun, bf16 in ((False, False, False, False), (True, True, True, False), (True, False, False, True)):
+        rcf = dataclasses.replace(rc, bounce=True, relocate=reloc, sunTrace=sun, fp16=fp16, recordHits=True,
+                                  bf16=bf16)
         w = rtgi.World(vox, rcf)
         w.upload_scene(scene)
         state = None
         hist = History(rc.probes0)
         for k, f in enumerate(frames):
-            say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, sunTrace={sun}, update {k + 1}")
+            say(f"case: multi-bounce, {case}, fp16={fp16}, relocate={reloc}, sunTrace={sun}, bf16={bf16}, "
+                f"update {k + 1}")
             w.set_frame(f)
             w.encode()
             w.sync()

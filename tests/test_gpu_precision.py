@@ -2,8 +2,9 @@
 over the same clipmap), on the atrium configuration (50k triangles, 4 x 64^3 clipmap, 32^3 c0 probes, 4 cascades).
 
 Per cascade, 10k seeded rays (origins at probe positions drawn uniformly from the cascade's grid, directions uniform
-on the sphere) go through the GPU ray query, in the plain FP32 form and in the precision form (FP16 radiance
-storage, FP32 arithmetic); the irradiance (SH) of all c0 probes comes from a full update. Rays that pass a voxel
+on the sphere) go through the GPU ray query, in the plain FP32 form and in the precision forms (FP16 radiance
+storage; FP16 radiance with BF16 albedo; FP32 arithmetic throughout); the irradiance (SH) of all c0 probes comes from a
+full update. Rays that pass a voxel
 edge, corner, window face or skip distance within 1e-6 (relative) are ties: their first voxel is not fixed by the
 arithmetic's neighbourhood, and they are counted apart.
 
@@ -42,8 +43,11 @@ def rel_err(x, ref, floor=1e-3):
     return np.abs(x - ref) / np.maximum(np.abs(ref), floor)
 
 
-def world(fp16):
-    scene, vox, rc, frame = rtgi.config("atrium", fp16=fp16)
+FORMS = {"fp32": dict(fp16=False), "fp16": dict(fp16=True), "fp16+bf16": dict(fp16=True, bf16=True)}
+
+
+def world(form):
+    scene, vox, rc, frame = rtgi.config("atrium", **FORMS[form])
     w = rtgi.World(vox, rc)
     w.upload_scene(scene)
     w.set_frame(frame)
@@ -54,10 +58,11 @@ def world(fp16):
     return w, o, rc
 
 
-@pytest.mark.parametrize("fp16", [False, True])
-def test_ray_query_matches_oracle(require_gpu, fp16):
-    say(f"case: GPU ray query vs f32 oracle, atrium, fp16={fp16}")
-    w, o, rc = world(fp16)
+@pytest.mark.parametrize("form", list(FORMS))
+def test_ray_query_matches_oracle(require_gpu, form):
+    say(f"case: GPU ray query vs f32 oracle, atrium, {form}")
+    fp16 = FORMS[form]["fp16"]
+    w, o, rc = world(form)
     G0 = w.layout()["grid_origin"]
     for c in range(rc.cascades):
         org, d = sphere_rays(rc, G0, c, 4000, seed=100 + c)
@@ -72,11 +77,12 @@ def test_ray_query_matches_oracle(require_gpu, fp16):
 
 
 def test_precision_report(require_gpu):
-    say(f"case: precision report, atrium, {NRAYS} rays per cascade, FP32 and FP16 forms")
-    report = {"config": "atrium (cfg1 scene), 10k seeded sphere rays per cascade; FP64 oracle over the same clipmap",
-              "forms": {}}
-    for fp16 in (False, True):
-        w, o, rc = world(fp16)
+    say(f"case: precision report, atrium, {NRAYS} rays per cascade, FP32, FP16 and FP16 + BF16 albedo forms")
+    report = {"config": "atrium (cfg1 scene), 10k seeded sphere rays per cascade; FP64 oracle over the same clipmap "
+                        "and the same stored albedo", "forms": {}}
+    for name in FORMS:
+        fp16 = FORMS[name]["fp16"]
+        w, o, rc = world(name)
         G0 = w.layout()["grid_origin"]
         bound = 1e-3 if fp16 else 1e-4
         form = {"cascades": []}
@@ -101,7 +107,7 @@ def test_precision_report(require_gpu):
         rel = np.abs(sh - sh64).max(axis=1) / np.maximum(np.abs(sh64).max(axis=1), 1e-6)
         form["sh_probe_rel_err"] = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
                                     "max": float(rel.max()), "rays_tied": float(tie.mean())}
-        report["forms"]["fp16" if fp16 else "fp32"] = form
+        report["forms"][name] = form
         w.destroy()
     say(json.dumps(report))
     path = os.environ.get("RTGI_PRECISION_REPORT")
@@ -110,16 +116,17 @@ def test_precision_report(require_gpu):
             json.dump(report, f, indent=1)
 
 
-@pytest.mark.parametrize("case,fp16", [("atrium", False), ("atrium", True), ("city", False), ("city", True)])
-def test_irradiance_error_vs_fp64(require_gpu, case, fp16):
+@pytest.mark.parametrize("case,form", [("atrium", "fp32"), ("atrium", "fp16"), ("atrium", "fp16+bf16"), ("city", "fp32"),
+                                       ("city", "fp16"), ("city", "fp16+bf16")])
+def test_irradiance_error_vs_fp64(require_gpu, case, form):
     """Irradiance (the SH of every c0 probe) against FP64 trace + merge + SH over the same clipmap, BASELINE's bound
     1e-3 relative per probe (the largest coefficient difference over the probe's largest coefficient). The FP64 rays
     take the f32 walk's first-hit voxel wherever the two walks may disagree (tie rays and rays whose FP64 first hit
     differs: there both voxels are right answers of the reference algorithm, and the f32 one is what the reference's
     own f32 arithmetic picks); every other ray is the FP64 walk's own. The tie-inclusive error (the plain FP64 walk
     everywhere) is reported beside it."""
-    scene, vox, rc, frame = rtgi.config(case, fp16=fp16)
-    say(f"case: irradiance vs FP64, {case}, fp16={fp16}")
+    scene, vox, rc, frame = rtgi.config(case, **FORMS[form])
+    say(f"case: irradiance vs FP64, {case}, {form}")
     w = rtgi.World(vox, rc)
     try:
         w.upload_scene(scene)

@@ -1212,3 +1212,38 @@ def test_generated_scene_is_its_rest_mesh_posed(kind):
     moved = np.any(s0.pos != s7.pos, axis=1)
     lo, hi = int(o["first_vertex"][0]), int(o["first_vertex"][-1] + o["vertex_count"][-1])
     assert moved[lo:hi].mean() > 0.9 and not moved[:lo].any() and not moved[hi:].any()
+
+
+def test_bf16_rounding_is_nearest_even():
+    """The precision form's f32 -> bf16 (orc_f2bf): on random floats of every exponent, edge patterns and exact ties,
+    the result is the nearer of the two bf16 values bracketing the input (found by truncation, independent of the
+    rounding formula), the even one on a tie; infinities stay, NaN stays NaN; bf16 -> f32 widens exactly."""
+    rng = np.random.default_rng(23)
+    bits = rng.integers(0, 2 ** 32, 20000, dtype=np.uint64).astype(np.uint32)
+    ties = (rng.integers(0, 2 ** 16, 2000, dtype=np.uint64).astype(np.uint32) << 16) | np.uint32(0x8000)
+    edge = np.array([0, 0x80000000, 1, 0x7F7FFFFF, 0xFF7FFFFF, 0x7F800000, 0xFF800000, 0x00807FFF, 0x3F808000,
+                     0x3F818000, 0x7F7F8000], np.uint32)
+    for u in np.concatenate([bits, ties, edge]).tolist():
+        f = np.array([u], np.uint32).view(np.float32)[0]
+        h = orc.lib().orc_f2bf(float(f))
+        back = np.float32(orc.lib().orc_bf2f(h))
+        assert np.array_equal(np.array([back], np.float32).view(np.uint32), np.array([h << 16], np.uint32))
+        if np.isnan(f):
+            assert np.isnan(back)
+            continue
+        lo = np.array([u & 0xFFFF0000], np.uint32).view(np.float32)[0]  # truncation toward zero
+        hi_bits = (u & 0xFFFF0000) + 0x10000
+        if (u & 0xFFFF) == 0:
+            assert h << 16 == u
+            continue
+        # the next bf16 value away from zero, as an unbounded binary float (past the largest finite it is 2^128,
+        # which IEEE rounding treats as the overflow threshold)
+        E = (u >> 23) & 0xFF
+        ulp = 2.0 ** ((E if E else 1) - 127 - 7)
+        hi_val = abs(float(lo)) + ulp
+        dl, dh = abs(abs(float(f)) - abs(float(lo))), abs(hi_val - abs(float(f)))
+        if (u & 0xFFFF) == 0x8000:  # exact tie in the low half: the even one of the two
+            want = u & 0xFFFF0000 if ((u >> 16) & 1) == 0 else hi_bits & 0xFFFFFFFF
+        else:
+            want = u & 0xFFFF0000 if dl < dh else hi_bits & 0xFFFFFFFF
+        assert h == want >> 16, hex(u)
