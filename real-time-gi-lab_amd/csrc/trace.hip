@@ -249,8 +249,18 @@ __device__ Hit trace_vox(const TraceK& a, f3 o, f3 d, f3 inv, float t0, float t1
       if (!inside3(nb, G)) break;
       t = te;
       tIn = te;
+      // The next iteration reads nb's pointer; read it here. When nb is empty too, that iteration jumps again, and a
+      // jump reads only the brick of c, t's successor and the ray: the entry cell and tMax of nb are never used, so
+      // they are formed only where the chain of empty bricks ends (the same values the step-by-step loop forms).
+      ptr = buf_u32(ptr_r, ptr_base + (((uint32_t)nb.z * (uint32_t)G + (uint32_t)nb.y) * (uint32_t)G + (uint32_t)nb.x) * 4u);
+      if (ptr >= PTR_MARK && ptr != PTR_DROP) {
+        c = nb * 8;
+        lc = c;
+        continue;
+      }
       c = clampi3(cell_at(o + d * t - lo, v, iv, p2), nb * 8, nb * 8 + 7);
       tMax = ((tof3(c) + sp) * v + lo - o) * inv;
+      lc = c;
       continue;
     }
     bool occupied;
