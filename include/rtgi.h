@@ -183,6 +183,32 @@ int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const float* nrm, u
 /* Replaces the posed-vertex writeBuffer (src/core/rcrun.ts:213-214): vertices [first, first+count). */
 int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_t count, const float* pos, const float* nrm);
 
+/* ---- Textured and alpha-tested surfaces (surfaceAt, src/core/voxel.ts:325-336; shadeSample, dynlayer.ts:451-452) ---- */
+
+/* The surface words of one material (struct Material of uploadScene, src/core/scene_gpu.ts:95-103). */
+typedef struct rtgi_surface_material {
+  uint32_t base_color_layer;  /* Material.layers.x: 0 the white default layer, k >= 1 texel layer k - 1 below */
+  uint32_t flags;             /* Material.layers.w: RTGI_SURF_ALPHA_MASK, RTGI_SURF_TOON (other bits ignored) */
+  float alpha_cutoff;         /* Material.params.z */
+} rtgi_surface_material;
+#define RTGI_SURF_ALPHA_MASK 1u /* voxels (and dynamic-layer samples) whose sampled alpha is below the cutoff are empty */
+#define RTGI_SURF_TOON 4u       /* the toon flag, stored in the albedo word's alpha (pack4x8unorm(albedo, toon)) */
+
+/* Base-colour textures, per-vertex UVs and per-material surface words of the uploaded scene, the inputs surfaceAt
+ * samples (uploadScene's baseColorImage array, uvs buffer and material words, src/core/scene_gpu.ts:56-106; the
+ * voxel passes' sampler, voxel.ts:793: linear min/mag/mip filter, repeat addressing). texels: `layers` image layers
+ * of size x size RGBA8 texels (rgb sRGB-encoded, alpha linear; row-major, 4 bytes per texel; the array's layers 1..layers,
+ * its layer 0 being the white default); size a power of two <= 4096. Mip levels are built on the device as the
+ * reference's generateMips builds them (each texel of level m the bilinear sample of level m-1 at its centre: the
+ * mean of 2 x 2 texels in linear space, stored sRGB-encoded, scene_gpu.ts:38-54). surfaceAt then samples level
+ * min(3 + L, last) bilinearly at the UV interpolated at the voxel centre's closest point: resolveAttr stores
+ * texture x baseColorFactor as the albedo and the toon flag as its alpha; with RTGI_SURF_ALPHA_MASK, voxBits leaves
+ * a voxel empty whose sampled alpha x factor alpha is below the cutoff; the dynamic layer's samples take the same
+ * albedo and alpha test. uvs: nverts x 2 floats (nverts and nmat those of the uploaded scene). uvs NULL: untextured
+ * again (every material samples white and none is masked). Allocates; call between updates. */
+int rtgi_world_set_surface(rtgi_world* w, const float* uvs, uint32_t nverts, const rtgi_surface_material* mats,
+                           uint32_t nmat, const uint8_t* texels, uint32_t size, uint32_t layers);
+
 /* ---- Moving objects, posed on the device (DynamicSpec and posedVertices, src/core/scenes.ts:41-53, 80-102) ---- */
 
 /* A rigid object: vertices [first_vertex, first_vertex + vertex_count) turning about the vertical axis through pivot

@@ -51,6 +51,11 @@ class OrcHist(C.Structure):
                 ("G0", (C.c_int32 * 3) * 2), ("n", C.c_int32)]
 
 
+class OrcSurface(C.Structure):
+    _fields_ = [("tex", C.c_void_p), ("uv", C.c_void_p), ("mext", C.c_void_p), ("lut", C.c_void_p),
+                ("size_log2", C.c_uint32), ("mips", C.c_uint32), ("layer_words", C.c_uint32), ("layers", C.c_uint32)]
+
+
 class OrcDyn(C.Structure):
     _fields_ = [("cap", C.c_int32), ("first_tri", C.c_uint32), ("end_tri", C.c_uint32), ("hdr", C.c_void_p),
                 ("ptr", C.c_void_p), ("bits", C.c_void_p), ("meta", C.c_void_p), ("mask", C.c_void_p),
@@ -91,6 +96,10 @@ _SIG = {
     "orc_trace_f64_take": (C.c_int, [C.POINTER(OrcVox), _P, C.POINTER(OrcRc), C.c_uint64, C.c_uint64, _P, _P, _P,
                                      C.c_double, C.c_int]),
     "orc_tri_box": (C.c_int, [_P, _P, _P, _P, _P]),
+    "orc_set_surface": (None, [_P]),
+    "orc_srgb_tables": (None, [_P]),
+    "orc_surf_mips": (None, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "orc_surf_sample": (None, [_P, C.c_uint32, C.c_float, C.c_float, C.c_uint32, _P]),
     "orc_closest_bary": (None, [_P, _P, _P, _P, _P]),
     "orc_pack4x8unorm": (C.c_uint32, [_P]),
     "orc_pack2x16snorm": (C.c_uint32, [C.c_float, C.c_float]),
@@ -257,6 +266,42 @@ def origins(v0, levels, G, eye):
     return out[:levels]
 
 
+def srgb_tables():
+    lut = np.zeros(511, np.float32)
+    lib().orc_srgb_tables(_p(lut))
+    return lut
+
+
+class SurfaceData:
+    """A scene's textured surface in the oracle's form (orc_surface): the mip chain of its images, its UVs, material
+    words and the sRGB tables. `struct` is what orc_set_surface takes (valid while this object lives)."""
+
+    def __init__(self, surface):
+        img = np.ascontiguousarray(surface.images, np.uint8)
+        layers, size = int(img.shape[0]), int(img.shape[1])
+        sl = int(size).bit_length() - 1
+        self.per = sum(1 << (2 * (sl - m)) for m in range(sl + 1))
+        self.lut = srgb_tables()
+        self.tex = np.zeros(max(1, layers * self.per), np.uint32)
+        lib().orc_surf_mips(_p(img) if layers else None, size, layers, _p(self.lut), _p(self.tex))
+        self.uv = np.ascontiguousarray(surface.uvs, np.float32).reshape(-1, 2)
+        n = len(surface.layer)
+        self.mext = np.zeros((max(1, n), 4), np.uint32)
+        self.mext[:n, 0] = np.asarray(surface.layer, np.uint32)
+        self.mext[:n, 1] = np.asarray(surface.flags, np.uint32)
+        self.mext[:n, 2] = np.asarray(surface.cutoff, np.float32).view(np.uint32)
+        s = OrcSurface()
+        s.tex, s.uv, s.mext, s.lut = (self.tex.ctypes.data, self.uv.ctypes.data, self.mext.ctypes.data,
+                                      self.lut.ctypes.data)
+        s.size_log2, s.mips, s.layer_words, s.layers = sl, sl + 1, self.per, layers
+        self.struct = s
+
+    def sample(self, layer, u, v, lod):
+        out = np.zeros(4, np.float32)
+        lib().orc_surf_sample(C.byref(self.struct), layer, u, v, lod, _p(out))
+        return out
+
+
 class Oracle:
     """CPU restatement of one world: build -> canonical clipmap; trace -> merge -> SH (f32), and the FP64 tracer."""
 
@@ -321,15 +366,21 @@ class Oracle:
                        f32(scene.materials))
         pos, nrm, tri, mat = self._scene
         nstatic = tri.shape[0] if dyn is None else min(tri.shape[0], dyn.first_tri)
-        rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), nstatic, _p(mat), _p(self.buf),
-                             _p(self.rep))
-        assert rc == 0
-        if dyn is not None:
-            self._dyn_struct = dyn.struct(tri.shape[0])
-            assert lib().orc_dyn_build(C.byref(self.vp), C.byref(self.rc), _p(pos), _p(nrm), _p(tri), _p(mat),
-                                       C.byref(self._dyn_struct)) == 0
-            self.rc.dyn = C.addressof(self._dyn_struct)
-            self.dyn = dyn
+        surf = getattr(scene, "surface", None)
+        self.surface = SurfaceData(surf) if surf is not None else None
+        lib().orc_set_surface(C.byref(self.surface.struct) if self.surface is not None else None)
+        try:
+            rc = lib().orc_build(C.byref(self.vp), _p(pos), _p(nrm), _p(tri), nstatic, _p(mat), _p(self.buf),
+                                 _p(self.rep))
+            assert rc == 0
+            if dyn is not None:
+                self._dyn_struct = dyn.struct(tri.shape[0])
+                assert lib().orc_dyn_build(C.byref(self.vp), C.byref(self.rc), _p(pos), _p(nrm), _p(tri), _p(mat),
+                                           C.byref(self._dyn_struct)) == 0
+                self.rc.dyn = C.addressof(self._dyn_struct)
+                self.dyn = dyn
+        finally:
+            lib().orc_set_surface(None)
         if self._sun_trace:
             self.sunvis = np.zeros(self.vox.maxVoxels, np.float32)
             assert lib().orc_sun(C.byref(self.vp), _p(self.buf), C.byref(self.rc), _p(self.sunvis)) == 0

@@ -185,6 +185,116 @@ static v3 closestBary(v3 p, v3 a, v3 b, v3 c) {
   float v = vb * dn, w = vc * dn;
   return V3(1.0f - v - w, v, w);
 }
+
+/* ------------------------------------------------------------------------------------------------------------ */
+/* Textured surfaces: textureSampleLevel(baseColorTex, samp, uv, layer, 3 + L) with the voxel passes' sampler     */
+/* (linear min/mag/mip filtering, repeat addressing, voxel.ts:793) over a mip chain built as the reference's       */
+/* generateMips builds it (scene_gpu.ts:38-54); rgb decoded from sRGB before filtering (rgba8unorm-srgb).          */
+
+static const orc_surface* g_surf; /* NULL: untextured (white layer 0 everywhere, nothing masked) */
+void orc_set_surface(const orc_surface* s) { g_surf = s; }
+
+/* IEC 61966-2-1 decoding of an encoded value c in [0, 1]. */
+static double srgbDecode(double c) { return c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4); }
+void orc_srgb_tables(float lut[511]) {
+  for (int c = 0; c < 256; c++) lut[c] = (float)srgbDecode(c / 255.0);
+  for (int k = 0; k < 255; k++) lut[256 + k] = (float)srgbDecode((k + 0.5) / 255.0);
+}
+static void texelAt(const orc_surface* s, uint32_t base, uint32_t w, uint32_t x, uint32_t y, float t[4]) {
+  uint32_t q = s->tex[base + y * w + x];
+  t[0] = s->lut[q & 255u];
+  t[1] = s->lut[(q >> 8) & 255u];
+  t[2] = s->lut[(q >> 16) & 255u];
+  t[3] = (float)(q >> 24) / 255.0f;
+}
+static uint32_t mipOff(uint32_t sl, uint32_t m) {
+  uint32_t off = 0;
+  for (uint32_t j = 0; j < m; j++) off += 1u << (2u * (sl - j));
+  return off;
+}
+/* Nearest code of a linear value in the sRGB encoding: the thresholds at or below it, counted one by one. */
+static uint32_t srgbEncode(const float* lut, float x) {
+  uint32_t c = 0;
+  for (int k = 0; k < 255; k++)
+    if (lut[256 + k] <= x) c++;
+  return c;
+}
+void orc_surf_mips(const uint8_t* texels, uint32_t size, uint32_t layers, const float* lut, uint32_t* out) {
+  uint32_t sl = 0;
+  while ((1u << sl) < size) sl++;
+  uint32_t per = mipOff(sl, sl + 1);
+  orc_surface S = {out, NULL, NULL, lut, sl, sl + 1, per, layers};
+  for (uint32_t l = 0; l < layers; l++) {
+    for (uint32_t i = 0; i < size * size; i++) {
+      const uint8_t* b = texels + ((size_t)l * size * size + i) * 4;
+      out[(size_t)l * per + i] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+    }
+    for (uint32_t m = 1; m <= sl; m++) {
+      uint32_t w = 1u << (sl - m), ws = 2u * w;
+      uint32_t src = l * per + mipOff(sl, m - 1), dst = l * per + mipOff(sl, m);
+      for (uint32_t y = 0; y < w; y++)
+        for (uint32_t x = 0; x < w; x++) {
+          float t00[4], t10[4], t01[4], t11[4], r[4];
+          texelAt(&S, src, ws, 2 * x, 2 * y, t00);
+          texelAt(&S, src, ws, 2 * x + 1, 2 * y, t10);
+          texelAt(&S, src, ws, 2 * x, 2 * y + 1, t01);
+          texelAt(&S, src, ws, 2 * x + 1, 2 * y + 1, t11);
+          for (int k = 0; k < 4; k++) {  /* the bilinear sample with weights 1/2, 1/2 on both axes */
+            float top = t00[k] * 0.5f + t10[k] * 0.5f, bot = t01[k] * 0.5f + t11[k] * 0.5f;
+            r[k] = top * 0.5f + bot * 0.5f;
+          }
+          uint32_t ca = (uint32_t)floorf(fminf(fmaxf(r[3], 0.0f), 1.0f) * 255.0f + 0.5f);
+          out[dst + y * w + x] = srgbEncode(lut, r[0]) | (srgbEncode(lut, r[1]) << 8) | (srgbEncode(lut, r[2]) << 16) |
+                                 (ca << 24);
+        }
+    }
+  }
+}
+/* Repeat addressing: the texel left of coordinate x wrapped into [0, w) and the right texel's weight. */
+static uint32_t wrapCoord(float x, float w, float* f) {
+  if (!(fabsf(x) < 1e30f)) x = 0.0f;
+  float xf = floorf(x);
+  *f = x - xf;
+  return (uint32_t)(xf - w * floorf(xf * (1.0f / w)));
+}
+void orc_surf_sample(const orc_surface* s, uint32_t layer, float u, float v, uint32_t lod, float out[4]) {
+  if (layer == 0) {
+    out[0] = out[1] = out[2] = out[3] = 1.0f;
+    return;
+  }
+  uint32_t m = lod < s->mips - 1 ? lod : s->mips - 1;
+  uint32_t w = 1u << (s->size_log2 - m);
+  float fw = (float)w, fu, fv;
+  uint32_t base = (layer - 1) * s->layer_words + mipOff(s->size_log2, m);
+  uint32_t x0 = wrapCoord(u * fw - 0.5f, fw, &fu), y0 = wrapCoord(v * fw - 0.5f, fw, &fv);
+  uint32_t x1 = (x0 + 1) & (w - 1), y1 = (y0 + 1) & (w - 1);
+  float t00[4], t10[4], t01[4], t11[4];
+  texelAt(s, base, w, x0, y0, t00);
+  texelAt(s, base, w, x1, y0, t10);
+  texelAt(s, base, w, x0, y1, t01);
+  texelAt(s, base, w, x1, y1, t11);
+  float gu = 1.0f - fu, gv = 1.0f - fv;
+  for (int k = 0; k < 4; k++) {
+    float top = t00[k] * gu + t10[k] * fu, bot = t01[k] * gu + t11[k] * fu;
+    out[k] = top * gv + bot * fv;
+  }
+}
+/* tex of surfaceAt / shadeSample: the material's texture times baseColorFactor at the UV of barycentrics (wa, wb, wc)
+ * of vertices i0, i1, i2 (UV(a) wa + UV(b) wb + UV(c) wc), on level 3 + L. */
+static void surfBaseColor(const float* mat, uint32_t m, uint32_t i0, uint32_t i1, uint32_t i2, float wa, float wb,
+                          float wc, uint32_t L, float out[4]) {
+  const orc_surface* s = g_surf;
+  float u = s->uv[i0 * 2] * wa + s->uv[i1 * 2] * wb + s->uv[i2 * 2] * wc;
+  float v = s->uv[i0 * 2 + 1] * wa + s->uv[i1 * 2 + 1] * wb + s->uv[i2 * 2 + 1] * wc;
+  float t[4];
+  orc_surf_sample(s, s->mext[m * 4], u, v, 3 + L, t);
+  for (int k = 0; k < 4; k++) out[k] = t[k] * mat[(size_t)m * 4 + k];
+}
+static float surfCutoff(uint32_t m) {
+  float c;
+  memcpy(&c, &g_surf->mext[m * 4 + 2], 4);
+  return c;
+}
 void orc_closest_bary(const float p[3], const float a[3], const float b[3], const float c[3], float out[3]) {
   v3 r = closestBary(V3(p[0], p[1], p[2]), V3(a[0], a[1], a[2]), V3(b[0], b[1], b[2]), V3(c[0], c[1], c[2]));
   out[0] = r.x;
@@ -327,7 +437,7 @@ static void allocBricks(Ctx* X) {
 }
 static uint32_t nBricks(const Ctx* X) { return X->vox[0] < (uint32_t)X->p->max_bricks ? X->vox[0] : (uint32_t)X->p->max_bricks; }
 
-/* voxelizeOne, voxel.ts:476-490 (no alpha test: opaque untextured materials). */
+/* voxelizeOne, voxel.ts:476-490, with surfaceAt's alpha test (voxel.ts:325-333) when a surface is set. */
 static void voxelizeOne(Ctx* X, Tri T, uint32_t t, uint32_t L, iv3 c, int single) {
   if (finerBuilt(X, L, c)) return;
   float v = voxSize(X->p, L);
@@ -337,6 +447,12 @@ static void voxelizeOne(Ctx* X, Tri T, uint32_t t, uint32_t L, iv3 c, int single
   uint32_t ptr = X->vox[voxPtrIndex(X, L, ishr(c, 3))];
   if (ptr >= V_MARK) return;
   uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
+  if (g_surf && (g_surf->mext[T.m * 4 + 1] & 1u)) {  /* alpha-masked: sampled at the closest point's UV */
+    v3 w = closestBary(p, T.a, T.b, T.c);
+    float tex[4];
+    surfBaseColor(X->mat, T.m, T.i0, T.i1, T.i2, w.x, w.y, w.z, L, tex);
+    if (tex[3] < surfCutoff(T.m)) return;
+  }
   X->vox[X->brickOff + ptr * BW + (bit >> 5)] |= 1u << (bit & 31u);
   uint32_t* rp = &X->rep[ptr * 512u + bit];
   if (*rp < t + 1u) *rp = t + 1u;
@@ -447,8 +563,8 @@ static void octEnc(v3 n, float* px, float* py) {
   *py = y;
 }
 void orc_oct_enc(const float n[3], float out[2]) { octEnc(V3(n[0], n[1], n[2]), &out[0], &out[1]); }
-/* resolveAttr, voxel.ts:590-620, with surfaceAt(wantAttr) (voxel.ts:325-336): white texture -> albedo =
- * baseColorFactor; toon flag 0. */
+/* resolveAttr, voxel.ts:590-620, with surfaceAt(wantAttr) (voxel.ts:325-336): albedo = texture x baseColorFactor
+ * (white texture when untextured), the toon flag in the albedo word's alpha. */
 static void resolveAttr(Ctx* X) {
   uint32_t nb = nBricks(X);
   for (uint32_t i = 0; i < nb * 512u; i++) {
@@ -474,6 +590,14 @@ static void resolveAttr(Ctx* X) {
     v3 q = vadd(vadd(vscale(T.a, wb.x), vscale(T.b, wb.y)), vscale(T.c, wb.z));
     const float* m = X->mat + (size_t)T.m * 4;
     float alb[4] = {1.0f * m[0], 1.0f * m[1], 1.0f * m[2], 0.0f};
+    if (g_surf) {
+      float tex[4];
+      surfBaseColor(X->mat, T.m, T.i0, T.i1, T.i2, wb.x, wb.y, wb.z, L, tex);
+      alb[0] = tex[0];
+      alb[1] = tex[1];
+      alb[2] = tex[2];
+      alb[3] = (g_surf->mext[T.m * 4 + 1] & 4u) ? 1.0f : 0.0f;
+    }
     float ox, oy;
     octEnc(n, &ox, &oy);
     X->vox[X->attrOff + ai * 2] = pack4x8unorm(alb);
@@ -1208,7 +1332,7 @@ int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const fl
     uint32_t m = (uint32_t)mf;
     float fm = (float)m, sa = area / (fm * fm);
     const float* M = mat + (size_t)q[3] * 4;
-    v3 alb = V3(1.0f * M[0], 1.0f * M[1], 1.0f * M[2]);
+    const int masked = g_surf && (g_surf->mext[q[3] * 4 + 1] & 1u);
     v3 n0 = V3(nrm[q[0] * 3], nrm[q[0] * 3 + 1], nrm[q[0] * 3 + 2]);
     v3 n1 = V3(nrm[q[1] * 3], nrm[q[1] * 3 + 1], nrm[q[1] * 3 + 2]);
     v3 n2 = V3(nrm[q[2] * 3], nrm[q[2] * 3 + 1], nrm[q[2] * 3 + 2]);
@@ -1221,6 +1345,14 @@ int orc_dyn_build(const orc_vox* p, const orc_rc* rc, const float* pos, const fl
           v3 pt = vadd(vadd(vscale(a, wa), vscale(b, wb)), vscale(c, wc));
           v3 nv = vadd(vadd(vscale(n0, wa), vscale(n1, wb)), vscale(n2, wc));
           nv = vnormalize(vdot(nv, nv) > 1e-12f ? nv : cr);
+          /* shadeSample's tex (dynlayer.ts:451-452): the alpha test drops the sample, its rgb is the albedo */
+          v3 alb = V3(1.0f * M[0], 1.0f * M[1], 1.0f * M[2]);
+          if (g_surf) {
+            float tex[4];
+            surfBaseColor(mat, q[3], q[0], q[1], q[2], wa, wb, wc, (uint32_t)L, tex);
+            if (masked && tex[3] < surfCutoff(q[3])) continue;
+            alb = V3(tex[0], tex[1], tex[2]);
+          }
           float ndl = fmx(vdot(nv, sd), 0.0f);
           v3 rad = V3(0, 0, 0);
           if (ndl > 0.0f) rad = vscale(vscale(vmul(vdivs(alb, PI_F), se), ndl), rc->sun_vis);

@@ -68,6 +68,29 @@ void orc_cascade_setup(orc_rc* rc, double v0, int levels, double s0, double k_in
 /* octSolidAngles (src/core/frame.ts:105-114). */
 void orc_oct_solid_angles(int res, int samples, float* out);
 
+/* Textured surfaces (surfaceAt, voxel.ts:325-336; shadeSample, dynlayer.ts:451-452): base-colour texels of `layers`
+ * layers of 2^size_log2 squared texels with their mip chain (level m of layer l at l * layer_words + the sizes of
+ * the levels above it), UVs (2 floats per vertex), material words (4 per material: texture layer (0 white, k >= 1
+ * texel layer k - 1), flags (1 alpha mask, 4 toon), alpha cutoff as float bits, 0) and the sRGB tables
+ * (orc_srgb_tables). orc_set_surface(NULL): untextured (the default). The build and the dynamic layer read it. */
+typedef struct orc_surface {
+  const uint32_t* tex;
+  const float* uv;
+  const uint32_t* mext;
+  const float* lut;
+  uint32_t size_log2, mips, layer_words, layers;
+} orc_surface;
+void orc_set_surface(const orc_surface* s);
+/* The 256 sRGB-decoded codes (f32 of the transfer function in double) and the 255 encode thresholds (f32 of the
+ * decoded value of (k + 0.5) / 255). */
+void orc_srgb_tables(float lut[511]);
+/* Mip chain of `layers` layers of size x size RGBA8 texels (level 0 as given; each further texel the bilinear sample
+ * of the level above at its centre, stored as the nearest code): out holds layers x layer_words words. */
+void orc_surf_mips(const uint8_t* texels, uint32_t size, uint32_t layers, const float* lut, uint32_t* out);
+/* Bilinear sample (linear filtering, repeat addressing) of texture layer `layer` (0: white) at (u, v), mip min(lod,
+ * last) -> rgba. */
+void orc_surf_sample(const orc_surface* s, uint32_t layer, float u, float v, uint32_t lod, float out[4]);
+
 /* Full voxel rebuild (src/core/voxel.ts:338-620 minus voxSun). vox: orc_vox_words() words; rep: max_bricks*512. */
 int orc_build(const orc_vox* p, const float* pos, const float* nrm, const uint32_t* tri, uint32_t ntris,
               const float* mat, uint32_t* vox, uint32_t* rep);

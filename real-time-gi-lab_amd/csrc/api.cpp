@@ -141,6 +141,8 @@ VoxK vox_args(const rtgi_world* w) {
   k.pose_first = w->pose_first;
   k.pose_span = w->pose_span;
   k.stats = w->stats;
+  if (w->surf_on) k.surf = SurfK{w->tex, w->uv, w->mext, w->lut, w->tex_log2, w->tex_mips, w->tex_layer_words,
+                                 w->tex_layers};
   if (w->dyn_hdr) {
     k.dyn.hdr = w->dyn_hdr;
     k.dyn.ptr = w->dyn_ptr;
@@ -326,8 +328,29 @@ void objects_free(rtgi_world* w) {
   w->pose_set = false;
 }
 
+void surface_free(rtgi_world* w) {
+  dfree(w, w->tex, w->tex_words);
+  dfree(w, w->uv, (size_t)w->uv_verts * 2);
+  dfree(w, w->mext, (size_t)w->mext_n * 4);
+  dfree(w, w->lut, 511);
+  w->tex_words = 0;
+  w->uv_verts = w->mext_n = 0;
+  w->tex_log2 = w->tex_mips = w->tex_layer_words = w->tex_layers = 0;
+  w->surf_on = false;
+}
+
+// The sRGB tables of the texture sampler: the decoded value of each 8-bit code (the sRGB transfer function,
+// IEC 61966-2-1, evaluated in double and rounded to f32), then the 255 encode thresholds (code k + 1 starts at the
+// decoded value of (k + 0.5) / 255: storing the nearest code in the encoded scale, as an sRGB render target does).
+double srgb_decode(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
+void srgb_tables(float* lut) {
+  for (int c = 0; c < 256; c++) lut[c] = (float)srgb_decode(c / 255.0);
+  for (int k = 0; k < 255; k++) lut[256 + k] = (float)srgb_decode((k + 0.5) / 255.0);
+}
+
 void free_all(rtgi_world* w) {
   objects_free(w);
+  surface_free(w);
   dfree(w, w->big_vox, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->big_mark, (size_t)w->NL * big_cap_for(w->cap_tris));
   dfree(w, w->tri, (size_t)w->cap_tris * 4);
@@ -642,7 +665,8 @@ extern "C" int rtgi_world_upload_scene(rtgi_world* w, const float* pos, const fl
   if ((uint64_t)ntris >= 0xffffffffull) return rtgi_fail(RTGI_E_INVALID, "too many triangles");
   HIPCHK(hipSetDevice(w->device));
   HIPCHK(world_wait(w));
-  objects_free(w);  // the objects belonged to the previous scene
+  objects_free(w);  // the objects and the surface belonged to the previous scene
+  surface_free(w);
   int e = RTGI_OK;
   if (ntris > w->cap_tris) {
     dfree(w, w->tri, (size_t)w->cap_tris * 4);
@@ -695,6 +719,68 @@ extern "C" int rtgi_world_update_vertices(rtgi_world* w, uint32_t first, uint32_
     HIPCHK(hipMemcpy(w->pos + (size_t)first * 3, pos, (size_t)count * 12, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(w->nrm + (size_t)first * 3, nrm, (size_t)count * 12, hipMemcpyHostToDevice));
   }
+  return RTGI_OK;
+}
+
+extern "C" int rtgi_world_set_surface(rtgi_world* w, const float* uvs, uint32_t nverts,
+                                      const rtgi_surface_material* mats, uint32_t nmat, const uint8_t* texels,
+                                      uint32_t size, uint32_t layers) {
+  if (!w) return rtgi_fail(RTGI_E_INVALID, "null world");
+  if (!w->scene_set) return rtgi_fail(RTGI_E_STATE, "no scene uploaded");
+  HIPCHK(hipSetDevice(w->device));
+  HIPCHK(world_wait(w));
+  if (!uvs) {  // untextured again
+    surface_free(w);
+    return RTGI_OK;
+  }
+  if (nverts != w->nverts) return rtgi_fail(RTGI_E_INVALID, "uvs: %u vertices, the scene has %u", nverts, w->nverts);
+  if (nmat != w->nmat) return rtgi_fail(RTGI_E_INVALID, "%u surface materials, the scene has %u", nmat, w->nmat);
+  if (nmat && !mats) return rtgi_fail(RTGI_E_INVALID, "null materials");
+  if (size == 0 || (size & (size - 1)) != 0 || size > (1u << SURF_MAX_LOG2))
+    return rtgi_fail(RTGI_E_INVALID, "texture size must be a power of two in 1..%u (got %u)", 1u << SURF_MAX_LOG2, size);
+  if (layers && !texels) return rtgi_fail(RTGI_E_INVALID, "null texels");
+  for (uint32_t m = 0; m < nmat; m++)
+    if (mats[m].base_color_layer > layers)
+      return rtgi_fail(RTGI_E_INVALID, "material %u samples layer %u of %u", m, mats[m].base_color_layer, layers);
+  uint32_t lg = 0;
+  while ((1u << lg) < size) lg++;
+  uint64_t per = 0;
+  for (uint32_t m = 0; m <= lg; m++) per += (uint64_t)1 << (2 * (lg - m));
+  if (per * layers >= (1ull << 32)) return rtgi_fail(RTGI_E_INVALID, "texture array too large (%u layers)", layers);
+  surface_free(w);
+  int e = dalloc(w, &w->uv, (size_t)nverts * 2);
+  if (!e) e = dalloc(w, &w->mext, (size_t)nmat * 4);
+  if (!e) e = dalloc(w, &w->lut, 511);
+  if (!e) e = dalloc(w, &w->tex, (size_t)(per * layers));
+  if (e) {
+    surface_free(w);
+    return e;
+  }
+  w->uv_verts = nverts;
+  w->mext_n = nmat;
+  w->tex_words = (size_t)(per * layers);
+  w->tex_log2 = lg;
+  w->tex_mips = lg + 1;
+  w->tex_layer_words = (uint32_t)per;
+  w->tex_layers = layers;
+  std::vector<uint32_t> me((size_t)nmat * 4, 0u);
+  for (uint32_t m = 0; m < nmat; m++) {
+    me[m * 4] = mats[m].base_color_layer;
+    me[m * 4 + 1] = mats[m].flags;
+    std::memcpy(&me[m * 4 + 2], &mats[m].alpha_cutoff, 4);
+  }
+  float lut[511];
+  srgb_tables(lut);
+  HIPCHK(hipMemcpy(w->uv, uvs, (size_t)nverts * 8, hipMemcpyHostToDevice));
+  if (nmat) HIPCHK(hipMemcpy(w->mext, me.data(), me.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(w->lut, lut, sizeof(lut), hipMemcpyHostToDevice));
+  for (uint32_t l = 0; l < layers; l++)
+    HIPCHK(hipMemcpy(w->tex + (size_t)l * per, texels + (size_t)l * size * size * 4, (size_t)size * size * 4,
+                     hipMemcpyHostToDevice));
+  w->surf_on = true;
+  VoxK k = vox_args(w);
+  HIPCHK((hipError_t)launch_surf_mips(k.surf, w->tex, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
   return RTGI_OK;
 }
 
@@ -1357,6 +1443,7 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   bool all_lds = true;
   p.stats = false;
   p.dyn = false;
+  p.surf = false;
   p.fused = n > 0;
   for (int i = 0; i < n; i++) p.fused = p.fused && worlds[i]->tuning.fused != 0;
   for (int i = 0; i < n; i++) {
@@ -1373,6 +1460,7 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
     lds = std::max(lds, m);
     p.stats = p.stats || worlds[i]->stats != nullptr;
     p.dyn = p.dyn || worlds[i]->dyn_hdr != nullptr;
+    p.surf = p.surf || worlds[i]->surf_on;
   }
   p.n = n;
   for (int l = 0; l < BL_COUNT; l++) {
@@ -1529,7 +1617,7 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : b->stream;
   const int n = (int)b->worlds.size();
   std::vector<uint64_t> sig;
-  sig.reserve((size_t)n * 7);
+  sig.reserve((size_t)n * 8);
   for (int i = 0; i < n; i++) {
     rtgi_world* w = b->worlds[i];
     if (!w->scene_set || !w->frame_set) return rtgi_fail(RTGI_E_STATE, "world %d has no scene or no frame", i);
@@ -1538,8 +1626,9 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
     sig.push_back(w->ray_off[w->N]);
     sig.push_back((uint64_t)w->NL << 32 | (uint32_t)w->per);
     sig.push_back((uint64_t)w->vp.max_bricks << 32 | w->rp.flags);
-    // The launch set also follows the tuning and the dynamic layer, either of which may change between enqueues.
+    // The launch set also follows the tuning, the dynamic layer and the surfaces, which may change between enqueues.
     sig.push_back((uint64_t)(uint32_t)w->tuning.probe_tile << 32 | (uint32_t)w->tuning.fused);
+    sig.push_back(w->surf_on ? 1ull : 0ull);
     sig.push_back(w->dyn_hdr ? ((uint64_t)w->dyn_first << 32 | (uint32_t)w->dyn_cap) : ~0ull);
     sig.push_back(w->pose_set ? ((uint64_t)w->objs.size() << 32 | w->pose_span) : 0ull);
   }

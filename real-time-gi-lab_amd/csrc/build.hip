@@ -37,6 +37,7 @@
 
 #include "rtgi_internal.h"
 #include "batch.h"
+#include "surface.h"
 #include "vmath.h"
 
 namespace {
@@ -464,6 +465,50 @@ __global__ void __launch_bounds__(BLK) k_init_bricks_b(BatchK b) {
   init_bricks_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
+// Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
+__device__ f3 closest_bary(f3 p, f3 a, f3 b, f3 c) {
+  f3 ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return mk3(1.0f, 0.0f, 0.0f);
+  f3 bp = p - b;
+  float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return mk3(0.0f, 1.0f, 0.0f);
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float w = d1 / (d1 - d3);
+    return mk3(1.0f - w, w, 0.0f);
+  }
+  f3 cp = p - c;
+  float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return mk3(0.0f, 0.0f, 1.0f);
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    return mk3(1.0f - w, 0.0f, w);
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return mk3(0.0f, 1.0f - w, w);
+  }
+  float dn = 1.0f / (va + vb + vc);
+  float v = vb * dn, w = vc * dn;
+  return mk3(1.0f - v - w, v, w);
+}
+
+// surfaceAt without wantAttr (src/core/voxel.ts:325-333), the alpha test of voxelizeOne (voxel.ts:485): a voxel of
+// an alpha-masked material is set only where the texture x baseColorFactor alpha, sampled at the UV of the closest
+// point of triangle t to the voxel centre p, is not below the material's cutoff. Untextured worlds and unmasked
+// materials pass without a sample.
+__device__ __forceinline__ bool surf_pass(const VoxK& k, uint32_t t, f3 p, f3 A, f3 B, f3 Cc, int L) {
+  if (!k.surf.tex) return true;
+  const uint4 q = reinterpret_cast<const uint4*>(k.tri)[t];
+  if (!(surf_flags(k.surf, q.w) & RTGI_SURF_ALPHA_MASK)) return true;
+  float u, v;
+  surf_uv(k.surf, q.x, q.y, q.z, closest_bary(p, A, B, Cc), u, v);
+  return !(surf_base_color(k.surf, k.mat, q.w, u, v, L).w < surf_cutoff(k.surf, q.w));
+}
+
 // Voxel updates of a workgroup are first merged in an LDS hash table keyed by (brick slot, bit): each unique voxel
 // then costs one global atomicMax of its representative and one atomicOr of its bit, however many of the
 // workgroup's triangles touch it (consecutive triangles are spatially coherent; on coarse levels dozens share a
@@ -499,15 +544,19 @@ __device__ __forceinline__ void voxel_table_flush(const VoxK& k, uint32_t* hkey,
 }
 
 // voxelizeOne (src/core/voxel.ts:476-490) of cell q of the small range r (n cells) of triangle t at level L.
+// TX: some world of the launch has textured surfaces (a separate instantiation: the alpha test costs registers).
+template <bool TX>
 __device__ __forceinline__ void voxelize_cell(const VoxK& k, uint32_t* hkey, uint32_t* hval, f3 A, f3 B, f3 Cc,
                                               uint32_t t, int L, const Range& r, SmallDiv m, int q, int n,
                                               uint32_t drops) {
   i3 c = n == 1 ? r.lo : cell_of_small(r, m, q);
   if (finer_built(k, L, c, drops)) return;
   float v = vox_size(k, L);
-  if (n != 1 && !tri_box(region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v, splat3(0.5f * v * 1.0001f), A, B, Cc)) return;
+  const f3 p = region_lo(k, L) + (tof3(c) + splat3(0.5f)) * v;
+  if (n != 1 && !tri_box(p, splat3(0.5f * v * 1.0001f), A, B, Cc)) return;
   uint32_t ptr = k.ptr[ptr_index(k, L, shr3(c, 3))];
   if (ptr >= PTR_MARK) return;
+  if (TX && !surf_pass(k, t, p, A, B, Cc, L)) return;
   voxel_update(k, hkey, hval, ptr, (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7)), t);
 }
 
@@ -516,6 +565,7 @@ constexpr int VOX_CHUNK = 4;
 // Batches of at least this many worlds fill the GPU on their own: their voxel passes pack work into fewer, fuller
 // wavefronts.
 constexpr int WIDE_BATCH = 8;
+template <bool TX>
 __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint32_t nblk) {
   __shared__ uint32_t hkey[VH_SIZE];
   __shared__ uint32_t hval[VH_SIZE];
@@ -571,17 +621,19 @@ __device__ __forceinline__ void vox_small_body(const VoxK& k, uint32_t lb, uint3
         ro.lo = i3{__shfl(r.lo.x, own), __shfl(r.lo.y, own), __shfl(r.lo.z, own)};
         ro.n = i3{__shfl(r.n.x, own), __shfl(r.n.y, own), __shfl(r.n.z, own)};
         const SmallDiv mo{(uint32_t)__shfl((int)md.mx, own), (uint32_t)__shfl((int)md.my, own)};
-        if (j < N) voxelize_cell(k, hkey, hval, A, B, Cc, to, L, ro, mo, q, no, drops);
+        if (j < N) voxelize_cell<TX>(k, hkey, hval, A, B, Cc, to, L, ro, mo, q, no, drops);
       }
     }
   }
   voxel_table_flush(k, hkey, hval);
 }
-__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) { vox_small_body(k, blockIdx.x, gridDim.x); }
+template <bool TX>
+__global__ void __launch_bounds__(BLK) k_vox_small(VoxK k) { vox_small_body<TX>(k, blockIdx.x, gridDim.x); }
+template <bool TX>
 __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
-  vox_small_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
+  vox_small_body<TX>(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // voxelizeBig (src/core/voxel.ts:515-531) over the listed (triangle, level) entries, `chunk` (a power of two up to
@@ -594,6 +646,7 @@ __global__ void __launch_bounds__(BLK) k_vox_small_b(BatchK b) {
 // Entries [base, base + chunk) of the listed (triangle, level) entries (up to 64, one per lane): a wavefront scan of
 // their cell counts lays the ranges end to end and the lanes take the cells in turn, each finding its entry by binary
 // search over the scan and its triangle by shuffles. Called with the same arguments by the whole wavefront.
+template <bool TX>
 __device__ __forceinline__ void vox_big_chunk(const VoxK& k, uint32_t base, uint32_t chunk, uint32_t total,
                                               uint32_t lane, uint32_t drops) {
   const uint32_t e = lane < chunk ? base + lane : total;
@@ -639,6 +692,7 @@ __device__ __forceinline__ void vox_big_chunk(const VoxK& k, uint32_t base, uint
     if (!tri_box(p, splat3(0.5f * v * 1.0001f), A, B, Cc)) continue;
     uint32_t ptr = k.ptr[ptr_index(k, Lo, shr3(c, 3))];
     if (ptr >= PTR_MARK) continue;
+    if (TX && !surf_pass(k, to, p, A, B, Cc, Lo)) continue;
     uint32_t bit = (uint32_t)((c.x & 7) + 8 * (c.y & 7) + 64 * (c.z & 7));
     atomicOr(&k.occ[ptr * 16u + (bit >> 5)], 1u << (bit & 31u));
     atomicMax(&k.rep[(size_t)ptr * 512u + bit], to + 1u);
@@ -651,6 +705,7 @@ __device__ __forceinline__ uint32_t big_total(const VoxK& k) {
   return total;
 }
 
+template <bool TX>
 __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
   const uint32_t total = big_total(k);
   const uint32_t drops = k.hdr[H_BRICK_OVF];
@@ -663,7 +718,7 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
     while (chunk < per && chunk < 64u) chunk <<= 1;
   }
   for (uint32_t base = wave * chunk; base < total; base += waves * chunk)  // the same rounds for the whole wavefront
-    vox_big_chunk(k, base, chunk, total, lane, drops);
+    vox_big_chunk<TX>(k, base, chunk, total, lane, drops);
 }
 
 // voxelizeBig over every world of a batch from one work queue: a world's listed entries are only counted on the
@@ -678,6 +733,7 @@ __device__ __forceinline__ void vox_big_body(const VoxK& k, uint32_t lb, uint32_
 constexpr uint32_t VQ_CHUNK = 16;
 constexpr int VQ_MAXW = 1024;  // worlds per batch the queue serves (larger batches use the static split)
 // (the launch has at least VQ_LANES workgroups, so every class of chunks has wavefronts to take it)
+template <bool TX>
 __device__ __forceinline__ void vox_big_queue(const BatchK& b) {
   __shared__ uint32_t s_pre[VQ_MAXW + 1];  // chunks of worlds [0, w)
   __shared__ uint32_t s_tot[VQ_MAXW];      // listed entries of world w
@@ -717,16 +773,19 @@ __device__ __forceinline__ void vox_big_queue(const BatchK& b) {
         hi = m;
     }
     const VoxK& k = b.w[lo].vx;
-    vox_big_chunk(k, (g - s_pre[lo]) * VQ_CHUNK, VQ_CHUNK, s_tot[lo], lane, k.hdr[H_BRICK_OVF]);
+    vox_big_chunk<TX>(k, (g - s_pre[lo]) * VQ_CHUNK, VQ_CHUNK, s_tot[lo], lane, k.hdr[H_BRICK_OVF]);
   }
 }
-__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body(k, blockIdx.x, gridDim.x, false); }
+template <bool TX>
+__global__ void __launch_bounds__(BLK) k_vox_big(VoxK k) { vox_big_body<TX>(k, blockIdx.x, gridDim.x, false); }
+template <bool TX>
 __global__ void __launch_bounds__(BLK) k_vox_big_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
-  vox_big_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
+  vox_big_body<TX>(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
-__global__ void __launch_bounds__(BLK) k_vox_big_q(BatchK b) { vox_big_queue(b); }
+template <bool TX>
+__global__ void __launch_bounds__(BLK) k_vox_big_q(BatchK b) { vox_big_queue<TX>(b); }
 
 // Level L from level L-1 (reduce, src/core/voxel.ts:540-577), gathered per voxel of every level-L brick: its 8
 // children's bits are ORed and their representatives maxed. One wave owns two occupancy words (64 consecutive voxels)
@@ -850,36 +909,6 @@ __global__ void __launch_bounds__(1024) k_alloc_voxels_b(BatchK b) {
   alloc_voxels_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
-// Closest point of a triangle, barycentrics (Ericson 5.1.5; closestBary, src/core/voxel.ts:250-267).
-__device__ f3 closest_bary(f3 p, f3 a, f3 b, f3 c) {
-  f3 ab = b - a, ac = c - a, ap = p - a;
-  float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
-  if (d1 <= 0.0f && d2 <= 0.0f) return mk3(1.0f, 0.0f, 0.0f);
-  f3 bp = p - b;
-  float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
-  if (d3 >= 0.0f && d4 <= d3) return mk3(0.0f, 1.0f, 0.0f);
-  float vc = d1 * d4 - d3 * d2;
-  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    float w = d1 / (d1 - d3);
-    return mk3(1.0f - w, w, 0.0f);
-  }
-  f3 cp = p - c;
-  float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
-  if (d6 >= 0.0f && d5 <= d6) return mk3(0.0f, 0.0f, 1.0f);
-  float vb = d5 * d2 - d1 * d6;
-  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    float w = d2 / (d2 - d6);
-    return mk3(1.0f - w, 0.0f, w);
-  }
-  float va = d3 * d6 - d5 * d4;
-  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    return mk3(0.0f, 1.0f - w, w);
-  }
-  float dn = 1.0f / (va + vb + vc);
-  float v = vb * dn, w = vc * dn;
-  return mk3(1.0f - v - w, v, w);
-}
 
 // WGSL pack4x8unorm / pack2x16snorm, by their specified formulas.
 __device__ __forceinline__ uint32_t unorm8(float e) {
@@ -890,6 +919,7 @@ __device__ __forceinline__ uint32_t snorm16(float e) {
 }
 
 // resolveAttr (src/core/voxel.ts:596-622) of the occupied voxel (ptr, bit) with attribute slot ai.
+template <bool TX>
 __device__ __forceinline__ void resolve_voxel(const VoxK& k, uint32_t ptr, uint32_t bit, uint32_t ai) {
   if (k.vslot) k.vslot[ai] = (ptr << 9) | bit;
   uint32_t code = k.bcode[ptr];
@@ -900,15 +930,29 @@ __device__ __forceinline__ void resolve_voxel(const VoxK& k, uint32_t ptr, uint3
   uint32_t rp = k.rep[ptr * 512u + bit];
   if (rp == 0u) return;  // every occupied voxel has a representative; kept as a guard on the load below
   Tri T = load_tri(k, rp - 1u);
-  // surfaceAt with wantAttr (src/core/voxel.ts:325-336): white texture, so the albedo is baseColorFactor.
+  // surfaceAt with wantAttr (src/core/voxel.ts:325-336): the albedo is the texture (white when untextured) times
+  // baseColorFactor at the closest point's UV, on mip 3 + L; the albedo word's alpha is the toon flag.
   f3 wb = closest_bary(p, T.a, T.b, T.c);
   f3 nn = ldp(k.nrm, T.i0) * wb.x + ldp(k.nrm, T.i1) * wb.y + ldp(k.nrm, T.i2) * wb.z;
   if (!(dot3(nn, nn) > 1e-12f)) nn = cross3(T.b - T.a, T.c - T.a);
   nn = normalize3(nn);
   f3 q = T.a * wb.x + T.b * wb.y + T.c * wb.z;
-  const float4 m = reinterpret_cast<const float4*>(k.mat)[T.m];
-  float alb[3] = {1.0f * m.x, 1.0f * m.y, 1.0f * m.z};
-  uint32_t pa = unorm8(alb[0]) | (unorm8(alb[1]) << 8) | (unorm8(alb[2]) << 16) | (unorm8(0.0f) << 24);
+  float alb[3], toon = 0.0f;
+  if (TX && k.surf.tex) {
+    float u, v;
+    surf_uv(k.surf, T.i0, T.i1, T.i2, wb, u, v);
+    const float4 tx = surf_base_color(k.surf, k.mat, T.m, u, v, L);
+    alb[0] = tx.x;
+    alb[1] = tx.y;
+    alb[2] = tx.z;
+    toon = (surf_flags(k.surf, T.m) & RTGI_SURF_TOON) ? 1.0f : 0.0f;
+  } else {
+    const float4 m = reinterpret_cast<const float4*>(k.mat)[T.m];
+    alb[0] = 1.0f * m.x;
+    alb[1] = 1.0f * m.y;
+    alb[2] = 1.0f * m.z;
+  }
+  uint32_t pa = unorm8(alb[0]) | (unorm8(alb[1]) << 8) | (unorm8(alb[2]) << 16) | (unorm8(toon) << 24);
   // octEnc (src/core/voxel.ts:584-588).
   float s = fabsf(nn.x) + fabsf(nn.y) + fabsf(nn.z);
   float px = nn.x / s, py = nn.y / s;
@@ -920,7 +964,7 @@ __device__ __forceinline__ void resolve_voxel(const VoxK& k, uint32_t ptr, uint3
   uint32_t pn = snorm16(px) | (snorm16(py) << 16);
   k.attr[ai * 2u] = pa;
   k.attr[ai * 2u + 1u] = pn;
-  if (k.alb16) *reinterpret_cast<uint2*>(k.alb16 + ai * 2u) = make_uint2(f2bf(alb[0]) | (f2bf(alb[1]) << 16), f2bf(alb[2]) | (f2bf(0.0f) << 16));
+  if (k.alb16) *reinterpret_cast<uint2*>(k.alb16 + ai * 2u) = make_uint2(f2bf(alb[0]) | (f2bf(alb[1]) << 16), f2bf(alb[2]) | (f2bf(toon) << 16));
   k.plane[ai] = dot3(nn, q - p);
 }
 
@@ -942,6 +986,7 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint32_t x, uint32_t n) {
 // a scan of the words' popcounts: the lanes take the voxels in turn, each finding its word by binary search over the
 // scan and its bit by a popcount search. One lane per voxel slot of the bricks kept about a quarter of the lanes busy.
 // R = 4 with many worlds in a launch (fullest lanes); 1 for one world alone, which then spreads over more wavefronts.
+template <bool TX>
 __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_t nblk, bool wide) {
   const uint32_t nb = n_bricks(k);
   const uint32_t lane = threadIdx.x & 63u;
@@ -971,15 +1016,28 @@ __device__ __forceinline__ void resolve_body(const VoxK& k, uint32_t lb, uint32_
       const uint32_t n = (uint32_t)(j - __shfl(start, own));
       if (j >= N) continue;
       const uint32_t bit = ((uint32_t)own & 15u) * 32u + nth_set_bit(wd, n);
-      resolve_voxel(k, b0 + ((uint32_t)own >> 4), bit, fs + n);
+      resolve_voxel<TX>(k, b0 + ((uint32_t)own >> 4), bit, fs + n);
     }
   }
 }
-__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body(k, blockIdx.x, gridDim.x, false); }
+// Mip level m >= 1 of every texture layer from level m - 1 (generateMips, src/core/scene_gpu.ts:38-54), one thread
+// per texel; levels run in order on the stream.
+__global__ void __launch_bounds__(BLK) k_surf_mip(SurfK s, uint32_t* tex, uint32_t m) {
+  const uint32_t wl = s.size_log2 - m, per = 1u << (2u * wl);
+  const uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= (uint64_t)per * s.layers) return;
+  const uint32_t layer = (uint32_t)(i >> (2u * wl)), j = (uint32_t)i & (per - 1u);
+  tex[(size_t)layer * s.layer_words + surf_mip_off(s.size_log2, m) + j] =
+      surf_mip_texel(s, layer, m, j & ((1u << wl) - 1u), j >> wl);
+}
+
+template <bool TX>
+__global__ void __launch_bounds__(BLK) k_resolve(VoxK k) { resolve_body<TX>(k, blockIdx.x, gridDim.x, false); }
+template <bool TX>
 __global__ void __launch_bounds__(BLK) k_resolve_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
-  resolve_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
+  resolve_body<TX>(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w], b.n >= WIDE_BATCH);
 }
 
 
@@ -1004,18 +1062,21 @@ int launch_vox_build_batch(const BatchPlan& p, void* stream) {
   if (p.grid[BL_MARK_PARENTS]) k_mark_parents_b<<<p.grid[BL_MARK_PARENTS], BLK, 0, s>>>(bk(p, BL_MARK_PARENTS));
   if (p.grid[BL_ALLOC_BRICKS]) k_alloc_bricks_b<<<p.grid[BL_ALLOC_BRICKS], 1024, 0, s>>>(bk(p, BL_ALLOC_BRICKS));
   if (p.grid[BL_INIT]) k_init_bricks_b<<<p.grid[BL_INIT], BLK, 0, s>>>(bk(p, BL_INIT));
-  if (p.grid[BL_VOX_SMALL]) k_vox_small_b<<<p.grid[BL_VOX_SMALL], BLK, 0, s>>>(bk(p, BL_VOX_SMALL));
+  const bool tx = p.surf;  // textured instantiations only where some world samples textures
+  if (p.grid[BL_VOX_SMALL])
+    (tx ? k_vox_small_b<true> : k_vox_small_b<false>)<<<p.grid[BL_VOX_SMALL], BLK, 0, s>>>(bk(p, BL_VOX_SMALL));
   if (p.grid[BL_VOX_BIG]) {
     if (p.queue && p.n <= VQ_MAXW)  // persistent wavefronts over the batch's work queue (zeroed by k_clear_b)
-      k_vox_big_q<<<std::max<uint32_t>(VQ_LANES, std::min<uint32_t>(p.grid[BL_VOX_BIG], 2048)), BLK, 0, s>>>(
-          bk(p, BL_VOX_BIG));
+      (tx ? k_vox_big_q<true> : k_vox_big_q<false>)<<<
+          std::max<uint32_t>(VQ_LANES, std::min<uint32_t>(p.grid[BL_VOX_BIG], 2048)), BLK, 0, s>>>(bk(p, BL_VOX_BIG));
     else
-      k_vox_big_b<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
+      (tx ? k_vox_big_b<true> : k_vox_big_b<false>)<<<p.grid[BL_VOX_BIG], BLK, 0, s>>>(bk(p, BL_VOX_BIG));
   }
   for (int L = 1; L < RTGI_MAX_LEVELS; L++)
     if (p.grid[BL_REDUCE + L - 1]) k_reduce_b<<<p.grid[BL_REDUCE + L - 1], BLK, 0, s>>>(bk(p, BL_REDUCE + L - 1, L));
   if (p.grid[BL_ALLOC_VOXELS]) k_alloc_voxels_b<<<p.grid[BL_ALLOC_VOXELS], 1024, 0, s>>>(bk(p, BL_ALLOC_VOXELS));
-  if (p.grid[BL_RESOLVE]) k_resolve_b<<<p.grid[BL_RESOLVE], BLK, 0, s>>>(bk(p, BL_RESOLVE));
+  if (p.grid[BL_RESOLVE])
+    (tx ? k_resolve_b<true> : k_resolve_b<false>)<<<p.grid[BL_RESOLVE], BLK, 0, s>>>(bk(p, BL_RESOLVE));
   return (int)hipGetLastError();
 }
 
@@ -1069,6 +1130,15 @@ void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]) {
   out[BL_SH] = cdiv(t.c[0].np, BLK);
 }
 
+int launch_surf_mips(const SurfK& s, uint32_t* tex, void* stream) {
+  for (uint32_t m = 1; m < s.mips; m++) {
+    const uint64_t n = ((uint64_t)1 << (2u * (s.size_log2 - m))) * s.layers;
+    if (n == 0) break;
+    k_surf_mip<<<cdiv(n, BLK), BLK, 0, (hipStream_t)stream>>>(s, tex, m);
+  }
+  return (int)hipGetLastError();
+}
+
 int launch_pose(const VoxK& k, void* stream) {
   if (k.npose) k_pose<<<cdiv(k.pose_span, BLK), BLK, 0, (hipStream_t)stream>>>(k);
   return (int)hipGetLastError();
@@ -1087,10 +1157,11 @@ int launch_vox_build(const VoxK& k, void* stream) {
   if (g[BL_MARK_PARENTS]) k_mark_parents<<<g[BL_MARK_PARENTS], BLK, 0, s>>>(k);
   k_alloc_bricks<<<1, 1024, 0, s>>>(k);
   k_init_bricks<<<g[BL_INIT], BLK, 0, s>>>(k);
-  if (g[BL_VOX_SMALL]) k_vox_small<<<g[BL_VOX_SMALL], BLK, 0, s>>>(k);
-  k_vox_big<<<g[BL_VOX_BIG], BLK, 0, s>>>(k);
+  const bool tx = k.surf.tex != nullptr;
+  if (g[BL_VOX_SMALL]) (tx ? k_vox_small<true> : k_vox_small<false>)<<<g[BL_VOX_SMALL], BLK, 0, s>>>(k);
+  (tx ? k_vox_big<true> : k_vox_big<false>)<<<g[BL_VOX_BIG], BLK, 0, s>>>(k);
   for (int L = 1; L < k.NL; L++) k_reduce<<<g[BL_REDUCE + L - 1], BLK, 0, s>>>(k, L);
   k_alloc_voxels<<<1, 1024, 0, s>>>(k);
-  k_resolve<<<g[BL_RESOLVE], BLK, 0, s>>>(k);
+  (tx ? k_resolve<true> : k_resolve<false>)<<<g[BL_RESOLVE], BLK, 0, s>>>(k);
   return (int)hipGetLastError();
 }

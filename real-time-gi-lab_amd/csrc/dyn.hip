@@ -17,12 +17,14 @@
 //                                                                       (splat, splatTri, splatSample, :392-540)
 //   k_dl_resolve  level L = 1..: own splats + 1/4 of each child with a value, as floats    (resolve, :542-580)
 //
-// Sample shading uses the frame's sun visibility (the reference taps its raster shadow map) and untextured albedo
-// (baseColorFactor).
+// Sample shading uses the frame's sun visibility (the reference taps its raster shadow map) and the base colour of
+// surfaceAt's sampler (texture x baseColorFactor at the sample's UV, mip 3 + L; white when untextured), skipping
+// samples of alpha-masked materials below their cutoff.
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
 #include "rtgi_internal.h"
+#include "surface.h"
 #include "vmath.h"
 
 namespace {
@@ -341,7 +343,7 @@ __device__ void splat_sample(const VoxK& k, SplatWin& W, int wl, i3 wo, i3 wd, i
 }
 
 // splatTri (dynlayer.ts:461-492): m^2 equal sub-triangles (m = ceil(longest edge / voxel) in 1..16), each a sample at
-// its centroid with its area; shadeSample (:440-452) with untextured albedo and the frame's sun visibility. The rounds
+// its centroid with its area; shadeSample (:440-452) with the frame's sun visibility. The rounds
 // of the loop are uniform over the wavefront (i0 < n), so the window reductions see every lane.
 __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, SplatWin& W, uint32_t lb, uint32_t nblk) {
   const DynK& D = k.dyn;
@@ -391,7 +393,7 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, Splat
         float fm = (float)m;
         float sa = area / (fm * fm);
         const float4 mat = reinterpret_cast<const float4*>(k.mat)[T.m];
-        const f3 alb = mk3(1.0f * mat.x, 1.0f * mat.y, 1.0f * mat.z);
+        const bool masked = k.surf.tex && (surf_flags(k.surf, T.m) & RTGI_SURF_ALPHA_MASK);
         const f3 n0 = ldp(k.nrm, T.i0), n1 = ldp(k.nrm, T.i1), n2 = ldp(k.nrm, T.i2);
         for (uint32_t s = sub; s < m * m; s += SPLAT_LPT) {
           // Sample s in the order u, then v < m - u, then flip (flip 1 only while u + v + 1 < m): row u holds
@@ -407,6 +409,15 @@ __device__ __forceinline__ void splat_body(const VoxK& k, const TraceK& t, Splat
           f3 p = T.a * wa + T.b * wb + T.c * wc;
           f3 nv = n0 * wa + n1 * wb + n2 * wc;
           nv = normalize3(dot3(nv, nv) > 1e-12f ? nv : cr);
+          // shadeSample's tex (dynlayer.ts:451-452): the alpha test drops the sample, its rgb is the albedo.
+          f3 alb = mk3(1.0f * mat.x, 1.0f * mat.y, 1.0f * mat.z);
+          if (k.surf.tex) {
+            float su, sv;
+            surf_uv(k.surf, T.i0, T.i1, T.i2, mk3(wa, wb, wc), su, sv);
+            const float4 tx = surf_base_color(k.surf, k.mat, T.m, su, sv, L);
+            if (masked && tx.w < surf_cutoff(k.surf, T.m)) continue;
+            alb = mk3(tx.x, tx.y, tx.z);
+          }
           float ndl = fmaxf(dot3(nv, sd), 0.0f);
           f3 rad = splat3(0.0f);
           if (ndl > 0.0f) rad = alb / PI_F * se * ndl * t.sun_vis;

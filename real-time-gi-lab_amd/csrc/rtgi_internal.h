@@ -65,6 +65,23 @@ struct DynK {
   uint32_t first_tri, end_tri;
 };
 
+// Textured surfaces (rtgi_world_set_surface): the base-colour array with its mip chain, UVs and material words.
+// tex == null: untextured (every material samples white, none is alpha-masked).
+//   tex   [layers][mips] RGBA8 texels (rgb sRGB-encoded, alpha linear), level m of layer l at l * layer_words +
+//         mip_off(m) with mip_off(m) = sum over j < m of (S >> j)^2
+//   uv    [nverts][2]
+//   mext  [nmat][4]: base-colour layer (0: white default), flags (RTGI_SURF_*), alpha cutoff (float bits), 0
+//   lut   [256] sRGB-decoded value of each code (f32 of the exact value), then [255] encode thresholds: code k + 1
+//         starts where the linear value reaches lut[256 + k] (the f32 of the decoded value of (k + 0.5) / 255)
+constexpr uint32_t SURF_MAX_LOG2 = 12;
+struct SurfK {
+  const uint32_t* tex;
+  const float* uv;
+  const uint32_t* mext;
+  const float* lut;
+  uint32_t size_log2, mips, layer_words, layers;
+};
+
 struct VoxK {
   float v0;
   float inv_v0;   // 1 / v0, used only when pow2 (then x / v == x * (1/v) exactly)
@@ -99,6 +116,7 @@ struct VoxK {
   uint32_t npose, pose_first, pose_span;
   unsigned long long* stats;  // tracer statistics (3 x 8 counters) zeroed by the clear stage, or null
   DynK dyn;                   // the dynamic layer (dyn.hdr null: none)
+  SurfK surf;                 // textured surfaces (surf.tex null: untextured)
 };
 
 // Bounce volume (DEFAULT_BOUNCE, src/core/bounce.ts:33): BN_DX x BN_DY x BN_DZ ambient-cube probes BN_CELL m apart
@@ -202,6 +220,8 @@ struct BatchK {
 // Launchers (each returns a hipError_t as int).
 // posedVertices of the world's objects into its vertex buffers (before the voxel build that reads them).
 int launch_pose(const VoxK& k, void* stream);
+// Mip levels 1.. of a world's base-colour array from its level 0 (tex: the array s.tex points to).
+int launch_surf_mips(const SurfK& s, uint32_t* tex, void* stream);
 size_t vox_mark_lds_bytes(const VoxK& k);
 int launch_vox_build(const VoxK& k, void* stream);
 int launch_trace(const TraceK& k, void* stream);
@@ -277,6 +297,7 @@ struct BatchPlan {
   bool stats;       // some world of the batch counts tracer statistics
   bool fused;       // per-cascade trace+merge launches (BL_TM) instead of one trace launch and merge steps
   bool dyn;         // some world has a dynamic layer (the trace reads it)
+  bool surf;        // some world has textured surfaces (the voxel passes sample them)
   bool xcd[BL_COUNT];  // the launch's worlds have about equal block counts: contiguous ranges per XCD pay
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).

@@ -65,6 +65,16 @@ struct rtgi_world {
   int32_t dyn_cap = 0;
   uint32_t *dyn_hdr = nullptr, *dyn_ptr = nullptr, *dyn_bits = nullptr, *dyn_meta = nullptr, *dyn_mask = nullptr,
            *dyn_data = nullptr;
+  // Textured surfaces (rtgi_world_set_surface): base-colour array with its mips, UVs, material words, sRGB tables.
+  bool surf_on = false;
+  uint32_t* tex = nullptr;
+  size_t tex_words = 0;
+  float* uv = nullptr;
+  uint32_t uv_verts = 0;
+  uint32_t* mext = nullptr;
+  uint32_t mext_n = 0;
+  float* lut = nullptr;
+  uint32_t tex_log2 = 0, tex_mips = 0, tex_layer_words = 0, tex_layers = 0;
   uint32_t* bounce = nullptr;             // bounce irradiance per attribute slot, 4 words (RTGI_RC_BOUNCE)
   uint32_t* bnvol[2] = {};                // the two bounce volumes (RTGI_RC_BOUNCE), BN_WORDS each
   int bn_par = 1;                         // the volume the last update wrote

@@ -2,6 +2,8 @@
 // JS face of the Node-API addon: the reference host's pass interface over librtgi.
 //   createVoxWorld(scene, params, frame, firstDynamicTri)
 //                                  <- createVoxWorld (src/core/voxel.ts:746): update(eye), encode(), readCounters(),
+//                                     (scene.surface: UVs, base-colour texels and material words of a textured,
+//                                     alpha-tested scene),
 //                                     usedBytes(c), destroy(); with firstDynamicTri below the triangle count, the
 //                                     triangles from it on go to the dynamic coverage layer (dynBricks bricks;
 //                                     src/core/dynlayer.ts) and dyn.readCounters() reads its counters
@@ -46,6 +48,10 @@ function createVoxWorld(scene, params, frame, firstDynamicTri) {
     if (vw.handle) addon.destroy(vw.handle);
     vw.handle = addon.createWorld(vw.params, vw.rcParams);
     addon.uploadScene(vw.handle, scene.pos, scene.nrm, scene.tri, scene.materials);
+    // scene.surface: {uvs, materials: [{baseColorLayer, flags, alphaCutoff}], texels, size, layers}, what surfaceAt
+    // samples (uploadScene's uvs, baseColor array and material words, src/core/scene_gpu.ts:56-106)
+    const sf = scene.surface;
+    if (sf) addon.setSurface(vw.handle, sf.uvs, sf.materials, sf.texels, sf.size, sf.layers);
     if (first < ntris) addon.setDynamic(vw.handle, first, vw.params.dynBricks);
     addon.setFrame(vw.handle, vw.frame);
   };

@@ -428,6 +428,58 @@ static napi_value js_set_objects(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* setSurface(h, uvs: Float32Array | null, materials: [{baseColorLayer, flags, alphaCutoff}], texels: Uint8Array, size,
+ * layers): what surfaceAt samples (uploadScene's uvs buffer, baseColor array and material words,
+ * src/core/scene_gpu.ts:56-106); uvs null: untextured again. */
+static napi_value js_set_surface(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6], e;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  rtgi_world* w = argc >= 2 ? world_of(env, argv[0]) : NULL;
+  napi_valuetype t = napi_undefined;
+  if (w) napi_typeof(env, argv[1], &t);
+  if (w && (t == napi_null || t == napi_undefined)) {
+    int st = rtgi_world_set_surface(w, NULL, 0, NULL, 0, NULL, 1, 0);
+    if (st) return throw_status(env, st);
+    return NULL;
+  }
+  size_t nuv = 0, ntex = 0;
+  uint32_t n = 0;
+  double size = 0, layers = 0;
+  float* uv = argc == 6 ? typed_data(env, argv[1], napi_float32_array, &nuv) : NULL;
+  uint8_t* tex = argc == 6 ? typed_data(env, argv[3], napi_uint8_array, &ntex) : NULL;
+  int ok = w && uv && nuv % 2 == 0 && napi_get_array_length(env, argv[2], &n) == napi_ok &&
+           napi_get_value_double(env, argv[4], &size) == napi_ok && napi_get_value_double(env, argv[5], &layers) == napi_ok &&
+           size >= 1 && size <= 4096 && layers >= 0 && layers < 65536 && (tex || layers == 0) &&
+           (double)ntex >= size * size * 4 * layers;
+  if (!ok) {
+    napi_throw_type_error(env, NULL, "setSurface(h, Float32Array uvs, [{baseColorLayer, flags, alphaCutoff}], "
+                                     "Uint8Array texels (layers x size x size x 4), size, layers)");
+    return NULL;
+  }
+  rtgi_surface_material* m = (rtgi_surface_material*)calloc(n ? n : 1, sizeof(rtgi_surface_material));
+  if (!m) {
+    napi_throw_error(env, NULL, "rtgi addon: out of memory");
+    return NULL;
+  }
+  for (uint32_t i = 0; i < n && ok; i++) {
+    double l = 0, f = 0, c = 0;
+    ok = napi_get_element(env, argv[2], i, &e) == napi_ok && get_f64(env, e, "baseColorLayer", &l) &&
+         get_f64(env, e, "flags", &f) && get_f64(env, e, "alphaCutoff", &c);
+    m[i].base_color_layer = (uint32_t)l;
+    m[i].flags = (uint32_t)f;
+    m[i].alpha_cutoff = (float)c;
+  }
+  int st = ok ? rtgi_world_set_surface(w, uv, (uint32_t)(nuv / 2), m, n, tex, (uint32_t)size, (uint32_t)layers) : 0;
+  free(m);
+  if (!ok) {
+    napi_throw_type_error(env, NULL, "setSurface: bad material record");
+    return NULL;
+  }
+  if (st) return throw_status(env, st);
+  return NULL;
+}
+
 /* setPose(h, poses: [{c, s, offset: [3]}]): each object's cos(rotY), sin(rotY) and offset, as posedVertices forms
  * them (src/core/scenes.ts:87-89) */
 static napi_value js_set_pose(napi_env env, napi_callback_info info) {
@@ -591,6 +643,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setDynamic", 0, js_set_dynamic, 0, 0, 0, napi_default, 0},
       {"setObjects", 0, js_set_objects, 0, 0, 0, napi_default, 0},
       {"setPose", 0, js_set_pose, 0, 0, 0, napi_default, 0},
+      {"setSurface", 0, js_set_surface, 0, 0, 0, napi_default, 0},
       {"readDynCounters", 0, js_read_dyn_counters, 0, 0, 0, napi_default, 0},
       {"destroy", 0, js_destroy, 0, 0, 0, napi_default, 0},
       {"generateScene", 0, js_generate_scene, 0, 0, 0, napi_default, 0},

@@ -7,6 +7,8 @@ Reference interface (owoDra/real-time-gi-lab) and what stands in for it here:
                                                                           read_counters, used_bytes)
   createRc(...) -> Rc                      src/core/rc.ts:1019-1236    -> World (cascade half: read_cascade, read_sh)
   SUN_E, SKY                               src/core/rcrun.ts:26-27     -> SUN_E, SKY
+  uploadScene's uvs, baseColor array and   src/core/scene_gpu.ts:56-106 -> Surface, World.set_surface
+  material words (what surfaceAt samples)
   parseConfig / named configs              src/runner/configs.ts:47-73 -> CONFIGS, config()
 
 Errors follow the reference's: bad configuration raises synchronously (RtgiError, like its JS throws); GPU pool
@@ -21,7 +23,7 @@ import numpy as np
 from . import _lib as _L
 from ._lib import RtgiError, check, lib, ptr
 
-__all__ = ["pack_irradiance", "write_irradiance", "read_irradiance", "VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "World", "Batch",
+__all__ = ["pack_irradiance", "write_irradiance", "read_irradiance", "VoxParams", "RcParams", "DEFAULT_VOX", "DEFAULT_PARAMS", "vox_params", "Frame", "Scene", "Surface", "SURF_ALPHA_MASK", "SURF_TOON", "World", "Batch",
            "Server", "estimate_bytes", "generate_scene", "CONFIGS", "config", "RtgiError", "SUN_E", "SKY",
            "device_count", "oct_solid_angles", "lib"]
 
@@ -108,9 +110,36 @@ class Frame:
         return f
 
 
+SURF_ALPHA_MASK = 1  # Material.layers.w bits (src/core/scene_gpu.ts:95): alpha-tested
+SURF_TOON = 4        # toon style (stored as the albedo word's alpha)
+SURFACE_MATERIAL_DTYPE = np.dtype([("base_color_layer", np.uint32), ("flags", np.uint32),
+                                   ("alpha_cutoff", np.float32)])
+
+
+@dataclasses.dataclass
+class Surface:
+    """Textured surfaces of a scene (what surfaceAt samples, src/core/voxel.ts:325-336): per-vertex UVs, the
+    base-colour image layers (layers x size x size x 4 uint8: rgb sRGB-encoded, alpha linear; the reference array's
+    layers 1.., its layer 0 being white) and per-material texture layer (0: white), flags (SURF_ALPHA_MASK,
+    SURF_TOON) and alpha cutoff (Material.layers.x, layers.w, params.z of uploadScene, src/core/scene_gpu.ts:95-103)."""
+    uvs: np.ndarray
+    images: np.ndarray
+    layer: np.ndarray
+    flags: np.ndarray
+    cutoff: np.ndarray
+
+    def materials(self):
+        m = np.zeros(len(self.layer), SURFACE_MATERIAL_DTYPE)
+        m["base_color_layer"] = self.layer
+        m["flags"] = self.flags
+        m["alpha_cutoff"] = self.cutoff
+        return m
+
+
 @dataclasses.dataclass
 class Scene:
-    """SceneData subset: positions, normals, triangle table (v0, v1, v2, material) and materials (rgba)."""
+    """SceneData subset: positions, normals, triangle table (v0, v1, v2, material) and materials (rgba); surface:
+    UVs, base-colour textures and material surface words (None: untextured, every material white and opaque)."""
     pos: np.ndarray
     nrm: np.ndarray
     tri: np.ndarray
@@ -125,6 +154,7 @@ class Scene:
     motion: np.ndarray = None
     rest_pos: np.ndarray = None
     rest_nrm: np.ndarray = None
+    surface: Surface = None
 
     @property
     def ntris(self):
@@ -283,11 +313,28 @@ class World:
                                             ptr(mat), mat.shape[0]))
         self.scene = scene
         self._nobj = 0  # a new scene has no objects until set_objects
+        if scene.surface is not None:
+            self.set_surface(scene.surface)
 
     def update_vertices(self, first, pos, nrm):
         pos = np.ascontiguousarray(pos, np.float32)
         nrm = np.ascontiguousarray(nrm, np.float32)
         check(lib().rtgi_world_update_vertices(self._h, first, pos.shape[0], ptr(pos), ptr(nrm)))
+
+    def set_surface(self, surface):
+        """rtgi_world_set_surface: the scene's UVs, base-colour textures and material surface words (None: untextured
+        again). Mips are built on the device."""
+        if surface is None:
+            check(lib().rtgi_world_set_surface(self._h, None, 0, None, 0, None, 1, 0))
+            return
+        uv = np.ascontiguousarray(surface.uvs, np.float32).reshape(-1, 2)
+        img = np.ascontiguousarray(surface.images, np.uint8)
+        if img.ndim != 4 or img.shape[1] != img.shape[2] or img.shape[3] != 4:
+            raise ValueError("images must be layers x size x size x 4 uint8")
+        mats = np.ascontiguousarray(surface.materials())
+        check(lib().rtgi_world_set_surface(self._h, ptr(uv), uv.shape[0], mats.ctypes.data if len(mats) else None,
+                                           len(mats), img.ctypes.data if img.shape[0] else None, img.shape[1],
+                                           img.shape[0]))
 
     def set_objects(self, objects, rest_pos, rest_nrm):
         """rtgi_world_set_objects: moving objects (OBJECT_DTYPE records) and the rest mesh of their vertex span, posed

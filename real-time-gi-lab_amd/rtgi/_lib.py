@@ -129,6 +129,7 @@ SIGNATURES = {
     "rtgi_world_create": (C.c_int, [C.POINTER(VoxParamsC), C.POINTER(RcParamsC), C.POINTER(_P)]),
     "rtgi_world_upload_scene": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32]),
     "rtgi_world_update_vertices": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, _P]),
+    "rtgi_world_set_surface": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32]),
     "rtgi_world_set_objects": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "rtgi_world_set_pose": (C.c_int, [_P, _P]),
     "rtgi_world_set_frame": (C.c_int, [_P, C.POINTER(FrameC)]),

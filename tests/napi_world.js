@@ -10,8 +10,28 @@ const [kind, seed, v0, levels, G, cascades, probes0, s0] = args.slice(0, 8).map(
 // 'dynamic': the scene's moving part (triangles from firstDynamicTri on) goes to the dynamic coverage layer.
 // 'motion': the scene's objects posed on the device at frames 3 and 11 (the runner's pose(t)); the poses sent are
 // printed so that the other side can send the same.
+// 'textured': a surface on the scene: UVs by planar projection along each normal's dominant axis (4 m tiles), one
+// 64 x 64 layer (rgb = 4 x, 4 y, 128; alpha 0 on its left half, 255 on its right), every material masked, the odd
+// ones toon.
 const full = args[8] === 'full', dynamic = args[8] === 'dynamic', motion = args[8] === 'motion';
+const tex = args[8] === 'textured';
 const scene = r.generateScene(kind, seed, 0);
+if (tex) {
+  const nv = scene.pos.length / 3, uvs = new Float32Array(nv * 2), texels = new Uint8Array(64 * 64 * 4);
+  for (let i = 0; i < nv; i++) {
+    const n = [0, 1, 2].map((a) => Math.abs(scene.nrm[i * 3 + a]));
+    const ax = n[0] >= n[1] && n[0] >= n[2] ? 0 : (n[1] >= n[2] ? 1 : 2);
+    const o = [0, 1, 2].filter((a) => a !== ax);
+    uvs[i * 2] = scene.pos[i * 3 + o[0]] / 4;
+    uvs[i * 2 + 1] = scene.pos[i * 3 + o[1]] / 4;
+  }
+  for (let y = 0; y < 64; y++) for (let x = 0; x < 64; x++) texels.set([x * 4, y * 4, 128, x < 32 ? 0 : 255], (y * 64 + x) * 4);
+  const materials = [];
+  for (let m = 0; m < scene.materials.length / 4; m++) {
+    materials.push({ baseColorLayer: 1, flags: 1 | (m % 2 ? 4 : 0), alphaCutoff: 0.5 });
+  }
+  scene.surface = { uvs, materials, texels, size: 64, layers: 1 };
+}
 const vw = r.createVoxWorld(scene, { v0, levels, G, maxBricks: levels * G * G * G, maxVoxels: 1 << 20 },
   { eye: scene.eye, sunDir: scene.sunDir }, dynamic ? scene.firstDynamicTri : undefined);
 const rc = r.createRc(vw, full ? { cascades, probes0, s0, relocate: 1, sunTrace: 1, bounce: 1 } : { cascades, probes0, s0 });

@@ -1,4 +1,5 @@
 """The Node-API addon (the drop-in binding for the reference's TypeScript host) over the same C ABI."""
+import dataclasses
 import json
 import os
 import shutil
@@ -62,6 +63,45 @@ def test_addon_update_matches_python_path(require_gpu):
         w.encode()
         c = w.read_counters()
         sh = w.read_sh()
+    for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow", "bricksPerLevel"):
+        assert out["counters"][k] == c[k]
+    np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
+    assert abs(out["shSum"] - float(sh.astype(np.float64).sum())) <= 1e-6 * abs(float(sh.sum()))
+
+
+@pytest.mark.gpu
+def test_addon_textured_scene_matches_python_path(require_gpu):
+    """setSurface through the addon (the textured scene built in JS) equals the Python path's rtgi_world_set_surface
+    with the same UVs, texels and material words."""
+    print("case: node addon, textured atrium seed 1", flush=True)
+    r = node(os.path.join(ROOT, "tests", "napi_world.js"), "atrium", "1", "0.125", "4", "8", "4", "32", "0.625",
+             "textured", timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    scene = rtgi.generate_scene("atrium", seed=1)
+    ax = np.argmax(np.abs(scene.nrm), axis=1)
+    uv = np.zeros((scene.pos.shape[0], 2), np.float32)
+    for a in range(3):
+        m = ax == a
+        uv[m] = scene.pos[m][:, [k for k in range(3) if k != a]] / np.float32(4)
+    y, x = np.mgrid[0:64, 0:64]
+    img = np.stack([x * 4, y * 4, np.full_like(x, 128), np.where(x < 32, 0, 255)], -1).astype(np.uint8)[None]
+    nm = scene.materials.shape[0]
+    flags = rtgi.SURF_ALPHA_MASK | np.where(np.arange(nm) % 2 == 1, rtgi.SURF_TOON, 0)
+    surf = rtgi.Surface(uvs=uv, images=img, layer=np.ones(nm, np.uint32), flags=flags.astype(np.uint32),
+                        cutoff=np.full(nm, 0.5, np.float32))
+    vox = rtgi.VoxParams(v0=0.125, levels=4, G=8, maxBricks=4 * 8 ** 3, maxVoxels=1 << 20)
+    rc = rtgi.RcParams(cascades=4, probes0=32, s0=0.625)
+    with rtgi.World(vox, rc) as w:
+        w.upload_scene(dataclasses.replace(scene, surface=surf))
+        w.set_frame(rtgi.Frame(eye=scene.eye, sunDir=scene.sunDir))
+        w.encode()
+        c = w.read_counters()
+        sh = w.read_sh()
+        w.set_surface(None)
+        w.encode()
+        plain = w.read_counters()
+    assert c["voxels"] < plain["voxels"]  # the transparent half of the layer empties voxels
     for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow", "bricksPerLevel"):
         assert out["counters"][k] == c[k]
     np.testing.assert_array_equal(np.float32(out["sh0"]), sh[0])
