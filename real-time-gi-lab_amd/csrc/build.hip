@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(BLK) k_clear_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
   clear_body(b.w[w].vx, blk - b.start[w], b.start[w + 1] - b.start[w]);
-  if (b.queue && blockIdx.x == 0 && threadIdx.x < VQ_LANES) b.queue[threadIdx.x * VQ_STRIDE] = 0u;  // the build's queue
+  // the batch's work queues (the build's and each trace+merge step's)
+  if (b.queue && blockIdx.x == 0 && threadIdx.x < VQ_LANES * (1 + RTGI_MAX_CASCADES)) b.queue[threadIdx.x * VQ_STRIDE] = 0u;
 }
 
 // markBricks over all triangles, grid-stride (every lane of a workgroup runs the same trip count, so the wave-level

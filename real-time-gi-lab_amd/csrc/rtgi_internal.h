@@ -286,7 +286,9 @@ enum BatchLaunch : int {
 // The work-queue counters of a batch: VQ_LANES counters VQ_STRIDE words (1 KiB) apart.
 constexpr uint32_t VQ_LANES = 8;
 constexpr uint32_t VQ_STRIDE = 256;
-constexpr size_t RTGI_QUEUE_WORDS = (size_t)VQ_LANES * VQ_STRIDE;
+// Counter sets: set 0 for the big-triangle voxelisation, set 1 + step for the trace+merge launch of each cascade step.
+constexpr size_t VQ_SET = (size_t)VQ_LANES * VQ_STRIDE;
+constexpr size_t RTGI_QUEUE_WORDS = VQ_SET * (1 + RTGI_MAX_CASCADES);
 struct BatchPlan {
   const TraceK* w;
   int32_t n;

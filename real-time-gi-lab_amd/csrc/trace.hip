@@ -944,10 +944,16 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 // Step of a batch: each world traces and merges its cascade N-1-step. Batches project the SH in their own launch
 // (k_sh_b): with the projection, this kernel needs 80 VGPRs and spills at 8 wavefronts per SIMD.
 template <bool ST, bool DY>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_merge_b(BatchK b) {
-  const uint32_t blk = batch_block(b);
+// With a work queue (queue: the step's counters, given where the launch's worlds have about equal block counts),
+// workgroups take their blocks from it (queue_block), else by batch_block (hardware order). The descriptors and the counters come as restrict parameters: the compiler then knows that the
+// counters' atomics do not write the descriptors and keeps reading them with scalar loads (with both behind one
+// struct, every descriptor field read after the atomic became a vector load and the trace spilled).
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_trace_merge_b(BatchK b, const TraceK* __restrict__ ws, uint32_t* __restrict__ queue) {
+  const uint32_t blk = queue ? queue_block(queue) : batch_block(b);
+  if (blk == ~0u) return;
   int w = batch_world(b, blk);
-  const TraceK& a = b.w[w];
+  const TraceK& a = ws[w];
   trace_merge_body<ST, DY, false>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
@@ -1678,7 +1684,7 @@ uint64_t trace_merge_threads(const TraceK& k, int c) {
 }
 
 using TraceMergeFn = void (*)(TraceK, int);
-using TraceMergeBatchFn = void (*)(BatchK);
+using TraceMergeBatchFn = void (*)(BatchK, const TraceK*, uint32_t*);
 template <bool SH>
 TraceMergeFn trace_merge_fn(bool st, bool dy) {
   return st ? (dy ? k_trace_merge<true, true, SH> : k_trace_merge<true, false, SH>)
@@ -1815,8 +1821,10 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
 }
 
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
-  if (p.grid[BL_TM + step])
-    trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[BL_TM + step], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_TM + step, step));
+  const int l = BL_TM + step;
+  uint32_t* queue = p.queue && p.xcd[l] ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
+  if (p.grid[l])
+    trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[l], BLK, 0, (hipStream_t)stream>>>(bk(p, l, step), p.w, queue);
   return (int)hipGetLastError();
 }
 
