@@ -600,8 +600,43 @@ __device__ __forceinline__ uint32_t parent_index(const CascK& U, i3 g, int kk) {
 // for which acc * (1 / n) is the correctly rounded acc / n.
 // With relocation, parents hidden by voxels are left out (linkParents, rc.ts:559-566); their count is then any of
 // 0..8, so the mean divides (0 parents: 0, as mergeWorld's select gives).
+// The common case (rows of a power-of-two grid, no relocation) reads the eight candidates unconditionally through a
+// buffer resource over the cascade's averages (a candidate past its end reads 0) and zeroes the absent ones by
+// select: adding +0 for an absent parent leaves the running sum exactly as skipping it does (the sum starts at +0 and
+// never is -0), so the result is the same sum in kk order.
+__device__ __forceinline__ float3 avg_rgb(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+  return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
 __device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t pi, uint32_t di) {
   const CascK& U = a.c[c + 1];
+  if (U.tile_log2 == 0 && U.n_log2 >= 0 && !a.reloc) {
+    const int nl = U.n_log2;
+    const i3 b = shr3(g, 1);
+    const bool hx = (g.x & 1) && b.x + 1 < U.n, hy = (g.y & 1) && b.y + 1 < U.n, hz = (g.z & 1) && b.z + 1 < U.n;
+    // U's averages: U.np entries per direction of cascade c (res_c^2 of them), float4 each
+    const uint32_t res = (uint32_t)a.c[c].res;
+    const __amdgpu_buffer_rsrc_t r =
+        buf_rsrc(reinterpret_cast<const float4*>(a.avg) + U.avg_off, U.np * res * res * 16u);
+    const uint32_t o = ((di << (3 * nl)) + ((uint32_t)b.x | ((uint32_t)b.y << nl) | ((uint32_t)b.z << (2 * nl)))) * 16u;
+    const uint32_t oy = o + (16u << nl), oz = o + (16u << (2 * nl)), oyz = oy + (16u << (2 * nl));
+    const float3 x000 = avg_rgb(r, o), x100 = avg_rgb(r, o + 16u), x010 = avg_rgb(r, oy), x110 = avg_rgb(r, oy + 16u),
+                 x001 = avg_rgb(r, oz), x101 = avg_rgb(r, oz + 16u), x011 = avg_rgb(r, oyz),
+                 x111 = avg_rgb(r, oyz + 16u);
+    const float3 z = make_float3(0.0f, 0.0f, 0.0f);
+    f3 acc = splat3(0.0f) + mk3(x000.x, x000.y, x000.z);
+    const float3 p100 = hx ? x100 : z, p010 = hy ? x010 : z, p110 = hx && hy ? x110 : z, p001 = hz ? x001 : z,
+                 p101 = hx && hz ? x101 : z, p011 = hy && hz ? x011 : z, p111 = hx && hy && hz ? x111 : z;
+    acc = acc + mk3(p100.x, p100.y, p100.z);
+    acc = acc + mk3(p010.x, p010.y, p010.z);
+    acc = acc + mk3(p110.x, p110.y, p110.z);
+    acc = acc + mk3(p001.x, p001.y, p001.z);
+    acc = acc + mk3(p101.x, p101.y, p101.z);
+    acc = acc + mk3(p011.x, p011.y, p011.z);
+    acc = acc + mk3(p111.x, p111.y, p111.z);
+    // 1 / (parent count) = 2^-(hx + hy + hz), exact
+    return acc * inv_pow2((hx ? 1 : 0) + (hy ? 1 : 0) + (hz ? 1 : 0));
+  }
   const float4* A = reinterpret_cast<const float4*>(a.avg) + U.avg_off + (size_t)di * U.np;
   uint32_t m;
   float4 x[8];
@@ -943,11 +978,12 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))
 }
 // Step of a batch: each world traces and merges its cascade N-1-step. Batches project the SH in their own launch
 // (k_sh_b): with the projection, this kernel needs 80 VGPRs and spills at 8 wavefronts per SIMD.
-template <bool ST, bool DY>
 // With a work queue (queue: the step's counters, given where the launch's worlds have about equal block counts),
-// workgroups take their blocks from it (queue_block), else by batch_block (hardware order). The descriptors and the counters come as restrict parameters: the compiler then knows that the
-// counters' atomics do not write the descriptors and keeps reading them with scalar loads (with both behind one
-// struct, every descriptor field read after the atomic became a vector load and the trace spilled).
+// workgroups take their blocks from it (queue_block), else by batch_block (hardware order). The descriptors and the
+// counters come as restrict parameters: the compiler then knows that the counters' atomics do not write the
+// descriptors and keeps reading them with scalar loads (with both behind one struct, every descriptor field read
+// after the atomic became a vector load and the trace spilled).
+template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_trace_merge_b(BatchK b, const TraceK* __restrict__ ws, uint32_t* __restrict__ queue) {
   const uint32_t blk = queue ? queue_block(queue) : batch_block(b);
