@@ -742,11 +742,11 @@ def main():
                      "l2_hit_profiled": prof.get("l2_hit"),
                      "wavefront_cycles_profiled": {k: prof.get(k) for k in ("wait_dependency_frac", "wait_issue_frac",
                                                                             "issuing_frac")},
-                     "limiter_note": "VALU issue is busy for about every cycle of the launch (4-cycle model) with 36 "
-                                     "of 64 lanes active, while HBM moves well below peak; a wavefront waits on its "
-                                     "own dependent reads about half its cycles, so the DDA is bound by divergent "
-                                     "vector issue and dependent-read latency; frac is against the HBM roofline "
-                                     "because nothing on the path is an MFMA contraction"},
+                     "limiter_note": "VALU issue is busy for about every cycle of the launch (4-cycle model) with "
+                                     "about half the lanes active, while HBM moves well below peak; a wavefront waits "
+                                     "on its own dependent reads about half its cycles, so the DDA is bound by "
+                                     "divergent vector issue and dependent-read latency; frac is against the HBM "
+                                     "roofline because nothing on the path is an MFMA contraction"},
         "cpu_baseline": cpu,
         "latency_vs_width": widths,
         "gather_1080p": gather,
