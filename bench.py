@@ -76,6 +76,10 @@ def mix_bench(a):
     from rtgi.workloads import world_mix
 
     mix = world_mix(a.mix_worlds, seed=5, tail_every=0)
+    # Batches are cut from the worlds in order of shape (c0 probes, cascades, clipmap edge), so that a batch's worlds
+    # need about equal block counts per launch and its launches take the XCD-contiguous order (32 mixed worlds in
+    # batches of 8: 5,836 -> 5,998 updates/s against seed order).
+    mix = sorted(mix, key=lambda m: (m.rc.probes0, m.rc.cascades, m.vox.G))
     worlds = []
     for m in mix:
         w = rtgi.World(m.vox, m.rc)
@@ -121,7 +125,8 @@ def mix_bench(a):
     log(f"mix: {value:.1f} updates/s batched ({best}), plain FP32 {plain_value:.1f} updates/s")
     return {
         "workload": f"{len(mix)} seeded mixed worlds (atrium/city, 10k-500k triangles log-uniform, clipmap edge "
-                    f"32-128 x 4 levels, 3-5 cascades), FP16 storage, batched hipGraph replays",
+                    f"32-128 x 4 levels, 3-5 cascades), FP16 storage, batched hipGraph replays, batches cut in "
+                    f"order of world shape",
         "updates_per_s": round(value, 2), "tuning": best, "autotune_table": tune["table"],
         "triangles_total_m": round(sum(m.scene.ntris for m in mix) / 1e6, 2),
         "plain_fp32_serial_updates_per_s": round(plain_value, 2), "speedup_vs_plain": round(value / plain_value, 3),
