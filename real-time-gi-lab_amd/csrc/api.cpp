@@ -1479,6 +1479,16 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   p.mark_lds = all_lds ? lds : 0;
 }
 
+bool plan_sh_pairs(rtgi_world* const* worlds, const TraceK* desc, int n) {
+  bool any = false;
+  for (int i = 0; i < n; i++) {
+    if (worlds[i]->rp.flags & RTGI_RC_NO_MERGE) continue;  // no SH projection
+    if (!sh_pairs(desc[i])) return false;
+    any = true;
+  }
+  return any;
+}
+
 void batch_world_begin(rtgi_world* w, const BatchPlan& p, TraceK* out) {
   trace_begin(w);
   bounce_advance(w);
@@ -1641,6 +1651,12 @@ extern "C" int rtgi_batch_enqueue(rtgi_batch* b, void* stream) {
   // Descriptors: the staging buffer being filled was last copied two enqueues ago; wait for that copy.
   HIPCHK(hipEventSynchronize(b->copied[b->cur]));
   for (int i = 0; i < n; i++) batch_world_begin(b->worlds[i], b->plan, &b->h_desc[b->cur][i]);
+  // The SH launch form follows this update's descriptors (their storage order); a change needs a new graph.
+  const bool shp = plan_sh_pairs(b->worlds.data(), b->h_desc[b->cur], n);
+  if (shp != b->plan.sh_pairs) {
+    b->plan.sh_pairs = shp;
+    b->graph_valid = false;
+  }
   HIPCHK(hipMemcpyAsync(b->d_desc, b->h_desc[b->cur], sizeof(TraceK) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipEventRecord(b->copied[b->cur], s));
   b->cur ^= 1;

@@ -217,6 +217,16 @@ struct BatchK {
   uint32_t* queue;  // the batch's work-queue counter (zeroed by k_clear_b, taken by k_vox_big_q), or null
 };
 
+// The SH projection takes two neighbouring c0 probes per thread where the radiance and SH are f16 in grid rows of
+// even length: each of its reads and writes then covers both probes. The host picks the launch form from the
+// descriptors of the update; the kernels test the same predicate.
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline bool sh_pairs(const TraceK& t) {
+  return t.fp16 && t.c[0].tile_log2 == 0 && (t.c[0].n & 1) == 0 && (t.c[0].np & 1u) == 0 && (t.c[0].ray_off & 1u) == 0;
+}
+
 // Launchers (each returns a hipError_t as int).
 // posedVertices of the world's objects into its vertex buffers (before the voxel build that reads them).
 int launch_pose(const VoxK& k, void* stream);
@@ -301,6 +311,7 @@ struct BatchPlan {
   bool dyn;         // some world has a dynamic layer (the trace reads it)
   bool surf;        // some world has textured surfaces (the voxel passes sample them)
   bool xcd[BL_COUNT];  // the launch's worlds have about equal block counts: contiguous ranges per XCD pay
+  bool sh_pairs;       // every world with an SH projection takes it in pairs (sh_pairs): the pair-only kernel
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);

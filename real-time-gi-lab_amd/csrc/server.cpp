@@ -288,6 +288,7 @@ void sched_loop(rtgi_server* s) {
     if (!e) {
       plan_blocks(ws.data(), n, sl.h_start, plan);
       for (int i = 0; i < n; i++) batch_world_begin(ws[i], plan, &sl.h_desc[i]);
+      plan.sh_pairs = plan_sh_pairs(ws.data(), sl.h_desc, n);
       if (hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(TraceK) * n, hipMemcpyHostToDevice, sl.stream) != hipSuccess ||
           hipMemcpyAsync(sl.d_start, sl.h_start, sizeof(uint32_t) * BL_COUNT * (n + 1), hipMemcpyHostToDevice,
                          sl.stream) != hipSuccess)

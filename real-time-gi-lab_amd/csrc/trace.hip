@@ -805,11 +805,64 @@ __device__ __forceinline__ void sh_body(const TraceK& a, uint32_t lb, uint32_t n
   }
   for (int k = 0; k < 27; k++) sh_put(a, C.np, row, ~0u, k, v[k]);
 }
+// Where the radiance and SH are f16 in grid rows of even length (sh_pairs), a thread projects two neighbouring
+// probes: rows r and r + 1 (r even) are grid points (x, y, z) and (x + 1, y, z), their radiance of one direction is
+// one 16-byte read and their SH value of one plane one 4-byte write, each probe getting the same operations as in
+// sh_body. The launches keep one thread per probe (the block counts are planned before an update's descriptors are
+// formed, and its storage order decides the form); in pair form the upper half of the threads has no work. Where
+// every world of a launch takes pairs, the pair-only kernel runs (78 VGPRs against the combined kernel's 138: twice
+// the wavefronts per SIMD; 32 mixed FP16 worlds' SH 0.65 -> 0.41 ms).
+__device__ __forceinline__ void sh_pair_body(const TraceK& a, uint32_t lb) {
+  const CascK& C = a.c[0];
+  const uint32_t row = 2u * (lb * BLK + threadIdx.x);
+  if (row >= C.np) return;
+  const uint32_t n = (uint32_t)C.n;
+  const i3 gl{(int)(row % n), (int)((row / n) % n), (int)(row / (n * n))};
+  const i3 gl1{gl.x + 1, gl.y, gl.z};
+  const uint4* __restrict__ R = reinterpret_cast<const uint4*>(a.rad);
+  f3 c0[9], c1[9];
+  for (int q = 0; q < 9; q++) c0[q] = c1[q] = splat3(0.0f);
+  for (uint32_t d = 0; d < 16; d++) {
+    const uint4 p = R[(C.ray_off + (size_t)d * C.np + row) >> 1];
+    const float w = a.omega0[d];
+    const f3 r0 = mk3(hlo(p.x), hhi(p.x), hlo(p.y)) * w, r1 = mk3(hlo(p.z), hhi(p.z), hlo(p.w)) * w;
+    float y[9];
+    sh_basis(a.dirs[C.dir_off + d], y);
+    for (int q = 0; q < 9; q++) {
+      c0[q] = c0[q] + r0 * y[q];
+      c1[q] = c1[q] + r1 * y[q];
+    }
+  }
+  const unsigned short* __restrict__ H = reinterpret_cast<const unsigned short*>(a.sh_prev);
+  uint32_t* __restrict__ S = reinterpret_cast<uint32_t*>(a.sh);
+  const uint32_t h0 = hist_row(a, gl), h1 = hist_row(a, gl1);
+  for (int q = 0; q < 9; q++) {
+    const float s = sh_scale(q);
+    const float v0[3] = {c0[q].x * s, c0[q].y * s, c0[q].z * s}, v1[3] = {c1[q].x * s, c1[q].y * s, c1[q].z * s};
+    for (int ch = 0; ch < 3; ch++) {
+      const size_t k = (size_t)(3 * q + ch) * C.np;
+      float x0 = v0[ch], x1 = v1[ch];
+      if (h0 != ~0u) x0 = __half2float(__ushort_as_half(H[k + h0])) * (1.0f - 0.5f) + x0 * 0.5f;
+      if (h1 != ~0u) x1 = __half2float(__ushort_as_half(H[k + h1])) * (1.0f - 0.5f) + x1 * 0.5f;
+      S[(k + row) >> 1] = (uint32_t)__half_as_ushort(__float2half_rn(x0)) |
+                          ((uint32_t)__half_as_ushort(__float2half_rn(x1)) << 16);
+    }
+  }
+}
 __global__ void __launch_bounds__(BLK) k_sh(TraceK a) { sh_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(BLK) k_sh_p(TraceK a) { sh_pair_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(BLK) k_sh_pb(BatchK b) {
+  const uint32_t blk = batch_block(b);
+  int w = batch_world(b, blk);
+  sh_pair_body(b.w[w], blk - b.start[w]);
+}
 __global__ void __launch_bounds__(BLK) k_sh_b(BatchK b) {
   const uint32_t blk = batch_block(b);
   int w = batch_world(b, blk);
-  sh_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  if (sh_pairs(b.w[w]))
+    sh_pair_body(b.w[w], blk - b.start[w]);
+  else
+    sh_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
 }
 
 // Merge step of a batch: each world merges its cascade N-1-step (c > 0 with the 2x2 averages, c = 0 per ray).
@@ -1812,7 +1865,7 @@ int launch_merge(const TraceK& k, int c, void* stream) {
 }
 
 int launch_sh(const TraceK& k, void* stream) {
-  k_sh<<<grid_for(k.c[0].np), BLK, 0, (hipStream_t)stream>>>(k);
+  (sh_pairs(k) ? k_sh_p : k_sh)<<<grid_for(k.c[0].np), BLK, 0, (hipStream_t)stream>>>(k);
   return (int)hipGetLastError();
 }
 
@@ -1865,6 +1918,6 @@ int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
 }
 
 int launch_sh_batch(const BatchPlan& p, void* stream) {
-  if (p.grid[BL_SH]) k_sh_b<<<p.grid[BL_SH], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SH));
+  if (p.grid[BL_SH]) (p.sh_pairs ? k_sh_pb : k_sh_b)<<<p.grid[BL_SH], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SH));
   return (int)hipGetLastError();
 }

@@ -136,6 +136,8 @@ int frame_origins(const rtgi_vox_params* vp, const rtgi_rc_params* rp, const rtg
 // Starts world w's part of a batched update with plan p: advances its per-update state (trace order, bounce volume,
 // SH history) and writes its launch descriptor.
 void batch_world_begin(rtgi_world* w, const BatchPlan& p, TraceK* out);
+// Whether every world with an SH projection takes it in pairs under its update descriptor (desc[i]).
+bool plan_sh_pairs(rtgi_world* const* worlds, const TraceK* desc, int n);
 // Temporal state a world carries from one update to the next (the SH history and the bounce volume); the server
 // saves it when it evicts a world and restores it when the world is admitted again.
 struct WorldState {
