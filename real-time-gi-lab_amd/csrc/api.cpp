@@ -1446,6 +1446,10 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   p.surf = false;
   p.fused = n > 0;
   for (int i = 0; i < n; i++) p.fused = p.fused && worlds[i]->tuning.fused != 0;
+  for (int st = 0; st < RTGI_MAX_CASCADES; st++) {
+    p.c0_step[st] = n > 0;
+    for (int i = 0; i < n; i++) p.c0_step[st] = p.c0_step[st] && worlds[i]->N - 1 - st == 0;
+  }
   for (int i = 0; i < n; i++) {
     TraceK t = trace_args(worlds[i]);
     uint32_t g[BL_COUNT];

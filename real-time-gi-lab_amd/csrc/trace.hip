@@ -1911,7 +1911,10 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
 
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
   const int l = BL_TM + step;
-  uint32_t* queue = p.queue && p.xcd[l] ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
+  // The cascade-0 step, the largest launch (one workgroup per 256 of each world's c0 rays), keeps the static order:
+  // its tail is a small share of it, and the queue's atomic per workgroup cost more than the balance it bought (64
+  // city worlds: 4.48 ms with the queue, 4.26 without).
+  uint32_t* queue = p.queue && p.xcd[l] && !p.c0_step[step] ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
   if (p.grid[l])
     trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[l], BLK, 0, (hipStream_t)stream>>>(bk(p, l, step), p.w, queue);
   return (int)hipGetLastError();
