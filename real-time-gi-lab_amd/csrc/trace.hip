@@ -929,13 +929,16 @@ __device__ __forceinline__ void sh0_block(const TraceK& a, Sh0Lds& s, uint32_t l
 // With sh_fuse, cascade 0's launch also projects the SH: a block holds 16 probes x all 16 c0 directions (wavefront w
 // = 2x2 block (w & 1, w >> 1) of the 4x4 texels, lane = 16 j + probe as above) and ends with sh0_block, so the c0
 // radiance is not read back. SH: the launch holds some world's cascade 0 (a separate instantiation).
+// tx: the thread's index in its block of BLK threads (threadIdx.x, or the block's quarter times 64 plus the lane where
+// a workgroup is one wavefront).
 template <bool ST, bool DY, bool SH>
-__device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk) {
+__device__ __forceinline__ void trace_merge_body(const TraceK& a, int c, uint32_t lb, uint32_t nblk,
+                                                 uint32_t tx = threadIdx.x) {
   __shared__ Sh0Lds s_sh[1];  // referenced only by the SH instantiation
   const CascK& C = a.c[c];
   const bool quad = !SH && !a.no_merge && c > 0 && C.res != a.c[c - 1].res;
   const bool sh0 = SH;  // the SH instantiation runs cascade 0 of worlds with sh_fuse only
-  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+  uint64_t t = (uint64_t)lb * BLK + tx;
   uint32_t lane = threadIdx.x & 63u, di = 0, pi = 0, dl = 0, j = 0;
   bool ok;
   if (sh0) {
@@ -1044,6 +1047,20 @@ k_trace_merge_b(BatchK b, const TraceK* __restrict__ ws, uint32_t* __restrict__ 
   int w = batch_world(b, blk);
   const TraceK& a = ws[w];
   trace_merge_body<ST, DY, false>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w]);
+}
+// The same step without a work queue, one wavefront per workgroup: workgroup h runs quarter h & 3 of block h >> 2.
+// A workgroup of four wavefronts holds its slots until its slowest wavefront ends, so that the next workgroup waits
+// for four free slots on one CU; single wavefronts leave and enter one at a time (64 city worlds: c0 step 4.27 ->
+// 3.95 ms; 32 mixed worlds' trace 3.85 -> 3.54 ms). With a queue each workgroup pays a device-scope atomic, and per
+// wavefront that cost more than it saved (c1 step 3.30 -> 3.68 ms), so queued steps keep four wavefronts.
+template <bool ST, bool DY>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_trace_merge_b1(BatchK b, const TraceK* __restrict__ ws) {
+  const uint32_t h = batch_block(b), blk = h >> 2;
+  int w = batch_world(b, blk);
+  const TraceK& a = ws[w];
+  trace_merge_body<ST, DY, false>(a, a.N - 1 - b.arg, blk - b.start[w], b.start[w + 1] - b.start[w],
+                                  (h & 3u) * 64u + threadIdx.x);
 }
 
 // ---- Gather: irradiance of the field at surface points (gatherWorld, src/core/rc.ts:871-927) ----
@@ -1783,6 +1800,11 @@ TraceMergeBatchFn trace_merge_batch_fn(bool st, bool dy) {
   return st ? (dy ? k_trace_merge_b<true, true> : k_trace_merge_b<true, false>)
             : (dy ? k_trace_merge_b<false, true> : k_trace_merge_b<false, false>);
 }
+using TraceMergeBatch1Fn = void (*)(BatchK, const TraceK*);
+TraceMergeBatch1Fn trace_merge_batch1_fn(bool st, bool dy) {
+  return st ? (dy ? k_trace_merge_b1<true, true> : k_trace_merge_b1<true, false>)
+            : (dy ? k_trace_merge_b1<false, true> : k_trace_merge_b1<false, false>);
+}
 
 int launch_trace_merge(const TraceK& k, int c, void* stream) {
   const bool dy = k.vx.dyn.hdr != nullptr;
@@ -1915,8 +1937,10 @@ int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
   // its tail is a small share of it, and the queue's atomic per workgroup cost more than the balance it bought (64
   // city worlds: 4.48 ms with the queue, 4.26 without).
   uint32_t* queue = p.queue && p.xcd[l] && !p.c0_step[step] ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
-  if (p.grid[l])
+  if (p.grid[l] && queue)
     trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[l], BLK, 0, (hipStream_t)stream>>>(bk(p, l, step), p.w, queue);
+  else if (p.grid[l])
+    trace_merge_batch1_fn(p.stats, p.dyn)<<<p.grid[l] * 4u, 64, 0, (hipStream_t)stream>>>(bk(p, l, step), p.w);
   return (int)hipGetLastError();
 }
 
