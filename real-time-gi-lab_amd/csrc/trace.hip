@@ -474,8 +474,8 @@ __device__ __forceinline__ void trace_probe_ray(const TraceK& a, int c, i3 g, ui
 }
 
 template <bool ST, bool DY>
-__device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
-  uint64_t ray = (uint64_t)lb * BLK + threadIdx.x;
+__device__ __forceinline__ void trace_body(const TraceK& a, uint32_t lb, uint32_t nblk, uint32_t tx = threadIdx.x) {
+  uint64_t ray = (uint64_t)lb * BLK + tx;
   if (ray >= a.total_rays) return;
   int c = cascade_of(a, ray);
   const CascK& C = a.c[c];
@@ -551,9 +551,11 @@ __global__ void __launch_bounds__(BLK) k_trace_rays(TraceK a, int c, const float
 
 // ST: count the tracer statistics (a separate instantiation, so the counting costs the plain trace nothing).
 // DY: some world has a dynamic layer (a separate instantiation: the read costs registers the plain trace keeps).
+// One wavefront per workgroup (workgroup h traces quarter h & 3 of block h >> 2): a workgroup of four would hold its
+// slots until its slowest ray ends.
 template <bool ST, bool DY>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace(TraceK a) {
-  trace_body<ST, DY>(a, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace(TraceK a) {
+  trace_body<ST, DY>(a, blockIdx.x >> 2, 0, (blockIdx.x & 3u) * 64u + threadIdx.x);
 }
 template <bool ST, bool DY>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_b(BatchK b) {
@@ -1764,10 +1766,11 @@ int grid_for(uint64_t n) {
 int launch_trace(const TraceK& k, void* stream) {
   const bool dy = k.vx.dyn.hdr != nullptr;
   if (k.total_rays) {
+    const uint32_t g = grid_for(k.total_rays) * 4u;
     if (k.vx.stats)
-      (dy ? k_trace<true, true> : k_trace<true, false>)<<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+      (dy ? k_trace<true, true> : k_trace<true, false>)<<<g, 64, 0, (hipStream_t)stream>>>(k);
     else
-      (dy ? k_trace<false, true> : k_trace<false, false>)<<<grid_for(k.total_rays), BLK, 0, (hipStream_t)stream>>>(k);
+      (dy ? k_trace<false, true> : k_trace<false, false>)<<<g, 64, 0, (hipStream_t)stream>>>(k);
   }
   return (int)hipGetLastError();
 }
