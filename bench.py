@@ -733,7 +733,8 @@ def main():
         "trace_merge_fused": worlds_tuning["fused"] != 0,
         "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
-        "roofline": {"kernel": ("k_trace_merge_b (one launch per cascade step of the batch)" if fused
+        "roofline": {"kernel": ("k_trace_merge_b / k_trace_merge_b1 (one launch per cascade step of the batch: "
+                                "queued steps / one-wavefront workgroups)" if fused
                                 else "k_trace_b + k_merge_step_b"),
                      "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
