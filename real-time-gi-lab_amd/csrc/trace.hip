@@ -1313,7 +1313,7 @@ __device__ __forceinline__ bool slot_voxel(const VoxK& k, uint32_t ai, uint32_t 
 // The voxel's sun visibility: the mean of 2 x 2 shadow rays from points a quarter voxel along two tangents of the
 // representative's plane, (0.25 voxel + 3 cm) off it on its sunward side, as voxSun places its shadow-map taps.
 // Four lanes per occupied voxel (dense over the attribute slots), one tap each; the quad sums the taps in tap order.
-__device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
+__device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t nblk, uint32_t tx = threadIdx.x) {
   const VoxK& k = a.vx;
   const uint32_t nb = min(k.hdr[H_MARKED], (uint32_t)k.max_bricks);
   const uint32_t nv = min(k.hdr[H_VOXELS], (uint32_t)k.max_voxels);
@@ -1322,7 +1322,7 @@ __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t 
               fabsf(sd.z) < 1e-9f ? 1e-9f : sd.z);
   f3 inv = mk3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
   // The stride is a multiple of 4 and so is the bound: a voxel's four lanes are always in the loop together.
-  for (uint64_t t = (uint64_t)lb * BLK + threadIdx.x; t < (uint64_t)nv * 4u; t += (uint64_t)nblk * BLK) {
+  for (uint64_t t = (uint64_t)lb * BLK + tx; t < (uint64_t)nv * 4u; t += (uint64_t)nblk * BLK) {
     const uint32_t ai = (uint32_t)(t >> 2), tap = (uint32_t)(t & 3u);
     uint32_t ptr, bit;
     if (!slot_voxel(k, ai, nb, ptr, bit)) continue;  // the same decision for the quad's four lanes
@@ -1358,10 +1358,11 @@ __device__ __forceinline__ void sun_body(const TraceK& a, uint32_t lb, uint32_t 
   }
 }
 __global__ void __launch_bounds__(BLK) k_sun(TraceK a) { sun_body(a, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(BLK) k_sun_b(BatchK b) {
-  const uint32_t blk = batch_block(b);
+// Batched: one wavefront per workgroup (workgroup h runs quarter h & 3 of block h >> 2), as the batched trace.
+__global__ void __launch_bounds__(64) k_sun_b(BatchK b) {
+  const uint32_t h = batch_block(b), blk = h >> 2;
   int w = batch_world(b, blk);
-  sun_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  sun_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w], (h & 3u) * 64u + threadIdx.x);
 }
 
 // ---- Bounce volume (src/core/bounce.ts): update (bnClassify / bnUpdate) and injection into the voxels (bnInject) ----
@@ -1715,8 +1716,8 @@ __device__ __forceinline__ int probe_cascade(const TraceK& a, uint64_t t) {
   return c;
 }
 
-__device__ __forceinline__ void relocate_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
-  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+__device__ __forceinline__ void relocate_body(const TraceK& a, uint32_t lb, uint32_t nblk, uint32_t tx = threadIdx.x) {
+  uint64_t t = (uint64_t)lb * BLK + tx;
   if (t >= a.total_probes) return;
   int c = probe_cascade(a, t);
   relocate_probe(a, c, (uint32_t)(t - a.c[c].probe_off), ppos4(a)[t], pnrm4(a)[t]);
@@ -1724,8 +1725,8 @@ __device__ __forceinline__ void relocate_body(const TraceK& a, uint32_t lb, uint
 
 // Parents of probe pi of cascade c < N-1 that exist and whose segment from the probe is free of occupied voxels at
 // the probe's trace level (voxOcc samples of linkParents = segOccluded's).
-__device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t nblk) {
-  uint64_t t = (uint64_t)lb * BLK + threadIdx.x;
+__device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t nblk, uint32_t tx = threadIdx.x) {
+  uint64_t t = (uint64_t)lb * BLK + tx;
   if (a.N < 2 || t >= a.c[a.N - 1].probe_off) return;
   int c = probe_cascade(a, t);
   const CascK& C = a.c[c];
@@ -1745,15 +1746,15 @@ __device__ __forceinline__ void link_body(const TraceK& a, uint32_t lb, uint32_t
 }
 __global__ void __launch_bounds__(BLK) k_relocate(TraceK a) { relocate_body(a, blockIdx.x, gridDim.x); }
 __global__ void __launch_bounds__(BLK) k_link(TraceK a) { link_body(a, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(BLK) k_relocate_b(BatchK b) {
-  const uint32_t blk = batch_block(b);
+__global__ void __launch_bounds__(64) k_relocate_b(BatchK b) {
+  const uint32_t h = batch_block(b), blk = h >> 2;
   int w = batch_world(b, blk);
-  relocate_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  relocate_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w], (h & 3u) * 64u + threadIdx.x);
 }
-__global__ void __launch_bounds__(BLK) k_link_b(BatchK b) {
-  const uint32_t blk = batch_block(b);
+__global__ void __launch_bounds__(64) k_link_b(BatchK b) {
+  const uint32_t h = batch_block(b), blk = h >> 2;
   int w = batch_world(b, blk);
-  link_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w]);
+  link_body(b.w[w], blk - b.start[w], b.start[w + 1] - b.start[w], (h & 3u) * 64u + threadIdx.x);
 }
 
 int grid_for(uint64_t n) {
@@ -1897,7 +1898,7 @@ int launch_sh(const TraceK& k, void* stream) {
 static BatchK bk(const BatchPlan& p, int l, int arg = 0) { return BatchK{p.w, p.start[l], p.n, arg, p.xcd[l] ? 1 : 0}; }
 
 int launch_sun_batch(const BatchPlan& p, void* stream) {
-  if (p.grid[BL_SUN]) k_sun_b<<<p.grid[BL_SUN], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_SUN));
+  if (p.grid[BL_SUN]) k_sun_b<<<p.grid[BL_SUN] * 4u, 64, 0, (hipStream_t)stream>>>(bk(p, BL_SUN));
   return (int)hipGetLastError();
 }
 
@@ -1913,8 +1914,8 @@ int launch_dyn_inject_batch(const BatchPlan& p, void* stream) {
 }
 
 int launch_relocate_batch(const BatchPlan& p, void* stream) {
-  if (p.grid[BL_RELOC]) k_relocate_b<<<p.grid[BL_RELOC], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_RELOC));
-  if (p.grid[BL_LINK]) k_link_b<<<p.grid[BL_LINK], BLK, 0, (hipStream_t)stream>>>(bk(p, BL_LINK));
+  if (p.grid[BL_RELOC]) k_relocate_b<<<p.grid[BL_RELOC] * 4u, 64, 0, (hipStream_t)stream>>>(bk(p, BL_RELOC));
+  if (p.grid[BL_LINK]) k_link_b<<<p.grid[BL_LINK] * 4u, 64, 0, (hipStream_t)stream>>>(bk(p, BL_LINK));
   return (int)hipGetLastError();
 }
 
