@@ -793,11 +793,14 @@ __global__ void __launch_bounds__(BLK) k_vox_big_q(BatchK b) { vox_big_queue<TX>
 // and each representative has one writer, so there are no atomics. The result equals the reference's scatter form:
 // a child brick is allocated only if its parent is (markParents and the coarse-first order), and a dropped child
 // contributes nothing in either form.
+// Only the level-L bricks are visited: the allocation hands out slots from the coarsest level down, so level L's
+// bricks are one run of slots after the coarser levels' runs (their counts are the header's per-level words).
 __device__ __forceinline__ void reduce_body(const VoxK& k, int L, uint32_t lb, uint32_t nblk) {
-  uint32_t nb = n_bricks(k);
-  uint32_t n = nb * 512u;
+  uint32_t first = 0;
+  for (int l = k.NL - 1; l > L; l--) first += k.hdr[H_BRICKS_L + l];
+  const uint32_t n = (first + k.hdr[H_BRICKS_L + L]) * 512u;
   const i3 oL = origin(k, L), oF = origin(k, L - 1);
-  for (uint32_t base = lb * BLK; base < n; base += nblk * BLK) {
+  for (uint32_t base = first * 512u + lb * BLK; base < n; base += nblk * BLK) {
     uint32_t i = base + threadIdx.x;
     uint32_t ptr = i >> 9, bit = i & 511u;
     bool any = false;
