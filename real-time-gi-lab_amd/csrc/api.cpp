@@ -1447,8 +1447,9 @@ void plan_blocks(rtgi_world* const* worlds, int n, uint32_t* start, BatchPlan& p
   p.fused = n > 0;
   for (int i = 0; i < n; i++) p.fused = p.fused && worlds[i]->tuning.fused != 0;
   for (int st = 0; st < RTGI_MAX_CASCADES; st++) {
-    p.c0_step[st] = n > 0;
-    for (int i = 0; i < n; i++) p.c0_step[st] = p.c0_step[st] && worlds[i]->N - 1 - st == 0;
+    p.min_casc[st] = RTGI_MAX_CASCADES;
+    for (int i = 0; i < n; i++)
+      if (worlds[i]->N - 1 - st >= 0) p.min_casc[st] = std::min(p.min_casc[st], worlds[i]->N - 1 - st);
   }
   for (int i = 0; i < n; i++) {
     TraceK t = trace_args(worlds[i]);

@@ -312,7 +312,7 @@ struct BatchPlan {
   bool surf;        // some world has textured surfaces (the voxel passes sample them)
   bool xcd[BL_COUNT];  // the launch's worlds have about equal block counts: contiguous ranges per XCD pay
   bool sh_pairs;       // every world with an SH projection takes it in pairs (sh_pairs): the pair-only kernel
-  bool c0_step[RTGI_MAX_CASCADES];  // trace+merge step st traces cascade 0 in every world of the batch
+  int32_t min_casc[RTGI_MAX_CASCADES];  // trace+merge step st: the finest cascade any world of the batch traces
 };
 // Blocks a world needs for each launch (the single-world launches use the same counts).
 void batch_blocks(const TraceK& t, uint32_t out[BL_COUNT]);

@@ -1937,10 +1937,11 @@ int launch_merge_batch(const BatchPlan& p, int step, void* stream) {
 
 int launch_trace_merge_batch(const BatchPlan& p, int step, void* stream) {
   const int l = BL_TM + step;
-  // The cascade-0 step, the largest launch (one workgroup per 256 of each world's c0 rays), keeps the static order:
-  // its tail is a small share of it, and the queue's atomic per workgroup cost more than the balance it bought (64
-  // city worlds: 4.48 ms with the queue, 4.26 without).
-  uint32_t* queue = p.queue && p.xcd[l] && !p.c0_step[step] ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
+  // Only the coarse cascades (3 and up) take the queue: few blocks per world of long rays, whose cost varies most
+  // between worlds. The finer steps run in the static order with one wavefront per workgroup, which there beats the
+  // queue's atomic per workgroup (64 city worlds, static against queued: c0 3.91 against 4.27 ms in four-wavefront
+  // workgroups, c1 3.01 against 3.30, c2 2.64 against 2.82; c3 2.34 against 2.01).
+  uint32_t* queue = p.queue && p.xcd[l] && p.min_casc[step] >= 3 ? p.queue + VQ_SET * (size_t)(1 + step) : nullptr;
   if (p.grid[l] && queue)
     trace_merge_batch_fn(p.stats, p.dyn)<<<p.grid[l], BLK, 0, (hipStream_t)stream>>>(bk(p, l, step), p.w, queue);
   else if (p.grid[l])
