@@ -545,6 +545,9 @@ int world_layout(rtgi_world* w, const rtgi_vox_params* vp, const rtgi_rc_params*
     off += rays_c;
     w->dir_off[c + 1] = w->dir_off[c] + (uint32_t)(w->res[c] * w->res[c]);
     uint64_t npc = (uint64_t)w->n[c] * w->n[c] * w->n[c];
+    // The merge reads a cascade's averages through a buffer resource with 32-bit byte offsets.
+    if (c > 0 && npc * w->res[c - 1] * w->res[c - 1] * 16u >= (1ull << 32))
+      return rtgi_fail(RTGI_E_INVALID, "cascade %d's 2x2 averages exceed 4 GiB (n^3 res_{c-1}^2 x 16 B)", c);
     w->avg_off[c + 1] = w->avg_off[c] + (c > 0 ? npc * w->res[c - 1] * w->res[c - 1] : 0);
     w->probe_off[c + 1] = w->probe_off[c] + npc;
   }

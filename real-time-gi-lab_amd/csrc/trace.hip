@@ -616,7 +616,8 @@ __device__ __forceinline__ f3 upper_mean(const TraceK& a, int c, i3 g, uint32_t 
     const int nl = U.n_log2;
     const i3 b = shr3(g, 1);
     const bool hx = (g.x & 1) && b.x + 1 < U.n, hy = (g.y & 1) && b.y + 1 < U.n, hz = (g.z & 1) && b.z + 1 < U.n;
-    // U's averages: U.np entries per direction of cascade c (res_c^2 of them), float4 each
+    // U's averages: U.np entries per direction of cascade c (res_c^2 of them), float4 each (below 4 GiB: the world
+    // layout refuses larger)
     const uint32_t res = (uint32_t)a.c[c].res;
     const __amdgpu_buffer_rsrc_t r =
         buf_rsrc(reinterpret_cast<const float4*>(a.avg) + U.avg_off, U.np * res * res * 16u);

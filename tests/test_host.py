@@ -72,6 +72,11 @@ def test_estimate_bytes_and_server_without_gpu():
     # A cascade's rays are indexed in 32 bits: 1024^3 c0 probes x 16 directions is refused before any allocation.
     with pytest.raises(rtgi.RtgiError, match="below 2\\^32"):
         rtgi.estimate_bytes(vox, dataclasses.replace(rc, probes0=1024, fp16=True), 0, 0, 0)
+    # The merge addresses a cascade's 2x2 averages in 32 bits: 512^3 c0 probes (2^31 rays) give c1 256^3 x 16
+    # averages of 16 B = 4 GiB, refused; 256^3 c0 probes are accepted.
+    with pytest.raises(rtgi.RtgiError, match="averages exceed 4 GiB"):
+        rtgi.estimate_bytes(vox, dataclasses.replace(rc, probes0=512, fp16=True), 0, 0, 0)
+    assert rtgi.estimate_bytes(vox, dataclasses.replace(rc, probes0=256, fp16=True), 0, 0, 0) > 0
     with pytest.raises(rtgi.RtgiError, match="no HIP device"):
         rtgi.Server()
 
