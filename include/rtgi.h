@@ -170,7 +170,9 @@ int rtgi_oct_solid_angles(int32_t res, int32_t samples, float* out);
 
 /* Creates a world on the current HIP device. Replaces both factories createVoxWorld (src/core/voxel.ts:746-747) and
  * createRc (src/core/rc.ts:1019), called by createRunContext (src/core/rcrun.ts:62-63) and runConfig
- * (src/core/rcrun.ts:269-271). Throws become RTGI_E_INVALID: levels > 8 (voxel.ts:749), cascades > 8 (rc.ts:1022). */
+ * (src/core/rcrun.ts:269-271). Throws become RTGI_E_INVALID: levels > 8 (voxel.ts:749), cascades > 8 (rc.ts:1022).
+ * Also refused (sizes this library addresses in 32 bits): a cascade of 2^32 rays or more, and a cascade c > 0 whose
+ * 2x2 averages (n_c^3 res_{c-1}^2 entries of 16 B) reach 4 GiB. */
 int rtgi_world_create(const rtgi_vox_params* vox, const rtgi_rc_params* rc, rtgi_world** out);
 
 /* Scene upload: positions and normals (3 floats per vertex), the triangle table (4 u32 per triangle: absolute vertex
