@@ -622,10 +622,8 @@ def main():
     worlds_tuning = worlds[0].tuning()
     groups = [worlds[i:i + bsz] for i in range(0, len(worlds), bsz)]
     batches = [rtgi.Batch(g, graph=True) for g in groups]
-    # Stage timing by HIP events recorded on each batch's stream between its per-stage graphs, read after every timed
-    # step: the stage durations of the timed updates themselves.
-    for b in batches:
-        b.set_timing(True)
+    # The timed steps replay one graph per batch; the stage durations come from further steps with HIP events
+    # recorded on each batch's stream between its per-stage graphs (four graphs per update instead of one).
     for _ in range(a.warmup):
         for b in batches:
             b.encode()
@@ -644,13 +642,21 @@ def main():
             b.sync()
             lat.extend([time.perf_counter() - ts] * len(g))
         step_ms.append((time.perf_counter() - ts) * 1e3)
-        for b, g in zip(batches, groups):
-            if len(g) == len(groups[0]):
-                for k, v in b.stage_ms().items():
-                    st[k].append(v)
         log(f"step {s + 1}/{a.steps}: {step_ms[-1]:.2f} ms")
     elapsed = time.perf_counter() - t_start
     value = a.steps * len(worlds) / elapsed
+    for b in batches:
+        b.set_timing(True)
+    for _ in range(max(3, a.steps // 2)):
+        if motion:
+            motion.advance()
+        for b in batches:
+            b.encode()
+        for b, g in zip(batches, groups):
+            b.sync()
+            if len(g) == len(groups[0]):
+                for k, v in b.stage_ms().items():
+                    st[k].append(v)
     stage = {k: float(np.mean(v)) for k, v in st.items()}
     # Roofline of the dominant kernel, the trace (fused with the merge: one launch per cascade step of the batch, 5
     # per update). Algorithmic bytes per world update are the trace's and merge's of the per-unit figures: 16 B of f32
@@ -729,7 +735,8 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 3),
         "mrays_per_s": round(rays * value / 1e6, 1),
         "stage_ms_per_batch": {k: round(v, 4) for k, v in stage.items()},
-        "stage_timing": "HIP events between each batch's stage graphs, mean over the timed steps",
+        "stage_timing": "HIP events between each batch's stage graphs, mean over further updates after the timed "
+                        "steps (the timed steps replay one graph per batch)",
         "trace_merge_fused": worlds_tuning["fused"] != 0,
         "autotune_table": tune["table"] if tune else None,
         "counters_world1": {k: cnt[k] for k in ("bricks", "voxels", "brickOverflow", "voxelOverflow")},
