@@ -75,7 +75,7 @@ typedef struct rtgi_vox_params {
 #define RTGI_RC_FP16 4u        /* precision form: cascade radiance and SH stored as f16 (round to nearest even) at the
                                   reference's own pack points (packRad, src/core/rc.ts:309, 621, 707; projectSH
                                   pack2x16float, rc.ts:770-771); all arithmetic stays f32 */
-#define RTGI_RC_BF16 512u      /* precision form (BASELINE cfg3 with RTGI_RC_FP16): the voxel albedo is kept as bf16 rgb
+#define RTGI_RC_BF16 512u      /* precision form (with RTGI_RC_FP16: FP16 radiance): the voxel albedo is kept as bf16 rgb
                                 * (round to nearest even from the material's f32 value) beside the reference's RGBA8
                                 * word, and hit shading reads it; with RTGI_RC_BOUNCE the injected bounce light of every
                                 * voxel (its emission toward the cascades) is stored as bf16 instead of f16. All

@@ -11,7 +11,7 @@ Reference interface (owoDra/real-time-gi-lab) and what stands in for it here:
   material words (what surfaceAt samples)
   parseConfig / named configs              src/runner/configs.ts:47-73 -> CONFIGS, config()
 
-Errors follow the reference's: bad configuration raises synchronously (RtgiError, like its JS throws); GPU pool
+Errors follow the reference's: bad configuration raises synchronously (RtgiError, like its JS throws); brick pool
 overflow is not an error and shows in read_counters().
 """
 import ctypes as C

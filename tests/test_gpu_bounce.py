@@ -68,7 +68,7 @@ def test_bounce_matches_oracle(require_gpu, case):
         scene, vox, rc, f1 = rtgi.config(case)
     f2 = moved(f1, (2.5, 0.0, -1.7))
     frames = [f1, f2, f2]
-    # the last form keeps the albedo and the injected bounce light as bf16 (the precision form of BASELINE cfg3)
+    # the last form keeps the albedo and the injected bounce light as bf16 (the FP16 radiance + BF16 albedo precision form)
     for fp16, reloc, sun, bf16 in ((False, False, False, False), (True, True, True, False), (True, False, False, True)):
         rcf = dataclasses.replace(rc, bounce=True, relocate=reloc, sunTrace=sun, fp16=fp16, recordHits=True,
                                   bf16=bf16)

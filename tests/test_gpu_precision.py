@@ -78,7 +78,7 @@ def test_ray_query_matches_oracle(require_gpu, form):
 
 def test_precision_report(require_gpu):
     say(f"case: precision report, atrium, {NRAYS} rays per cascade, FP32, FP16 and FP16 + BF16 albedo forms")
-    report = {"config": "atrium (cfg1 scene), 10k seeded sphere rays per cascade; FP64 oracle over the same clipmap "
+    report = {"config": "atrium scene, 10k seeded sphere rays per cascade; FP64 oracle over the same clipmap "
                         "and the same stored albedo", "forms": {}}
     for name in FORMS:
         fp16 = FORMS[name]["fp16"]
@@ -119,7 +119,7 @@ def test_precision_report(require_gpu):
 @pytest.mark.parametrize("case,form", [("atrium", "fp32"), ("atrium", "fp16"), ("atrium", "fp16+bf16"), ("city", "fp32"),
                                        ("city", "fp16"), ("city", "fp16+bf16")])
 def test_irradiance_error_vs_fp64(require_gpu, case, form):
-    """Irradiance (the SH of every c0 probe) against FP64 trace + merge + SH over the same clipmap, BASELINE's bound
+    """Irradiance (the SH of every c0 probe) against FP64 trace + merge + SH over the same clipmap, bound
     1e-3 relative per probe (the largest coefficient difference over the probe's largest coefficient). The FP64 rays
     take the f32 walk's first-hit voxel wherever the two walks may disagree (tie rays and rays whose FP64 first hit
     differs: there both voxels are right answers of the reference algorithm, and the f32 one is what the reference's
